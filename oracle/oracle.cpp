@@ -1,0 +1,1389 @@
+// Oracle (test infrastructure only; parity checker and CPU baseline).
+//
+// A CPU restatement of samurai v0.33.0's adaptive-MR time step on the unit box:
+// MRMesh construction, update_ghost_mr (with Dirichlet<1> and corner/BC
+// projection/prediction), the Harten adaptation (detail, criteria, maximum,
+// update_cell_array_from_tag, make_graduation, update_fields) and the upwind
+// finite-volume step.  Sets are dense bitmaps (dense_set.hpp); every
+// floating-point expression is written in the reference's operation order and
+// the file is compiled with -ffp-contract=off.  Each function cites the
+// reference file:line it follows (paths relative to the samurai source tree).
+//
+// Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
+// this library.  The product (samurai_amd/) never links or calls it.
+#include "dense_set.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <string>
+
+namespace oracle
+{
+    constexpr int MAXL = 21; // CellArray max_size 20 -> levels 0..20 (samurai_config.hpp:50)
+
+    enum MeshId
+    {
+        CELLS            = 0,
+        CELLS_AND_GHOSTS = 1,
+        PROJ_CELLS       = 2,
+        UNION_CELLS      = 3,
+        REFERENCE        = 4
+    };
+
+    enum Flag : uint8_t
+    {
+        KEEP    = 1,
+        COARSEN = 2,
+        REFINE  = 4
+    };
+
+    using Dir = std::array<int, 3>;
+
+    // static_nested_loop<dim,-1,2> order (static_algorithm.hpp:18-37): the last
+    // dimension is the outermost loop, x the innermost.
+    inline std::vector<Dir> all_directions(int dim)
+    {
+        std::vector<Dir> out;
+        if (dim == 2)
+        {
+            for (int y = -1; y <= 1; ++y)
+            {
+                for (int x = -1; x <= 1; ++x)
+                {
+                    if (x != 0 || y != 0)
+                    {
+                        out.push_back({x, y, 0});
+                    }
+                }
+            }
+        }
+        else
+        {
+            for (int z = -1; z <= 1; ++z)
+            {
+                for (int y = -1; y <= 1; ++y)
+                {
+                    for (int x = -1; x <= 1; ++x)
+                    {
+                        if (x != 0 || y != 0 || z != 0)
+                        {
+                            out.push_back({x, y, z});
+                        }
+                    }
+                }
+            }
+        }
+        return out;
+    }
+
+    inline int n_nonzero(const Dir& d)
+    {
+        return (d[0] != 0) + (d[1] != 0) + (d[2] != 0);
+    }
+
+    // for_each_diagonal_direction (stencil.hpp:337-349)
+    inline std::vector<Dir> diagonal_directions(int dim)
+    {
+        std::vector<Dir> out;
+        for (auto& d : all_directions(dim))
+        {
+            if (n_nonzero(d) > 1)
+            {
+                out.push_back(d);
+            }
+        }
+        return out;
+    }
+
+    // for_each_cartesian_direction (stencil.hpp:300-316): +e0, -e0, +e1, -e1, ...
+    inline std::vector<Dir> cartesian_directions(int dim)
+    {
+        std::vector<Dir> out;
+        for (int d = 0; d < dim; ++d)
+        {
+            Dir v{0, 0, 0};
+            v[d] = 1;
+            out.push_back(v);
+            v[d] = -1;
+            out.push_back(v);
+        }
+        return out;
+    }
+
+    inline Dir scale(const Dir& d, int k)
+    {
+        return {d[0] * k, d[1] * k, d[2] * k};
+    }
+
+    struct Mesh
+    {
+        int dim = 2, min_level = 0, max_level = 0;
+        int pred_s = 1, radius = 1, grad_width = 1;
+        std::vector<DSet> ids[5];
+        std::vector<DSet> domain; // m_domain pyramid, levels 0..max_level
+        std::vector<DSet> corners;
+        std::vector<Dir> corner_dirs;
+        // storage index of the reference cells
+        std::vector<std::vector<int64_t>> idx;
+        std::vector<int64_t> level_base;
+        int64_t nref = 0;
+
+        const DSet& get(int id, int l) const { return ids[id][size_t(l)]; }
+
+        int64_t index(int l, int x, int y, int z = 0) const
+        {
+            const DSet& r = ids[REFERENCE][size_t(l)];
+            if (l < 0 || l >= MAXL || !r.get(x, y, z))
+            {
+                char buf[160];
+                std::snprintf(buf, sizeof(buf), "oracle: cell (%d; %d,%d,%d) not in the reference mesh", l, x, y, z);
+                throw std::out_of_range(buf);
+            }
+            return idx[size_t(l)][size_t(r.off(x, y, z))];
+        }
+        bool exists(int l, int x, int y, int z = 0) const
+        {
+            return l >= 0 && l < MAXL && ids[REFERENCE][size_t(l)].get(x, y, z);
+        }
+        int ca_min_level(int id) const
+        {
+            for (int l = 0; l < MAXL; ++l)
+            {
+                if (!ids[id][size_t(l)].empty())
+                {
+                    return l;
+                }
+            }
+            return MAXL;
+        }
+        int ca_max_level(int id) const
+        {
+            for (int l = MAXL - 1; l >= 0; --l)
+            {
+                if (!ids[id][size_t(l)].empty())
+                {
+                    return l;
+                }
+            }
+            return 0;
+        }
+    };
+
+    inline std::vector<DSet> empty_ca(int dim)
+    {
+        std::vector<DSet> v;
+        for (int l = 0; l < MAXL; ++l)
+        {
+            v.push_back(empty_set(dim, l));
+        }
+        return v;
+    }
+
+    inline DSet domain_at(int dim, int level)
+    {
+        int n = 1 << level;
+        return box_set(dim, level, {0, 0, 0}, {n, n, dim == 3 ? n : 1});
+    }
+
+    // Mesh_base::construct_union (mesh.hpp:1231-1262)
+    static void construct_union(Mesh& m)
+    {
+        auto& uni                  = m.ids[UNION_CELLS];
+        uni                        = empty_ca(m.dim);
+        for (int level = m.max_level; level >= 1; --level)
+        {
+            uni[size_t(level - 1)] = tighten(on(union_(m.ids[CELLS][size_t(level)], uni[size_t(level)]), level - 1));
+        }
+    }
+
+    // MRMesh::update_sub_mesh_impl (mr/mesh.hpp:221-455), serial non-periodic.
+    static void update_sub_mesh(Mesh& m)
+    {
+        const int r = m.radius, s = m.pred_s;
+        std::vector<DSet> cell_list = empty_ca(m.dim);
+        for (int l = 0; l < MAXL; ++l)
+        {
+            if (!m.ids[CELLS][size_t(l)].empty())
+            {
+                // cells expanded by the stencil radius in x and in every row offset (:240-253)
+                cell_list[size_t(l)] = expand(m.ids[CELLS][size_t(l)], r);
+            }
+        }
+        m.ids[CELLS_AND_GHOSTS] = cell_list;
+        m.ids[PROJ_CELLS]       = empty_ca(m.dim);
+
+        if (m.max_level != m.min_level)
+        {
+            for (int level = 0; level < MAXL; ++level)
+            {
+                const DSet& cl = m.ids[CELLS][size_t(level)];
+                if (cl.empty())
+                {
+                    continue;
+                }
+                // prediction ghosts one level below (:331-342, :355-361)
+                DSet sub_cg = intersection(expand(on(m.ids[CELLS_AND_GHOSTS][size_t(level)], level - 1), s),
+                                           expand(m.domain[size_t(level - 1)], s));
+                cell_list[size_t(level - 1)] = tighten(union_(cell_list[size_t(level - 1)], on(sub_cg, level - 1)));
+                if (level - 1 > 0)
+                {
+                    DSet sub_c                   = expand(on(cl, level - 2), s);
+                    cell_list[size_t(level - 2)] = tighten(union_(cell_list[size_t(level - 2)], sub_c));
+                }
+            }
+            m.ids[REFERENCE] = cell_list;
+
+            // projection ghosts (:432-452); the level loop re-reads the (growing)
+            // reference like for_each_level does (algorithm.hpp:52-61).
+            for (int level = 0; level < MAXL - 1; ++level)
+            {
+                if (level > m.ca_max_level(REFERENCE))
+                {
+                    break;
+                }
+                if (m.ids[REFERENCE][size_t(level)].empty())
+                {
+                    continue;
+                }
+                DSet expr  = tighten(on(intersection(m.ids[REFERENCE][size_t(level)], m.ids[UNION_CELLS][size_t(level)]), level));
+                DSet& lcl  = cell_list[size_t(level + 1)];
+                lcl        = tighten(union_(lcl, refine(expr, 1)));
+                m.ids[REFERENCE][size_t(level + 1)] = lcl;
+                m.ids[PROJ_CELLS][size_t(level)]    = expr;
+            }
+        }
+        m.ids[REFERENCE] = cell_list;
+    }
+
+    // Mesh_base::construct_corners (mesh.hpp:913-997)
+    static void construct_corners(Mesh& m)
+    {
+        m.corners.clear();
+        m.corner_dirs.clear();
+        const DSet& dom = m.domain[size_t(m.max_level)];
+        for (auto& d : diagonal_directions(m.dim))
+        {
+            DSet u = empty_set(m.dim, m.max_level);
+            for (int i = 0; i < m.dim; ++i)
+            {
+                if (d[i] != 0)
+                {
+                    Dir t{0, 0, 0};
+                    t[i] = -d[i];
+                    u    = union_(u, translate(dom, t));
+                }
+            }
+            m.corners.push_back(tighten(difference(dom, u)));
+            m.corner_dirs.push_back(d);
+        }
+    }
+
+    // Mesh_base::renumbering + LevelCellArray::update_index (mesh.hpp:893-911,
+    // level_cell_array.hpp:774-784): cumulative count over (level, z, y, x).
+    static void renumbering(Mesh& m)
+    {
+        m.idx.assign(MAXL, {});
+        m.level_base.assign(MAXL + 1, 0);
+        int64_t n = 0;
+        for (int l = 0; l < MAXL; ++l)
+        {
+            m.level_base[size_t(l)] = n;
+            const DSet& r           = m.ids[REFERENCE][size_t(l)];
+            m.idx[size_t(l)].assign(size_t(r.volume()), -1);
+            r.for_each(
+                [&](int x, int y, int z)
+                {
+                    m.idx[size_t(l)][size_t(r.off(x, y, z))] = n++;
+                });
+        }
+        m.level_base[MAXL] = n;
+        m.nref             = n;
+    }
+
+    // Mesh_base(ca, cfg) / Mesh_base(ca, ref_mesh) -> finalize_mesh (mesh.hpp:386-439)
+    static Mesh make_mesh_from_cells(const Mesh& cfg, const std::vector<DSet>& cells)
+    {
+        Mesh m;
+        m.dim        = cfg.dim;
+        m.min_level  = cfg.min_level;
+        m.max_level  = cfg.max_level;
+        m.pred_s     = cfg.pred_s;
+        m.radius     = cfg.radius;
+        m.grad_width = cfg.grad_width;
+        m.domain.clear();
+        for (int l = 0; l <= m.max_level; ++l)
+        {
+            m.domain.push_back(domain_at(m.dim, l));
+        }
+        for (auto& v : m.ids)
+        {
+            v = empty_ca(m.dim);
+        }
+        for (int l = 0; l < MAXL; ++l)
+        {
+            m.ids[CELLS][size_t(l)] = tighten(cells[size_t(l)]);
+        }
+        construct_union(m);
+        update_sub_mesh(m);
+        construct_corners(m);
+        renumbering(m);
+        return m;
+    }
+
+    struct Sim
+    {
+        Mesh mesh;
+        std::vector<double> u;
+        std::vector<double> detail;
+        std::vector<uint8_t> tag;
+        bool has_bc     = false;
+        double bc_value = 0.0;
+        bool poison     = true; // NaN-fill storage nobody wrote (SAMURAI_CHECK_NAN idea)
+        int last_harten_iterations = 0;
+        // the tags/details of the last harten call, on the mesh they were computed on
+        Mesh tag_mesh;
+        std::vector<uint8_t> last_tag;
+        std::vector<double> last_detail;
+    };
+
+    inline double& U(Sim& s, int l, int x, int y, int z = 0)
+    {
+        return s.u[size_t(s.mesh.index(l, x, y, z))];
+    }
+
+    // ------------------------------------------------------------------
+    // ghost update (algorithm/update_ghost_mr.hpp:194-237)
+    // ------------------------------------------------------------------
+
+    // Dirichlet<1>: u[ghost] = 2*v - u[cell] (bc/dirichlet.hpp:29) on the cells of
+    // the boundary region (bc/apply_field_bc.hpp:53-101, bc/bc.hpp:326-363).
+    static void apply_field_bc(Sim& s, int level, const Dir& dir)
+    {
+        if (!s.has_bc)
+        {
+            return;
+        }
+        const Mesh& m    = s.mesh;
+        const DSet& dom  = m.domain[size_t(m.max_level)];
+        DSet region      = difference(dom, translate(dom, scale(dir, -1)));
+        DSet bdry        = on(intersection(m.ids[CELLS][size_t(level)], region), level);
+        const double val = s.bc_value;
+        bdry.for_each(
+            [&](int x, int y, int z)
+            {
+                double uin                                  = U(s, level, x, y, z);
+                U(s, level, x + dir[0], y + dir[1], z + dir[2]) = 2 * val - uin;
+            });
+    }
+
+    // update_outer_corners_by_polynomial_extrapolation, ghost width 1
+    // (bc/apply_field_bc.hpp:315-466; polynomial_extrapolation.hpp:63-66): the
+    // diagonal ghost of each corner cell is a copy of the cell.
+    static void corner_extrapolation(Sim& s, int level, size_t c)
+    {
+        const Mesh& m = s.mesh;
+        const Dir& d  = m.corner_dirs[c];
+        DSet corner   = on(m.corners[c], level);
+        DSet cells    = on(intersection(m.ids[CELLS][size_t(level)], corner), level);
+        cells.for_each(
+            [&](int x, int y, int z)
+            {
+                U(s, level, x + d[0], y + d[1], z + d[2]) = U(s, level, x, y, z);
+            });
+    }
+
+    // project_corner_below (algorithm/update_outer_ghost.hpp:267-336)
+    static void project_corner_below(Sim& s, int level, size_t c)
+    {
+        const Mesh& m = s.mesh;
+        const Dir& d  = m.corner_dirs[c];
+        if (level == 0)
+        {
+            return;
+        }
+        for (int dl = 1; dl <= 2; ++dl)
+        {
+            int pl                 = level - dl;
+            DSet fine_inner        = on(m.corners[c], level);
+            DSet fine_outer        = intersection(translate(fine_inner, d), m.ids[REFERENCE][size_t(level)]);
+            DSet proj_ghost        = intersection(on(fine_outer, pl), m.ids[REFERENCE][size_t(pl)]);
+            const int last         = (1 << dl) - 1;
+            proj_ghost.for_each(
+                [&](int x, int y, int z)
+                {
+                    int cx = (x << dl) + (d[0] == -1 ? last : 0);
+                    int cy = (y << dl) + (d[1] == -1 ? last : 0);
+                    int cz = m.dim == 3 ? ((z << dl) + (d[2] == -1 ? last : 0)) : 0;
+                    if (m.exists(level, cx, cy, cz))
+                    {
+                        U(s, pl, x, y, z) = U(s, level, cx, cy, cz);
+                    }
+                });
+            if (pl == 0)
+            {
+                break;
+            }
+        }
+    }
+
+    // project_bc for layer 1 on a box domain (algorithm/update_outer_ghost.hpp:20-152)
+    static void project_bc(Sim& s, int pl, const Dir& dir)
+    {
+        const Mesh& m  = s.mesh;
+        DSet outside   = difference(translate(m.ids[UNION_CELLS][size_t(pl)], dir), m.domain[size_t(pl)]);
+        DSet ghosts    = on(intersection(outside, m.ids[REFERENCE][size_t(pl)]), pl);
+        ghosts.for_each(
+            [&](int x, int y, int z)
+            {
+                U(s, pl, x, y, z) = 0;
+            });
+        ghosts.for_each(
+            [&](int x, int y, int z)
+            {
+                double& g      = U(s, pl, x, y, z);
+                int n_children = 0;
+                for (int cl = pl + 1; cl <= pl + 2; ++cl)
+                {
+                    if (cl >= MAXL)
+                    {
+                        break;
+                    }
+                    DSet one = box_set(m.dim, pl, {x, y, z}, {x + 1, y + 1, z + 1});
+                    DSet ch  = intersection(on(one, cl), m.ids[REFERENCE][size_t(cl)]);
+                    ch.for_each(
+                        [&](int cx, int cy, int cz)
+                        {
+                            g += U(s, cl, cx, cy, cz);
+                            n_children++;
+                        });
+                    if (n_children > 0)
+                    {
+                        break;
+                    }
+                }
+                if (n_children > 0)
+                {
+                    g /= n_children;
+                }
+            });
+    }
+
+    // domain_boundary_outer_layer(mesh, level, direction, 1) (boundary.hpp:79-99)
+    static DSet outer_layer(const Mesh& m, int level, const Dir& dir)
+    {
+        const DSet& dom = m.domain[size_t(level)];
+        DSet inner      = difference(m.ids[CELLS][size_t(level)], translate(dom, scale(dir, -1)));
+        return difference(translate(inner, dir), dom);
+    }
+
+    // predict_bc (algorithm/update_outer_ghost.hpp:154-262): order-0 copy to level+1
+    static void predict_bc(Sim& s, int pred_level, const Dir& dir)
+    {
+        const Mesh& m  = s.mesh;
+        DSet bc_ghosts = outer_layer(m, pred_level - 1, dir);
+        DSet ghosts    = on(intersection(bc_ghosts, m.ids[REFERENCE][size_t(pred_level)]), pred_level);
+        ghosts.for_each(
+            [&](int x, int y, int z)
+            {
+                U(s, pred_level, x, y, z) = U(s, pred_level - 1, x >> 1, y >> 1, m.dim == 3 ? (z >> 1) : 0);
+            });
+    }
+
+    // update_outer_ghosts(level) (algorithm/update_outer_ghost.hpp:338-432)
+    static void update_outer_ghosts(Sim& s, int level)
+    {
+        const Mesh& m = s.mesh;
+        if (level <= m.max_level && level >= m.min_level)
+        {
+            for (size_t c = 0; c < m.corner_dirs.size(); ++c)
+            {
+                corner_extrapolation(s, level, c);
+                project_corner_below(s, level, c);
+            }
+        }
+        for (auto& dir : cartesian_directions(m.dim))
+        {
+            if (level < m.max_level)
+            {
+                project_bc(s, level, dir);
+            }
+            if (level >= m.min_level)
+            {
+                apply_field_bc(s, level, dir);
+            }
+            if (level < m.max_level && level >= m.min_level)
+            {
+                predict_bc(s, level + 1, dir);
+            }
+        }
+        // update_further_ghosts_by_polynomial_extrapolation: no layer beyond the
+        // first one exists for ghost width 1 (bc/apply_field_bc.hpp:499-563).
+    }
+
+    // projection_op_ (numeric/projection.hpp:15-65)
+    static double project_value(const Sim& s, int lf, int x, int y, int z)
+    {
+        const Mesh& m    = s.mesh;
+        const double inv = 1.0 / double(1ULL << m.dim);
+        double sum       = 0;
+        if (m.dim == 2)
+        {
+            for (int r = 0; r < 2; ++r)
+            {
+                sum += s.u[size_t(m.index(lf, 2 * x, 2 * y + r))] + s.u[size_t(m.index(lf, 2 * x + 1, 2 * y + r))];
+            }
+        }
+        else
+        {
+            for (int rz = 0; rz < 2; ++rz)
+            {
+                for (int ry = 0; ry < 2; ++ry)
+                {
+                    sum += s.u[size_t(m.index(lf, 2 * x, 2 * y + ry, 2 * z + rz))]
+                         + s.u[size_t(m.index(lf, 2 * x + 1, 2 * y + ry, 2 * z + rz))];
+                }
+            }
+        }
+        return sum * inv;
+    }
+
+    inline std::array<std::array<double, 3>, 2> interp_pair()
+    {
+        // interp_coeffs<3>(+1), interp_coeffs<3>(-1) (numeric/prediction.hpp:31-35)
+        return {{{1. / 8., 1, -1. / 8.}, {-1. / 8., 1, 1. / 8.}}};
+    }
+
+    // prediction_op <s,false> (numeric/prediction.hpp:259-357): value of the fine
+    // cell (l, x, y, z) from the coarse stencil at l-1.
+    static double predict_fine(const Sim& s, int l, int x, int y, int z)
+    {
+        const Mesh& m = s.mesh;
+        const int xc = x >> 1, yc = y >> 1, zc = m.dim == 3 ? (z >> 1) : 0;
+        if (m.pred_s == 0)
+        {
+            return s.u[size_t(m.index(l - 1, xc, yc, zc))];
+        }
+        auto pair = interp_pair();
+        int px = x & 1, py = y & 1, pz = z & 1;
+        double v = 0;
+        if (m.dim == 2)
+        {
+            for (int sy = 0; sy < 3; ++sy)
+            {
+                for (int sx = 0; sx < 3; ++sx)
+                {
+                    double f     = s.u[size_t(m.index(l - 1, xc + sx - 1, yc + sy - 1))];
+                    double coeff = 1.;
+                    coeff *= pair[size_t(px)][size_t(sx)];
+                    coeff *= pair[size_t(py)][size_t(sy)];
+                    v += f * coeff;
+                }
+            }
+        }
+        else
+        {
+            for (int sz = 0; sz < 3; ++sz)
+            {
+                for (int sy = 0; sy < 3; ++sy)
+                {
+                    for (int sx = 0; sx < 3; ++sx)
+                    {
+                        double f     = s.u[size_t(m.index(l - 1, xc + sx - 1, yc + sy - 1, zc + sz - 1))];
+                        double coeff = 1.;
+                        coeff *= pair[size_t(px)][size_t(sx)];
+                        coeff *= pair[size_t(py)][size_t(sy)];
+                        coeff *= pair[size_t(pz)][size_t(sz)];
+                        v += f * coeff;
+                    }
+                }
+            }
+        }
+        return v;
+    }
+
+    // update_ghost_mr_aggregated (algorithm/update_ghost_mr.hpp:194-237), serial.
+    void update_ghost_mr(Sim& s)
+    {
+        const Mesh& m = s.mesh;
+        for (int level = m.max_level; level >= 0; --level)
+        {
+            update_outer_ghosts(s, level);
+            if (level > 0)
+            {
+                DSet set = on(intersection(m.ids[REFERENCE][size_t(level)], m.ids[PROJ_CELLS][size_t(level - 1)]), level - 1);
+                set.for_each(
+                    [&](int x, int y, int z)
+                    {
+                        double v          = project_value(s, level, x, y, z);
+                        U(s, level - 1, x, y, z) = v;
+                    });
+            }
+        }
+        for (int level = 1; level <= m.max_level; ++level)
+        {
+            DSet pred_ghosts = difference(m.ids[REFERENCE][size_t(level)],
+                                          union_(m.ids[CELLS][size_t(level)], m.ids[PROJ_CELLS][size_t(level)]));
+            DSet expr = on(intersection(intersection(pred_ghosts, m.domain[size_t(level)]), m.ids[REFERENCE][size_t(level - 1)]),
+                           level);
+            expr.for_each(
+                [&](int x, int y, int z)
+                {
+                    double v             = predict_fine(s, level, x, y, z);
+                    U(s, level, x, y, z) = v;
+                });
+        }
+    }
+
+    // ------------------------------------------------------------------
+    // adaptation (mr/adapt.hpp:147-389)
+    // ------------------------------------------------------------------
+
+    // compute_detail_op (mr/operators.hpp:139-540) for the coarse cell (l; x,y,z)
+    static void compute_detail(Sim& s, int l, int x, int y, int z)
+    {
+        const Mesh& m = s.mesh;
+        const int lf  = l + 1;
+        const int nch = 1 << m.dim;
+        int64_t fidx[8];
+        for (int c = 0; c < nch; ++c)
+        {
+            int cx = 2 * x + (c & 1), cy = 2 * y + ((c >> 1) & 1), cz = m.dim == 3 ? 2 * z + ((c >> 2) & 1) : 0;
+            fidx[c] = m.index(lf, cx, cy, cz);
+        }
+        if (m.pred_s == 0)
+        {
+            double uc = s.u[size_t(m.index(l, x, y, z))];
+            for (int c = 0; c < nch; ++c)
+            {
+                s.detail[size_t(fidx[c])] = s.u[size_t(fidx[c])] - uc;
+            }
+            return;
+        }
+        auto pair   = interp_pair();
+        auto& ie    = pair[0];
+        auto& io    = pair[1];
+        double d[8];
+        for (int c = 0; c < nch; ++c)
+        {
+            d[c] = s.u[size_t(fidx[c])];
+        }
+        if (m.dim == 2)
+        {
+            for (int kj = 0; kj < 3; ++kj)
+            {
+                for (int ki = 0; ki < 3; ++ki)
+                {
+                    double src = s.u[size_t(m.index(l, x + ki - 1, y + kj - 1))];
+                    double c4[4] = {ie[size_t(ki)] * ie[size_t(kj)], io[size_t(ki)] * ie[size_t(kj)], ie[size_t(ki)] * io[size_t(kj)],
+                                    io[size_t(ki)] * io[size_t(kj)]};
+                    for (int k = 0; k < 4; ++k)
+                    {
+                        d[k] -= c4[k] * src;
+                    }
+                }
+            }
+        }
+        else
+        {
+            for (int kk = 0; kk < 3; ++kk)
+            {
+                for (int kj = 0; kj < 3; ++kj)
+                {
+                    for (int ki = 0; ki < 3; ++ki)
+                    {
+                        double src = s.u[size_t(m.index(l, x + ki - 1, y + kj - 1, z + kk - 1))];
+                        const double* cx[2] = {ie.data(), io.data()};
+                        double c8[8];
+                        for (int c = 0; c < 8; ++c)
+                        {
+                            c8[c] = cx[c & 1][ki] * cx[(c >> 1) & 1][kj] * cx[(c >> 2) & 1][kk];
+                        }
+                        for (int c = 0; c < 8; ++c)
+                        {
+                            d[c] -= c8[c] * src;
+                        }
+                    }
+                }
+            }
+        }
+        for (int c = 0; c < nch; ++c)
+        {
+            s.detail[size_t(fidx[c])] = d[c];
+        }
+    }
+
+    // mr_criteria_op (mr/criteria.hpp:13-122) for the coarse cell (level; x,y,z)
+    static void criteria(Sim& s, int level, int x, int y, int z, double eps, double regularity)
+    {
+        const Mesh& m          = s.mesh;
+        const int fine_level   = level + 1;
+        const int nch          = 1 << m.dim;
+        const double fine_eps  = std::pow(2.0, regularity) * eps;
+        const double coarse_eps = fine_eps / (1 << m.dim);
+        int64_t fidx[8];
+        for (int c = 0; c < nch; ++c)
+        {
+            fidx[c] = m.index(fine_level, 2 * x + (c & 1), 2 * y + ((c >> 1) & 1), m.dim == 3 ? 2 * z + ((c >> 2) & 1) : 0);
+        }
+        if (fine_level > m.min_level)
+        {
+            bool cond = std::abs(s.detail[size_t(m.index(level, x, y, z))]) > coarse_eps;
+            if (!cond)
+            {
+                bool coarsen_ok = true;
+                for (int c = 0; c < nch; ++c)
+                {
+                    if (std::abs(s.detail[size_t(fidx[c])]) > eps)
+                    {
+                        coarsen_ok = false;
+                    }
+                }
+                if (coarsen_ok)
+                {
+                    for (int c = 0; c < nch; ++c)
+                    {
+                        s.tag[size_t(fidx[c])] = COARSEN;
+                    }
+                }
+            }
+        }
+        if (fine_level < m.max_level)
+        {
+            for (int c = 0; c < nch; ++c)
+            {
+                bool r = std::abs(s.detail[size_t(fidx[c])]) > fine_eps;
+                s.tag[size_t(fidx[c])] |= uint8_t(r) * REFINE;
+            }
+        }
+    }
+
+    // maximum_op (mr/operators.hpp:22-96) for the coarse cell (level; x,y,z)
+    static void maximum(Sim& s, int level, int x, int y, int z)
+    {
+        const Mesh& m = s.mesh;
+        const int nch = 1 << m.dim;
+        int64_t fidx[8];
+        for (int c = 0; c < nch; ++c)
+        {
+            fidx[c] = m.index(level + 1, 2 * x + (c & 1), 2 * y + ((c >> 1) & 1), m.dim == 3 ? 2 * z + ((c >> 2) & 1) : 0);
+        }
+        int64_t ci = m.index(level, x, y, z);
+        bool keep  = false;
+        for (int c = 0; c < nch; ++c)
+        {
+            keep = keep || (s.tag[size_t(fidx[c])] & KEEP);
+        }
+        if (keep)
+        {
+            for (int c = 0; c < nch; ++c)
+            {
+                s.tag[size_t(fidx[c])] |= KEEP;
+            }
+            s.tag[size_t(ci)] |= KEEP;
+            return;
+        }
+        bool all_coarsen = true;
+        for (int c = 0; c < nch; ++c)
+        {
+            all_coarsen = all_coarsen && (s.tag[size_t(fidx[c])] & COARSEN);
+        }
+        if (all_coarsen)
+        {
+            s.tag[size_t(ci)] |= KEEP;
+        }
+        else
+        {
+            for (int c = 0; c < nch; ++c)
+            {
+                s.tag[size_t(fidx[c])] &= uint8_t(~COARSEN);
+            }
+        }
+    }
+
+    // update_cell_array_from_tag (algorithm/graduation.hpp:743-843)
+    static std::vector<DSet> update_cell_array_from_tag(const Sim& s)
+    {
+        const Mesh& m = s.mesh;
+        const int dim = m.dim;
+        std::vector<DSet> add_m = empty_ca(dim), rm_m = empty_ca(dim), add_p = empty_ca(dim), rm_p = empty_ca(dim);
+        // build each as a dense grid over the domain (+ nothing outside: cells are inside)
+        for (int l = 0; l < MAXL; ++l)
+        {
+            int n = 1 << std::min(l, 30);
+            if (l <= m.max_level + 1)
+            {
+                rm_p[size_t(l)] = box_set(dim, l, {0, 0, 0}, {n, n, dim == 3 ? n : 1});
+                std::fill(rm_p[size_t(l)].b.begin(), rm_p[size_t(l)].b.end(), 0);
+                rm_m[size_t(l)] = rm_p[size_t(l)];
+                add_p[size_t(l)] = rm_p[size_t(l)];
+                add_m[size_t(l)] = rm_p[size_t(l)];
+            }
+        }
+        const int start = m.ca_min_level(CELLS), end = m.ca_max_level(CELLS) + 1;
+        for (int level = start; level < end; ++level)
+        {
+            m.ids[CELLS][size_t(level)].for_each(
+                [&](int x, int y, int z)
+                {
+                    uint8_t t       = s.tag[size_t(m.index(level, x, y, z))];
+                    bool refine     = t & REFINE;
+                    bool coarsen_nk = (t & COARSEN) && !(t & KEEP);
+                    if (refine && level < m.max_level)
+                    {
+                        rm_p[size_t(level)].set(x, y, z);
+                        for (int c = 0; c < (1 << dim); ++c)
+                        {
+                            add_p[size_t(level + 1)].set(2 * x + (c & 1), 2 * y + ((c >> 1) & 1), dim == 3 ? 2 * z + ((c >> 2) & 1) : 0);
+                        }
+                    }
+                    else if (coarsen_nk && level > m.min_level)
+                    {
+                        bool yz_even = (y % 2 == 0) && (dim == 2 || z % 2 == 0);
+                        if (x % 2 == 0 && yz_even)
+                        {
+                            add_m[size_t(level - 1)].set(x >> 1, y >> 1, dim == 3 ? (z >> 1) : 0);
+                        }
+                        rm_m[size_t(level)].set(x, y, z);
+                    }
+                });
+        }
+        std::vector<DSet> out = empty_ca(dim);
+        for (int level = m.min_level; level <= m.max_level; ++level)
+        {
+            DSet a = union_(union_(m.ids[CELLS][size_t(level)], add_m[size_t(level)]), add_p[size_t(level)]);
+            DSet r = union_(rm_m[size_t(level)], rm_p[size_t(level)]);
+            out[size_t(level)] = tighten(difference(a, r));
+        }
+        return out;
+    }
+
+    static int ca_min(const std::vector<DSet>& ca)
+    {
+        for (int l = 0; l < MAXL; ++l)
+        {
+            if (!ca[size_t(l)].empty())
+            {
+                return l;
+            }
+        }
+        return MAXL;
+    }
+    static int ca_max(const std::vector<DSet>& ca)
+    {
+        for (int l = MAXL - 1; l >= 0; --l)
+        {
+            if (!ca[size_t(l)].empty())
+            {
+                return l;
+            }
+        }
+        return 0;
+    }
+    static bool ca_equal(const std::vector<DSet>& a, const std::vector<DSet>& b)
+    {
+        for (int l = 0; l < MAXL; ++l)
+        {
+            if (!same_cells(a[size_t(l)], b[size_t(l)]))
+            {
+                return false;
+            }
+        }
+        return true;
+    }
+
+    // make_graduation (algorithm/graduation.hpp:573-726) with the graduation
+    // listing of list_interval_to_refine_for_graduation (:245-330); the
+    // contiguous-boundary refinement (:369-497) is a no-op for stencil radius 1.
+    static int make_graduation(std::vector<DSet>& ca, int dim, int grad_width)
+    {
+        const int max_level = ca_max(ca);
+        const int min_level = ca_min(ca);
+        int nit             = 0;
+        bool changed        = true;
+        while (changed)
+        {
+            ++nit;
+            std::vector<DSet> remove = empty_ca(dim);
+            {
+                const int fmax     = ca_max(ca) + 1;
+                const int cmin     = ca_min(ca);
+                const int min_fine = (cmin + 2) - 1;
+                for (int fine_level = fmax; fine_level > min_fine; --fine_level)
+                {
+                    if (fine_level >= MAXL || ca[size_t(fine_level)].empty())
+                    {
+                        continue;
+                    }
+                    DSet proj = on(expand(ca[size_t(fine_level)], 2 * grad_width), fine_level - 2);
+                    for (int coarse_level = fine_level - 2;; --coarse_level)
+                    {
+                        if (!proj.empty())
+                        {
+                            remove[size_t(coarse_level)] = union_(remove[size_t(coarse_level)], intersection(proj, ca[size_t(coarse_level)]));
+                        }
+                        if (coarse_level == cmin || proj.empty())
+                        {
+                            break;
+                        }
+                        proj = on(proj, coarse_level - 1);
+                    }
+                }
+            }
+            bool any_change = false;
+            std::vector<DSet> add_p = empty_ca(dim);
+            for (int level = min_level; level < max_level + 1; ++level)
+            {
+                remove[size_t(level)] = tighten(remove[size_t(level)]);
+                if (!remove[size_t(level)].empty())
+                {
+                    any_change = true;
+                    add_p[size_t(level + 1)] = refine(remove[size_t(level)], 1);
+                }
+            }
+            if (!any_change)
+            {
+                changed = false;
+                continue;
+            }
+            std::vector<DSet> new_ca = empty_ca(dim);
+            int lo = std::min(ca_min(ca), ca_min(add_p)), hi = std::max(ca_max(ca), ca_max(add_p));
+            for (int level = lo; level < hi + 1; ++level)
+            {
+                new_ca[size_t(level)] = tighten(difference(union_(ca[size_t(level)], add_p[size_t(level)]), remove[size_t(level)]));
+            }
+            changed = !ca_equal(new_ca, ca);
+            std::swap(new_ca, ca);
+        }
+        return nit - 1;
+    }
+
+    static std::vector<double> new_storage(const Sim& s, int64_t n)
+    {
+        return std::vector<double>(size_t(n), s.poison ? std::numeric_limits<double>::quiet_NaN() : 0.0);
+    }
+
+    // prediction_op <s,true> (numeric/prediction.hpp:149-257): children of the
+    // coarse cell (l-1; xc,yc,zc) of the new mesh from the old field.
+    static void predict_children(const Mesh& om, const std::vector<double>& src, const Mesh& nm, std::vector<double>& dst,
+                                 int lc, int xc, int yc, int zc)
+    {
+        const int dim = om.dim, nch = 1 << dim;
+        double vals[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (om.pred_s == 0)
+        {
+            double v = src[size_t(om.index(lc, xc, yc, zc))];
+            for (int c = 0; c < nch; ++c)
+            {
+                vals[c] = v;
+            }
+        }
+        else
+        {
+            auto pair = interp_pair();
+            const int nzs = dim == 3 ? 3 : 1;
+            for (int sz = 0; sz < nzs; ++sz)
+            {
+                for (int sy = 0; sy < 3; ++sy)
+                {
+                    for (int sx = 0; sx < 3; ++sx)
+                    {
+                        double f = src[size_t(om.index(lc, xc + sx - 1, yc + sy - 1, dim == 3 ? zc + sz - 1 : 0))];
+                        for (int c = 0; c < nch; ++c)
+                        {
+                            double coeff = 1.;
+                            coeff *= pair[size_t(c & 1)][size_t(sx)];
+                            coeff *= pair[size_t((c >> 1) & 1)][size_t(sy)];
+                            if (dim == 3)
+                            {
+                                coeff *= pair[size_t((c >> 2) & 1)][size_t(sz)];
+                            }
+                            vals[c] += f * coeff;
+                        }
+                    }
+                }
+            }
+        }
+        for (int c = 0; c < nch; ++c)
+        {
+            int cx = 2 * xc + (c & 1), cy = 2 * yc + ((c >> 1) & 1), cz = dim == 3 ? 2 * zc + ((c >> 2) & 1) : 0;
+            dst[size_t(nm.index(lc + 1, cx, cy, cz))] = vals[c];
+        }
+    }
+
+    // update_fields (algorithm/update_fields.hpp:27-127) for one scalar field.
+    static void update_fields(Sim& s, Mesh& nm)
+    {
+        const Mesh& om          = s.mesh;
+        std::vector<double> dst(size_t(nm.nref), 0.0); // make_new_field fills 0 (:56-66)
+        for (int level = om.min_level; level <= om.max_level; ++level)
+        {
+            DSet set = intersection(om.ids[REFERENCE][size_t(level)], nm.ids[CELLS][size_t(level)]);
+            set.for_each(
+                [&](int x, int y, int z)
+                {
+                    dst[size_t(nm.index(level, x, y, z))] = s.u[size_t(om.index(level, x, y, z))];
+                });
+        }
+        for (int level = om.min_level + 1; level <= om.max_level; ++level)
+        {
+            DSet set_coarsen = on(intersection(om.ids[CELLS][size_t(level)], nm.ids[CELLS][size_t(level - 1)]), level - 1);
+            set_coarsen.for_each(
+                [&](int x, int y, int z)
+                {
+                    dst[size_t(nm.index(level - 1, x, y, z))] = project_value(s, level, x, y, z);
+                });
+            DSet set_refine = on(intersection(nm.ids[CELLS][size_t(level)], om.ids[CELLS][size_t(level - 1)]), level - 1);
+            set_refine.for_each(
+                [&](int x, int y, int z)
+                {
+                    predict_children(om, s.u, nm, dst, level - 1, x, y, z);
+                });
+        }
+        s.u = std::move(dst);
+    }
+
+    // Adapt::harten (mr/adapt.hpp:276-389); returns true at the fixed point.
+    bool harten(Sim& s, int ite, double eps, double reg)
+    {
+        Mesh& m                 = s.mesh;
+        const int min_level = m.min_level, max_level = m.max_level, dim = m.dim;
+        s.detail.assign(size_t(m.nref), 0.0);
+        s.tag.assign(size_t(m.nref), 0);
+        for (int l = 0; l < MAXL; ++l)
+        {
+            m.ids[CELLS][size_t(l)].for_each(
+                [&](int x, int y, int z)
+                {
+                    s.tag[size_t(m.index(l, x, y, z))] = KEEP;
+                });
+        }
+        update_ghost_mr(s);
+
+        for (int level = (min_level > 0 ? min_level - 1 : 0); level < max_level - ite; ++level)
+        {
+            DSet below = on(intersection(m.ids[REFERENCE][size_t(level)],
+                                         union_(m.ids[CELLS][size_t(level + 1)], m.ids[CELLS][size_t(std::min(level + 2, MAXL - 1))])),
+                            level);
+            below.for_each(
+                [&](int x, int y, int z)
+                {
+                    compute_detail(s, level, x, y, z);
+                });
+        }
+        for (int level = min_level; level <= max_level - ite; ++level)
+        {
+            int exponent             = dim * (max_level - level);
+            double eps_l             = eps / (1 << exponent);
+            double regularity_to_use = reg + dim;
+            DSet sub = on(intersection(m.ids[CELLS][size_t(level)], m.ids[REFERENCE][size_t(level - 1)]), level - 1);
+            sub.for_each(
+                [&](int x, int y, int z)
+                {
+                    criteria(s, level - 1, x, y, z, eps_l, regularity_to_use);
+                });
+        }
+        for (int level = max_level; level > 0; --level)
+        {
+            DSet sub = on(intersection(m.ids[CELLS][size_t(level)], m.ids[REFERENCE][size_t(level - 1)]), level - 1);
+            sub.for_each(
+                [&](int x, int y, int z)
+                {
+                    maximum(s, level - 1, x, y, z);
+                });
+        }
+        s.tag_mesh    = m;
+        s.last_tag    = s.tag;
+        s.last_detail = s.detail;
+
+        std::vector<DSet> new_ca = update_cell_array_from_tag(s);
+        make_graduation(new_ca, dim, m.grad_width);
+        if (ca_equal(m.ids[CELLS], new_ca))
+        {
+            return true;
+        }
+        Mesh nm = make_mesh_from_cells(m, new_ca);
+        update_fields(s, nm);
+        s.mesh = std::move(nm);
+        return false;
+    }
+
+    // Adapt::operator()(mra_config&) (mr/adapt.hpp:147-195)
+    int adapt(Sim& s, double eps, double reg)
+    {
+        const int min_level = s.mesh.min_level, max_level = s.mesh.max_level;
+        if (min_level == max_level)
+        {
+            s.last_harten_iterations = 0;
+            return 0;
+        }
+        int i = 0;
+        for (; i < max_level - min_level; ++i)
+        {
+            if (harten(s, i, eps, reg))
+            {
+                ++i;
+                break;
+            }
+        }
+        s.last_harten_iterations = i;
+        return i;
+    }
+
+    // flux of upwind_op (stencil_field.hpp:91-97)
+    inline double flux(double a, double ul, double ur)
+    {
+        return .5 * a * (ul + ur) + .5 * std::abs(a) * (ul - ur);
+    }
+
+    // unp1 = u - dt * upwind(a, u) over mesh[cells] (stencil_field.hpp:37-53,
+    // field/field_base.hpp:230-242), then std::swap(u.array(), unp1.array())
+    // (demos/FiniteVolume/advection_2d.cpp:142-146).
+    void upwind(Sim& s, const double* a, double dt)
+    {
+        const Mesh& m = s.mesh;
+        std::vector<double> unp1 = new_storage(s, m.nref);
+        for (int level = 0; level < MAXL; ++level)
+        {
+            const double dx = 1.0 / (1 << level); // cell_length (cell.hpp:17-20), unit box
+            m.ids[CELLS][size_t(level)].for_each(
+                [&](int x, int y, int z)
+                {
+                    double uc = s.u[size_t(m.index(level, x, y, z))];
+                    double S;
+                    if (m.dim == 2)
+                    {
+                        double FL = flux(a[0], s.u[size_t(m.index(level, x - 1, y))], uc);
+                        double FR = flux(a[0], uc, s.u[size_t(m.index(level, x + 1, y))]);
+                        double FD = flux(a[1], s.u[size_t(m.index(level, x, y - 1))], uc);
+                        double FU = flux(a[1], uc, s.u[size_t(m.index(level, x, y + 1))]);
+                        S         = -FL + FR + -FD + FU;
+                    }
+                    else
+                    {
+                        double FL = flux(a[0], s.u[size_t(m.index(level, x - 1, y, z))], uc);
+                        double FR = flux(a[0], uc, s.u[size_t(m.index(level, x + 1, y, z))]);
+                        double FD = flux(a[1], s.u[size_t(m.index(level, x, y - 1, z))], uc);
+                        double FU = flux(a[1], uc, s.u[size_t(m.index(level, x, y + 1, z))]);
+                        double FF = flux(a[2], s.u[size_t(m.index(level, x, y, z - 1))], uc);
+                        double FB = flux(a[2], uc, s.u[size_t(m.index(level, x, y, z + 1))]);
+                        S         = -FL + FR + -FD + FU + -FF + FB;
+                    }
+                    unp1[size_t(m.index(level, x, y, z))] = uc - dt * (S / dx);
+                });
+        }
+        s.u.swap(unp1);
+    }
+
+    Sim* make_box_sim(int dim, int min_level, int max_level, int pred_s, int radius, int grad_width)
+    {
+        auto* s           = new Sim();
+        Mesh cfg;
+        cfg.dim           = dim;
+        cfg.min_level     = min_level;
+        cfg.max_level     = max_level;
+        cfg.pred_s        = pred_s;
+        cfg.radius        = radius;
+        cfg.grad_width    = grad_width;
+        std::vector<DSet> cells = empty_ca(dim);
+        cells[size_t(max_level)] = domain_at(dim, max_level); // start_level = max_level (mr/mesh.hpp:510-518)
+        s->mesh = make_mesh_from_cells(cfg, cells);
+        s->u    = new_storage(*s, s->mesh.nref);
+        return s;
+    }
+} // namespace oracle
+
+// ----------------------------------------------------------------------
+// C API used by the Python tests (ctypes).
+// ----------------------------------------------------------------------
+using namespace oracle;
+
+static thread_local std::string g_err;
+
+#define ORC_TRY(body)                                                                                                                 \
+    try                                                                                                                               \
+    {                                                                                                                                 \
+        body;                                                                                                                         \
+    }                                                                                                                                 \
+    catch (const std::exception& e)                                                                                                  \
+    {                                                                                                                                 \
+        g_err = e.what();                                                                                                             \
+        return -1;                                                                                                                    \
+    }
+
+extern "C"
+{
+    const char* orc_last_error() { return g_err.c_str(); }
+
+    void* orc_create_box(int dim, int min_level, int max_level, int pred_s, int radius, int grad_width)
+    {
+        try
+        {
+            if (dim != 2 && dim != 3)
+            {
+                throw std::invalid_argument("dim must be 2 or 3");
+            }
+            if (radius != 1 || grad_width < 1 || pred_s < 0 || pred_s > 1 || max_level < min_level || min_level < 1)
+            {
+                throw std::invalid_argument("unsupported mesh configuration");
+            }
+            return make_box_sim(dim, min_level, max_level, pred_s, radius, grad_width);
+        }
+        catch (const std::exception& e)
+        {
+            g_err = e.what();
+            return nullptr;
+        }
+    }
+
+    void orc_free(void* p) { delete static_cast<Sim*>(p); }
+
+    void orc_set_poison(void* p, int on) { static_cast<Sim*>(p)->poison = on != 0; }
+
+    // number of cells of a mesh id; REFERENCE gives the storage size.
+    int64_t orc_nb_cells(void* p, int id)
+    {
+        auto* s = static_cast<Sim*>(p);
+        if (id == REFERENCE)
+        {
+            return s->mesh.nref;
+        }
+        int64_t n = 0;
+        for (int l = 0; l < MAXL; ++l)
+        {
+            n += s->mesh.ids[id][size_t(l)].count();
+        }
+        return n;
+    }
+
+    // cells of a mesh id in for_each_cell order: rows of (level, x, y, z, index)
+    int orc_get_cells(void* p, int id, int64_t* out)
+    {
+        auto* s = static_cast<Sim*>(p);
+        ORC_TRY({
+            int64_t k = 0;
+            for (int l = 0; l < MAXL; ++l)
+            {
+                s->mesh.ids[id][size_t(l)].for_each(
+                    [&](int x, int y, int z)
+                    {
+                        out[5 * k + 0] = l;
+                        out[5 * k + 1] = x;
+                        out[5 * k + 2] = y;
+                        out[5 * k + 3] = z;
+                        out[5 * k + 4] = s->mesh.exists(l, x, y, z) ? s->mesh.index(l, x, y, z) : -1;
+                        ++k;
+                    });
+            }
+        });
+        return 0;
+    }
+
+    int orc_set_cells_values(void* p, const double* v)
+    {
+        auto* s = static_cast<Sim*>(p);
+        ORC_TRY({
+            int64_t k = 0;
+            for (int l = 0; l < MAXL; ++l)
+            {
+                s->mesh.ids[CELLS][size_t(l)].for_each(
+                    [&](int x, int y, int z)
+                    {
+                        s->u[size_t(s->mesh.index(l, x, y, z))] = v[k++];
+                    });
+            }
+        });
+        return 0;
+    }
+
+    int orc_get_cells_values(void* p, double* v)
+    {
+        auto* s = static_cast<Sim*>(p);
+        ORC_TRY({
+            int64_t k = 0;
+            for (int l = 0; l < MAXL; ++l)
+            {
+                s->mesh.ids[CELLS][size_t(l)].for_each(
+                    [&](int x, int y, int z)
+                    {
+                        v[k++] = s->u[size_t(s->mesh.index(l, x, y, z))];
+                    });
+            }
+        });
+        return 0;
+    }
+
+    int orc_get_storage(void* p, double* v)
+    {
+        auto* s = static_cast<Sim*>(p);
+        std::memcpy(v, s->u.data(), s->u.size() * sizeof(double));
+        return 0;
+    }
+
+    int orc_set_storage(void* p, const double* v)
+    {
+        auto* s = static_cast<Sim*>(p);
+        std::memcpy(s->u.data(), v, s->u.size() * sizeof(double));
+        return 0;
+    }
+
+    void orc_set_dirichlet(void* p, double value)
+    {
+        auto* s     = static_cast<Sim*>(p);
+        s->has_bc   = true;
+        s->bc_value = value;
+    }
+
+    int orc_update_ghost_mr(void* p)
+    {
+        ORC_TRY(update_ghost_mr(*static_cast<Sim*>(p)));
+        return 0;
+    }
+
+    // one MRadaptation(mra_config) call; returns the number of harten calls
+    int orc_adapt(void* p, double eps, double reg)
+    {
+        int n = 0;
+        ORC_TRY(n = adapt(*static_cast<Sim*>(p), eps, reg));
+        return n;
+    }
+
+    // one harten(ite) call: 1 at the fixed point, 0 if the mesh changed
+    int orc_harten(void* p, int ite, double eps, double reg)
+    {
+        int r = 0;
+        ORC_TRY(r = harten(*static_cast<Sim*>(p), ite, eps, reg) ? 1 : 0);
+        return r;
+    }
+
+    // tags/details of the last harten call, over the storage of the mesh they
+    // were computed on (size orc_tag_storage_size)
+    int64_t orc_tag_storage_size(void* p) { return static_cast<Sim*>(p)->tag_mesh.nref; }
+    int orc_get_last_tags(void* p, uint8_t* t, double* d)
+    {
+        auto* s = static_cast<Sim*>(p);
+        std::memcpy(t, s->last_tag.data(), s->last_tag.size());
+        std::memcpy(d, s->last_detail.data(), s->last_detail.size() * sizeof(double));
+        return 0;
+    }
+
+    // update_ghost_mr(u) then unp1 = u - dt*upwind(a,u), swap
+    int orc_upwind_step(void* p, const double* a, double dt)
+    {
+        auto* s = static_cast<Sim*>(p);
+        ORC_TRY({
+            update_ghost_mr(*s);
+            upwind(*s, a, dt);
+        });
+        return 0;
+    }
+
+    // unp1 = u - dt*upwind(a,u) alone (no ghost update), swap
+    int orc_upwind_only(void* p, const double* a, double dt)
+    {
+        ORC_TRY(upwind(*static_cast<Sim*>(p), a, dt));
+        return 0;
+    }
+}

@@ -1,0 +1,131 @@
+"""ctypes binding of the oracle (oracle/liboracle.so) for the tests and the
+CPU-baseline leg of bench.py.  The oracle is the checker, never the product."""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+
+CELLS, CELLS_AND_GHOSTS, PROJ_CELLS, UNION_CELLS, REFERENCE = range(5)
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle not built: {LIB_PATH} (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i32, i64, dbl = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double
+        P = ctypes.POINTER
+        L.orc_last_error.restype = ctypes.c_char_p
+        L.orc_create_box.restype = vp
+        L.orc_create_box.argtypes = [i32] * 6
+        L.orc_free.argtypes = [vp]
+        L.orc_set_poison.argtypes = [vp, i32]
+        L.orc_nb_cells.restype = i64
+        L.orc_nb_cells.argtypes = [vp, i32]
+        L.orc_get_cells.argtypes = [vp, i32, P(i64)]
+        L.orc_set_cells_values.argtypes = [vp, P(dbl)]
+        L.orc_get_cells_values.argtypes = [vp, P(dbl)]
+        L.orc_get_storage.argtypes = [vp, P(dbl)]
+        L.orc_set_storage.argtypes = [vp, P(dbl)]
+        L.orc_set_dirichlet.argtypes = [vp, dbl]
+        L.orc_update_ghost_mr.argtypes = [vp]
+        L.orc_adapt.argtypes = [vp, dbl, dbl]
+        L.orc_harten.argtypes = [vp, i32, dbl, dbl]
+        L.orc_tag_storage_size.restype = i64
+        L.orc_tag_storage_size.argtypes = [vp]
+        L.orc_get_last_tags.argtypes = [vp, P(ctypes.c_uint8), P(dbl)]
+        L.orc_upwind_step.argtypes = [vp, P(dbl), dbl]
+        L.orc_upwind_only.argtypes = [vp, P(dbl), dbl]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _check(rc):
+    if rc < 0:
+        raise RuntimeError(lib().orc_last_error().decode())
+    return rc
+
+
+class OracleSim:
+    """One scalar field on an MR mesh of the unit box (oracle side)."""
+
+    def __init__(self, dim, min_level, max_level, pred_s=1, radius=1, grad_width=1):
+        self.dim = dim
+        self.h = lib().orc_create_box(dim, min_level, max_level, pred_s, radius, grad_width)
+        if not self.h:
+            raise RuntimeError(lib().orc_last_error().decode())
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_free(self.h)
+            self.h = None
+
+    def set_poison(self, on):
+        lib().orc_set_poison(self.h, int(on))
+
+    def nb_cells(self, mesh_id=CELLS):
+        return lib().orc_nb_cells(self.h, mesh_id)
+
+    def cells(self, mesh_id=CELLS):
+        """(n, 5) int64 rows (level, x, y, z, storage index) in traversal order."""
+        n = self.nb_cells(mesh_id)
+        out = np.zeros((n, 5), dtype=np.int64)
+        _check(lib().orc_get_cells(self.h, mesh_id, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+        return out
+
+    def set_values(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        assert v.size == self.nb_cells(CELLS)
+        _check(lib().orc_set_cells_values(self.h, _dp(v)))
+
+    def values(self):
+        out = np.zeros(self.nb_cells(CELLS))
+        _check(lib().orc_get_cells_values(self.h, _dp(out)))
+        return out
+
+    def storage(self):
+        out = np.zeros(self.nb_cells(REFERENCE))
+        lib().orc_get_storage(self.h, _dp(out))
+        return out
+
+    def set_storage(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        assert v.size == self.nb_cells(REFERENCE)
+        lib().orc_set_storage(self.h, _dp(v))
+
+    def set_dirichlet(self, value=0.0):
+        lib().orc_set_dirichlet(self.h, float(value))
+
+    def update_ghost_mr(self):
+        _check(lib().orc_update_ghost_mr(self.h))
+
+    def adapt(self, eps, reg):
+        return _check(lib().orc_adapt(self.h, float(eps), float(reg)))
+
+    def harten(self, ite, eps, reg):
+        return _check(lib().orc_harten(self.h, int(ite), float(eps), float(reg)))
+
+    def last_tags(self):
+        n = lib().orc_tag_storage_size(self.h)
+        t = np.zeros(n, dtype=np.uint8)
+        d = np.zeros(n)
+        lib().orc_get_last_tags(self.h, t.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), _dp(d))
+        return t, d
+
+    def upwind_step(self, a, dt):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        _check(lib().orc_upwind_step(self.h, _dp(a), float(dt)))
+
+    def upwind_only(self, a, dt):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        _check(lib().orc_upwind_only(self.h, _dp(a), float(dt)))
