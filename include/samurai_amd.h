@@ -1,0 +1,145 @@
+/* C ABI of the MI355X implementation of samurai's adaptive-MR time step.
+ *
+ * samurai (hpc-maths/samurai v0.33.0) is a header-only C++ library with no FFI:
+ * each entry point below replaces one C++ call of the reference's public API
+ * for this path (cited as file:line relative to the samurai source tree).  The
+ * C++ drop-in headers in include/samurai_amd/ wrap these functions under the
+ * reference's names; the Python tests bind them with ctypes.
+ *
+ * Conventions: every function returns 0 (or a documented non-negative value)
+ * on success and a negative status on failure, with the message available
+ * from smr_last_error() on the calling thread.  No C++ exception crosses the
+ * ABI.  All device work of a context is ordered on the context's HIP stream; a
+ * context, and every object created from it, is used by one thread at a time.
+ * Fields are FP64, stored on the device in the reference's storage order
+ * (levels ascending, then rows (z, y), then x; mesh.hpp:893-911).
+ */
+#ifndef SAMURAI_AMD_H
+#define SAMURAI_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct smr_ctx smr_ctx;
+typedef struct smr_mesh smr_mesh;
+typedef struct smr_field smr_field;
+typedef struct smr_adapt smr_adapt;
+
+/* MRMeshId (mr/mesh.hpp:25-34) */
+enum smr_mesh_id
+{
+    SMR_CELLS            = 0,
+    SMR_CELLS_AND_GHOSTS = 1,
+    SMR_PROJ_CELLS       = 2,
+    SMR_UNION_CELLS      = 3,
+    SMR_REFERENCE        = 4
+};
+
+enum smr_status
+{
+    SMR_OK            = 0,
+    SMR_ERR_ARGUMENT  = -1, /* std::invalid_argument in the reference */
+    SMR_ERR_RANGE     = -2, /* std::out_of_range (cell absent from the mesh) */
+    SMR_ERR_RUNTIME   = -3, /* HIP or other runtime failure */
+    SMR_ERR_NO_DEVICE = -4  /* a device call on a host-only context */
+};
+
+/* samurai::mesh_config<dim, prediction_stencil_radius> (mesh_config.hpp:16-432),
+ * with max_stencil_size(2).disable_minimal_ghost_width() as in
+ * demos/FiniteVolume/advection_2d.cpp:101 (max_stencil_radius must be 1). */
+typedef struct smr_mesh_config
+{
+    int dim;                       /* 2 or 3 */
+    int min_level;
+    int max_level;
+    int prediction_stencil_radius; /* 0 or 1 */
+    int max_stencil_radius;        /* 1 */
+    int graduation_width;          /* default 1 */
+} smr_mesh_config;
+
+/* samurai::initialize (samurai.hpp:47-113): a context on HIP device `device`,
+ * or a host-only context (mesh construction only) when device < 0. */
+int smr_ctx_create(int device, smr_ctx** out);
+int smr_ctx_destroy(smr_ctx* ctx);
+int smr_ctx_synchronize(smr_ctx* ctx);
+const char* smr_last_error(void);
+
+/* samurai::mra::make_mesh(Box unit square/cube, cfg) (mr/mesh.hpp:481-529) */
+int smr_mesh_create_box(smr_ctx* ctx, const smr_mesh_config* cfg, smr_mesh** out);
+/* MRMesh(ca, cfg) (mesh.hpp:386-396): leaves given as n rows (level, x, y, z) */
+int smr_mesh_create_from_cells(smr_ctx* ctx, const smr_mesh_config* cfg, const int32_t* cells, int64_t n, smr_mesh** out);
+int smr_mesh_destroy(smr_mesh* mesh);
+/* mesh.nb_cells(mesh_id) (mesh.hpp:540-551); SMR_REFERENCE gives the storage size */
+int64_t smr_mesh_nb_cells(const smr_mesh* mesh, int mesh_id);
+/* for_each_cell(mesh[mesh_id]) (algorithm.hpp:319-323): n rows of
+ * (level, x, y, z, storage index) in traversal order */
+int smr_mesh_get_cells(const smr_mesh* mesh, int mesh_id, int64_t* out);
+/* the host half of Adapt::harten's mesh update (mr/adapt.hpp:359-386):
+ * update_cell_array_from_tag (algorithm/graduation.hpp:743-843) and
+ * make_graduation (:573-726) from tags indexed by the reference storage, then
+ * MRMesh(new_ca, mesh) (mesh.hpp:398-408).  Returns 1 at the fixed point (mesh
+ * unchanged), 0 if the mesh was rebuilt.  Host only: fields on the mesh must
+ * be resized afterwards, their values are not transferred. */
+int smr_mesh_rebuild_from_tags(smr_mesh* mesh, const uint8_t* tags, int64_t n);
+
+/* make_scalar_field<double>(name, mesh) (field/scalar_field.hpp:171-176) */
+int smr_field_create(smr_mesh* mesh, const char* name, smr_field** out);
+int smr_field_destroy(smr_field* field);
+/* field.resize() after a mesh change (field/access_base.hpp:105-119) */
+int smr_field_resize(smr_field* field);
+/* values on mesh[cells] in traversal order (like for_each_cell + u[cell]) */
+int smr_field_set_cells(smr_field* field, const double* values);
+int smr_field_get_cells(smr_field* field, double* values);
+/* the whole storage (u.array()) */
+int smr_field_get_storage(smr_field* field, double* values);
+int smr_field_set_storage(smr_field* field, const double* values);
+int64_t smr_field_size(const smr_field* field);
+/* device pointer of the storage (for interop with torch or other HIP code) */
+double* smr_field_device_data(smr_field* field);
+/* make_bc<Dirichlet<1>>(u, value) (bc/bc.hpp:771-795, bc/dirichlet.hpp:29) */
+int smr_field_attach_dirichlet(smr_field* field, double value);
+/* samurai::swap(u, v) / std::swap(u.array(), v.array()) (field/swap.hpp:12-19) */
+int smr_field_swap(smr_field* a, smr_field* b);
+
+/* update_ghost_mr(u) (algorithm/update_ghost_mr.hpp:260-270) */
+int smr_update_ghost_mr(smr_field* field);
+/* unp1 = u - dt * upwind(a, u) over mesh[cells] (stencil_field.hpp:175-179,
+ * field/field_base.hpp:230-242); a has dim components */
+int smr_upwind(const double* a, double dt, smr_field* u, smr_field* unp1);
+
+/* make_MRAdapt(u) (mr/adapt.hpp:391-397) */
+int smr_adapt_create(smr_field* field, smr_adapt** out);
+int smr_adapt_destroy(smr_adapt* adapt);
+/* MRadaptation(mra_config().epsilon(eps).regularity(reg)) (mr/adapt.hpp:147-195):
+ * returns the number of harten iterations run (>= 0) */
+int smr_adapt_run(smr_adapt* adapt, double eps, double regularity);
+/* one Adapt::harten(ite) call (mr/adapt.hpp:276-389): 1 at the fixed point, 0 if
+ * the mesh changed */
+int smr_adapt_harten(smr_adapt* adapt, int ite, double eps, double regularity);
+/* tags (CellFlag bits, cell_flag.hpp:11-17) and details of the last harten
+ * call, indexed by the storage of the mesh they were computed on */
+int64_t smr_adapt_last_size(const smr_adapt* adapt);
+int smr_adapt_last_tags(const smr_adapt* adapt, uint8_t* tags, double* details);
+
+/* Diagnostics (no counterpart in the reference): checks on the host that
+ * every work-list row the mesh's kernels will touch exists and that each
+ * run's access window lies inside one reference interval -- the same check
+ * the device performs before any operator kernel runs.  0 if valid,
+ * SMR_ERR_RANGE otherwise. */
+int smr_mesh_validate(const smr_mesh* mesh);
+/* The HIP stream (hipStream_t) the context orders its work on, for timing
+ * with HIP events; NULL for a host-only context. */
+void* smr_ctx_stream(smr_ctx* ctx);
+/* Algorithmic HBM bytes of one upwind launch on mesh[cells]: 8 bytes per
+ * distinct cell read (the leaves and their face neighbours) plus 8 bytes per
+ * leaf written. */
+int64_t smr_mesh_upwind_bytes(const smr_mesh* mesh);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1238,6 +1238,50 @@ extern "C"
         }
     }
 
+    // MRMesh(ca, cfg) from n leaves given as rows (level, x, y, z)
+    void* orc_create_from_cells(int dim, int min_level, int max_level, int pred_s, int radius, int grad_width, const int32_t* cells,
+                                int64_t n)
+    {
+        try
+        {
+            if ((dim != 2 && dim != 3) || radius != 1 || pred_s < 0 || pred_s > 1 || max_level < min_level || min_level < 1)
+            {
+                throw std::invalid_argument("unsupported mesh configuration");
+            }
+            Mesh cfg;
+            cfg.dim        = dim;
+            cfg.min_level  = min_level;
+            cfg.max_level  = max_level;
+            cfg.pred_s     = pred_s;
+            cfg.radius     = radius;
+            cfg.grad_width = grad_width;
+            std::vector<DSet> ca = empty_ca(dim);
+            for (int l = 0; l < MAXL; ++l)
+            {
+                int n_l = 1 << std::min(l, 30);
+                if (l <= max_level)
+                {
+                    ca[size_t(l)] = box_set(dim, l, {0, 0, 0}, {n_l, n_l, dim == 3 ? n_l : 1});
+                    std::fill(ca[size_t(l)].b.begin(), ca[size_t(l)].b.end(), 0);
+                }
+            }
+            for (int64_t k = 0; k < n; ++k)
+            {
+                const int32_t* r = cells + 4 * k;
+                ca[size_t(r[0])].set(r[1], r[2], dim == 3 ? r[3] : 0);
+            }
+            auto* s = new Sim();
+            s->mesh = make_mesh_from_cells(cfg, ca);
+            s->u    = new_storage(*s, s->mesh.nref);
+            return s;
+        }
+        catch (const std::exception& e)
+        {
+            g_err = e.what();
+            return nullptr;
+        }
+    }
+
     void orc_free(void* p) { delete static_cast<Sim*>(p); }
 
     void orc_set_poison(void* p, int on) { static_cast<Sim*>(p)->poison = on != 0; }

@@ -1,0 +1,281 @@
+"""MI355X-native adaptive-MR time step with samurai's API (ctypes binding of
+libsamurai_amd.so, whose C ABI is declared in include/samurai_amd.h).
+
+Names follow the reference's C++ API for this path (hpc-maths/samurai v0.33.0):
+``mra.make_mesh`` -> :func:`make_mesh`, ``make_scalar_field`` -> :func:`make_scalar_field`,
+``make_bc<Dirichlet<1>>`` -> :func:`make_bc_dirichlet`, ``update_ghost_mr`` ->
+:func:`update_ghost_mr`, ``unp1 = u - dt * upwind(a, u)`` -> :func:`upwind`,
+``make_MRAdapt`` -> :func:`make_MRAdapt`, ``samurai::swap`` -> :func:`swap`.
+The library must be built (``python samurai_amd/build.py``); nothing here
+falls back to a CPU implementation.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsamurai_amd.so")
+
+CELLS, CELLS_AND_GHOSTS, PROJ_CELLS, UNION_CELLS, REFERENCE = range(5)
+KEEP, COARSEN, REFINE = 1, 2, 4
+
+
+class SamuraiError(RuntimeError):
+    pass
+
+
+class MeshConfig(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int), ("min_level", ctypes.c_int), ("max_level", ctypes.c_int),
+                ("prediction_stencil_radius", ctypes.c_int), ("max_stencil_radius", ctypes.c_int),
+                ("graduation_width", ctypes.c_int)]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded C ABI (raises if the shared library has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SamuraiError(f"{LIB_PATH} is missing: build it with python samurai_amd/build.py")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i32, i64, dbl = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double
+        P, PP = ctypes.POINTER, lambda t: ctypes.POINTER(ctypes.POINTER(t))
+        L.smr_last_error.restype = ctypes.c_char_p
+        L.smr_ctx_create.argtypes = [i32, ctypes.POINTER(vp)]
+        L.smr_ctx_destroy.argtypes = [vp]
+        L.smr_ctx_synchronize.argtypes = [vp]
+        L.smr_mesh_create_box.argtypes = [vp, P(MeshConfig), ctypes.POINTER(vp)]
+        L.smr_mesh_create_from_cells.argtypes = [vp, P(MeshConfig), P(ctypes.c_int32), i64, ctypes.POINTER(vp)]
+        L.smr_mesh_destroy.argtypes = [vp]
+        L.smr_mesh_nb_cells.restype = i64
+        L.smr_mesh_nb_cells.argtypes = [vp, i32]
+        L.smr_mesh_get_cells.argtypes = [vp, i32, P(i64)]
+        L.smr_mesh_rebuild_from_tags.argtypes = [vp, P(ctypes.c_uint8), i64]
+        L.smr_mesh_validate.argtypes = [vp]
+        L.smr_ctx_stream.restype = vp
+        L.smr_ctx_stream.argtypes = [vp]
+        L.smr_mesh_upwind_bytes.restype = i64
+        L.smr_mesh_upwind_bytes.argtypes = [vp]
+        L.smr_field_create.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(vp)]
+        L.smr_field_destroy.argtypes = [vp]
+        L.smr_field_resize.argtypes = [vp]
+        L.smr_field_set_cells.argtypes = [vp, P(dbl)]
+        L.smr_field_get_cells.argtypes = [vp, P(dbl)]
+        L.smr_field_get_storage.argtypes = [vp, P(dbl)]
+        L.smr_field_set_storage.argtypes = [vp, P(dbl)]
+        L.smr_field_size.restype = i64
+        L.smr_field_size.argtypes = [vp]
+        L.smr_field_device_data.restype = vp
+        L.smr_field_device_data.argtypes = [vp]
+        L.smr_field_attach_dirichlet.argtypes = [vp, dbl]
+        L.smr_field_swap.argtypes = [vp, vp]
+        L.smr_update_ghost_mr.argtypes = [vp]
+        L.smr_upwind.argtypes = [P(dbl), dbl, vp, vp]
+        L.smr_adapt_create.argtypes = [vp, ctypes.POINTER(vp)]
+        L.smr_adapt_destroy.argtypes = [vp]
+        L.smr_adapt_run.argtypes = [vp, dbl, dbl]
+        L.smr_adapt_harten.argtypes = [vp, i32, dbl, dbl]
+        L.smr_adapt_last_size.restype = i64
+        L.smr_adapt_last_size.argtypes = [vp]
+        L.smr_adapt_last_tags.argtypes = [vp, P(ctypes.c_uint8), P(dbl)]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc < 0:
+        raise SamuraiError(f"samurai_amd error {rc}: {lib().smr_last_error().decode()}")
+    return rc
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Context:
+    """A device (``device >= 0``) or host-only (``device < 0``) context."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        _check(lib().smr_ctx_create(int(device), ctypes.byref(h)))
+        self.h = h
+        self.device = device
+
+    def synchronize(self):
+        _check(lib().smr_ctx_synchronize(self.h))
+
+    def stream(self):
+        """The hipStream_t (as an integer) all work of this context runs on."""
+        return lib().smr_ctx_stream(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            lib().smr_ctx_destroy(self.h)
+            self.h = None
+
+
+class MRMesh:
+    """samurai::MRMesh on the unit square/cube (mr/mesh.hpp)."""
+
+    def __init__(self, ctx, handle, cfg):
+        self.ctx, self.h, self.cfg = ctx, handle, cfg
+        self.dim = cfg.dim
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            lib().smr_mesh_destroy(self.h)
+            self.h = None
+
+    @property
+    def min_level(self):
+        return self.cfg.min_level
+
+    @property
+    def max_level(self):
+        return self.cfg.max_level
+
+    def nb_cells(self, mesh_id=CELLS):
+        return _check(lib().smr_mesh_nb_cells(self.h, mesh_id))
+
+    def cells(self, mesh_id=CELLS):
+        """(n, 5) int64 rows (level, x, y, z, storage index) in traversal order."""
+        n = self.nb_cells(mesh_id)
+        out = np.zeros((n, 5), dtype=np.int64)
+        _check(lib().smr_mesh_get_cells(self.h, mesh_id, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+        return out
+
+    def upwind_bytes(self):
+        """Algorithmic HBM bytes of one upwind launch on this mesh."""
+        return _check(lib().smr_mesh_upwind_bytes(self.h))
+
+    def validate(self):
+        """Host check that every kernel access of this mesh's programs is in bounds."""
+        return _check(lib().smr_mesh_validate(self.h))
+
+    def rebuild_from_tags(self, tags):
+        tags = np.ascontiguousarray(tags, dtype=np.uint8)
+        return _check(lib().smr_mesh_rebuild_from_tags(
+            self.h, tags.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), tags.size))
+
+
+def mesh_config(dim, min_level, max_level, prediction_stencil_radius=1, max_stencil_radius=1, graduation_width=1):
+    return MeshConfig(dim, min_level, max_level, prediction_stencil_radius, max_stencil_radius, graduation_width)
+
+
+def make_mesh(ctx, cfg):
+    """samurai::mra::make_mesh(unit box, cfg): every cell at max_level."""
+    h = ctypes.c_void_p()
+    _check(lib().smr_mesh_create_box(ctx.h, ctypes.byref(cfg), ctypes.byref(h)))
+    return MRMesh(ctx, h, cfg)
+
+
+def make_mesh_from_cells(ctx, cfg, cells):
+    """MRMesh(ca, cfg) from (n, >=4) rows (level, x, y, z)."""
+    c = np.ascontiguousarray(np.asarray(cells)[:, :4], dtype=np.int32)
+    h = ctypes.c_void_p()
+    _check(lib().smr_mesh_create_from_cells(ctx.h, ctypes.byref(cfg), c.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                            c.shape[0], ctypes.byref(h)))
+    return MRMesh(ctx, h, cfg)
+
+
+class ScalarField:
+    """samurai::ScalarField<mesh, double> stored on the device."""
+
+    def __init__(self, mesh, name):
+        h = ctypes.c_void_p()
+        _check(lib().smr_field_create(mesh.h, name.encode(), ctypes.byref(h)))
+        self.mesh, self.h, self.name = mesh, h, name
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            lib().smr_field_destroy(self.h)
+            self.h = None
+
+    def resize(self):
+        _check(lib().smr_field_resize(self.h))
+
+    def size(self):
+        return lib().smr_field_size(self.h)
+
+    def device_ptr(self):
+        return lib().smr_field_device_data(self.h)
+
+    def set_cells(self, values):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        if v.size != self.mesh.nb_cells(CELLS):
+            raise ValueError("one value per leaf cell expected")
+        _check(lib().smr_field_set_cells(self.h, _dp(v)))
+
+    def cells(self):
+        out = np.zeros(self.mesh.nb_cells(CELLS))
+        _check(lib().smr_field_get_cells(self.h, _dp(out)))
+        return out
+
+    def storage(self):
+        out = np.zeros(self.size())
+        _check(lib().smr_field_get_storage(self.h, _dp(out)))
+        return out
+
+    def set_storage(self, values):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        if v.size != self.size():
+            raise ValueError("storage size mismatch")
+        _check(lib().smr_field_set_storage(self.h, _dp(v)))
+
+
+def make_scalar_field(name, mesh):
+    return ScalarField(mesh, name)
+
+
+def make_bc_dirichlet(field, value=0.0):
+    """make_bc<Dirichlet<1>>(u, value)."""
+    _check(lib().smr_field_attach_dirichlet(field.h, float(value)))
+
+
+def swap(u, v):
+    _check(lib().smr_field_swap(u.h, v.h))
+
+
+def update_ghost_mr(field):
+    _check(lib().smr_update_ghost_mr(field.h))
+
+
+def upwind(a, dt, u, unp1):
+    """unp1 = u - dt * upwind(a, u) on mesh[cells]."""
+    av = np.ascontiguousarray(a, dtype=np.float64)
+    _check(lib().smr_upwind(_dp(av), float(dt), u.h, unp1.h))
+
+
+class Adapt:
+    """samurai::Adapt built by make_MRAdapt(u) (mr/adapt.hpp)."""
+
+    def __init__(self, field):
+        h = ctypes.c_void_p()
+        _check(lib().smr_adapt_create(field.h, ctypes.byref(h)))
+        self.field, self.h = field, h
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            lib().smr_adapt_destroy(self.h)
+            self.h = None
+
+    def __call__(self, epsilon=1e-4, regularity=1.0):
+        """MRadaptation(mra_config().epsilon(eps).regularity(reg)); returns #harten calls."""
+        return _check(lib().smr_adapt_run(self.h, float(epsilon), float(regularity)))
+
+    def harten(self, ite, epsilon, regularity):
+        return _check(lib().smr_adapt_harten(self.h, int(ite), float(epsilon), float(regularity)))
+
+    def last_tags(self):
+        n = lib().smr_adapt_last_size(self.h)
+        t = np.zeros(max(n, 0), dtype=np.uint8)
+        d = np.zeros(max(n, 0))
+        _check(lib().smr_adapt_last_tags(self.h, t.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), _dp(d)))
+        return t, d
+
+
+def make_MRAdapt(field):  # noqa: N802 (reference name)
+    return Adapt(field)

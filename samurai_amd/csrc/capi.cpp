@@ -1,0 +1,587 @@
+// extern "C" boundary (include/samurai_amd.h) over the runtime.
+#include "../../include/samurai_amd.h"
+#include "runtime.hpp"
+
+#include <cstring>
+#include <string>
+
+struct smr_ctx
+{
+    smr::Ctx c;
+};
+struct smr_mesh
+{
+    smr::Mesh m;
+};
+struct smr_field
+{
+    smr::Field f;
+};
+struct smr_adapt
+{
+    smr::AdaptState a;
+};
+
+namespace
+{
+    thread_local std::string g_error;
+
+    template <class F>
+    int guarded(F&& f)
+    {
+        try
+        {
+            return f();
+        }
+        catch (const std::invalid_argument& e)
+        {
+            g_error = e.what();
+            return SMR_ERR_ARGUMENT;
+        }
+        catch (const std::out_of_range& e)
+        {
+            g_error = e.what();
+            return SMR_ERR_RANGE;
+        }
+        catch (const std::exception& e)
+        {
+            g_error = e.what();
+            std::string w = e.what();
+            return w.find("has no GPU") != std::string::npos ? SMR_ERR_NO_DEVICE : SMR_ERR_RUNTIME;
+        }
+        catch (...)
+        {
+            g_error = "unknown error";
+            return SMR_ERR_RUNTIME;
+        }
+    }
+
+    void require(bool cond, const char* what)
+    {
+        if (!cond)
+        {
+            throw std::invalid_argument(what);
+        }
+    }
+
+    smr::MeshConfig to_cfg(const smr_mesh_config* c)
+    {
+        require(c != nullptr, "null mesh config");
+        require(c->dim == 2 || c->dim == 3, "dim must be 2 or 3");
+        smr::MeshConfig cfg;
+        cfg.dim        = c->dim;
+        cfg.min_level  = c->min_level;
+        cfg.max_level  = c->max_level;
+        cfg.pred_s     = c->prediction_stencil_radius;
+        cfg.radius     = c->max_stencil_radius;
+        cfg.grad_width = c->graduation_width;
+        require(cfg.grad_width >= 1, "graduation width must be >= 1");
+        return cfg;
+    }
+
+    std::vector<int64_t> cell_storage_index(const smr::MRMeshData& m)
+    {
+        std::vector<int64_t> idx;
+        idx.reserve(size_t(m.nb_cells(smr::CELLS)));
+        for (int l = 0; l < smr::MAXL; ++l)
+        {
+            const smr::LevelSet& s = m.ids[smr::CELLS][size_t(l)];
+            s.for_each_run(
+                [&](int32_t x0, int32_t x1, int32_t y, int32_t z)
+                {
+                    const int64_t b = m.index(l, x0, y, z) - x0;
+                    for (int32_t x = x0; x < x1; ++x)
+                    {
+                        idx.push_back(b + x);
+                    }
+                });
+        }
+        return idx;
+    }
+} // namespace
+
+extern "C"
+{
+    const char* smr_last_error(void)
+    {
+        return g_error.c_str();
+    }
+
+    int smr_ctx_create(int device, smr_ctx** out)
+    {
+        return guarded(
+            [&]
+            {
+                require(out != nullptr, "null output pointer");
+                auto* c     = new smr_ctx();
+                c->c.device = device;
+                if (device >= 0)
+                {
+                    try
+                    {
+                        smr::hip_check(hipSetDevice(device), "hipSetDevice");
+                        smr::hip_check(hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking), "hipStreamCreate");
+                        smr::hip_check(hipMalloc(reinterpret_cast<void**>(&c->c.d_err), sizeof(int)), "hipMalloc(err)");
+                        smr::hip_check(hipMemsetAsync(c->c.d_err, 0, sizeof(int), c->c.stream), "hipMemsetAsync(err)");
+                        c->c.sync();
+                    }
+                    catch (...)
+                    {
+                        delete c;
+                        throw;
+                    }
+                }
+                *out = c;
+                return 0;
+            });
+    }
+
+    int smr_ctx_destroy(smr_ctx* ctx)
+    {
+        return guarded(
+            [&]
+            {
+                delete ctx;
+                return 0;
+            });
+    }
+
+    int smr_ctx_synchronize(smr_ctx* ctx)
+    {
+        return guarded(
+            [&]
+            {
+                require(ctx != nullptr, "null context");
+                if (ctx->c.has_device())
+                {
+                    ctx->c.sync();
+                }
+                return 0;
+            });
+    }
+
+    int smr_mesh_create_box(smr_ctx* ctx, const smr_mesh_config* cfg, smr_mesh** out)
+    {
+        return guarded(
+            [&]
+            {
+                require(ctx && out, "null argument");
+                auto* m   = new smr_mesh();
+                m->m.ctx  = &ctx->c;
+                try
+                {
+                    m->m.host = smr::build_box_mesh(to_cfg(cfg));
+                }
+                catch (...)
+                {
+                    delete m;
+                    throw;
+                }
+                *out = m;
+                return 0;
+            });
+    }
+
+    int smr_mesh_create_from_cells(smr_ctx* ctx, const smr_mesh_config* cfg, const int32_t* cells, int64_t n, smr_mesh** out)
+    {
+        return guarded(
+            [&]
+            {
+                require(ctx && out && (cells || n == 0), "null argument");
+                smr::MeshConfig c = to_cfg(cfg);
+                std::vector<std::vector<std::array<int32_t, 4>>> q(smr::MAXL);
+                for (int64_t k = 0; k < n; ++k)
+                {
+                    const int32_t* r = cells + 4 * k;
+                    require(r[0] >= 0 && r[0] < smr::MAXL, "cell level out of range");
+                    q[size_t(r[0])].push_back({r[2], c.dim == 3 ? r[3] : 0, r[1], r[1] + 1});
+                }
+                smr::CellArray ca = smr::empty_cell_array(c.dim);
+                for (int l = 0; l < smr::MAXL; ++l)
+                {
+                    ca[size_t(l)] = smr::from_runs(c.dim, l, q[size_t(l)]);
+                }
+                auto* m  = new smr_mesh();
+                m->m.ctx = &ctx->c;
+                try
+                {
+                    m->m.host = smr::build_mesh(c, ca);
+                }
+                catch (...)
+                {
+                    delete m;
+                    throw;
+                }
+                *out = m;
+                return 0;
+            });
+    }
+
+    int smr_mesh_destroy(smr_mesh* mesh)
+    {
+        return guarded(
+            [&]
+            {
+                delete mesh;
+                return 0;
+            });
+    }
+
+    int64_t smr_mesh_nb_cells(const smr_mesh* mesh, int mesh_id)
+    {
+        if (!mesh || mesh_id < 0 || mesh_id >= smr::N_MESH_IDS)
+        {
+            g_error = "bad mesh or mesh id";
+            return SMR_ERR_ARGUMENT;
+        }
+        return mesh->m.host.nb_cells(mesh_id);
+    }
+
+    int smr_mesh_get_cells(const smr_mesh* mesh, int mesh_id, int64_t* out)
+    {
+        return guarded(
+            [&]
+            {
+                require(mesh && out && mesh_id >= 0 && mesh_id < smr::N_MESH_IDS, "bad argument");
+                const smr::MRMeshData& m = mesh->m.host;
+                int64_t k                = 0;
+                for (int l = 0; l < smr::MAXL; ++l)
+                {
+                    m.ids[mesh_id][size_t(l)].for_each_run(
+                        [&](int32_t x0, int32_t x1, int32_t y, int32_t z)
+                        {
+                            int64_t b     = 0;
+                            const bool ok = m.run_base(l, x0, y, z, b);
+                            for (int32_t x = x0; x < x1; ++x)
+                            {
+                                int64_t* r = out + 5 * k++;
+                                r[0]       = l;
+                                r[1]       = x;
+                                r[2]       = y;
+                                r[3]       = z;
+                                r[4]       = ok ? b + x : -1;
+                            }
+                        });
+                }
+                return 0;
+            });
+    }
+
+    int smr_mesh_rebuild_from_tags(smr_mesh* mesh, const uint8_t* tags, int64_t n)
+    {
+        return guarded(
+            [&]
+            {
+                require(mesh && tags, "null argument");
+                smr::MRMeshData& m = mesh->m.host;
+                require(n == m.nref, "tag array size differs from the storage size");
+                smr::CellArray ca = smr::update_cell_array_from_tag(m, tags);
+                smr::make_graduation(ca, m.cfg.dim, m.cfg.grad_width);
+                if (smr::ca_equal(ca, m.ids[smr::CELLS]))
+                {
+                    return 1;
+                }
+                smr::MRMeshData nm = smr::build_mesh(m.cfg, ca);
+                smr::WorkList copy, coarsen, refine;
+                smr::build_update_fields_lists(m, nm, copy, coarsen, refine);
+                for (const smr::WorkList* w : {&copy, &coarsen, &refine})
+                {
+                    std::string why = smr::validate_list(*w, nm, m);
+                    if (!why.empty())
+                    {
+                        throw std::out_of_range("update_fields: " + why);
+                    }
+                }
+                mesh->m.replace(std::move(nm), nullptr);
+                return 0;
+            });
+    }
+
+    void* smr_ctx_stream(smr_ctx* ctx)
+    {
+        return ctx ? static_cast<void*>(ctx->c.stream) : nullptr;
+    }
+
+    int64_t smr_mesh_upwind_bytes(const smr_mesh* mesh)
+    {
+        if (!mesh)
+        {
+            g_error = "null mesh";
+            return SMR_ERR_ARGUMENT;
+        }
+        const smr::MRMeshData& m = mesh->m.host;
+        int64_t bytes            = 0;
+        for (int l = 0; l < smr::MAXL; ++l)
+        {
+            const smr::LevelSet& c = m.ids[smr::CELLS][size_t(l)];
+            if (c.empty())
+            {
+                continue;
+            }
+            smr::LevelSet read = c;
+            for (const smr::Dir& d : smr::cartesian_directions(m.cfg.dim))
+            {
+                read = smr::set_union(read, smr::translate(c, d));
+            }
+            bytes += 8 * read.count() + 8 * c.count();
+        }
+        return bytes;
+    }
+
+    int smr_mesh_validate(const smr_mesh* mesh)
+    {
+        return guarded(
+            [&]
+            {
+                require(mesh != nullptr, "null mesh");
+                std::string why = smr::validate_programs(mesh->m.host);
+                if (!why.empty())
+                {
+                    throw std::out_of_range(why);
+                }
+                return 0;
+            });
+    }
+
+    int smr_field_create(smr_mesh* mesh, const char* name, smr_field** out)
+    {
+        return guarded(
+            [&]
+            {
+                require(mesh && out, "null argument");
+                auto* f   = new smr_field();
+                f->f.mesh = &mesh->m;
+                f->f.name = name ? name : "";
+                try
+                {
+                    f->f.allocate(true);
+                }
+                catch (...)
+                {
+                    delete f;
+                    throw;
+                }
+                *out = f;
+                return 0;
+            });
+    }
+
+    int smr_field_destroy(smr_field* field)
+    {
+        return guarded(
+            [&]
+            {
+                delete field;
+                return 0;
+            });
+    }
+
+    int smr_field_resize(smr_field* field)
+    {
+        return guarded(
+            [&]
+            {
+                require(field != nullptr, "null field");
+                if (field->f.version != field->f.mesh->version || field->f.size != field->f.mesh->host.nref)
+                {
+                    field->f.allocate(true);
+                }
+                return 0;
+            });
+    }
+
+    int smr_field_set_cells(smr_field* field, const double* values)
+    {
+        return guarded(
+            [&]
+            {
+                require(field && values, "null argument");
+                smr::Field& f = field->f;
+                f.check_current();
+                auto idx = cell_storage_index(f.mesh->host);
+                std::vector<double> st(size_t(f.size));
+                hipStream_t s = f.mesh->ctx->stream;
+                smr::hip_check(hipMemcpyAsync(st.data(), f.d, st.size() * 8, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+                f.mesh->ctx->sync();
+                for (size_t k = 0; k < idx.size(); ++k)
+                {
+                    st[size_t(idx[k])] = values[k];
+                }
+                smr::hip_check(hipMemcpyAsync(f.d, st.data(), st.size() * 8, hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+                f.mesh->ctx->sync();
+                return 0;
+            });
+    }
+
+    int smr_field_get_cells(smr_field* field, double* values)
+    {
+        return guarded(
+            [&]
+            {
+                require(field && values, "null argument");
+                smr::Field& f = field->f;
+                f.check_current();
+                auto idx = cell_storage_index(f.mesh->host);
+                std::vector<double> st(size_t(f.size));
+                smr::hip_check(hipMemcpyAsync(st.data(), f.d, st.size() * 8, hipMemcpyDeviceToHost, f.mesh->ctx->stream), "hipMemcpyAsync");
+                f.mesh->ctx->sync();
+                for (size_t k = 0; k < idx.size(); ++k)
+                {
+                    values[k] = st[size_t(idx[k])];
+                }
+                return 0;
+            });
+    }
+
+    int smr_field_get_storage(smr_field* field, double* values)
+    {
+        return guarded(
+            [&]
+            {
+                require(field && values, "null argument");
+                smr::Field& f = field->f;
+                f.check_current();
+                smr::hip_check(hipMemcpyAsync(values, f.d, size_t(f.size) * 8, hipMemcpyDeviceToHost, f.mesh->ctx->stream), "hipMemcpyAsync");
+                f.mesh->ctx->sync();
+                return 0;
+            });
+    }
+
+    int smr_field_set_storage(smr_field* field, const double* values)
+    {
+        return guarded(
+            [&]
+            {
+                require(field && values, "null argument");
+                smr::Field& f = field->f;
+                f.check_current();
+                smr::hip_check(hipMemcpyAsync(f.d, values, size_t(f.size) * 8, hipMemcpyHostToDevice, f.mesh->ctx->stream), "hipMemcpyAsync");
+                f.mesh->ctx->sync();
+                return 0;
+            });
+    }
+
+    int64_t smr_field_size(const smr_field* field)
+    {
+        return field ? field->f.size : -1;
+    }
+
+    double* smr_field_device_data(smr_field* field)
+    {
+        return field ? field->f.d : nullptr;
+    }
+
+    int smr_field_attach_dirichlet(smr_field* field, double value)
+    {
+        return guarded(
+            [&]
+            {
+                require(field != nullptr, "null field");
+                field->f.has_bc   = true;
+                field->f.bc_value = value;
+                return 0;
+            });
+    }
+
+    int smr_field_swap(smr_field* a, smr_field* b)
+    {
+        return guarded(
+            [&]
+            {
+                require(a && b, "null field");
+                require(a->f.mesh == b->f.mesh, "swap of fields on different meshes");
+                std::swap(a->f.d, b->f.d);
+                std::swap(a->f.size, b->f.size);
+                std::swap(a->f.version, b->f.version);
+                return 0;
+            });
+    }
+
+    int smr_update_ghost_mr(smr_field* field)
+    {
+        return guarded(
+            [&]
+            {
+                require(field != nullptr, "null field");
+                smr::update_ghost_mr(field->f);
+                return 0;
+            });
+    }
+
+    int smr_upwind(const double* a, double dt, smr_field* u, smr_field* unp1)
+    {
+        return guarded(
+            [&]
+            {
+                require(a && u && unp1, "null argument");
+                smr::upwind(a, dt, u->f, unp1->f);
+                return 0;
+            });
+    }
+
+    int smr_adapt_create(smr_field* field, smr_adapt** out)
+    {
+        return guarded(
+            [&]
+            {
+                require(field && out, "null argument");
+                auto* a    = new smr_adapt();
+                a->a.field = &field->f;
+                *out       = a;
+                return 0;
+            });
+    }
+
+    int smr_adapt_destroy(smr_adapt* adapt)
+    {
+        return guarded(
+            [&]
+            {
+                delete adapt;
+                return 0;
+            });
+    }
+
+    int smr_adapt_run(smr_adapt* adapt, double eps, double regularity)
+    {
+        return guarded(
+            [&]
+            {
+                require(adapt != nullptr, "null adapt");
+                return smr::adapt(adapt->a, eps, regularity);
+            });
+    }
+
+    int smr_adapt_harten(smr_adapt* adapt, int ite, double eps, double regularity)
+    {
+        return guarded(
+            [&]
+            {
+                require(adapt != nullptr, "null adapt");
+                return smr::harten(adapt->a, ite, eps, regularity) ? 1 : 0;
+            });
+    }
+
+    int64_t smr_adapt_last_size(const smr_adapt* adapt)
+    {
+        return adapt ? adapt->a.last_size : -1;
+    }
+
+    int smr_adapt_last_tags(const smr_adapt* adapt, uint8_t* tags, double* details)
+    {
+        return guarded(
+            [&]
+            {
+                require(adapt != nullptr, "null adapt");
+                if (tags)
+                {
+                    std::memcpy(tags, adapt->a.last_tag.data(), adapt->a.last_tag.size());
+                }
+                if (details)
+                {
+                    std::memcpy(details, adapt->a.last_detail.data(), adapt->a.last_detail.size() * 8);
+                }
+                return 0;
+            });
+    }
+}

@@ -1,0 +1,61 @@
+// Launchers of the MR-step kernels (kernels.hip).  Plain pointers and sizes;
+// every launch is asynchronous on the given stream.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+
+namespace smr::dev
+{
+    // Flattened reference interval table of one mesh (all levels).
+    struct Table
+    {
+        const int64_t* row_key; // ((int64)z << 32) + (y + 2^31), sorted per level
+        const int64_t* row_off; // MAXL+1: first row of each level
+        const int64_t* rp;      // first run of each row (+1 sentinel)
+        const int32_t* run_s;
+        const int32_t* run_e;
+        const int64_t* run_idx; // value index = run_idx + x
+    };
+
+    // A work list: x-runs (x0, x1, y, z, level), cell prefix, chunk hints and
+    // the resolved storage bases of its taps.
+    struct List
+    {
+        const int32_t* iv;
+        const int64_t* pref;
+        const int32_t* hint; // run holding cell 256*j, j = 0..ceil(ncells/256)
+        int64_t* base;       // nruns * ntaps
+        const int8_t* taps;  // 7 per tap: table, dl, dx, dy, dz, wlo, whi
+        int64_t nruns;
+        int64_t ncells;
+        int ntaps;
+    };
+
+    struct Thresholds
+    {
+        double eps[21];
+        double fine[21];
+        double coarse[21];
+    };
+
+    constexpr int HINT_SHIFT = 8; // 256 cells per hint chunk
+
+    void resolve(const List& l, const Table& t0, const Table& t1, int* err, hipStream_t s);
+    void upwind(int dim, const List& l, int64_t c0, int64_t c1, const double* u, double* unp1, double a0, double a1, double a2,
+                double dt, hipStream_t s);
+    void project(int dim, const List& l, int64_t c0, int64_t c1, double* dst, const double* src, hipStream_t s);
+    void predict_fine(int dim, int pred_s, const List& l, int64_t c0, int64_t c1, double* u, hipStream_t s);
+    void detail(int dim, int pred_s, const List& l, int64_t c0, int64_t c1, const double* u, double* det, hipStream_t s);
+    void criteria(int dim, const List& l, int64_t c0, int64_t c1, const double* det, uint8_t* tag, const Thresholds& thr, int min_level,
+                  int max_level, hipStream_t s);
+    void maximum(int dim, const List& l, int64_t c0, int64_t c1, uint8_t* tag, hipStream_t s);
+    void tag_init(const List& l, int64_t c0, int64_t c1, uint8_t* tag, hipStream_t s);
+    void copy_cells(const List& l, int64_t c0, int64_t c1, double* dst, const double* src, hipStream_t s);
+    void predict_children(int dim, int pred_s, const List& l, int64_t c0, int64_t c1, double* dst, const double* src, hipStream_t s);
+    // one dependency-free stage of outer-ghost recipes
+    // (Dirichlet recipes are skipped when the field has no boundary condition)
+    void recipes(const int64_t* dst, const int32_t* op, const int64_t* src_ptr, const int64_t* src, int64_t r0, int64_t r1, double* u,
+                 bool has_bc, double bc_value, hipStream_t s);
+} // namespace smr::dev

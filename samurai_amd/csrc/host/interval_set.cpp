@@ -1,0 +1,461 @@
+#include "interval_set.hpp"
+
+#include <algorithm>
+#include <stdexcept>
+
+namespace smr
+{
+    int64_t LevelSet::count() const
+    {
+        int64_t n = 0;
+        for (const auto& r : runs)
+        {
+            n += r.e - r.s;
+        }
+        return n;
+    }
+
+    bool LevelSet::operator==(const LevelSet& o) const
+    {
+        if (runs.size() != o.runs.size() || ry.size() != o.ry.size())
+        {
+            return false;
+        }
+        if (!runs.empty() && level != o.level)
+        {
+            return false;
+        }
+        if (ry != o.ry || rz != o.rz || rp != o.rp)
+        {
+            return false;
+        }
+        for (size_t k = 0; k < runs.size(); ++k)
+        {
+            if (runs[k].s != o.runs[k].s || runs[k].e != o.runs[k].e)
+            {
+                return false;
+            }
+        }
+        return true;
+    }
+
+    void LevelSet::push(int32_t y, int32_t z, int32_t s, int32_t e)
+    {
+        if (e <= s)
+        {
+            return;
+        }
+        if (!ry.empty() && ry.back() == y && rz.back() == z)
+        {
+            Run& b = runs.back();
+            if (s <= b.e)
+            {
+                if (e > b.e)
+                {
+                    b.e = e;
+                }
+                return;
+            }
+            runs.push_back({s, e});
+            rp.back() = int64_t(runs.size());
+        }
+        else
+        {
+            ry.push_back(y);
+            rz.push_back(z);
+            runs.push_back({s, e});
+            rp.push_back(int64_t(runs.size()));
+        }
+    }
+
+    int64_t LevelSet::find_row(int32_t y, int32_t z) const
+    {
+        int64_t lo = 0, hi = int64_t(ry.size());
+        while (lo < hi)
+        {
+            int64_t mid = (lo + hi) / 2;
+            if (row_less(ry[size_t(mid)], rz[size_t(mid)], y, z))
+            {
+                lo = mid + 1;
+            }
+            else
+            {
+                hi = mid;
+            }
+        }
+        if (lo < int64_t(ry.size()) && ry[size_t(lo)] == y && rz[size_t(lo)] == z)
+        {
+            return lo;
+        }
+        return -1;
+    }
+
+    int64_t LevelSet::find_run(int64_t r, int32_t x) const
+    {
+        int64_t lo = rp[size_t(r)], hi = rp[size_t(r) + 1];
+        // last run with s <= x
+        while (lo < hi)
+        {
+            int64_t mid = (lo + hi) / 2;
+            if (runs[size_t(mid)].s <= x)
+            {
+                lo = mid + 1;
+            }
+            else
+            {
+                hi = mid;
+            }
+        }
+        int64_t k = lo - 1;
+        if (k >= rp[size_t(r)] && x < runs[size_t(k)].e)
+        {
+            return k;
+        }
+        return -1;
+    }
+
+    bool LevelSet::contains(int32_t x, int32_t y, int32_t z) const
+    {
+        int64_t r = find_row(y, z);
+        return r >= 0 && find_run(r, x) >= 0;
+    }
+
+    namespace
+    {
+        enum class Op
+        {
+            Union,
+            Intersection,
+            Difference
+        };
+
+        void union_runs(const Run* a, size_t na, const Run* b, size_t nb, int32_t y, int32_t z, LevelSet& out)
+        {
+            size_t p = 0, q = 0;
+            bool have = false;
+            Run cur{0, 0};
+            while (p < na || q < nb)
+            {
+                Run r;
+                if (q >= nb || (p < na && a[p].s <= b[q].s))
+                {
+                    r = a[p++];
+                }
+                else
+                {
+                    r = b[q++];
+                }
+                if (!have)
+                {
+                    cur  = r;
+                    have = true;
+                }
+                else if (r.s <= cur.e)
+                {
+                    cur.e = std::max(cur.e, r.e);
+                }
+                else
+                {
+                    out.push(y, z, cur.s, cur.e);
+                    cur = r;
+                }
+            }
+            if (have)
+            {
+                out.push(y, z, cur.s, cur.e);
+            }
+        }
+
+        void intersect_runs(const Run* a, size_t na, const Run* b, size_t nb, int32_t y, int32_t z, LevelSet& out)
+        {
+            size_t p = 0, q = 0;
+            while (p < na && q < nb)
+            {
+                int32_t lo = std::max(a[p].s, b[q].s);
+                int32_t hi = std::min(a[p].e, b[q].e);
+                if (lo < hi)
+                {
+                    out.push(y, z, lo, hi);
+                }
+                if (a[p].e < b[q].e)
+                {
+                    ++p;
+                }
+                else
+                {
+                    ++q;
+                }
+            }
+        }
+
+        void difference_runs(const Run* a, size_t na, const Run* b, size_t nb, int32_t y, int32_t z, LevelSet& out)
+        {
+            size_t q = 0;
+            for (size_t p = 0; p < na; ++p)
+            {
+                int32_t s = a[p].s;
+                const int32_t e = a[p].e;
+                while (q < nb && b[q].e <= s)
+                {
+                    ++q;
+                }
+                for (size_t k = q; k < nb && b[k].s < e; ++k)
+                {
+                    if (b[k].s > s)
+                    {
+                        out.push(y, z, s, b[k].s);
+                    }
+                    s = std::max(s, b[k].e);
+                    if (s >= e)
+                    {
+                        break;
+                    }
+                }
+                if (s < e)
+                {
+                    out.push(y, z, s, e);
+                }
+            }
+        }
+
+        void copy_row(const LevelSet& a, size_t r, LevelSet& out)
+        {
+            for (int64_t k = a.rp[r]; k < a.rp[r + 1]; ++k)
+            {
+                out.push(a.ry[r], a.rz[r], a.runs[size_t(k)].s, a.runs[size_t(k)].e);
+            }
+        }
+
+        LevelSet combine(const LevelSet& A, const LevelSet& B, Op op)
+        {
+            const int L = std::max(A.level, B.level);
+            LevelSet ta, tb;
+            const LevelSet* a = &A;
+            const LevelSet* b = &B;
+            if (A.level < L)
+            {
+                ta = on(A, L);
+                a  = &ta;
+            }
+            if (B.level < L)
+            {
+                tb = on(B, L);
+                b  = &tb;
+            }
+            LevelSet out(A.dim, L);
+            size_t i = 0, j = 0;
+            const size_t na = a->nrows(), nb = b->nrows();
+            while (i < na || j < nb)
+            {
+                int which;
+                if (i < na && j < nb)
+                {
+                    if (row_less(a->ry[i], a->rz[i], b->ry[j], b->rz[j]))
+                    {
+                        which = 1;
+                    }
+                    else if (row_less(b->ry[j], b->rz[j], a->ry[i], a->rz[i]))
+                    {
+                        which = 2;
+                    }
+                    else
+                    {
+                        which = 3;
+                    }
+                }
+                else
+                {
+                    which = i < na ? 1 : 2;
+                }
+                if (which == 1)
+                {
+                    if (op != Op::Intersection)
+                    {
+                        copy_row(*a, i, out);
+                    }
+                    ++i;
+                }
+                else if (which == 2)
+                {
+                    if (op == Op::Union)
+                    {
+                        copy_row(*b, j, out);
+                    }
+                    ++j;
+                }
+                else
+                {
+                    const Run* ra  = a->runs.data() + a->rp[i];
+                    const size_t la = size_t(a->rp[i + 1] - a->rp[i]);
+                    const Run* rb  = b->runs.data() + b->rp[j];
+                    const size_t lb = size_t(b->rp[j + 1] - b->rp[j]);
+                    if (op == Op::Union)
+                    {
+                        union_runs(ra, la, rb, lb, a->ry[i], a->rz[i], out);
+                    }
+                    else if (op == Op::Intersection)
+                    {
+                        intersect_runs(ra, la, rb, lb, a->ry[i], a->rz[i], out);
+                    }
+                    else
+                    {
+                        difference_runs(ra, la, rb, lb, a->ry[i], a->rz[i], out);
+                    }
+                    ++i;
+                    ++j;
+                }
+            }
+            return out;
+        }
+
+        inline int32_t floor_shift(int32_t v, int k)
+        {
+            return v >> k; // arithmetic shift: floor division by 2^k
+        }
+    } // namespace
+
+    LevelSet set_union(const LevelSet& a, const LevelSet& b)
+    {
+        return combine(a, b, Op::Union);
+    }
+    LevelSet set_intersection(const LevelSet& a, const LevelSet& b)
+    {
+        return combine(a, b, Op::Intersection);
+    }
+    LevelSet set_difference(const LevelSet& a, const LevelSet& b)
+    {
+        return combine(a, b, Op::Difference);
+    }
+
+    LevelSet from_runs(int dim, int level, std::vector<std::array<int32_t, 4>>& q)
+    {
+        std::sort(q.begin(),
+                  q.end(),
+                  [](const std::array<int32_t, 4>& u, const std::array<int32_t, 4>& v)
+                  {
+                      if (u[1] != v[1])
+                      {
+                          return u[1] < v[1];
+                      }
+                      if (u[0] != v[0])
+                      {
+                          return u[0] < v[0];
+                      }
+                      return u[2] < v[2];
+                  });
+        LevelSet out(dim, level);
+        for (const auto& r : q)
+        {
+            out.push(r[0], r[1], r[2], r[3]);
+        }
+        return out;
+    }
+
+    LevelSet on(const LevelSet& a, int level)
+    {
+        if (level == a.level)
+        {
+            return a;
+        }
+        LevelSet out(a.dim, level);
+        if (level > a.level)
+        {
+            const int k     = level - a.level;
+            const int32_t f = int32_t(1) << k;
+            const int32_t nz = a.dim == 3 ? f : 1;
+            size_t r0        = 0;
+            while (r0 < a.nrows())
+            {
+                size_t r1 = r0;
+                while (r1 < a.nrows() && a.rz[r1] == a.rz[r0])
+                {
+                    ++r1;
+                }
+                for (int32_t dz = 0; dz < nz; ++dz)
+                {
+                    int32_t z = a.dim == 3 ? a.rz[r0] * f + dz : 0;
+                    for (size_t r = r0; r < r1; ++r)
+                    {
+                        for (int32_t dy = 0; dy < f; ++dy)
+                        {
+                            for (int64_t k2 = a.rp[r]; k2 < a.rp[r + 1]; ++k2)
+                            {
+                                out.push(a.ry[r] * f + dy, z, a.runs[size_t(k2)].s * f, a.runs[size_t(k2)].e * f);
+                            }
+                        }
+                    }
+                }
+                r0 = r1;
+            }
+            return out;
+        }
+        const int k = a.level - level;
+        std::vector<std::array<int32_t, 4>> q;
+        q.reserve(a.runs.size());
+        a.for_each_run(
+            [&](int32_t s, int32_t e, int32_t y, int32_t z)
+            {
+                q.push_back({floor_shift(y, k), a.dim == 3 ? floor_shift(z, k) : 0, floor_shift(s, k), floor_shift(e - 1, k) + 1});
+            });
+        return from_runs(a.dim, level, q);
+    }
+
+    LevelSet translate(const LevelSet& a, const std::array<int, 3>& t)
+    {
+        LevelSet out = a;
+        for (auto& y : out.ry)
+        {
+            y += t[1];
+        }
+        if (a.dim == 3)
+        {
+            for (auto& z : out.rz)
+            {
+                z += t[2];
+            }
+        }
+        for (auto& r : out.runs)
+        {
+            r.s += t[0];
+            r.e += t[0];
+        }
+        return out;
+    }
+
+    LevelSet expand(const LevelSet& a, int w)
+    {
+        if (w == 0 || a.empty())
+        {
+            return a;
+        }
+        std::vector<std::array<int32_t, 4>> q;
+        const int wz = a.dim == 3 ? w : 0;
+        q.reserve(a.runs.size() * size_t(2 * w + 1) * size_t(2 * wz + 1));
+        a.for_each_run(
+            [&](int32_t s, int32_t e, int32_t y, int32_t z)
+            {
+                for (int dz = -wz; dz <= wz; ++dz)
+                {
+                    for (int dy = -w; dy <= w; ++dy)
+                    {
+                        q.push_back({y + dy, z + dz, s - w, e + w});
+                    }
+                }
+            });
+        return from_runs(a.dim, a.level, q);
+    }
+
+    LevelSet box(int dim, int level, const std::array<int, 3>& lo, const std::array<int, 3>& hi)
+    {
+        LevelSet out(dim, level);
+        const int z0 = dim == 3 ? lo[2] : 0, z1 = dim == 3 ? hi[2] : 1;
+        for (int z = z0; z < z1; ++z)
+        {
+            for (int y = lo[1]; y < hi[1]; ++y)
+            {
+                out.push(y, z, lo[0], hi[0]);
+            }
+        }
+        return out;
+    }
+} // namespace smr

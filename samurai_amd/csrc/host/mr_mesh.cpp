@@ -1,0 +1,429 @@
+#include "mr_mesh.hpp"
+
+#include <algorithm>
+#include <cstdio>
+#include <stdexcept>
+
+namespace smr
+{
+    CellArray empty_cell_array(int dim)
+    {
+        CellArray ca;
+        ca.reserve(MAXL);
+        for (int l = 0; l < MAXL; ++l)
+        {
+            ca.emplace_back(dim, l);
+        }
+        return ca;
+    }
+
+    int ca_min_level(const CellArray& ca)
+    {
+        for (int l = 0; l < MAXL; ++l)
+        {
+            if (!ca[size_t(l)].empty())
+            {
+                return l;
+            }
+        }
+        return MAXL;
+    }
+
+    int ca_max_level(const CellArray& ca)
+    {
+        for (int l = MAXL - 1; l >= 0; --l)
+        {
+            if (!ca[size_t(l)].empty())
+            {
+                return l;
+            }
+        }
+        return 0;
+    }
+
+    bool ca_equal(const CellArray& a, const CellArray& b)
+    {
+        for (int l = 0; l < MAXL; ++l)
+        {
+            if (a[size_t(l)] != b[size_t(l)])
+            {
+                return false;
+            }
+        }
+        return true;
+    }
+
+    // for_each_cartesian_direction (stencil.hpp:300-316)
+    std::vector<Dir> cartesian_directions(int dim)
+    {
+        std::vector<Dir> out;
+        for (int d = 0; d < dim; ++d)
+        {
+            Dir v{0, 0, 0};
+            v[size_t(d)] = 1;
+            out.push_back(v);
+            v[size_t(d)] = -1;
+            out.push_back(v);
+        }
+        return out;
+    }
+
+    // for_each_diagonal_direction (stencil.hpp:337-349): static_nested_loop<dim,-1,2>
+    // order (last dimension outermost) keeping directions with >1 non-zero entry
+    std::vector<Dir> diagonal_directions(int dim)
+    {
+        std::vector<Dir> out;
+        const int zr = dim == 3 ? 1 : 0;
+        for (int z = -zr; z <= zr; ++z)
+        {
+            for (int y = -1; y <= 1; ++y)
+            {
+                for (int x = -1; x <= 1; ++x)
+                {
+                    if ((x != 0) + (y != 0) + (z != 0) > 1)
+                    {
+                        out.push_back({x, y, z});
+                    }
+                }
+            }
+        }
+        return out;
+    }
+
+    bool MRMeshData::run_base(int level, int x, int y, int z, int64_t& base) const
+    {
+        if (level < 0 || level >= MAXL)
+        {
+            return false;
+        }
+        const LevelSet& r = ids[REFERENCE][size_t(level)];
+        int64_t row       = r.find_row(y, z);
+        if (row < 0)
+        {
+            return false;
+        }
+        int64_t k = r.find_run(row, x);
+        if (k < 0)
+        {
+            return false;
+        }
+        base = ridx[size_t(level)][size_t(k)];
+        return true;
+    }
+
+    int64_t MRMeshData::index(int level, int x, int y, int z) const
+    {
+        int64_t base = 0;
+        if (!run_base(level, x, y, z, base))
+        {
+            char buf[160];
+            std::snprintf(buf, sizeof(buf), "cell (level %d; %d, %d, %d) is not in the reference mesh", level, x, y, z);
+            throw std::out_of_range(buf);
+        }
+        return base + x;
+    }
+
+    bool MRMeshData::exists(int level, int x, int y, int z) const
+    {
+        int64_t base = 0;
+        return run_base(level, x, y, z, base);
+    }
+
+    int64_t MRMeshData::nb_cells(int id) const
+    {
+        if (id == REFERENCE)
+        {
+            return nref;
+        }
+        int64_t n = 0;
+        for (int l = 0; l < MAXL; ++l)
+        {
+            n += ids[id][size_t(l)].count();
+        }
+        return n;
+    }
+
+    namespace
+    {
+        LevelSet domain_at(int dim, int level)
+        {
+            const int n = 1 << level;
+            return box(dim, level, {0, 0, 0}, {n, n, dim == 3 ? n : 1});
+        }
+
+        // Mesh_base::construct_union (mesh.hpp:1231-1262)
+        void construct_union(MRMeshData& m)
+        {
+            CellArray& uni = m.ids[UNION_CELLS];
+            uni            = empty_cell_array(m.cfg.dim);
+            for (int level = m.cfg.max_level; level >= 1; --level)
+            {
+                uni[size_t(level - 1)] = on(set_union(m.ids[CELLS][size_t(level)], uni[size_t(level)]), level - 1);
+            }
+        }
+
+        // MRMesh::update_sub_mesh_impl (mr/mesh.hpp:221-455), serial and non-periodic
+        void update_sub_mesh(MRMeshData& m)
+        {
+            const int r = m.cfg.radius, s = m.cfg.pred_s, dim = m.cfg.dim;
+            CellArray cell_list = empty_cell_array(dim);
+            for (int l = 0; l < MAXL; ++l)
+            {
+                if (!m.ids[CELLS][size_t(l)].empty())
+                {
+                    cell_list[size_t(l)] = expand(m.ids[CELLS][size_t(l)], r);
+                }
+            }
+            m.ids[CELLS_AND_GHOSTS] = cell_list;
+            m.ids[PROJ_CELLS]       = empty_cell_array(dim);
+            if (m.cfg.max_level != m.cfg.min_level)
+            {
+                for (int level = 1; level < MAXL; ++level)
+                {
+                    const LevelSet& cl = m.ids[CELLS][size_t(level)];
+                    if (cl.empty())
+                    {
+                        continue;
+                    }
+                    // ghosts for the prediction one and two levels below
+                    LevelSet cg = set_intersection(expand(on(m.ids[CELLS_AND_GHOSTS][size_t(level)], level - 1), s),
+                                                   expand(m.domain[size_t(level - 1)], s));
+                    cell_list[size_t(level - 1)] = set_union(cell_list[size_t(level - 1)], cg);
+                    if (level - 1 > 0)
+                    {
+                        cell_list[size_t(level - 2)] = set_union(cell_list[size_t(level - 2)], expand(on(cl, level - 2), s));
+                    }
+                }
+                m.ids[REFERENCE] = cell_list;
+                // projection ghosts, level by level on the growing reference
+                for (int level = 0; level < MAXL - 1; ++level)
+                {
+                    if (level > ca_max_level(m.ids[REFERENCE]))
+                    {
+                        break;
+                    }
+                    if (m.ids[REFERENCE][size_t(level)].empty())
+                    {
+                        continue;
+                    }
+                    LevelSet expr = set_intersection(m.ids[REFERENCE][size_t(level)], m.ids[UNION_CELLS][size_t(level)]);
+                    cell_list[size_t(level + 1)] = set_union(cell_list[size_t(level + 1)], on(expr, level + 1));
+                    m.ids[REFERENCE][size_t(level + 1)] = cell_list[size_t(level + 1)];
+                    m.ids[PROJ_CELLS][size_t(level)]    = expr;
+                }
+            }
+            m.ids[REFERENCE] = cell_list;
+        }
+
+        // Mesh_base::construct_corners (mesh.hpp:913-997)
+        void construct_corners(MRMeshData& m)
+        {
+            m.corners.clear();
+            m.corner_dirs.clear();
+            const LevelSet& dom = m.domain[size_t(m.cfg.max_level)];
+            for (const Dir& d : diagonal_directions(m.cfg.dim))
+            {
+                LevelSet u(m.cfg.dim, m.cfg.max_level);
+                for (int i = 0; i < m.cfg.dim; ++i)
+                {
+                    if (d[size_t(i)] != 0)
+                    {
+                        Dir t{0, 0, 0};
+                        t[size_t(i)] = -d[size_t(i)];
+                        u            = set_union(u, translate(dom, t));
+                    }
+                }
+                m.corners.push_back(set_difference(dom, u));
+                m.corner_dirs.push_back(d);
+            }
+        }
+
+        // Mesh_base::renumbering / LevelCellArray::update_index
+        // (mesh.hpp:893-911, level_cell_array.hpp:774-784)
+        void renumbering(MRMeshData& m)
+        {
+            m.ridx.assign(MAXL, {});
+            m.level_base.assign(MAXL + 1, 0);
+            int64_t n = 0;
+            for (int l = 0; l < MAXL; ++l)
+            {
+                m.level_base[size_t(l)] = n;
+                const LevelSet& r       = m.ids[REFERENCE][size_t(l)];
+                auto& idx               = m.ridx[size_t(l)];
+                idx.resize(r.runs.size());
+                for (size_t k = 0; k < r.runs.size(); ++k)
+                {
+                    idx[k] = n - r.runs[k].s;
+                    n += r.runs[k].e - r.runs[k].s;
+                }
+            }
+            m.level_base[MAXL] = n;
+            m.nref             = n;
+        }
+    } // namespace
+
+    MRMeshData build_mesh(const MeshConfig& cfg, const CellArray& cells)
+    {
+        if (cfg.radius != 1)
+        {
+            throw std::invalid_argument("only a scheme stencil radius of 1 is supported");
+        }
+        if (cfg.pred_s < 0 || cfg.pred_s > 1)
+        {
+            throw std::invalid_argument("prediction stencil radius must be 0 or 1");
+        }
+        if (cfg.max_level < cfg.min_level)
+        {
+            throw std::invalid_argument("Max level must be greater than min level.");
+        }
+        if (cfg.min_level < 1 || cfg.max_level >= MAXL - 1)
+        {
+            throw std::invalid_argument("levels must satisfy 1 <= min_level <= max_level <= 19");
+        }
+        MRMeshData m;
+        m.cfg = cfg;
+        for (int l = 0; l <= cfg.max_level; ++l)
+        {
+            m.domain.push_back(domain_at(cfg.dim, l));
+        }
+        for (auto& v : m.ids)
+        {
+            v = empty_cell_array(cfg.dim);
+        }
+        for (int l = 0; l < MAXL; ++l)
+        {
+            m.ids[CELLS][size_t(l)] = cells[size_t(l)];
+        }
+        construct_union(m);
+        update_sub_mesh(m);
+        construct_corners(m);
+        renumbering(m);
+        return m;
+    }
+
+    MRMeshData build_box_mesh(const MeshConfig& cfg)
+    {
+        CellArray cells                = empty_cell_array(cfg.dim);
+        cells[size_t(cfg.max_level)]   = domain_at(cfg.dim, cfg.max_level);
+        return build_mesh(cfg, cells);
+    }
+
+    CellArray update_cell_array_from_tag(const MRMeshData& mesh, const uint8_t* tags)
+    {
+        const int dim = mesh.cfg.dim;
+        std::vector<std::vector<std::array<int32_t, 4>>> add_m(MAXL), rm_m(MAXL), add_p(MAXL), rm_p(MAXL);
+        const CellArray& cells = mesh.ids[CELLS];
+        const int start = ca_min_level(cells), end = ca_max_level(cells) + 1;
+        for (int level = start; level < end; ++level)
+        {
+            const LevelSet& cl = cells[size_t(level)];
+            for (size_t r = 0; r < cl.nrows(); ++r)
+            {
+                const int32_t y = cl.ry[r], z = cl.rz[r];
+                const bool yz_even = (y % 2 == 0) && (dim == 2 || z % 2 == 0);
+                for (int64_t k = cl.rp[r]; k < cl.rp[r + 1]; ++k)
+                {
+                    const Run run = cl.runs[size_t(k)];
+                    int64_t base  = mesh.index(level, run.s, y, z) - run.s;
+                    for (int32_t x = run.s; x < run.e; ++x)
+                    {
+                        const uint8_t t   = tags[base + x];
+                        const bool refine = t & FLAG_REFINE;
+                        const bool cnk    = (t & FLAG_COARSEN) && !(t & FLAG_KEEP);
+                        if (refine && level < mesh.cfg.max_level)
+                        {
+                            rm_p[size_t(level)].push_back({y, z, x, x + 1});
+                            for (int dz = 0; dz < (dim == 3 ? 2 : 1); ++dz)
+                            {
+                                for (int dy = 0; dy < 2; ++dy)
+                                {
+                                    add_p[size_t(level + 1)].push_back({2 * y + dy, dim == 3 ? 2 * z + dz : 0, 2 * x, 2 * x + 2});
+                                }
+                            }
+                        }
+                        else if (cnk && level > mesh.cfg.min_level)
+                        {
+                            if (x % 2 == 0 && yz_even)
+                            {
+                                add_m[size_t(level - 1)].push_back({y >> 1, dim == 3 ? (z >> 1) : 0, x >> 1, (x >> 1) + 1});
+                            }
+                            rm_m[size_t(level)].push_back({y, z, x, x + 1});
+                        }
+                    }
+                }
+            }
+        }
+        CellArray out = empty_cell_array(dim);
+        for (int level = mesh.cfg.min_level; level <= mesh.cfg.max_level; ++level)
+        {
+            LevelSet am = from_runs(dim, level, add_m[size_t(level)]);
+            LevelSet ap = from_runs(dim, level, add_p[size_t(level)]);
+            LevelSet rm = from_runs(dim, level, rm_m[size_t(level)]);
+            LevelSet rp = from_runs(dim, level, rm_p[size_t(level)]);
+            out[size_t(level)] = set_difference(set_union(cells[size_t(level)], am, ap), set_union(rm, rp));
+        }
+        return out;
+    }
+
+    int make_graduation(CellArray& ca, int dim, int grad_width)
+    {
+        const int max_level = ca_max_level(ca);
+        const int min_level = ca_min_level(ca);
+        int nit             = 0;
+        bool changed        = true;
+        while (changed)
+        {
+            ++nit;
+            CellArray remove = empty_cell_array(dim);
+            // list_interval_to_refine_for_graduation (graduation.hpp:245-330)
+            const int fmax = ca_max_level(ca) + 1;
+            const int cmin = ca_min_level(ca);
+            for (int fine_level = fmax; fine_level > cmin + 1; --fine_level)
+            {
+                if (fine_level >= MAXL || ca[size_t(fine_level)].empty())
+                {
+                    continue;
+                }
+                LevelSet proj = on(expand(ca[size_t(fine_level)], 2 * grad_width), fine_level - 2);
+                for (int coarse_level = fine_level - 2;; --coarse_level)
+                {
+                    if (!proj.empty())
+                    {
+                        remove[size_t(coarse_level)] = set_union(remove[size_t(coarse_level)],
+                                                                 set_intersection(proj, ca[size_t(coarse_level)]));
+                    }
+                    if (coarse_level == cmin || proj.empty())
+                    {
+                        break;
+                    }
+                    proj = on(proj, coarse_level - 1);
+                }
+            }
+            bool any_change   = false;
+            CellArray add_p   = empty_cell_array(dim);
+            for (int level = min_level; level < max_level + 1; ++level)
+            {
+                if (!remove[size_t(level)].empty())
+                {
+                    any_change               = true;
+                    add_p[size_t(level + 1)] = on(remove[size_t(level)], level + 1);
+                }
+            }
+            if (!any_change)
+            {
+                changed = false;
+                continue;
+            }
+            CellArray new_ca = empty_cell_array(dim);
+            const int lo = std::min(ca_min_level(ca), ca_min_level(add_p));
+            const int hi = std::max(ca_max_level(ca), ca_max_level(add_p));
+            for (int level = lo; level < hi + 1; ++level)
+            {
+                new_ca[size_t(level)] = set_difference(set_union(ca[size_t(level)], add_p[size_t(level)]), remove[size_t(level)]);
+            }
+            changed = !ca_equal(new_ca, ca);
+            std::swap(new_ca, ca);
+        }
+        return nit - 1;
+    }
+} // namespace smr

@@ -1,0 +1,490 @@
+#include "programs.hpp"
+
+#include <algorithm>
+#include <numeric>
+#include <stdexcept>
+#include <unordered_map>
+
+namespace smr
+{
+    // ------------------------------------------------------------------
+    // WorkList
+    // ------------------------------------------------------------------
+    void WorkList::begin_level(int level)
+    {
+        if (level_run.empty())
+        {
+            level_run.assign(MAXL + 1, -1);
+        }
+        if (level_run[size_t(level)] < 0)
+        {
+            level_run[size_t(level)] = nruns();
+        }
+    }
+
+    void WorkList::add(const LevelSet& s)
+    {
+        begin_level(s.level);
+        s.for_each_run(
+            [&](int32_t x0, int32_t x1, int32_t y, int32_t z)
+            {
+                iv.push_back(x0);
+                iv.push_back(x1);
+                iv.push_back(y);
+                iv.push_back(z);
+                iv.push_back(s.level);
+                pref.push_back(pref.back() + (x1 - x0));
+            });
+    }
+
+    void WorkList::finish()
+    {
+        if (level_run.empty())
+        {
+            level_run.assign(MAXL + 1, -1);
+        }
+        // levels never started begin where the next started level begins
+        level_run[MAXL] = nruns();
+        for (int l = MAXL - 1; l >= 0; --l)
+        {
+            if (level_run[size_t(l)] < 0)
+            {
+                level_run[size_t(l)] = level_run[size_t(l + 1)];
+            }
+        }
+    }
+
+    int64_t WorkList::runs_of_levels(int l0, int l1, int64_t& r0) const
+    {
+        l0 = std::max(l0, 0);
+        l1 = std::min(l1, MAXL);
+        if (l1 <= l0)
+        {
+            r0 = 0;
+            return 0;
+        }
+        r0 = level_run[size_t(l0)];
+        return level_run[size_t(l1)] - r0;
+    }
+
+    // ------------------------------------------------------------------
+    // Recipes
+    // ------------------------------------------------------------------
+    void Recipes::push(int64_t d, int32_t o, const int64_t* s, int n)
+    {
+        dst.push_back(d);
+        op.push_back(o);
+        for (int k = 0; k < n; ++k)
+        {
+            src.push_back(s[k]);
+        }
+        src_ptr.push_back(int64_t(src.size()));
+    }
+
+    void Recipes::stage()
+    {
+        const size_t n = dst.size();
+        std::vector<int> st(n, 0);
+        std::unordered_map<int64_t, int> last_write, last_read;
+        int nstages = 0;
+        for (size_t r = 0; r < n; ++r)
+        {
+            int s = 0;
+            for (int64_t k = src_ptr[r]; k < src_ptr[r + 1]; ++k)
+            {
+                auto it = last_write.find(src[size_t(k)]);
+                if (it != last_write.end())
+                {
+                    s = std::max(s, it->second + 1);
+                }
+            }
+            auto w = last_write.find(dst[r]);
+            if (w != last_write.end())
+            {
+                s = std::max(s, w->second + 1);
+            }
+            auto rd = last_read.find(dst[r]);
+            if (rd != last_read.end())
+            {
+                s = std::max(s, rd->second + 1);
+            }
+            st[r]               = s;
+            last_write[dst[r]]  = s;
+            for (int64_t k = src_ptr[r]; k < src_ptr[r + 1]; ++k)
+            {
+                int& lr = last_read[src[size_t(k)]];
+                lr      = std::max(lr, s);
+            }
+            nstages = std::max(nstages, s + 1);
+        }
+        std::vector<size_t> order(n);
+        std::iota(order.begin(), order.end(), size_t(0));
+        std::stable_sort(order.begin(),
+                         order.end(),
+                         [&](size_t a, size_t b)
+                         {
+                             return st[a] < st[b];
+                         });
+        Recipes out;
+        out.stage_ptr.assign(size_t(nstages) + 1, 0);
+        for (size_t r : order)
+        {
+            out.push(dst[r], op[r], src.data() + src_ptr[r], int(src_ptr[r + 1] - src_ptr[r]));
+            out.stage_ptr[size_t(st[r]) + 1]++;
+        }
+        for (size_t k = 1; k < out.stage_ptr.size(); ++k)
+        {
+            out.stage_ptr[k] += out.stage_ptr[k - 1];
+        }
+        *this = std::move(out);
+    }
+
+    namespace
+    {
+        void add_row_taps(WorkList& w, int8_t table, int8_t dl, int8_t dx, int lo, int hi, int dim, int8_t wlo, int8_t whi)
+        {
+            const int zlo = dim == 3 ? lo : 0, zhi = dim == 3 ? hi : 0;
+            for (int dz = zlo; dz <= zhi; ++dz)
+            {
+                for (int dy = lo; dy <= hi; ++dy)
+                {
+                    w.taps.push_back({table, dl, dx, int8_t(dy), int8_t(dz), wlo, whi});
+                }
+            }
+        }
+
+        LevelSet domain_layer(const MRMeshData& m, int level, const Dir& dir)
+        {
+            // cells of domain(level) whose neighbour in `dir` is outside the box
+            const LevelSet& dom = m.domain[size_t(level)];
+            return set_difference(dom, translate(dom, {-dir[0], -dir[1], -dir[2]}));
+        }
+
+        // update_outer_ghosts(level) (algorithm/update_outer_ghost.hpp:338-432)
+        Recipes outer_recipes(const MRMeshData& m, int level, const std::vector<Dir>& cart)
+        {
+            Recipes rec;
+            const int dim = m.cfg.dim;
+            const auto& cells = m.ids[CELLS];
+            const auto& ref   = m.ids[REFERENCE];
+            if (level <= m.cfg.max_level && level >= m.cfg.min_level)
+            {
+                for (size_t c = 0; c < m.corners.size(); ++c)
+                {
+                    const Dir& d = m.corner_dirs[c];
+                    // corner extrapolation, ghost width 1 (bc/apply_field_bc.hpp:315-466):
+                    // the diagonal ghost copies the corner cell
+                    LevelSet corner = on(m.corners[c], level);
+                    LevelSet cc     = set_intersection(cells[size_t(level)], corner);
+                    cc.for_each_run(
+                        [&](int32_t x0, int32_t x1, int32_t y, int32_t z)
+                        {
+                            for (int32_t x = x0; x < x1; ++x)
+                            {
+                                int64_t s = m.index(level, x, y, z);
+                                rec.push(m.index(level, x + d[0], y + d[1], z + d[2]), RECIPE_COPY, &s, 1);
+                            }
+                        });
+                    // project_corner_below (algorithm/update_outer_ghost.hpp:267-336)
+                    if (level > 0)
+                    {
+                        LevelSet fine_outer = set_intersection(translate(corner, d), ref[size_t(level)]);
+                        for (int dl = 1; dl <= 2; ++dl)
+                        {
+                            const int pl   = level - dl;
+                            LevelSet ghost = set_intersection(on(fine_outer, pl), ref[size_t(pl)]);
+                            const int last = (1 << dl) - 1;
+                            ghost.for_each_run(
+                                [&](int32_t x0, int32_t x1, int32_t y, int32_t z)
+                                {
+                                    for (int32_t x = x0; x < x1; ++x)
+                                    {
+                                        int cx = x * (1 << dl) + (d[0] == -1 ? last : 0);
+                                        int cy = y * (1 << dl) + (d[1] == -1 ? last : 0);
+                                        int cz = dim == 3 ? z * (1 << dl) + (d[2] == -1 ? last : 0) : 0;
+                                        int64_t s;
+                                        if (m.run_base(level, cx, cy, cz, s))
+                                        {
+                                            s += cx;
+                                            rec.push(m.index(pl, x, y, z), RECIPE_COPY, &s, 1);
+                                        }
+                                    }
+                                });
+                            if (pl == 0)
+                            {
+                                break;
+                            }
+                        }
+                    }
+                }
+            }
+            for (const Dir& dir : cart)
+            {
+                if (level < m.cfg.max_level)
+                {
+                    // project_bc, layer 1 (algorithm/update_outer_ghost.hpp:20-152)
+                    LevelSet outside = set_difference(translate(m.ids[UNION_CELLS][size_t(level)], dir), m.domain[size_t(level)]);
+                    LevelSet ghosts  = set_intersection(outside, ref[size_t(level)]);
+                    std::vector<int64_t> ch;
+                    ghosts.for_each_run(
+                        [&](int32_t x0, int32_t x1, int32_t y, int32_t z)
+                        {
+                            for (int32_t x = x0; x < x1; ++x)
+                            {
+                                ch.clear();
+                                for (int cl = level + 1; cl <= level + 2 && cl < MAXL; ++cl)
+                                {
+                                    const int k = cl - level, f = 1 << k;
+                                    const int nz = dim == 3 ? f : 1;
+                                    for (int dz = 0; dz < nz; ++dz)
+                                    {
+                                        for (int dy = 0; dy < f; ++dy)
+                                        {
+                                            for (int dx = 0; dx < f; ++dx)
+                                            {
+                                                int64_t b;
+                                                int cx = x * f + dx, cy = y * f + dy, cz = dim == 3 ? z * f + dz : 0;
+                                                if (m.run_base(cl, cx, cy, cz, b))
+                                                {
+                                                    ch.push_back(b + cx);
+                                                }
+                                            }
+                                        }
+                                    }
+                                    if (!ch.empty())
+                                    {
+                                        break;
+                                    }
+                                }
+                                rec.push(m.index(level, x, y, z), RECIPE_AVERAGE, ch.data(), int(ch.size()));
+                            }
+                        });
+                }
+                if (level >= m.cfg.min_level)
+                {
+                    // Dirichlet<1> on the boundary cells (bc/apply_field_bc.hpp:53-101)
+                    LevelSet bdry = set_intersection(cells[size_t(level)], domain_layer(m, level, dir));
+                    bdry.for_each_run(
+                        [&](int32_t x0, int32_t x1, int32_t y, int32_t z)
+                        {
+                            for (int32_t x = x0; x < x1; ++x)
+                            {
+                                int64_t s = m.index(level, x, y, z);
+                                rec.push(m.index(level, x + dir[0], y + dir[1], z + dir[2]), RECIPE_DIRICHLET, &s, 1);
+                            }
+                        });
+                }
+                if (level < m.cfg.max_level && level >= m.cfg.min_level)
+                {
+                    // predict_bc, order 0 (algorithm/update_outer_ghost.hpp:154-262)
+                    const LevelSet& dom = m.domain[size_t(level)];
+                    LevelSet inner      = set_difference(cells[size_t(level)], translate(dom, {-dir[0], -dir[1], -dir[2]}));
+                    LevelSet bc_ghosts  = set_difference(translate(inner, dir), dom);
+                    LevelSet ghosts     = set_intersection(bc_ghosts, ref[size_t(level + 1)]);
+                    ghosts.for_each_run(
+                        [&](int32_t x0, int32_t x1, int32_t y, int32_t z)
+                        {
+                            for (int32_t x = x0; x < x1; ++x)
+                            {
+                                int64_t s = m.index(level, x >> 1, y >> 1, dim == 3 ? (z >> 1) : 0);
+                                rec.push(m.index(level + 1, x, y, z), RECIPE_COPY, &s, 1);
+                            }
+                        });
+                }
+            }
+            rec.stage();
+            return rec;
+        }
+    } // namespace
+
+    GhostProgram build_ghost_program(const MRMeshData& m)
+    {
+        GhostProgram p;
+        const int dim = m.cfg.dim, s = m.cfg.pred_s;
+        const auto cart = cartesian_directions(dim);
+        p.outer.resize(size_t(m.cfg.max_level) + 1);
+        for (int level = m.cfg.max_level; level >= 0; --level)
+        {
+            p.outer[size_t(level)] = outer_recipes(m, level, cart);
+        }
+        // projection (numeric/projection.hpp:15-65): runs at the coarse level
+        p.projection.taps.push_back({0, 0, 0, 0, 0, 0, 0});
+        add_row_taps(p.projection, 0, 1, 0, 0, 1, dim, 0, 1);
+        for (int level = 1; level <= m.cfg.max_level; ++level)
+        {
+            p.projection.add(set_intersection(on(m.ids[REFERENCE][size_t(level)], level - 1), m.ids[PROJ_CELLS][size_t(level - 1)]));
+        }
+        p.projection.finish();
+        // prediction <s,false> (numeric/prediction.hpp:259-357)
+        p.prediction.taps.push_back({0, 0, 0, 0, 0, 0, 0});
+        add_row_taps(p.prediction, 0, -1, int8_t(-s), -s, s, dim, 0, int8_t(2 * s));
+        for (int level = 1; level <= m.cfg.max_level; ++level)
+        {
+            const LevelSet& ref = m.ids[REFERENCE][size_t(level)];
+            LevelSet pred_ghosts = set_difference(ref, set_union(m.ids[CELLS][size_t(level)], m.ids[PROJ_CELLS][size_t(level)]));
+            LevelSet expr        = set_intersection(set_intersection(pred_ghosts, m.domain[size_t(level)]),
+                                             m.ids[REFERENCE][size_t(level - 1)]);
+            p.prediction.add(expr);
+        }
+        p.prediction.finish();
+        return p;
+    }
+
+    AdaptProgram build_adapt_program(const MRMeshData& m)
+    {
+        AdaptProgram p;
+        const int dim = m.cfg.dim, s = m.cfg.pred_s;
+        const int minl = m.cfg.min_level, maxl = m.cfg.max_level;
+        // compute_detail (mr/operators.hpp:139-540): fine rows, then coarse stencil rows
+        add_row_taps(p.detail, 0, 1, 0, 0, 1, dim, 0, 1);
+        add_row_taps(p.detail, 0, 0, int8_t(-s), -s, s, dim, 0, int8_t(2 * s));
+        for (int level = (minl > 0 ? minl - 1 : 0); level < maxl; ++level)
+        {
+            LevelSet fine  = set_union(m.ids[CELLS][size_t(level + 1)], m.ids[CELLS][size_t(level + 2)]);
+            LevelSet below = set_intersection(m.ids[REFERENCE][size_t(level)], on(fine, level));
+            p.detail.add(below);
+        }
+        p.detail.finish();
+        // mr_criteria / maximum (mr/criteria.hpp:13-122, mr/operators.hpp:22-96)
+        p.criteria.taps.push_back({0, 0, 0, 0, 0, 0, 0});
+        add_row_taps(p.criteria, 0, 1, 0, 0, 1, dim, 0, 1);
+        for (int level = minl; level <= maxl; ++level)
+        {
+            p.criteria.add(set_intersection(on(m.ids[CELLS][size_t(level)], level - 1), m.ids[REFERENCE][size_t(level - 1)]));
+        }
+        p.criteria.finish();
+        p.maximum.taps = p.criteria.taps;
+        for (int level = 1; level <= maxl; ++level)
+        {
+            p.maximum.add(set_intersection(on(m.ids[CELLS][size_t(level)], level - 1), m.ids[REFERENCE][size_t(level - 1)]));
+        }
+        p.maximum.finish();
+        return p;
+    }
+
+    WorkList build_cells_list(const MRMeshData& m)
+    {
+        WorkList w;
+        w.taps.push_back({0, 0, 0, 0, 0, -1, 1});
+        w.taps.push_back({0, 0, 0, -1, 0, 0, 0});
+        w.taps.push_back({0, 0, 0, 1, 0, 0, 0});
+        if (m.cfg.dim == 3)
+        {
+            w.taps.push_back({0, 0, 0, 0, -1, 0, 0});
+            w.taps.push_back({0, 0, 0, 0, 1, 0, 0});
+        }
+        for (int level = 0; level < MAXL; ++level)
+        {
+            if (!m.ids[CELLS][size_t(level)].empty())
+            {
+                w.add(m.ids[CELLS][size_t(level)]);
+            }
+        }
+        w.finish();
+        return w;
+    }
+
+    void build_update_fields_lists(const MRMeshData& om, const MRMeshData& nm, WorkList& copy, WorkList& coarsen, WorkList& refine)
+    {
+        const int dim = om.cfg.dim, s = om.cfg.pred_s;
+        const int minl = om.cfg.min_level, maxl = om.cfg.max_level;
+        copy    = WorkList();
+        coarsen = WorkList();
+        refine  = WorkList();
+        copy.taps.push_back({0, 0, 0, 0, 0, 0, 0});
+        copy.taps.push_back({1, 0, 0, 0, 0, 0, 0});
+        for (int level = minl; level <= maxl; ++level)
+        {
+            copy.add(set_intersection(om.ids[REFERENCE][size_t(level)], nm.ids[CELLS][size_t(level)]));
+        }
+        copy.finish();
+        coarsen.taps.push_back({0, 0, 0, 0, 0, 0, 0});
+        add_row_taps(coarsen, 1, 1, 0, 0, 1, dim, 0, 1);
+        add_row_taps(refine, 0, 1, 0, 0, 1, dim, 0, 1);
+        add_row_taps(refine, 1, 0, int8_t(-s), -s, s, dim, 0, int8_t(2 * s));
+        for (int level = minl + 1; level <= maxl; ++level)
+        {
+            coarsen.add(set_intersection(on(om.ids[CELLS][size_t(level)], level - 1), nm.ids[CELLS][size_t(level - 1)]));
+            refine.add(set_intersection(on(nm.ids[CELLS][size_t(level)], level - 1), om.ids[CELLS][size_t(level - 1)]));
+        }
+        coarsen.finish();
+        refine.finish();
+    }
+} // namespace smr
+
+namespace smr
+{
+    std::string validate_list(const WorkList& w, const MRMeshData& t0, const MRMeshData& t1)
+    {
+        for (int64_t r = 0; r < w.nruns(); ++r)
+        {
+            const int32_t* iv = w.iv.data() + 5 * r;
+            const int32_t x0 = iv[0], x1 = iv[1], y = iv[2], z = iv[3];
+            const int level  = iv[4];
+            for (const Tap& t : w.taps)
+            {
+                int tl;
+                int32_t tx, last, ty, tz;
+                if (t.dl == 0)
+                {
+                    tl = level, tx = x0 + t.dx, last = x1 - 1 + t.dx, ty = y + t.dy, tz = z + t.dz;
+                }
+                else if (t.dl > 0)
+                {
+                    tl = level + 1, tx = 2 * x0 + t.dx, last = 2 * (x1 - 1) + t.dx, ty = 2 * y + t.dy, tz = 2 * z + t.dz;
+                }
+                else
+                {
+                    tl = level - 1, tx = (x0 >> 1) + t.dx, last = ((x1 - 1) >> 1) + t.dx, ty = (y >> 1) + t.dy, tz = (z >> 1) + t.dz;
+                }
+                const MRMeshData& m = t.table == 0 ? t0 : t1;
+                bool ok             = tl >= 0 && tl < MAXL;
+                if (ok)
+                {
+                    const LevelSet& ref = m.ids[REFERENCE][size_t(tl)];
+                    const int64_t row   = ref.find_row(ty, tz);
+                    const int64_t k     = row >= 0 ? ref.find_run(row, tx) : -1;
+                    ok = k >= 0 && ref.runs[size_t(k)].s <= tx + t.wlo && last + t.whi < ref.runs[size_t(k)].e;
+                }
+                if (!ok)
+                {
+                    return "run (level " + std::to_string(level) + "; x " + std::to_string(x0) + ".." + std::to_string(x1) + ", y " +
+                           std::to_string(y) + ", z " + std::to_string(z) + ") has a tap outside the reference";
+                }
+            }
+        }
+        return "";
+    }
+
+    std::string validate_programs(const MRMeshData& m)
+    {
+        std::string why = validate_list(build_cells_list(m), m, m);
+        if (!why.empty())
+        {
+            return "cells: " + why;
+        }
+        GhostProgram g = build_ghost_program(m);
+        if (!(why = validate_list(g.projection, m, m)).empty())
+        {
+            return "projection: " + why;
+        }
+        if (!(why = validate_list(g.prediction, m, m)).empty())
+        {
+            return "prediction: " + why;
+        }
+        AdaptProgram a = build_adapt_program(m);
+        if (!(why = validate_list(a.detail, m, m)).empty())
+        {
+            return "detail: " + why;
+        }
+        if (!(why = validate_list(a.criteria, m, m)).empty())
+        {
+            return "criteria: " + why;
+        }
+        if (!(why = validate_list(a.maximum, m, m)).empty())
+        {
+            return "maximum: " + why;
+        }
+        return "";
+    }
+} // namespace smr

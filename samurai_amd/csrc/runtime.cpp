@@ -1,0 +1,512 @@
+#include "runtime.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace smr
+{
+    void hip_check(hipError_t e, const char* what)
+    {
+        if (e != hipSuccess)
+        {
+            throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+        }
+    }
+
+    // ------------------------------------------------------------------
+    // Blob
+    // ------------------------------------------------------------------
+    Blob::~Blob()
+    {
+        if (d_)
+        {
+            (void)hipFree(d_);
+        }
+    }
+
+    size_t Blob::reserve(size_t n)
+    {
+        size_t off = (host_.size() + 255) & ~size_t(255);
+        host_.resize(off + std::max<size_t>(n, 8), 0);
+        return off;
+    }
+
+    size_t Blob::add(const void* src, size_t n)
+    {
+        size_t off = reserve(n);
+        if (n)
+        {
+            std::memcpy(host_.data() + off, src, n);
+        }
+        return off;
+    }
+
+    void Blob::upload(hipStream_t s)
+    {
+        if (d_)
+        {
+            throw std::logic_error("blob uploaded twice");
+        }
+        hip_check(hipMalloc(reinterpret_cast<void**>(&d_), std::max<size_t>(host_.size(), 256)), "hipMalloc(blob)");
+        if (!host_.empty())
+        {
+            hip_check(hipMemcpyAsync(d_, host_.data(), host_.size(), hipMemcpyHostToDevice, s), "hipMemcpyAsync(blob)");
+            // the host staging buffer must outlive the copy
+            hip_check(hipStreamSynchronize(s), "hipStreamSynchronize(blob)");
+        }
+    }
+
+    // ------------------------------------------------------------------
+    // Ctx
+    // ------------------------------------------------------------------
+    Ctx::~Ctx()
+    {
+        if (d_err)
+        {
+            (void)hipFree(d_err);
+        }
+        if (stream)
+        {
+            (void)hipStreamDestroy(stream);
+        }
+    }
+
+    void Ctx::require_device() const
+    {
+        if (!has_device())
+        {
+            throw std::runtime_error("this context has no GPU: device kernels are unavailable");
+        }
+    }
+
+    void Ctx::sync()
+    {
+        hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    }
+
+    void Ctx::check_resolve()
+    {
+        int err = 0;
+        hip_check(hipMemcpyAsync(&err, d_err, sizeof(int), hipMemcpyDeviceToHost, stream), "hipMemcpyAsync(err)");
+        sync();
+        if (err)
+        {
+            hip_check(hipMemsetAsync(d_err, 0, sizeof(int), stream), "hipMemsetAsync(err)");
+            throw std::out_of_range("a work-list row is missing from the reference interval table");
+        }
+    }
+
+    // ------------------------------------------------------------------
+    // device images
+    // ------------------------------------------------------------------
+    namespace
+    {
+        void upload_table(Ctx& ctx, const MRMeshData& m, DevMesh& dm)
+        {
+            std::vector<int64_t> key, row_off(MAXL + 1, 0), rp, idx;
+            std::vector<int32_t> rs, re;
+            rp.push_back(0);
+            for (int l = 0; l < MAXL; ++l)
+            {
+                row_off[size_t(l)] = int64_t(key.size());
+                const LevelSet& r  = m.ids[REFERENCE][size_t(l)];
+                for (size_t row = 0; row < r.nrows(); ++row)
+                {
+                    key.push_back((int64_t(r.rz[row]) << 32) + (int64_t(r.ry[row]) + 2147483648LL));
+                    for (int64_t k = r.rp[row]; k < r.rp[row + 1]; ++k)
+                    {
+                        rs.push_back(r.runs[size_t(k)].s);
+                        re.push_back(r.runs[size_t(k)].e);
+                        idx.push_back(m.ridx[size_t(l)][size_t(k)]);
+                    }
+                    rp.push_back(int64_t(rs.size()));
+                }
+            }
+            row_off[MAXL] = int64_t(key.size());
+            auto b        = std::make_unique<Blob>();
+            size_t o_key = b->add(key.data(), key.size() * 8), o_off = b->add(row_off.data(), row_off.size() * 8);
+            size_t o_rp = b->add(rp.data(), rp.size() * 8), o_rs = b->add(rs.data(), rs.size() * 4);
+            size_t o_re = b->add(re.data(), re.size() * 4), o_idx = b->add(idx.data(), idx.size() * 8);
+            b->upload(ctx.stream);
+            dm.table.row_key = reinterpret_cast<const int64_t*>(b->dptr(o_key));
+            dm.table.row_off = reinterpret_cast<const int64_t*>(b->dptr(o_off));
+            dm.table.rp      = reinterpret_cast<const int64_t*>(b->dptr(o_rp));
+            dm.table.run_s   = reinterpret_cast<const int32_t*>(b->dptr(o_rs));
+            dm.table.run_e   = reinterpret_cast<const int32_t*>(b->dptr(o_re));
+            dm.table.run_idx = reinterpret_cast<const int64_t*>(b->dptr(o_idx));
+            dm.table_blob    = std::move(b);
+        }
+
+        void upload_recipes(Ctx& ctx, DevRecipes& r)
+        {
+            auto b     = std::make_unique<Blob>();
+            size_t od  = b->add(r.host.dst.data(), r.host.dst.size() * 8);
+            size_t oo  = b->add(r.host.op.data(), r.host.op.size() * 4);
+            size_t osp = b->add(r.host.src_ptr.data(), r.host.src_ptr.size() * 8);
+            size_t os  = b->add(r.host.src.data(), r.host.src.size() * 8);
+            b->upload(ctx.stream);
+            r.dst  = reinterpret_cast<const int64_t*>(b->dptr(od));
+            r.op   = reinterpret_cast<const int32_t*>(b->dptr(oo));
+            r.sp   = reinterpret_cast<const int64_t*>(b->dptr(osp));
+            r.src  = reinterpret_cast<const int64_t*>(b->dptr(os));
+            r.blob = std::move(b);
+        }
+    } // namespace
+
+    void DevWorkList::cells_of_levels(int l0, int l1, int64_t& c0, int64_t& c1) const
+    {
+        int64_t r0    = 0;
+        int64_t nr    = host.runs_of_levels(l0, l1, r0);
+        c0            = host.pref[size_t(r0)];
+        c1            = host.pref[size_t(r0 + nr)];
+    }
+
+    void upload_list(Ctx& ctx, DevWorkList& l, const dev::Table& t0, const dev::Table& t1)
+    {
+        const WorkList& w     = l.host;
+        const int64_t nruns   = w.nruns();
+        const int64_t ncells  = w.ncells();
+        const int64_t nchunks = (ncells + (1 << dev::HINT_SHIFT) - 1) >> dev::HINT_SHIFT;
+        l.hint.assign(size_t(nchunks) + 1, 0);
+        int64_t r = 0;
+        for (int64_t j = 0; j < nchunks; ++j)
+        {
+            const int64_t c = j << dev::HINT_SHIFT;
+            while (w.pref[size_t(r + 1)] <= c)
+            {
+                ++r;
+            }
+            l.hint[size_t(j)] = int32_t(r);
+        }
+        l.hint[size_t(nchunks)] = int32_t(std::max<int64_t>(nruns - 1, 0));
+        std::vector<int8_t> taps;
+        for (const Tap& t : w.taps)
+        {
+            taps.insert(taps.end(), {t.table, t.dl, t.dx, t.dy, t.dz, t.wlo, t.whi});
+        }
+        auto b      = std::make_unique<Blob>();
+        size_t oiv  = b->add(w.iv.data(), w.iv.size() * 4);
+        size_t opf  = b->add(w.pref.data(), w.pref.size() * 8);
+        size_t oh   = b->add(l.hint.data(), l.hint.size() * 4);
+        size_t ot   = b->add(taps.data(), taps.size());
+        size_t obas = b->reserve(size_t(nruns) * w.taps.size() * 8);
+        b->upload(ctx.stream);
+        l.d.iv     = reinterpret_cast<const int32_t*>(b->dptr(oiv));
+        l.d.pref   = reinterpret_cast<const int64_t*>(b->dptr(opf));
+        l.d.hint   = reinterpret_cast<const int32_t*>(b->dptr(oh));
+        l.d.taps   = reinterpret_cast<const int8_t*>(b->dptr(ot));
+        l.d.base   = reinterpret_cast<int64_t*>(b->dptr(obas));
+        l.d.nruns  = nruns;
+        l.d.ncells = ncells;
+        l.d.ntaps  = int(w.taps.size());
+        l.blob     = std::move(b);
+        dev::resolve(l.d, t0, t1, ctx.d_err, ctx.stream);
+    }
+
+    std::unique_ptr<DevMesh> make_dev_mesh(Ctx& ctx, const MRMeshData& m)
+    {
+        ctx.require_device();
+        auto dm = std::make_unique<DevMesh>();
+        upload_table(ctx, m, *dm);
+        dm->cells.host = build_cells_list(m);
+        upload_list(ctx, dm->cells, dm->table, dm->table);
+        ctx.check_resolve();
+        return dm;
+    }
+
+    DevMesh& Mesh::device()
+    {
+        if (!dev)
+        {
+            dev = make_dev_mesh(*ctx, host);
+        }
+        return *dev;
+    }
+
+    DevMesh& Mesh::device_ghost()
+    {
+        DevMesh& dm = device();
+        if (!dm.has_ghost)
+        {
+            GhostProgram gp = build_ghost_program(host);
+            dm.outer.resize(gp.outer.size());
+            for (size_t l = 0; l < gp.outer.size(); ++l)
+            {
+                dm.outer[l].host = std::move(gp.outer[l]);
+                upload_recipes(*ctx, dm.outer[l]);
+            }
+            dm.projection.host = std::move(gp.projection);
+            dm.prediction.host = std::move(gp.prediction);
+            upload_list(*ctx, dm.projection, dm.table, dm.table);
+            upload_list(*ctx, dm.prediction, dm.table, dm.table);
+            ctx->check_resolve();
+            dm.has_ghost = true;
+        }
+        return dm;
+    }
+
+    DevMesh& Mesh::device_adapt()
+    {
+        DevMesh& dm = device_ghost();
+        if (!dm.has_adapt)
+        {
+            AdaptProgram ap  = build_adapt_program(host);
+            dm.detail.host   = std::move(ap.detail);
+            dm.criteria.host = std::move(ap.criteria);
+            dm.maximum.host  = std::move(ap.maximum);
+            upload_list(*ctx, dm.detail, dm.table, dm.table);
+            upload_list(*ctx, dm.criteria, dm.table, dm.table);
+            upload_list(*ctx, dm.maximum, dm.table, dm.table);
+            ctx->check_resolve();
+            dm.has_adapt = true;
+        }
+        return dm;
+    }
+
+    void Mesh::replace(MRMeshData&& m, std::unique_ptr<DevMesh>&& d)
+    {
+        host = std::move(m);
+        dev  = std::move(d);
+        ++version;
+    }
+
+    // ------------------------------------------------------------------
+    // Field
+    // ------------------------------------------------------------------
+    Field::~Field()
+    {
+        if (d)
+        {
+            (void)hipFree(d);
+        }
+    }
+
+    void Field::allocate(bool zero)
+    {
+        mesh->ctx->require_device();
+        if (d)
+        {
+            hip_check(hipFree(d), "hipFree(field)");
+            d = nullptr;
+        }
+        size = mesh->host.nref;
+        hip_check(hipMalloc(reinterpret_cast<void**>(&d), size_t(std::max<int64_t>(size, 1)) * sizeof(double)), "hipMalloc(field)");
+        if (zero)
+        {
+            hip_check(hipMemsetAsync(d, 0, size_t(size) * sizeof(double), mesh->ctx->stream), "hipMemsetAsync(field)");
+        }
+        version = mesh->version;
+    }
+
+    void Field::check_current() const
+    {
+        if (version != mesh->version || !d)
+        {
+            throw std::runtime_error("field '" + name + "' was not resized after its mesh changed");
+        }
+    }
+
+    AdaptState::~AdaptState()
+    {
+        if (detail)
+        {
+            (void)hipFree(detail);
+        }
+        if (tag)
+        {
+            (void)hipFree(tag);
+        }
+    }
+
+    // ------------------------------------------------------------------
+    // update_ghost_mr (algorithm/update_ghost_mr.hpp:194-237)
+    // ------------------------------------------------------------------
+    void update_ghost_mr(Field& f)
+    {
+        f.check_current();
+        Mesh& m         = *f.mesh;
+        DevMesh& dm     = m.device_ghost();
+        hipStream_t s   = m.ctx->stream;
+        const int dim   = m.host.cfg.dim;
+        const int maxl  = m.host.cfg.max_level;
+        const double bc = f.bc_value;
+        for (int level = maxl; level >= 0; --level)
+        {
+            DevRecipes& r  = dm.outer[size_t(level)];
+            const auto& sp = r.host.stage_ptr;
+            for (size_t k = 0; k + 1 < sp.size(); ++k)
+            {
+                dev::recipes(r.dst, r.op, r.sp, r.src, sp[k], sp[k + 1], f.d, f.has_bc, bc, s);
+            }
+            if (level > 0)
+            {
+                int64_t c0, c1;
+                dm.projection.cells_of_levels(level - 1, level, c0, c1);
+                dev::project(dim, dm.projection.d, c0, c1, f.d, f.d, s);
+            }
+        }
+        for (int level = 1; level <= maxl; ++level)
+        {
+            int64_t c0, c1;
+            dm.prediction.cells_of_levels(level, level + 1, c0, c1);
+            dev::predict_fine(dim, m.host.cfg.pred_s, dm.prediction.d, c0, c1, f.d, s);
+        }
+    }
+
+    void upwind(const double* a, double dt, Field& u, Field& unp1)
+    {
+        u.check_current();
+        unp1.check_current();
+        if (u.mesh != unp1.mesh)
+        {
+            throw std::invalid_argument("upwind: fields on different meshes");
+        }
+        Mesh& m     = *u.mesh;
+        DevMesh& dm = m.device();
+        const int dim = m.host.cfg.dim;
+        dev::upwind(dim, dm.cells.d, 0, dm.cells.d.ncells, u.d, unp1.d, a[0], a[1], dim == 3 ? a[2] : 0.0, dt, m.ctx->stream);
+    }
+
+    // ------------------------------------------------------------------
+    // adaptation (mr/adapt.hpp:147-389)
+    // ------------------------------------------------------------------
+    namespace
+    {
+        void ensure_adapt_buffers(AdaptState& st, int64_t n)
+        {
+            if (n > st.capacity)
+            {
+                if (st.detail)
+                {
+                    hip_check(hipFree(st.detail), "hipFree(detail)");
+                }
+                if (st.tag)
+                {
+                    hip_check(hipFree(st.tag), "hipFree(tag)");
+                }
+                st.capacity = std::max<int64_t>(n, 1);
+                hip_check(hipMalloc(reinterpret_cast<void**>(&st.detail), size_t(st.capacity) * 8), "hipMalloc(detail)");
+                hip_check(hipMalloc(reinterpret_cast<void**>(&st.tag), size_t(st.capacity)), "hipMalloc(tag)");
+            }
+        }
+
+        // update_fields (algorithm/update_fields.hpp:27-127) into a new mesh image
+        void update_fields(Field& f, MRMeshData& nm, DevMesh& ndev)
+        {
+            Mesh& m         = *f.mesh;
+            Ctx& ctx        = *m.ctx;
+            hipStream_t s   = ctx.stream;
+            DevMesh& odev   = *m.dev;
+            DevWorkList copy, coarsen, refine;
+            build_update_fields_lists(m.host, nm, copy.host, coarsen.host, refine.host);
+            upload_list(ctx, copy, ndev.table, odev.table);
+            upload_list(ctx, coarsen, ndev.table, odev.table);
+            upload_list(ctx, refine, ndev.table, odev.table);
+            ctx.check_resolve();
+            double* nd = nullptr;
+            hip_check(hipMalloc(reinterpret_cast<void**>(&nd), size_t(std::max<int64_t>(nm.nref, 1)) * 8), "hipMalloc(new field)");
+            hip_check(hipMemsetAsync(nd, 0, size_t(nm.nref) * 8, s), "hipMemsetAsync(new field)");
+            dev::copy_cells(copy.d, 0, copy.d.ncells, nd, f.d, s);
+            dev::project(nm.cfg.dim, coarsen.d, 0, coarsen.d.ncells, nd, f.d, s);
+            dev::predict_children(nm.cfg.dim, nm.cfg.pred_s, refine.d, 0, refine.d.ncells, nd, f.d, s);
+            ctx.sync(); // the lists die at scope exit
+            hip_check(hipFree(f.d), "hipFree(old field)");
+            f.d    = nd;
+            f.size = nm.nref;
+        }
+    } // namespace
+
+    bool harten(AdaptState& st, int ite, double eps, double regularity)
+    {
+        Field& f        = *st.field;
+        f.check_current();
+        Mesh& m         = *f.mesh;
+        Ctx& ctx        = *m.ctx;
+        hipStream_t s   = ctx.stream;
+        const MeshConfig cfg = m.host.cfg;
+        const int dim = cfg.dim, minl = cfg.min_level, maxl = cfg.max_level;
+        DevMesh& dm     = m.device_adapt();
+        const int64_t n = m.host.nref;
+        ensure_adapt_buffers(st, n);
+        hip_check(hipMemsetAsync(st.detail, 0, size_t(n) * 8, s), "hipMemsetAsync(detail)");
+        hip_check(hipMemsetAsync(st.tag, 0, size_t(n), s), "hipMemsetAsync(tag)");
+        dev::tag_init(dm.cells.d, 0, dm.cells.d.ncells, st.tag, s);
+
+        update_ghost_mr(f);
+
+        // detail computation: levels [min-1, max-ite)
+        {
+            int64_t c0, c1;
+            dm.detail.cells_of_levels(minl > 0 ? minl - 1 : 0, maxl - ite, c0, c1);
+            dev::detail(dim, cfg.pred_s, dm.detail.d, c0, c1, f.d, st.detail, s);
+        }
+        // tag cells: fine levels [min, max-ite], i.e. coarse levels [min-1, max-ite-1]
+        {
+            dev::Thresholds thr{};
+            for (int level = minl; level <= maxl - ite; ++level)
+            {
+                const int exponent  = dim * (maxl - level);
+                const double eps_l  = eps / (1 << exponent);
+                const double reg    = regularity + dim;
+                const double fine   = std::pow(2.0, reg) * eps_l;
+                thr.eps[level]      = eps_l;
+                thr.fine[level]     = fine;
+                thr.coarse[level]   = fine / (1 << dim);
+            }
+            int64_t c0, c1;
+            dm.criteria.cells_of_levels(minl - 1, maxl - ite, c0, c1);
+            dev::criteria(dim, dm.criteria.d, c0, c1, st.detail, st.tag, thr, minl, maxl, s);
+        }
+        // tag finalization: levels max..1, sequential
+        for (int level = maxl; level > 0; --level)
+        {
+            int64_t c0, c1;
+            dm.maximum.cells_of_levels(level - 1, level, c0, c1);
+            dev::maximum(dim, dm.maximum.d, c0, c1, st.tag, s);
+        }
+        st.last_tag.resize(size_t(n));
+        st.last_detail.resize(size_t(n));
+        hip_check(hipMemcpyAsync(st.last_tag.data(), st.tag, size_t(n), hipMemcpyDeviceToHost, s), "hipMemcpyAsync(tag)");
+        hip_check(hipMemcpyAsync(st.last_detail.data(), st.detail, size_t(n) * 8, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(detail)");
+        ctx.sync();
+        st.last_size = n;
+
+        // mesh update on the host (graduation.hpp:743-843, 573-726)
+        CellArray new_ca = update_cell_array_from_tag(m.host, st.last_tag.data());
+        make_graduation(new_ca, dim, cfg.grad_width);
+        if (ca_equal(m.host.ids[CELLS], new_ca))
+        {
+            return true;
+        }
+        MRMeshData nm = build_mesh(cfg, new_ca);
+        std::unique_ptr<DevMesh> ndev = make_dev_mesh(ctx, nm);
+        update_fields(f, nm, *ndev);
+        m.replace(std::move(nm), std::move(ndev));
+        f.version = m.version;
+        return false;
+    }
+
+    int adapt(AdaptState& st, double eps, double regularity, int max_iterations)
+    {
+        Mesh& m        = *st.field->mesh;
+        const int minl = m.host.cfg.min_level, maxl = m.host.cfg.max_level;
+        if (minl == maxl)
+        {
+            return 0;
+        }
+        int limit = maxl - minl;
+        if (max_iterations >= 0)
+        {
+            limit = std::min(limit, max_iterations);
+        }
+        for (int i = 0; i < limit; ++i)
+        {
+            if (harten(st, i, eps, regularity))
+            {
+                return i + 1;
+            }
+        }
+        return limit;
+    }
+} // namespace smr

@@ -1,0 +1,135 @@
+// Runtime of the MI355X MR step: device images of the meshes, fields, and
+// the host/device orchestration of update_ghost_mr, the Harten adaptation,
+// update_fields and the upwind step.
+#pragma once
+
+#include "device/device_api.hpp"
+#include "host/programs.hpp"
+
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace smr
+{
+    struct Ctx;
+
+    void hip_check(hipError_t e, const char* what);
+
+    // One device allocation holding several host arrays (uploaded together).
+    class Blob
+    {
+      public:
+        Blob() = default;
+        ~Blob();
+        Blob(const Blob&)            = delete;
+        Blob& operator=(const Blob&) = delete;
+
+        // reserves space for n bytes copied from src (8-byte aligned); returns the offset
+        size_t add(const void* src, size_t n);
+        size_t reserve(size_t n);
+        void upload(hipStream_t s);
+        char* dptr(size_t off) const { return d_ + off; }
+        size_t bytes() const { return host_.size(); }
+
+      private:
+        std::vector<char> host_;
+        char* d_ = nullptr;
+    };
+
+    struct DevWorkList
+    {
+        WorkList host;
+        dev::List d{};
+        std::vector<int32_t> hint;
+        std::unique_ptr<Blob> blob;
+        // cell range [c0, c1) of the runs of levels [l0, l1)
+        void cells_of_levels(int l0, int l1, int64_t& c0, int64_t& c1) const;
+    };
+
+    struct DevRecipes
+    {
+        Recipes host;
+        std::unique_ptr<Blob> blob;
+        const int64_t* dst = nullptr;
+        const int32_t* op  = nullptr;
+        const int64_t* sp  = nullptr;
+        const int64_t* src = nullptr;
+    };
+
+    // Device image of one mesh version.
+    struct DevMesh
+    {
+        std::unique_ptr<Blob> table_blob;
+        dev::Table table{};
+        DevWorkList cells;
+        bool has_ghost = false, has_adapt = false;
+        std::vector<DevRecipes> outer;
+        DevWorkList projection, prediction;
+        DevWorkList detail, criteria, maximum;
+    };
+
+    struct Mesh
+    {
+        Ctx* ctx = nullptr;
+        MRMeshData host;
+        uint64_t version = 0;
+        std::unique_ptr<DevMesh> dev;
+
+        DevMesh& device();          // table + cells list (uploaded and resolved)
+        DevMesh& device_ghost();    // + ghost program
+        DevMesh& device_adapt();    // + adapt program
+        void replace(MRMeshData&& m, std::unique_ptr<DevMesh>&& d);
+    };
+
+    struct Field
+    {
+        Mesh* mesh = nullptr;
+        std::string name;
+        double* d       = nullptr;
+        int64_t size    = 0;
+        uint64_t version = 0;
+        bool has_bc     = false;
+        double bc_value = 0;
+        ~Field();
+        void allocate(bool zero);
+        void check_current() const;
+    };
+
+    struct AdaptState
+    {
+        Field* field   = nullptr;
+        double* detail = nullptr;
+        uint8_t* tag   = nullptr;
+        int64_t capacity = 0;
+        // tags/details of the last harten call on the mesh they belong to
+        std::vector<uint8_t> last_tag;
+        std::vector<double> last_detail;
+        int64_t last_size = 0;
+        ~AdaptState();
+    };
+
+    struct Ctx
+    {
+        int device         = -1;
+        hipStream_t stream = nullptr;
+        int* d_err         = nullptr;
+        ~Ctx();
+        bool has_device() const { return device >= 0; }
+        void require_device() const;
+        void sync();
+        void check_resolve();
+    };
+
+    std::unique_ptr<DevMesh> make_dev_mesh(Ctx& ctx, const MRMeshData& m);
+    // DevWorkList upload + resolve against the given tables
+    void upload_list(Ctx& ctx, DevWorkList& l, const dev::Table& t0, const dev::Table& t1);
+
+    void update_ghost_mr(Field& f);
+    void upwind(const double* a, double dt, Field& u, Field& unp1);
+    // Adapt::operator() (mr/adapt.hpp:147-195); returns the number of harten calls
+    int adapt(AdaptState& st, double eps, double regularity, int max_iterations = -1);
+    // one harten call; true at the fixed point
+    bool harten(AdaptState& st, int ite, double eps, double regularity);
+} // namespace smr

@@ -24,6 +24,8 @@ def lib():
         L.orc_last_error.restype = ctypes.c_char_p
         L.orc_create_box.restype = vp
         L.orc_create_box.argtypes = [i32] * 6
+        L.orc_create_from_cells.restype = vp
+        L.orc_create_from_cells.argtypes = [i32] * 6 + [P(ctypes.c_int32), i64]
         L.orc_free.argtypes = [vp]
         L.orc_set_poison.argtypes = [vp, i32]
         L.orc_nb_cells.restype = i64
@@ -59,9 +61,15 @@ def _check(rc):
 class OracleSim:
     """One scalar field on an MR mesh of the unit box (oracle side)."""
 
-    def __init__(self, dim, min_level, max_level, pred_s=1, radius=1, grad_width=1):
+    def __init__(self, dim, min_level, max_level, pred_s=1, radius=1, grad_width=1, cells=None):
+        """A box mesh at max_level, or the mesh of the given leaves (rows level, x, y, z)."""
         self.dim = dim
-        self.h = lib().orc_create_box(dim, min_level, max_level, pred_s, radius, grad_width)
+        if cells is None:
+            self.h = lib().orc_create_box(dim, min_level, max_level, pred_s, radius, grad_width)
+        else:
+            c = np.ascontiguousarray(np.asarray(cells)[:, :4], dtype=np.int32)
+            self.h = lib().orc_create_from_cells(dim, min_level, max_level, pred_s, radius, grad_width,
+                                                 c.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), c.shape[0])
         if not self.h:
             raise RuntimeError(lib().orc_last_error().decode())
 

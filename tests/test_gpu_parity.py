@@ -1,0 +1,176 @@
+"""Parity of the HIP path (through the C ABI) with the oracle and with the
+reference's golden fixtures.  Integer/flag data must be identical; FP64 fields
+are compared bit for bit against the oracle (same operation order, no
+contraction) and within 1e-12 against the fixtures."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import samurai_amd as sa
+from initial_conditions import disc_2d, seeded_2d, sphere_3d
+from oracle_lib import REFERENCE, OracleSim
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def pair(gpu_ctx, dim, lmin, lmax, s, ic="disc", adapt_eps=None, seed=0):
+    """An oracle sim and a product field on the same (optionally adapted) mesh,
+    with identical full storage (ghosts zero) and a Dirichlet-0 BC."""
+    o = OracleSim(dim, lmin, lmax, pred_s=s)
+    o.set_poison(False)
+    cells = o.cells()
+    if ic == "disc":
+        v = disc_2d(cells) if dim == 2 else sphere_3d(cells)
+    else:
+        v = seeded_2d(cells, seed)
+    o.set_values(v)
+    o.set_dirichlet(0.0)
+    if adapt_eps is not None:
+        o.adapt(adapt_eps, 2.0)
+    mesh = sa.make_mesh_from_cells(gpu_ctx, sa.mesh_config(dim, lmin, lmax, s), o.cells())
+    u = sa.make_scalar_field("u", mesh)
+    sa.make_bc_dirichlet(u, 0.0)
+    st = o.storage()
+    st = np.where(np.isnan(st), 0.0, st)
+    o.set_storage(st)
+    u.set_storage(st)
+    return o, mesh, u
+
+
+def bits(a):
+    return np.ascontiguousarray(a).view(np.uint64)
+
+
+@pytest.mark.parametrize("dim,level", [(2, 6), (3, 4)])
+def test_uniform_upwind(gpu_ctx, dim, level):
+    say("test_uniform_upwind", dim, level)
+    o, mesh, u = pair(gpu_ctx, dim, level, level, 1)
+    rng = np.random.default_rng(3)
+    v = rng.random(mesh.nb_cells())
+    o.set_values(v)
+    u.set_cells(v)
+    unp1 = sa.make_scalar_field("unp1", mesh)
+    a = np.array([np.cos(np.pi / 5), np.sin(np.pi / 5), 0.3][:dim])
+    dt = 0.5 / (1 << level)
+    o.upwind_step(a, dt)
+    sa.update_ghost_mr(u)
+    sa.upwind(a, dt, u, unp1)
+    assert np.array_equal(bits(o.values()), bits(unp1.cells()))
+
+
+@pytest.mark.parametrize("dim,lmin,lmax,s", [(2, 2, 6, 1), (2, 3, 7, 0), (3, 2, 5, 1), (3, 2, 4, 0)])
+def test_ghost_update_matches_oracle(gpu_ctx, dim, lmin, lmax, s):
+    say("test_ghost_update_matches_oracle", dim, lmin, lmax, s)
+    o, mesh, u = pair(gpu_ctx, dim, lmin, lmax, s, adapt_eps=1e-3)
+    rng = np.random.default_rng(7)
+    st = rng.random(mesh.nb_cells(sa.REFERENCE))
+    o.set_storage(st)
+    u.set_storage(st)
+    o.update_ghost_mr()
+    sa.update_ghost_mr(u)
+    assert np.array_equal(bits(o.storage()), bits(u.storage()))
+
+
+@pytest.mark.parametrize("dim,lmin,lmax,s", [(2, 2, 6, 1), (3, 2, 5, 1)])
+def test_upwind_adapted_matches_oracle(gpu_ctx, dim, lmin, lmax, s):
+    say("test_upwind_adapted_matches_oracle", dim, lmin, lmax, s)
+    o, mesh, u = pair(gpu_ctx, dim, lmin, lmax, s, adapt_eps=1e-3)
+    unp1 = sa.make_scalar_field("unp1", mesh)
+    a = np.array([1.0, 0.5, -0.25][:dim])
+    dt = 0.4 / (1 << lmax)
+    o.upwind_step(a, dt)
+    sa.update_ghost_mr(u)
+    sa.upwind(a, dt, u, unp1)
+    assert np.array_equal(bits(o.values()), bits(unp1.cells()))
+
+
+@pytest.mark.parametrize("dim,lmin,lmax,s,ic", [(2, 3, 7, 1, "disc"), (2, 2, 7, 1, "seeded"), (2, 3, 6, 0, "disc"),
+                                                (3, 2, 5, 1, "disc")])
+def test_harten_matches_oracle(gpu_ctx, dim, lmin, lmax, s, ic):
+    """Flags, details, new mesh and transferred field of every harten call."""
+    say("test_harten_matches_oracle", dim, lmin, lmax, s, ic)
+    o, mesh, u = pair(gpu_ctx, dim, lmin, lmax, s, ic=ic)
+    ad = sa.make_MRAdapt(u)
+    eps = 1e-4 if dim == 2 else 1e-3
+    for ite in range(lmax - lmin):
+        fo = o.harten(ite, eps, 2.0)
+        fp = ad.harten(ite, eps, 2.0)
+        to, do = o.last_tags()
+        tp, dp = ad.last_tags()
+        assert fo == fp
+        assert np.array_equal(to, tp), ite
+        assert np.array_equal(bits(do), bits(dp)), ite
+        assert np.array_equal(o.cells(), mesh.cells()), ite
+        assert np.array_equal(o.cells(REFERENCE), mesh.cells(sa.REFERENCE)), ite
+        assert np.array_equal(bits(o.storage()), bits(u.storage())), ite
+        if fo:
+            break
+
+
+def run_demo(gpu_ctx, s, n_steps=None):
+    cfg = sa.mesh_config(2, 4, 10, s)
+    mesh = sa.make_mesh(gpu_ctx, cfg)
+    u = sa.make_scalar_field("u", mesh)
+    u.set_cells(disc_2d(mesh.cells()))
+    sa.make_bc_dirichlet(u, 0.0)
+    unp1 = sa.make_scalar_field("unp1", mesh)
+    adapt = sa.make_MRAdapt(u)
+    adapt(2e-4, 1.0)
+    init = (mesh.cells(), u.cells())
+    dt, Tf, t, n = 0.5 * (1.0 / (1 << 10)), 0.01, 0.0, 0
+    a = np.array([1.0, 1.0])
+    while t != Tf and (n_steps is None or n < n_steps):
+        adapt(2e-4, 1.0)
+        t += dt
+        if t > Tf:
+            dt += Tf - t
+            t = Tf
+        sa.update_ghost_mr(u)
+        unp1.resize()
+        sa.upwind(a, dt, u, unp1)
+        sa.swap(u, unp1)
+        n += 1
+    return init, (mesh.cells(), u.cells()), n
+
+
+@pytest.mark.parametrize("s", [0, 1])
+def test_advection_2d_demo_against_fixtures(gpu_ctx, s):
+    say("test_advection_2d_demo_against_fixtures", s)
+    (c0, v0), (c1, v1), n = run_demo(gpu_ctx, s)
+    assert n == 21
+    for (c, v), name in [((c0, v0), f"advection_2d_pred{s}_init.npz"), ((c1, v1), f"advection_2d_pred{s}_t0.01.npz")]:
+        fx = np.load(os.path.join(GOLDEN, name))
+        assert c.shape[0] == fx["level"].size, name
+        assert np.array_equal(c[:, 0], fx["level"]) and np.array_equal(c[:, 1], fx["i"]) and np.array_equal(c[:, 2], fx["j"])
+        assert np.max(np.abs(v - fx["u"])) <= 1e-12, name
+
+
+@pytest.mark.parametrize("dim,lmin,lmax", [(2, 3, 7), (3, 2, 5)])
+def test_time_steps_match_oracle(gpu_ctx, dim, lmin, lmax):
+    """Adapt + ghost update + upwind for several steps, bit for bit."""
+    say("test_time_steps_match_oracle", dim, lmin, lmax)
+    o, mesh, u = pair(gpu_ctx, dim, lmin, lmax, 1)
+    unp1 = sa.make_scalar_field("unp1", mesh)
+    ad = sa.make_MRAdapt(u)
+    a = np.array([1.0, 1.0, 1.0][:dim])
+    dt = (0.5 if dim == 2 else 0.4 / 3) / (1 << lmax)
+    eps = 1e-4 if dim == 2 else 1e-3
+    for step in range(4):
+        no = o.adapt(eps, 2.0)
+        npr = ad(eps, 2.0)
+        assert no == npr
+        o.upwind_step(a, dt)
+        sa.update_ghost_mr(u)
+        unp1.resize()
+        sa.upwind(a, dt, u, unp1)
+        sa.swap(u, unp1)
+        assert np.array_equal(o.cells(), mesh.cells()), step
+        assert np.array_equal(bits(o.values()), bits(u.cells())), step
