@@ -106,6 +106,7 @@ def main():
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--seed", type=int, default=0)
     p.add_argument("--uniform-reps", type=int, default=20)
+    p.add_argument("--phase-steps", type=int, default=3, help="extra steps timed per phase (0 disables)")
     p.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle time (0 disables)")
     args = p.parse_args()
     if args.gpus != 1:
@@ -162,6 +163,19 @@ def main():
     upw_bytes = float(np.mean(byts))
     upw_ms = float(np.mean(kms))
     adaptive_gbs = upw_bytes / (upw_ms * 1e-3) / 1e9
+
+    # phase breakdown under the reference's timer names (synchronizing, so
+    # measured on extra steps outside the timed region)
+    phases = None
+    if args.phase_steps > 0:
+        sa.timers_reset()
+        sa.timers_enable(True)
+        for _ in range(args.phase_steps):
+            step()
+        ctx.synchronize()
+        sa.timers_enable(False)
+        phases = {k: {"ms_per_step": v["seconds"] / args.phase_steps * 1e3, "calls": v["calls"]}
+                  for k, v in sa.timers_report().items()}
     final_cells = mesh.nb_cells()
 
     # ---------------- uniform level-12 roofline check ----------------
@@ -216,6 +230,7 @@ def main():
                               "avg_launch_ms": upw_ms, "achieved": adaptive_gbs,
                               "frac": adaptive_gbs / HBM_PEAK_GBS},
         "cpu_baseline": cpu,
+        "phases": phases,
     }
     print(json.dumps(out), flush=True)
 

@@ -137,6 +137,21 @@ void* smr_ctx_stream(smr_ctx* ctx);
  * distinct cell read (the leaves and their face neighbours) plus 8 bytes per
  * leaf written. */
 int64_t smr_mesh_upwind_bytes(const smr_mesh* mesh);
+/* Phase timers under the reference's ScopedTimer names (timers.hpp:979-1001;
+ * "mesh adaptation", "ghost update", "detail computation", "tag cells",
+ * "tag finalization", "mesh update", "fields update", "field expressions"),
+ * process-wide.  While enabled, every phase synchronizes its stream at entry
+ * and exit, so use it for breakdowns, not for throughput.  The report is a JSON
+ * object {name: {"seconds": s, "calls": n}} written into buf (truncated to
+ * size); returns its full length. */
+/* Tuning knob (no counterpart in the reference): selects the launch shape of
+ * the upwind kernel, 0 <= id < smr_upwind_variant_count(); used by the
+ * autotuner.  Every variant computes identical results. */
+int smr_upwind_variant_count(void);
+int smr_set_upwind_variant(int id);
+int smr_timers_enable(int on);
+int smr_timers_reset(void);
+int64_t smr_timers_report(char* buf, int64_t size);
 
 #ifdef __cplusplus
 }

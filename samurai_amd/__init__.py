@@ -59,6 +59,9 @@ def lib():
         L.smr_ctx_stream.argtypes = [vp]
         L.smr_mesh_upwind_bytes.restype = i64
         L.smr_mesh_upwind_bytes.argtypes = [vp]
+        L.smr_timers_enable.argtypes = [i32]
+        L.smr_timers_report.restype = i64
+        L.smr_timers_report.argtypes = [ctypes.c_char_p, i64]
         L.smr_field_create.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(vp)]
         L.smr_field_destroy.argtypes = [vp]
         L.smr_field_resize.argtypes = [vp]
@@ -226,6 +229,23 @@ class ScalarField:
         _check(lib().smr_field_set_storage(self.h, _dp(v)))
 
 
+def timers_enable(on=True):
+    """Phase timers under the reference's timer names (synchronizing: diagnostics only)."""
+    lib().smr_timers_enable(int(on))
+
+
+def timers_reset():
+    lib().smr_timers_reset()
+
+
+def timers_report():
+    import json
+    n = lib().smr_timers_report(None, 0)
+    buf = ctypes.create_string_buffer(int(n) + 1)
+    lib().smr_timers_report(buf, n + 1)
+    return json.loads(buf.value.decode())
+
+
 def make_scalar_field(name, mesh):
     return ScalarField(mesh, name)
 
@@ -279,3 +299,12 @@ class Adapt:
 
 def make_MRAdapt(field):  # noqa: N802 (reference name)
     return Adapt(field)
+
+
+def upwind_variant_count():
+    return lib().smr_upwind_variant_count()
+
+
+def set_upwind_variant(i):
+    """Launch shape of the upwind kernel (tuning knob; results are identical)."""
+    _check(lib().smr_set_upwind_variant(int(i)))

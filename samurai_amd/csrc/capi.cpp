@@ -121,6 +121,12 @@ extern "C"
                     {
                         smr::hip_check(hipSetDevice(device), "hipSetDevice");
                         smr::hip_check(hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking), "hipStreamCreate");
+                        // keep freed stream-ordered memory cached in the default pool
+                        hipMemPool_t pool = nullptr;
+                        smr::hip_check(hipDeviceGetDefaultMemPool(&pool, device), "hipDeviceGetDefaultMemPool");
+                        uint64_t threshold = UINT64_MAX;
+                        smr::hip_check(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &threshold),
+                                       "hipMemPoolSetAttribute");
                         smr::hip_check(hipMalloc(reinterpret_cast<void**>(&c->c.d_err), sizeof(int)), "hipMalloc(err)");
                         smr::hip_check(hipMemsetAsync(c->c.d_err, 0, sizeof(int), c->c.stream), "hipMemsetAsync(err)");
                         c->c.sync();
@@ -275,13 +281,24 @@ extern "C"
                 require(mesh && tags, "null argument");
                 smr::MRMeshData& m = mesh->m.host;
                 require(n == m.nref, "tag array size differs from the storage size");
-                smr::CellArray ca = smr::update_cell_array_from_tag(m, tags);
-                smr::make_graduation(ca, m.cfg.dim, m.cfg.grad_width);
+                smr::CellArray ca;
+                {
+                    smr::ScopedTimer t1(nullptr, "update_cell_array_from_tag");
+                    ca = smr::update_cell_array_from_tag(m, tags);
+                }
+                {
+                    smr::ScopedTimer t2(nullptr, "make_graduation");
+                    smr::make_graduation(ca, m.cfg.dim, m.cfg.grad_width);
+                }
                 if (smr::ca_equal(ca, m.ids[smr::CELLS]))
                 {
                     return 1;
                 }
-                smr::MRMeshData nm = smr::build_mesh(m.cfg, ca);
+                smr::MRMeshData nm;
+                {
+                    smr::ScopedTimer t3(nullptr, "mesh construction");
+                    nm = smr::build_mesh(m.cfg, ca);
+                }
                 smr::WorkList copy, coarsen, refine;
                 smr::build_update_fields_lists(m, nm, copy, coarsen, refine);
                 for (const smr::WorkList* w : {&copy, &coarsen, &refine})
@@ -295,6 +312,46 @@ extern "C"
                 mesh->m.replace(std::move(nm), nullptr);
                 return 0;
             });
+    }
+
+    int smr_upwind_variant_count(void)
+    {
+        return smr::dev::upwind_variant_count();
+    }
+
+    int smr_set_upwind_variant(int id)
+    {
+        if (id < 0 || id >= smr::dev::upwind_variant_count())
+        {
+            g_error = "upwind variant out of range";
+            return SMR_ERR_ARGUMENT;
+        }
+        smr::dev::set_upwind_variant(id);
+        return 0;
+    }
+
+    int smr_timers_enable(int on)
+    {
+        smr::timers().enabled = on != 0;
+        return 0;
+    }
+
+    int smr_timers_reset(void)
+    {
+        smr::timers().entries.clear();
+        return 0;
+    }
+
+    int64_t smr_timers_report(char* buf, int64_t size)
+    {
+        const std::string r = smr::timers().report();
+        if (buf && size > 0)
+        {
+            const size_t n = std::min<size_t>(r.size(), size_t(size - 1));
+            std::memcpy(buf, r.data(), n);
+            buf[n] = 0;
+        }
+        return int64_t(r.size());
     }
 
     void* smr_ctx_stream(smr_ctx* ctx)
@@ -573,14 +630,7 @@ extern "C"
             [&]
             {
                 require(adapt != nullptr, "null adapt");
-                if (tags)
-                {
-                    std::memcpy(tags, adapt->a.last_tag.data(), adapt->a.last_tag.size());
-                }
-                if (details)
-                {
-                    std::memcpy(details, adapt->a.last_detail.data(), adapt->a.last_detail.size() * 8);
-                }
+                adapt->a.copy_last(tags, details, adapt->a.field->mesh->ctx->stream);
                 return 0;
             });
     }

@@ -28,8 +28,10 @@ namespace smr::dev
         const int32_t* hint; // run holding cell 256*j, j = 0..ceil(ncells/256)
         int64_t* base;       // nruns * ntaps
         const int8_t* taps;  // 7 per tap: table, dl, dx, dy, dz, wlo, whi
+        const int32_t* tiles; // (run, x0) of every 64-cell tile of a run, runs in order
         int64_t nruns;
         int64_t ncells;
+        int64_t ntiles;
         int ntaps;
     };
 
@@ -43,6 +45,9 @@ namespace smr::dev
     constexpr int HINT_SHIFT = 8; // 256 cells per hint chunk
 
     void resolve(const List& l, const Table& t0, const Table& t1, int* err, hipStream_t s);
+    // launch shape of the upwind kernel (cells per thread, XCD remap, stores)
+    int upwind_variant_count();
+    void set_upwind_variant(int id);
     void upwind(int dim, const List& l, int64_t c0, int64_t c1, const double* u, double* unp1, double a0, double a1, double a2,
                 double dt, hipStream_t s);
     void project(int dim, const List& l, int64_t c0, int64_t c1, double* dst, const double* src, hipStream_t s);

@@ -10,6 +10,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 namespace smr::dev
 {
     namespace
@@ -167,31 +169,119 @@ namespace smr::dev
             return l.base + run * l.ntaps;
         }
 
-        // flux of upwind_op (stencil_field.hpp:91-97)
-        __device__ __forceinline__ double flux(double a, double ul, double ur)
+        // The hardware deals consecutive workgroups to the 8 XCDs round-robin.
+        // Re-map them so that each XCD owns a contiguous range of tiles: rows
+        // that are y-neighbours are then read through the same L2.  The grid is
+        // launched as a multiple of 8 blocks; `nlogical` tiles exist.
+        __device__ __forceinline__ int64_t xcd_tile(unsigned nlogical)
         {
-            return .5 * a * (ul + ur) + .5 * fabs(a) * (ul - ur);
+            const unsigned per = gridDim.x >> 3;
+            const unsigned t   = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+            return t < nlogical ? int64_t(t) : -1;
         }
 
-        // unp1 = u - dt * upwind(a, u) on mesh[cells] (stencil_field.hpp:37-53)
-        template <int DIM>
-        __global__ __launch_bounds__(BLOCK) void k_upwind(List l, int64_t c0, int64_t c1, const double* __restrict__ u,
-                                                          double* __restrict__ unp1, double a0, double a1, double a2, double dt)
+        // flux of upwind_op (stencil_field.hpp:91-97),
+        //   .5 * a * (ul + ur) + .5 * |a| * (ul - ur),
+        // with the scalar factors (0.5*a, 0.5*|a|) formed once per launch
+        __device__ __forceinline__ double flux_h(double ha, double habs, double ul, double ur)
         {
-            const int64_t c = c0 + int64_t(blockIdx.x) * BLOCK + threadIdx.x;
-            if (c >= c1)
+            return ha * (ul + ur) + habs * (ul - ur);
+        }
+
+        // unp1 = u - dt * upwind(a, u) on mesh[cells] (stencil_field.hpp:37-53).
+        // Each block processes CPT*BLOCK consecutive cells, BLOCK at a time.
+        // S / dx with dx = 2^-level is computed as S * 2^level: both are the
+        // exact value S*2^level, so the result is bit-identical.
+        template <int DIM, int CPT, bool REMAP, bool NT>
+        __global__ __launch_bounds__(BLOCK) void k_upwind(List l, int64_t c0, int64_t c1, const double* __restrict__ u,
+                                                          double* __restrict__ unp1, double a0, double a1, double a2, double dt,
+                                                          unsigned nlogical)
+        {
+            const int64_t tile = REMAP ? xcd_tile(nlogical) : int64_t(blockIdx.x);
+            if (tile < 0 || tile >= int64_t(nlogical))
             {
                 return;
             }
-            const Cell cell  = locate(l, c);
-            const int64_t* b = bases(l, cell.run);
-            const int64_t ic = b[0] + cell.x;
-            const double uc  = u[ic];
-            const double dx  = 1.0 / (1 << cell.level);
-            const double FL  = flux(a0, u[ic - 1], uc);
-            const double FR  = flux(a0, uc, u[ic + 1]);
-            const double FD  = flux(a1, u[b[1] + cell.x], uc);
-            const double FU  = flux(a1, uc, u[b[2] + cell.x]);
+            const double ha0 = .5 * a0, hb0 = .5 * fabs(a0);
+            const double ha1 = .5 * a1, hb1 = .5 * fabs(a1);
+            const double ha2 = .5 * a2, hb2 = .5 * fabs(a2);
+            const int64_t first = c0 + tile * (BLOCK * CPT) + threadIdx.x;
+#pragma unroll
+            for (int k = 0; k < CPT; ++k)
+            {
+                const int64_t c = first + k * BLOCK;
+                if (c >= c1)
+                {
+                    return;
+                }
+                const Cell cell    = locate(l, c);
+                const int64_t* b   = bases(l, cell.run);
+                const double* uc_p = u + b[0] + cell.x;
+                const double uc    = uc_p[0];
+                const double FL    = flux_h(ha0, hb0, uc_p[-1], uc);
+                const double FR    = flux_h(ha0, hb0, uc, uc_p[1]);
+                const double FD    = flux_h(ha1, hb1, u[b[1] + cell.x], uc);
+                const double FU    = flux_h(ha1, hb1, uc, u[b[2] + cell.x]);
+                double S;
+                if constexpr (DIM == 2)
+                {
+                    S = -FL + FR + -FD + FU;
+                }
+                else
+                {
+                    const double FF = flux_h(ha2, hb2, u[b[3] + cell.x], uc);
+                    const double FB = flux_h(ha2, hb2, uc, u[b[4] + cell.x]);
+                    S               = -FL + FR + -FD + FU + -FF + FB;
+                }
+                const double inv_dx = double(1 << cell.level);
+                const double v      = uc - dt * (S * inv_dx);
+                if constexpr (NT)
+                {
+                    __builtin_nontemporal_store(v, unp1 + b[0] + cell.x);
+                }
+                else
+                {
+                    unp1[b[0] + cell.x] = v;
+                }
+            }
+        }
+
+        // Upwind on wave tiles: every wavefront owns one tile of <= 64 consecutive
+        // cells of a single run, so the run's level, end and resolved row bases
+        // are wave-uniform and live in scalar registers; each lane then does five
+        // coalesced loads and one store.  Same arithmetic as k_upwind.
+        template <int DIM, bool NT>
+        __global__ __launch_bounds__(BLOCK) void k_upwind_tiles(List l, int64_t t0, int64_t t1, const double* __restrict__ u,
+                                                                double* __restrict__ unp1, double a0, double a1, double a2, double dt,
+                                                                unsigned nlogical)
+        {
+            const int64_t blk = xcd_tile(nlogical);
+            if (blk < 0)
+            {
+                return;
+            }
+            const int64_t t = t0 + blk * (BLOCK / 64) + (threadIdx.x >> 6);
+            if (t >= t1)
+            {
+                return;
+            }
+            const int32_t run = __builtin_amdgcn_readfirstlane(l.tiles[2 * t]);
+            const int32_t xs  = __builtin_amdgcn_readfirstlane(l.tiles[2 * t + 1]);
+            const int32_t x   = xs + int32_t(threadIdx.x & 63);
+            const int32_t* iv = l.iv + 5 * int64_t(run);
+            if (x >= iv[1])
+            {
+                return;
+            }
+            const int64_t* b   = l.base + int64_t(run) * l.ntaps;
+            const double ha0 = .5 * a0, hb0 = .5 * fabs(a0);
+            const double ha1 = .5 * a1, hb1 = .5 * fabs(a1);
+            const double* uc_p = u + b[0] + x;
+            const double uc    = uc_p[0];
+            const double FL    = flux_h(ha0, hb0, uc_p[-1], uc);
+            const double FR    = flux_h(ha0, hb0, uc, uc_p[1]);
+            const double FD    = flux_h(ha1, hb1, u[b[1] + x], uc);
+            const double FU    = flux_h(ha1, hb1, uc, u[b[2] + x]);
             double S;
             if constexpr (DIM == 2)
             {
@@ -199,11 +289,20 @@ namespace smr::dev
             }
             else
             {
-                const double FF = flux(a2, u[b[3] + cell.x], uc);
-                const double FB = flux(a2, uc, u[b[4] + cell.x]);
-                S               = -FL + FR + -FD + FU + -FF + FB;
+                const double ha2 = .5 * a2, hb2 = .5 * fabs(a2);
+                const double FF  = flux_h(ha2, hb2, u[b[3] + x], uc);
+                const double FB  = flux_h(ha2, hb2, uc, u[b[4] + x]);
+                S                = -FL + FR + -FD + FU + -FF + FB;
             }
-            unp1[ic] = uc - dt * (S / dx);
+            const double v = uc - dt * (S * double(1 << iv[4]));
+            if constexpr (NT)
+            {
+                __builtin_nontemporal_store(v, unp1 + b[0] + x);
+            }
+            else
+            {
+                unp1[b[0] + x] = v;
+            }
         }
 
         // projection_op_ (numeric/projection.hpp:15-65): taps dst, then the 2^(dim-1) fine rows
@@ -594,7 +693,44 @@ namespace smr::dev
         {
             return unsigned((n + BLOCK - 1) / BLOCK);
         }
+
+        // launch shapes of the upwind kernel: cells per thread, XCD remap,
+        // non-temporal stores
+        struct UpwindVariant
+        {
+            int id, cpt;
+            bool remap, nt;
+        };
+        // cpt 0: wave tiles (k_upwind_tiles)
+        constexpr UpwindVariant kUpwindVariants[] = {{0, 1, false, false}, {1, 1, true, false}, {2, 2, false, false},
+                                                     {3, 2, true, false},  {4, 4, false, false}, {5, 4, true, true},
+                                                     {6, 1, false, true},  {7, 8, false, false}, {8, 0, true, true},
+                                                     {9, 0, true, false}};
+        constexpr int kNumUpwindVariants = 10;
+        constexpr int kDefaultUpwindVariant = 5;
+        int g_upwind_variant = -1;
+
+        UpwindVariant upwind_variant()
+        {
+            if (g_upwind_variant < 0)
+            {
+                const char* e    = std::getenv("SMR_UPWIND_VARIANT");
+                int id           = e ? std::atoi(e) : kDefaultUpwindVariant;
+                g_upwind_variant = (id >= 0 && id < kNumUpwindVariants) ? id : kDefaultUpwindVariant;
+            }
+            return kUpwindVariants[g_upwind_variant];
+        }
     } // namespace
+
+    int upwind_variant_count()
+    {
+        return kNumUpwindVariants;
+    }
+
+    void set_upwind_variant(int id)
+    {
+        g_upwind_variant = (id >= 0 && id < kNumUpwindVariants) ? id : kDefaultUpwindVariant;
+    }
 
     void resolve(const List& l, const Table& t0, const Table& t1, int* err, hipStream_t s)
     {
@@ -612,14 +748,65 @@ namespace smr::dev
         {
             return;
         }
+        const UpwindVariant v = upwind_variant();
+        if (v.cpt == 0 && c0 == 0 && c1 == l.ncells)
+        {
+            const int64_t nblk    = (l.ntiles + BLOCK / 64 - 1) / (BLOCK / 64);
+            const unsigned ngrid  = unsigned((nblk + 7) / 8 * 8);
+            const unsigned nlog   = unsigned(nblk);
+            if (dim == 2)
+            {
+                if (v.nt)
+                {
+                    hipLaunchKernelGGL((k_upwind_tiles<2, true>), dim3(ngrid), dim3(BLOCK), 0, s, l, 0, l.ntiles, u, unp1, a0, a1, a2, dt, nlog);
+                }
+                else
+                {
+                    hipLaunchKernelGGL((k_upwind_tiles<2, false>), dim3(ngrid), dim3(BLOCK), 0, s, l, 0, l.ntiles, u, unp1, a0, a1, a2, dt, nlog);
+                }
+            }
+            else
+            {
+                if (v.nt)
+                {
+                    hipLaunchKernelGGL((k_upwind_tiles<3, true>), dim3(ngrid), dim3(BLOCK), 0, s, l, 0, l.ntiles, u, unp1, a0, a1, a2, dt, nlog);
+                }
+                else
+                {
+                    hipLaunchKernelGGL((k_upwind_tiles<3, false>), dim3(ngrid), dim3(BLOCK), 0, s, l, 0, l.ntiles, u, unp1, a0, a1, a2, dt, nlog);
+                }
+            }
+            return;
+        }
+        const int cpt = v.cpt == 0 ? 4 : v.cpt;
+        const int64_t ntiles  = (c1 - c0 + BLOCK * cpt - 1) / (BLOCK * cpt);
+        const unsigned ngrid  = v.remap ? unsigned((ntiles + 7) / 8 * 8) : unsigned(ntiles);
+        const unsigned nlog   = unsigned(ntiles);
+        const dim3 g(ngrid), bl(BLOCK);
+#define SMR_UPWIND_LAUNCH(D, C, R, N)                                                                                                  \
+    hipLaunchKernelGGL((k_upwind<D, C, R, N>), g, bl, 0, s, l, c0, c1, u, unp1, a0, a1, a2, dt, nlog)
+#define SMR_UPWIND_DISPATCH(D)                                                                                                         \
+    switch (v.id)                                                                                                                      \
+    {                                                                                                                                  \
+        case 0: SMR_UPWIND_LAUNCH(D, 1, false, false); break;                                                                         \
+        case 1: SMR_UPWIND_LAUNCH(D, 1, true, false); break;                                                                          \
+        case 2: SMR_UPWIND_LAUNCH(D, 2, false, false); break;                                                                         \
+        case 3: SMR_UPWIND_LAUNCH(D, 2, true, false); break;                                                                          \
+        case 4: SMR_UPWIND_LAUNCH(D, 4, false, false); break;                                                                         \
+        case 5: SMR_UPWIND_LAUNCH(D, 4, true, true); break;                                                                           \
+        case 6: SMR_UPWIND_LAUNCH(D, 1, false, true); break;                                                                          \
+        default: SMR_UPWIND_LAUNCH(D, 8, false, false); break;                                                                        \
+    }
         if (dim == 2)
         {
-            hipLaunchKernelGGL(k_upwind<2>, dim3(nblocks(c1 - c0)), dim3(BLOCK), 0, s, l, c0, c1, u, unp1, a0, a1, a2, dt);
+            SMR_UPWIND_DISPATCH(2)
         }
         else
         {
-            hipLaunchKernelGGL(k_upwind<3>, dim3(nblocks(c1 - c0)), dim3(BLOCK), 0, s, l, c0, c1, u, unp1, a0, a1, a2, dt);
+            SMR_UPWIND_DISPATCH(3)
         }
+#undef SMR_UPWIND_DISPATCH
+#undef SMR_UPWIND_LAUNCH
     }
 
     void project(int dim, const List& l, int64_t c0, int64_t c1, double* dst, const double* src, hipStream_t s)

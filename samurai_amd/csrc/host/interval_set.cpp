@@ -329,6 +329,63 @@ namespace smr
 
     LevelSet from_runs(int dim, int level, std::vector<std::array<int32_t, 4>>& q)
     {
+        if (q.empty())
+        {
+            return LevelSet(dim, level);
+        }
+        // Rows are dense integers: bucket the runs by row (counting sort), then
+        // order each row's few runs by start.  Falls back to a comparison sort
+        // when the row range is much larger than the number of runs.
+        int32_t ymin = q[0][0], ymax = q[0][0], zmin = q[0][1], zmax = q[0][1];
+        for (const auto& r : q)
+        {
+            ymin = std::min(ymin, r[0]);
+            ymax = std::max(ymax, r[0]);
+            zmin = std::min(zmin, r[1]);
+            zmax = std::max(zmax, r[1]);
+        }
+        const int64_t ny = int64_t(ymax) - ymin + 1, nz = int64_t(zmax) - zmin + 1;
+        const int64_t nrows = ny * nz;
+        if (nrows <= 4 * int64_t(q.size()) + 4096)
+        {
+            std::vector<int64_t> start(size_t(nrows) + 1, 0);
+            for (const auto& r : q)
+            {
+                start[size_t((int64_t(r[1]) - zmin) * ny + (r[0] - ymin)) + 1]++;
+            }
+            for (size_t k = 1; k < start.size(); ++k)
+            {
+                start[k] += start[k - 1];
+            }
+            std::vector<std::array<int32_t, 2>> byrow(q.size());
+            std::vector<int64_t> fill(start.begin(), start.end() - 1);
+            for (const auto& r : q)
+            {
+                const size_t row = size_t((int64_t(r[1]) - zmin) * ny + (r[0] - ymin));
+                byrow[size_t(fill[row]++)] = {r[2], r[3]};
+            }
+            LevelSet out(dim, level);
+            out.ry.reserve(size_t(nrows));
+            out.runs.reserve(q.size());
+            for (int64_t row = 0; row < nrows; ++row)
+            {
+                auto b = byrow.begin() + start[size_t(row)], e = byrow.begin() + start[size_t(row) + 1];
+                if (b == e)
+                {
+                    continue;
+                }
+                if (!std::is_sorted(b, e))
+                {
+                    std::sort(b, e);
+                }
+                const int32_t y = int32_t(ymin + row % ny), z = int32_t(zmin + row / ny);
+                for (auto it = b; it != e; ++it)
+                {
+                    out.push(y, z, (*it)[0], (*it)[1]);
+                }
+            }
+            return out;
+        }
         std::sort(q.begin(),
                   q.end(),
                   [](const std::array<int32_t, 4>& u, const std::array<int32_t, 4>& v)
@@ -443,6 +500,34 @@ namespace smr
                 }
             });
         return from_runs(a.dim, a.level, q);
+    }
+
+    LevelSet expand_on(const LevelSet& a, int w, int level)
+    {
+        const int k = a.level - level;
+        if (k < 0)
+        {
+            return on(expand(a, w), level);
+        }
+        // the box dilation of a run projects onto a box of coarse cells
+        std::vector<std::array<int32_t, 4>> q;
+        const int wz = a.dim == 3 ? w : 0;
+        q.reserve(a.runs.size() * 3);
+        a.for_each_run(
+            [&](int32_t s, int32_t e, int32_t y, int32_t z)
+            {
+                const int32_t x0 = floor_shift(s - w, k), x1 = floor_shift(e - 1 + w, k) + 1;
+                const int32_t y0 = floor_shift(y - w, k), y1 = floor_shift(y + w, k);
+                const int32_t z0 = a.dim == 3 ? floor_shift(z - wz, k) : 0, z1 = a.dim == 3 ? floor_shift(z + wz, k) : 0;
+                for (int32_t cz = z0; cz <= z1; ++cz)
+                {
+                    for (int32_t cy = y0; cy <= y1; ++cy)
+                    {
+                        q.push_back({cy, cz, x0, x1});
+                    }
+                }
+            });
+        return from_runs(a.dim, level, q);
     }
 
     LevelSet box(int dim, int level, const std::array<int, 3>& lo, const std::array<int, 3>& hi)

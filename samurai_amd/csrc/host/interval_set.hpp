@@ -86,6 +86,8 @@ namespace smr
     LevelSet on(const LevelSet& a, int level);
     LevelSet translate(const LevelSet& a, const std::array<int, 3>& t);
     LevelSet expand(const LevelSet& a, int w);
+    // on(expand(a, w), level) in one pass
+    LevelSet expand_on(const LevelSet& a, int w, int level);
     LevelSet box(int dim, int level, const std::array<int, 3>& lo, const std::array<int, 3>& hi);
 
     // (y, z, s, e) quadruples in any order -> canonical set

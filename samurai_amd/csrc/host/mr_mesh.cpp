@@ -1,4 +1,5 @@
 #include "mr_mesh.hpp"
+#include "parallel.hpp"
 
 #include <algorithm>
 #include <cstdio>
@@ -167,31 +168,51 @@ namespace smr
         {
             const int r = m.cfg.radius, s = m.cfg.pred_s, dim = m.cfg.dim;
             CellArray cell_list = empty_cell_array(dim);
-            for (int l = 0; l < MAXL; ++l)
-            {
-                if (!m.ids[CELLS][size_t(l)].empty())
-                {
-                    cell_list[size_t(l)] = expand(m.ids[CELLS][size_t(l)], r);
-                }
-            }
+            parallel_for(MAXL,
+                         [&](int l)
+                         {
+                             if (!m.ids[CELLS][size_t(l)].empty())
+                             {
+                                 cell_list[size_t(l)] = expand(m.ids[CELLS][size_t(l)], r);
+                             }
+                         });
             m.ids[CELLS_AND_GHOSTS] = cell_list;
             m.ids[PROJ_CELLS]       = empty_cell_array(dim);
             if (m.cfg.max_level != m.cfg.min_level)
             {
+                // ghosts for the prediction one and two levels below each level
+                // with cells; the sets are built per level in parallel and merged
+                // into the cell list in level order
+                CellArray below1 = empty_cell_array(dim), below2 = empty_cell_array(dim);
+                parallel_for(2 * MAXL,
+                             [&](int task)
+                             {
+                                 const int level    = task / 2;
+                                 const LevelSet& cl = m.ids[CELLS][size_t(level)];
+                                 if (level < 1 || cl.empty())
+                                 {
+                                     return;
+                                 }
+                                 if (task % 2 == 0)
+                                 {
+                                     below1[size_t(level)] = set_intersection(expand(on(m.ids[CELLS_AND_GHOSTS][size_t(level)], level - 1), s),
+                                                                              expand(m.domain[size_t(level - 1)], s));
+                                 }
+                                 else if (level - 1 > 0)
+                                 {
+                                     below2[size_t(level)] = expand(on(cl, level - 2), s);
+                                 }
+                             });
                 for (int level = 1; level < MAXL; ++level)
                 {
-                    const LevelSet& cl = m.ids[CELLS][size_t(level)];
-                    if (cl.empty())
+                    if (m.ids[CELLS][size_t(level)].empty())
                     {
                         continue;
                     }
-                    // ghosts for the prediction one and two levels below
-                    LevelSet cg = set_intersection(expand(on(m.ids[CELLS_AND_GHOSTS][size_t(level)], level - 1), s),
-                                                   expand(m.domain[size_t(level - 1)], s));
-                    cell_list[size_t(level - 1)] = set_union(cell_list[size_t(level - 1)], cg);
+                    cell_list[size_t(level - 1)] = set_union(cell_list[size_t(level - 1)], below1[size_t(level)]);
                     if (level - 1 > 0)
                     {
-                        cell_list[size_t(level - 2)] = set_union(cell_list[size_t(level - 2)], expand(on(cl, level - 2), s));
+                        cell_list[size_t(level - 2)] = set_union(cell_list[size_t(level - 2)], below2[size_t(level)]);
                     }
                 }
                 m.ids[REFERENCE] = cell_list;
@@ -378,25 +399,44 @@ namespace smr
             // list_interval_to_refine_for_graduation (graduation.hpp:245-330)
             const int fmax = ca_max_level(ca) + 1;
             const int cmin = ca_min_level(ca);
-            for (int fine_level = fmax; fine_level > cmin + 1; --fine_level)
+            // the cascade of each fine level is independent: run them in
+            // parallel, then merge the coarse cells they flag level by level
+            std::vector<CellArray> part(MAXL);
+            parallel_for(MAXL,
+                         [&](int fine_level)
+                         {
+                             if (fine_level > fmax || fine_level <= cmin + 1 || ca[size_t(fine_level)].empty())
+                             {
+                                 return;
+                             }
+                             CellArray& out = part[size_t(fine_level)];
+                             out            = empty_cell_array(dim);
+                             LevelSet proj  = expand_on(ca[size_t(fine_level)], 2 * grad_width, fine_level - 2);
+                             for (int coarse_level = fine_level - 2;; --coarse_level)
+                             {
+                                 if (!proj.empty())
+                                 {
+                                     out[size_t(coarse_level)] = set_intersection(proj, ca[size_t(coarse_level)]);
+                                 }
+                                 if (coarse_level == cmin || proj.empty())
+                                 {
+                                     break;
+                                 }
+                                 proj = on(proj, coarse_level - 1);
+                             }
+                         });
+            for (int fine_level = MAXL - 1; fine_level >= 0; --fine_level)
             {
-                if (fine_level >= MAXL || ca[size_t(fine_level)].empty())
+                if (part[size_t(fine_level)].empty())
                 {
                     continue;
                 }
-                LevelSet proj = on(expand(ca[size_t(fine_level)], 2 * grad_width), fine_level - 2);
-                for (int coarse_level = fine_level - 2;; --coarse_level)
+                for (int l = 0; l < MAXL; ++l)
                 {
-                    if (!proj.empty())
+                    if (!part[size_t(fine_level)][size_t(l)].empty())
                     {
-                        remove[size_t(coarse_level)] = set_union(remove[size_t(coarse_level)],
-                                                                 set_intersection(proj, ca[size_t(coarse_level)]));
+                        remove[size_t(l)] = set_union(remove[size_t(l)], part[size_t(fine_level)][size_t(l)]);
                     }
-                    if (coarse_level == cmin || proj.empty())
-                    {
-                        break;
-                    }
-                    proj = on(proj, coarse_level - 1);
                 }
             }
             bool any_change   = false;

@@ -1,4 +1,5 @@
 #include "programs.hpp"
+#include "parallel.hpp"
 
 #include <algorithm>
 #include <numeric>
@@ -300,32 +301,45 @@ namespace smr
     GhostProgram build_ghost_program(const MRMeshData& m)
     {
         GhostProgram p;
-        const int dim = m.cfg.dim, s = m.cfg.pred_s;
+        const int dim = m.cfg.dim, s = m.cfg.pred_s, maxl = m.cfg.max_level;
         const auto cart = cartesian_directions(dim);
-        p.outer.resize(size_t(m.cfg.max_level) + 1);
-        for (int level = m.cfg.max_level; level >= 0; --level)
-        {
-            p.outer[size_t(level)] = outer_recipes(m, level, cart);
-        }
+        p.outer.resize(size_t(maxl) + 1);
+        std::vector<LevelSet> proj(size_t(maxl) + 1), pred(size_t(maxl) + 1);
+        // one task per level for each of: outer recipes, projection set, prediction set
+        parallel_for(3 * (maxl + 1),
+                     [&](int task)
+                     {
+                         const int kind = task % 3, level = task / 3;
+                         if (kind == 0)
+                         {
+                             p.outer[size_t(level)] = outer_recipes(m, level, cart);
+                         }
+                         else if (kind == 1 && level >= 1)
+                         {
+                             proj[size_t(level)] = set_intersection(on(m.ids[REFERENCE][size_t(level)], level - 1),
+                                                                    m.ids[PROJ_CELLS][size_t(level - 1)]);
+                         }
+                         else if (kind == 2 && level >= 1)
+                         {
+                             const LevelSet& ref = m.ids[REFERENCE][size_t(level)];
+                             LevelSet pred_ghosts =
+                                 set_difference(ref, set_union(m.ids[CELLS][size_t(level)], m.ids[PROJ_CELLS][size_t(level)]));
+                             pred[size_t(level)] = set_intersection(set_intersection(pred_ghosts, m.domain[size_t(level)]),
+                                                                    m.ids[REFERENCE][size_t(level - 1)]);
+                         }
+                     });
         // projection (numeric/projection.hpp:15-65): runs at the coarse level
         p.projection.taps.push_back({0, 0, 0, 0, 0, 0, 0});
         add_row_taps(p.projection, 0, 1, 0, 0, 1, dim, 0, 1);
-        for (int level = 1; level <= m.cfg.max_level; ++level)
-        {
-            p.projection.add(set_intersection(on(m.ids[REFERENCE][size_t(level)], level - 1), m.ids[PROJ_CELLS][size_t(level - 1)]));
-        }
-        p.projection.finish();
         // prediction <s,false> (numeric/prediction.hpp:259-357)
         p.prediction.taps.push_back({0, 0, 0, 0, 0, 0, 0});
         add_row_taps(p.prediction, 0, -1, int8_t(-s), -s, s, dim, 0, int8_t(2 * s));
-        for (int level = 1; level <= m.cfg.max_level; ++level)
+        for (int level = 1; level <= maxl; ++level)
         {
-            const LevelSet& ref = m.ids[REFERENCE][size_t(level)];
-            LevelSet pred_ghosts = set_difference(ref, set_union(m.ids[CELLS][size_t(level)], m.ids[PROJ_CELLS][size_t(level)]));
-            LevelSet expr        = set_intersection(set_intersection(pred_ghosts, m.domain[size_t(level)]),
-                                             m.ids[REFERENCE][size_t(level - 1)]);
-            p.prediction.add(expr);
+            p.projection.add(proj[size_t(level)]);
+            p.prediction.add(pred[size_t(level)]);
         }
+        p.projection.finish();
         p.prediction.finish();
         return p;
     }
@@ -335,29 +349,46 @@ namespace smr
         AdaptProgram p;
         const int dim = m.cfg.dim, s = m.cfg.pred_s;
         const int minl = m.cfg.min_level, maxl = m.cfg.max_level;
+        const int l0   = minl > 0 ? minl - 1 : 0;
+        std::vector<LevelSet> det(size_t(maxl) + 1), crit(size_t(maxl) + 1);
+        parallel_for(2 * (maxl + 1),
+                     [&](int task)
+                     {
+                         const int level = task / 2;
+                         if (task % 2 == 0 && level >= l0 && level < maxl)
+                         {
+                             LevelSet fine      = set_union(m.ids[CELLS][size_t(level + 1)], m.ids[CELLS][size_t(level + 2)]);
+                             det[size_t(level)] = set_intersection(m.ids[REFERENCE][size_t(level)], on(fine, level));
+                         }
+                         else if (task % 2 == 1 && level >= 1)
+                         {
+                             crit[size_t(level)] =
+                                 set_intersection(on(m.ids[CELLS][size_t(level)], level - 1), m.ids[REFERENCE][size_t(level - 1)]);
+                         }
+                     });
         // compute_detail (mr/operators.hpp:139-540): fine rows, then coarse stencil rows
         add_row_taps(p.detail, 0, 1, 0, 0, 1, dim, 0, 1);
         add_row_taps(p.detail, 0, 0, int8_t(-s), -s, s, dim, 0, int8_t(2 * s));
-        for (int level = (minl > 0 ? minl - 1 : 0); level < maxl; ++level)
+        for (int level = l0; level < maxl; ++level)
         {
-            LevelSet fine  = set_union(m.ids[CELLS][size_t(level + 1)], m.ids[CELLS][size_t(level + 2)]);
-            LevelSet below = set_intersection(m.ids[REFERENCE][size_t(level)], on(fine, level));
-            p.detail.add(below);
+            p.detail.add(det[size_t(level)]);
         }
         p.detail.finish();
-        // mr_criteria / maximum (mr/criteria.hpp:13-122, mr/operators.hpp:22-96)
+        // mr_criteria / maximum (mr/criteria.hpp:13-122, mr/operators.hpp:22-96): the
+        // criteria sets are those of the fine levels [min, max], the maximum sets
+        // those of [1, max]; both are (cells[l] n reference[l-1]).on(l-1)
         p.criteria.taps.push_back({0, 0, 0, 0, 0, 0, 0});
         add_row_taps(p.criteria, 0, 1, 0, 0, 1, dim, 0, 1);
-        for (int level = minl; level <= maxl; ++level)
-        {
-            p.criteria.add(set_intersection(on(m.ids[CELLS][size_t(level)], level - 1), m.ids[REFERENCE][size_t(level - 1)]));
-        }
-        p.criteria.finish();
         p.maximum.taps = p.criteria.taps;
         for (int level = 1; level <= maxl; ++level)
         {
-            p.maximum.add(set_intersection(on(m.ids[CELLS][size_t(level)], level - 1), m.ids[REFERENCE][size_t(level - 1)]));
+            if (level >= minl)
+            {
+                p.criteria.add(crit[size_t(level)]);
+            }
+            p.maximum.add(crit[size_t(level)]);
         }
+        p.criteria.finish();
         p.maximum.finish();
         return p;
     }
@@ -391,11 +422,31 @@ namespace smr
         copy    = WorkList();
         coarsen = WorkList();
         refine  = WorkList();
+        std::vector<LevelSet> cp(size_t(maxl) + 1), co(size_t(maxl) + 1), re(size_t(maxl) + 1);
+        parallel_for(3 * (maxl + 1),
+                     [&](int task)
+                     {
+                         const int kind = task % 3, level = task / 3;
+                         if (kind == 0 && level >= minl)
+                         {
+                             cp[size_t(level)] = set_intersection(om.ids[REFERENCE][size_t(level)], nm.ids[CELLS][size_t(level)]);
+                         }
+                         else if (kind == 1 && level >= minl + 1)
+                         {
+                             co[size_t(level)] =
+                                 set_intersection(on(om.ids[CELLS][size_t(level)], level - 1), nm.ids[CELLS][size_t(level - 1)]);
+                         }
+                         else if (kind == 2 && level >= minl + 1)
+                         {
+                             re[size_t(level)] =
+                                 set_intersection(on(nm.ids[CELLS][size_t(level)], level - 1), om.ids[CELLS][size_t(level - 1)]);
+                         }
+                     });
         copy.taps.push_back({0, 0, 0, 0, 0, 0, 0});
         copy.taps.push_back({1, 0, 0, 0, 0, 0, 0});
         for (int level = minl; level <= maxl; ++level)
         {
-            copy.add(set_intersection(om.ids[REFERENCE][size_t(level)], nm.ids[CELLS][size_t(level)]));
+            copy.add(cp[size_t(level)]);
         }
         copy.finish();
         coarsen.taps.push_back({0, 0, 0, 0, 0, 0, 0});
@@ -404,8 +455,8 @@ namespace smr
         add_row_taps(refine, 1, 0, int8_t(-s), -s, s, dim, 0, int8_t(2 * s));
         for (int level = minl + 1; level <= maxl; ++level)
         {
-            coarsen.add(set_intersection(on(om.ids[CELLS][size_t(level)], level - 1), nm.ids[CELLS][size_t(level - 1)]));
-            refine.add(set_intersection(on(nm.ids[CELLS][size_t(level)], level - 1), om.ids[CELLS][size_t(level - 1)]));
+            coarsen.add(co[size_t(level)]);
+            refine.add(re[size_t(level)]);
         }
         coarsen.finish();
         refine.finish();

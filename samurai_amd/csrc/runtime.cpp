@@ -1,7 +1,9 @@
 #include "runtime.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 
 namespace smr
@@ -15,14 +17,96 @@ namespace smr
     }
 
     // ------------------------------------------------------------------
+    // timers
+    // ------------------------------------------------------------------
+    namespace
+    {
+        double now_s()
+        {
+            return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+        }
+    }
+
+    Timers& timers()
+    {
+        static Timers t;
+        return t;
+    }
+
+    void Timers::add(const std::string& name, double seconds)
+    {
+        for (auto& e : entries)
+        {
+            if (e.first == name)
+            {
+                e.second.first += seconds;
+                e.second.second += 1;
+                return;
+            }
+        }
+        entries.push_back({name, {seconds, 1}});
+    }
+
+    std::string Timers::report() const
+    {
+        std::string s = "{";
+        for (size_t k = 0; k < entries.size(); ++k)
+        {
+            char buf[256];
+            std::snprintf(buf, sizeof(buf), "%s\"%s\": {\"seconds\": %.6f, \"calls\": %lld}", k ? ", " : "", entries[k].first.c_str(),
+                          entries[k].second.first, static_cast<long long>(entries[k].second.second));
+            s += buf;
+        }
+        return s + "}";
+    }
+
+    ScopedTimer::ScopedTimer(Ctx* ctx, const char* name)
+        : ctx_(ctx)
+        , name_(name)
+    {
+        if (timers().enabled)
+        {
+            if (ctx_ && ctx_->has_device())
+            {
+                ctx_->sync();
+            }
+            t0_ = now_s();
+        }
+    }
+
+    ScopedTimer::~ScopedTimer()
+    {
+        if (timers().enabled)
+        {
+            if (ctx_ && ctx_->has_device())
+            {
+                (void)hipStreamSynchronize(ctx_->stream);
+            }
+            timers().add(name_, now_s() - t0_);
+        }
+    }
+
+    // ------------------------------------------------------------------
     // Blob
     // ------------------------------------------------------------------
+    void* dev_alloc(size_t bytes, hipStream_t s)
+    {
+        void* p = nullptr;
+        hip_check(hipMallocAsync(&p, std::max<size_t>(bytes, 256), s), "hipMallocAsync");
+        return p;
+    }
+
+    void dev_free(void* p, hipStream_t s)
+    {
+        if (p)
+        {
+            (void)hipFreeAsync(p, s);
+        }
+    }
+
     Blob::~Blob()
     {
-        if (d_)
-        {
-            (void)hipFree(d_);
-        }
+        dev_free(d_, stream_);
     }
 
     size_t Blob::reserve(size_t n)
@@ -48,12 +132,11 @@ namespace smr
         {
             throw std::logic_error("blob uploaded twice");
         }
-        hip_check(hipMalloc(reinterpret_cast<void**>(&d_), std::max<size_t>(host_.size(), 256)), "hipMalloc(blob)");
+        stream_ = s;
+        d_      = static_cast<char*>(dev_alloc(host_.size(), s));
         if (!host_.empty())
         {
             hip_check(hipMemcpyAsync(d_, host_.data(), host_.size(), hipMemcpyHostToDevice, s), "hipMemcpyAsync(blob)");
-            // the host staging buffer must outlive the copy
-            hip_check(hipStreamSynchronize(s), "hipStreamSynchronize(blob)");
         }
     }
 
@@ -185,13 +268,27 @@ namespace smr
         {
             taps.insert(taps.end(), {t.table, t.dl, t.dx, t.dy, t.dz, t.wlo, t.whi});
         }
+        // 64-cell wave tiles of every run
+        std::vector<int32_t> tiles;
+        tiles.reserve(size_t(nruns) * 2 + size_t(ncells / 64) * 2);
+        for (int64_t k = 0; k < nruns; ++k)
+        {
+            for (int32_t x = w.iv[size_t(5 * k)]; x < w.iv[size_t(5 * k + 1)]; x += 64)
+            {
+                tiles.push_back(int32_t(k));
+                tiles.push_back(x);
+            }
+        }
         auto b      = std::make_unique<Blob>();
         size_t oiv  = b->add(w.iv.data(), w.iv.size() * 4);
         size_t opf  = b->add(w.pref.data(), w.pref.size() * 8);
         size_t oh   = b->add(l.hint.data(), l.hint.size() * 4);
         size_t ot   = b->add(taps.data(), taps.size());
+        size_t otl  = b->add(tiles.data(), tiles.size() * 4);
         size_t obas = b->reserve(size_t(nruns) * w.taps.size() * 8);
         b->upload(ctx.stream);
+        l.d.tiles  = reinterpret_cast<const int32_t*>(b->dptr(otl));
+        l.d.ntiles = int64_t(tiles.size() / 2);
         l.d.iv     = reinterpret_cast<const int32_t*>(b->dptr(oiv));
         l.d.pref   = reinterpret_cast<const int64_t*>(b->dptr(opf));
         l.d.hint   = reinterpret_cast<const int32_t*>(b->dptr(oh));
@@ -229,6 +326,7 @@ namespace smr
         DevMesh& dm = device();
         if (!dm.has_ghost)
         {
+            ScopedTimer timer(ctx, "ghost program build");
             GhostProgram gp = build_ghost_program(host);
             dm.outer.resize(gp.outer.size());
             for (size_t l = 0; l < gp.outer.size(); ++l)
@@ -251,6 +349,7 @@ namespace smr
         DevMesh& dm = device_ghost();
         if (!dm.has_adapt)
         {
+            ScopedTimer timer(ctx, "adapt program build");
             AdaptProgram ap  = build_adapt_program(host);
             dm.detail.host   = std::move(ap.detail);
             dm.criteria.host = std::move(ap.criteria);
@@ -278,7 +377,7 @@ namespace smr
     {
         if (d)
         {
-            (void)hipFree(d);
+            dev_free(d, mesh->ctx->stream);
         }
     }
 
@@ -287,11 +386,11 @@ namespace smr
         mesh->ctx->require_device();
         if (d)
         {
-            hip_check(hipFree(d), "hipFree(field)");
+            dev_free(d, mesh->ctx->stream);
             d = nullptr;
         }
         size = mesh->host.nref;
-        hip_check(hipMalloc(reinterpret_cast<void**>(&d), size_t(std::max<int64_t>(size, 1)) * sizeof(double)), "hipMalloc(field)");
+        d    = static_cast<double*>(dev_alloc(size_t(std::max<int64_t>(size, 1)) * sizeof(double), mesh->ctx->stream));
         if (zero)
         {
             hip_check(hipMemsetAsync(d, 0, size_t(size) * sizeof(double), mesh->ctx->stream), "hipMemsetAsync(field)");
@@ -317,6 +416,23 @@ namespace smr
         {
             (void)hipFree(tag);
         }
+        if (host_tag)
+        {
+            (void)hipHostFree(host_tag);
+        }
+    }
+
+    void AdaptState::copy_last(uint8_t* tags, double* details, hipStream_t s) const
+    {
+        if (tags && last_size > 0)
+        {
+            std::memcpy(tags, host_tag, size_t(last_size));
+        }
+        if (details && last_size > 0)
+        {
+            hip_check(hipMemcpyAsync(details, detail, size_t(last_size) * 8, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(detail)");
+            hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+        }
     }
 
     // ------------------------------------------------------------------
@@ -327,6 +443,7 @@ namespace smr
         f.check_current();
         Mesh& m         = *f.mesh;
         DevMesh& dm     = m.device_ghost();
+        ScopedTimer timer(m.ctx, "ghost update");
         hipStream_t s   = m.ctx->stream;
         const int dim   = m.host.cfg.dim;
         const int maxl  = m.host.cfg.max_level;
@@ -364,6 +481,7 @@ namespace smr
         }
         Mesh& m     = *u.mesh;
         DevMesh& dm = m.device();
+        ScopedTimer timer(m.ctx, "field expressions");
         const int dim = m.host.cfg.dim;
         dev::upwind(dim, dm.cells.d, 0, dm.cells.d.ncells, u.d, unp1.d, a[0], a[1], dim == 3 ? a[2] : 0.0, dt, m.ctx->stream);
     }
@@ -396,6 +514,7 @@ namespace smr
         {
             Mesh& m         = *f.mesh;
             Ctx& ctx        = *m.ctx;
+            ScopedTimer timer(&ctx, "fields update");
             hipStream_t s   = ctx.stream;
             DevMesh& odev   = *m.dev;
             DevWorkList copy, coarsen, refine;
@@ -404,14 +523,13 @@ namespace smr
             upload_list(ctx, coarsen, ndev.table, odev.table);
             upload_list(ctx, refine, ndev.table, odev.table);
             ctx.check_resolve();
-            double* nd = nullptr;
-            hip_check(hipMalloc(reinterpret_cast<void**>(&nd), size_t(std::max<int64_t>(nm.nref, 1)) * 8), "hipMalloc(new field)");
+            double* nd = static_cast<double*>(dev_alloc(size_t(std::max<int64_t>(nm.nref, 1)) * 8, s));
             hip_check(hipMemsetAsync(nd, 0, size_t(nm.nref) * 8, s), "hipMemsetAsync(new field)");
             dev::copy_cells(copy.d, 0, copy.d.ncells, nd, f.d, s);
             dev::project(nm.cfg.dim, coarsen.d, 0, coarsen.d.ncells, nd, f.d, s);
             dev::predict_children(nm.cfg.dim, nm.cfg.pred_s, refine.d, 0, refine.d.ncells, nd, f.d, s);
-            ctx.sync(); // the lists die at scope exit
-            hip_check(hipFree(f.d), "hipFree(old field)");
+            // the lists and the old storage are released in stream order after these kernels
+            dev_free(f.d, s);
             f.d    = nd;
             f.size = nm.nref;
         }
@@ -437,12 +555,14 @@ namespace smr
 
         // detail computation: levels [min-1, max-ite)
         {
+            ScopedTimer timer(&ctx, "detail computation");
             int64_t c0, c1;
             dm.detail.cells_of_levels(minl > 0 ? minl - 1 : 0, maxl - ite, c0, c1);
             dev::detail(dim, cfg.pred_s, dm.detail.d, c0, c1, f.d, st.detail, s);
         }
         // tag cells: fine levels [min, max-ite], i.e. coarse levels [min-1, max-ite-1]
         {
+            ScopedTimer timer(&ctx, "tag cells");
             dev::Thresholds thr{};
             for (int level = minl; level <= maxl - ite; ++level)
             {
@@ -459,28 +579,54 @@ namespace smr
             dev::criteria(dim, dm.criteria.d, c0, c1, st.detail, st.tag, thr, minl, maxl, s);
         }
         // tag finalization: levels max..1, sequential
-        for (int level = maxl; level > 0; --level)
         {
-            int64_t c0, c1;
-            dm.maximum.cells_of_levels(level - 1, level, c0, c1);
-            dev::maximum(dim, dm.maximum.d, c0, c1, st.tag, s);
+            ScopedTimer timer(&ctx, "tag finalization");
+            for (int level = maxl; level > 0; --level)
+            {
+                int64_t c0, c1;
+                dm.maximum.cells_of_levels(level - 1, level, c0, c1);
+                dev::maximum(dim, dm.maximum.d, c0, c1, st.tag, s);
+            }
         }
-        st.last_tag.resize(size_t(n));
-        st.last_detail.resize(size_t(n));
-        hip_check(hipMemcpyAsync(st.last_tag.data(), st.tag, size_t(n), hipMemcpyDeviceToHost, s), "hipMemcpyAsync(tag)");
-        hip_check(hipMemcpyAsync(st.last_detail.data(), st.detail, size_t(n) * 8, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(detail)");
-        ctx.sync();
-        st.last_size = n;
+        std::unique_ptr<DevMesh> ndev;
+        MRMeshData nm;
+        {
+            ScopedTimer timer(&ctx, "mesh update");
+            if (n > st.host_tag_capacity)
+            {
+                if (st.host_tag)
+                {
+                    hip_check(hipHostFree(st.host_tag), "hipHostFree(tag)");
+                    st.host_tag = nullptr;
+                }
+                hip_check(hipHostMalloc(reinterpret_cast<void**>(&st.host_tag), size_t(n), hipHostMallocDefault), "hipHostMalloc(tag)");
+                st.host_tag_capacity = n;
+            }
+            hip_check(hipMemcpyAsync(st.host_tag, st.tag, size_t(n), hipMemcpyDeviceToHost, s), "hipMemcpyAsync(tag)");
+            ctx.sync();
+            st.last_size = n;
 
-        // mesh update on the host (graduation.hpp:743-843, 573-726)
-        CellArray new_ca = update_cell_array_from_tag(m.host, st.last_tag.data());
-        make_graduation(new_ca, dim, cfg.grad_width);
-        if (ca_equal(m.host.ids[CELLS], new_ca))
-        {
-            return true;
+            // mesh update on the host (graduation.hpp:743-843, 573-726)
+            CellArray new_ca;
+            {
+                ScopedTimer t1(nullptr, "update_cell_array_from_tag");
+                new_ca = update_cell_array_from_tag(m.host, st.host_tag);
+            }
+            {
+                ScopedTimer t2(nullptr, "make_graduation");
+                make_graduation(new_ca, dim, cfg.grad_width);
+            }
+            if (ca_equal(m.host.ids[CELLS], new_ca))
+            {
+                return true;
+            }
+            ScopedTimer t3(nullptr, "mesh construction");
+            nm = build_mesh(cfg, new_ca);
         }
-        MRMeshData nm = build_mesh(cfg, new_ca);
-        std::unique_ptr<DevMesh> ndev = make_dev_mesh(ctx, nm);
+        {
+            ScopedTimer timer(&ctx, "device upload");
+            ndev = make_dev_mesh(ctx, nm);
+        }
         update_fields(f, nm, *ndev);
         m.replace(std::move(nm), std::move(ndev));
         f.version = m.version;
@@ -495,6 +641,7 @@ namespace smr
         {
             return 0;
         }
+        ScopedTimer timer(m.ctx, "mesh adaptation");
         int limit = maxl - minl;
         if (max_iterations >= 0)
         {

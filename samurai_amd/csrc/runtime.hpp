@@ -29,14 +29,22 @@ namespace smr
         // reserves space for n bytes copied from src (8-byte aligned); returns the offset
         size_t add(const void* src, size_t n);
         size_t reserve(size_t n);
+        // stream-ordered allocation + copy; the host staging copy is kept until
+        // the blob dies, so no host synchronization is needed here
         void upload(hipStream_t s);
         char* dptr(size_t off) const { return d_ + off; }
         size_t bytes() const { return host_.size(); }
 
       private:
         std::vector<char> host_;
-        char* d_ = nullptr;
+        char* d_           = nullptr;
+        hipStream_t stream_ = nullptr;
     };
+
+    // stream-ordered device memory from the device's default pool, which is
+    // told to keep freed memory cached (no trip to the driver per mesh version)
+    void* dev_alloc(size_t bytes, hipStream_t s);
+    void dev_free(void* p, hipStream_t s);
 
     struct DevWorkList
     {
@@ -103,11 +111,13 @@ namespace smr
         double* detail = nullptr;
         uint8_t* tag   = nullptr;
         int64_t capacity = 0;
-        // tags/details of the last harten call on the mesh they belong to
-        std::vector<uint8_t> last_tag;
-        std::vector<double> last_detail;
-        int64_t last_size = 0;
+        // pinned copy of the tags of the last harten call (on the mesh they
+        // were computed on); the details of that call stay in `detail`
+        uint8_t* host_tag         = nullptr;
+        int64_t host_tag_capacity = 0;
+        int64_t last_size         = 0;
         ~AdaptState();
+        void copy_last(uint8_t* tags, double* details, hipStream_t s) const;
     };
 
     struct Ctx
@@ -120,6 +130,31 @@ namespace smr
         void require_device() const;
         void sync();
         void check_resolve();
+    };
+
+    // Phase timers named after the reference's ScopedTimer names (timers.hpp,
+    // mr/adapt.hpp:151-384, update_ghost_mr.hpp:263, field_base.hpp:234).  When
+    // enabled, each phase synchronizes the stream on entry and exit so that its
+    // wall time includes the device work it launched.
+    struct Timers
+    {
+        bool enabled = false;
+        std::vector<std::pair<std::string, std::pair<double, int64_t>>> entries;
+        void add(const std::string& name, double seconds);
+        std::string report() const;
+    };
+    Timers& timers();
+
+    class ScopedTimer
+    {
+      public:
+        ScopedTimer(Ctx* ctx, const char* name);
+        ~ScopedTimer();
+
+      private:
+        Ctx* ctx_;
+        const char* name_;
+        double t0_ = 0;
     };
 
     std::unique_ptr<DevMesh> make_dev_mesh(Ctx& ctx, const MRMeshData& m);
