@@ -5,6 +5,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <cstdint>
+#include <vector>
 
 namespace smr::dev
 {
@@ -43,6 +44,68 @@ namespace smr::dev
     };
 
     constexpr int HINT_SHIFT = 8; // 256 cells per hint chunk
+
+    // One operator applied to a cell range of one instance: the unit of the
+    // fused stage program that advances many independent meshes together.
+    enum SegmentOp : int32_t
+    {
+        SEG_RECIPE        = 0,
+        SEG_PROJECT       = 1,
+        SEG_PREDICT_FINE  = 2,
+        SEG_DETAIL        = 3,
+        SEG_CRITERIA      = 4,
+        SEG_MAXIMUM       = 5,
+        SEG_TAG_INIT      = 6,
+        SEG_UPWIND        = 7,
+        // update_fields: `out` is the new storage, `u` the old one
+        SEG_COPY             = 8,
+        SEG_PROJECT_TO       = 9,
+        SEG_PREDICT_CHILDREN = 10,
+        // storage bases of a list's (run, tap) pairs against tabs[0], tabs[1];
+        // the "cells" of the segment are run * ntaps + tap
+        SEG_RESOLVE = 11,
+        // zero fills of out[c] / tag[c]
+        SEG_FILL_F64 = 12,
+        SEG_FILL_U8  = 13
+    };
+
+    struct Segment
+    {
+        int32_t op, dim, pred_s, has_bc;
+        int32_t min_level, max_level, pad0, pad1;
+        int64_t c0, c1;  // cell (or recipe) range of the list
+        List list;       // work list (list ops)
+        const int64_t* rdst;  // recipes
+        const int32_t* rop;
+        const int64_t* rsp;
+        const int64_t* rsrc;
+        double* u;       // field storage
+        double* out;     // unp1 (upwind) or details
+        uint8_t* tag;
+        const double* thr; // eps/fine/coarse per fine level (criteria)
+        const Table* tabs; // resolve: the two tables of the list's taps
+        int* err;          // resolve: set to 1 when a pair fails
+        double a0, a1, a2, dt, bc;
+    };
+
+    // Runs stages [0, nstages): stage k is the segments [stage_seg[k],
+    // stage_seg[k+1]); seg_pref[i] is the first global cell of segment i within
+    // its stage.  Stages are separated by grid barriers (cooperative launch)
+    // or, in stage-wise mode, by kernel boundaries.
+    struct Program
+    {
+        const Segment* segs;
+        const int64_t* seg_pref;
+        const int32_t* stage_seg;
+        int nstages;
+    };
+
+    // maximum co-resident blocks of the fused kernel for a block size
+    int program_max_blocks(int block);
+    // one cooperative launch; returns false if the runtime refused it
+    bool run_program_fused(const Program& p, int block, int nblocks, int cpt, hipStream_t s);
+    // one launch per stage (capturable in a hipGraph); cells per stage given by the host
+    void run_program_stagewise(const Program& p, const std::vector<int64_t>& stage_cells, int block, int cpt, hipStream_t s);
 
     void resolve(const List& l, const Table& t0, const Table& t1, int* err, hipStream_t s);
     // launch shape of the upwind kernel (cells per thread, XCD remap, stores)

@@ -1,0 +1,204 @@
+// Fused stage program: many independent MR instances advanced together.
+//
+// A stage is a set of segments (operator, instance, cell range) with no
+// dependencies between them; consecutive stages are ordered.  The fused
+// kernel is one cooperative launch for the whole program, with a grid barrier
+// between stages; the stage-wise form launches one kernel per stage (for hipGraph
+// capture).  Each thread walks the flattened cells of the stage and runs the
+// same per-cell operator bodies as the standalone kernels (cell_ops.hpp).
+#include "cell_ops.hpp"
+
+#include <hip/hip_cooperative_groups.h>
+
+namespace smr::dev
+{
+    namespace
+    {
+        template <int DIM>
+        __device__ __forceinline__ void run_cell_dim(const Segment& g, int64_t c)
+        {
+            switch (g.op)
+            {
+                case SEG_PROJECT: op_project<DIM>(g.list, c, g.u, g.u); break;
+                case SEG_PREDICT_FINE:
+                    if (g.pred_s == 0)
+                    {
+                        op_predict_fine<DIM, 0>(g.list, c, g.u);
+                    }
+                    else
+                    {
+                        op_predict_fine<DIM, 1>(g.list, c, g.u);
+                    }
+                    break;
+                case SEG_DETAIL:
+                    if (g.pred_s == 0)
+                    {
+                        op_detail<DIM, 0>(g.list, c, g.u, g.out);
+                    }
+                    else
+                    {
+                        op_detail<DIM, 1>(g.list, c, g.u, g.out);
+                    }
+                    break;
+                case SEG_CRITERIA: op_criteria<DIM>(g.list, c, g.out, g.tag, g.thr, g.min_level, g.max_level); break;
+                case SEG_MAXIMUM: op_maximum<DIM>(g.list, c, g.tag); break;
+                case SEG_TAG_INIT: op_tag_init(g.list, c, g.tag); break;
+                case SEG_COPY: op_copy(g.list, c, g.out, g.u); break;
+                case SEG_PROJECT_TO: op_project<DIM>(g.list, c, g.out, g.u); break;
+                case SEG_PREDICT_CHILDREN:
+                    if (g.pred_s == 0)
+                    {
+                        op_predict_children<DIM, 0>(g.list, c, g.out, g.u);
+                    }
+                    else
+                    {
+                        op_predict_children<DIM, 1>(g.list, c, g.out, g.u);
+                    }
+                    break;
+                case SEG_UPWIND:
+                {
+                    const Cell cell  = locate(g.list, c);
+                    const int64_t* b = bases(g.list, cell.run);
+                    g.out[b[0] + cell.x] = upwind_value<DIM>(b, cell.x, cell.level, g.u, .5 * g.a0, .5 * fabs(g.a0), .5 * g.a1,
+                                                             .5 * fabs(g.a1), .5 * g.a2, .5 * fabs(g.a2), g.dt);
+                    break;
+                }
+                default: break;
+            }
+        }
+
+        __device__ __forceinline__ void run_cell(const Segment& g, int64_t c)
+        {
+            if (g.op == SEG_RECIPE)
+            {
+                op_recipe(g.rdst, g.rop, g.rsp, g.rsrc, c, g.u, g.has_bc, g.bc);
+            }
+            else if (g.op == SEG_RESOLVE)
+            {
+                op_resolve(g.list, c, g.tabs[0], g.tabs[1], g.err);
+            }
+            else if (g.op == SEG_FILL_F64)
+            {
+                g.out[c] = 0.0;
+            }
+            else if (g.op == SEG_FILL_U8)
+            {
+                g.tag[c] = 0;
+            }
+            else if (g.dim == 2)
+            {
+                run_cell_dim<2>(g, c);
+            }
+            else
+            {
+                run_cell_dim<3>(g, c);
+            }
+        }
+
+        // the cells of one stage, grid-strided; CPT consecutive cells per thread
+        __device__ __forceinline__ void run_stage(const Program& p, int k, int64_t first_thread, int64_t nthreads, int cpt)
+        {
+            const int32_t sb = p.stage_seg[k], se = p.stage_seg[k + 1];
+            if (se <= sb)
+            {
+                return;
+            }
+            const int64_t base  = p.seg_pref[sb];
+            const int64_t total = p.seg_pref[se] - base;
+            for (int64_t i0 = first_thread * cpt; i0 < total; i0 += nthreads * cpt)
+            {
+                // segment of the first cell, then walk forward
+                int32_t lo = sb, hi = se - 1;
+                while (lo < hi)
+                {
+                    const int32_t mid = (lo + hi + 1) >> 1;
+                    if (p.seg_pref[mid] - base <= i0)
+                    {
+                        lo = mid;
+                    }
+                    else
+                    {
+                        hi = mid - 1;
+                    }
+                }
+                int32_t seg = lo;
+                for (int j = 0; j < cpt; ++j)
+                {
+                    const int64_t i = i0 + j;
+                    if (i >= total)
+                    {
+                        break;
+                    }
+                    while (seg + 1 < se && p.seg_pref[seg + 1] - base <= i)
+                    {
+                        ++seg;
+                    }
+                    const Segment& g = p.segs[seg];
+                    run_cell(g, g.c0 + (i - (p.seg_pref[seg] - base)));
+                }
+            }
+        }
+
+        __global__ void k_program_fused(Program p, int cpt)
+        {
+            cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+            const int64_t tid                  = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+            const int64_t nthreads             = int64_t(gridDim.x) * blockDim.x;
+            for (int k = 0; k < p.nstages; ++k)
+            {
+                run_stage(p, k, tid, nthreads, cpt);
+                grid.sync();
+            }
+        }
+
+        __global__ void k_program_stage(Program p, int k, int cpt)
+        {
+            const int64_t tid      = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+            const int64_t nthreads = int64_t(gridDim.x) * blockDim.x;
+            run_stage(p, k, tid, nthreads, cpt);
+        }
+    } // namespace
+
+    int program_max_blocks(int block)
+    {
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_program_fused, block, 0) != hipSuccess || per_cu <= 0)
+        {
+            return 0;
+        }
+        int dev = 0, ncu = 0;
+        (void)hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        {
+            return 0;
+        }
+        return per_cu * ncu;
+    }
+
+    bool run_program_fused(const Program& p, int block, int nblocks, int cpt, hipStream_t s)
+    {
+        if (p.nstages <= 0)
+        {
+            return true;
+        }
+        Program prog = p;
+        void* args[] = {&prog, &cpt};
+        return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_program_fused), dim3(nblocks), dim3(block), args, 0, s) ==
+               hipSuccess;
+    }
+
+    void run_program_stagewise(const Program& p, const std::vector<int64_t>& stage_cells, int block, int cpt, hipStream_t s)
+    {
+        for (int k = 0; k < p.nstages; ++k)
+        {
+            const int64_t n = stage_cells[size_t(k)];
+            if (n <= 0)
+            {
+                continue;
+            }
+            const int64_t threads = (n + cpt - 1) / cpt;
+            const unsigned grid   = unsigned((threads + block - 1) / block);
+            hipLaunchKernelGGL(k_program_stage, dim3(grid), dim3(block), 0, s, p, k, cpt);
+        }
+    }
+} // namespace smr::dev

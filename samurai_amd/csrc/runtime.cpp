@@ -132,11 +132,21 @@ namespace smr
         {
             throw std::logic_error("blob uploaded twice");
         }
+        alloc(s);
+        copy();
+    }
+
+    void Blob::alloc(hipStream_t s)
+    {
         stream_ = s;
         d_      = static_cast<char*>(dev_alloc(host_.size(), s));
+    }
+
+    void Blob::copy()
+    {
         if (!host_.empty())
         {
-            hip_check(hipMemcpyAsync(d_, host_.data(), host_.size(), hipMemcpyHostToDevice, s), "hipMemcpyAsync(blob)");
+            hip_check(hipMemcpyAsync(d_, host_.data(), host_.size(), hipMemcpyHostToDevice, stream_), "hipMemcpyAsync(blob)");
         }
     }
 
@@ -183,69 +193,65 @@ namespace smr
     // ------------------------------------------------------------------
     // device images
     // ------------------------------------------------------------------
-    namespace
+    // ------------------------------------------------------------------
+    // Packer: several arrays of one device image in one allocation
+    // ------------------------------------------------------------------
+    void Packer::table(const MRMeshData& m, dev::Table& t)
     {
-        void upload_table(Ctx& ctx, const MRMeshData& m, DevMesh& dm)
+        std::vector<int64_t> key, row_off(MAXL + 1, 0), rp, idx;
+        std::vector<int32_t> rs, re;
+        rp.push_back(0);
+        for (int l = 0; l < MAXL; ++l)
         {
-            std::vector<int64_t> key, row_off(MAXL + 1, 0), rp, idx;
-            std::vector<int32_t> rs, re;
-            rp.push_back(0);
-            for (int l = 0; l < MAXL; ++l)
+            row_off[size_t(l)] = int64_t(key.size());
+            const LevelSet& r  = m.ids[REFERENCE][size_t(l)];
+            for (size_t row = 0; row < r.nrows(); ++row)
             {
-                row_off[size_t(l)] = int64_t(key.size());
-                const LevelSet& r  = m.ids[REFERENCE][size_t(l)];
-                for (size_t row = 0; row < r.nrows(); ++row)
+                key.push_back((int64_t(r.rz[row]) << 32) + (int64_t(r.ry[row]) + 2147483648LL));
+                for (int64_t k = r.rp[row]; k < r.rp[row + 1]; ++k)
                 {
-                    key.push_back((int64_t(r.rz[row]) << 32) + (int64_t(r.ry[row]) + 2147483648LL));
-                    for (int64_t k = r.rp[row]; k < r.rp[row + 1]; ++k)
-                    {
-                        rs.push_back(r.runs[size_t(k)].s);
-                        re.push_back(r.runs[size_t(k)].e);
-                        idx.push_back(m.ridx[size_t(l)][size_t(k)]);
-                    }
-                    rp.push_back(int64_t(rs.size()));
+                    rs.push_back(r.runs[size_t(k)].s);
+                    re.push_back(r.runs[size_t(k)].e);
+                    idx.push_back(m.ridx[size_t(l)][size_t(k)]);
                 }
+                rp.push_back(int64_t(rs.size()));
             }
-            row_off[MAXL] = int64_t(key.size());
-            auto b        = std::make_unique<Blob>();
-            size_t o_key = b->add(key.data(), key.size() * 8), o_off = b->add(row_off.data(), row_off.size() * 8);
-            size_t o_rp = b->add(rp.data(), rp.size() * 8), o_rs = b->add(rs.data(), rs.size() * 4);
-            size_t o_re = b->add(re.data(), re.size() * 4), o_idx = b->add(idx.data(), idx.size() * 8);
-            b->upload(ctx.stream);
-            dm.table.row_key = reinterpret_cast<const int64_t*>(b->dptr(o_key));
-            dm.table.row_off = reinterpret_cast<const int64_t*>(b->dptr(o_off));
-            dm.table.rp      = reinterpret_cast<const int64_t*>(b->dptr(o_rp));
-            dm.table.run_s   = reinterpret_cast<const int32_t*>(b->dptr(o_rs));
-            dm.table.run_e   = reinterpret_cast<const int32_t*>(b->dptr(o_re));
-            dm.table.run_idx = reinterpret_cast<const int64_t*>(b->dptr(o_idx));
-            dm.table_blob    = std::move(b);
         }
-
-        void upload_recipes(Ctx& ctx, DevRecipes& r)
-        {
-            auto b     = std::make_unique<Blob>();
-            size_t od  = b->add(r.host.dst.data(), r.host.dst.size() * 8);
-            size_t oo  = b->add(r.host.op.data(), r.host.op.size() * 4);
-            size_t osp = b->add(r.host.src_ptr.data(), r.host.src_ptr.size() * 8);
-            size_t os  = b->add(r.host.src.data(), r.host.src.size() * 8);
-            b->upload(ctx.stream);
-            r.dst  = reinterpret_cast<const int64_t*>(b->dptr(od));
-            r.op   = reinterpret_cast<const int32_t*>(b->dptr(oo));
-            r.sp   = reinterpret_cast<const int64_t*>(b->dptr(osp));
-            r.src  = reinterpret_cast<const int64_t*>(b->dptr(os));
-            r.blob = std::move(b);
-        }
-    } // namespace
-
-    void DevWorkList::cells_of_levels(int l0, int l1, int64_t& c0, int64_t& c1) const
-    {
-        int64_t r0    = 0;
-        int64_t nr    = host.runs_of_levels(l0, l1, r0);
-        c0            = host.pref[size_t(r0)];
-        c1            = host.pref[size_t(r0 + nr)];
+        row_off[MAXL] = int64_t(key.size());
+        const size_t o_key = b_.add(key.data(), key.size() * 8), o_off = b_.add(row_off.data(), row_off.size() * 8);
+        const size_t o_rp = b_.add(rp.data(), rp.size() * 8), o_rs = b_.add(rs.data(), rs.size() * 4);
+        const size_t o_re = b_.add(re.data(), re.size() * 4), o_idx = b_.add(idx.data(), idx.size() * 8);
+        Blob* b = &b_;
+        binds_.push_back(
+            [=, &t]
+            {
+                t.row_key = reinterpret_cast<const int64_t*>(b->dptr(o_key));
+                t.row_off = reinterpret_cast<const int64_t*>(b->dptr(o_off));
+                t.rp      = reinterpret_cast<const int64_t*>(b->dptr(o_rp));
+                t.run_s   = reinterpret_cast<const int32_t*>(b->dptr(o_rs));
+                t.run_e   = reinterpret_cast<const int32_t*>(b->dptr(o_re));
+                t.run_idx = reinterpret_cast<const int64_t*>(b->dptr(o_idx));
+            });
     }
 
-    void upload_list(Ctx& ctx, DevWorkList& l, const dev::Table& t0, const dev::Table& t1)
+    void Packer::recipes(DevRecipes& r)
+    {
+        const size_t od  = b_.add(r.host.dst.data(), r.host.dst.size() * 8);
+        const size_t oo  = b_.add(r.host.op.data(), r.host.op.size() * 4);
+        const size_t osp = b_.add(r.host.src_ptr.data(), r.host.src_ptr.size() * 8);
+        const size_t os  = b_.add(r.host.src.data(), r.host.src.size() * 8);
+        Blob* b          = &b_;
+        binds_.push_back(
+            [=, &r]
+            {
+                r.dst = reinterpret_cast<const int64_t*>(b->dptr(od));
+                r.op  = reinterpret_cast<const int32_t*>(b->dptr(oo));
+                r.sp  = reinterpret_cast<const int64_t*>(b->dptr(osp));
+                r.src = reinterpret_cast<const int64_t*>(b->dptr(os));
+            });
+    }
+
+    void Packer::list(DevWorkList& l)
     {
         const WorkList& w     = l.host;
         const int64_t nruns   = w.nruns();
@@ -279,36 +285,136 @@ namespace smr
                 tiles.push_back(x);
             }
         }
-        auto b      = std::make_unique<Blob>();
-        size_t oiv  = b->add(w.iv.data(), w.iv.size() * 4);
-        size_t opf  = b->add(w.pref.data(), w.pref.size() * 8);
-        size_t oh   = b->add(l.hint.data(), l.hint.size() * 4);
-        size_t ot   = b->add(taps.data(), taps.size());
-        size_t otl  = b->add(tiles.data(), tiles.size() * 4);
-        size_t obas = b->reserve(size_t(nruns) * w.taps.size() * 8);
-        b->upload(ctx.stream);
-        l.d.tiles  = reinterpret_cast<const int32_t*>(b->dptr(otl));
-        l.d.ntiles = int64_t(tiles.size() / 2);
-        l.d.iv     = reinterpret_cast<const int32_t*>(b->dptr(oiv));
-        l.d.pref   = reinterpret_cast<const int64_t*>(b->dptr(opf));
-        l.d.hint   = reinterpret_cast<const int32_t*>(b->dptr(oh));
-        l.d.taps   = reinterpret_cast<const int8_t*>(b->dptr(ot));
-        l.d.base   = reinterpret_cast<int64_t*>(b->dptr(obas));
-        l.d.nruns  = nruns;
-        l.d.ncells = ncells;
-        l.d.ntaps  = int(w.taps.size());
-        l.blob     = std::move(b);
+        const size_t oiv  = b_.add(w.iv.data(), w.iv.size() * 4);
+        const size_t opf  = b_.add(w.pref.data(), w.pref.size() * 8);
+        const size_t oh   = b_.add(l.hint.data(), l.hint.size() * 4);
+        const size_t ot   = b_.add(taps.data(), taps.size());
+        const size_t otl  = b_.add(tiles.data(), tiles.size() * 4);
+        const size_t obas = b_.reserve(size_t(nruns) * w.taps.size() * 8);
+        const int64_t ntiles = int64_t(tiles.size() / 2);
+        const int ntaps      = int(w.taps.size());
+        Blob* b              = &b_;
+        binds_.push_back(
+            [=, &l]
+            {
+                l.d.iv     = reinterpret_cast<const int32_t*>(b->dptr(oiv));
+                l.d.pref   = reinterpret_cast<const int64_t*>(b->dptr(opf));
+                l.d.hint   = reinterpret_cast<const int32_t*>(b->dptr(oh));
+                l.d.taps   = reinterpret_cast<const int8_t*>(b->dptr(ot));
+                l.d.tiles  = reinterpret_cast<const int32_t*>(b->dptr(otl));
+                l.d.base   = reinterpret_cast<int64_t*>(b->dptr(obas));
+                l.d.ntiles = ntiles;
+                l.d.nruns  = nruns;
+                l.d.ncells = ncells;
+                l.d.ntaps  = ntaps;
+            });
+    }
+
+    void Packer::finish(hipStream_t s)
+    {
+        b_.alloc(s);
+        for (auto& f : binds_)
+        {
+            f();
+        }
+        binds_.clear();
+        b_.copy();
+    }
+
+    namespace
+    {
+        void upload_recipes(Ctx& ctx, DevRecipes& r)
+        {
+            r.blob = std::make_unique<Blob>();
+            Packer p(*r.blob);
+            p.recipes(r);
+            p.finish(ctx.stream);
+        }
+    } // namespace
+
+    void DevWorkList::cells_of_levels(int l0, int l1, int64_t& c0, int64_t& c1) const
+    {
+        int64_t r0    = 0;
+        int64_t nr    = host.runs_of_levels(l0, l1, r0);
+        c0            = host.pref[size_t(r0)];
+        c1            = host.pref[size_t(r0 + nr)];
+    }
+
+    void upload_list(Ctx& ctx, DevWorkList& l, const dev::Table& t0, const dev::Table& t1)
+    {
+        l.blob = std::make_unique<Blob>();
+        Packer p(*l.blob);
+        p.list(l);
+        p.finish(ctx.stream);
         dev::resolve(l.d, t0, t1, ctx.d_err, ctx.stream);
     }
 
     std::unique_ptr<DevMesh> make_dev_mesh(Ctx& ctx, const MRMeshData& m)
     {
         ctx.require_device();
-        auto dm = std::make_unique<DevMesh>();
-        upload_table(ctx, m, *dm);
+        auto dm        = std::make_unique<DevMesh>();
+        dm->table_blob = std::make_unique<Blob>();
+        Packer p(*dm->table_blob);
+        p.table(m, dm->table);
         dm->cells.host = build_cells_list(m);
-        upload_list(ctx, dm->cells, dm->table, dm->table);
+        p.list(dm->cells);
+        p.finish(ctx.stream);
+        dev::resolve(dm->cells.d, dm->table, dm->table, ctx.d_err, ctx.stream);
         ctx.check_resolve();
+        return dm;
+    }
+
+    HostImage build_host_image(const MRMeshData& m)
+    {
+        HostImage h;
+        h.cells = build_cells_list(m);
+        h.ghost = build_ghost_program(m);
+        h.adapt = build_adapt_program(m);
+        return h;
+    }
+
+    void pack_image(Packer& p, DevMesh& dm, const MRMeshData& m, HostImage&& h)
+    {
+        p.table(m, dm.table);
+        dm.cells.host = std::move(h.cells);
+        p.list(dm.cells);
+        dm.outer.resize(h.ghost.outer.size());
+        for (size_t l = 0; l < h.ghost.outer.size(); ++l)
+        {
+            dm.outer[l].host = std::move(h.ghost.outer[l]);
+            p.recipes(dm.outer[l]);
+        }
+        dm.projection.host = std::move(h.ghost.projection);
+        dm.prediction.host = std::move(h.ghost.prediction);
+        dm.detail.host     = std::move(h.adapt.detail);
+        dm.criteria.host   = std::move(h.adapt.criteria);
+        dm.maximum.host    = std::move(h.adapt.maximum);
+        for (DevWorkList* l : dm.lists())
+        {
+            if (l != &dm.cells)
+            {
+                p.list(*l);
+            }
+        }
+        dm.has_ghost = dm.has_adapt = true;
+    }
+
+    std::unique_ptr<DevMesh> upload_image(Ctx& ctx, const MRMeshData& m, HostImage&& h, bool check)
+    {
+        ctx.require_device();
+        auto dm        = std::make_unique<DevMesh>();
+        dm->table_blob = std::make_unique<Blob>();
+        Packer p(*dm->table_blob);
+        pack_image(p, *dm, m, std::move(h));
+        p.finish(ctx.stream);
+        for (DevWorkList* l : dm->lists())
+        {
+            dev::resolve(l->d, dm->table, dm->table, ctx.d_err, ctx.stream);
+        }
+        if (check)
+        {
+            ctx.check_resolve();
+        }
         return dm;
     }
 

@@ -6,6 +6,7 @@
 #include "device/device_api.hpp"
 #include "host/programs.hpp"
 
+#include <functional>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -32,7 +33,11 @@ namespace smr
         // stream-ordered allocation + copy; the host staging copy is kept until
         // the blob dies, so no host synchronization is needed here
         void upload(hipStream_t s);
+        // upload in two steps, to patch device pointers into the host copy between them
+        void alloc(hipStream_t s);
+        void copy();
         char* dptr(size_t off) const { return d_ + off; }
+        char* hptr(size_t off) { return host_.data() + off; }
         size_t bytes() const { return host_.size(); }
 
       private:
@@ -76,6 +81,24 @@ namespace smr
         std::vector<DevRecipes> outer;
         DevWorkList projection, prediction;
         DevWorkList detail, criteria, maximum;
+        // the work lists of a complete image (cells first)
+        std::vector<DevWorkList*> lists() { return {&cells, &projection, &prediction, &detail, &criteria, &maximum}; }
+    };
+
+    // Packs tables, work lists and recipes into one Blob; the device pointers
+    // are bound when finish() has allocated it (then the data is copied).
+    class Packer
+    {
+      public:
+        explicit Packer(Blob& b) : b_(b) {}
+        void table(const MRMeshData& m, dev::Table& t);
+        void list(DevWorkList& l);
+        void recipes(DevRecipes& r);
+        void finish(hipStream_t s);
+
+      private:
+        Blob& b_;
+        std::vector<std::function<void()>> binds_;
     };
 
     struct Mesh
@@ -158,6 +181,19 @@ namespace smr
     };
 
     std::unique_ptr<DevMesh> make_dev_mesh(Ctx& ctx, const MRMeshData& m);
+
+    // host half of a complete device image (thread-safe: no HIP calls)
+    struct HostImage
+    {
+        WorkList cells;
+        GhostProgram ghost;
+        AdaptProgram adapt;
+    };
+    HostImage build_host_image(const MRMeshData& m);
+    // packs a complete image (table, cells, ghost and adapt programs) into p
+    void pack_image(Packer& p, DevMesh& dm, const MRMeshData& m, HostImage&& h);
+    // device half: upload and resolve (check = wait for and verify the resolve)
+    std::unique_ptr<DevMesh> upload_image(Ctx& ctx, const MRMeshData& m, HostImage&& h, bool check);
     // DevWorkList upload + resolve against the given tables
     void upload_list(Ctx& ctx, DevWorkList& l, const dev::Table& t0, const dev::Table& t1);
 
