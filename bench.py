@@ -99,6 +99,89 @@ def cpu_baseline(cells, values, a, dt, eps, reg, budget_s):
                       f"{setup:.1f} s untimed; single thread, dense-set oracle"}
 
 
+def make_batch_set(sa, ctx, n, levels, seed0):
+    """n instances of the batched workload (tests/initial_conditions.py:batch_instance)."""
+    from initial_conditions import batch_field, batch_instance
+    out = []
+    for k in range(n):
+        seed = seed0 + k
+        max_level, a, nb, ng = batch_instance(seed, levels)
+        mesh = sa.make_mesh(ctx, sa.mesh_config(2, 2, max_level, 1))
+        u = sa.make_scalar_field("u", mesh)
+        u.set_cells(batch_field(mesh.cells(), seed, nb, ng))
+        sa.make_bc_dirichlet(u, 0.0)
+        unp1 = sa.make_scalar_field("unp1", mesh)
+        out.append({"mesh": mesh, "u": u, "unp1": unp1, "a": np.array(a), "dt": 0.5 * 2.0 ** -max_level})
+    return out
+
+
+def bench_batched(sa, ctx, args, eps, reg):
+    """The batched workload advanced by one Batch, and the same instances
+    advanced one by one through the plain entry points (same start state)."""
+    levels = (args.batch_min_max_level, args.batch_max_max_level)
+    t_setup = time.perf_counter()
+    bset = make_batch_set(sa, ctx, args.batch, levels, 1000 + args.seed)
+    pset = make_batch_set(sa, ctx, args.batch, levels, 1000 + args.seed)
+    for group in (bset, pset):
+        b = sa.Batch(ctx, [(x["u"], x["unp1"], x["a"], x["dt"]) for x in group])
+        b.adapt(eps, reg)  # initial adaptation (setup, like the demo before its loop)
+        del b
+    ctx.synchronize()
+    batch = sa.Batch(ctx, [(x["u"], x["unp1"], x["a"], x["dt"]) for x in bset])
+    tune_file = sa.default_tuning_file()
+    tuned = batch.load_tuning(tune_file)
+    if not tuned or args.retune:
+        batch.autotune(reps=3, cache_file=tune_file)
+    setup = time.perf_counter() - t_setup
+
+    for _ in range(args.warmup):
+        batch.step(eps, reg)
+    ctx.synchronize()
+    updates = 0
+    t0 = time.perf_counter()
+    for _ in range(args.batch_steps):
+        batch.step(eps, reg)
+        updates += batch.nb_cells()
+    ctx.synchronize()
+    bwall = time.perf_counter() - t0
+
+    adapts = [sa.make_MRAdapt(x["u"]) for x in pset]
+
+    def plain_step():
+        n = 0
+        for x, ad in zip(pset, adapts):
+            ad(eps, reg)
+            sa.update_ghost_mr(x["u"])
+            x["unp1"].resize()
+            sa.upwind(x["a"], x["dt"], x["u"], x["unp1"])
+            sa.swap(x["u"], x["unp1"])
+            n += x["mesh"].nb_cells()
+        return n
+
+    for _ in range(args.warmup):
+        plain_step()
+    ctx.synchronize()
+    pupdates = 0
+    t0 = time.perf_counter()
+    for _ in range(args.batch_steps):
+        pupdates += plain_step()
+    ctx.synchronize()
+    pwall = time.perf_counter() - t0
+    leaves = [x["mesh"].nb_cells() for x in bset]
+    return {
+        "workload": f"{args.batch} independent 2D advection instances, max_level U{{{levels[0]}..{levels[1]}}}, "
+                    f"min_level 2, unit velocity at angle U[0,2pi), 1-5 seeded boxes/Gaussians, eps=1e-4, "
+                    f"regularity 2, dt=0.5*dx_finest, Dirichlet 0",
+        "value": updates / bwall, "unit": "cell-updates/s", "steps": args.batch_steps,
+        "ms_per_step": bwall / args.batch_steps * 1e3,
+        "plain_loop_value": pupdates / pwall, "plain_loop_ms_per_step": pwall / args.batch_steps * 1e3,
+        "speedup_vs_plain_loop": (updates / bwall) / (pupdates / pwall),
+        "leaves_total": int(sum(leaves)), "leaves_min": int(min(leaves)), "leaves_max": int(max(leaves)),
+        "tuning": batch.tuning(), "tuning_from_cache": bool(tuned and not args.retune),
+        "setup_s": setup,
+    }
+
+
 def main():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
@@ -108,6 +191,11 @@ def main():
     p.add_argument("--uniform-reps", type=int, default=20)
     p.add_argument("--phase-steps", type=int, default=3, help="extra steps timed per phase (0 disables)")
     p.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle time (0 disables)")
+    p.add_argument("--batch", type=int, default=256, help="instances of the batched workload (0 disables)")
+    p.add_argument("--batch-steps", type=int, default=5)
+    p.add_argument("--batch-min-max-level", type=int, default=7)
+    p.add_argument("--batch-max-max-level", type=int, default=10)
+    p.add_argument("--retune", action="store_true", help="rerun the batch autotuner even if a cached shape exists")
     args = p.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): this build has no multi-GPU path; "
@@ -199,6 +287,10 @@ def main():
     uniform_gbs = ubytes / (u_ms * 1e-3) / 1e9
     uniform_cups = umesh.nb_cells() / (u_ms * 1e-3)
 
+    batched = None
+    if args.batch > 0:
+        batched = bench_batched(sa, ctx, args, eps, reg)
+
     cpu = None
     if args.cpu_budget > 0:
         cpu = cpu_baseline(mesh.cells(), u.cells(), a, dt, eps, reg, args.cpu_budget)
@@ -231,6 +323,7 @@ def main():
                               "frac": adaptive_gbs / HBM_PEAK_GBS},
         "cpu_baseline": cpu,
         "phases": phases,
+        "batched": batched,
     }
     print(json.dumps(out), flush=True)
 

@@ -27,6 +27,7 @@ typedef struct smr_ctx smr_ctx;
 typedef struct smr_mesh smr_mesh;
 typedef struct smr_field smr_field;
 typedef struct smr_adapt smr_adapt;
+typedef struct smr_batch smr_batch;
 
 /* MRMeshId (mr/mesh.hpp:25-34) */
 enum smr_mesh_id
@@ -152,6 +153,40 @@ int smr_set_upwind_variant(int id);
 int smr_timers_enable(int on);
 int smr_timers_reset(void);
 int64_t smr_timers_report(char* buf, int64_t size);
+
+/* ------------------------------------------------------------------------
+ * Batched form: many independent instances of the advection demo loop
+ * (demos/FiniteVolume/advection_2d.cpp:129-153) advanced together.  No
+ * single reference entry point corresponds: each call does, for every
+ * instance, what the reference does per mesh with make_MRAdapt(u)(eps, reg)
+ * (mr/adapt.hpp:147-203), unp1.resize(), update_ghost_mr(u)
+ * (algorithm/update_ghost_mr.hpp:260-270), unp1 = u - dt * upwind(a, u)
+ * (stencil_field.hpp:175-179) and swap(u, unp1) (field/swap.hpp:12-19).
+ * The results are bit-identical to those per-instance calls.
+ * ---------------------------------------------------------------------- */
+/* u[i], unp1[i]: the two fields of instance i (same mesh, one mesh per
+ * instance); a: n*3 velocities (third ignored in 2D); dt: n time steps */
+int smr_batch_create(smr_ctx* ctx, int n, smr_field* const* u, smr_field* const* unp1, const double* a, const double* dt,
+                     smr_batch** out);
+int smr_batch_destroy(smr_batch* batch);
+/* adaptation of every u; returns the total number of harten calls (>= 0) */
+int smr_batch_adapt(smr_batch* batch, double eps, double regularity);
+/* ghost update, upwind into unp1 and swap, for every instance */
+int smr_batch_advance(smr_batch* batch);
+/* adapt + advance; returns the total number of harten calls (>= 0) */
+int smr_batch_step(smr_batch* batch, double eps, double regularity);
+int64_t smr_batch_nb_cells(const smr_batch* batch);
+/* launch shape of the stage programs: block size (multiple of 64, <= 1024),
+ * consecutive cells per thread, mode (0 one cooperative launch per program,
+ * 1 one launch per stage, 2 stage launches replayed from a hipGraph) */
+int smr_batch_set_tuning(smr_batch* batch, int block, int cells_per_thread, int mode);
+int smr_batch_get_tuning(const smr_batch* batch, int* block, int* cells_per_thread, int* mode);
+/* times every launch shape on the advance program of the current state and
+ * keeps the fastest; stores it for this GPU architecture in cache_file when
+ * non-null */
+int smr_batch_autotune(smr_batch* batch, int reps, const char* cache_file);
+/* 1 if cache_file holds a launch shape for this GPU architecture (then applied), else 0 */
+int smr_batch_load_tuning(smr_batch* batch, const char* cache_file);
 
 #ifdef __cplusplus
 }

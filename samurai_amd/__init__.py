@@ -84,6 +84,17 @@ def lib():
         L.smr_adapt_last_size.restype = i64
         L.smr_adapt_last_size.argtypes = [vp]
         L.smr_adapt_last_tags.argtypes = [vp, P(ctypes.c_uint8), P(dbl)]
+        L.smr_batch_create.argtypes = [vp, i32, P(vp), P(vp), P(dbl), P(dbl), ctypes.POINTER(vp)]
+        L.smr_batch_destroy.argtypes = [vp]
+        L.smr_batch_adapt.argtypes = [vp, dbl, dbl]
+        L.smr_batch_advance.argtypes = [vp]
+        L.smr_batch_step.argtypes = [vp, dbl, dbl]
+        L.smr_batch_nb_cells.restype = i64
+        L.smr_batch_nb_cells.argtypes = [vp]
+        L.smr_batch_set_tuning.argtypes = [vp, i32, i32, i32]
+        L.smr_batch_get_tuning.argtypes = [vp, P(i32), P(i32), P(i32)]
+        L.smr_batch_autotune.argtypes = [vp, i32, ctypes.c_char_p]
+        L.smr_batch_load_tuning.argtypes = [vp, ctypes.c_char_p]
         _lib = L
     return _lib
 
@@ -308,3 +319,71 @@ def upwind_variant_count():
 def set_upwind_variant(i):
     """Launch shape of the upwind kernel (tuning knob; results are identical)."""
     _check(lib().smr_set_upwind_variant(int(i)))
+
+
+# launch modes of the batched stage programs
+MODE_FUSED, MODE_STAGES, MODE_GRAPH = 0, 1, 2
+
+
+def default_tuning_file():
+    """Where the batch autotuner keeps its choice per GPU architecture."""
+    d = os.environ.get("SMR_TUNE_DIR") or os.path.join(os.path.expanduser("~"), ".cache", "samurai_amd")
+    return os.path.join(d, "batch_tuning.txt")
+
+
+class Batch:
+    """Many independent instances of the advection demo loop advanced together.
+
+    instances: list of (u, unp1, a, dt) with u, unp1 ScalarFields on one mesh
+    per instance.  step(eps, reg) is, for every instance, MRadaptation(eps,
+    reg); unp1.resize(); update_ghost_mr(u); unp1 = u - dt * upwind(a, u);
+    swap(u, unp1) -- bit-identical to those per-instance calls."""
+
+    def __init__(self, ctx, instances):
+        n = len(instances)
+        vp = ctypes.c_void_p
+        us = (vp * max(n, 1))(*[i[0].h for i in instances])
+        vs = (vp * max(n, 1))(*[i[1].h for i in instances])
+        a = np.zeros((max(n, 1), 3))
+        dt = np.zeros(max(n, 1))
+        for k, (_, _, ak, dk) in enumerate(instances):
+            ak = np.asarray(ak, dtype=np.float64).ravel()
+            a[k, :len(ak)] = ak
+            dt[k] = dk
+        h = vp()
+        _check(lib().smr_batch_create(ctx.h, n, us, vs, _dp(a), _dp(dt), ctypes.byref(h)))
+        self.ctx, self.instances, self.h = ctx, list(instances), h
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            lib().smr_batch_destroy(self.h)
+            self.h = None
+
+    def adapt(self, epsilon, regularity):
+        return _check(lib().smr_batch_adapt(self.h, float(epsilon), float(regularity)))
+
+    def advance(self):
+        _check(lib().smr_batch_advance(self.h))
+
+    def step(self, epsilon, regularity):
+        return _check(lib().smr_batch_step(self.h, float(epsilon), float(regularity)))
+
+    def nb_cells(self):
+        return lib().smr_batch_nb_cells(self.h)
+
+    def set_tuning(self, block, cells_per_thread, mode):
+        _check(lib().smr_batch_set_tuning(self.h, int(block), int(cells_per_thread), int(mode)))
+
+    def tuning(self):
+        b, c, m = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(lib().smr_batch_get_tuning(self.h, ctypes.byref(b), ctypes.byref(c), ctypes.byref(m)))
+        return {"block": b.value, "cells_per_thread": c.value, "mode": m.value}
+
+    def autotune(self, reps=5, cache_file=None):
+        if cache_file:
+            os.makedirs(os.path.dirname(cache_file) or ".", exist_ok=True)
+        _check(lib().smr_batch_autotune(self.h, int(reps), cache_file.encode() if cache_file else None))
+        return self.tuning()
+
+    def load_tuning(self, cache_file):
+        return bool(_check(lib().smr_batch_load_tuning(self.h, cache_file.encode())))

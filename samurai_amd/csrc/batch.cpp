@@ -1,0 +1,971 @@
+#include "batch.hpp"
+
+#include "host/parallel.hpp"
+
+#include <algorithm>
+#include <array>
+#include <cstddef>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <stdexcept>
+
+namespace smr
+{
+    namespace
+    {
+        constexpr int AUX_THR  = 1;
+        constexpr int AUX_TABS = 2;
+
+        size_t align_up(size_t n, size_t a)
+        {
+            return (n + a - 1) / a * a;
+        }
+
+        dev::Segment make_seg(int op, const MeshConfig& cfg)
+        {
+            dev::Segment g;
+            std::memset(&g, 0, sizeof(g));
+            g.op        = op;
+            g.dim       = cfg.dim;
+            g.pred_s    = cfg.pred_s;
+            g.min_level = cfg.min_level;
+            g.max_level = cfg.max_level;
+            return g;
+        }
+
+        dev::Segment list_seg(int op, const MeshConfig& cfg, const DevWorkList& l, int64_t c0, int64_t c1)
+        {
+            dev::Segment g = make_seg(op, cfg);
+            g.list         = l.d;
+            g.c0           = c0;
+            g.c1           = c1;
+            return g;
+        }
+
+        void push_stage(Chain& ch, const dev::Segment& g)
+        {
+            if (g.c1 > g.c0)
+            {
+                ch.push_back(Stage{g});
+            }
+        }
+
+        // update_ghost_mr (update_ghost_mr.hpp:194-237) on one instance, in
+        // the order of the standalone launches: per level from the finest, the
+        // outer-ghost recipe stages then the projection to the level below;
+        // then the prediction levels from the coarsest
+        void ghost_chain(Chain& ch, DevMesh& dm, const MeshConfig& cfg, Field& f)
+        {
+            for (int level = cfg.max_level; level >= 0; --level)
+            {
+                DevRecipes& r  = dm.outer[size_t(level)];
+                const auto& sp = r.host.stage_ptr;
+                if (!sp.empty() && size_t(sp.back()) > r.host.size())
+                {
+                    throw std::logic_error("recipe stages exceed the recipe list");
+                }
+                for (size_t k = 0; k + 1 < sp.size(); ++k)
+                {
+                    dev::Segment g = make_seg(dev::SEG_RECIPE, cfg);
+                    g.rdst         = r.dst;
+                    g.rop          = r.op;
+                    g.rsp          = r.sp;
+                    g.rsrc         = r.src;
+                    g.c0           = sp[k];
+                    g.c1           = sp[k + 1];
+                    g.u            = f.d;
+                    g.has_bc       = f.has_bc ? 1 : 0;
+                    g.bc           = f.bc_value;
+                    push_stage(ch, g);
+                }
+                if (level > 0)
+                {
+                    int64_t c0, c1;
+                    dm.projection.cells_of_levels(level - 1, level, c0, c1);
+                    dev::Segment g = list_seg(dev::SEG_PROJECT, cfg, dm.projection, c0, c1);
+                    g.u            = f.d;
+                    push_stage(ch, g);
+                }
+            }
+            for (int level = 1; level <= cfg.max_level; ++level)
+            {
+                int64_t c0, c1;
+                dm.prediction.cells_of_levels(level, level + 1, c0, c1);
+                dev::Segment g = list_seg(dev::SEG_PREDICT_FINE, cfg, dm.prediction, c0, c1);
+                g.u            = f.d;
+                push_stage(ch, g);
+            }
+        }
+
+        // stage k of the result is stage k of a and of b
+        Chain zip(Chain a, const Chain& b)
+        {
+            if (a.size() < b.size())
+            {
+                a.resize(b.size());
+            }
+            for (size_t k = 0; k < b.size(); ++k)
+            {
+                a[k].insert(a[k].end(), b[k].begin(), b[k].end());
+            }
+            return a;
+        }
+
+        std::string gpu_arch(int device)
+        {
+            hipDeviceProp_t prop;
+            if (hipGetDeviceProperties(&prop, device) != hipSuccess)
+            {
+                return "unknown";
+            }
+            std::string a = prop.gcnArchName;
+            const size_t c = a.find(':');
+            return c == std::string::npos ? a : a.substr(0, c);
+        }
+
+        int max_blocks_cached(int block)
+        {
+            static std::map<int, int> cache;
+            auto it = cache.find(block);
+            if (it == cache.end())
+            {
+                it = cache.emplace(block, dev::program_max_blocks(block)).first;
+            }
+            return it->second;
+        }
+
+        // host-side bounds of a segment before it reaches the device
+        void check_segment(const dev::Segment& g)
+        {
+            bool ok = g.c0 >= 0 && g.c0 < g.c1;
+            switch (g.op)
+            {
+                case dev::SEG_RECIPE: ok = ok && g.rdst && g.rop && g.rsp && g.rsrc && g.u; break;
+                case dev::SEG_RESOLVE: ok = ok && g.list.base && g.c1 <= g.list.nruns * g.list.ntaps && g.tabs && g.err; break;
+                case dev::SEG_FILL_F64: ok = ok && g.out; break;
+                case dev::SEG_FILL_U8: ok = ok && g.tag; break;
+                case dev::SEG_TAG_INIT:
+                case dev::SEG_MAXIMUM: ok = ok && g.list.iv && g.c1 <= g.list.ncells && g.tag; break;
+                case dev::SEG_CRITERIA: ok = ok && g.list.iv && g.c1 <= g.list.ncells && g.tag && g.out && g.thr; break;
+                case dev::SEG_PROJECT:
+                case dev::SEG_PREDICT_FINE: ok = ok && g.list.iv && g.c1 <= g.list.ncells && g.u; break;
+                case dev::SEG_DETAIL:
+                case dev::SEG_UPWIND:
+                case dev::SEG_COPY:
+                case dev::SEG_PROJECT_TO:
+                case dev::SEG_PREDICT_CHILDREN: ok = ok && g.list.iv && g.c1 <= g.list.ncells && g.u && g.out; break;
+                default: ok = false;
+            }
+            ok = ok && (g.dim == 2 || g.dim == 3) && (g.pred_s == 0 || g.pred_s == 1);
+            if (!ok)
+            {
+                throw std::logic_error("stage program segment out of bounds (op " + std::to_string(g.op) + ")");
+            }
+        }
+
+        struct Rebuild
+        {
+            bool converged = false;
+            MRMeshData nm;
+            HostImage img;
+            WorkList copy, coarsen, refine;
+        };
+    } // namespace
+
+    // ------------------------------------------------------------------
+    // ProgramBuf
+    // ------------------------------------------------------------------
+    ProgramBuf::~ProgramBuf()
+    {
+        drop_graph();
+        if (copied_)
+        {
+            (void)hipEventSynchronize(copied_);
+            (void)hipEventDestroy(copied_);
+        }
+        if (hbuf_)
+        {
+            (void)hipHostFree(hbuf_);
+        }
+        if (dbuf_)
+        {
+            dev_free(dbuf_, ctx_->stream);
+        }
+    }
+
+    void ProgramBuf::drop_graph()
+    {
+        if (gexec_)
+        {
+            // a launched replay may still be running
+            (void)hipStreamSynchronize(ctx_->stream);
+            (void)hipGraphExecDestroy(gexec_);
+            gexec_ = nullptr;
+        }
+    }
+
+    void ProgramBuf::clear()
+    {
+        segs_.clear();
+        seg_pref_.assign(1, 0);
+        stage_seg_.assign(1, 0);
+        stage_cells_.clear();
+        aux_.clear();
+        uploaded_ = false;
+    }
+
+    void ProgramBuf::append(const std::vector<Chain>& chains)
+    {
+        size_t depth = 0;
+        for (const Chain& c : chains)
+        {
+            depth = std::max(depth, c.size());
+        }
+        for (size_t k = 0; k < depth; ++k)
+        {
+            int64_t cells = 0;
+            for (const Chain& c : chains)
+            {
+                if (k >= c.size())
+                {
+                    continue;
+                }
+                for (const dev::Segment& g : c[k])
+                {
+                    if (g.c1 <= g.c0)
+                    {
+                        continue;
+                    }
+                    check_segment(g);
+                    segs_.push_back(g);
+                    seg_pref_.push_back(seg_pref_.back() + (g.c1 - g.c0));
+                    cells += g.c1 - g.c0;
+                }
+            }
+            if (cells > 0)
+            {
+                stage_seg_.push_back(int32_t(segs_.size()));
+                stage_cells_.push_back(cells);
+            }
+        }
+        uploaded_ = false;
+    }
+
+    size_t ProgramBuf::add_aux(const void* src, size_t n)
+    {
+        const size_t off = align_up(aux_.size(), 16);
+        aux_.resize(off + n);
+        std::memcpy(aux_.data() + off, src, n);
+        return off;
+    }
+
+    int64_t ProgramBuf::max_stage_cells() const
+    {
+        int64_t m = 0;
+        for (int64_t c : stage_cells_)
+        {
+            m = std::max(m, c);
+        }
+        return m;
+    }
+
+    int64_t ProgramBuf::total_cells() const
+    {
+        int64_t t = 0;
+        for (int64_t c : stage_cells_)
+        {
+            t += c;
+        }
+        return t;
+    }
+
+    void ProgramBuf::upload()
+    {
+        hipStream_t s        = ctx_->stream;
+        const size_t nseg    = segs_.size();
+        const size_t o_segs  = 0;
+        const size_t o_pref  = align_up(o_segs + nseg * sizeof(dev::Segment), 256);
+        const size_t o_stage = align_up(o_pref + (nseg + 1) * sizeof(int64_t), 256);
+        const size_t o_aux   = align_up(o_stage + stage_seg_.size() * sizeof(int32_t), 256);
+        const size_t total   = std::max<size_t>(o_aux + aux_.size(), 256);
+        drop_graph();
+        if (!copied_)
+        {
+            hip_check(hipEventCreateWithFlags(&copied_, hipEventDisableTiming), "hipEventCreate");
+        }
+        else
+        {
+            // the staging buffer is reused: the previous copy must have read it
+            hip_check(hipEventSynchronize(copied_), "hipEventSynchronize");
+        }
+        if (total > dcap_)
+        {
+            if (dbuf_)
+            {
+                dev_free(dbuf_, s);
+            }
+            dcap_ = std::max(total, 2 * dcap_);
+            dbuf_ = static_cast<char*>(dev_alloc(dcap_, s));
+        }
+        if (total > hcap_)
+        {
+            if (hbuf_)
+            {
+                hip_check(hipHostFree(hbuf_), "hipHostFree(program)");
+                hbuf_ = nullptr;
+            }
+            hcap_ = std::max(total, 2 * hcap_);
+            hip_check(hipHostMalloc(reinterpret_cast<void**>(&hbuf_), hcap_, hipHostMallocDefault), "hipHostMalloc(program)");
+        }
+        auto* hsegs = reinterpret_cast<dev::Segment*>(hbuf_ + o_segs);
+        std::memcpy(hsegs, segs_.data(), nseg * sizeof(dev::Segment));
+        for (size_t i = 0; i < nseg; ++i)
+        {
+            dev::Segment& g = hsegs[i];
+            if (g.aux & AUX_THR)
+            {
+                g.thr = reinterpret_cast<const double*>(dbuf_ + o_aux + reinterpret_cast<uintptr_t>(g.thr));
+            }
+            if (g.aux & AUX_TABS)
+            {
+                g.tabs = reinterpret_cast<const dev::Table*>(dbuf_ + o_aux + reinterpret_cast<uintptr_t>(g.tabs));
+            }
+            g.aux = 0;
+        }
+        std::memcpy(hbuf_ + o_pref, seg_pref_.data(), (nseg + 1) * sizeof(int64_t));
+        std::memcpy(hbuf_ + o_stage, stage_seg_.data(), stage_seg_.size() * sizeof(int32_t));
+        if (!aux_.empty())
+        {
+            std::memcpy(hbuf_ + o_aux, aux_.data(), aux_.size());
+        }
+        hip_check(hipMemcpyAsync(dbuf_, hbuf_, total, hipMemcpyHostToDevice, s), "hipMemcpyAsync(program)");
+        hip_check(hipEventRecord(copied_, s), "hipEventRecord");
+        prog_.segs      = reinterpret_cast<const dev::Segment*>(dbuf_ + o_segs);
+        prog_.seg_pref  = reinterpret_cast<const int64_t*>(dbuf_ + o_pref);
+        prog_.stage_seg = reinterpret_cast<const int32_t*>(dbuf_ + o_stage);
+        prog_.nstages   = int(stage_cells_.size());
+        uploaded_       = true;
+    }
+
+    void ProgramBuf::run(const Tuning& t)
+    {
+        if (!uploaded_)
+        {
+            throw std::logic_error("stage program run before upload");
+        }
+        if (prog_.nstages == 0)
+        {
+            return;
+        }
+        hipStream_t s = ctx_->stream;
+        if (t.mode == MODE_FUSED)
+        {
+            const int maxb        = max_blocks_cached(t.block);
+            const int64_t per     = int64_t(t.block) * t.cpt;
+            const int64_t needed  = (max_stage_cells() + per - 1) / per;
+            const int nblocks     = int(std::max<int64_t>(1, std::min<int64_t>(needed, maxb)));
+            if (maxb > 0 && dev::run_program_fused(prog_, t.block, nblocks, t.cpt, s))
+            {
+                return;
+            }
+            // the runtime refused the cooperative launch: same program, one launch per stage
+            (void)hipGetLastError();
+            dev::run_program_stagewise(prog_, stage_cells_, t.block, t.cpt, s);
+            return;
+        }
+        if (t.mode == MODE_GRAPH)
+        {
+            if (!gexec_ || g_block_ != t.block || g_cpt_ != t.cpt)
+            {
+                drop_graph();
+                hipGraph_t g = nullptr;
+                hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+                dev::run_program_stagewise(prog_, stage_cells_, t.block, t.cpt, s);
+                hip_check(hipStreamEndCapture(s, &g), "hipStreamEndCapture");
+                const hipError_t e = hipGraphInstantiate(&gexec_, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                hip_check(e, "hipGraphInstantiate");
+                g_block_ = t.block;
+                g_cpt_   = t.cpt;
+            }
+            hip_check(hipGraphLaunch(gexec_, s), "hipGraphLaunch");
+            return;
+        }
+        dev::run_program_stagewise(prog_, stage_cells_, t.block, t.cpt, s);
+    }
+
+    // ------------------------------------------------------------------
+    // Batch
+    // ------------------------------------------------------------------
+    Batch::Batch(Ctx* ctx, std::vector<BatchInstance> inst)
+        : ctx_(ctx), inst_(std::move(inst)), adapt_prog_(ctx), resolve_prog_(ctx), uf_prog_(ctx)
+    {
+        ctx_->require_device();
+        std::vector<const Mesh*> meshes;
+        for (const BatchInstance& b : inst_)
+        {
+            if (!b.u || !b.unp1 || b.u == b.unp1)
+            {
+                throw std::invalid_argument("batch instance needs two distinct fields u and unp1");
+            }
+            if (b.u->mesh != b.unp1->mesh)
+            {
+                throw std::invalid_argument("batch instance fields u and unp1 are on different meshes");
+            }
+            if (b.u->mesh->ctx != ctx_)
+            {
+                throw std::invalid_argument("batch instance mesh belongs to another context");
+            }
+            meshes.push_back(b.u->mesh);
+        }
+        std::sort(meshes.begin(), meshes.end());
+        if (std::adjacent_find(meshes.begin(), meshes.end()) != meshes.end())
+        {
+            throw std::invalid_argument("two batch instances share a mesh");
+        }
+        fv_prog_[0] = std::make_unique<ProgramBuf>(ctx_);
+        fv_prog_[1] = std::make_unique<ProgramBuf>(ctx_);
+    }
+
+    Batch::~Batch()
+    {
+        hipStream_t s = ctx_->stream;
+        if (det_arena_)
+        {
+            dev_free(det_arena_, s);
+        }
+        if (tag_arena_)
+        {
+            dev_free(tag_arena_, s);
+        }
+        (void)hipStreamSynchronize(s);
+        if (host_tags_)
+        {
+            (void)hipHostFree(host_tags_);
+        }
+    }
+
+    void Batch::check_instances() const
+    {
+        for (const BatchInstance& b : inst_)
+        {
+            b.u->check_current();
+            b.unp1->check_current();
+        }
+    }
+
+    int64_t Batch::nb_cells() const
+    {
+        int64_t n = 0;
+        for (const BatchInstance& b : inst_)
+        {
+            n += b.u->mesh->host.nb_cells(CELLS);
+        }
+        return n;
+    }
+
+    void Batch::set_tuning(const Tuning& t)
+    {
+        if (t.block < 64 || t.block > 1024 || (t.block & 63) != 0)
+        {
+            throw std::invalid_argument("block size must be a multiple of 64 in [64, 1024]");
+        }
+        if (t.cpt < 1 || t.cpt > 64)
+        {
+            throw std::invalid_argument("cells per thread must be in [1, 64]");
+        }
+        if (t.mode < MODE_FUSED || t.mode > MODE_GRAPH)
+        {
+            throw std::invalid_argument("unknown program mode");
+        }
+        tuning_ = t;
+    }
+
+    void Batch::ensure_images(const std::vector<int>& ids)
+    {
+        std::vector<int> need;
+        for (int i : ids)
+        {
+            const Mesh& m = *inst_[size_t(i)].u->mesh;
+            if (!m.dev || !m.dev->has_ghost || !m.dev->has_adapt)
+            {
+                need.push_back(i);
+            }
+        }
+        if (need.empty())
+        {
+            return;
+        }
+        std::vector<HostImage> imgs(need.size());
+        parallel_for(int(need.size()), [&](int k) { imgs[size_t(k)] = build_host_image(inst_[size_t(need[size_t(k)])].u->mesh->host); });
+        for (size_t k = 0; k < need.size(); ++k)
+        {
+            Mesh& m = *inst_[size_t(need[k])].u->mesh;
+            m.dev   = upload_image(*ctx_, m.host, std::move(imgs[k]), false);
+        }
+        ctx_->check_resolve();
+    }
+
+    void Batch::ensure_arenas(int64_t n)
+    {
+        hipStream_t s = ctx_->stream;
+        if (n > arena_cap_)
+        {
+            if (det_arena_)
+            {
+                dev_free(det_arena_, s);
+                dev_free(tag_arena_, s);
+            }
+            arena_cap_ = std::max<int64_t>(n, arena_cap_ + arena_cap_ / 2);
+            det_arena_ = static_cast<double*>(dev_alloc(size_t(arena_cap_) * 8, s));
+            tag_arena_ = static_cast<uint8_t*>(dev_alloc(size_t(arena_cap_), s));
+        }
+        if (n > host_cap_)
+        {
+            if (host_tags_)
+            {
+                hip_check(hipHostFree(host_tags_), "hipHostFree(tags)");
+                host_tags_ = nullptr;
+            }
+            host_cap_ = std::max<int64_t>(n, host_cap_ + host_cap_ / 2);
+            hip_check(hipHostMalloc(reinterpret_cast<void**>(&host_tags_), size_t(host_cap_), hipHostMallocDefault), "hipHostMalloc(tags)");
+        }
+    }
+
+    int Batch::adapt(double eps, double regularity)
+    {
+        check_instances();
+        ScopedTimer timer(ctx_, "mesh adaptation");
+        hipStream_t s = ctx_->stream;
+        std::vector<int> act;
+        for (size_t i = 0; i < inst_.size(); ++i)
+        {
+            const MeshConfig& c = inst_[i].u->mesh->host.cfg;
+            if (c.min_level < c.max_level)
+            {
+                act.push_back(int(i));
+            }
+        }
+        int calls = 0;
+        for (int ite = 0;; ++ite)
+        {
+            act.erase(std::remove_if(act.begin(), act.end(),
+                                     [&](int i)
+                                     {
+                                         const MeshConfig& c = inst_[size_t(i)].u->mesh->host.cfg;
+                                         return ite >= c.max_level - c.min_level;
+                                     }),
+                      act.end());
+            if (act.empty())
+            {
+                break;
+            }
+            ensure_images(act);
+            const size_t na = act.size();
+            std::vector<int64_t> off(na + 1, 0);
+            for (size_t k = 0; k < na; ++k)
+            {
+                off[k + 1] = off[k] + int64_t(align_up(size_t(inst_[size_t(act[k])].u->mesh->host.nref), 256));
+            }
+            ensure_arenas(std::max<int64_t>(off[na], 1));
+
+            // harten (mr/adapt.hpp:276-389) on the device: tags, ghosts, details, criteria, maximum
+            {
+                ScopedTimer t(ctx_, "harten device pass");
+                adapt_prog_.clear();
+                std::vector<Chain> chains(na);
+                for (size_t k = 0; k < na; ++k)
+                {
+                    Field& f            = *inst_[size_t(act[k])].u;
+                    Mesh& m             = *f.mesh;
+                    DevMesh& dm         = *m.dev;
+                    const MeshConfig& c = m.host.cfg;
+                    const int64_t n     = m.host.nref;
+                    double* det         = det_arena_ + off[k];
+                    uint8_t* tag        = tag_arena_ + off[k];
+                    Chain pre(2);
+                    dev::Segment fd = make_seg(dev::SEG_FILL_F64, c);
+                    fd.out          = det;
+                    fd.c1           = n;
+                    dev::Segment ft = make_seg(dev::SEG_FILL_U8, c);
+                    ft.tag          = tag;
+                    ft.c1           = n;
+                    pre[0]          = {fd, ft};
+                    dev::Segment ti = list_seg(dev::SEG_TAG_INIT, c, dm.cells, 0, dm.cells.d.ncells);
+                    ti.tag          = tag;
+                    pre[1]          = {ti};
+                    Chain g;
+                    ghost_chain(g, dm, c, f);
+                    Chain ch = zip(pre, g);
+                    int64_t c0, c1;
+                    dm.detail.cells_of_levels(c.min_level > 0 ? c.min_level - 1 : 0, c.max_level - ite, c0, c1);
+                    dev::Segment sd = list_seg(dev::SEG_DETAIL, c, dm.detail, c0, c1);
+                    sd.u            = f.d;
+                    sd.out          = det;
+                    push_stage(ch, sd);
+                    const dev::Thresholds thr = make_thresholds(c, ite, eps, regularity);
+                    const size_t h            = adapt_prog_.add_aux(&thr, sizeof(thr));
+                    dm.criteria.cells_of_levels(c.min_level - 1, c.max_level - ite, c0, c1);
+                    dev::Segment sc = list_seg(dev::SEG_CRITERIA, c, dm.criteria, c0, c1);
+                    sc.out          = det;
+                    sc.tag          = tag;
+                    sc.thr          = reinterpret_cast<const double*>(uintptr_t(h));
+                    sc.aux          = AUX_THR;
+                    push_stage(ch, sc);
+                    for (int level = c.max_level; level > 0; --level)
+                    {
+                        dm.maximum.cells_of_levels(level - 1, level, c0, c1);
+                        dev::Segment sm = list_seg(dev::SEG_MAXIMUM, c, dm.maximum, c0, c1);
+                        sm.tag          = tag;
+                        push_stage(ch, sm);
+                    }
+                    chains[k] = std::move(ch);
+                }
+                adapt_prog_.append(chains);
+                adapt_prog_.upload();
+                adapt_prog_.run(tuning_);
+                calls += int(na);
+            }
+
+            // tags to the host, then every instance's mesh update in parallel
+            std::vector<Rebuild> rb(na);
+            {
+                ScopedTimer t(ctx_, "mesh update");
+                hip_check(hipMemcpyAsync(host_tags_, tag_arena_, size_t(off[na]), hipMemcpyDeviceToHost, s), "hipMemcpyAsync(tags)");
+                ctx_->sync();
+                pending_.clear();
+                parallel_for(int(na),
+                             [&](int k)
+                             {
+                                 const Mesh& m       = *inst_[size_t(act[size_t(k)])].u->mesh;
+                                 const MeshConfig& c = m.host.cfg;
+                                 Rebuild& r          = rb[size_t(k)];
+                                 CellArray ca        = update_cell_array_from_tag(m.host, host_tags_ + off[size_t(k)]);
+                                 make_graduation(ca, c.dim, c.grad_width);
+                                 if (ca_equal(m.host.ids[CELLS], ca))
+                                 {
+                                     r.converged = true;
+                                     return;
+                                 }
+                                 r.nm  = build_mesh(c, ca);
+                                 r.img = build_host_image(r.nm);
+                                 build_update_fields_lists(m.host, r.nm, r.copy, r.coarsen, r.refine);
+                             });
+            }
+            std::vector<size_t> changed;
+            for (size_t k = 0; k < na; ++k)
+            {
+                if (!rb[k].converged)
+                {
+                    changed.push_back(k);
+                }
+            }
+            if (!changed.empty())
+            {
+                const size_t nc = changed.size();
+                std::vector<std::unique_ptr<DevMesh>> ndev(nc);
+                std::vector<std::array<DevWorkList, 3>> ufl(nc);
+                auto ufblob = std::make_unique<Blob>();
+                {
+                    ScopedTimer t(ctx_, "device upload");
+                    Packer up(*ufblob);
+                    for (size_t j = 0; j < nc; ++j)
+                    {
+                        Rebuild& r          = rb[changed[j]];
+                        ndev[j]             = std::make_unique<DevMesh>();
+                        ndev[j]->table_blob = std::make_unique<Blob>();
+                        Packer p(*ndev[j]->table_blob);
+                        pack_image(p, *ndev[j], r.nm, std::move(r.img));
+                        p.finish(s);
+                        ufl[j][0].host = std::move(r.copy);
+                        ufl[j][1].host = std::move(r.coarsen);
+                        ufl[j][2].host = std::move(r.refine);
+                        for (DevWorkList& l : ufl[j])
+                        {
+                            up.list(l);
+                        }
+                    }
+                    up.finish(s);
+                    // resolve every new list; nothing runs on them unless all pairs resolve
+                    resolve_prog_.clear();
+                    std::vector<Chain> chains(nc);
+                    for (size_t j = 0; j < nc; ++j)
+                    {
+                        const Mesh& m       = *inst_[size_t(act[changed[j]])].u->mesh;
+                        const MeshConfig& c = m.host.cfg;
+                        const dev::Table same[2] = {ndev[j]->table, ndev[j]->table};
+                        const dev::Table both[2] = {ndev[j]->table, m.dev->table};
+                        const size_t hs          = resolve_prog_.add_aux(same, sizeof(same));
+                        const size_t hb          = resolve_prog_.add_aux(both, sizeof(both));
+                        Stage st;
+                        auto add = [&](DevWorkList& l, size_t h)
+                        {
+                            dev::Segment g = list_seg(dev::SEG_RESOLVE, c, l, 0, l.d.nruns * l.d.ntaps);
+                            g.tabs         = reinterpret_cast<const dev::Table*>(uintptr_t(h));
+                            g.err          = ctx_->d_err;
+                            g.aux          = AUX_TABS;
+                            if (g.c1 > g.c0)
+                            {
+                                st.push_back(g);
+                            }
+                        };
+                        for (DevWorkList* l : ndev[j]->lists())
+                        {
+                            add(*l, hs);
+                        }
+                        for (DevWorkList& l : ufl[j])
+                        {
+                            add(l, hb);
+                        }
+                        chains[j].push_back(std::move(st));
+                    }
+                    resolve_prog_.append(chains);
+                    resolve_prog_.upload();
+                    resolve_prog_.run(tuning_);
+                    ctx_->check_resolve();
+                }
+                // update_fields (update_fields.hpp:27-127) into new storage; unp1 is resized (zero-filled)
+                {
+                    ScopedTimer t(ctx_, "fields update");
+                    uf_prog_.clear();
+                    std::vector<Chain> chains(nc);
+                    std::vector<double*> nu(nc), nv(nc);
+                    for (size_t j = 0; j < nc; ++j)
+                    {
+                        const BatchInstance& b = inst_[size_t(act[changed[j]])];
+                        const MeshConfig& c    = b.u->mesh->host.cfg;
+                        const int64_t n        = rb[changed[j]].nm.nref;
+                        nu[j] = static_cast<double*>(dev_alloc(size_t(std::max<int64_t>(n, 1)) * 8, s));
+                        nv[j] = static_cast<double*>(dev_alloc(size_t(std::max<int64_t>(n, 1)) * 8, s));
+                        Chain ch(1);
+                        dev::Segment z0 = make_seg(dev::SEG_FILL_F64, c);
+                        z0.out          = nu[j];
+                        z0.c1           = n;
+                        dev::Segment z1 = z0;
+                        z1.out          = nv[j];
+                        ch[0]           = {z0, z1};
+                        const int ops[3] = {dev::SEG_COPY, dev::SEG_PROJECT_TO, dev::SEG_PREDICT_CHILDREN};
+                        for (int q = 0; q < 3; ++q)
+                        {
+                            dev::Segment g = list_seg(ops[q], c, ufl[j][size_t(q)], 0, ufl[j][size_t(q)].d.ncells);
+                            g.out          = nu[j];
+                            g.u            = b.u->d;
+                            push_stage(ch, g);
+                        }
+                        chains[j] = std::move(ch);
+                    }
+                    uf_prog_.append(chains);
+                    uf_prog_.upload();
+                    uf_prog_.run(tuning_);
+                    for (size_t j = 0; j < nc; ++j)
+                    {
+                        BatchInstance& b = inst_[size_t(act[changed[j]])];
+                        Mesh& m          = *b.u->mesh;
+                        const int64_t n  = rb[changed[j]].nm.nref;
+                        // old storage and old image are released in stream order after the program
+                        dev_free(b.u->d, s);
+                        dev_free(b.unp1->d, s);
+                        m.replace(std::move(rb[changed[j]].nm), std::move(ndev[j]));
+                        b.u->d          = nu[j];
+                        b.unp1->d       = nv[j];
+                        b.u->size       = n;
+                        b.unp1->size    = n;
+                        b.u->version    = m.version;
+                        b.unp1->version = m.version;
+                    }
+                }
+                pending_.push_back(std::move(ufblob));
+            }
+            std::vector<int> next;
+            for (size_t k : changed)
+            {
+                next.push_back(act[k]);
+            }
+            act.swap(next);
+        }
+        return calls;
+    }
+
+    ProgramBuf& Batch::fv_program()
+    {
+        // identity of the program: image serials, storages and boundary values
+        std::vector<const void*> key;
+        key.reserve(inst_.size() * 4);
+        for (const BatchInstance& b : inst_)
+        {
+            key.push_back(reinterpret_cast<const void*>(uintptr_t(b.u->mesh->dev->serial)));
+            key.push_back(b.u->d);
+            key.push_back(b.unp1->d);
+            key.push_back(reinterpret_cast<const void*>(uintptr_t(b.u->has_bc ? 1 : 0)));
+            uint64_t bits;
+            std::memcpy(&bits, &b.u->bc_value, 8);
+            key.push_back(reinterpret_cast<const void*>(uintptr_t(bits)));
+        }
+        for (int q = 0; q < 2; ++q)
+        {
+            if (fv_key_[q] == key)
+            {
+                return *fv_prog_[q];
+            }
+        }
+        const int q   = fv_next_;
+        fv_next_      = 1 - fv_next_;
+        ProgramBuf& p = *fv_prog_[q];
+        p.clear();
+        std::vector<Chain> chains(inst_.size());
+        for (size_t i = 0; i < inst_.size(); ++i)
+        {
+            BatchInstance& b    = inst_[i];
+            DevMesh& dm         = *b.u->mesh->dev;
+            const MeshConfig& c = b.u->mesh->host.cfg;
+            Chain ch;
+            ghost_chain(ch, dm, c, *b.u);
+            dev::Segment g = list_seg(dev::SEG_UPWIND, c, dm.cells, 0, dm.cells.d.ncells);
+            g.u            = b.u->d;
+            g.out          = b.unp1->d;
+            g.a0           = b.a[0];
+            g.a1           = b.a[1];
+            g.a2           = c.dim == 3 ? b.a[2] : 0.0;
+            g.dt           = b.dt;
+            push_stage(ch, g);
+            chains[i] = std::move(ch);
+        }
+        p.append(chains);
+        p.upload();
+        fv_key_[q] = std::move(key);
+        return p;
+    }
+
+    void Batch::advance()
+    {
+        check_instances();
+        std::vector<int> all(inst_.size());
+        for (size_t i = 0; i < all.size(); ++i)
+        {
+            all[i] = int(i);
+        }
+        ensure_images(all);
+        {
+            ScopedTimer t(ctx_, "ghost update + field expressions");
+            fv_program().run(tuning_);
+        }
+        for (BatchInstance& b : inst_)
+        {
+            std::swap(b.u->d, b.unp1->d);
+            std::swap(b.u->size, b.unp1->size);
+            std::swap(b.u->version, b.unp1->version);
+        }
+    }
+
+    int Batch::step(double eps, double regularity)
+    {
+        const int calls = adapt(eps, regularity);
+        advance();
+        return calls;
+    }
+
+    // ------------------------------------------------------------------
+    // autotuner
+    // ------------------------------------------------------------------
+    bool Batch::load_tuning(const std::string& cache_file, Tuning& t)
+    {
+        if (cache_file.empty())
+        {
+            return false;
+        }
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        const std::string arch = gpu_arch(dev);
+        std::ifstream in(cache_file);
+        std::string line;
+        while (std::getline(in, line))
+        {
+            std::istringstream ls(line);
+            std::string a;
+            Tuning v;
+            if (ls >> a >> v.block >> v.cpt >> v.mode && a == arch)
+            {
+                t = v;
+                return true;
+            }
+        }
+        return false;
+    }
+
+    Tuning Batch::autotune(int reps, const std::string& cache_file)
+    {
+        check_instances();
+        std::vector<int> all(inst_.size());
+        for (size_t i = 0; i < all.size(); ++i)
+        {
+            all[i] = int(i);
+        }
+        ensure_images(all);
+        ProgramBuf& p = fv_program();
+        hipStream_t s = ctx_->stream;
+        hipEvent_t e0, e1;
+        hip_check(hipEventCreate(&e0), "hipEventCreate");
+        hip_check(hipEventCreate(&e1), "hipEventCreate");
+        Tuning best   = tuning_;
+        float best_ms = -1;
+        for (int mode : {MODE_FUSED, MODE_STAGES, MODE_GRAPH})
+        {
+            for (int block : {64, 128, 256, 512, 1024})
+            {
+                if (mode == MODE_FUSED && max_blocks_cached(block) <= 0)
+                {
+                    continue;
+                }
+                for (int cpt : {1, 2, 4, 8})
+                {
+                    const Tuning t{block, cpt, mode};
+                    p.run(t); // warm-up (and graph instantiation)
+                    hip_check(hipEventRecord(e0, s), "hipEventRecord");
+                    for (int r = 0; r < reps; ++r)
+                    {
+                        p.run(t);
+                    }
+                    hip_check(hipEventRecord(e1, s), "hipEventRecord");
+                    hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+                    float ms = 0;
+                    hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+                    if (best_ms < 0 || ms < best_ms)
+                    {
+                        best_ms = ms;
+                        best    = t;
+                    }
+                }
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        tuning_ = best;
+        if (!cache_file.empty())
+        {
+            const std::string arch = gpu_arch(ctx_->device);
+            std::vector<std::string> keep;
+            {
+                std::ifstream in(cache_file);
+                std::string line;
+                while (std::getline(in, line))
+                {
+                    std::istringstream ls(line);
+                    std::string a;
+                    if (ls >> a && a != arch)
+                    {
+                        keep.push_back(line);
+                    }
+                }
+            }
+            std::ofstream out(cache_file, std::ios::trunc);
+            for (const std::string& l : keep)
+            {
+                out << l << "\n";
+            }
+            out << arch << " " << best.block << " " << best.cpt << " " << best.mode << "\n";
+        }
+        return best;
+    }
+} // namespace smr

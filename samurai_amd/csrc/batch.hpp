@@ -1,0 +1,129 @@
+// Batched MR time steps: many independent adaptive instances (each a mesh
+// with its fields u and unp1) advanced together.  Every device phase of a step
+// (the Harten adaptation passes, update_fields, update_ghost_mr + upwind) is
+// one stage program over all instances: a single persistent cooperative launch,
+// or one launch per stage replayed from a hipGraph.  The host mesh rebuilds of
+// the instances run in parallel.
+#pragma once
+
+#include "runtime.hpp"
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace smr
+{
+    enum ProgramMode : int
+    {
+        MODE_FUSED  = 0, // one cooperative launch, grid barriers between stages
+        MODE_STAGES = 1, // one launch per stage
+        MODE_GRAPH  = 2  // MODE_STAGES captured in a hipGraph, replayed while the program is unchanged
+    };
+
+    struct Tuning
+    {
+        int block = 256;
+        int cpt   = 1; // consecutive cells per thread (interval chunk length)
+        int mode  = MODE_FUSED;
+    };
+
+    using Stage = std::vector<dev::Segment>;
+    using Chain = std::vector<Stage>;
+
+    // A stage program being assembled on the host and its device copy.
+    class ProgramBuf
+    {
+      public:
+        explicit ProgramBuf(Ctx* ctx) : ctx_(ctx) {}
+        ~ProgramBuf();
+        ProgramBuf(const ProgramBuf&)            = delete;
+        ProgramBuf& operator=(const ProgramBuf&) = delete;
+
+        void clear();
+        // appends the chains side by side: new stage k holds stage k of every chain
+        void append(const std::vector<Chain>& chains);
+        // bytes copied with the program; returns their offset, which a segment
+        // stores in its thr / tabs pointer with the matching `aux` bit set
+        size_t add_aux(const void* src, size_t n);
+        size_t nsegments() const { return segs_.size(); }
+        int nstages() const { return int(stage_cells_.size()); }
+        int64_t max_stage_cells() const;
+        int64_t total_cells() const;
+        void upload();
+        void run(const Tuning& t);
+
+      private:
+        Ctx* ctx_;
+        std::vector<dev::Segment> segs_;
+        std::vector<int64_t> seg_pref_{0}; // cumulative cells over all segments (+ total)
+        std::vector<int32_t> stage_seg_{0};
+        std::vector<int64_t> stage_cells_;
+        std::vector<char> aux_;
+        char* dbuf_ = nullptr;
+        size_t dcap_ = 0;
+        char* hbuf_ = nullptr; // pinned staging
+        size_t hcap_ = 0;
+        hipEvent_t copied_ = nullptr;
+        dev::Program prog_{};
+        // graph of the stage-wise launches of the uploaded program
+        hipGraphExec_t gexec_ = nullptr;
+        int g_block_ = 0, g_cpt_ = 0;
+        bool uploaded_ = false;
+        void drop_graph();
+    };
+
+    struct BatchInstance
+    {
+        Field* u    = nullptr;
+        Field* unp1 = nullptr;
+        double a[3] = {0, 0, 0};
+        double dt   = 0;
+    };
+
+    class Batch
+    {
+      public:
+        Batch(Ctx* ctx, std::vector<BatchInstance> inst);
+        ~Batch();
+
+        // Adapt::operator()(eps, regularity) on every instance (mr/adapt.hpp:147-195),
+        // then unp1 is resized on every changed mesh; returns the total number of harten calls
+        int adapt(double eps, double regularity);
+        // update_ghost_mr(u); unp1 = u - dt * upwind(a, u); swap(u, unp1) on every instance
+        void advance();
+        // one time step of the advection demo loop (advection_2d.cpp:129-153)
+        int step(double eps, double regularity);
+
+        int64_t nb_cells() const;
+        const Tuning& tuning() const { return tuning_; }
+        void set_tuning(const Tuning& t);
+        // times the advance program for each candidate shape and keeps the fastest;
+        // persisted per GPU architecture in `cache_file` when not empty
+        Tuning autotune(int reps, const std::string& cache_file);
+        static bool load_tuning(const std::string& cache_file, Tuning& t);
+
+      private:
+        Ctx* ctx_;
+        std::vector<BatchInstance> inst_;
+        Tuning tuning_;
+        ProgramBuf adapt_prog_, resolve_prog_, uf_prog_;
+        // advance programs for the two u/unp1 parities, reused while the instances' storages are unchanged
+        std::unique_ptr<ProgramBuf> fv_prog_[2];
+        std::vector<const void*> fv_key_[2];
+        int fv_next_ = 0;
+        // per-instance detail (f64) and tag (u8) storage, concatenated
+        double* det_arena_ = nullptr;
+        uint8_t* tag_arena_ = nullptr;
+        uint8_t* host_tags_ = nullptr;
+        int64_t arena_cap_ = 0, host_cap_ = 0;
+        // blobs whose pageable host staging must outlive the copies queued from it
+        std::vector<std::unique_ptr<Blob>> pending_;
+
+        void check_instances() const;
+        void ensure_images(const std::vector<int>& ids);
+        void ensure_arenas(int64_t n);
+        ProgramBuf& fv_program();
+    };
+} // namespace smr

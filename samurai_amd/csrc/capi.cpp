@@ -1,5 +1,6 @@
 // extern "C" boundary (include/samurai_amd.h) over the runtime.
 #include "../../include/samurai_amd.h"
+#include "batch.hpp"
 #include "runtime.hpp"
 
 #include <cstring>
@@ -20,6 +21,10 @@ struct smr_field
 struct smr_adapt
 {
     smr::AdaptState a;
+};
+struct smr_batch
+{
+    std::unique_ptr<smr::Batch> b;
 };
 
 namespace
@@ -632,6 +637,144 @@ extern "C"
                 require(adapt != nullptr, "null adapt");
                 adapt->a.copy_last(tags, details, adapt->a.field->mesh->ctx->stream);
                 return 0;
+            });
+    }
+
+    int smr_batch_create(smr_ctx* ctx, int n, smr_field* const* u, smr_field* const* unp1, const double* a, const double* dt,
+                         smr_batch** out)
+    {
+        return guarded(
+            [&]
+            {
+                require(ctx && out && n >= 0 && (n == 0 || (u && unp1 && a && dt)), "null argument");
+                ctx->c.require_device();
+                std::vector<smr::BatchInstance> inst{size_t(n)};
+                for (int i = 0; i < n; ++i)
+                {
+                    require(u[i] && unp1[i], "null field");
+                    smr::BatchInstance& b = inst[size_t(i)];
+                    b.u                   = &u[i]->f;
+                    b.unp1                = &unp1[i]->f;
+                    b.a[0]                = a[3 * i];
+                    b.a[1]                = a[3 * i + 1];
+                    b.a[2]                = a[3 * i + 2];
+                    b.dt                  = dt[i];
+                }
+                auto* r = new smr_batch();
+                try
+                {
+                    r->b = std::make_unique<smr::Batch>(&ctx->c, std::move(inst));
+                }
+                catch (...)
+                {
+                    delete r;
+                    throw;
+                }
+                *out = r;
+                return 0;
+            });
+    }
+
+    int smr_batch_destroy(smr_batch* batch)
+    {
+        return guarded(
+            [&]
+            {
+                delete batch;
+                return 0;
+            });
+    }
+
+    int smr_batch_adapt(smr_batch* batch, double eps, double regularity)
+    {
+        return guarded(
+            [&]
+            {
+                require(batch != nullptr, "null batch");
+                return batch->b->adapt(eps, regularity);
+            });
+    }
+
+    int smr_batch_advance(smr_batch* batch)
+    {
+        return guarded(
+            [&]
+            {
+                require(batch != nullptr, "null batch");
+                batch->b->advance();
+                return 0;
+            });
+    }
+
+    int smr_batch_step(smr_batch* batch, double eps, double regularity)
+    {
+        return guarded(
+            [&]
+            {
+                require(batch != nullptr, "null batch");
+                return batch->b->step(eps, regularity);
+            });
+    }
+
+    int64_t smr_batch_nb_cells(const smr_batch* batch)
+    {
+        if (!batch)
+        {
+            g_error = "null batch";
+            return -1;
+        }
+        return batch->b->nb_cells();
+    }
+
+    int smr_batch_set_tuning(smr_batch* batch, int block, int cells_per_thread, int mode)
+    {
+        return guarded(
+            [&]
+            {
+                require(batch != nullptr, "null batch");
+                batch->b->set_tuning(smr::Tuning{block, cells_per_thread, mode});
+                return 0;
+            });
+    }
+
+    int smr_batch_get_tuning(const smr_batch* batch, int* block, int* cells_per_thread, int* mode)
+    {
+        return guarded(
+            [&]
+            {
+                require(batch && block && cells_per_thread && mode, "null argument");
+                const smr::Tuning& t = batch->b->tuning();
+                *block               = t.block;
+                *cells_per_thread    = t.cpt;
+                *mode                = t.mode;
+                return 0;
+            });
+    }
+
+    int smr_batch_autotune(smr_batch* batch, int reps, const char* cache_file)
+    {
+        return guarded(
+            [&]
+            {
+                require(batch != nullptr && reps > 0, "bad argument");
+                batch->b->autotune(reps, cache_file ? std::string(cache_file) : std::string());
+                return 0;
+            });
+    }
+
+    int smr_batch_load_tuning(smr_batch* batch, const char* cache_file)
+    {
+        return guarded(
+            [&]
+            {
+                require(batch != nullptr, "null batch");
+                smr::Tuning t;
+                if (!cache_file || !smr::Batch::load_tuning(cache_file, t))
+                {
+                    return 0;
+                }
+                batch->b->set_tuning(t);
+                return 1;
             });
     }
 }

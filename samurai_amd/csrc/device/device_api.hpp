@@ -72,7 +72,10 @@ namespace smr::dev
     struct Segment
     {
         int32_t op, dim, pred_s, has_bc;
-        int32_t min_level, max_level, pad0, pad1;
+        int32_t min_level, max_level;
+        // host use while a program is assembled: bit 0 (thr) and bit 1 (tabs)
+        // mark pointers that still hold offsets into the program's aux bytes
+        int32_t aux, pad;
         int64_t c0, c1;  // cell (or recipe) range of the list
         List list;       // work list (list ops)
         const int64_t* rdst;  // recipes
@@ -89,9 +92,10 @@ namespace smr::dev
     };
 
     // Runs stages [0, nstages): stage k is the segments [stage_seg[k],
-    // stage_seg[k+1]); seg_pref[i] is the first global cell of segment i within
-    // its stage.  Stages are separated by grid barriers (cooperative launch)
-    // or, in stage-wise mode, by kernel boundaries.
+    // stage_seg[k+1]); seg_pref is the running cell count over all segments
+    // (segment i covers [seg_pref[i], seg_pref[i+1]), nsegs + 1 entries).
+    // Stages are separated by grid barriers (cooperative launch) or, in
+    // stage-wise mode, by kernel boundaries.
     struct Program
     {
         const Segment* segs;

@@ -1,5 +1,7 @@
 #include "runtime.hpp"
 
+#include <atomic>
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -418,6 +420,12 @@ namespace smr
         return dm;
     }
 
+    DevMesh::DevMesh()
+    {
+        static std::atomic<uint64_t> next{1};
+        serial = next++;
+    }
+
     DevMesh& Mesh::device()
     {
         if (!dev)
@@ -641,6 +649,23 @@ namespace smr
         }
     } // namespace
 
+    dev::Thresholds make_thresholds(const MeshConfig& cfg, int ite, double eps, double regularity)
+    {
+        const int dim = cfg.dim, minl = cfg.min_level, maxl = cfg.max_level;
+        dev::Thresholds thr{};
+        for (int level = minl; level <= maxl - ite; ++level)
+        {
+            const int exponent = dim * (maxl - level);
+            const double eps_l = eps / (1 << exponent);
+            const double reg   = regularity + dim;
+            const double fine  = std::pow(2.0, reg) * eps_l;
+            thr.eps[level]     = eps_l;
+            thr.fine[level]    = fine;
+            thr.coarse[level]  = fine / (1 << dim);
+        }
+        return thr;
+    }
+
     bool harten(AdaptState& st, int ite, double eps, double regularity)
     {
         Field& f        = *st.field;
@@ -669,17 +694,7 @@ namespace smr
         // tag cells: fine levels [min, max-ite], i.e. coarse levels [min-1, max-ite-1]
         {
             ScopedTimer timer(&ctx, "tag cells");
-            dev::Thresholds thr{};
-            for (int level = minl; level <= maxl - ite; ++level)
-            {
-                const int exponent  = dim * (maxl - level);
-                const double eps_l  = eps / (1 << exponent);
-                const double reg    = regularity + dim;
-                const double fine   = std::pow(2.0, reg) * eps_l;
-                thr.eps[level]      = eps_l;
-                thr.fine[level]     = fine;
-                thr.coarse[level]   = fine / (1 << dim);
-            }
+            const dev::Thresholds thr = make_thresholds(cfg, ite, eps, regularity);
             int64_t c0, c1;
             dm.criteria.cells_of_levels(minl - 1, maxl - ite, c0, c1);
             dev::criteria(dim, dm.criteria.d, c0, c1, st.detail, st.tag, thr, minl, maxl, s);

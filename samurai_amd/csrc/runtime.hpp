@@ -74,6 +74,8 @@ namespace smr
     // Device image of one mesh version.
     struct DevMesh
     {
+        DevMesh();
+        uint64_t serial; // unique per image of the process
         std::unique_ptr<Blob> table_blob;
         dev::Table table{};
         DevWorkList cells;
@@ -196,6 +198,9 @@ namespace smr
     std::unique_ptr<DevMesh> upload_image(Ctx& ctx, const MRMeshData& m, HostImage&& h, bool check);
     // DevWorkList upload + resolve against the given tables
     void upload_list(Ctx& ctx, DevWorkList& l, const dev::Table& t0, const dev::Table& t1);
+
+    // criteria thresholds of harten iteration `ite` (mr/adapt.hpp:325-339)
+    dev::Thresholds make_thresholds(const MeshConfig& cfg, int ite, double eps, double regularity);
 
     void update_ghost_mr(Field& f);
     void upwind(const double* a, double dt, Field& u, Field& unp1);

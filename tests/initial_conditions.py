@@ -55,3 +55,20 @@ def seeded_2d(cells, seed, n_boxes=3, n_gauss=2):
         r2 = (c[:, 0] - cx) * (c[:, 0] - cx) + (c[:, 1] - cy) * (c[:, 1] - cy)
         u = u + amp * np.exp(-r2 / (2.0 * sigma * sigma))
     return u
+
+
+def batch_instance(seed, levels=(7, 10)):
+    """Parameters of one instance of the batched 2D workload: max_level drawn in
+    levels (inclusive), a unit velocity at an angle in [0, 2 pi), and 1 to 5
+    shapes split into indicator boxes and Gaussians (drawn as in seeded_2d with
+    the seed `seed + 1000003`).  Returns (max_level, (ax, ay), n_boxes, n_gauss)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    max_level = int(rng.integers(levels[0], levels[1] + 1))
+    theta = rng.uniform(0.0, 2.0 * np.pi)
+    k = int(rng.integers(1, 6))
+    n_boxes = int(rng.integers(0, k + 1))
+    return max_level, (float(np.cos(theta)), float(np.sin(theta))), n_boxes, k - n_boxes
+
+
+def batch_field(cells, seed, n_boxes, n_gauss):
+    return seeded_2d(cells, seed + 1000003, n_boxes, n_gauss)

@@ -100,3 +100,9 @@ def test_programs_in_bounds_along_adaptation(host_ctx, dim, lmin, lmax, s, eps):
         assert m.validate() == 0
         if fixed:
             break
+
+
+def test_batch_needs_a_gpu_context(host_ctx):
+    import samurai_amd as sa
+    with pytest.raises(sa.SamuraiError):
+        sa.Batch(host_ctx, [])
