@@ -33,6 +33,10 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 HBM_PEAK_GBS = 8000.0
 
 
+def progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 class HipEvents:
     """hipEvent timing on an explicit stream (the library's own stream)."""
 
@@ -127,12 +131,14 @@ def bench_batched(sa, ctx, args, eps, reg):
         b.adapt(eps, reg)  # initial adaptation (setup, like the demo before its loop)
         del b
     ctx.synchronize()
+    progress(f"batched: {args.batch} instances set up and adapted ({time.perf_counter() - t_setup:.1f} s)")
     batch = sa.Batch(ctx, [(x["u"], x["unp1"], x["a"], x["dt"]) for x in bset])
     tune_file = sa.default_tuning_file()
     tuned = batch.load_tuning(tune_file)
     if not tuned or args.retune:
         batch.autotune(reps=3, cache_file=tune_file)
     setup = time.perf_counter() - t_setup
+    progress(f"batched: tuning {batch.tuning()}")
 
     for _ in range(args.warmup):
         batch.step(eps, reg)
@@ -144,6 +150,7 @@ def bench_batched(sa, ctx, args, eps, reg):
         updates += batch.nb_cells()
     ctx.synchronize()
     bwall = time.perf_counter() - t0
+    progress(f"batched: {args.batch_steps} steps in {bwall:.2f} s")
 
     adapts = [sa.make_MRAdapt(x["u"]) for x in pset]
 
@@ -167,6 +174,7 @@ def bench_batched(sa, ctx, args, eps, reg):
         pupdates += plain_step()
     ctx.synchronize()
     pwall = time.perf_counter() - t0
+    progress(f"plain loop: {args.batch_steps} steps in {pwall:.2f} s")
     leaves = [x["mesh"].nb_cells() for x in bset]
     return {
         "workload": f"{args.batch} independent 2D advection instances, max_level U{{{levels[0]}..{levels[1]}}}, "
@@ -244,6 +252,7 @@ def main():
         updates += step(times, byts)
     ctx.synchronize()
     wall = time.perf_counter() - t0
+    progress(f"adaptive L4..12: {args.steps} steps in {wall:.2f} s")
     kms = [ev.elapsed_ms(x, y) for x, y in times]
     for x, y in times:
         ev.destroy(x)
