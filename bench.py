@@ -190,6 +190,96 @@ def bench_batched(sa, ctx, args, eps, reg):
     }
 
 
+def bench_adjoint(sa, ctx, args, eps, reg):
+    """Gradients of L = 0.5 sum area (u_T - u_target)^2 after 100 frozen-mesh
+    steps on 32 instances (L2..10, adapted once), w.r.t. u0 and a; u_target is
+    the forward solution from a field of a perturbed seed."""
+    import torch
+    from initial_conditions import seeded_2d
+    n, steps, every, lmax = args.adjoint, args.adjoint_steps, 10, 10
+    theta = math.pi / 5
+    a = np.array([[math.cos(theta), math.sin(theta)]] * n)
+    t_setup = time.perf_counter()
+    inst = []
+    for k in range(n):
+        mesh = sa.make_mesh(ctx, sa.mesh_config(2, 2, lmax, 1))
+        u = sa.make_scalar_field("u", mesh)
+        u.set_cells(seeded_2d(mesh.cells(), 2000 + k))
+        sa.make_bc_dirichlet(u, 0.0)
+        inst.append({"mesh": mesh, "u": u, "unp1": sa.make_scalar_field("unp1", mesh)})
+    b = sa.Batch(ctx, [(x["u"], x["unp1"], a[0], 0.5 * 2.0 ** -lmax) for x in inst])
+    b.adapt(eps, reg)  # adapt once at t=0, then the meshes stay frozen
+    del b
+    prob = sa.AdjointProblem(ctx, [x["mesh"] for x in inst], [0.5 * 2.0 ** -lmax] * n, bc=[0.0] * n, steps=steps,
+                             checkpoint_every=every)
+    prob.set_velocity(a)
+    cells = [x["mesh"].cells() for x in inst]
+    u0 = torch.tensor(np.concatenate([x["u"].cells() for x in inst]), dtype=torch.float64, device="cuda")
+    area = torch.tensor(np.concatenate([np.ldexp(1.0, -2 * c[:, 0].astype(np.int64)) for c in cells]),
+                        dtype=torch.float64, device="cuda")
+    pert = torch.tensor(np.concatenate([seeded_2d(c, 3000 + k) for k, c in enumerate(cells)]), dtype=torch.float64,
+                        device="cuda")
+    target = torch.empty_like(u0)
+    uT = torch.empty_like(u0)
+    torch.cuda.synchronize()
+    prob.forward_ptr(pert.data_ptr(), target.data_ptr())
+    setup = time.perf_counter() - t_setup
+    progress(f"adjoint: {n} frozen instances, {prob.nb_cells()} cells ({setup:.1f} s)")
+
+    def loss_of(u):
+        torch.cuda.synchronize()
+        prob.forward_ptr(u.data_ptr(), uT.data_ptr())
+        return 0.5 * float((area * (uT - target) ** 2).sum())
+
+    g0 = torch.empty_like(u0)
+    loss_of(u0)  # warm-up (programs built)
+    gT = (area * (uT - target)).contiguous()
+    torch.cuda.synchronize()
+    prob.backward_ptr(gT.data_ptr(), g0.data_ptr())
+    reps = args.adjoint_reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        prob.forward_ptr(u0.data_ptr(), uT.data_ptr())
+    t_fwd = (time.perf_counter() - t0) / reps
+    gT = (area * (uT - target)).contiguous()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ga = prob.backward_ptr(gT.data_ptr(), g0.data_ptr())
+    t_bwd = (time.perf_counter() - t0) / reps
+    progress(f"adjoint: forward {t_fwd * 1e3:.1f} ms, backward {t_bwd * 1e3:.1f} ms")
+
+    # central differences, h = 1e-6, along a random direction of u0 and along a[0, 0]
+    h = 1e-6
+    rng = np.random.Generator(np.random.PCG64(11))
+    v = torch.tensor(rng.standard_normal(u0.numel()), dtype=torch.float64, device="cuda")
+    fd_u = (loss_of(u0 + h * v) - loss_of(u0 - h * v)) / (2 * h)
+    ad_u = float((g0 * v).sum())
+    ap, am = a.copy(), a.copy()
+    ap[0, 0] += h
+    am[0, 0] -= h
+    prob.set_velocity(ap)
+    lp = loss_of(u0)
+    prob.set_velocity(am)
+    lm = loss_of(u0)
+    prob.set_velocity(a)
+    fd_a = (lp - lm) / (2 * h)
+    ncells = prob.nb_cells()
+    return {
+        "workload": f"{n} frozen 2D instances L2..{lmax} (seeded 3 boxes + 2 Gaussians, adapted once with eps=1e-4), "
+                    f"{steps} upwind steps, a=(cos pi/5, sin pi/5), checkpoint every {every}, "
+                    f"loss 0.5*sum area*(u_T-u_target)^2",
+        "cells": int(ncells), "steps": steps,
+        "forward_ms": t_fwd * 1e3, "backward_ms": t_bwd * 1e3,
+        "backward_over_forward": t_bwd / t_fwd,
+        "forward_cell_updates_per_s": ncells * steps / t_fwd,
+        "backward_cell_updates_per_s": ncells * steps / t_bwd,
+        "fd_u0_rel_err": abs(fd_u - ad_u) / abs(ad_u), "fd_a_rel_err": abs(fd_a - ga[0, 0]) / abs(ga[0, 0]),
+        "fd_h": h, "tuning": "block 256, 1 cell per thread, cooperative fused programs",
+        "setup_s": setup,
+    }
+
+
 def main():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
@@ -204,6 +294,9 @@ def main():
     p.add_argument("--batch-min-max-level", type=int, default=7)
     p.add_argument("--batch-max-max-level", type=int, default=10)
     p.add_argument("--retune", action="store_true", help="rerun the batch autotuner even if a cached shape exists")
+    p.add_argument("--adjoint", type=int, default=32, help="instances of the adjoint workload (0 disables)")
+    p.add_argument("--adjoint-steps", type=int, default=100)
+    p.add_argument("--adjoint-reps", type=int, default=2)
     args = p.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): this build has no multi-GPU path; "
@@ -300,6 +393,10 @@ def main():
     if args.batch > 0:
         batched = bench_batched(sa, ctx, args, eps, reg)
 
+    adjoint = None
+    if args.adjoint > 0:
+        adjoint = bench_adjoint(sa, ctx, args, eps, reg)
+
     cpu = None
     if args.cpu_budget > 0:
         cpu = cpu_baseline(mesh.cells(), u.cells(), a, dt, eps, reg, args.cpu_budget)
@@ -333,6 +430,7 @@ def main():
         "cpu_baseline": cpu,
         "phases": phases,
         "batched": batched,
+        "adjoint": adjoint,
     }
     print(json.dumps(out), flush=True)
 

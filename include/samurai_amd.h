@@ -28,6 +28,7 @@ typedef struct smr_mesh smr_mesh;
 typedef struct smr_field smr_field;
 typedef struct smr_adapt smr_adapt;
 typedef struct smr_batch smr_batch;
+typedef struct smr_adjoint smr_adjoint;
 
 /* MRMeshId (mr/mesh.hpp:25-34) */
 enum smr_mesh_id
@@ -187,6 +188,33 @@ int smr_batch_get_tuning(const smr_batch* batch, int* block, int* cells_per_thre
 int smr_batch_autotune(smr_batch* batch, int reps, const char* cache_file);
 /* 1 if cache_file holds a launch shape for this GPU architecture (then applied), else 0 */
 int smr_batch_load_tuning(smr_batch* batch, const char* cache_file);
+
+/* ------------------------------------------------------------------------
+ * Adjoint form: gradients of a loss of the cell values after `steps` steps
+ * of update_ghost_mr(u) (algorithm/update_ghost_mr.hpp:194-237) +
+ * unp1 = u - dt * upwind(a, u) (stencil_field.hpp:16-97) + swap, on frozen
+ * meshes, with respect to the initial cell values and the velocity.  The
+ * reference has no adjoint; these entry points differentiate exactly the
+ * operators above.  Many instances (one mesh each) are handled together.
+ * Cell vectors are compact: the cells of each instance in
+ * for_each_cell(mesh[cells]) order, instances concatenated.
+ * ---------------------------------------------------------------------- */
+/* meshes stay frozen while the object lives; dt: n steps; bc_value: n
+ * Dirichlet values (null: no boundary condition); steps must be a multiple of
+ * checkpoint_every (one checkpoint of the state per checkpoint_every steps) */
+int smr_adjoint_create(smr_ctx* ctx, int n, smr_mesh* const* meshes, const double* dt, const double* bc_value, int steps,
+                       int checkpoint_every, smr_adjoint** out);
+int smr_adjoint_destroy(smr_adjoint* adj);
+/* compact cell count of one instance, or of all instances when instance < 0 */
+int64_t smr_adjoint_nb_cells(const smr_adjoint* adj, int instance);
+/* n*3 velocities (third ignored in 2D) */
+int smr_adjoint_set_velocity(smr_adjoint* adj, const double* a);
+/* u0 -> uT; device pointers (compact cell vectors); returns when uT is written */
+int smr_adjoint_forward(smr_adjoint* adj, const double* u0, double* uT);
+/* grad_uT (device) -> grad_u0 (device) and grad_a (host, n*3) of the last forward */
+int smr_adjoint_backward(smr_adjoint* adj, const double* grad_uT, double* grad_u0, double* grad_a);
+/* launch shape of the adjoint's stage programs (as smr_batch_set_tuning) */
+int smr_adjoint_set_tuning(smr_adjoint* adj, int block, int cells_per_thread, int mode);
 
 #ifdef __cplusplus
 }

@@ -95,6 +95,14 @@ def lib():
         L.smr_batch_get_tuning.argtypes = [vp, P(i32), P(i32), P(i32)]
         L.smr_batch_autotune.argtypes = [vp, i32, ctypes.c_char_p]
         L.smr_batch_load_tuning.argtypes = [vp, ctypes.c_char_p]
+        L.smr_adjoint_create.argtypes = [vp, i32, P(vp), P(dbl), P(dbl), i32, i32, ctypes.POINTER(vp)]
+        L.smr_adjoint_destroy.argtypes = [vp]
+        L.smr_adjoint_nb_cells.restype = i64
+        L.smr_adjoint_nb_cells.argtypes = [vp, i32]
+        L.smr_adjoint_set_velocity.argtypes = [vp, P(dbl)]
+        L.smr_adjoint_forward.argtypes = [vp, vp, vp]
+        L.smr_adjoint_backward.argtypes = [vp, vp, vp, P(dbl)]
+        L.smr_adjoint_set_tuning.argtypes = [vp, i32, i32, i32]
         _lib = L
     return _lib
 
@@ -387,3 +395,55 @@ class Batch:
 
     def load_tuning(self, cache_file):
         return bool(_check(lib().smr_batch_load_tuning(self.h, cache_file.encode())))
+
+
+class AdjointProblem:
+    """Frozen-mesh upwind steps of many instances and their adjoint.
+
+    meshes: adapted MRMesh objects (not adapted again while this lives);
+    dt: per-instance time steps; bc: per-instance Dirichlet values (None: no
+    boundary condition).  forward/backward take and return device pointers
+    of compact cell vectors (see samurai_amd.torch_adjoint for tensors)."""
+
+    def __init__(self, ctx, meshes, dt, bc=None, steps=100, checkpoint_every=10):
+        n = len(meshes)
+        vp = ctypes.c_void_p
+        ms = (vp * max(n, 1))(*[m.h for m in meshes])
+        dt = np.ascontiguousarray(np.asarray(dt, dtype=np.float64).reshape(n))
+        bcp = None
+        if bc is not None:
+            bc = np.ascontiguousarray(np.asarray(bc, dtype=np.float64).reshape(n))
+            bcp = _dp(bc)
+        h = vp()
+        _check(lib().smr_adjoint_create(ctx.h, n, ms, _dp(dt), bcp, int(steps), int(checkpoint_every), ctypes.byref(h)))
+        self.ctx, self.meshes, self.h = ctx, list(meshes), h
+        self.n, self.steps, self.checkpoint_every = n, int(steps), int(checkpoint_every)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            lib().smr_adjoint_destroy(self.h)
+            self.h = None
+
+    def nb_cells(self, instance=-1):
+        return _check(lib().smr_adjoint_nb_cells(self.h, int(instance)))
+
+    def offsets(self):
+        """First compact cell of each instance, plus the total."""
+        return np.concatenate([[0], np.cumsum([self.nb_cells(i) for i in range(self.n)])]).astype(np.int64)
+
+    def set_velocity(self, a):
+        a3 = np.zeros((max(self.n, 1), 3))
+        a = np.asarray(a, dtype=np.float64).reshape(self.n, -1)
+        a3[:self.n, :a.shape[1]] = a
+        _check(lib().smr_adjoint_set_velocity(self.h, _dp(a3)))
+
+    def set_tuning(self, block, cells_per_thread, mode):
+        _check(lib().smr_adjoint_set_tuning(self.h, int(block), int(cells_per_thread), int(mode)))
+
+    def forward_ptr(self, u0_ptr, uT_ptr):
+        _check(lib().smr_adjoint_forward(self.h, ctypes.c_void_p(u0_ptr), ctypes.c_void_p(uT_ptr)))
+
+    def backward_ptr(self, gT_ptr, g0_ptr):
+        ga = np.zeros((max(self.n, 1), 3))
+        _check(lib().smr_adjoint_backward(self.h, ctypes.c_void_p(gT_ptr), ctypes.c_void_p(g0_ptr), _dp(ga)))
+        return ga[:self.n]

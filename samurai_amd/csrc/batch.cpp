@@ -22,97 +22,100 @@ namespace smr
         {
             return (n + a - 1) / a * a;
         }
+    } // namespace
 
-        dev::Segment make_seg(int op, const MeshConfig& cfg)
+    dev::Segment make_seg(int op, const MeshConfig& cfg)
+    {
+        dev::Segment g;
+        std::memset(&g, 0, sizeof(g));
+        g.op        = op;
+        g.dim       = cfg.dim;
+        g.pred_s    = cfg.pred_s;
+        g.min_level = cfg.min_level;
+        g.max_level = cfg.max_level;
+        return g;
+    }
+
+    dev::Segment list_seg(int op, const MeshConfig& cfg, const DevWorkList& l, int64_t c0, int64_t c1)
+    {
+        dev::Segment g = make_seg(op, cfg);
+        g.list         = l.d;
+        g.c0           = c0;
+        g.c1           = c1;
+        return g;
+    }
+
+    void push_stage(Chain& ch, const dev::Segment& g)
+    {
+        if (g.c1 > g.c0)
         {
-            dev::Segment g;
-            std::memset(&g, 0, sizeof(g));
-            g.op        = op;
-            g.dim       = cfg.dim;
-            g.pred_s    = cfg.pred_s;
-            g.min_level = cfg.min_level;
-            g.max_level = cfg.max_level;
-            return g;
+            ch.push_back(Stage{g});
         }
+    }
 
-        dev::Segment list_seg(int op, const MeshConfig& cfg, const DevWorkList& l, int64_t c0, int64_t c1)
+    // update_ghost_mr (update_ghost_mr.hpp:194-237) on one instance, in
+    // the order of the standalone launches: per level from the finest, the
+    // outer-ghost recipe stages then the projection to the level below;
+    // then the prediction levels from the coarsest
+    void ghost_chain(Chain& ch, DevMesh& dm, const MeshConfig& cfg, double* u, bool has_bc, double bc)
+    {
+        for (int level = cfg.max_level; level >= 0; --level)
         {
-            dev::Segment g = make_seg(op, cfg);
-            g.list         = l.d;
-            g.c0           = c0;
-            g.c1           = c1;
-            return g;
-        }
-
-        void push_stage(Chain& ch, const dev::Segment& g)
-        {
-            if (g.c1 > g.c0)
+            DevRecipes& r  = dm.outer[size_t(level)];
+            const auto& sp = r.host.stage_ptr;
+            if (!sp.empty() && size_t(sp.back()) > r.host.size())
             {
-                ch.push_back(Stage{g});
+                throw std::logic_error("recipe stages exceed the recipe list");
             }
-        }
-
-        // update_ghost_mr (update_ghost_mr.hpp:194-237) on one instance, in
-        // the order of the standalone launches: per level from the finest, the
-        // outer-ghost recipe stages then the projection to the level below;
-        // then the prediction levels from the coarsest
-        void ghost_chain(Chain& ch, DevMesh& dm, const MeshConfig& cfg, Field& f)
-        {
-            for (int level = cfg.max_level; level >= 0; --level)
+            for (size_t k = 0; k + 1 < sp.size(); ++k)
             {
-                DevRecipes& r  = dm.outer[size_t(level)];
-                const auto& sp = r.host.stage_ptr;
-                if (!sp.empty() && size_t(sp.back()) > r.host.size())
-                {
-                    throw std::logic_error("recipe stages exceed the recipe list");
-                }
-                for (size_t k = 0; k + 1 < sp.size(); ++k)
-                {
-                    dev::Segment g = make_seg(dev::SEG_RECIPE, cfg);
-                    g.rdst         = r.dst;
-                    g.rop          = r.op;
-                    g.rsp          = r.sp;
-                    g.rsrc         = r.src;
-                    g.c0           = sp[k];
-                    g.c1           = sp[k + 1];
-                    g.u            = f.d;
-                    g.has_bc       = f.has_bc ? 1 : 0;
-                    g.bc           = f.bc_value;
-                    push_stage(ch, g);
-                }
-                if (level > 0)
-                {
-                    int64_t c0, c1;
-                    dm.projection.cells_of_levels(level - 1, level, c0, c1);
-                    dev::Segment g = list_seg(dev::SEG_PROJECT, cfg, dm.projection, c0, c1);
-                    g.u            = f.d;
-                    push_stage(ch, g);
-                }
+                dev::Segment g = make_seg(dev::SEG_RECIPE, cfg);
+                g.rdst         = r.dst;
+                g.rop          = r.op;
+                g.rsp          = r.sp;
+                g.rsrc         = r.src;
+                g.c0           = sp[k];
+                g.c1           = sp[k + 1];
+                g.u            = u;
+                g.has_bc       = has_bc ? 1 : 0;
+                g.bc           = bc;
+                push_stage(ch, g);
             }
-            for (int level = 1; level <= cfg.max_level; ++level)
+            if (level > 0)
             {
                 int64_t c0, c1;
-                dm.prediction.cells_of_levels(level, level + 1, c0, c1);
-                dev::Segment g = list_seg(dev::SEG_PREDICT_FINE, cfg, dm.prediction, c0, c1);
-                g.u            = f.d;
+                dm.projection.cells_of_levels(level - 1, level, c0, c1);
+                dev::Segment g = list_seg(dev::SEG_PROJECT, cfg, dm.projection, c0, c1);
+                g.u            = u;
                 push_stage(ch, g);
             }
         }
-
-        // stage k of the result is stage k of a and of b
-        Chain zip(Chain a, const Chain& b)
+        for (int level = 1; level <= cfg.max_level; ++level)
         {
-            if (a.size() < b.size())
-            {
-                a.resize(b.size());
-            }
-            for (size_t k = 0; k < b.size(); ++k)
-            {
-                a[k].insert(a[k].end(), b[k].begin(), b[k].end());
-            }
-            return a;
+            int64_t c0, c1;
+            dm.prediction.cells_of_levels(level, level + 1, c0, c1);
+            dev::Segment g = list_seg(dev::SEG_PREDICT_FINE, cfg, dm.prediction, c0, c1);
+            g.u            = u;
+            push_stage(ch, g);
         }
+    }
 
+    // stage k of the result is stage k of a and of b
+    Chain zip(Chain a, const Chain& b)
+    {
+        if (a.size() < b.size())
+        {
+            a.resize(b.size());
+        }
+        for (size_t k = 0; k < b.size(); ++k)
+        {
+            a[k].insert(a[k].end(), b[k].begin(), b[k].end());
+        }
+        return a;
+    }
+
+    namespace
+    {
         std::string gpu_arch(int device)
         {
             hipDeviceProp_t prop;
@@ -155,7 +158,16 @@ namespace smr
                 case dev::SEG_UPWIND:
                 case dev::SEG_COPY:
                 case dev::SEG_PROJECT_TO:
-                case dev::SEG_PREDICT_CHILDREN: ok = ok && g.list.iv && g.c1 <= g.list.ncells && g.u && g.out; break;
+                case dev::SEG_PREDICT_CHILDREN:
+                case dev::SEG_CELLS_TO_COMPACT: ok = ok && g.list.iv && g.c1 <= g.list.ncells && g.u && g.out; break;
+                case dev::SEG_CELLS_FROM_COMPACT: ok = ok && g.list.iv && g.c1 <= g.list.ncells && g.in2 && g.out; break;
+                case dev::SEG_COPY_RAW: ok = ok && g.u && g.out; break;
+                case dev::SEG_ADJ_GATHER: ok = ok && g.rdst && g.rsp && g.rsrc && g.thr && g.out; break;
+                case dev::SEG_ADJ_ZERO: ok = ok && g.rdst && g.out; break;
+                case dev::SEG_ADJ_UPWIND: ok = ok && g.list.iv && g.c1 <= g.list.ncells && g.u && g.out && g.in2 && g.out2; break;
+                case dev::SEG_ADJ_SCATTER:
+                    ok = ok && g.list.iv && g.c1 <= g.list.ncells && g.u && g.out && g.pad >= 0 && g.pad < 2 * g.dim;
+                    break;
                 default: ok = false;
             }
             ok = ok && (g.dim == 2 || g.dim == 3) && (g.pred_s == 0 || g.pred_s == 1);
@@ -597,7 +609,7 @@ namespace smr
                     ti.tag          = tag;
                     pre[1]          = {ti};
                     Chain g;
-                    ghost_chain(g, dm, c, f);
+                    ghost_chain(g, dm, c, f.d, f.has_bc, f.bc_value);
                     Chain ch = zip(pre, g);
                     int64_t c0, c1;
                     dm.detail.cells_of_levels(c.min_level > 0 ? c.min_level - 1 : 0, c.max_level - ite, c0, c1);
@@ -821,7 +833,7 @@ namespace smr
             DevMesh& dm         = *b.u->mesh->dev;
             const MeshConfig& c = b.u->mesh->host.cfg;
             Chain ch;
-            ghost_chain(ch, dm, c, *b.u);
+            ghost_chain(ch, dm, c, b.u->d, b.u->has_bc, b.u->bc_value);
             dev::Segment g = list_seg(dev::SEG_UPWIND, c, dm.cells, 0, dm.cells.d.ncells);
             g.u            = b.u->d;
             g.out          = b.unp1->d;

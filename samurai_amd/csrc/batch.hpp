@@ -32,6 +32,17 @@ namespace smr
     using Stage = std::vector<dev::Segment>;
     using Chain = std::vector<Stage>;
 
+    // segment of operator `op` with the mesh configuration filled in
+    dev::Segment make_seg(int op, const MeshConfig& cfg);
+    // segment over cells [c0, c1) of a work list
+    dev::Segment list_seg(int op, const MeshConfig& cfg, const DevWorkList& l, int64_t c0, int64_t c1);
+    // appends g as a stage of its own when it is not empty
+    void push_stage(Chain& ch, const dev::Segment& g);
+    // update_ghost_mr of storage u as a chain of stages
+    void ghost_chain(Chain& ch, DevMesh& dm, const MeshConfig& cfg, double* u, bool has_bc, double bc);
+    // stage k of the result holds stage k of a and of b
+    Chain zip(Chain a, const Chain& b);
+
     // A stage program being assembled on the host and its device copy.
     class ProgramBuf
     {

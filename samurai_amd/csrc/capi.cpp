@@ -1,5 +1,6 @@
 // extern "C" boundary (include/samurai_amd.h) over the runtime.
 #include "../../include/samurai_amd.h"
+#include "adjoint.hpp"
 #include "batch.hpp"
 #include "runtime.hpp"
 
@@ -25,6 +26,10 @@ struct smr_adapt
 struct smr_batch
 {
     std::unique_ptr<smr::Batch> b;
+};
+struct smr_adjoint
+{
+    std::unique_ptr<smr::Adjoint> a;
 };
 
 namespace
@@ -775,6 +780,102 @@ extern "C"
                 }
                 batch->b->set_tuning(t);
                 return 1;
+            });
+    }
+
+    int smr_adjoint_create(smr_ctx* ctx, int n, smr_mesh* const* meshes, const double* dt, const double* bc_value, int steps,
+                           int checkpoint_every, smr_adjoint** out)
+    {
+        return guarded(
+            [&]
+            {
+                require(ctx && out && n >= 0 && (n == 0 || (meshes && dt)), "null argument");
+                ctx->c.require_device();
+                std::vector<smr::Mesh*> ms(static_cast<size_t>(n));
+                for (int i = 0; i < n; ++i)
+                {
+                    require(meshes[i] != nullptr, "null mesh");
+                    ms[size_t(i)] = &meshes[i]->m;
+                }
+                auto* r = new smr_adjoint();
+                try
+                {
+                    r->a = std::make_unique<smr::Adjoint>(&ctx->c, std::move(ms), dt, bc_value, steps, checkpoint_every);
+                }
+                catch (...)
+                {
+                    delete r;
+                    throw;
+                }
+                *out = r;
+                return 0;
+            });
+    }
+
+    int smr_adjoint_destroy(smr_adjoint* adj)
+    {
+        return guarded(
+            [&]
+            {
+                delete adj;
+                return 0;
+            });
+    }
+
+    int64_t smr_adjoint_nb_cells(const smr_adjoint* adj, int instance)
+    {
+        if (!adj || instance >= adj->a->ninstances())
+        {
+            g_error = "bad adjoint or instance";
+            return -1;
+        }
+        return adj->a->nb_cells(instance);
+    }
+
+    int smr_adjoint_set_velocity(smr_adjoint* adj, const double* a)
+    {
+        return guarded(
+            [&]
+            {
+                require(adj && (a || adj->a->ninstances() == 0), "null argument");
+                adj->a->set_velocity(a);
+                return 0;
+            });
+    }
+
+    int smr_adjoint_forward(smr_adjoint* adj, const double* u0, double* uT)
+    {
+        return guarded(
+            [&]
+            {
+                require(adj && u0 && uT, "null argument");
+                adj->a->forward(u0, uT);
+                return 0;
+            });
+    }
+
+    int smr_adjoint_backward(smr_adjoint* adj, const double* grad_uT, double* grad_u0, double* grad_a)
+    {
+        return guarded(
+            [&]
+            {
+                require(adj && grad_uT && grad_u0 && grad_a, "null argument");
+                adj->a->backward(grad_uT, grad_u0, grad_a);
+                return 0;
+            });
+    }
+
+    int smr_adjoint_set_tuning(smr_adjoint* adj, int block, int cells_per_thread, int mode)
+    {
+        return guarded(
+            [&]
+            {
+                require(adj != nullptr, "null adjoint");
+                require(block >= 64 && block <= 1024 && block % 64 == 0, "block size must be a multiple of 64 in [64, 1024]");
+                require(cells_per_thread >= 1 && cells_per_thread <= 64, "cells per thread must be in [1, 64]");
+                require(mode >= 0 && mode <= 2, "unknown program mode");
+                adj->a->set_tuning(smr::Tuning{block, cells_per_thread, mode});
+                return 0;
             });
     }
 }

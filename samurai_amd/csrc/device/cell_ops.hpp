@@ -534,4 +534,96 @@ namespace smr::dev
         }
         u[dst[r]] = v;
     }
+    // ------------------------------------------------------------------
+    // Adjoint of the upwind step on a frozen mesh.  With
+    //   F(a, ul, ur) = a+ ul + a- ur   (a+ = max(a, 0), a- = min(a, 0)),
+    // unp1[c] = u[c] - k S, k = dt 2^level, is linear in u:
+    //   d unp1[c] / d u[c]      = 1 - k sum_d |a_d|
+    //   d unp1[c] / d u[c - e_d] =  k a_d+,   d unp1[c] / d u[c + e_d] = -k a_d-
+    // so the transpose at a cell s gathers lam from its face neighbours
+    // (lam is zero outside the cells).  The velocity derivative uses
+    //   dF/da = .5 (ul + ur) + .5 sign(a) (ul - ur).
+    // ------------------------------------------------------------------
+    __device__ __forceinline__ double dflux_da(double a, double ul, double ur)
+    {
+        const double sg = a > 0 ? 1.0 : (a < 0 ? -1.0 : 0.0);
+        return .5 * (ul + ur) + .5 * sg * (ul - ur);
+    }
+
+    template <int DIM>
+    __device__ __forceinline__ void op_adj_upwind(const List& l, int64_t c, const double* lam, double* mu, const double* un, double* ga,
+                                                  double a0, double a1, double a2, double dt)
+    {
+        const Cell cell  = locate(l, c);
+        const int64_t* b = bases(l, cell.run);
+        const int32_t x  = cell.x;
+        const double k   = dt * ldexp(1.0, cell.level);
+        const int64_t i  = b[0] + x;
+        double self      = fabs(a0) + fabs(a1);
+        if constexpr (DIM == 3)
+        {
+            self += fabs(a2);
+        }
+        double m = (1.0 - k * self) * lam[i];
+        m += k * fmax(a0, 0.0) * lam[i + 1] - k * fmin(a0, 0.0) * lam[i - 1];
+        m += k * fmax(a1, 0.0) * lam[b[2] + x] - k * fmin(a1, 0.0) * lam[b[1] + x];
+        if constexpr (DIM == 3)
+        {
+            m += k * fmax(a2, 0.0) * lam[b[4] + x] - k * fmin(a2, 0.0) * lam[b[3] + x];
+        }
+        mu[i] = m;
+        const double li = lam[i];
+        const double uc = un[i];
+        ga[DIM * c + 0] += -k * li * (dflux_da(a0, uc, un[i + 1]) - dflux_da(a0, un[i - 1], uc));
+        ga[DIM * c + 1] += -k * li * (dflux_da(a1, uc, un[b[2] + x]) - dflux_da(a1, un[b[1] + x], uc));
+        if constexpr (DIM == 3)
+        {
+            ga[DIM * c + 2] += -k * li * (dflux_da(a2, uc, un[b[4] + x]) - dflux_da(a2, un[b[3] + x], uc));
+        }
+    }
+
+    // dir: 0 x-1, 1 x+1, 2 y-1, 3 y+1, 4 z-1, 5 z+1 (the neighbour is a ghost)
+    template <int DIM>
+    __device__ __forceinline__ void op_adj_scatter(const List& l, int64_t c, int dir, const double* lam, double* mu, double a0, double a1,
+                                                   double a2, double dt)
+    {
+        const Cell cell  = locate(l, c);
+        const int64_t* b = bases(l, cell.run);
+        const int32_t x  = cell.x;
+        const double k   = dt * ldexp(1.0, cell.level);
+        const double li  = lam[b[0] + x];
+        switch (dir)
+        {
+            case 0: mu[b[0] + x - 1] += k * fmax(a0, 0.0) * li; break;
+            case 1: mu[b[0] + x + 1] += -k * fmin(a0, 0.0) * li; break;
+            case 2: mu[b[1] + x] += k * fmax(a1, 0.0) * li; break;
+            case 3: mu[b[2] + x] += -k * fmin(a1, 0.0) * li; break;
+            case 4: mu[b[3] + x] += k * fmax(a2, 0.0) * li; break;
+            case 5: mu[b[4] + x] += -k * fmin(a2, 0.0) * li; break;
+            default: break;
+        }
+    }
+
+    __device__ __forceinline__ void op_adj_gather(const int64_t* __restrict__ tgt, const int64_t* __restrict__ ptr,
+                                                  const int64_t* __restrict__ src, const double* __restrict__ w, int64_t r, double* mu)
+    {
+        double acc = 0;
+        for (int64_t k = ptr[r]; k < ptr[r + 1]; ++k)
+        {
+            acc += w[k] * mu[src[k]];
+        }
+        mu[tgt[r]] += acc;
+    }
+
+    __device__ __forceinline__ void op_cells_to_compact(const List& l, int64_t c, double* out, const double* u)
+    {
+        const Cell cell = locate(l, c);
+        out[c]          = u[bases(l, cell.run)[0] + cell.x];
+    }
+
+    __device__ __forceinline__ void op_cells_from_compact(const List& l, int64_t c, double* u, const double* in)
+    {
+        const Cell cell                        = locate(l, c);
+        u[bases(l, cell.run)[0] + cell.x] = in[c];
+    }
 } // namespace smr::dev

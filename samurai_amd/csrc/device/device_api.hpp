@@ -66,7 +66,26 @@ namespace smr::dev
         SEG_RESOLVE = 11,
         // zero fills of out[c] / tag[c]
         SEG_FILL_F64 = 12,
-        SEG_FILL_U8  = 13
+        SEG_FILL_U8  = 13,
+        // out[c] = u[c]
+        SEG_COPY_RAW = 14,
+        // cells of a list between storage and a compact vector (list order):
+        // to: out[c] = u[storage(c)]; from: out[storage(c)] = in2[c]
+        SEG_CELLS_TO_COMPACT   = 15,
+        SEG_CELLS_FROM_COMPACT = 16,
+        // adjoint of a linear stage, as gathers over its transpose (CSR rows r:
+        // target rdst[r], entries rsp[r]..rsp[r+1] of (rsrc, thr)):
+        // out[rdst[r]] += sum thr[k] * out[rsrc[k]]
+        SEG_ADJ_GATHER = 17,
+        // out[rdst[r]] = 0 (the outputs of the stage whose adjoint was gathered)
+        SEG_ADJ_ZERO = 18,
+        // adjoint of the upwind step on the cells list: out = transposed
+        // stencil applied to u (zero outside cells); velocity-gradient terms
+        // accumulated into out2[dim * c + d] using the forward state in2
+        SEG_ADJ_UPWIND = 19,
+        // the part of the transposed stencil that lands on ghost neighbours:
+        // cells of the list whose neighbour in direction `pad` is a ghost
+        SEG_ADJ_SCATTER = 20
     };
 
     struct Segment
@@ -88,6 +107,8 @@ namespace smr::dev
         const double* thr; // eps/fine/coarse per fine level (criteria)
         const Table* tabs; // resolve: the two tables of the list's taps
         int* err;          // resolve: set to 1 when a pair fails
+        const double* in2; // second input (adjoint: forward state; compact input)
+        double* out2;      // second output (adjoint: velocity-gradient terms)
         double a0, a1, a2, dt, bc;
     };
 

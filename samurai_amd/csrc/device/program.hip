@@ -55,6 +55,10 @@ namespace smr::dev
                         op_predict_children<DIM, 1>(g.list, c, g.out, g.u);
                     }
                     break;
+                case SEG_CELLS_TO_COMPACT: op_cells_to_compact(g.list, c, g.out, g.u); break;
+                case SEG_CELLS_FROM_COMPACT: op_cells_from_compact(g.list, c, g.out, g.in2); break;
+                case SEG_ADJ_UPWIND: op_adj_upwind<DIM>(g.list, c, g.u, g.out, g.in2, g.out2, g.a0, g.a1, g.a2, g.dt); break;
+                case SEG_ADJ_SCATTER: op_adj_scatter<DIM>(g.list, c, g.pad, g.u, g.out, g.a0, g.a1, g.a2, g.dt); break;
                 case SEG_UPWIND:
                 {
                     const Cell cell  = locate(g.list, c);
@@ -84,6 +88,18 @@ namespace smr::dev
             else if (g.op == SEG_FILL_U8)
             {
                 g.tag[c] = 0;
+            }
+            else if (g.op == SEG_COPY_RAW)
+            {
+                g.out[c] = g.u[c];
+            }
+            else if (g.op == SEG_ADJ_GATHER)
+            {
+                op_adj_gather(g.rdst, g.rsp, g.rsrc, g.thr, c, g.out);
+            }
+            else if (g.op == SEG_ADJ_ZERO)
+            {
+                g.out[g.rdst[c]] = 0.0;
             }
             else if (g.dim == 2)
             {
