@@ -459,12 +459,17 @@ namespace smr
         }
     }
 
-    void Batch::check_instances() const
+    void Batch::check_instances()
     {
-        for (const BatchInstance& b : inst_)
+        for (BatchInstance& b : inst_)
         {
             b.u->check_current();
-            b.unp1->check_current();
+            // unp1 is the step's output: resized (zero-filled) here when stale, as
+            // the demo loop does with unp1.resize() (advection_2d.cpp:143)
+            if (b.unp1->version != b.unp1->mesh->version || !b.unp1->d || b.unp1->size != b.unp1->mesh->host.nref)
+            {
+                b.unp1->allocate(true);
+            }
         }
     }
 

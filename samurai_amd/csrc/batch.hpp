@@ -132,7 +132,7 @@ namespace smr
         // blobs whose pageable host staging must outlive the copies queued from it
         std::vector<std::unique_ptr<Blob>> pending_;
 
-        void check_instances() const;
+        void check_instances();
         void ensure_images(const std::vector<int>& ids);
         void ensure_arenas(int64_t n);
         ProgramBuf& fv_program();
