@@ -16,8 +16,22 @@ namespace smr
             double w;
         };
 
-        LinearStage make_stage(std::vector<int64_t>&& dst, std::vector<Triplet>& t)
+        LinearStage make_stage(std::vector<int64_t>&& dst, std::vector<Triplet>& t, int64_t nref)
         {
+            for (int64_t d : dst)
+            {
+                if (d < 0 || d >= nref)
+                {
+                    throw std::out_of_range("adjoint: ghost output outside the field storage");
+                }
+            }
+            for (const Triplet& e : t)
+            {
+                if (e.src < 0 || e.src >= nref || e.dst < 0 || e.dst >= nref)
+                {
+                    throw std::out_of_range("adjoint: ghost stencil outside the field storage");
+                }
+            }
             LinearStage st;
             st.dst = std::move(dst);
             std::sort(t.begin(), t.end(), [](const Triplet& a, const Triplet& b) { return a.src != b.src ? a.src < b.src : a.dst < b.dst; });
@@ -115,7 +129,7 @@ namespace smr
                 }
                 if (!dst.empty())
                 {
-                    out.push_back(make_stage(std::move(dst), t));
+                    out.push_back(make_stage(std::move(dst), t, m.nref));
                 }
             }
             if (level > 0)
@@ -153,7 +167,7 @@ namespace smr
                 }
                 if (!dst.empty())
                 {
-                    out.push_back(make_stage(std::move(dst), t));
+                    out.push_back(make_stage(std::move(dst), t, m.nref));
                 }
             }
         }
@@ -213,7 +227,7 @@ namespace smr
             }
             if (!dst.empty())
             {
-                out.push_back(make_stage(std::move(dst), t));
+                out.push_back(make_stage(std::move(dst), t, m.nref));
             }
         }
         return out;
