@@ -369,25 +369,53 @@ def main():
     final_cells = mesh.nb_cells()
 
     # ---------------- uniform level-12 roofline check ----------------
-    umesh, uu, uunp1 = make_problem(sa, ctx, 12, 12, args.seed)
-    sa.update_ghost_mr(uu)
-    sa.upwind(a, dt, uu, uunp1)
-    ctx.synchronize()
-    uk = []
-    for _ in range(args.uniform_reps):
+    # Four (u, unp1) pairs on the uniform mesh, each ghost-updated once; the
+    # timed region is back-to-back upwind launches cycling through the pairs,
+    # so every launch reads a u last touched >= 1 GB of traffic earlier (no
+    # reuse through the 256 MB last-level cache).
+    from initial_conditions import seeded_2d
+    umesh = sa.make_mesh(ctx, sa.mesh_config(2, 12, 12, 1))
+    uvals = seeded_2d(umesh.cells(), args.seed)
+    pairs = []
+    for _ in range(4):
+        uu = sa.make_scalar_field("u", umesh)
+        uu.set_cells(uvals)
+        sa.make_bc_dirichlet(uu, 0.0)
         sa.update_ghost_mr(uu)
-        e0, e1 = ev.create(), ev.create()
-        ev.record(e0, stream)
-        sa.upwind(a, dt, uu, uunp1)
-        ev.record(e1, stream)
-        uk.append((e0, e1))
-        sa.swap(uu, uunp1)
+        pairs.append((uu, sa.make_scalar_field("unp1", umesh)))
+    for uu, vv in pairs:
+        sa.upwind(a, dt, uu, vv)
     ctx.synchronize()
-    ums = [ev.elapsed_ms(x, y) for x, y in uk]
+    e0, e1 = ev.create(), ev.create()
+    ev.record(e0, stream)
+    for _ in range(args.uniform_reps):
+        for uu, vv in pairs:
+            sa.upwind(a, dt, uu, vv)
+    ev.record(e1, stream)
+    u_ms = ev.elapsed_ms(e0, e1) / (args.uniform_reps * len(pairs))
     ubytes = umesh.upwind_bytes()
-    u_ms = float(np.mean(ums))
     uniform_gbs = ubytes / (u_ms * 1e-3) / 1e9
     uniform_cups = umesh.nb_cells() / (u_ms * 1e-3)
+    del pairs
+
+    # measured copy bandwidth (device-to-device copy of the same 2 x 134 MB per launch)
+    import torch
+    bufs = [(torch.ones(1 << 24, dtype=torch.float64, device="cuda"), torch.empty(1 << 24, dtype=torch.float64, device="cuda"))
+            for _ in range(4)]
+    for x, y in bufs:
+        y.copy_(x)
+    torch.cuda.synchronize()
+    c0, c1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    c0.record()
+    for _ in range(args.uniform_reps):
+        for x, y in bufs:
+            y.copy_(x)
+    c1.record()
+    c1.synchronize()
+    copy_ms = c0.elapsed_time(c1) / (args.uniform_reps * len(bufs))
+    copy_gbs = 2 * 8 * (1 << 24) / (copy_ms * 1e-3) / 1e9
+    del bufs
+    progress(f"uniform L12 upwind {u_ms * 1e3:.1f} us ({uniform_gbs:.0f} GB/s), copy {copy_gbs:.0f} GB/s")
 
     batched = None
     if args.batch > 0:
@@ -421,9 +449,10 @@ def main():
                    "mean_leaves": updates / args.steps, "final_leaves": final_cells},
         "roofline": {"bound": "hbm", "achieved": uniform_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": uniform_gbs / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_upwind<2> on the uniform level-12 mesh (16,777,216 leaves)",
+                     "kernel": "upwind (wave-tile kernel k_upwind_tiles<2>) on the uniform level-12 mesh (16,777,216 leaves)",
                      "bytes_per_launch": ubytes, "avg_launch_ms": u_ms,
-                     "cell_updates_per_s": uniform_cups},
+                     "cell_updates_per_s": uniform_cups, "measured_copy_GBs": copy_gbs,
+                     "frac_of_measured_copy": uniform_gbs / copy_gbs},
         "roofline_adaptive": {"kernel": "k_upwind<2> inside the timed adaptive steps", "bytes_per_launch": upw_bytes,
                               "avg_launch_ms": upw_ms, "achieved": adaptive_gbs,
                               "frac": adaptive_gbs / HBM_PEAK_GBS},

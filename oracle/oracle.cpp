@@ -1177,6 +1177,227 @@ namespace oracle
         s.u.swap(unp1);
     }
 
+    // maximal x-runs of a set in traversal order (z, y ascending, then x)
+    template <class F>
+    void for_each_run(const DSet& s, F&& f)
+    {
+        if (s.box_empty())
+        {
+            return;
+        }
+        for (int z = s.lo[2]; z < s.hi[2]; ++z)
+        {
+            for (int y = s.lo[1]; y < s.hi[1]; ++y)
+            {
+                int x = s.lo[0];
+                while (x < s.hi[0])
+                {
+                    if (!s.get(x, y, z))
+                    {
+                        ++x;
+                        continue;
+                    }
+                    int e = x;
+                    while (e < s.hi[0] && s.get(e, y, z))
+                    {
+                        ++e;
+                    }
+                    f(x, e, y, z);
+                    x = e;
+                }
+            }
+        }
+    }
+
+    // h_factor (flux_based_scheme__lin_hom.hpp:62-67)
+    inline double h_factor(int dim, double h_face, double h_cell)
+    {
+        return std::pow(h_face, dim - 1) / std::pow(h_cell, dim);
+    }
+
+    // make_convection_upwind(v)(u) (schemes/fv/operators/convection_lin.hpp:15-89)
+    // applied explicitly (FV_scheme.hpp:202-210): the output is zero-filled
+    // (explicit_FV_scheme.hpp:51-56); then for each direction d the interior
+    // interfaces -- same level for every level, then the level jumps in the
+    // direction and in the opposite direction (flux_based_scheme__lin_hom.hpp:74-159,
+    // interface.hpp:35-315) -- and the boundary interfaces of every level
+    // (flux_based_scheme__lin_hom.hpp:166-207, interface.hpp:440-509),
+    // accumulated in that order as explicit_flux_based_scheme__lin_hom.hpp:39-119,
+    // 233-319 does (per stencil cell c: the left cells of an interval, then its
+    // right cells; at a jump the coarse side adds a pair of fine products).
+    std::vector<double> convection_upwind(const Sim& s, const double* v)
+    {
+        const Mesh& m = s.mesh;
+        const int dim = m.dim;
+        std::vector<double> out(size_t(m.nref), 0.0);
+        auto idx = [&](int l, int x, int y, int z) { return size_t(m.index(l, x, y, z)); };
+        const auto& cells = m.ids[CELLS];
+        const int cmin = m.ca_min_level(CELLS), cmax = m.ca_max_level(CELLS);
+        const int minl = std::max(m.min_level, cmin - 1), maxl = std::min(m.max_level, cmax + 1);
+        for (int d = 0; d < dim; ++d)
+        {
+            std::array<int, 3> e{0, 0, 0};
+            e[size_t(d)] = 1;
+            const std::array<int, 3> me{-e[0], -e[1], -e[2]};
+            // coefficients of the stencil {left cell, right cell} (convection_lin.hpp:36-77)
+            const double fc[2] = {v[d] >= 0 ? v[d] : 0.0, v[d] >= 0 ? 0.0 : v[d]};
+            // same level
+            for (int level = minl; level <= maxl; ++level)
+            {
+                const double h  = 1.0 / (1 << level);
+                const double fa = h_factor(dim, h, h);
+                double L[2], R[2];
+                for (int c = 0; c < 2; ++c)
+                {
+                    L[c] = fa * fc[c];
+                    R[c] = -L[c];
+                }
+                const DSet I = intersection(cells[size_t(level)], translate(cells[size_t(level)], me));
+                for_each_run(I,
+                             [&](int x0, int x1, int y, int z)
+                             {
+                                 for (int c = 0; c < 2; ++c)
+                                 {
+                                     for (int x = x0; x < x1; ++x)
+                                     {
+                                         out[idx(level, x, y, z)] += L[c] * s.u[idx(level, x + c * e[0], y + c * e[1], z + c * e[2])];
+                                     }
+                                     for (int x = x0; x < x1; ++x)
+                                     {
+                                         out[idx(level, x + e[0], y + e[1], z + e[2])] +=
+                                             R[c] * s.u[idx(level, x + c * e[0], y + c * e[1], z + c * e[2])];
+                                     }
+                                 }
+                             });
+            }
+            // level jumps l -- l+1
+            for (int level = minl; level < maxl; ++level)
+            {
+                if (level >= m.max_level)
+                {
+                    break;
+                }
+                const int lf       = level + 1;
+                const double hl    = 1.0 / (1 << level);
+                const double hf    = 1.0 / (1 << lf);
+                const int zsh      = dim == 3 ? 1 : 0;
+                auto coarse        = [&](int x, int y, int z) { return idx(level, x >> 1, y >> 1, z >> zsh); };
+                auto fine_u        = [&](int x, int y, int z) { return s.u[idx(lf, x, y, z)]; };
+                // coarse on the left, fine on the right (interface.hpp:123-203)
+                {
+                    const double lfac = h_factor(dim, hf, hl), rfac = h_factor(dim, hf, hf);
+                    double L[2], R[2];
+                    for (int c = 0; c < 2; ++c)
+                    {
+                        L[c] = lfac * fc[c];
+                        R[c] = -rfac * fc[c];
+                    }
+                    const DSet J = on(intersection(cells[size_t(level)], translate(cells[size_t(lf)], me)), lf);
+                    for_each_run(J,
+                                 [&](int x0, int x1, int y, int z)
+                                 {
+                                     const int n = x1 - x0;
+                                     for (int c = 0; c < 2; ++c)
+                                     {
+                                         const int cx = c * e[0], cy = c * e[1], cz = c * e[2];
+                                         if (n == 1)
+                                         {
+                                             out[coarse(x0, y, z)] += L[c] * fine_u(x0 + cx, y + cy, z + cz);
+                                             out[idx(lf, x0 + e[0], y + e[1], z + e[2])] += R[c] * fine_u(x0 + cx, y + cy, z + cz);
+                                         }
+                                         else
+                                         {
+                                             for (int i = 0; i < n / 2; ++i)
+                                             {
+                                                 const int xa = x0 + 2 * i;
+                                                 out[coarse(xa, y, z)] += L[c] * fine_u(xa + cx, y + cy, z + cz) + L[c] * fine_u(xa + 1 + cx, y + cy, z + cz);
+                                             }
+                                             for (int x = x0; x < x1; ++x)
+                                             {
+                                                 out[idx(lf, x + e[0], y + e[1], z + e[2])] += R[c] * fine_u(x + cx, y + cy, z + cz);
+                                             }
+                                         }
+                                     }
+                                 });
+                }
+                // fine on the left, coarse on the right (interface.hpp:205-315)
+                {
+                    const double lfac = h_factor(dim, hf, hf), rfac = h_factor(dim, hf, hl);
+                    double L[2], R[2];
+                    for (int c = 0; c < 2; ++c)
+                    {
+                        L[c] = lfac * fc[c];
+                        R[c] = -rfac * fc[c];
+                    }
+                    const DSet J = on(intersection(cells[size_t(level)], translate(cells[size_t(lf)], e)), lf);
+                    for_each_run(J,
+                                 [&](int x0, int x1, int y, int z)
+                                 {
+                                     const int n = x1 - x0;
+                                     for (int c = 0; c < 2; ++c)
+                                     {
+                                         // stencil {p - e, p}
+                                         const int cx = (c - 1) * e[0], cy = (c - 1) * e[1], cz = (c - 1) * e[2];
+                                         if (n == 1)
+                                         {
+                                             out[idx(lf, x0 - e[0], y - e[1], z - e[2])] += L[c] * fine_u(x0 + cx, y + cy, z + cz);
+                                             out[coarse(x0, y, z)] += R[c] * fine_u(x0 + cx, y + cy, z + cz);
+                                         }
+                                         else
+                                         {
+                                             for (int x = x0; x < x1; ++x)
+                                             {
+                                                 out[idx(lf, x - e[0], y - e[1], z - e[2])] += L[c] * fine_u(x + cx, y + cy, z + cz);
+                                             }
+                                             for (int i = 0; i < n / 2; ++i)
+                                             {
+                                                 const int xa = x0 + 2 * i;
+                                                 out[coarse(xa, y, z)] += R[c] * fine_u(xa + cx, y + cy, z + cz) + R[c] * fine_u(xa + 1 + cx, y + cy, z + cz);
+                                             }
+                                         }
+                                     }
+                                 });
+                }
+            }
+            // boundary interfaces (domain_boundary, boundary.hpp:6-33)
+            for (int level = cmin; level <= cmax; ++level)
+            {
+                const double h  = 1.0 / (1 << level);
+                const double fa = h_factor(dim, h, h);
+                double F[2]     = {fc[0] * fa, fc[1] * fa};
+                const DSet& dom = m.domain[size_t(level)];
+                const DSet B    = difference(cells[size_t(level)], translate(dom, me));
+                for_each_run(B,
+                             [&](int x0, int x1, int y, int z)
+                             {
+                                 for (int c = 0; c < 2; ++c)
+                                 {
+                                     for (int x = x0; x < x1; ++x)
+                                     {
+                                         out[idx(level, x, y, z)] += F[c] * s.u[idx(level, x + c * e[0], y + c * e[1], z + c * e[2])];
+                                     }
+                                 }
+                             });
+                F[0] *= -1;
+                F[1] *= -1;
+                const DSet Bo = difference(cells[size_t(level)], translate(dom, e));
+                for_each_run(Bo,
+                             [&](int x0, int x1, int y, int z)
+                             {
+                                 for (int c = 0; c < 2; ++c)
+                                 {
+                                     const int cx = (c - 1) * e[0], cy = (c - 1) * e[1], cz = (c - 1) * e[2];
+                                     for (int x = x0; x < x1; ++x)
+                                     {
+                                         out[idx(level, x, y, z)] += F[c] * s.u[idx(level, x + cx, y + cy, z + cz)];
+                                     }
+                                 }
+                             });
+            }
+        }
+        return out;
+    }
+
     Sim* make_box_sim(int dim, int min_level, int max_level, int pred_s, int radius, int grad_width)
     {
         auto* s           = new Sim();
@@ -1414,6 +1635,18 @@ extern "C"
     }
 
     // update_ghost_mr(u) then unp1 = u - dt*upwind(a,u), swap
+    // out (storage size) = make_convection_upwind(v)(u) after update_ghost_mr(u)
+    int orc_convection_upwind(void* p, const double* v, double* out)
+    {
+        auto* s = static_cast<Sim*>(p);
+        ORC_TRY({
+            update_ghost_mr(*s);
+            std::vector<double> r = convection_upwind(*s, v);
+            std::copy(r.begin(), r.end(), out);
+        });
+        return 0;
+    }
+
     int orc_upwind_step(void* p, const double* a, double dt)
     {
         auto* s = static_cast<Sim*>(p);

@@ -146,12 +146,16 @@ namespace smr
             switch (g.op)
             {
                 case dev::SEG_RECIPE: ok = ok && g.rdst && g.rop && g.rsp && g.rsrc && g.u; break;
-                case dev::SEG_RESOLVE: ok = ok && g.list.base && g.c1 <= g.list.nruns * g.list.ntaps && g.tabs && g.err; break;
+                case dev::SEG_RESOLVE:
+                    ok = ok && g.list.base && g.c1 <= g.list.nruns * g.list.ntaps && (g.tabs || (g.aux & AUX_TABS)) && g.err;
+                    break;
                 case dev::SEG_FILL_F64: ok = ok && g.out; break;
                 case dev::SEG_FILL_U8: ok = ok && g.tag; break;
                 case dev::SEG_TAG_INIT:
                 case dev::SEG_MAXIMUM: ok = ok && g.list.iv && g.c1 <= g.list.ncells && g.tag; break;
-                case dev::SEG_CRITERIA: ok = ok && g.list.iv && g.c1 <= g.list.ncells && g.tag && g.out && g.thr; break;
+                case dev::SEG_CRITERIA:
+                    ok = ok && g.list.iv && g.c1 <= g.list.ncells && g.tag && g.out && (g.thr || (g.aux & AUX_THR));
+                    break;
                 case dev::SEG_PROJECT:
                 case dev::SEG_PREDICT_FINE: ok = ok && g.list.iv && g.c1 <= g.list.ncells && g.u; break;
                 case dev::SEG_DETAIL:

@@ -155,8 +155,9 @@ namespace smr::dev
             return unsigned((n + BLOCK - 1) / BLOCK);
         }
 
-        // launch shapes of the upwind kernel: cells per thread (0: wave tiles),
-        // XCD remap, non-temporal stores
+        // launch shapes of the upwind kernel: cells per thread (0: wave tiles,
+        // -1: wave tiles when at least 3/4 of the tile lanes hold a cell, else 4
+        // cells per thread), XCD remap, non-temporal stores
         struct UpwindVariant
         {
             int id, cpt;
@@ -165,9 +166,9 @@ namespace smr::dev
         constexpr UpwindVariant kUpwindVariants[] = {{0, 1, false, false}, {1, 1, true, false}, {2, 2, false, false},
                                                      {3, 2, true, false},  {4, 4, false, false}, {5, 4, true, true},
                                                      {6, 1, false, true},  {7, 8, false, false}, {8, 0, true, true},
-                                                     {9, 0, true, false}};
-        constexpr int kNumUpwindVariants    = 10;
-        constexpr int kDefaultUpwindVariant = 5;
+                                                     {9, 0, true, false},  {10, -1, true, true}};
+        constexpr int kNumUpwindVariants    = 11;
+        constexpr int kDefaultUpwindVariant = 10;
         int g_upwind_variant                = -1;
 
         UpwindVariant upwind_variant()
@@ -208,7 +209,12 @@ namespace smr::dev
         {
             return;
         }
-        const UpwindVariant v = upwind_variant();
+        UpwindVariant v = upwind_variant();
+        if (v.cpt < 0)
+        {
+            const bool dense = 4 * l.ncells >= 3 * 64 * l.ntiles;
+            v                = dense ? kUpwindVariants[8] : kUpwindVariants[5];
+        }
         if (v.cpt == 0 && c0 == 0 && c1 == l.ncells)
         {
             const int64_t nblk   = (l.ntiles + BLOCK / 64 - 1) / (BLOCK / 64);
