@@ -155,6 +155,20 @@ int smr_timers_enable(int on);
 int smr_timers_reset(void);
 int64_t smr_timers_report(char* buf, int64_t size);
 
+/* out = make_convection_upwind<Field>(velocity)(u): the flux-based linear
+ * upwind convection operator (schemes/fv/operators/convection_lin.hpp:15-89)
+ * applied explicitly (FV_scheme.hpp:202-210): out is zero-filled and
+ * receives the interface fluxes divided by the cell measure.  u's ghosts are
+ * updated first (update_ghost_mr, as update_ghosts_if_needed does);
+ * velocity: dim values; u and out: two fields on one mesh. */
+int smr_convection_upwind(const double* velocity, smr_field* u, smr_field* out);
+/* Diagnostic: the ordered contribution lists behind smr_convection_upwind,
+ * computed on the host.  Call with null arrays to get *nrows and *nterms;
+ * row: nrows storage slots, ptr: nrows+1, src: 2*nterms (second -1 for a
+ * single product), coef: nterms. */
+int smr_mesh_convection_terms(const smr_mesh* mesh, const double* velocity, int64_t* nrows, int64_t* nterms, int64_t* row, int64_t* ptr,
+                              int64_t* src, double* coef);
+
 /* ------------------------------------------------------------------------
  * Batched form: many independent instances of the advection demo loop
  * (demos/FiniteVolume/advection_2d.cpp:129-153) advanced together.  No

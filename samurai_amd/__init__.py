@@ -95,6 +95,8 @@ def lib():
         L.smr_batch_get_tuning.argtypes = [vp, P(i32), P(i32), P(i32)]
         L.smr_batch_autotune.argtypes = [vp, i32, ctypes.c_char_p]
         L.smr_batch_load_tuning.argtypes = [vp, ctypes.c_char_p]
+        L.smr_convection_upwind.argtypes = [P(dbl), vp, vp]
+        L.smr_mesh_convection_terms.argtypes = [vp, P(dbl), P(i64), P(i64), P(i64), P(i64), P(i64), P(dbl)]
         L.smr_adjoint_create.argtypes = [vp, i32, P(vp), P(dbl), P(dbl), i32, i32, ctypes.POINTER(vp)]
         L.smr_adjoint_destroy.argtypes = [vp]
         L.smr_adjoint_nb_cells.restype = i64
@@ -286,6 +288,42 @@ def upwind(a, dt, u, unp1):
     """unp1 = u - dt * upwind(a, u) on mesh[cells]."""
     av = np.ascontiguousarray(a, dtype=np.float64)
     _check(lib().smr_upwind(_dp(av), float(dt), u.h, unp1.h))
+
+
+class ConvectionUpwind:
+    """make_convection_upwind<Field>(velocity) (schemes/fv/operators/convection_lin.hpp:15-89);
+    calling it on u returns make_convection_upwind(velocity)(u) in a new field."""
+
+    def __init__(self, velocity):
+        self.velocity = np.ascontiguousarray(np.asarray(velocity, dtype=np.float64).ravel())
+
+    def __call__(self, u, out=None):
+        if out is None:
+            out = make_scalar_field("convection(" + u.name + ")", u.mesh)
+        _check(lib().smr_convection_upwind(_dp(self.velocity), u.h, out.h))
+        return out
+
+
+def make_convection_upwind(velocity):
+    return ConvectionUpwind(velocity)
+
+
+def convection_terms(mesh, velocity):
+    """The ordered contribution lists behind make_convection_upwind (host)."""
+    v = np.ascontiguousarray(np.asarray(velocity, dtype=np.float64).ravel())
+    i64 = ctypes.c_int64
+    nr, nt = i64(0), i64(0)
+    null = ctypes.POINTER(i64)()
+    _check(lib().smr_mesh_convection_terms(mesh.h, _dp(v), ctypes.byref(nr), ctypes.byref(nt), null, null, null,
+                                           ctypes.POINTER(ctypes.c_double)()))
+    row = np.zeros(nr.value, dtype=np.int64)
+    ptr = np.zeros(nr.value + 1, dtype=np.int64)
+    src = np.zeros(2 * nt.value, dtype=np.int64)
+    coef = np.zeros(nt.value)
+    P64 = ctypes.POINTER(i64)
+    _check(lib().smr_mesh_convection_terms(mesh.h, _dp(v), ctypes.byref(nr), ctypes.byref(nt), row.ctypes.data_as(P64),
+                                           ptr.ctypes.data_as(P64), src.ctypes.data_as(P64), _dp(coef)))
+    return row, ptr, src, coef
 
 
 class Adapt:

@@ -15,7 +15,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libsamurai_amd.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
-HOST_SOURCES = ["host/interval_set.cpp", "host/mr_mesh.cpp", "host/programs.cpp", "runtime.cpp", "batch.cpp", "adjoint.cpp", "capi.cpp"]
+HOST_SOURCES = ["host/interval_set.cpp", "host/mr_mesh.cpp", "host/programs.cpp", "host/convection.cpp", "runtime.cpp", "batch.cpp", "adjoint.cpp", "capi.cpp"]
 HIP_SOURCES = ["device/kernels.hip", "device/program.hip"]
 
 CXXFLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall", "-Wextra",

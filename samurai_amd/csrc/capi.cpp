@@ -4,6 +4,7 @@
 #include "batch.hpp"
 #include "runtime.hpp"
 
+#include <algorithm>
 #include <cstring>
 #include <string>
 
@@ -875,6 +876,40 @@ extern "C"
                 require(cells_per_thread >= 1 && cells_per_thread <= 64, "cells per thread must be in [1, 64]");
                 require(mode >= 0 && mode <= 2, "unknown program mode");
                 adj->a->set_tuning(smr::Tuning{block, cells_per_thread, mode});
+                return 0;
+            });
+    }
+
+    int smr_convection_upwind(const double* velocity, smr_field* u, smr_field* out)
+    {
+        return guarded(
+            [&]
+            {
+                require(velocity && u && out, "null argument");
+                smr::convection_upwind(velocity, u->f, out->f);
+                return 0;
+            });
+    }
+
+    int smr_mesh_convection_terms(const smr_mesh* mesh, const double* velocity, int64_t* nrows, int64_t* nterms, int64_t* row, int64_t* ptr,
+                                  int64_t* src, double* coef)
+    {
+        return guarded(
+            [&]
+            {
+                require(mesh && velocity && nrows && nterms, "null argument");
+                const smr::ConvTerms t = smr::build_convection_terms(mesh->m.host, velocity);
+                const bool fill        = row && ptr && src && coef;
+                if (fill)
+                {
+                    require(*nrows >= int64_t(t.row.size()) && *nterms >= t.nterms(), "output arrays too small");
+                    std::copy(t.row.begin(), t.row.end(), row);
+                    std::copy(t.ptr.begin(), t.ptr.end(), ptr);
+                    std::copy(t.src.begin(), t.src.end(), src);
+                    std::copy(t.coef.begin(), t.coef.end(), coef);
+                }
+                *nrows  = int64_t(t.row.size());
+                *nterms = t.nterms();
                 return 0;
             });
     }

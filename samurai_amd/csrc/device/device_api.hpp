@@ -133,6 +133,9 @@ namespace smr::dev
     void run_program_stagewise(const Program& p, const std::vector<int64_t>& stage_cells, int block, int cpt, hipStream_t s);
 
     void resolve(const List& l, const Table& t0, const Table& t1, int* err, hipStream_t s);
+    // out[row[r]] = ordered sum of the terms of row r (flux-based convection)
+    void conv_gather(const int64_t* row, const int64_t* ptr, const int64_t* src, const double* coef, int64_t nrows, const double* u,
+                     double* out, hipStream_t s);
     // launch shape of the upwind kernel (cells per thread, XCD remap, stores)
     int upwind_variant_count();
     void set_upwind_variant(int id);

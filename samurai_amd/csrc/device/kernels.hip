@@ -110,6 +110,32 @@ namespace smr::dev
             }
         }
 
+        // make_convection_upwind(v)(u): one thread per output slot, adding its
+        // ordered contributions (a pair term is (c*u0 + c*u1), added at once)
+        __global__ void k_conv_gather(const int64_t* __restrict__ row, const int64_t* __restrict__ ptr, const int64_t* __restrict__ src,
+                                      const double* __restrict__ coef, int64_t nrows, const double* __restrict__ u, double* __restrict__ out)
+        {
+            const int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+            if (r >= nrows)
+            {
+                return;
+            }
+            double acc = 0;
+            for (int64_t k = ptr[r]; k < ptr[r + 1]; ++k)
+            {
+                const int64_t s1 = src[2 * k + 1];
+                if (s1 < 0)
+                {
+                    acc += coef[k] * u[src[2 * k]];
+                }
+                else
+                {
+                    acc += coef[k] * u[src[2 * k]] + coef[k] * u[s1];
+                }
+            }
+            out[row[r]] = acc;
+        }
+
 #define SMR_CELL_KERNEL(NAME, BODY, ...)                                                                                             \
     __global__ __launch_bounds__(BLOCK) void NAME(List l, int64_t c0, int64_t c1, __VA_ARGS__)                                        \
     {                                                                                                                                  \
@@ -191,6 +217,15 @@ namespace smr::dev
     void set_upwind_variant(int id)
     {
         g_upwind_variant = (id >= 0 && id < kNumUpwindVariants) ? id : kDefaultUpwindVariant;
+    }
+
+    void conv_gather(const int64_t* row, const int64_t* ptr, const int64_t* src, const double* coef, int64_t nrows, const double* u,
+                     double* out, hipStream_t s)
+    {
+        if (nrows > 0)
+        {
+            hipLaunchKernelGGL(k_conv_gather, dim3(nblocks(nrows)), dim3(BLOCK), 0, s, row, ptr, src, coef, nrows, u, out);
+        }
     }
 
     void resolve(const List& l, const Table& t0, const Table& t1, int* err, hipStream_t s)

@@ -1,0 +1,269 @@
+#include "convection.hpp"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <stdexcept>
+
+namespace smr
+{
+    namespace
+    {
+        struct Event
+        {
+            int64_t out, s0, s1;
+            double c;
+        };
+
+        int64_t base_of(const MRMeshData& m, int level, int32_t x, int32_t y, int32_t z)
+        {
+            int64_t b = 0;
+            if (!m.run_base(level, x, y, z, b))
+            {
+                throw std::out_of_range("convection: an interface stencil cell is missing from the reference mesh");
+            }
+            return b;
+        }
+
+        // h_factor (flux_based_scheme__lin_hom.hpp:62-67)
+        double h_factor(int dim, double h_face, double h_cell)
+        {
+            return std::pow(h_face, dim - 1) / std::pow(h_cell, dim);
+        }
+    } // namespace
+
+    ConvTerms build_convection_terms(const MRMeshData& m, const double* v)
+    {
+        const int dim     = m.cfg.dim;
+        const auto& cells = m.ids[CELLS];
+        const int cmin = ca_min_level(cells), cmax = ca_max_level(cells);
+        const int minl = std::max(m.cfg.min_level, cmin - 1), maxl = std::min(m.cfg.max_level, cmax + 1);
+        const int zs   = dim == 3 ? 1 : 0;
+        std::vector<Event> ev;
+        auto add = [&](int64_t out, int64_t s0, int64_t s1, double c)
+        {
+            if (c != 0.0)
+            {
+                ev.push_back({out, s0, s1, c});
+            }
+        };
+        for (int d = 0; d < dim; ++d)
+        {
+            std::array<int, 3> e{0, 0, 0};
+            e[size_t(d)] = 1;
+            const std::array<int, 3> me{-e[0], -e[1], -e[2]};
+            // stencil {left cell, right cell}: the upwind side carries the velocity (convection_lin.hpp:36-77)
+            const double fc[2] = {v[d] >= 0 ? v[d] : 0.0, v[d] >= 0 ? 0.0 : v[d]};
+
+            // interfaces between cells of the same level (interface.hpp:35-108)
+            for (int level = minl; level <= maxl; ++level)
+            {
+                const double h  = 1.0 / double(1 << level);
+                const double fa = h_factor(dim, h, h);
+                double L[2], R[2];
+                for (int c = 0; c < 2; ++c)
+                {
+                    L[c] = fa * fc[c];
+                    R[c] = -L[c];
+                }
+                const LevelSet I = set_intersection(cells[size_t(level)], translate(cells[size_t(level)], me));
+                I.for_each_run(
+                    [&](int32_t x0, int32_t x1, int32_t y, int32_t z)
+                    {
+                        const int64_t bp = base_of(m, level, x0, y, z);
+                        const int64_t bq = base_of(m, level, x0 + e[0], y + e[1], z + e[2]) + e[0];
+                        for (int c = 0; c < 2; ++c)
+                        {
+                            const int64_t bs = c == 0 ? bp : bq;
+                            for (int32_t x = x0; x < x1; ++x)
+                            {
+                                add(bp + x, bs + x, -1, L[c]);
+                            }
+                            for (int32_t x = x0; x < x1; ++x)
+                            {
+                                add(bq + x, bs + x, -1, R[c]);
+                            }
+                        }
+                    });
+            }
+
+            // level jumps level -- level+1 (interface.hpp:110-315)
+            for (int level = minl; level < maxl && level < m.cfg.max_level; ++level)
+            {
+                const int lf     = level + 1;
+                const double hl  = 1.0 / double(1 << level);
+                const double hf  = 1.0 / double(1 << lf);
+                auto coarse_base = [&](int32_t x0, int32_t y, int32_t z) { return base_of(m, level, x0 >> 1, y >> 1, z >> zs); };
+                // coarse cell on the left, fine cells on the right
+                {
+                    const double lfac = h_factor(dim, hf, hl), rfac = h_factor(dim, hf, hf);
+                    double L[2], R[2];
+                    for (int c = 0; c < 2; ++c)
+                    {
+                        L[c] = lfac * fc[c];
+                        R[c] = -rfac * fc[c];
+                    }
+                    const LevelSet J = on(set_intersection(cells[size_t(level)], translate(cells[size_t(lf)], me)), lf);
+                    J.for_each_run(
+                        [&](int32_t x0, int32_t x1, int32_t y, int32_t z)
+                        {
+                            const int32_t n  = x1 - x0;
+                            const int64_t bg = base_of(m, lf, x0, y, z);                                // p (ghost in the coarse cell)
+                            const int64_t bf = base_of(m, lf, x0 + e[0], y + e[1], z + e[2]) + e[0];    // p + e (fine cell)
+                            const int64_t bc = coarse_base(x0, y, z);
+                            for (int c = 0; c < 2; ++c)
+                            {
+                                const int64_t bs = c == 0 ? bg : bf;
+                                if (n == 1)
+                                {
+                                    add(bc + (x0 >> 1), bs + x0, -1, L[c]);
+                                    add(bf + x0, bs + x0, -1, R[c]);
+                                }
+                                else
+                                {
+                                    for (int32_t i = 0; i < n / 2; ++i)
+                                    {
+                                        const int32_t xa = x0 + 2 * i;
+                                        add(bc + (xa >> 1), bs + xa, bs + xa + 1, L[c]);
+                                    }
+                                    for (int32_t x = x0; x < x1; ++x)
+                                    {
+                                        add(bf + x, bs + x, -1, R[c]);
+                                    }
+                                }
+                            }
+                        });
+                }
+                // fine cells on the left, coarse cell on the right
+                {
+                    const double lfac = h_factor(dim, hf, hf), rfac = h_factor(dim, hf, hl);
+                    double L[2], R[2];
+                    for (int c = 0; c < 2; ++c)
+                    {
+                        L[c] = lfac * fc[c];
+                        R[c] = -rfac * fc[c];
+                    }
+                    const LevelSet J = on(set_intersection(cells[size_t(level)], translate(cells[size_t(lf)], e)), lf);
+                    J.for_each_run(
+                        [&](int32_t x0, int32_t x1, int32_t y, int32_t z)
+                        {
+                            const int32_t n  = x1 - x0;
+                            const int64_t bg = base_of(m, lf, x0, y, z);                                // p (ghost in the coarse cell)
+                            const int64_t bf = base_of(m, lf, x0 - e[0], y - e[1], z - e[2]) - e[0];    // p - e (fine cell)
+                            const int64_t bc = coarse_base(x0, y, z);
+                            for (int c = 0; c < 2; ++c)
+                            {
+                                // stencil {p - e, p}
+                                const int64_t bs = c == 0 ? bf : bg;
+                                if (n == 1)
+                                {
+                                    add(bf + x0, bs + x0, -1, L[c]);
+                                    add(bc + (x0 >> 1), bs + x0, -1, R[c]);
+                                }
+                                else
+                                {
+                                    for (int32_t x = x0; x < x1; ++x)
+                                    {
+                                        add(bf + x, bs + x, -1, L[c]);
+                                    }
+                                    for (int32_t i = 0; i < n / 2; ++i)
+                                    {
+                                        const int32_t xa = x0 + 2 * i;
+                                        add(bc + (xa >> 1), bs + xa, bs + xa + 1, R[c]);
+                                    }
+                                }
+                            }
+                        });
+                }
+            }
+
+            // boundary interfaces (interface.hpp:440-509, boundary.hpp:6-33)
+            for (int level = cmin; level <= cmax; ++level)
+            {
+                const LevelSet& cl = cells[size_t(level)];
+                if (cl.empty())
+                {
+                    continue;
+                }
+                const double h  = 1.0 / double(1 << level);
+                const double fa = h_factor(dim, h, h);
+                double F[2]     = {fc[0] * fa, fc[1] * fa};
+                const LevelSet& dom = m.domain[size_t(level)];
+                const LevelSet B    = set_difference(cl, translate(dom, me));
+                B.for_each_run(
+                    [&](int32_t x0, int32_t x1, int32_t y, int32_t z)
+                    {
+                        const int64_t bp = base_of(m, level, x0, y, z);
+                        const int64_t bo = base_of(m, level, x0 + e[0], y + e[1], z + e[2]) + e[0];
+                        for (int c = 0; c < 2; ++c)
+                        {
+                            const int64_t bs = c == 0 ? bp : bo;
+                            for (int32_t x = x0; x < x1; ++x)
+                            {
+                                add(bp + x, bs + x, -1, F[c]);
+                            }
+                        }
+                    });
+                F[0] *= -1;
+                F[1] *= -1;
+                const LevelSet Bo = set_difference(cl, translate(dom, e));
+                Bo.for_each_run(
+                    [&](int32_t x0, int32_t x1, int32_t y, int32_t z)
+                    {
+                        const int64_t bp = base_of(m, level, x0, y, z);
+                        const int64_t bo = base_of(m, level, x0 - e[0], y - e[1], z - e[2]) - e[0];
+                        for (int c = 0; c < 2; ++c)
+                        {
+                            // stencil {p - e, p}
+                            const int64_t bs = c == 0 ? bo : bp;
+                            for (int32_t x = x0; x < x1; ++x)
+                            {
+                                add(bp + x, bs + x, -1, F[c]);
+                            }
+                        }
+                    });
+            }
+        }
+
+        // group by output slot, keeping the order of each slot's terms
+        const int64_t n = m.nref;
+        std::vector<int64_t> cnt(size_t(n) + 1, 0);
+        for (const Event& x : ev)
+        {
+            if (x.out < 0 || x.out >= n || x.s0 < 0 || x.s0 >= n || x.s1 >= n)
+            {
+                throw std::out_of_range("convection: a term is outside the field storage");
+            }
+            ++cnt[size_t(x.out) + 1];
+        }
+        ConvTerms t;
+        std::vector<int64_t> start(size_t(n) + 1, 0);
+        for (int64_t i = 0; i < n; ++i)
+        {
+            start[size_t(i) + 1] = start[size_t(i)] + cnt[size_t(i) + 1];
+            if (cnt[size_t(i) + 1] > 0)
+            {
+                t.row.push_back(i);
+            }
+        }
+        const size_t nt = ev.size();
+        t.src.assign(2 * nt, -1);
+        t.coef.assign(nt, 0.0);
+        std::vector<int64_t> pos(start.begin(), start.end() - 1);
+        for (const Event& x : ev)
+        {
+            const int64_t k       = pos[size_t(x.out)]++;
+            t.src[size_t(2 * k)]     = x.s0;
+            t.src[size_t(2 * k + 1)] = x.s1;
+            t.coef[size_t(k)]        = x.c;
+        }
+        t.ptr.clear();
+        t.ptr.reserve(t.row.size() + 1);
+        for (int64_t r : t.row)
+        {
+            t.ptr.push_back(start[size_t(r)]);
+        }
+        t.ptr.push_back(int64_t(nt));
+        return t;
+    }
+} // namespace smr

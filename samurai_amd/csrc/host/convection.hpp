@@ -1,0 +1,34 @@
+// Flux-based linear convection (make_convection_upwind) as per-cell ordered
+// contribution lists, so that a gather kernel reproduces the reference's
+// accumulation order exactly.
+#pragma once
+
+#include "mr_mesh.hpp"
+
+#include <cstdint>
+#include <vector>
+
+namespace smr
+{
+    // Row r adds, in order, the terms [ptr[r], ptr[r+1]) into out[row[r]]
+    // (which starts at 0): term k is coef[k] * u[src[2k]] when src[2k+1] < 0,
+    // else coef[k] * u[src[2k]] + coef[k] * u[src[2k+1]].
+    struct ConvTerms
+    {
+        std::vector<int64_t> row;
+        std::vector<int64_t> ptr{0};
+        std::vector<int64_t> src;
+        std::vector<double> coef;
+        int64_t nterms() const { return int64_t(coef.size()); }
+    };
+
+    // make_convection_upwind(velocity)(u) (schemes/fv/operators/convection_lin.hpp:15-89,
+    // FV_scheme.hpp:202-210) on mesh m: for each direction, the same-level
+    // interfaces of every level, the level jumps in the direction and in the
+    // opposite direction, then the boundary interfaces
+    // (flux_based_scheme__lin_hom.hpp:74-207, interface.hpp:35-509), in the
+    // order explicit_flux_based_scheme__lin_hom.hpp:39-119, 233-319 adds them.
+    // Terms with a zero coefficient are left out: adding 0 * u to a sum that
+    // started at +0 never changes it (for finite u).
+    ConvTerms build_convection_terms(const MRMeshData& m, const double* velocity);
+} // namespace smr

@@ -600,6 +600,50 @@ namespace smr
         dev::upwind(dim, dm.cells.d, 0, dm.cells.d.ncells, u.d, unp1.d, a[0], a[1], dim == 3 ? a[2] : 0.0, dt, m.ctx->stream);
     }
 
+    void convection_upwind(const double* v, Field& u, Field& out)
+    {
+        u.check_current();
+        out.check_current();
+        if (u.mesh != out.mesh || &u == &out)
+        {
+            throw std::invalid_argument("convection: input and output must be two fields on one mesh");
+        }
+        Mesh& m       = *u.mesh;
+        const int dim = m.host.cfg.dim;
+        update_ghost_mr(u);
+        DevMesh& dm = m.device();
+        ScopedTimer timer(m.ctx, "convection operator");
+        bool same = dm.conv != nullptr;
+        for (int d = 0; same && d < dim; ++d)
+        {
+            same = dm.conv->v[d] == v[d];
+        }
+        if (!same)
+        {
+            ConvTerms t = build_convection_terms(m.host, v);
+            auto c      = std::make_unique<DevMesh::Conv>();
+            for (int d = 0; d < dim; ++d)
+            {
+                c->v[d] = v[d];
+            }
+            c->blob       = std::make_unique<Blob>();
+            const size_t orow = c->blob->add(t.row.data(), t.row.size() * 8);
+            const size_t optr = c->blob->add(t.ptr.data(), t.ptr.size() * 8);
+            const size_t osrc = c->blob->add(t.src.data(), t.src.size() * 8);
+            const size_t ocof = c->blob->add(t.coef.data(), t.coef.size() * 8);
+            c->blob->upload(m.ctx->stream);
+            c->row   = reinterpret_cast<const int64_t*>(c->blob->dptr(orow));
+            c->ptr   = reinterpret_cast<const int64_t*>(c->blob->dptr(optr));
+            c->src   = reinterpret_cast<const int64_t*>(c->blob->dptr(osrc));
+            c->coef  = reinterpret_cast<const double*>(c->blob->dptr(ocof));
+            c->nrows = int64_t(t.row.size());
+            dm.conv  = std::move(c);
+        }
+        hipStream_t s = m.ctx->stream;
+        hip_check(hipMemsetAsync(out.d, 0, size_t(out.size) * 8, s), "hipMemsetAsync(convection)");
+        dev::conv_gather(dm.conv->row, dm.conv->ptr, dm.conv->src, dm.conv->coef, dm.conv->nrows, u.d, out.d, s);
+    }
+
     // ------------------------------------------------------------------
     // adaptation (mr/adapt.hpp:147-389)
     // ------------------------------------------------------------------

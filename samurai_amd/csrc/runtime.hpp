@@ -4,6 +4,7 @@
 #pragma once
 
 #include "device/device_api.hpp"
+#include "host/convection.hpp"
 #include "host/programs.hpp"
 
 #include <functional>
@@ -83,6 +84,16 @@ namespace smr
         std::vector<DevRecipes> outer;
         DevWorkList projection, prediction;
         DevWorkList detail, criteria, maximum;
+        // flux-based convection terms of the last velocity used on this image
+        struct Conv
+        {
+            double v[3] = {0, 0, 0};
+            std::unique_ptr<Blob> blob;
+            const int64_t *row = nullptr, *ptr = nullptr, *src = nullptr;
+            const double* coef = nullptr;
+            int64_t nrows      = 0;
+        };
+        std::unique_ptr<Conv> conv;
         // the work lists of a complete image (cells first)
         std::vector<DevWorkList*> lists() { return {&cells, &projection, &prediction, &detail, &criteria, &maximum}; }
     };
@@ -204,6 +215,8 @@ namespace smr
 
     void update_ghost_mr(Field& f);
     void upwind(const double* a, double dt, Field& u, Field& unp1);
+    // out = make_convection_upwind(v)(u), after update_ghost_mr(u)
+    void convection_upwind(const double* v, Field& u, Field& out);
     // Adapt::operator() (mr/adapt.hpp:147-195); returns the number of harten calls
     int adapt(AdaptState& st, double eps, double regularity, int max_iterations = -1);
     // one harten call; true at the fixed point

@@ -44,6 +44,7 @@ def lib():
         L.orc_get_last_tags.argtypes = [vp, P(ctypes.c_uint8), P(dbl)]
         L.orc_upwind_step.argtypes = [vp, P(dbl), dbl]
         L.orc_upwind_only.argtypes = [vp, P(dbl), dbl]
+        L.orc_convection_upwind.argtypes = [vp, P(dbl), P(dbl)]
         _lib = L
     return _lib
 
@@ -137,3 +138,10 @@ class OracleSim:
     def upwind_only(self, a, dt):
         a = np.ascontiguousarray(a, dtype=np.float64)
         _check(lib().orc_upwind_only(self.h, _dp(a), float(dt)))
+
+    def convection_upwind(self, v):
+        """make_convection_upwind(v)(u) after update_ghost_mr(u): the output storage."""
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        out = np.zeros(self.nb_cells(REFERENCE))
+        _check(lib().orc_convection_upwind(self.h, _dp(v), _dp(out)))
+        return out

@@ -174,3 +174,18 @@ def test_time_steps_match_oracle(gpu_ctx, dim, lmin, lmax):
         sa.swap(u, unp1)
         assert np.array_equal(o.cells(), mesh.cells()), step
         assert np.array_equal(bits(o.values()), bits(u.cells())), step
+
+
+@pytest.mark.parametrize("dim,lmin,lmax,v", [(2, 2, 7, (1.0, 0.5)), (2, 3, 7, (-0.7, 1.3)), (3, 2, 5, (1.0, -0.5, 0.25)),
+                                             (3, 2, 5, (-0.2, 0.9, -1.0))])
+def test_convection_upwind_matches_oracle(gpu_ctx, dim, lmin, lmax, v):
+    """make_convection_upwind(v)(u) on adapted meshes (level jumps in every direction)."""
+    say("convection", dim, lmin, lmax, v)
+    o, mesh, u = pair(gpu_ctx, dim, lmin, lmax, 1, adapt_eps=1e-3)
+    ref = o.convection_upwind(np.array(v))
+    out = sa.make_convection_upwind(np.array(v))(u)
+    got = out.storage()
+    assert np.array_equal(bits(got), bits(ref))
+    # the cached contribution lists are reused for a second field on the mesh
+    out2 = sa.make_convection_upwind(np.array(v))(u)
+    assert np.array_equal(bits(out2.storage()), bits(ref))
