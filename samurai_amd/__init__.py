@@ -40,6 +40,14 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise SamuraiError(f"{LIB_PATH} is missing: build it with python samurai_amd/build.py")
+        # One HIP runtime per process: torch ships its own libamdhip64 and
+        # cannot initialize the GPU once another copy of the runtime holds it.
+        # Loading torch first makes the library bind to torch's copy (same
+        # soname), so torch tensors and this library can share the device.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         vp, i32, i64, dbl = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double
         P, PP = ctypes.POINTER, lambda t: ctypes.POINTER(ctypes.POINTER(t))
