@@ -155,6 +155,12 @@ int smr_timers_enable(int on);
 int smr_timers_reset(void);
 int64_t smr_timers_report(char* buf, int64_t size);
 
+/* The README's hand-written upwind loop (README.md:135-154): for every cell,
+ *   unp1(level, i, j) = u(level, i, j) - dt / dx * (u(level, i, j) - u(level, i - 1, j)
+ *                                                + u(level, i, j) - u(level, i, j - 1))
+ * with dx = cell_length(level), as for_each_interval(mesh, ...) runs it (2D,
+ * velocity (1, 1)).  u's ghosts must be up to date (update_ghost_mr). */
+int smr_upwind_hand_loop(double dt, smr_field* u, smr_field* unp1);
 /* out = make_convection_upwind<Field>(velocity)(u): the flux-based linear
  * upwind convection operator (schemes/fv/operators/convection_lin.hpp:15-89)
  * applied explicitly (FV_scheme.hpp:202-210): out is zero-filled and

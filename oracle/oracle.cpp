@@ -1177,6 +1177,29 @@ namespace oracle
         s.u.swap(unp1);
     }
 
+    // README.md:135-154: for_each_interval(mesh, ...) over the cells,
+    //   unp1(level, i, j) = u(level, i, j) - dt / dx * (u(level, i, j) - u(level, i - 1, j)
+    //                                                 + u(level, i, j) - u(level, i, j - 1));
+    // then std::swap(unp1.array(), u.array())
+    void upwind_hand_loop(Sim& s, double dt)
+    {
+        const Mesh& m = s.mesh;
+        std::vector<double> unp1 = new_storage(s, m.nref);
+        for (int level = 0; level < MAXL; ++level)
+        {
+            const double dx = 1.0 / (1 << level);
+            m.ids[CELLS][size_t(level)].for_each(
+                [&](int x, int y, int z)
+                {
+                    const double uc = s.u[size_t(m.index(level, x, y, z))];
+                    const double ul = s.u[size_t(m.index(level, x - 1, y, z))];
+                    const double ud = s.u[size_t(m.index(level, x, y - 1, z))];
+                    unp1[size_t(m.index(level, x, y, z))] = uc - dt / dx * (uc - ul + uc - ud);
+                });
+        }
+        s.u.swap(unp1);
+    }
+
     // maximal x-runs of a set in traversal order (z, y ascending, then x)
     template <class F>
     void for_each_run(const DSet& s, F&& f)
@@ -1635,6 +1658,17 @@ extern "C"
     }
 
     // update_ghost_mr(u) then unp1 = u - dt*upwind(a,u), swap
+    // update_ghost_mr(u), then the README hand loop (with its swap)
+    int orc_upwind_hand_loop(void* p, double dt)
+    {
+        auto* s = static_cast<Sim*>(p);
+        ORC_TRY({
+            update_ghost_mr(*s);
+            upwind_hand_loop(*s, dt);
+        });
+        return 0;
+    }
+
     // out (storage size) = make_convection_upwind(v)(u) after update_ghost_mr(u)
     int orc_convection_upwind(void* p, const double* v, double* out)
     {

@@ -104,6 +104,7 @@ def lib():
         L.smr_batch_autotune.argtypes = [vp, i32, ctypes.c_char_p]
         L.smr_batch_load_tuning.argtypes = [vp, ctypes.c_char_p]
         L.smr_convection_upwind.argtypes = [P(dbl), vp, vp]
+        L.smr_upwind_hand_loop.argtypes = [dbl, vp, vp]
         L.smr_mesh_convection_terms.argtypes = [vp, P(dbl), P(i64), P(i64), P(i64), P(i64), P(i64), P(dbl)]
         L.smr_adjoint_create.argtypes = [vp, i32, P(vp), P(dbl), P(dbl), i32, i32, ctypes.POINTER(vp)]
         L.smr_adjoint_destroy.argtypes = [vp]
@@ -296,6 +297,11 @@ def upwind(a, dt, u, unp1):
     """unp1 = u - dt * upwind(a, u) on mesh[cells]."""
     av = np.ascontiguousarray(a, dtype=np.float64)
     _check(lib().smr_upwind(_dp(av), float(dt), u.h, unp1.h))
+
+
+def upwind_hand_loop(dt, u, unp1):
+    """The README's for_each_interval loop: unp1 = u - dt/dx*(u - u(i-1) + u - u(j-1)) (2D)."""
+    _check(lib().smr_upwind_hand_loop(float(dt), u.h, unp1.h))
 
 
 class ConvectionUpwind:

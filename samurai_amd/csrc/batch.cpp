@@ -153,6 +153,7 @@ namespace smr
                 case dev::SEG_FILL_U8: ok = ok && g.tag; break;
                 case dev::SEG_TAG_INIT:
                 case dev::SEG_MAXIMUM: ok = ok && g.list.iv && g.c1 <= g.list.ncells && g.tag; break;
+                case dev::SEG_TAG_CHANGE: ok = ok && g.list.iv && g.c1 <= g.list.ncells && g.tag && g.err; break;
                 case dev::SEG_CRITERIA:
                     ok = ok && g.list.iv && g.c1 <= g.list.ncells && g.tag && g.out && (g.thr || (g.aux & AUX_THR));
                     break;
@@ -456,10 +457,18 @@ namespace smr
         {
             dev_free(tag_arena_, s);
         }
+        if (flags_)
+        {
+            dev_free(flags_, s);
+        }
         (void)hipStreamSynchronize(s);
         if (host_tags_)
         {
             (void)hipHostFree(host_tags_);
+        }
+        if (host_flags_)
+        {
+            (void)hipHostFree(host_flags_);
         }
     }
 
@@ -532,6 +541,13 @@ namespace smr
     void Batch::ensure_arenas(int64_t n)
     {
         hipStream_t s = ctx_->stream;
+        if (!flags_)
+        {
+            flags_ = static_cast<int*>(dev_alloc(std::max<size_t>(inst_.size(), 1) * sizeof(int), s));
+            hip_check(hipHostMalloc(reinterpret_cast<void**>(&host_flags_), std::max<size_t>(inst_.size(), 1) * sizeof(int),
+                                    hipHostMallocDefault),
+                      "hipHostMalloc(flags)");
+        }
         if (n > arena_cap_)
         {
             if (det_arena_)
@@ -613,7 +629,10 @@ namespace smr
                     dev::Segment ft = make_seg(dev::SEG_FILL_U8, c);
                     ft.tag          = tag;
                     ft.c1           = n;
-                    pre[0]          = {fd, ft};
+                    dev::Segment ff = make_seg(dev::SEG_FILL_U8, c);
+                    ff.tag          = reinterpret_cast<uint8_t*>(flags_ + k);
+                    ff.c1           = sizeof(int);
+                    pre[0]          = {fd, ft, ff};
                     dev::Segment ti = list_seg(dev::SEG_TAG_INIT, c, dm.cells, 0, dm.cells.d.ncells);
                     ti.tag          = tag;
                     pre[1]          = {ti};
@@ -642,6 +661,10 @@ namespace smr
                         sm.tag          = tag;
                         push_stage(ch, sm);
                     }
+                    dev::Segment tc = list_seg(dev::SEG_TAG_CHANGE, c, dm.cells, 0, dm.cells.d.ncells);
+                    tc.tag          = tag;
+                    tc.err          = flags_ + k;
+                    push_stage(ch, tc);
                     chains[k] = std::move(ch);
                 }
                 adapt_prog_.append(chains);
@@ -654,23 +677,41 @@ namespace smr
             std::vector<Rebuild> rb(na);
             {
                 ScopedTimer t(ctx_, "mesh update");
-                hip_check(hipMemcpyAsync(host_tags_, tag_arena_, size_t(off[na]), hipMemcpyDeviceToHost, s), "hipMemcpyAsync(tags)");
+                // which instances' tags change their cell array (graduated meshes
+                // whose tags change nothing are at the fixed point already)
+                hip_check(hipMemcpyAsync(host_flags_, flags_, na * sizeof(int), hipMemcpyDeviceToHost, s), "hipMemcpyAsync(flags)");
                 ctx_->sync();
                 pending_.clear();
-                parallel_for(int(na),
-                             [&](int k)
+                std::vector<size_t> work;
+                for (size_t k = 0; k < na; ++k)
+                {
+                    const Mesh& m = *inst_[size_t(act[k])].u->mesh;
+                    if (host_flags_[k] == 0 && m.host.graduated)
+                    {
+                        rb[k].converged = true;
+                        continue;
+                    }
+                    work.push_back(k);
+                    hip_check(hipMemcpyAsync(host_tags_ + off[k], tag_arena_ + off[k], size_t(m.host.nref), hipMemcpyDeviceToHost, s),
+                              "hipMemcpyAsync(tags)");
+                }
+                ctx_->sync();
+                parallel_for(int(work.size()),
+                             [&](int q)
                              {
-                                 const Mesh& m       = *inst_[size_t(act[size_t(k)])].u->mesh;
+                                 const size_t k      = work[size_t(q)];
+                                 const Mesh& m       = *inst_[size_t(act[k])].u->mesh;
                                  const MeshConfig& c = m.host.cfg;
-                                 Rebuild& r          = rb[size_t(k)];
-                                 CellArray ca        = update_cell_array_from_tag(m.host, host_tags_ + off[size_t(k)]);
+                                 Rebuild& r          = rb[k];
+                                 CellArray ca        = update_cell_array_from_tag(m.host, host_tags_ + off[k]);
                                  make_graduation(ca, c.dim, c.grad_width);
                                  if (ca_equal(m.host.ids[CELLS], ca))
                                  {
                                      r.converged = true;
                                      return;
                                  }
-                                 r.nm  = build_mesh(c, ca);
+                                 r.nm           = build_mesh(c, ca);
+                                 r.nm.graduated = true;
                                  r.img = build_host_image(r.nm);
                                  build_update_fields_lists(m.host, r.nm, r.copy, r.coarsen, r.refine);
                              });

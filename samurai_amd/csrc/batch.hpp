@@ -128,6 +128,8 @@ namespace smr
         double* det_arena_ = nullptr;
         uint8_t* tag_arena_ = nullptr;
         uint8_t* host_tags_ = nullptr;
+        int* flags_         = nullptr; // per active instance: tags change the cell array
+        int* host_flags_    = nullptr;
         int64_t arena_cap_ = 0, host_cap_ = 0;
         // blobs whose pageable host staging must outlive the copies queued from it
         std::vector<std::unique_ptr<Blob>> pending_;

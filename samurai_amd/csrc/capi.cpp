@@ -138,8 +138,8 @@ extern "C"
                         uint64_t threshold = UINT64_MAX;
                         smr::hip_check(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &threshold),
                                        "hipMemPoolSetAttribute");
-                        smr::hip_check(hipMalloc(reinterpret_cast<void**>(&c->c.d_err), sizeof(int)), "hipMalloc(err)");
-                        smr::hip_check(hipMemsetAsync(c->c.d_err, 0, sizeof(int), c->c.stream), "hipMemsetAsync(err)");
+                        smr::hip_check(hipMalloc(reinterpret_cast<void**>(&c->c.d_err), 2 * sizeof(int)), "hipMalloc(err)");
+                        smr::hip_check(hipMemsetAsync(c->c.d_err, 0, 2 * sizeof(int), c->c.stream), "hipMemsetAsync(err)");
                         c->c.sync();
                     }
                     catch (...)
@@ -222,7 +222,10 @@ extern "C"
                 m->m.ctx = &ctx->c;
                 try
                 {
-                    m->m.host = smr::build_mesh(c, ca);
+                    smr::CellArray g = ca;
+                    smr::make_graduation(g, c.dim, c.grad_width);
+                    m->m.host           = smr::build_mesh(c, ca);
+                    m->m.host.graduated = smr::ca_equal(g, ca);
                 }
                 catch (...)
                 {
@@ -308,7 +311,8 @@ extern "C"
                 smr::MRMeshData nm;
                 {
                     smr::ScopedTimer t3(nullptr, "mesh construction");
-                    nm = smr::build_mesh(m.cfg, ca);
+                    nm           = smr::build_mesh(m.cfg, ca);
+                    nm.graduated = true;
                 }
                 smr::WorkList copy, coarsen, refine;
                 smr::build_update_fields_lists(m, nm, copy, coarsen, refine);
@@ -910,6 +914,17 @@ extern "C"
                 }
                 *nrows  = int64_t(t.row.size());
                 *nterms = t.nterms();
+                return 0;
+            });
+    }
+
+    int smr_upwind_hand_loop(double dt, smr_field* u, smr_field* unp1)
+    {
+        return guarded(
+            [&]
+            {
+                require(u && unp1, "null field");
+                smr::upwind_hand_loop(dt, u->f, unp1->f);
                 return 0;
             });
     }

@@ -431,6 +431,31 @@ namespace smr::dev
         tag[bases(l, cell.run)[0] + cell.x] = 1;
     }
 
+    // README.md:135-154 hand loop over for_each_interval (2D, a = (1, 1)):
+    //   unp1 = u - dt / dx * (u - u(i-1) + u - u(j-1)),
+    // with dt / dx = dt * 2^level exactly (dx = 2^-level)
+    __device__ __forceinline__ double hand_loop_value(const int64_t* b, int32_t x, int level, const double* __restrict__ u, double dt)
+    {
+        const double uc = u[b[0] + x];
+        const double k  = dt / ldexp(1.0, -level);
+        return uc - k * (uc - u[b[0] + x - 1] + uc - u[b[1] + x]);
+    }
+
+    // flag[0] = 1 when the tag of a cell changes the cell array: a refine below
+    // max_level or a coarsen without keep above min_level
+    // (update_cell_array_from_tag, graduation.hpp:743-843)
+    __device__ __forceinline__ void op_tag_change(const List& l, int64_t c, const uint8_t* tag, int min_level, int max_level, int* flag)
+    {
+        const Cell cell  = locate(l, c);
+        const uint8_t t  = tag[bases(l, cell.run)[0] + cell.x];
+        const bool ref   = (t & 4) && cell.level < max_level;
+        const bool crs   = (t & 2) && !(t & 1) && cell.level > min_level;
+        if (ref || crs)
+        {
+            flag[0] = 1;
+        }
+    }
+
     __device__ __forceinline__ void op_copy(const List& l, int64_t c, double* dst, const double* src)
     {
         const Cell cell  = locate(l, c);

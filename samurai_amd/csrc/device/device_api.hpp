@@ -85,7 +85,9 @@ namespace smr::dev
         SEG_ADJ_UPWIND = 19,
         // the part of the transposed stencil that lands on ghost neighbours:
         // cells of the list whose neighbour in direction `pad` is a ghost
-        SEG_ADJ_SCATTER = 20
+        SEG_ADJ_SCATTER = 20,
+        // err[0] = 1 when a cell's tag changes the cell array (cells list)
+        SEG_TAG_CHANGE = 21
     };
 
     struct Segment
@@ -141,6 +143,8 @@ namespace smr::dev
     void set_upwind_variant(int id);
     void upwind(int dim, const List& l, int64_t c0, int64_t c1, const double* u, double* unp1, double a0, double a1, double a2,
                 double dt, hipStream_t s);
+    // the README's for_each_interval upwind loop (2D, a = (1, 1))
+    void hand_loop(const List& l, int64_t c0, int64_t c1, const double* u, double* unp1, double dt, hipStream_t s);
     void project(int dim, const List& l, int64_t c0, int64_t c1, double* dst, const double* src, hipStream_t s);
     void predict_fine(int dim, int pred_s, const List& l, int64_t c0, int64_t c1, double* u, hipStream_t s);
     void detail(int dim, int pred_s, const List& l, int64_t c0, int64_t c1, const double* u, double* det, hipStream_t s);
@@ -148,6 +152,8 @@ namespace smr::dev
                   int max_level, hipStream_t s);
     void maximum(int dim, const List& l, int64_t c0, int64_t c1, uint8_t* tag, hipStream_t s);
     void tag_init(const List& l, int64_t c0, int64_t c1, uint8_t* tag, hipStream_t s);
+    // flag[0] = 1 if a cell's tag changes the cell array (cells list)
+    void tag_change(const List& l, int64_t c0, int64_t c1, const uint8_t* tag, int min_level, int max_level, int* flag, hipStream_t s);
     void copy_cells(const List& l, int64_t c0, int64_t c1, double* dst, const double* src, hipStream_t s);
     void predict_children(int dim, int pred_s, const List& l, int64_t c0, int64_t c1, double* dst, const double* src, hipStream_t s);
     // one dependency-free stage of outer-ghost recipes

@@ -159,6 +159,14 @@ namespace smr::dev
         template <int DIM>
         SMR_CELL_KERNEL(k_maximum, (op_maximum<DIM>(l, c, tag)), uint8_t* tag)
         SMR_CELL_KERNEL(k_tag_init, (op_tag_init(l, c, tag)), uint8_t* tag)
+        SMR_CELL_KERNEL(k_hand_loop, ({
+                            const Cell cell         = locate(l, c);
+                            const int64_t* b        = bases(l, cell.run);
+                            unp1[b[0] + cell.x] = hand_loop_value(b, cell.x, cell.level, u, dt);
+                        }),
+                        const double* u, double* unp1, double dt)
+        SMR_CELL_KERNEL(k_tag_change, (op_tag_change(l, c, tag, min_level, max_level, flag)), const uint8_t* tag, int min_level,
+                        int max_level, int* flag)
         SMR_CELL_KERNEL(k_copy, (op_copy(l, c, dst, src)), double* dst, const double* src)
         template <int DIM, int S>
         SMR_CELL_KERNEL(k_predict_children, (op_predict_children<DIM, S>(l, c, dst, src)), double* dst, const double* src)
@@ -378,6 +386,22 @@ namespace smr::dev
         if (c1 > c0)
         {
             SMR_DISPATCH_DIM(k_maximum, tag)
+        }
+    }
+
+    void hand_loop(const List& l, int64_t c0, int64_t c1, const double* u, double* unp1, double dt, hipStream_t s)
+    {
+        if (c1 > c0)
+        {
+            hipLaunchKernelGGL(k_hand_loop, dim3(nblocks(c1 - c0)), dim3(BLOCK), 0, s, l, c0, c1, u, unp1, dt);
+        }
+    }
+
+    void tag_change(const List& l, int64_t c0, int64_t c1, const uint8_t* tag, int min_level, int max_level, int* flag, hipStream_t s)
+    {
+        if (c1 > c0)
+        {
+            hipLaunchKernelGGL(k_tag_change, dim3(nblocks(c1 - c0)), dim3(BLOCK), 0, s, l, c0, c1, tag, min_level, max_level, flag);
         }
     }
 

@@ -43,6 +43,7 @@ namespace smr::dev
                 case SEG_CRITERIA: op_criteria<DIM>(g.list, c, g.out, g.tag, g.thr, g.min_level, g.max_level); break;
                 case SEG_MAXIMUM: op_maximum<DIM>(g.list, c, g.tag); break;
                 case SEG_TAG_INIT: op_tag_init(g.list, c, g.tag); break;
+                case SEG_TAG_CHANGE: op_tag_change(g.list, c, g.tag, g.min_level, g.max_level, g.err); break;
                 case SEG_COPY: op_copy(g.list, c, g.out, g.u); break;
                 case SEG_PROJECT_TO: op_project<DIM>(g.list, c, g.out, g.u); break;
                 case SEG_PREDICT_CHILDREN:

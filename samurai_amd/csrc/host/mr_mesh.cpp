@@ -326,7 +326,9 @@ namespace smr
     {
         CellArray cells                = empty_cell_array(cfg.dim);
         cells[size_t(cfg.max_level)]   = domain_at(cfg.dim, cfg.max_level);
-        return build_mesh(cfg, cells);
+        MRMeshData m                   = build_mesh(cfg, cells);
+        m.graduated                    = true; // a single level
+        return m;
     }
 
     CellArray update_cell_array_from_tag(const MRMeshData& mesh, const uint8_t* tags)

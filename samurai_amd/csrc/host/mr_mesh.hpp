@@ -62,6 +62,8 @@ namespace smr
         std::vector<std::vector<int64_t>> ridx;
         std::vector<int64_t> level_base; // first storage index of each level
         int64_t nref = 0;
+        // the cells are known to satisfy the graduation (make_graduation leaves them unchanged)
+        bool graduated = false;
 
         const LevelSet& get(int id, int level) const { return ids[id][size_t(level)]; }
         // storage index of a reference cell; throws std::out_of_range if absent

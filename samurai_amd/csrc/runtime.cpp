@@ -540,7 +540,15 @@ namespace smr
     {
         if (tags && last_size > 0)
         {
-            std::memcpy(tags, host_tag, size_t(last_size));
+            if (!host_tag_valid)
+            {
+                hip_check(hipMemcpyAsync(tags, tag, size_t(last_size), hipMemcpyDeviceToHost, s), "hipMemcpyAsync(tag)");
+                hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+            }
+            else
+            {
+                std::memcpy(tags, host_tag, size_t(last_size));
+            }
         }
         if (details && last_size > 0)
         {
@@ -598,6 +606,24 @@ namespace smr
         ScopedTimer timer(m.ctx, "field expressions");
         const int dim = m.host.cfg.dim;
         dev::upwind(dim, dm.cells.d, 0, dm.cells.d.ncells, u.d, unp1.d, a[0], a[1], dim == 3 ? a[2] : 0.0, dt, m.ctx->stream);
+    }
+
+    void upwind_hand_loop(double dt, Field& u, Field& unp1)
+    {
+        u.check_current();
+        unp1.check_current();
+        if (u.mesh != unp1.mesh || &u == &unp1)
+        {
+            throw std::invalid_argument("hand loop: u and unp1 must be two fields on one mesh");
+        }
+        Mesh& m = *u.mesh;
+        if (m.host.cfg.dim != 2)
+        {
+            throw std::invalid_argument("hand loop: the README loop is two-dimensional");
+        }
+        DevMesh& dm = m.device();
+        ScopedTimer timer(m.ctx, "field expressions");
+        dev::hand_loop(dm.cells.d, 0, dm.cells.d.ncells, u.d, unp1.d, dt, m.ctx->stream);
     }
 
     void convection_upwind(const double* v, Field& u, Field& out)
@@ -757,6 +783,23 @@ namespace smr
         MRMeshData nm;
         {
             ScopedTimer timer(&ctx, "mesh update");
+            st.last_size      = n;
+            st.host_tag_valid = false;
+            if (m.host.graduated)
+            {
+                // fixed point without the host: no tag changes the cell array of
+                // a graduated mesh, so update_cell_array_from_tag and
+                // make_graduation would return the cells unchanged
+                hip_check(hipMemsetAsync(ctx.d_flag(), 0, sizeof(int), s), "hipMemsetAsync(flag)");
+                dev::tag_change(dm.cells.d, 0, dm.cells.d.ncells, st.tag, minl, maxl, ctx.d_flag(), s);
+                int changed = 1;
+                hip_check(hipMemcpyAsync(&changed, ctx.d_flag(), sizeof(int), hipMemcpyDeviceToHost, s), "hipMemcpyAsync(flag)");
+                ctx.sync();
+                if (!changed)
+                {
+                    return true;
+                }
+            }
             if (n > st.host_tag_capacity)
             {
                 if (st.host_tag)
@@ -769,7 +812,7 @@ namespace smr
             }
             hip_check(hipMemcpyAsync(st.host_tag, st.tag, size_t(n), hipMemcpyDeviceToHost, s), "hipMemcpyAsync(tag)");
             ctx.sync();
-            st.last_size = n;
+            st.host_tag_valid = true;
 
             // mesh update on the host (graduation.hpp:743-843, 573-726)
             CellArray new_ca;
@@ -786,7 +829,8 @@ namespace smr
                 return true;
             }
             ScopedTimer t3(nullptr, "mesh construction");
-            nm = build_mesh(cfg, new_ca);
+            nm           = build_mesh(cfg, new_ca);
+            nm.graduated = true;
         }
         {
             ScopedTimer timer(&ctx, "device upload");

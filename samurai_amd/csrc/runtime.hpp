@@ -152,6 +152,7 @@ namespace smr
         uint8_t* host_tag         = nullptr;
         int64_t host_tag_capacity = 0;
         int64_t last_size         = 0;
+        bool host_tag_valid       = false; // else the tags are only in `tag`
         ~AdaptState();
         void copy_last(uint8_t* tags, double* details, hipStream_t s) const;
     };
@@ -160,8 +161,9 @@ namespace smr
     {
         int device         = -1;
         hipStream_t stream = nullptr;
-        int* d_err         = nullptr;
+        int* d_err         = nullptr; // two ints: resolve error, then a scratch flag
         ~Ctx();
+        int* d_flag() const { return d_err + 1; }
         bool has_device() const { return device >= 0; }
         void require_device() const;
         void sync();
@@ -215,6 +217,8 @@ namespace smr
 
     void update_ghost_mr(Field& f);
     void upwind(const double* a, double dt, Field& u, Field& unp1);
+    // README.md:135-154: unp1 = u - dt / dx * (u - u(i-1) + u - u(j-1)) on the cells (2D)
+    void upwind_hand_loop(double dt, Field& u, Field& unp1);
     // out = make_convection_upwind(v)(u), after update_ghost_mr(u)
     void convection_upwind(const double* v, Field& u, Field& out);
     // Adapt::operator() (mr/adapt.hpp:147-195); returns the number of harten calls

@@ -45,6 +45,7 @@ def lib():
         L.orc_upwind_step.argtypes = [vp, P(dbl), dbl]
         L.orc_upwind_only.argtypes = [vp, P(dbl), dbl]
         L.orc_convection_upwind.argtypes = [vp, P(dbl), P(dbl)]
+        L.orc_upwind_hand_loop.argtypes = [vp, dbl]
         _lib = L
     return _lib
 
@@ -145,3 +146,7 @@ class OracleSim:
         out = np.zeros(self.nb_cells(REFERENCE))
         _check(lib().orc_convection_upwind(self.h, _dp(v), _dp(out)))
         return out
+
+    def upwind_hand_loop(self, dt):
+        """update_ghost_mr, then the README's for_each_interval upwind loop and swap."""
+        _check(lib().orc_upwind_hand_loop(self.h, float(dt)))

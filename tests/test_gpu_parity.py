@@ -189,3 +189,16 @@ def test_convection_upwind_matches_oracle(gpu_ctx, dim, lmin, lmax, v):
     # the cached contribution lists are reused for a second field on the mesh
     out2 = sa.make_convection_upwind(np.array(v))(u)
     assert np.array_equal(bits(out2.storage()), bits(ref))
+
+
+@pytest.mark.parametrize("lmin,lmax", [(5, 5), (2, 7)])
+def test_hand_loop_matches_oracle(gpu_ctx, lmin, lmax):
+    """The README's for_each_interval upwind loop."""
+    say("hand loop", lmin, lmax)
+    o, mesh, u = pair(gpu_ctx, 2, lmin, lmax, 1, ic="seeded", adapt_eps=1e-3 if lmin < lmax else None, seed=2)
+    dt = 0.5 * 2.0 ** -lmax
+    unp1 = sa.make_scalar_field("unp1", mesh)
+    sa.update_ghost_mr(u)
+    sa.upwind_hand_loop(dt, u, unp1)
+    o.upwind_hand_loop(dt)
+    assert np.array_equal(bits(unp1.cells()), bits(o.values()))

@@ -84,3 +84,24 @@ def test_adaptation_invariants(dim):
     # the leaves tile the unit box
     vol = np.sum(np.ldexp(1.0, -dim * cells[:, 0].astype(np.int64)))
     assert vol == 1.0
+
+
+@pytest.mark.parametrize("lmin,lmax", [(5, 5), (3, 7)])
+def test_hand_loop_equals_upwind_at_unit_velocity(lmin, lmax):
+    """The README's hand loop (README.md:135-154) is the upwind scheme of
+    stencil_field.hpp at a = (1, 1): F(1, ul, ur) = ul, so both give
+    u - dt/dx ((u - u(i-1)) + (u - u(j-1))) up to rounding, on uniform and
+    adapted meshes (the upwind oracle is pinned by the golden fixtures)."""
+    from initial_conditions import seeded_2d
+    a = OracleSim(2, lmin, lmax, pred_s=1)
+    a.set_values(seeded_2d(a.cells(), 4))
+    a.set_dirichlet(0.0)
+    if lmin < lmax:
+        a.adapt(1e-3, 2.0)
+    b = OracleSim(2, lmin, lmax, pred_s=1, cells=a.cells())
+    b.set_values(a.values())
+    b.set_dirichlet(0.0)
+    dt = 0.5 * 2.0 ** -lmax
+    a.upwind_hand_loop(dt)
+    b.upwind_step(np.array([1.0, 1.0]), dt)
+    assert np.allclose(a.values(), b.values(), rtol=0, atol=1e-13)
