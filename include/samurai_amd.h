@@ -155,6 +155,10 @@ int smr_timers_enable(int on);
 int smr_timers_reset(void);
 int64_t smr_timers_report(char* buf, int64_t size);
 
+/* out = u - a * v on the cells: the field expression unp1 = u - dt * f of
+ * the flux-based demos (field_base.hpp:230-242 assignment of an expression;
+ * per cell u - (a * v)).  All three fields on one mesh; out may alias u. */
+int smr_field_sub_scaled(smr_field* u, double a, smr_field* v, smr_field* out);
 /* The README's hand-written upwind loop (README.md:135-154): for every cell,
  *   unp1(level, i, j) = u(level, i, j) - dt / dx * (u(level, i, j) - u(level, i - 1, j)
  *                                                + u(level, i, j) - u(level, i, j - 1))

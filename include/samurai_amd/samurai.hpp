@@ -1,0 +1,565 @@
+// C++ drop-in layer over the C ABI (include/samurai_amd.h) with the names and
+// call shapes of samurai's API for the adaptive time step, so that the loop of
+// demos/FiniteVolume/advection_2d.cpp compiles against it almost unchanged:
+//
+//   samurai::initialize(argc, argv);                          samurai.hpp:25-113
+//   auto mesh = samurai::mra::make_mesh(box, config);         mr/mesh.hpp:481-529
+//   auto u = samurai::make_scalar_field<double>("u", mesh);   scalar_field.hpp:171-176
+//   samurai::for_each_cell(mesh, [&](auto& cell) {...});      algorithm.hpp:319-323
+//   samurai::make_bc<samurai::Dirichlet<1>>(u, 0.);           bc/bc.hpp:771-795
+//   auto MRadaptation = samurai::make_MRAdapt(u);             mr/adapt.hpp:391-397
+//   MRadaptation(samurai::mra_config().epsilon(e).regularity(r));
+//   samurai::update_ghost_mr(u);                              update_ghost_mr.hpp:260-270
+//   unp1 = u - dt * samurai::upwind(a, u);                    stencil_field.hpp:175-179
+//   auto conv = samurai::make_convection_upwind<decltype(u)>(v);
+//   unp1 = u - dt * conv(u);                                  convection_lin.hpp:15-89
+//   samurai::swap(u, unp1);                                   field/swap.hpp:12-19
+//
+// Everything runs through the ABI on the GPU; values written through u[cell]
+// live in a host mirror that is uploaded before the next device operation,
+// and reading u[cell] after a device operation downloads the storage first.
+// The domain is the unit box [0, 1]^dim (the box of the demos).
+#pragma once
+
+#include "../samurai_amd.h"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace samurai_amd
+{
+    inline void check(int rc)
+    {
+        if (rc >= 0)
+        {
+            return;
+        }
+        const std::string msg = smr_last_error();
+        if (rc == SMR_ERR_RANGE)
+        {
+            throw std::out_of_range(msg);
+        }
+        if (rc == SMR_ERR_ARGUMENT)
+        {
+            throw std::invalid_argument(msg);
+        }
+        throw std::runtime_error(msg);
+    }
+
+    // ------------------------------------------------------------------
+    // context: samurai::initialize / finalize
+    // ------------------------------------------------------------------
+    inline std::shared_ptr<smr_ctx>& context_slot()
+    {
+        static std::shared_ptr<smr_ctx> ctx;
+        return ctx;
+    }
+
+    inline void initialize(int device = 0)
+    {
+        smr_ctx* c = nullptr;
+        check(smr_ctx_create(device, &c));
+        context_slot().reset(c, [](smr_ctx* p) { smr_ctx_destroy(p); });
+    }
+
+    inline void initialize(int& /* argc */, char** /* argv */)
+    {
+        initialize(0);
+    }
+
+    inline void finalize()
+    {
+        context_slot().reset();
+    }
+
+    inline smr_ctx* context()
+    {
+        if (!context_slot())
+        {
+            initialize(0);
+        }
+        return context_slot().get();
+    }
+
+    // ------------------------------------------------------------------
+    // configuration
+    // ------------------------------------------------------------------
+    template <class T, std::size_t dim>
+    struct Box
+    {
+        std::array<T, dim> min_corner, max_corner;
+        Box(const std::array<T, dim>& a, const std::array<T, dim>& b)
+            : min_corner(a)
+            , max_corner(b)
+        {
+        }
+    };
+
+    enum class mesh_id_t : int
+    {
+        cells            = SMR_CELLS,
+        cells_and_ghosts = SMR_CELLS_AND_GHOSTS,
+        proj_cells       = SMR_PROJ_CELLS,
+        union_cells      = SMR_UNION_CELLS,
+        reference        = SMR_REFERENCE
+    };
+
+    // mesh_config<dim> (mesh_config.hpp:16-432): the options this path uses
+    template <std::size_t dim_, int prediction_stencil_radius_ = 1>
+    class mesh_config
+    {
+      public:
+        static constexpr std::size_t dim = dim_;
+        mesh_config& min_level(int l)
+        {
+            c_.min_level = l;
+            return *this;
+        }
+        mesh_config& max_level(int l)
+        {
+            c_.max_level = l;
+            return *this;
+        }
+        mesh_config& max_stencil_size(int s)
+        {
+            c_.max_stencil_radius = s / 2;
+            return *this;
+        }
+        mesh_config& max_stencil_radius(int r)
+        {
+            c_.max_stencil_radius = r;
+            return *this;
+        }
+        mesh_config& disable_minimal_ghost_width()
+        {
+            return *this;
+        }
+        mesh_config& graduation_width(int w)
+        {
+            c_.graduation_width = w;
+            return *this;
+        }
+        const smr_mesh_config& raw() const
+        {
+            return c_;
+        }
+
+      private:
+        smr_mesh_config c_{int(dim_), 0, 6, prediction_stencil_radius_, 1, 1};
+    };
+
+    // ------------------------------------------------------------------
+    // mesh
+    // ------------------------------------------------------------------
+    template <std::size_t dim_>
+    struct Cell
+    {
+        static constexpr std::size_t dim = dim_;
+        int level;
+        std::array<int, dim_> indices;
+        int64_t index; // storage index
+        double length;
+        std::array<double, dim_> center() const
+        {
+            std::array<double, dim_> c;
+            for (std::size_t d = 0; d < dim_; ++d)
+            {
+                c[d] = length * (indices[d] + 0.5);
+            }
+            return c;
+        }
+    };
+
+    template <std::size_t dim_>
+    class MRMesh
+    {
+      public:
+        static constexpr std::size_t dim = dim_;
+        explicit MRMesh(const smr_mesh_config& cfg)
+        {
+            smr_mesh* m = nullptr;
+            check(smr_mesh_create_box(context(), &cfg, &m));
+            h_.reset(m, [](smr_mesh* p) { smr_mesh_destroy(p); });
+            cfg_ = cfg;
+        }
+        smr_mesh* handle() const
+        {
+            return h_.get();
+        }
+        std::size_t nb_cells(mesh_id_t id = mesh_id_t::cells) const
+        {
+            return std::size_t(smr_mesh_nb_cells(h_.get(), int(id)));
+        }
+        int min_level() const
+        {
+            return cfg_.min_level;
+        }
+        int max_level() const
+        {
+            return cfg_.max_level;
+        }
+        double cell_length(int level) const
+        {
+            return 1.0 / double(1 << level); // cell.hpp:17-20 on the unit box
+        }
+        double min_cell_length() const
+        {
+            return cell_length(cfg_.max_level);
+        }
+        std::vector<Cell<dim_>> cells(mesh_id_t id = mesh_id_t::cells) const
+        {
+            const int64_t n = smr_mesh_nb_cells(h_.get(), int(id));
+            std::vector<int64_t> raw(size_t(std::max<int64_t>(n, 0)) * 5);
+            check(smr_mesh_get_cells(h_.get(), int(id), raw.data()));
+            std::vector<Cell<dim_>> out(size_t(std::max<int64_t>(n, 0)));
+            for (int64_t k = 0; k < n; ++k)
+            {
+                const int64_t* r = &raw[size_t(5 * k)];
+                Cell<dim_>& c    = out[size_t(k)];
+                c.level          = int(r[0]);
+                for (std::size_t d = 0; d < dim_; ++d)
+                {
+                    c.indices[d] = int(r[1 + d]);
+                }
+                c.index  = r[4];
+                c.length = cell_length(c.level);
+            }
+            return out;
+        }
+
+      private:
+        std::shared_ptr<smr_mesh> h_;
+        smr_mesh_config cfg_{};
+    };
+
+    namespace mra
+    {
+        template <class T, std::size_t dim, class Config>
+        MRMesh<dim> make_mesh(const Box<T, dim>& box, const Config& cfg)
+        {
+            for (std::size_t d = 0; d < dim; ++d)
+            {
+                if (box.min_corner[d] != 0 || box.max_corner[d] != 1)
+                {
+                    throw std::invalid_argument("make_mesh: the domain must be the unit box");
+                }
+            }
+            return MRMesh<dim>(cfg.raw());
+        }
+    } // namespace mra
+
+    template <class Mesh, class F>
+    void for_each_cell(const Mesh& mesh, F&& f)
+    {
+        for (auto& c : mesh.cells())
+        {
+            f(c);
+        }
+    }
+
+    // ------------------------------------------------------------------
+    // fields and expressions
+    // ------------------------------------------------------------------
+    template <std::size_t dim_>
+    class ScalarField;
+
+    template <std::size_t dim_>
+    struct UpwindExpr
+    {
+        std::array<double, 3> a;
+        ScalarField<dim_>* u;
+    };
+    template <std::size_t dim_>
+    struct ScaledUpwind
+    {
+        double dt;
+        UpwindExpr<dim_> e;
+    };
+    template <std::size_t dim_>
+    struct UpwindStep
+    {
+        ScalarField<dim_>* u;
+        ScaledUpwind<dim_> s;
+    };
+    template <std::size_t dim_>
+    struct ScaledField
+    {
+        double a;
+        ScalarField<dim_>* f;
+    };
+    template <std::size_t dim_>
+    struct FieldStep
+    {
+        ScalarField<dim_>* u;
+        ScaledField<dim_> s;
+    };
+
+    template <std::size_t dim_>
+    class ScalarField
+    {
+      public:
+        static constexpr std::size_t dim = dim_;
+        ScalarField(const ScalarField&)            = delete;
+        ScalarField& operator=(const ScalarField&) = delete;
+        ScalarField(ScalarField&&)                 = default;
+        ScalarField& operator=(ScalarField&&)      = default;
+        ScalarField(std::string name, MRMesh<dim_>& mesh)
+            : name_(std::move(name))
+            , mesh_(&mesh)
+        {
+            smr_field* f = nullptr;
+            check(smr_field_create(mesh.handle(), name_.c_str(), &f));
+            h_.reset(f, [](smr_field* p) { smr_field_destroy(p); });
+        }
+        smr_field* handle()
+        {
+            flush();
+            return h_.get();
+        }
+        MRMesh<dim_>& mesh()
+        {
+            return *mesh_;
+        }
+        const std::string& name() const
+        {
+            return name_;
+        }
+        void resize()
+        {
+            check(smr_field_resize(h_.get()));
+            host_.clear();
+            dirty_ = false;
+        }
+        void fill(double v)
+        {
+            pull();
+            std::fill(host_.begin(), host_.end(), v);
+            dirty_ = true;
+        }
+        // host access by cell (storage index), as u[cell]
+        double& operator[](const Cell<dim_>& c)
+        {
+            pull();
+            dirty_ = true;
+            return host_.at(size_t(c.index));
+        }
+        // the device storage changed: the host mirror is stale
+        void touched()
+        {
+            host_.clear();
+        }
+        ScalarField& operator=(const UpwindStep<dim_>& st)
+        {
+            if (st.u == this)
+            {
+                throw std::invalid_argument("u = u - dt * upwind(a, u) needs a second field");
+            }
+            check(smr_upwind(st.s.e.a.data(), st.s.dt, st.u->handle(), handle()));
+            touched();
+            return *this;
+        }
+        ScalarField& operator=(const FieldStep<dim_>& st)
+        {
+            check(smr_field_sub_scaled(st.u->handle(), st.s.a, st.s.f->handle(), handle()));
+            touched();
+            return *this;
+        }
+
+      private:
+        void pull()
+        {
+            if (host_.empty())
+            {
+                const int64_t n = smr_field_size(h_.get());
+                host_.assign(size_t(std::max<int64_t>(n, 0)), 0.0);
+                check(smr_field_get_storage(h_.get(), host_.data()));
+            }
+        }
+        void flush()
+        {
+            if (dirty_)
+            {
+                check(smr_field_set_storage(h_.get(), host_.data()));
+                dirty_ = false;
+            }
+        }
+        std::string name_;
+        MRMesh<dim_>* mesh_;
+        std::shared_ptr<smr_field> h_;
+        std::vector<double> host_;
+        bool dirty_ = false;
+    };
+
+    template <class T, std::size_t dim>
+    ScalarField<dim> make_scalar_field(const std::string& name, MRMesh<dim>& mesh)
+    {
+        return ScalarField<dim>(name, mesh);
+    }
+
+    // make_field<double, 1> (README.md:104)
+    template <class T, std::size_t n_comp, std::size_t dim>
+    ScalarField<dim> make_field(const std::string& name, MRMesh<dim>& mesh)
+    {
+        static_assert(n_comp == 1, "scalar fields only");
+        return ScalarField<dim>(name, mesh);
+    }
+
+    template <std::size_t order>
+    struct Dirichlet
+    {
+    };
+
+    template <class BC, std::size_t dim>
+    void make_bc(ScalarField<dim>& u, double value)
+    {
+        check(smr_field_attach_dirichlet(u.handle(), value));
+    }
+
+    template <std::size_t dim>
+    void update_ghost_mr(ScalarField<dim>& u)
+    {
+        check(smr_update_ghost_mr(u.handle()));
+        u.touched();
+    }
+
+    template <std::size_t dim>
+    void swap(ScalarField<dim>& u, ScalarField<dim>& v)
+    {
+        check(smr_field_swap(u.handle(), v.handle()));
+        u.touched();
+        v.touched();
+    }
+
+    template <class Vec, std::size_t dim>
+    UpwindExpr<dim> upwind(const Vec& a, ScalarField<dim>& u)
+    {
+        UpwindExpr<dim> e{{0, 0, 0}, &u};
+        for (std::size_t d = 0; d < dim; ++d)
+        {
+            e.a[d] = a[d];
+        }
+        return e;
+    }
+    template <std::size_t dim>
+    ScaledUpwind<dim> operator*(double dt, const UpwindExpr<dim>& e)
+    {
+        return {dt, e};
+    }
+    template <std::size_t dim>
+    UpwindStep<dim> operator-(ScalarField<dim>& u, const ScaledUpwind<dim>& s)
+    {
+        return {&u, s};
+    }
+    template <std::size_t dim>
+    ScaledField<dim> operator*(double a, ScalarField<dim>& f)
+    {
+        return {a, &f};
+    }
+    template <std::size_t dim>
+    FieldStep<dim> operator-(ScalarField<dim>& u, const ScaledField<dim>& s)
+    {
+        return {&u, s};
+    }
+
+    // make_convection_upwind<Field>(velocity) (convection_lin.hpp:15-89)
+    template <class Field>
+    class ConvectionUpwind
+    {
+      public:
+        explicit ConvectionUpwind(const std::array<double, Field::dim>& v)
+            : v_(v)
+        {
+        }
+        Field operator()(Field& u) const
+        {
+            Field out("convection(" + u.name() + ")", u.mesh());
+            check(smr_convection_upwind(v_.data(), u.handle(), out.handle()));
+            u.touched();
+            out.touched();
+            return out;
+        }
+
+      private:
+        std::array<double, Field::dim> v_;
+    };
+    template <class Field, class Vec>
+    ConvectionUpwind<Field> make_convection_upwind(const Vec& v)
+    {
+        std::array<double, Field::dim> a{};
+        for (std::size_t d = 0; d < Field::dim; ++d)
+        {
+            a[d] = v[d];
+        }
+        return ConvectionUpwind<Field>(a);
+    }
+
+    // mra_config (mr/config.hpp:10-68)
+    class mra_config
+    {
+      public:
+        mra_config& epsilon(double e)
+        {
+            eps_ = e;
+            return *this;
+        }
+        mra_config& regularity(double r)
+        {
+            reg_ = r;
+            return *this;
+        }
+        double epsilon() const
+        {
+            return eps_;
+        }
+        double regularity() const
+        {
+            return reg_;
+        }
+
+      private:
+        double eps_ = 1e-4, reg_ = 1.0; // mr/config.hpp defaults
+    };
+
+    template <std::size_t dim>
+    class MRAdapt
+    {
+      public:
+        explicit MRAdapt(ScalarField<dim>& u)
+            : u_(&u)
+        {
+            smr_adapt* a = nullptr;
+            check(smr_adapt_create(u.handle(), &a));
+            h_.reset(a, [](smr_adapt* p) { smr_adapt_destroy(p); });
+        }
+        void operator()(const mra_config& cfg)
+        {
+            (*this)(cfg.epsilon(), cfg.regularity());
+        }
+        void operator()(double eps, double regularity)
+        {
+            check(smr_adapt_run(h_.get(), eps, regularity));
+            u_->touched();
+        }
+
+      private:
+        ScalarField<dim>* u_;
+        std::shared_ptr<smr_adapt> h_;
+    };
+
+    template <std::size_t dim>
+    MRAdapt<dim> make_MRAdapt(ScalarField<dim>& u)
+    {
+        return MRAdapt<dim>(u);
+    }
+} // namespace samurai_amd
+
+#ifdef SAMURAI_AMD_AS_SAMURAI
+namespace samurai = samurai_amd;
+#endif

@@ -51,71 +51,79 @@ def lib():
         L = ctypes.CDLL(LIB_PATH)
         vp, i32, i64, dbl = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double
         P, PP = ctypes.POINTER, lambda t: ctypes.POINTER(ctypes.POINTER(t))
-        L.smr_last_error.restype = ctypes.c_char_p
-        L.smr_ctx_create.argtypes = [i32, ctypes.POINTER(vp)]
-        L.smr_ctx_destroy.argtypes = [vp]
-        L.smr_ctx_synchronize.argtypes = [vp]
-        L.smr_mesh_create_box.argtypes = [vp, P(MeshConfig), ctypes.POINTER(vp)]
-        L.smr_mesh_create_from_cells.argtypes = [vp, P(MeshConfig), P(ctypes.c_int32), i64, ctypes.POINTER(vp)]
-        L.smr_mesh_destroy.argtypes = [vp]
-        L.smr_mesh_nb_cells.restype = i64
-        L.smr_mesh_nb_cells.argtypes = [vp, i32]
-        L.smr_mesh_get_cells.argtypes = [vp, i32, P(i64)]
-        L.smr_mesh_rebuild_from_tags.argtypes = [vp, P(ctypes.c_uint8), i64]
-        L.smr_mesh_validate.argtypes = [vp]
-        L.smr_ctx_stream.restype = vp
-        L.smr_ctx_stream.argtypes = [vp]
-        L.smr_mesh_upwind_bytes.restype = i64
-        L.smr_mesh_upwind_bytes.argtypes = [vp]
-        L.smr_timers_enable.argtypes = [i32]
-        L.smr_timers_report.restype = i64
-        L.smr_timers_report.argtypes = [ctypes.c_char_p, i64]
-        L.smr_field_create.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(vp)]
-        L.smr_field_destroy.argtypes = [vp]
-        L.smr_field_resize.argtypes = [vp]
-        L.smr_field_set_cells.argtypes = [vp, P(dbl)]
-        L.smr_field_get_cells.argtypes = [vp, P(dbl)]
-        L.smr_field_get_storage.argtypes = [vp, P(dbl)]
-        L.smr_field_set_storage.argtypes = [vp, P(dbl)]
-        L.smr_field_size.restype = i64
-        L.smr_field_size.argtypes = [vp]
-        L.smr_field_device_data.restype = vp
-        L.smr_field_device_data.argtypes = [vp]
-        L.smr_field_attach_dirichlet.argtypes = [vp, dbl]
-        L.smr_field_swap.argtypes = [vp, vp]
-        L.smr_update_ghost_mr.argtypes = [vp]
-        L.smr_upwind.argtypes = [P(dbl), dbl, vp, vp]
-        L.smr_adapt_create.argtypes = [vp, ctypes.POINTER(vp)]
-        L.smr_adapt_destroy.argtypes = [vp]
-        L.smr_adapt_run.argtypes = [vp, dbl, dbl]
-        L.smr_adapt_harten.argtypes = [vp, i32, dbl, dbl]
-        L.smr_adapt_last_size.restype = i64
-        L.smr_adapt_last_size.argtypes = [vp]
-        L.smr_adapt_last_tags.argtypes = [vp, P(ctypes.c_uint8), P(dbl)]
-        L.smr_batch_create.argtypes = [vp, i32, P(vp), P(vp), P(dbl), P(dbl), ctypes.POINTER(vp)]
-        L.smr_batch_destroy.argtypes = [vp]
-        L.smr_batch_adapt.argtypes = [vp, dbl, dbl]
-        L.smr_batch_advance.argtypes = [vp]
-        L.smr_batch_step.argtypes = [vp, dbl, dbl]
-        L.smr_batch_nb_cells.restype = i64
-        L.smr_batch_nb_cells.argtypes = [vp]
-        L.smr_batch_set_tuning.argtypes = [vp, i32, i32, i32]
-        L.smr_batch_get_tuning.argtypes = [vp, P(i32), P(i32), P(i32)]
-        L.smr_batch_autotune.argtypes = [vp, i32, ctypes.c_char_p]
-        L.smr_batch_load_tuning.argtypes = [vp, ctypes.c_char_p]
-        L.smr_convection_upwind.argtypes = [P(dbl), vp, vp]
-        L.smr_upwind_hand_loop.argtypes = [dbl, vp, vp]
-        L.smr_mesh_convection_terms.argtypes = [vp, P(dbl), P(i64), P(i64), P(i64), P(i64), P(i64), P(dbl)]
-        L.smr_adjoint_create.argtypes = [vp, i32, P(vp), P(dbl), P(dbl), i32, i32, ctypes.POINTER(vp)]
-        L.smr_adjoint_destroy.argtypes = [vp]
-        L.smr_adjoint_nb_cells.restype = i64
-        L.smr_adjoint_nb_cells.argtypes = [vp, i32]
-        L.smr_adjoint_set_velocity.argtypes = [vp, P(dbl)]
-        L.smr_adjoint_forward.argtypes = [vp, vp, vp]
-        L.smr_adjoint_backward.argtypes = [vp, vp, vp, P(dbl)]
-        L.smr_adjoint_set_tuning.argtypes = [vp, i32, i32, i32]
+        _sig(L, "smr_last_error", "restype", ctypes.c_char_p)
+        _sig(L, "smr_ctx_create", "argtypes", [i32, ctypes.POINTER(vp)])
+        _sig(L, "smr_ctx_destroy", "argtypes", [vp])
+        _sig(L, "smr_ctx_synchronize", "argtypes", [vp])
+        _sig(L, "smr_mesh_create_box", "argtypes", [vp, P(MeshConfig), ctypes.POINTER(vp)])
+        _sig(L, "smr_mesh_create_from_cells", "argtypes", [vp, P(MeshConfig), P(ctypes.c_int32), i64, ctypes.POINTER(vp)])
+        _sig(L, "smr_mesh_destroy", "argtypes", [vp])
+        _sig(L, "smr_mesh_nb_cells", "restype", i64)
+        _sig(L, "smr_mesh_nb_cells", "argtypes", [vp, i32])
+        _sig(L, "smr_mesh_get_cells", "argtypes", [vp, i32, P(i64)])
+        _sig(L, "smr_mesh_rebuild_from_tags", "argtypes", [vp, P(ctypes.c_uint8), i64])
+        _sig(L, "smr_mesh_validate", "argtypes", [vp])
+        _sig(L, "smr_ctx_stream", "restype", vp)
+        _sig(L, "smr_ctx_stream", "argtypes", [vp])
+        _sig(L, "smr_mesh_upwind_bytes", "restype", i64)
+        _sig(L, "smr_mesh_upwind_bytes", "argtypes", [vp])
+        _sig(L, "smr_timers_enable", "argtypes", [i32])
+        _sig(L, "smr_timers_report", "restype", i64)
+        _sig(L, "smr_timers_report", "argtypes", [ctypes.c_char_p, i64])
+        _sig(L, "smr_field_create", "argtypes", [vp, ctypes.c_char_p, ctypes.POINTER(vp)])
+        _sig(L, "smr_field_destroy", "argtypes", [vp])
+        _sig(L, "smr_field_resize", "argtypes", [vp])
+        _sig(L, "smr_field_set_cells", "argtypes", [vp, P(dbl)])
+        _sig(L, "smr_field_get_cells", "argtypes", [vp, P(dbl)])
+        _sig(L, "smr_field_get_storage", "argtypes", [vp, P(dbl)])
+        _sig(L, "smr_field_set_storage", "argtypes", [vp, P(dbl)])
+        _sig(L, "smr_field_size", "restype", i64)
+        _sig(L, "smr_field_size", "argtypes", [vp])
+        _sig(L, "smr_field_device_data", "restype", vp)
+        _sig(L, "smr_field_device_data", "argtypes", [vp])
+        _sig(L, "smr_field_attach_dirichlet", "argtypes", [vp, dbl])
+        _sig(L, "smr_field_swap", "argtypes", [vp, vp])
+        _sig(L, "smr_update_ghost_mr", "argtypes", [vp])
+        _sig(L, "smr_upwind", "argtypes", [P(dbl), dbl, vp, vp])
+        _sig(L, "smr_adapt_create", "argtypes", [vp, ctypes.POINTER(vp)])
+        _sig(L, "smr_adapt_destroy", "argtypes", [vp])
+        _sig(L, "smr_adapt_run", "argtypes", [vp, dbl, dbl])
+        _sig(L, "smr_adapt_harten", "argtypes", [vp, i32, dbl, dbl])
+        _sig(L, "smr_adapt_last_size", "restype", i64)
+        _sig(L, "smr_adapt_last_size", "argtypes", [vp])
+        _sig(L, "smr_adapt_last_tags", "argtypes", [vp, P(ctypes.c_uint8), P(dbl)])
+        _sig(L, "smr_batch_create", "argtypes", [vp, i32, P(vp), P(vp), P(dbl), P(dbl), ctypes.POINTER(vp)])
+        _sig(L, "smr_batch_destroy", "argtypes", [vp])
+        _sig(L, "smr_batch_adapt", "argtypes", [vp, dbl, dbl])
+        _sig(L, "smr_batch_advance", "argtypes", [vp])
+        _sig(L, "smr_batch_step", "argtypes", [vp, dbl, dbl])
+        _sig(L, "smr_batch_nb_cells", "restype", i64)
+        _sig(L, "smr_batch_nb_cells", "argtypes", [vp])
+        _sig(L, "smr_batch_set_tuning", "argtypes", [vp, i32, i32, i32])
+        _sig(L, "smr_batch_get_tuning", "argtypes", [vp, P(i32), P(i32), P(i32)])
+        _sig(L, "smr_batch_autotune", "argtypes", [vp, i32, ctypes.c_char_p])
+        _sig(L, "smr_batch_load_tuning", "argtypes", [vp, ctypes.c_char_p])
+        _sig(L, "smr_convection_upwind", "argtypes", [P(dbl), vp, vp])
+        _sig(L, "smr_upwind_hand_loop", "argtypes", [dbl, vp, vp])
+        _sig(L, "smr_field_sub_scaled", "argtypes", [vp, dbl, vp, vp])
+        _sig(L, "smr_mesh_convection_terms", "argtypes", [vp, P(dbl), P(i64), P(i64), P(i64), P(i64), P(i64), P(dbl)])
+        _sig(L, "smr_adjoint_create", "argtypes", [vp, i32, P(vp), P(dbl), P(dbl), i32, i32, ctypes.POINTER(vp)])
+        _sig(L, "smr_adjoint_destroy", "argtypes", [vp])
+        _sig(L, "smr_adjoint_nb_cells", "restype", i64)
+        _sig(L, "smr_adjoint_nb_cells", "argtypes", [vp, i32])
+        _sig(L, "smr_adjoint_set_velocity", "argtypes", [vp, P(dbl)])
+        _sig(L, "smr_adjoint_forward", "argtypes", [vp, vp, vp])
+        _sig(L, "smr_adjoint_backward", "argtypes", [vp, vp, vp, P(dbl)])
+        _sig(L, "smr_adjoint_set_tuning", "argtypes", [vp, i32, i32, i32])
         _lib = L
     return _lib
+
+
+def _sig(L, name, kind, value):
+    """Declare a C signature; a symbol missing from an older build stays
+    undeclared and fails when that entry point is called."""
+    if hasattr(L, name):
+        setattr(getattr(L, name), kind, value)
 
 
 def _check(rc):
@@ -297,6 +305,11 @@ def upwind(a, dt, u, unp1):
     """unp1 = u - dt * upwind(a, u) on mesh[cells]."""
     av = np.ascontiguousarray(a, dtype=np.float64)
     _check(lib().smr_upwind(_dp(av), float(dt), u.h, unp1.h))
+
+
+def sub_scaled(u, a, v, out):
+    """out = u - a * v on the cells (unp1 = u - dt * conv(u))."""
+    _check(lib().smr_field_sub_scaled(u.h, float(a), v.h, out.h))
 
 
 def upwind_hand_loop(dt, u, unp1):

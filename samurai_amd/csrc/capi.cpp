@@ -928,4 +928,15 @@ extern "C"
                 return 0;
             });
     }
+
+    int smr_field_sub_scaled(smr_field* u, double a, smr_field* v, smr_field* out)
+    {
+        return guarded(
+            [&]
+            {
+                require(u && v && out, "null field");
+                smr::sub_scaled(u->f, a, v->f, out->f);
+                return 0;
+            });
+    }
 }

@@ -143,6 +143,8 @@ namespace smr::dev
     void set_upwind_variant(int id);
     void upwind(int dim, const List& l, int64_t c0, int64_t c1, const double* u, double* unp1, double a0, double a1, double a2,
                 double dt, hipStream_t s);
+    // out = u - a * v on the cells of a list
+    void sub_scaled(const List& l, int64_t c0, int64_t c1, const double* u, double a, const double* v, double* out, hipStream_t s);
     // the README's for_each_interval upwind loop (2D, a = (1, 1))
     void hand_loop(const List& l, int64_t c0, int64_t c1, const double* u, double* unp1, double dt, hipStream_t s);
     void project(int dim, const List& l, int64_t c0, int64_t c1, double* dst, const double* src, hipStream_t s);

@@ -159,6 +159,12 @@ namespace smr::dev
         template <int DIM>
         SMR_CELL_KERNEL(k_maximum, (op_maximum<DIM>(l, c, tag)), uint8_t* tag)
         SMR_CELL_KERNEL(k_tag_init, (op_tag_init(l, c, tag)), uint8_t* tag)
+        SMR_CELL_KERNEL(k_sub_scaled, ({
+                            const Cell cell  = locate(l, c);
+                            const int64_t i  = bases(l, cell.run)[0] + cell.x;
+                            out[i]           = u[i] - a * v[i];
+                        }),
+                        const double* u, double a, const double* v, double* out)
         SMR_CELL_KERNEL(k_hand_loop, ({
                             const Cell cell         = locate(l, c);
                             const int64_t* b        = bases(l, cell.run);
@@ -386,6 +392,14 @@ namespace smr::dev
         if (c1 > c0)
         {
             SMR_DISPATCH_DIM(k_maximum, tag)
+        }
+    }
+
+    void sub_scaled(const List& l, int64_t c0, int64_t c1, const double* u, double a, const double* v, double* out, hipStream_t s)
+    {
+        if (c1 > c0)
+        {
+            hipLaunchKernelGGL(k_sub_scaled, dim3(nblocks(c1 - c0)), dim3(BLOCK), 0, s, l, c0, c1, u, a, v, out);
         }
     }
 

@@ -608,6 +608,21 @@ namespace smr
         dev::upwind(dim, dm.cells.d, 0, dm.cells.d.ncells, u.d, unp1.d, a[0], a[1], dim == 3 ? a[2] : 0.0, dt, m.ctx->stream);
     }
 
+    void sub_scaled(Field& u, double a, Field& v, Field& out)
+    {
+        u.check_current();
+        v.check_current();
+        out.check_current();
+        if (u.mesh != v.mesh || u.mesh != out.mesh)
+        {
+            throw std::invalid_argument("field expression: fields on different meshes");
+        }
+        Mesh& m     = *u.mesh;
+        DevMesh& dm = m.device();
+        ScopedTimer timer(m.ctx, "field expressions");
+        dev::sub_scaled(dm.cells.d, 0, dm.cells.d.ncells, u.d, a, v.d, out.d, m.ctx->stream);
+    }
+
     void upwind_hand_loop(double dt, Field& u, Field& unp1)
     {
         u.check_current();

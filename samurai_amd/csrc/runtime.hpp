@@ -217,6 +217,8 @@ namespace smr
 
     void update_ghost_mr(Field& f);
     void upwind(const double* a, double dt, Field& u, Field& unp1);
+    // out = u - a * v on the cells (the field expression of the flux-based demos)
+    void sub_scaled(Field& u, double a, Field& v, Field& out);
     // README.md:135-154: unp1 = u - dt / dx * (u - u(i-1) + u - u(j-1)) on the cells (2D)
     void upwind_hand_loop(double dt, Field& u, Field& unp1);
     // out = make_convection_upwind(v)(u), after update_ghost_mr(u)
