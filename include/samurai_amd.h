@@ -147,8 +147,8 @@ int64_t smr_mesh_upwind_bytes(const smr_mesh* mesh);
  * object {name: {"seconds": s, "calls": n}} written into buf (truncated to
  * size); returns its full length. */
 /* Tuning knob (no counterpart in the reference): selects the launch shape of
- * the upwind kernel, 0 <= id < smr_upwind_variant_count(); used by the
- * autotuner.  Every variant computes identical results. */
+ * the upwind kernel, 0 <= id < smr_upwind_variant_count(), or -1 for the
+ * default shape.  Every variant computes identical results. */
 int smr_upwind_variant_count(void);
 int smr_set_upwind_variant(int id);
 int smr_timers_enable(int on);

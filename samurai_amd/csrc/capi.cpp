@@ -336,11 +336,12 @@ extern "C"
 
     int smr_set_upwind_variant(int id)
     {
-        if (id < 0 || id >= smr::dev::upwind_variant_count())
+        if (id < -1 || id >= smr::dev::upwind_variant_count())
         {
             g_error = "upwind variant out of range";
             return SMR_ERR_ARGUMENT;
         }
+        // -1: the default shape
         smr::dev::set_upwind_variant(id);
         return 0;
     }

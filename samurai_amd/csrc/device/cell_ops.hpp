@@ -204,6 +204,19 @@ namespace smr::dev
         return uc - dt * (S * double(1 << level));
     }
 
+    // upwind_value<2> on values already loaded: left, right, down, up neighbours
+    // of uc (same arithmetic, same order)
+    __device__ __forceinline__ double upwind_vals_2d(double uc, double ul, double ur, double ud, double uu, int level, double ha0,
+                                                     double hb0, double ha1, double hb1, double dt)
+    {
+        const double FL = flux_h(ha0, hb0, ul, uc);
+        const double FR = flux_h(ha0, hb0, uc, ur);
+        const double FD = flux_h(ha1, hb1, ud, uc);
+        const double FU = flux_h(ha1, hb1, uc, uu);
+        const double S  = -FL + FR + -FD + FU;
+        return uc - dt * (S * double(1 << level));
+    }
+
     // projection_op_ (numeric/projection.hpp:15-65): taps dst, then the 2^(dim-1) fine rows
     template <int DIM>
     __device__ __forceinline__ void op_project(const List& l, int64_t c, double* dst, const double* src)

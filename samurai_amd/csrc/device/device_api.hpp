@@ -30,9 +30,14 @@ namespace smr::dev
         int64_t* base;       // nruns * ntaps
         const int8_t* taps;  // 7 per tap: table, dl, dx, dy, dz, wlo, whi
         const int32_t* tiles; // (run, x0) of every 64-cell tile of a run, runs in order
+        // column walks (2D cells list): (first run, x0, number of runs) per walk;
+        // next[r] is the run right above run r with the same x range, or -1
+        const int32_t* walks;
+        const int32_t* next;
         int64_t nruns;
         int64_t ncells;
         int64_t ntiles;
+        int64_t nwalks;
         int ntaps;
     };
 

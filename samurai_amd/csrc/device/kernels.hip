@@ -136,6 +136,64 @@ namespace smr::dev
             out[row[r]] = acc;
         }
 
+        // Upwind (2D) by column walks: a wavefront owns 64 columns of a stack of
+        // runs with one x range at rows y, y+1, ...; walking up, the row above
+        // becomes the centre row and the centre row the row below, so each row
+        // is fetched once per walk (plus the x-1 / x+1 neighbours of the centre
+        // row).  Same arithmetic as k_upwind.
+        template <bool NT>
+        __global__ __launch_bounds__(BLOCK) void k_upwind_walk(List l, const double* __restrict__ u, double* __restrict__ unp1, double a0,
+                                                               double a1, double dt, unsigned nlogical)
+        {
+            const int64_t blk = xcd_tile(nlogical);
+            if (blk < 0)
+            {
+                return;
+            }
+            const int64_t t = blk * (BLOCK / 64) + (threadIdx.x >> 6);
+            if (t >= l.nwalks)
+            {
+                return;
+            }
+            int32_t r         = __builtin_amdgcn_readfirstlane(l.walks[3 * t]);
+            const int32_t xs  = __builtin_amdgcn_readfirstlane(l.walks[3 * t + 1]);
+            const int32_t n   = __builtin_amdgcn_readfirstlane(l.walks[3 * t + 2]);
+            const int32_t x   = xs + int32_t(threadIdx.x & 63);
+            const int32_t* iv = l.iv + 5 * int64_t(r);
+            if (x >= iv[1])
+            {
+                return; // every run of the walk has the same x range
+            }
+            const int level  = iv[4];
+            const double ha0 = .5 * a0, hb0 = .5 * fabs(a0), ha1 = .5 * a1, hb1 = .5 * fabs(a1);
+            const int64_t* b = l.base + int64_t(r) * l.ntaps;
+            double ud        = u[b[1] + x];
+            double uc        = u[b[0] + x];
+            for (int32_t k = 0; k < n; ++k)
+            {
+                const double uu = u[b[2] + x];
+                const double ul = u[b[0] + x - 1];
+                const double ur = u[b[0] + x + 1];
+                const double v  = upwind_vals_2d(uc, ul, ur, ud, uu, level, ha0, hb0, ha1, hb1, dt);
+                if constexpr (NT)
+                {
+                    __builtin_nontemporal_store(v, unp1 + b[0] + x);
+                }
+                else
+                {
+                    unp1[b[0] + x] = v;
+                }
+                if (k + 1 < n)
+                {
+                    // the run above: its own row is this run's upper row, its lower row this run's row
+                    r  = __builtin_amdgcn_readfirstlane(l.next[r]);
+                    b  = l.base + int64_t(r) * l.ntaps;
+                    ud = uc;
+                    uc = uu;
+                }
+            }
+        }
+
 #define SMR_CELL_KERNEL(NAME, BODY, ...)                                                                                             \
     __global__ __launch_bounds__(BLOCK) void NAME(List l, int64_t c0, int64_t c1, __VA_ARGS__)                                        \
     {                                                                                                                                  \
@@ -196,8 +254,9 @@ namespace smr::dev
         }
 
         // launch shapes of the upwind kernel: cells per thread (0: wave tiles,
-        // -1: wave tiles when at least 3/4 of the tile lanes hold a cell, else 4
-        // cells per thread), XCD remap, non-temporal stores
+        // -2: column walks (2D), -1: column walks (2D) or wave tiles (3D) when at
+        // least 3/4 of the lanes hold a cell, else 4 cells per thread), XCD
+        // remap, non-temporal stores
         struct UpwindVariant
         {
             int id, cpt;
@@ -206,8 +265,9 @@ namespace smr::dev
         constexpr UpwindVariant kUpwindVariants[] = {{0, 1, false, false}, {1, 1, true, false}, {2, 2, false, false},
                                                      {3, 2, true, false},  {4, 4, false, false}, {5, 4, true, true},
                                                      {6, 1, false, true},  {7, 8, false, false}, {8, 0, true, true},
-                                                     {9, 0, true, false},  {10, -1, true, true}};
-        constexpr int kNumUpwindVariants    = 11;
+                                                     {9, 0, true, false},  {10, -1, true, true}, {11, -2, true, true},
+                                                     {12, -2, true, false}};
+        constexpr int kNumUpwindVariants    = 13;
         constexpr int kDefaultUpwindVariant = 10;
         int g_upwind_variant                = -1;
 
@@ -259,10 +319,29 @@ namespace smr::dev
             return;
         }
         UpwindVariant v = upwind_variant();
-        if (v.cpt < 0)
+        const bool whole = c0 == 0 && c1 == l.ncells;
+        if (v.cpt == -1)
         {
             const bool dense = 4 * l.ncells >= 3 * 64 * l.ntiles;
-            v                = dense ? kUpwindVariants[8] : kUpwindVariants[5];
+            v                = !dense ? kUpwindVariants[5] : (dim == 2 && l.nwalks > 0 ? kUpwindVariants[11] : kUpwindVariants[8]);
+        }
+        if (v.cpt == -2)
+        {
+            if (dim == 2 && l.nwalks > 0 && whole)
+            {
+                const int64_t nblk   = (l.nwalks + BLOCK / 64 - 1) / (BLOCK / 64);
+                const unsigned ngrid = unsigned((nblk + 7) / 8 * 8);
+                if (v.nt)
+                {
+                    hipLaunchKernelGGL((k_upwind_walk<true>), dim3(ngrid), dim3(BLOCK), 0, s, l, u, unp1, a0, a1, dt, unsigned(nblk));
+                }
+                else
+                {
+                    hipLaunchKernelGGL((k_upwind_walk<false>), dim3(ngrid), dim3(BLOCK), 0, s, l, u, unp1, a0, a1, dt, unsigned(nblk));
+                }
+                return;
+            }
+            v = kUpwindVariants[5];
         }
         if (v.cpt == 0 && c0 == 0 && c1 == l.ncells)
         {

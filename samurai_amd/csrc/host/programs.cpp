@@ -396,6 +396,7 @@ namespace smr
     WorkList build_cells_list(const MRMeshData& m)
     {
         WorkList w;
+        w.walk = m.cfg.dim == 2;
         w.taps.push_back({0, 0, 0, 0, 0, -1, 1});
         w.taps.push_back({0, 0, 0, -1, 0, 0, 0});
         w.taps.push_back({0, 0, 0, 1, 0, 0, 0});

@@ -43,6 +43,8 @@ namespace smr
         std::vector<int64_t> pref{0};   // cell prefix over runs
         std::vector<Tap> taps;
         std::vector<int64_t> level_run; // first run of each level (MAXL+1)
+        // column walks for the upwind list: runs stacked at y, y+1, ... with one x range
+        bool walk = false;
 
         int64_t nruns() const { return int64_t(iv.size() / 5); }
         int64_t ncells() const { return pref.back(); }

@@ -287,13 +287,81 @@ namespace smr
                 tiles.push_back(x);
             }
         }
+        // column walks: chains of runs stacked in y with equal x ranges, cut in
+        // 64-column strips of at most kWalkRows runs
+        std::vector<int32_t> next, walks;
+        if (w.walk)
+        {
+            constexpr int kWalkRows = 32;
+            next.assign(size_t(nruns), -1);
+            std::vector<uint8_t> below(size_t(nruns), 0);
+            // runs are sorted by level, then (z, y), then x: the run above r with
+            // the same x range is found by a merge over consecutive rows
+            int64_t r = 0;
+            while (r < nruns)
+            {
+                const int32_t* a = &w.iv[size_t(5 * r)];
+                int64_t e        = r; // end of the row of r
+                while (e < nruns && w.iv[size_t(5 * e + 2)] == a[2] && w.iv[size_t(5 * e + 3)] == a[3] && w.iv[size_t(5 * e + 4)] == a[4])
+                {
+                    ++e;
+                }
+                // row above: starts at e if it is row y+1 of the same level and z
+                if (e < nruns && w.iv[size_t(5 * e + 2)] == a[2] + 1 && w.iv[size_t(5 * e + 3)] == a[3] && w.iv[size_t(5 * e + 4)] == a[4])
+                {
+                    int64_t q = e;
+                    for (int64_t k = r; k < e; ++k)
+                    {
+                        const int32_t x0 = w.iv[size_t(5 * k)], x1 = w.iv[size_t(5 * k + 1)];
+                        while (q < nruns && w.iv[size_t(5 * q + 2)] == a[2] + 1 && w.iv[size_t(5 * q + 3)] == a[3] &&
+                               w.iv[size_t(5 * q + 4)] == a[4] && w.iv[size_t(5 * q)] < x0)
+                        {
+                            ++q;
+                        }
+                        if (q < nruns && w.iv[size_t(5 * q + 2)] == a[2] + 1 && w.iv[size_t(5 * q + 3)] == a[3] &&
+                            w.iv[size_t(5 * q + 4)] == a[4] && w.iv[size_t(5 * q)] == x0 && w.iv[size_t(5 * q + 1)] == x1)
+                        {
+                            next[size_t(k)] = int32_t(q);
+                            below[size_t(q)] = 1;
+                        }
+                    }
+                }
+                r = e;
+            }
+            for (int64_t s0 = 0; s0 < nruns; ++s0)
+            {
+                if (below[size_t(s0)])
+                {
+                    continue;
+                }
+                const int32_t x0 = w.iv[size_t(5 * s0)], x1 = w.iv[size_t(5 * s0 + 1)];
+                for (int32_t xs = x0; xs < x1; xs += 64)
+                {
+                    int64_t k = s0;
+                    while (k >= 0)
+                    {
+                        const int64_t first = k;
+                        int32_t n           = 0;
+                        while (k >= 0 && n < kWalkRows)
+                        {
+                            ++n;
+                            k = next[size_t(k)];
+                        }
+                        walks.insert(walks.end(), {int32_t(first), xs, n});
+                    }
+                }
+            }
+        }
         const size_t oiv  = b_.add(w.iv.data(), w.iv.size() * 4);
         const size_t opf  = b_.add(w.pref.data(), w.pref.size() * 8);
         const size_t oh   = b_.add(l.hint.data(), l.hint.size() * 4);
         const size_t ot   = b_.add(taps.data(), taps.size());
         const size_t otl  = b_.add(tiles.data(), tiles.size() * 4);
+        const size_t onx  = b_.add(next.data(), next.size() * 4);
+        const size_t owk  = b_.add(walks.data(), walks.size() * 4);
         const size_t obas = b_.reserve(size_t(nruns) * w.taps.size() * 8);
         const int64_t ntiles = int64_t(tiles.size() / 2);
+        const int64_t nwalks = int64_t(walks.size() / 3);
         const int ntaps      = int(w.taps.size());
         Blob* b              = &b_;
         binds_.push_back(
@@ -304,6 +372,9 @@ namespace smr
                 l.d.hint   = reinterpret_cast<const int32_t*>(b->dptr(oh));
                 l.d.taps   = reinterpret_cast<const int8_t*>(b->dptr(ot));
                 l.d.tiles  = reinterpret_cast<const int32_t*>(b->dptr(otl));
+                l.d.next   = nwalks > 0 ? reinterpret_cast<const int32_t*>(b->dptr(onx)) : nullptr;
+                l.d.walks  = nwalks > 0 ? reinterpret_cast<const int32_t*>(b->dptr(owk)) : nullptr;
+                l.d.nwalks = nwalks;
                 l.d.base   = reinterpret_cast<int64_t*>(b->dptr(obas));
                 l.d.ntiles = ntiles;
                 l.d.nruns  = nruns;
