@@ -190,6 +190,47 @@ def bench_batched(sa, ctx, args, eps, reg):
     }
 
 
+def bench_3d(sa, ctx, args):
+    """The 3D workload: unit cube, a = (1, 1, 1), L2..8, sphere r = 0.15 at the
+    centre, eps 1e-3, regularity 2, dt = 0.4 * 2^-8 / 3, Dirichlet 0."""
+    from initial_conditions import sphere_3d
+    t0 = time.perf_counter()
+    mesh = sa.make_mesh(ctx, sa.mesh_config(3, 2, 8, 1))
+    u = sa.make_scalar_field("u", mesh)
+    u.set_cells(sphere_3d(mesh.cells()))
+    sa.make_bc_dirichlet(u, 0.0)
+    unp1 = sa.make_scalar_field("unp1", mesh)
+    adapt = sa.make_MRAdapt(u)
+    adapt(1e-3, 2.0)
+    setup = time.perf_counter() - t0
+    a = np.array([1.0, 1.0, 1.0])
+    dt = 0.4 * 2.0 ** -8 / 3
+
+    def step():
+        adapt(1e-3, 2.0)
+        n = mesh.nb_cells()
+        sa.update_ghost_mr(u)
+        unp1.resize()
+        sa.upwind(a, dt, u, unp1)
+        sa.swap(u, unp1)
+        return n
+
+    for _ in range(args.warmup):
+        step()
+    ctx.synchronize()
+    updates = 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps_3d):
+        updates += step()
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    progress(f"3D L2..8: {args.steps_3d} steps in {wall:.2f} s")
+    return {"workload": "3D upwind advection, unit cube, a=(1,1,1), MRMesh L2..8, sphere r=0.15 at the centre, "
+                        "eps=1e-3, regularity 2, dt=0.4*2^-8/3, Dirichlet 0",
+            "value": updates / wall, "unit": "cell-updates/s", "steps": args.steps_3d,
+            "ms_per_step": wall / args.steps_3d * 1e3, "mean_leaves": updates / args.steps_3d, "setup_s": setup}
+
+
 def bench_adjoint(sa, ctx, args, eps, reg):
     """Gradients of L = 0.5 sum area (u_T - u_target)^2 after 100 frozen-mesh
     steps on 32 instances (L2..10, adapted once), w.r.t. u0 and a; u_target is
@@ -294,6 +335,7 @@ def main():
     p.add_argument("--batch-min-max-level", type=int, default=7)
     p.add_argument("--batch-max-max-level", type=int, default=10)
     p.add_argument("--retune", action="store_true", help="rerun the batch autotuner even if a cached shape exists")
+    p.add_argument("--steps-3d", type=int, default=10, help="timed steps of the 3D workload (0 disables)")
     p.add_argument("--adjoint", type=int, default=32, help="instances of the adjoint workload (0 disables)")
     p.add_argument("--adjoint-steps", type=int, default=100)
     p.add_argument("--adjoint-reps", type=int, default=2)
@@ -418,6 +460,10 @@ def main():
     del bufs
     progress(f"uniform L12 upwind {u_ms * 1e3:.1f} us ({uniform_gbs:.0f} GB/s), copy {copy_gbs:.0f} GB/s")
 
+    plain_3d = None
+    if args.steps_3d > 0:
+        plain_3d = bench_3d(sa, ctx, args)
+
     batched = None
     if args.batch > 0:
         batched = bench_batched(sa, ctx, args, eps, reg)
@@ -459,6 +505,7 @@ def main():
                               "frac": adaptive_gbs / HBM_PEAK_GBS},
         "cpu_baseline": cpu,
         "phases": phases,
+        "plain_3d": plain_3d,
         "batched": batched,
         "adjoint": adjoint,
     }
