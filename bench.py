@@ -103,6 +103,45 @@ def cpu_baseline(cells, values, a, dt, eps, reg, budget_s):
                       f"{setup:.1f} s untimed; single thread, dense-set oracle"}
 
 
+def _oracle_worker(job):
+    """One instance on the oracle for `budget` seconds (a worker process of
+    cpu_baseline_parallel; imports nothing that touches the GPU)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import OracleSim
+    cells, values, min_level, max_level, a, dt, eps, reg, adapt, budget = job
+    o = OracleSim(2, min_level, max_level, pred_s=1, cells=cells)
+    o.set_values(values)
+    o.set_dirichlet(0.0)
+    steps, updates, spent = 0, 0, 0.0
+    while steps == 0 or spent < budget:
+        t = time.perf_counter()
+        if adapt:
+            o.adapt(eps, reg)
+        n = o.nb_cells()
+        o.upwind_step(a, dt)
+        spent += time.perf_counter() - t
+        steps += 1
+        updates += n
+    return updates, spent, steps
+
+
+def cpu_baseline_parallel(jobs, label):
+    """The oracle on several instances at once, one process per instance
+    (processes started fresh, not forked from this GPU process)."""
+    import multiprocessing as mp
+    workers = max(1, min(len(jobs), 16, os.cpu_count() or 1))
+    jobs = jobs[:workers]
+    t0 = time.perf_counter()
+    with mp.get_context("spawn").Pool(workers) as pool:
+        res = pool.map(_oracle_worker, jobs)
+    wall = time.perf_counter() - t0
+    updates = sum(r[0] for r in res)
+    busy = max(r[1] for r in res)
+    return {"value": updates / busy, "unit": "cell-updates/s", "cores": workers, "kind": "port",
+            "sample": f"{label}: {workers} instances, one oracle process each, {sum(r[2] for r in res)} steps in "
+                      f"{busy:.1f} s of concurrent CPU time ({wall:.1f} s wall with process start); dense-set oracle"}
+
+
 def make_batch_set(sa, ctx, n, levels, seed0):
     """n instances of the batched workload (tests/initial_conditions.py:batch_instance)."""
     from initial_conditions import batch_field, batch_instance
@@ -176,6 +215,12 @@ def bench_batched(sa, ctx, args, eps, reg):
     pwall = time.perf_counter() - t0
     progress(f"plain loop: {args.batch_steps} steps in {pwall:.2f} s")
     leaves = [x["mesh"].nb_cells() for x in bset]
+    cpu = None
+    if args.cpu_budget > 0:
+        jobs = [(x["mesh"].cells(), x["u"].cells(), 2, x["mesh"].max_level(), x["a"], x["dt"], eps, reg, True,
+                 args.cpu_budget / 2) for x in bset]
+        cpu = cpu_baseline_parallel(jobs, "MRadaptation + update_ghost_mr + upwind on the batched instances' final meshes")
+        progress(f"batched cpu baseline: {cpu['value']:.3g} cell-updates/s on {cpu['cores']} cores")
     return {
         "workload": f"{args.batch} independent 2D advection instances, max_level U{{{levels[0]}..{levels[1]}}}, "
                     f"min_level 2, unit velocity at angle U[0,2pi), 1-5 seeded boxes/Gaussians, eps=1e-4, "
@@ -186,7 +231,7 @@ def bench_batched(sa, ctx, args, eps, reg):
         "speedup_vs_plain_loop": (updates / bwall) / (pupdates / pwall),
         "leaves_total": int(sum(leaves)), "leaves_min": int(min(leaves)), "leaves_max": int(max(leaves)),
         "tuning": batch.tuning(), "tuning_from_cache": bool(tuned and not args.retune),
-        "setup_s": setup,
+        "setup_s": setup, "cpu_baseline": cpu,
     }
 
 
@@ -306,6 +351,14 @@ def bench_adjoint(sa, ctx, args, eps, reg):
     prob.set_velocity(a)
     fd_a = (lp - lm) / (2 * h)
     ncells = prob.nb_cells()
+    cpu = None
+    if args.cpu_budget > 0:
+        off = prob.offsets()
+        u0h = u0.cpu().numpy()
+        jobs = [(c, u0h[off[k]:off[k + 1]], 2, lmax, a[k], 0.5 * 2.0 ** -lmax, eps, reg, False, args.cpu_budget / 2)
+                for k, c in enumerate(cells)]
+        cpu = cpu_baseline_parallel(jobs, "frozen-mesh forward steps (update_ghost_mr + upwind) of the adjoint instances")
+        progress(f"adjoint cpu baseline: {cpu['value']:.3g} cell-updates/s on {cpu['cores']} cores")
     return {
         "workload": f"{n} frozen 2D instances L2..{lmax} (seeded 3 boxes + 2 Gaussians, adapted once with eps=1e-4), "
                     f"{steps} upwind steps, a=(cos pi/5, sin pi/5), checkpoint every {every}, "
@@ -317,7 +370,7 @@ def bench_adjoint(sa, ctx, args, eps, reg):
         "backward_cell_updates_per_s": ncells * steps / t_bwd,
         "fd_u0_rel_err": abs(fd_u - ad_u) / abs(ad_u), "fd_a_rel_err": abs(fd_a - ga[0, 0]) / abs(ga[0, 0]),
         "fd_h": h, "tuning": "block 256, 1 cell per thread, cooperative fused programs",
-        "setup_s": setup,
+        "setup_s": setup, "cpu_baseline_forward": cpu,
     }
 
 
