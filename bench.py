@@ -190,6 +190,13 @@ def bench_batched(sa, ctx, args, eps, reg):
     ctx.synchronize()
     bwall = time.perf_counter() - t0
     progress(f"batched: {args.batch_steps} steps in {bwall:.2f} s")
+    # phase breakdown of one more step (the timers synchronize)
+    sa.timers_reset()
+    sa.timers_enable(True)
+    batch.step(eps, reg)
+    ctx.synchronize()
+    sa.timers_enable(False)
+    bphases = {k: {"ms": v["seconds"] * 1e3, "calls": v["calls"]} for k, v in sa.timers_report().items()}
 
     adapts = [sa.make_MRAdapt(x["u"]) for x in pset]
 
@@ -231,7 +238,7 @@ def bench_batched(sa, ctx, args, eps, reg):
         "speedup_vs_plain_loop": (updates / bwall) / (pupdates / pwall),
         "leaves_total": int(sum(leaves)), "leaves_min": int(min(leaves)), "leaves_max": int(max(leaves)),
         "tuning": batch.tuning(), "tuning_from_cache": bool(tuned and not args.retune),
-        "setup_s": setup, "cpu_baseline": cpu,
+        "setup_s": setup, "cpu_baseline": cpu, "phases_one_step": bphases,
     }
 
 

@@ -337,8 +337,11 @@ namespace smr
         std::vector<std::vector<std::array<int32_t, 4>>> add_m(MAXL), rm_m(MAXL), add_p(MAXL), rm_p(MAXL);
         const CellArray& cells = mesh.ids[CELLS];
         const int start = ca_min_level(cells), end = ca_max_level(cells) + 1;
-        for (int level = start; level < end; ++level)
+        // level l only appends to add_m[l-1], rm_m[l], add_p[l+1] and rm_p[l]:
+        // the levels are independent
+        parallel_for(std::max(end - start, 0), [&](int li)
         {
+            const int level    = start + li;
             const LevelSet& cl = cells[size_t(level)];
             for (size_t r = 0; r < cl.nrows(); ++r)
             {
@@ -375,16 +378,19 @@ namespace smr
                     }
                 }
             }
-        }
+        });
         CellArray out = empty_cell_array(dim);
-        for (int level = mesh.cfg.min_level; level <= mesh.cfg.max_level; ++level)
-        {
-            LevelSet am = from_runs(dim, level, add_m[size_t(level)]);
-            LevelSet ap = from_runs(dim, level, add_p[size_t(level)]);
-            LevelSet rm = from_runs(dim, level, rm_m[size_t(level)]);
-            LevelSet rp = from_runs(dim, level, rm_p[size_t(level)]);
-            out[size_t(level)] = set_difference(set_union(cells[size_t(level)], am, ap), set_union(rm, rp));
-        }
+        const int lmin = mesh.cfg.min_level, lmax = mesh.cfg.max_level;
+        parallel_for(lmax - lmin + 1,
+                     [&](int li)
+                     {
+                         const int level    = lmin + li;
+                         LevelSet am        = from_runs(dim, level, add_m[size_t(level)]);
+                         LevelSet ap        = from_runs(dim, level, add_p[size_t(level)]);
+                         LevelSet rm        = from_runs(dim, level, rm_m[size_t(level)]);
+                         LevelSet rp        = from_runs(dim, level, rm_p[size_t(level)]);
+                         out[size_t(level)] = set_difference(set_union(cells[size_t(level)], am, ap), set_union(rm, rp));
+                     });
         return out;
     }
 
