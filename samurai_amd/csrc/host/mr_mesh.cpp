@@ -1,6 +1,9 @@
 #include "mr_mesh.hpp"
 #include "parallel.hpp"
 
+#include <map>
+#include <mutex>
+
 #include <algorithm>
 #include <cstdio>
 #include <stdexcept>
@@ -236,9 +239,24 @@ namespace smr
             m.ids[REFERENCE] = cell_list;
         }
 
-        // Mesh_base::construct_corners (mesh.hpp:913-997)
+        // Mesh_base::construct_corners (mesh.hpp:913-997).  The corners depend on
+        // the domain at max_level only, so they are computed once per (dim,
+        // max_level) and shared by every mesh version.
         void construct_corners(MRMeshData& m)
         {
+            static std::mutex mu;
+            static std::map<std::pair<int, int>, std::pair<std::vector<LevelSet>, std::vector<Dir>>> cache;
+            const std::pair<int, int> key{m.cfg.dim, m.cfg.max_level};
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                auto it = cache.find(key);
+                if (it != cache.end())
+                {
+                    m.corners     = it->second.first;
+                    m.corner_dirs = it->second.second;
+                    return;
+                }
+            }
             m.corners.clear();
             m.corner_dirs.clear();
             const LevelSet& dom = m.domain[size_t(m.cfg.max_level)];
@@ -257,6 +275,8 @@ namespace smr
                 m.corners.push_back(set_difference(dom, u));
                 m.corner_dirs.push_back(d);
             }
+            std::lock_guard<std::mutex> lk(mu);
+            cache.emplace(key, std::make_pair(m.corners, m.corner_dirs));
         }
 
         // Mesh_base::renumbering / LevelCellArray::update_index
