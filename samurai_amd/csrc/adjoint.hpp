@@ -95,7 +95,9 @@ namespace smr
         Ctx* ctx_;
         std::vector<Instance> inst_;
         int steps_, every_, nseg_;
-        Tuning tuning_;
+        // the programs are built once per velocity and replayed: graph replays of
+        // the stage launches by default
+        Tuning tuning_{256, 1, MODE_GRAPH};
         bool programs_ready_ = false;
         const double* bound_u0_ = nullptr;
         double* bound_uT_       = nullptr;

@@ -129,6 +129,21 @@ namespace smr
             return c == std::string::npos ? a : a.substr(0, c);
         }
 
+        int cu_count()
+        {
+            static int n = -1;
+            if (n < 0)
+            {
+                int dev = 0;
+                (void)hipGetDevice(&dev);
+                if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+                {
+                    n = 1;
+                }
+            }
+            return n;
+        }
+
         int max_blocks_cached(int block)
         {
             static std::map<int, int> cache;
@@ -380,7 +395,9 @@ namespace smr
         hipStream_t s = ctx_->stream;
         if (t.mode == MODE_FUSED)
         {
-            const int maxb        = max_blocks_cached(t.block);
+            // the grid barrier costs grow with the number of blocks that arrive
+            // at it: at most one block per CU, the rest is grid-strided
+            const int maxb        = std::min(max_blocks_cached(t.block), cu_count());
             const int64_t per     = int64_t(t.block) * t.cpt;
             const int64_t needed  = (max_stage_cells() + per - 1) / per;
             const int nblocks     = int(std::max<int64_t>(1, std::min<int64_t>(needed, maxb)));

@@ -292,7 +292,7 @@ namespace smr
         std::vector<int32_t> next, walks;
         if (w.walk)
         {
-            constexpr int kWalkRows = 32;
+            constexpr int kWalkRows = 64;
             next.assign(size_t(nruns), -1);
             std::vector<uint8_t> below(size_t(nruns), 0);
             // runs are sorted by level, then (z, y), then x: the run above r with
@@ -334,19 +334,22 @@ namespace smr
                 {
                     continue;
                 }
+                // a band of up to kWalkRows runs, then every 64-column strip of it:
+                // neighbouring strips (which share the x-1 / x+1 cache lines) are
+                // consecutive walks, run by neighbouring waves of one XCD
                 const int32_t x0 = w.iv[size_t(5 * s0)], x1 = w.iv[size_t(5 * s0 + 1)];
-                for (int32_t xs = x0; xs < x1; xs += 64)
+                int64_t k = s0;
+                while (k >= 0)
                 {
-                    int64_t k = s0;
-                    while (k >= 0)
+                    const int64_t first = k;
+                    int32_t n           = 0;
+                    while (k >= 0 && n < kWalkRows)
                     {
-                        const int64_t first = k;
-                        int32_t n           = 0;
-                        while (k >= 0 && n < kWalkRows)
-                        {
-                            ++n;
-                            k = next[size_t(k)];
-                        }
+                        ++n;
+                        k = next[size_t(k)];
+                    }
+                    for (int32_t xs = x0; xs < x1; xs += 64)
+                    {
                         walks.insert(walks.end(), {int32_t(first), xs, n});
                     }
                 }
