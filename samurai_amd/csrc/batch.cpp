@@ -170,6 +170,7 @@ namespace smr
                 case dev::SEG_TAG_INIT:
                 case dev::SEG_MAXIMUM: ok = ok && g.list.iv && g.c1 <= g.list.ncells && g.tag; break;
                 case dev::SEG_TAG_CHANGE: ok = ok && g.list.iv && g.c1 <= g.list.ncells && g.tag && g.err; break;
+                case dev::SEG_WALK_ROWS: ok = ok && g.list.walks && g.list.walk_rows && g.c1 <= g.list.nwalks; break;
                 case dev::SEG_CRITERIA:
                     ok = ok && g.list.iv && g.c1 <= g.list.ncells && g.tag && g.out && (g.thr || (g.aux & AUX_THR));
                     break;
@@ -800,6 +801,13 @@ namespace smr
                             add(l, hb);
                         }
                         chains[j].push_back(std::move(st));
+                        // then the row bases of the cells list's column walks
+                        DevWorkList& cl = ndev[j]->cells;
+                        if (cl.d.nwalks > 0)
+                        {
+                            dev::Segment g = list_seg(dev::SEG_WALK_ROWS, c, cl, 0, cl.d.nwalks);
+                            chains[j].push_back(Stage{g});
+                        }
                     }
                     resolve_prog_.append(chains);
                     resolve_prog_.upload();

@@ -204,6 +204,29 @@ namespace smr::dev
         return uc - dt * (S * double(1 << level));
     }
 
+    // storage bases of the rows of column walk w: the row below its first run,
+    // the rows of its n runs, the row above its last run
+    __device__ __forceinline__ void op_walk_rows(const List& l, int64_t w)
+    {
+        const int32_t* wk = l.walks + 4 * w;
+        int32_t r         = wk[0];
+        const int32_t n   = wk[2];
+        int64_t* rows     = l.walk_rows + wk[3];
+        rows[0]           = l.base[int64_t(r) * l.ntaps + 1];
+        for (int32_t k = 0; k < n; ++k)
+        {
+            rows[k + 1] = l.base[int64_t(r) * l.ntaps];
+            if (k + 1 == n)
+            {
+                rows[n + 1] = l.base[int64_t(r) * l.ntaps + 2];
+            }
+            else
+            {
+                r = l.next[r];
+            }
+        }
+    }
+
     // upwind_value<2> on values already loaded: left, right, down, up neighbours
     // of uc (same arithmetic, same order)
     __device__ __forceinline__ double upwind_vals_2d(double uc, double ul, double ur, double ud, double uu, int level, double ha0,

@@ -30,10 +30,14 @@ namespace smr::dev
         int64_t* base;       // nruns * ntaps
         const int8_t* taps;  // 7 per tap: table, dl, dx, dy, dz, wlo, whi
         const int32_t* tiles; // (run, x0) of every 64-cell tile of a run, runs in order
-        // column walks (2D cells list): (first run, x0, number of runs) per walk;
-        // next[r] is the run right above run r with the same x range, or -1
+        // column walks (2D cells list): (first run, x0, number of runs n, first
+        // row slot) per walk; next[r] is the run right above run r with the same
+        // x range, or -1; walk_rows holds, per walk, the storage bases of the n + 2
+        // rows it touches (the row below, the n rows, the row above), filled on
+        // the device from the resolved bases
         const int32_t* walks;
         const int32_t* next;
+        int64_t* walk_rows;
         int64_t nruns;
         int64_t ncells;
         int64_t ntiles;
@@ -92,7 +96,9 @@ namespace smr::dev
         // cells of the list whose neighbour in direction `pad` is a ghost
         SEG_ADJ_SCATTER = 20,
         // err[0] = 1 when a cell's tag changes the cell array (cells list)
-        SEG_TAG_CHANGE = 21
+        SEG_TAG_CHANGE = 21,
+        // the row bases of column walk c of the list (after its resolve)
+        SEG_WALK_ROWS = 22
     };
 
     struct Segment
@@ -140,6 +146,8 @@ namespace smr::dev
     void run_program_stagewise(const Program& p, const std::vector<int64_t>& stage_cells, int block, int cpt, hipStream_t s);
 
     void resolve(const List& l, const Table& t0, const Table& t1, int* err, hipStream_t s);
+    // fills l.walk_rows from the resolved bases (after resolve; no-op without walks)
+    void walk_rows(const List& l, hipStream_t s);
     // out[row[r]] = ordered sum of the terms of row r (flux-based convection)
     void conv_gather(const int64_t* row, const int64_t* ptr, const int64_t* src, const double* coef, int64_t nrows, const double* u,
                      double* out, hipStream_t s);

@@ -136,11 +136,22 @@ namespace smr::dev
             out[row[r]] = acc;
         }
 
+        __device__ __forceinline__ int64_t readlane64(int64_t v, int lane)
+        {
+            const uint64_t w  = uint64_t(v);
+            const uint32_t lo = uint32_t(__builtin_amdgcn_readlane(int32_t(uint32_t(w)), lane));
+            const uint32_t hi = uint32_t(__builtin_amdgcn_readlane(int32_t(uint32_t(w >> 32)), lane));
+            return int64_t((uint64_t(hi) << 32) | lo);
+        }
+
         // Upwind (2D) by column walks: a wavefront owns 64 columns of a stack of
-        // runs with one x range at rows y, y+1, ...; walking up, the row above
-        // becomes the centre row and the centre row the row below, so each row
-        // is fetched once per walk (plus the x-1 / x+1 neighbours of the centre
-        // row).  Same arithmetic as k_upwind.
+        // up to 62 runs with one x range at rows y, y+1, ...  Lane k holds the
+        // storage base of row k of the walk (the row below, the runs' rows, the
+        // row above), read by every lane with readlane, so the walk carries no
+        // dependent loads; walking up, the row above becomes the centre row and
+        // the centre row the row below, so each row is fetched once per walk
+        // (plus the x-1 / x+1 neighbours of the centre row).  Same arithmetic as
+        // k_upwind.
         template <bool NT>
         __global__ __launch_bounds__(BLOCK) void k_upwind_walk(List l, const double* __restrict__ u, double* __restrict__ unp1, double a0,
                                                                double a1, double dt, unsigned nlogical)
@@ -155,42 +166,57 @@ namespace smr::dev
             {
                 return;
             }
-            int32_t r         = __builtin_amdgcn_readfirstlane(l.walks[3 * t]);
-            const int32_t xs  = __builtin_amdgcn_readfirstlane(l.walks[3 * t + 1]);
-            const int32_t n   = __builtin_amdgcn_readfirstlane(l.walks[3 * t + 2]);
-            const int32_t x   = xs + int32_t(threadIdx.x & 63);
-            const int32_t* iv = l.iv + 5 * int64_t(r);
-            if (x >= iv[1])
-            {
-                return; // every run of the walk has the same x range
-            }
-            const int level  = iv[4];
+            const int32_t* wk = l.walks + 4 * t;
+            const int32_t r0  = __builtin_amdgcn_readfirstlane(wk[0]);
+            const int32_t xs  = __builtin_amdgcn_readfirstlane(wk[1]);
+            const int32_t n   = __builtin_amdgcn_readfirstlane(wk[2]);
+            const int32_t ro  = __builtin_amdgcn_readfirstlane(wk[3]);
+            const int lane    = int(threadIdx.x & 63);
+            const int64_t rb  = lane < n + 2 ? l.walk_rows[ro + lane] : 0;
+            const int32_t* iv = l.iv + 5 * int64_t(r0);
+            const int32_t x   = xs + lane;
+            const bool active = x < iv[1]; // every run of the walk has the same x range
+            const int level   = iv[4];
             const double ha0 = .5 * a0, hb0 = .5 * fabs(a0), ha1 = .5 * a1, hb1 = .5 * fabs(a1);
-            const int64_t* b = l.base + int64_t(r) * l.ntaps;
-            double ud        = u[b[1] + x];
-            double uc        = u[b[0] + x];
+            const int64_t bd0 = readlane64(rb, 0), bc0 = readlane64(rb, 1);
+            double ud = 0, uc = 0;
+            if (active)
+            {
+                ud = u[bd0 + x];
+                uc = u[bc0 + x];
+            }
+
             for (int32_t k = 0; k < n; ++k)
             {
-                const double uu = u[b[2] + x];
-                const double ul = u[b[0] + x - 1];
-                const double ur = u[b[0] + x + 1];
-                const double v  = upwind_vals_2d(uc, ul, ur, ud, uu, level, ha0, hb0, ha1, hb1, dt);
-                if constexpr (NT)
+                const int64_t bc = readlane64(rb, k + 1);
+                const int64_t bu = readlane64(rb, k + 2);
+                double uu        = 0;
+                if (active)
                 {
-                    __builtin_nontemporal_store(v, unp1 + b[0] + x);
+                    uu              = u[bu + x];
+                    const double ul = u[bc + x - 1];
+                    const double ur = u[bc + x + 1];
+                    const double v  = upwind_vals_2d(uc, ul, ur, ud, uu, level, ha0, hb0, ha1, hb1, dt);
+                    if constexpr (NT)
+                    {
+                        __builtin_nontemporal_store(v, unp1 + bc + x);
+                    }
+                    else
+                    {
+                        unp1[bc + x] = v;
+                    }
                 }
-                else
-                {
-                    unp1[b[0] + x] = v;
-                }
-                if (k + 1 < n)
-                {
-                    // the run above: its own row is this run's upper row, its lower row this run's row
-                    r  = __builtin_amdgcn_readfirstlane(l.next[r]);
-                    b  = l.base + int64_t(r) * l.ntaps;
-                    ud = uc;
-                    uc = uu;
-                }
+                ud = uc;
+                uc = uu;
+            }
+        }
+
+        __global__ void k_walk_rows(List l)
+        {
+            const int64_t w = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+            if (w < l.nwalks)
+            {
+                op_walk_rows(l, w);
             }
         }
 
@@ -299,6 +325,14 @@ namespace smr::dev
         if (nrows > 0)
         {
             hipLaunchKernelGGL(k_conv_gather, dim3(nblocks(nrows)), dim3(BLOCK), 0, s, row, ptr, src, coef, nrows, u, out);
+        }
+    }
+
+    void walk_rows(const List& l, hipStream_t s)
+    {
+        if (l.nwalks > 0)
+        {
+            hipLaunchKernelGGL(k_walk_rows, dim3(nblocks(l.nwalks)), dim3(BLOCK), 0, s, l);
         }
     }
 

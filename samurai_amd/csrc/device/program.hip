@@ -98,6 +98,10 @@ namespace smr::dev
             {
                 op_adj_gather(g.rdst, g.rsp, g.rsrc, g.thr, c, g.out);
             }
+            else if (g.op == SEG_WALK_ROWS)
+            {
+                op_walk_rows(g.list, c);
+            }
             else if (g.op == SEG_ADJ_ZERO)
             {
                 g.out[g.rdst[c]] = 0.0;

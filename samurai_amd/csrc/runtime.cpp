@@ -290,9 +290,10 @@ namespace smr
         // column walks: chains of runs stacked in y with equal x ranges, cut in
         // 64-column strips of at most kWalkRows runs
         std::vector<int32_t> next, walks;
+        int64_t nrows_total = 0;
         if (w.walk)
         {
-            constexpr int kWalkRows = 64;
+            constexpr int kWalkRows = 62; // n + 2 row bases fit one wavefront
             next.assign(size_t(nruns), -1);
             std::vector<uint8_t> below(size_t(nruns), 0);
             // runs are sorted by level, then (z, y), then x: the run above r with
@@ -350,7 +351,8 @@ namespace smr
                     }
                     for (int32_t xs = x0; xs < x1; xs += 64)
                     {
-                        walks.insert(walks.end(), {int32_t(first), xs, n});
+                        walks.insert(walks.end(), {int32_t(first), xs, n, int32_t(nrows_total)});
+                        nrows_total += n + 2;
                     }
                 }
             }
@@ -362,9 +364,10 @@ namespace smr
         const size_t otl  = b_.add(tiles.data(), tiles.size() * 4);
         const size_t onx  = b_.add(next.data(), next.size() * 4);
         const size_t owk  = b_.add(walks.data(), walks.size() * 4);
+        const size_t owr  = b_.reserve(size_t(nrows_total) * 8);
         const size_t obas = b_.reserve(size_t(nruns) * w.taps.size() * 8);
         const int64_t ntiles = int64_t(tiles.size() / 2);
-        const int64_t nwalks = int64_t(walks.size() / 3);
+        const int64_t nwalks = int64_t(walks.size() / 4);
         const int ntaps      = int(w.taps.size());
         Blob* b              = &b_;
         binds_.push_back(
@@ -377,6 +380,7 @@ namespace smr
                 l.d.tiles  = reinterpret_cast<const int32_t*>(b->dptr(otl));
                 l.d.next   = nwalks > 0 ? reinterpret_cast<const int32_t*>(b->dptr(onx)) : nullptr;
                 l.d.walks  = nwalks > 0 ? reinterpret_cast<const int32_t*>(b->dptr(owk)) : nullptr;
+                l.d.walk_rows = nwalks > 0 ? reinterpret_cast<int64_t*>(b->dptr(owr)) : nullptr;
                 l.d.nwalks = nwalks;
                 l.d.base   = reinterpret_cast<int64_t*>(b->dptr(obas));
                 l.d.ntiles = ntiles;
@@ -423,6 +427,7 @@ namespace smr
         p.list(l);
         p.finish(ctx.stream);
         dev::resolve(l.d, t0, t1, ctx.d_err, ctx.stream);
+        dev::walk_rows(l.d, ctx.stream);
     }
 
     std::unique_ptr<DevMesh> make_dev_mesh(Ctx& ctx, const MRMeshData& m)
@@ -436,6 +441,7 @@ namespace smr
         p.list(dm->cells);
         p.finish(ctx.stream);
         dev::resolve(dm->cells.d, dm->table, dm->table, ctx.d_err, ctx.stream);
+        dev::walk_rows(dm->cells.d, ctx.stream);
         ctx.check_resolve();
         return dm;
     }
@@ -487,6 +493,7 @@ namespace smr
         {
             dev::resolve(l->d, dm->table, dm->table, ctx.d_err, ctx.stream);
         }
+        dev::walk_rows(dm->cells.d, ctx.stream);
         if (check)
         {
             ctx.check_resolve();
