@@ -32,6 +32,13 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per launch of the uniform level-12 upwind, from two rocprofv3 --pmc
+# runs of this program (FETCH_SIZE, WRITE_SIZE), each scaled by the counter's
+# reading for the copy kernel of known size measured in the same run
+# (profiles/upwind_walk_uniform_L12.md).  None when not measured for the
+# current kernel.
+MEASURED_UPWIND_TRAFFIC_BYTES = None
+
 
 def progress(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
@@ -555,7 +562,7 @@ def main():
                    "seed": args.seed, "min_level": 4, "max_level": 12,
                    "mean_leaves": updates / args.steps, "final_leaves": final_cells},
         "roofline": {"bound": "hbm", "achieved": uniform_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": uniform_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "frac": uniform_gbs / HBM_PEAK_GBS, "traffic": MEASURED_UPWIND_TRAFFIC_BYTES,
                      "kernel": "upwind (wave-tile kernel k_upwind_tiles<2>) on the uniform level-12 mesh (16,777,216 leaves)",
                      "bytes_per_launch": ubytes, "avg_launch_ms": u_ms,
                      "cell_updates_per_s": uniform_cups, "measured_copy_GBs": copy_gbs,
