@@ -929,8 +929,16 @@ namespace smr
             nm.graduated = true;
         }
         {
+            // the complete image of the new mesh (cells, ghost and adapt
+            // programs) in one allocation; its resolve is checked together with
+            // the update_fields lists
+            HostImage img;
+            {
+                ScopedTimer t4(nullptr, "program build");
+                img = build_host_image(nm);
+            }
             ScopedTimer timer(&ctx, "device upload");
-            ndev = make_dev_mesh(ctx, nm);
+            ndev = upload_image(ctx, nm, std::move(img), false);
         }
         update_fields(f, nm, *ndev);
         m.replace(std::move(nm), std::move(ndev));
