@@ -424,16 +424,12 @@ namespace smr
         {
             ++nit;
             CellArray remove = empty_cell_array(dim);
-            // list_interval_to_refine_for_graduation (graduation.hpp:245-330): the
-            // cells of level L, box-dilated by 2*grad_width and projected to
-            // level L-2, then coarsened level by level, flag the overlapping
-            // coarse cells.  Coarsening distributes over unions, so the cascades
-            // of all fine levels are merged into one pass from the top: at level
-            // c, P_c = on(P_{c+1}, c) | E_{c+2}, with E_L the dilated projection of
-            // level L (computed for all levels in parallel).
+            // list_interval_to_refine_for_graduation (graduation.hpp:245-330)
             const int fmax = ca_max_level(ca) + 1;
             const int cmin = ca_min_level(ca);
-            std::vector<LevelSet> E(MAXL);
+            // the cascade of each fine level is independent: run them in
+            // parallel, then merge the coarse cells they flag level by level
+            std::vector<CellArray> part(MAXL);
             parallel_for(MAXL,
                          [&](int fine_level)
                          {
@@ -441,25 +437,34 @@ namespace smr
                              {
                                  return;
                              }
-                             E[size_t(fine_level)] = expand_on(ca[size_t(fine_level)], 2 * grad_width, fine_level - 2);
+                             CellArray& out = part[size_t(fine_level)];
+                             out            = empty_cell_array(dim);
+                             LevelSet proj  = expand_on(ca[size_t(fine_level)], 2 * grad_width, fine_level - 2);
+                             for (int coarse_level = fine_level - 2;; --coarse_level)
+                             {
+                                 if (!proj.empty())
+                                 {
+                                     out[size_t(coarse_level)] = set_intersection(proj, ca[size_t(coarse_level)]);
+                                 }
+                                 if (coarse_level == cmin || proj.empty())
+                                 {
+                                     break;
+                                 }
+                                 proj = on(proj, coarse_level - 1);
+                             }
                          });
-            LevelSet P(dim, 0);
-            bool have = false;
-            for (int c = std::min(fmax, MAXL - 1) - 2; c >= cmin; --c)
+            for (int fine_level = MAXL - 1; fine_level >= 0; --fine_level)
             {
-                if (have)
+                if (part[size_t(fine_level)].empty())
                 {
-                    P = on(P, c);
+                    continue;
                 }
-                const LevelSet& e = E[size_t(c + 2)];
-                if (!e.empty())
+                for (int l = 0; l < MAXL; ++l)
                 {
-                    P    = have ? set_union(P, e) : e;
-                    have = true;
-                }
-                if (have && !P.empty())
-                {
-                    remove[size_t(c)] = set_intersection(P, ca[size_t(c)]);
+                    if (!part[size_t(fine_level)][size_t(l)].empty())
+                    {
+                        remove[size_t(l)] = set_union(remove[size_t(l)], part[size_t(fine_level)][size_t(l)]);
+                    }
                 }
             }
             bool any_change   = false;
