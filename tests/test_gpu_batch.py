@@ -119,3 +119,40 @@ def test_batch_rejects_shared_mesh(gpu_ctx):
     w = sa.make_scalar_field("w", x["mesh"])
     with pytest.raises(sa.SamuraiError):
         sa.Batch(gpu_ctx, [(x["u"], x["unp1"], x["a"], x["dt"]), (w, x["unp1"], x["a"], x["dt"])])
+
+
+def test_batch_3d_and_mixed_dimensions(gpu_ctx):
+    """3D instances (and a 2D one beside them) in one batch."""
+    import samurai_amd as sa
+    from initial_conditions import sphere_3d
+
+    def make(dim, lmax, seed):
+        mesh = sa.make_mesh(gpu_ctx, sa.mesh_config(dim, 2, lmax, 1))
+        u = sa.make_scalar_field("u", mesh)
+        cells = mesh.cells()
+        if dim == 3:
+            u.set_cells(sphere_3d(cells, center=(0.45 + 0.02 * seed, 0.5, 0.55), radius=0.2))
+        else:
+            u.set_cells(batch_field(cells, seed, 1, 1))
+        sa.make_bc_dirichlet(u, 0.0)
+        a = np.array([1.0, -0.5, 0.25][:dim])
+        return {"mesh": mesh, "u": u, "unp1": sa.make_scalar_field("unp1", mesh), "a": a,
+                "dt": 0.4 / 3 * 2.0 ** -lmax, "adapt": sa.make_MRAdapt(u)}
+
+    specs = [(3, 4, 1), (3, 5, 2), (2, 6, 3)]
+    plain = [make(*sp) for sp in specs]
+    bat = [make(*sp) for sp in specs]
+    for x in plain + bat:
+        x["adapt"](1e-3, REG)
+    batch = sa.Batch(gpu_ctx, [(x["u"], x["unp1"], x["a"], x["dt"]) for x in bat])
+    batch.set_tuning(256, 2, 1)
+    for step in range(3):
+        for x in plain:
+            x["adapt"](1e-3, REG)
+            sa.update_ghost_mr(x["u"])
+            x["unp1"].resize()
+            sa.upwind(x["a"], x["dt"], x["u"], x["unp1"])
+            sa.swap(x["u"], x["unp1"])
+        batch.step(1e-3, REG)
+        gpu_ctx.synchronize()
+        assert_same(plain, bat, f"3d step {step}")
