@@ -298,99 +298,171 @@ namespace smr
         }
     } // namespace
 
+    namespace
+    {
+        // the per-level set computations of the ghost and adapt programs
+        struct ProgramSets
+        {
+            std::vector<LevelSet> proj, pred, det, crit;
+        };
+
+        int ghost_tasks(const MRMeshData& m)
+        {
+            return 3 * (m.cfg.max_level + 1);
+        }
+        int adapt_tasks(const MRMeshData& m)
+        {
+            return 2 * (m.cfg.max_level + 1);
+        }
+
+        // task t of the ghost program: outer recipes, projection set or
+        // prediction set of one level (finest levels first)
+        void ghost_task(const MRMeshData& m, const std::vector<Dir>& cart, int task, GhostProgram& p, ProgramSets& sets)
+        {
+            const int maxl = m.cfg.max_level;
+            const int kind = task % 3, level = maxl - task / 3;
+            if (kind == 0)
+            {
+                p.outer[size_t(level)] = outer_recipes(m, level, cart);
+            }
+            else if (kind == 1 && level >= 1)
+            {
+                sets.proj[size_t(level)] =
+                    set_intersection(on(m.ids[REFERENCE][size_t(level)], level - 1), m.ids[PROJ_CELLS][size_t(level - 1)]);
+            }
+            else if (kind == 2 && level >= 1)
+            {
+                const LevelSet& ref  = m.ids[REFERENCE][size_t(level)];
+                LevelSet pred_ghosts = set_difference(ref, set_union(m.ids[CELLS][size_t(level)], m.ids[PROJ_CELLS][size_t(level)]));
+                sets.pred[size_t(level)] =
+                    set_intersection(set_intersection(pred_ghosts, m.domain[size_t(level)]), m.ids[REFERENCE][size_t(level - 1)]);
+            }
+        }
+
+        void adapt_task(const MRMeshData& m, int task, ProgramSets& sets)
+        {
+            const int minl = m.cfg.min_level, maxl = m.cfg.max_level;
+            const int l0    = minl > 0 ? minl - 1 : 0;
+            const int level = maxl - task / 2;
+            if (task % 2 == 0 && level >= l0 && level < maxl)
+            {
+                LevelSet fine           = set_union(m.ids[CELLS][size_t(level + 1)], m.ids[CELLS][size_t(level + 2)]);
+                sets.det[size_t(level)] = set_intersection(m.ids[REFERENCE][size_t(level)], on(fine, level));
+            }
+            else if (task % 2 == 1 && level >= 1)
+            {
+                sets.crit[size_t(level)] = set_intersection(on(m.ids[CELLS][size_t(level)], level - 1), m.ids[REFERENCE][size_t(level - 1)]);
+            }
+        }
+
+        void assemble_ghost(const MRMeshData& m, GhostProgram& p, ProgramSets& sets)
+        {
+            const int dim = m.cfg.dim, s = m.cfg.pred_s, maxl = m.cfg.max_level;
+            // projection (numeric/projection.hpp:15-65): runs at the coarse level
+            p.projection.taps.push_back({0, 0, 0, 0, 0, 0, 0});
+            add_row_taps(p.projection, 0, 1, 0, 0, 1, dim, 0, 1);
+            // prediction <s,false> (numeric/prediction.hpp:259-357)
+            p.prediction.taps.push_back({0, 0, 0, 0, 0, 0, 0});
+            add_row_taps(p.prediction, 0, -1, int8_t(-s), -s, s, dim, 0, int8_t(2 * s));
+            for (int level = 1; level <= maxl; ++level)
+            {
+                p.projection.add(sets.proj[size_t(level)]);
+                p.prediction.add(sets.pred[size_t(level)]);
+            }
+            p.projection.finish();
+            p.prediction.finish();
+        }
+
+        void assemble_adapt(const MRMeshData& m, AdaptProgram& p, ProgramSets& sets)
+        {
+            const int dim = m.cfg.dim, s = m.cfg.pred_s;
+            const int minl = m.cfg.min_level, maxl = m.cfg.max_level;
+            const int l0   = minl > 0 ? minl - 1 : 0;
+            // compute_detail (mr/operators.hpp:139-540): fine rows, then coarse stencil rows
+            add_row_taps(p.detail, 0, 1, 0, 0, 1, dim, 0, 1);
+            add_row_taps(p.detail, 0, 0, int8_t(-s), -s, s, dim, 0, int8_t(2 * s));
+            for (int level = l0; level < maxl; ++level)
+            {
+                p.detail.add(sets.det[size_t(level)]);
+            }
+            p.detail.finish();
+            // mr_criteria / maximum (mr/criteria.hpp:13-122, mr/operators.hpp:22-96): the
+            // criteria sets are those of the fine levels [min, max], the maximum sets
+            // those of [1, max]; both are (cells[l] n reference[l-1]).on(l-1)
+            p.criteria.taps.push_back({0, 0, 0, 0, 0, 0, 0});
+            add_row_taps(p.criteria, 0, 1, 0, 0, 1, dim, 0, 1);
+            p.maximum.taps = p.criteria.taps;
+            for (int level = 1; level <= maxl; ++level)
+            {
+                if (level >= minl)
+                {
+                    p.criteria.add(sets.crit[size_t(level)]);
+                }
+                p.maximum.add(sets.crit[size_t(level)]);
+            }
+            p.criteria.finish();
+            p.maximum.finish();
+        }
+
+        ProgramSets make_sets(const MRMeshData& m)
+        {
+            ProgramSets sets;
+            const size_t n = size_t(m.cfg.max_level) + 1;
+            sets.proj.resize(n);
+            sets.pred.resize(n);
+            sets.det.resize(n);
+            sets.crit.resize(n);
+            return sets;
+        }
+    } // namespace
+
     GhostProgram build_ghost_program(const MRMeshData& m)
     {
         GhostProgram p;
-        const int dim = m.cfg.dim, s = m.cfg.pred_s, maxl = m.cfg.max_level;
-        const auto cart = cartesian_directions(dim);
-        p.outer.resize(size_t(maxl) + 1);
-        std::vector<LevelSet> proj(size_t(maxl) + 1), pred(size_t(maxl) + 1);
-        // one task per level for each of: outer recipes, projection set, prediction set
-        parallel_for(3 * (maxl + 1),
-                     [&](int task)
-                     {
-                         const int kind = task % 3, level = task / 3;
-                         if (kind == 0)
-                         {
-                             p.outer[size_t(level)] = outer_recipes(m, level, cart);
-                         }
-                         else if (kind == 1 && level >= 1)
-                         {
-                             proj[size_t(level)] = set_intersection(on(m.ids[REFERENCE][size_t(level)], level - 1),
-                                                                    m.ids[PROJ_CELLS][size_t(level - 1)]);
-                         }
-                         else if (kind == 2 && level >= 1)
-                         {
-                             const LevelSet& ref = m.ids[REFERENCE][size_t(level)];
-                             LevelSet pred_ghosts =
-                                 set_difference(ref, set_union(m.ids[CELLS][size_t(level)], m.ids[PROJ_CELLS][size_t(level)]));
-                             pred[size_t(level)] = set_intersection(set_intersection(pred_ghosts, m.domain[size_t(level)]),
-                                                                    m.ids[REFERENCE][size_t(level - 1)]);
-                         }
-                     });
-        // projection (numeric/projection.hpp:15-65): runs at the coarse level
-        p.projection.taps.push_back({0, 0, 0, 0, 0, 0, 0});
-        add_row_taps(p.projection, 0, 1, 0, 0, 1, dim, 0, 1);
-        // prediction <s,false> (numeric/prediction.hpp:259-357)
-        p.prediction.taps.push_back({0, 0, 0, 0, 0, 0, 0});
-        add_row_taps(p.prediction, 0, -1, int8_t(-s), -s, s, dim, 0, int8_t(2 * s));
-        for (int level = 1; level <= maxl; ++level)
-        {
-            p.projection.add(proj[size_t(level)]);
-            p.prediction.add(pred[size_t(level)]);
-        }
-        p.projection.finish();
-        p.prediction.finish();
+        const auto cart = cartesian_directions(m.cfg.dim);
+        p.outer.resize(size_t(m.cfg.max_level) + 1);
+        ProgramSets sets = make_sets(m);
+        parallel_for(ghost_tasks(m), [&](int task) { ghost_task(m, cart, task, p, sets); });
+        assemble_ghost(m, p, sets);
         return p;
     }
 
     AdaptProgram build_adapt_program(const MRMeshData& m)
     {
         AdaptProgram p;
-        const int dim = m.cfg.dim, s = m.cfg.pred_s;
-        const int minl = m.cfg.min_level, maxl = m.cfg.max_level;
-        const int l0   = minl > 0 ? minl - 1 : 0;
-        std::vector<LevelSet> det(size_t(maxl) + 1), crit(size_t(maxl) + 1);
-        parallel_for(2 * (maxl + 1),
+        ProgramSets sets = make_sets(m);
+        parallel_for(adapt_tasks(m), [&](int task) { adapt_task(m, task, sets); });
+        assemble_adapt(m, p, sets);
+        return p;
+    }
+
+    void build_programs(const MRMeshData& m, WorkList& cells, GhostProgram& ghost, AdaptProgram& adapt)
+    {
+        ghost            = GhostProgram();
+        adapt            = AdaptProgram();
+        const auto cart  = cartesian_directions(m.cfg.dim);
+        ghost.outer.resize(size_t(m.cfg.max_level) + 1);
+        ProgramSets sets = make_sets(m);
+        const int ng = ghost_tasks(m), na = adapt_tasks(m);
+        // one pool of tasks for both programs and the cells list
+        parallel_for(ng + na + 1,
                      [&](int task)
                      {
-                         const int level = task / 2;
-                         if (task % 2 == 0 && level >= l0 && level < maxl)
+                         if (task < ng)
                          {
-                             LevelSet fine      = set_union(m.ids[CELLS][size_t(level + 1)], m.ids[CELLS][size_t(level + 2)]);
-                             det[size_t(level)] = set_intersection(m.ids[REFERENCE][size_t(level)], on(fine, level));
+                             ghost_task(m, cart, task, ghost, sets);
                          }
-                         else if (task % 2 == 1 && level >= 1)
+                         else if (task < ng + na)
                          {
-                             crit[size_t(level)] =
-                                 set_intersection(on(m.ids[CELLS][size_t(level)], level - 1), m.ids[REFERENCE][size_t(level - 1)]);
+                             adapt_task(m, task - ng, sets);
+                         }
+                         else
+                         {
+                             cells = build_cells_list(m);
                          }
                      });
-        // compute_detail (mr/operators.hpp:139-540): fine rows, then coarse stencil rows
-        add_row_taps(p.detail, 0, 1, 0, 0, 1, dim, 0, 1);
-        add_row_taps(p.detail, 0, 0, int8_t(-s), -s, s, dim, 0, int8_t(2 * s));
-        for (int level = l0; level < maxl; ++level)
-        {
-            p.detail.add(det[size_t(level)]);
-        }
-        p.detail.finish();
-        // mr_criteria / maximum (mr/criteria.hpp:13-122, mr/operators.hpp:22-96): the
-        // criteria sets are those of the fine levels [min, max], the maximum sets
-        // those of [1, max]; both are (cells[l] n reference[l-1]).on(l-1)
-        p.criteria.taps.push_back({0, 0, 0, 0, 0, 0, 0});
-        add_row_taps(p.criteria, 0, 1, 0, 0, 1, dim, 0, 1);
-        p.maximum.taps = p.criteria.taps;
-        for (int level = 1; level <= maxl; ++level)
-        {
-            if (level >= minl)
-            {
-                p.criteria.add(crit[size_t(level)]);
-            }
-            p.maximum.add(crit[size_t(level)]);
-        }
-        p.criteria.finish();
-        p.maximum.finish();
-        return p;
+        assemble_ghost(m, ghost, sets);
+        assemble_adapt(m, adapt, sets);
     }
 
     WorkList build_cells_list(const MRMeshData& m)

@@ -449,9 +449,7 @@ namespace smr
     HostImage build_host_image(const MRMeshData& m)
     {
         HostImage h;
-        h.cells = build_cells_list(m);
-        h.ghost = build_ghost_program(m);
-        h.adapt = build_adapt_program(m);
+        build_programs(m, h.cells, h.ghost, h.adapt);
         return h;
     }
 
