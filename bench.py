@@ -231,7 +231,7 @@ def bench_batched(sa, ctx, args, eps, reg):
     leaves = [x["mesh"].nb_cells() for x in bset]
     cpu = None
     if args.cpu_budget > 0:
-        jobs = [(x["mesh"].cells(), x["u"].cells(), 2, x["mesh"].max_level(), x["a"], x["dt"], eps, reg, True,
+        jobs = [(x["mesh"].cells(), x["u"].cells(), 2, x["mesh"].max_level, x["a"], x["dt"], eps, reg, True,
                  args.cpu_budget / 2) for x in bset]
         cpu = cpu_baseline_parallel(jobs, "MRadaptation + update_ghost_mr + upwind on the batched instances' final meshes")
         progress(f"batched cpu baseline: {cpu['value']:.3g} cell-updates/s on {cpu['cores']} cores")

@@ -6,6 +6,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 namespace smr
@@ -293,7 +294,14 @@ namespace smr
         int64_t nrows_total = 0;
         if (w.walk)
         {
-            constexpr int kWalkRows = 62; // n + 2 row bases fit one wavefront
+            // rows per walk: n + 2 row bases fit one wavefront (n <= 62); shorter
+            // walks give more wavefronts to hide latency with
+            static const int kWalkRows = []
+            {
+                const char* e = std::getenv("SMR_WALK_ROWS");
+                const int v   = e ? std::atoi(e) : 16;
+                return std::max(1, std::min(v, 62));
+            }();
             next.assign(size_t(nruns), -1);
             std::vector<uint8_t> below(size_t(nruns), 0);
             // runs are sorted by level, then (z, y), then x: the run above r with
