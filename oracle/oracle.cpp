@@ -11,6 +11,15 @@
 //
 // Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
 // this library.  The product (samurai_amd/) never links or calls it.
+//
+// Pinning: the 2D path is checked against the reference's advection_2d golden
+// fixtures (cells exact, values <= 1e-12); upwind, projection/prediction,
+// graduation and the flux-based convection against closed forms and the
+// reference's own known answers (tests/test_oracle_*.py, tests/test_convection.py).
+// Parity unpinned by reference data: the 3D outer-ghost corner/edge logic
+// (project_corner_below, corner extrapolation in 3D) -- no 3D fixture exists;
+// it is checked only by invariants and by agreement with the product's
+// independent interval-set implementation.
 #include "dense_set.hpp"
 
 #include <cmath>
