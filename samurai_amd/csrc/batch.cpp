@@ -9,6 +9,7 @@
 #include <cstring>
 #include <fstream>
 #include <map>
+#include <optional>
 #include <sstream>
 #include <stdexcept>
 
@@ -700,7 +701,6 @@ namespace smr
                 // whose tags change nothing are at the fixed point already)
                 hip_check(hipMemcpyAsync(host_flags_, flags_, na * sizeof(int), hipMemcpyDeviceToHost, s), "hipMemcpyAsync(flags)");
                 ctx_->sync();
-                pending_.clear();
                 std::vector<size_t> work;
                 for (size_t k = 0; k < na; ++k)
                 {
@@ -748,27 +748,43 @@ namespace smr
                 const size_t nc = changed.size();
                 std::vector<std::unique_ptr<DevMesh>> ndev(nc);
                 std::vector<std::array<DevWorkList, 3>> ufl(nc);
-                auto ufblob = std::make_unique<Blob>();
                 {
                     ScopedTimer t(ctx_, "device upload");
-                    Packer up(*ufblob);
+                    // each new image (with the update_fields lists, which share its
+                    // allocation) is packed on the host in parallel, staged in
+                    // pinned memory, then allocated and copied in stream order
+                    std::vector<std::unique_ptr<Packer>> packers(nc);
+                    std::optional<ScopedTimer> tp(std::in_place, ctx_, "image packing");
+                    parallel_for(int(nc),
+                                 [&](int q)
+                                 {
+                                     const size_t j      = size_t(q);
+                                     Rebuild& r          = rb[changed[j]];
+                                     ndev[j]             = std::make_unique<DevMesh>();
+                                     ndev[j]->table_blob = std::make_unique<Blob>();
+                                     packers[j]          = std::make_unique<Packer>(*ndev[j]->table_blob);
+                                     pack_image(*packers[j], *ndev[j], r.nm, std::move(r.img));
+                                     ufl[j][0].host = std::move(r.copy);
+                                     ufl[j][1].host = std::move(r.coarsen);
+                                     ufl[j][2].host = std::move(r.refine);
+                                     for (DevWorkList& l : ufl[j])
+                                     {
+                                         packers[j]->list(l);
+                                     }
+                                 });
+                    tp.reset();
+                    std::vector<size_t> soff(nc + 1, 0);
                     for (size_t j = 0; j < nc; ++j)
                     {
-                        Rebuild& r          = rb[changed[j]];
-                        ndev[j]             = std::make_unique<DevMesh>();
-                        ndev[j]->table_blob = std::make_unique<Blob>();
-                        Packer p(*ndev[j]->table_blob);
-                        pack_image(p, *ndev[j], r.nm, std::move(r.img));
-                        p.finish(s);
-                        ufl[j][0].host = std::move(r.copy);
-                        ufl[j][1].host = std::move(r.coarsen);
-                        ufl[j][2].host = std::move(r.refine);
-                        for (DevWorkList& l : ufl[j])
-                        {
-                            up.list(l);
-                        }
+                        soff[j + 1] = soff[j] + ((ndev[j]->table_blob->host_bytes() + 255) & ~size_t(255));
                     }
-                    up.finish(s);
+                    char* stage = staging_.acquire(soff[nc]);
+                    parallel_for(int(nc), [&](int q) { ndev[size_t(q)]->table_blob->stage(stage + soff[size_t(q)]); });
+                    for (size_t j = 0; j < nc; ++j)
+                    {
+                        packers[j]->finish(s);
+                    }
+                    staging_.release(s);
                     // resolve every new list; nothing runs on them unless all pairs resolve
                     resolve_prog_.clear();
                     std::vector<Chain> chains(nc);
@@ -864,7 +880,6 @@ namespace smr
                         b.unp1->version = m.version;
                     }
                 }
-                pending_.push_back(std::move(ufblob));
             }
             std::vector<int> next;
             for (size_t k : changed)

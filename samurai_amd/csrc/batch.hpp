@@ -131,8 +131,8 @@ namespace smr
         int* flags_         = nullptr; // per active instance: tags change the cell array
         int* host_flags_    = nullptr;
         int64_t arena_cap_ = 0, host_cap_ = 0;
-        // blobs whose pageable host staging must outlive the copies queued from it
-        std::vector<std::unique_ptr<Blob>> pending_;
+        // pinned staging of the new images' host parts
+        PinnedStaging staging_;
 
         void check_instances();
         void ensure_images(const std::vector<int>& ids);

@@ -144,15 +144,29 @@ namespace smr::dev
             return int64_t((uint64_t(hi) << 32) | lo);
         }
 
+        // lane i <- lane i - 1 (CTRL 0x138, wave_shr:1) or lane i + 1 (0x130,
+        // wave_shl:1) of a double; lanes shifted in from outside the wave get 0
+        template <int CTRL>
+        __device__ __forceinline__ double wave_shift(double v)
+        {
+            const uint64_t w = uint64_t(__double_as_longlong(v));
+            const int lo     = __builtin_amdgcn_update_dpp(0, int(uint32_t(w)), CTRL, 0xF, 0xF, false);
+            const int hi     = __builtin_amdgcn_update_dpp(0, int(uint32_t(w >> 32)), CTRL, 0xF, 0xF, false);
+            return __longlong_as_double(int64_t((uint64_t(uint32_t(hi)) << 32) | uint32_t(lo)));
+        }
+
         // Upwind (2D) by column walks: a wavefront owns 64 columns of a stack of
         // up to 62 runs with one x range at rows y, y+1, ...  Lane k holds the
         // storage base of row k of the walk (the row below, the runs' rows, the
         // row above), read by every lane with readlane, so the walk carries no
         // dependent loads; walking up, the row above becomes the centre row and
-        // the centre row the row below, so each row is fetched once per walk
-        // (plus the x-1 / x+1 neighbours of the centre row).  Same arithmetic as
-        // k_upwind.
-        template <bool NT>
+        // the centre row the row below, so each row is fetched once per walk.
+        // G rows are fetched ahead of their arithmetic (G loads in flight per
+        // wave).  SHIFT: the x - 1 / x + 1 neighbours come from the centre row's
+        // registers of the neighbouring lanes, and only the two strip ends are
+        // loaded (by one lane each); else each lane loads them.  Same arithmetic
+        // as k_upwind.
+        template <bool NT, int G, bool SHIFT>
         __global__ __launch_bounds__(BLOCK) void k_upwind_walk(List l, const double* __restrict__ u, double* __restrict__ unp1, double a0,
                                                                double a1, double dt, unsigned nlogical)
         {
@@ -174,8 +188,10 @@ namespace smr::dev
             const int lane    = int(threadIdx.x & 63);
             const int64_t rb  = lane < n + 2 ? l.walk_rows[ro + lane] : 0;
             const int32_t* iv = l.iv + 5 * int64_t(r0);
+            const int32_t xe  = __builtin_amdgcn_readfirstlane(iv[1]);
             const int32_t x   = xs + lane;
-            const bool active = x < iv[1]; // every run of the walk has the same x range
+            const bool active = x < xe; // every run of the walk has the same x range
+            const int last    = min(63, xe - 1 - xs); // last active lane
             const int level   = iv[4];
             const double ha0 = .5 * a0, hb0 = .5 * fabs(a0), ha1 = .5 * a1, hb1 = .5 * fabs(a1);
             const int64_t bd0 = readlane64(rb, 0), bc0 = readlane64(rb, 1);
@@ -186,28 +202,71 @@ namespace smr::dev
                 uc = u[bc0 + x];
             }
 
-            for (int32_t k = 0; k < n; ++k)
+            for (int32_t k = 0; k < n; k += G)
             {
-                const int64_t bc = readlane64(rb, k + 1);
-                const int64_t bu = readlane64(rb, k + 2);
-                double uu        = 0;
-                if (active)
+                double uu[G], el[G], er[G];
+#pragma unroll
+                for (int g = 0; g < G; ++g)
                 {
-                    uu              = u[bu + x];
-                    const double ul = u[bc + x - 1];
-                    const double ur = u[bc + x + 1];
-                    const double v  = upwind_vals_2d(uc, ul, ur, ud, uu, level, ha0, hb0, ha1, hb1, dt);
-                    if constexpr (NT)
+                    uu[g] = 0;
+                    el[g] = 0;
+                    er[g] = 0;
+                    if (G == 1 || k + g < n)
                     {
-                        __builtin_nontemporal_store(v, unp1 + bc + x);
-                    }
-                    else
-                    {
-                        unp1[bc + x] = v;
+                        const int64_t bu = readlane64(rb, k + g + 2);
+                        const int64_t bc = readlane64(rb, k + g + 1);
+                        if (active)
+                        {
+                            uu[g] = u[bu + x];
+                            if constexpr (SHIFT)
+                            {
+                                if (lane == 0)
+                                {
+                                    el[g] = u[bc + x - 1];
+                                }
+                                if (lane == last)
+                                {
+                                    er[g] = u[bc + x + 1];
+                                }
+                            }
+                            else
+                            {
+                                el[g] = u[bc + x - 1];
+                                er[g] = u[bc + x + 1];
+                            }
+                        }
                     }
                 }
-                ud = uc;
-                uc = uu;
+#pragma unroll
+                for (int g = 0; g < G; ++g)
+                {
+                    if (G == 1 || k + g < n)
+                    {
+                        const int64_t bc = readlane64(rb, k + g + 1);
+                        double ul = el[g], ur = er[g];
+                        if constexpr (SHIFT)
+                        {
+                            const double sl = wave_shift<0x138>(uc);
+                            const double sr = wave_shift<0x130>(uc);
+                            ul              = lane == 0 ? el[g] : sl;
+                            ur              = lane == last ? er[g] : sr;
+                        }
+                        if (active)
+                        {
+                            const double v = upwind_vals_2d(uc, ul, ur, ud, uu[g], level, ha0, hb0, ha1, hb1, dt);
+                            if constexpr (NT)
+                            {
+                                __builtin_nontemporal_store(v, unp1 + bc + x);
+                            }
+                            else
+                            {
+                                unp1[bc + x] = v;
+                            }
+                        }
+                        ud = uc;
+                        uc = uu[g];
+                    }
+                }
             }
         }
 
@@ -283,17 +342,21 @@ namespace smr::dev
         // -2: column walks (2D), -1: column walks (2D) or wave tiles (3D) when at
         // least 3/4 of the lanes hold a cell, else 4 cells per thread), XCD
         // remap, non-temporal stores
+        // (column walks: rows fetched ahead per wave, x +- 1 by lane shifts)
         struct UpwindVariant
         {
             int id, cpt;
             bool remap, nt;
+            int group  = 1;
+            bool shift = false;
         };
-        constexpr UpwindVariant kUpwindVariants[] = {{0, 1, false, false}, {1, 1, true, false}, {2, 2, false, false},
-                                                     {3, 2, true, false},  {4, 4, false, false}, {5, 4, true, true},
-                                                     {6, 1, false, true},  {7, 8, false, false}, {8, 0, true, true},
-                                                     {9, 0, true, false},  {10, -1, true, true}, {11, -2, true, true},
-                                                     {12, -2, true, false}};
-        constexpr int kNumUpwindVariants    = 13;
+        constexpr UpwindVariant kUpwindVariants[] = {
+            {0, 1, false, false},       {1, 1, true, false},        {2, 2, false, false},       {3, 2, true, false},
+            {4, 4, false, false},       {5, 4, true, true},         {6, 1, false, true},        {7, 8, false, false},
+            {8, 0, true, true},         {9, 0, true, false},        {10, -1, true, true},       {11, -2, true, true},
+            {12, -2, true, false},      {13, -2, true, true, 2},    {14, -2, true, true, 4},    {15, -2, true, true, 8},
+            {16, -2, true, true, 1, true}, {17, -2, true, true, 4, true}, {18, -2, true, true, 8, true}};
+        constexpr int kNumUpwindVariants    = 19;
         constexpr int kDefaultUpwindVariant = 10;
         int g_upwind_variant                = -1;
 
@@ -365,14 +428,20 @@ namespace smr::dev
             {
                 const int64_t nblk   = (l.nwalks + BLOCK / 64 - 1) / (BLOCK / 64);
                 const unsigned ngrid = unsigned((nblk + 7) / 8 * 8);
-                if (v.nt)
+#define SMR_WALK_LAUNCH(N, G, S) \
+    hipLaunchKernelGGL((k_upwind_walk<N, G, S>), dim3(ngrid), dim3(BLOCK), 0, s, l, u, unp1, a0, a1, dt, unsigned(nblk))
+                switch (v.id)
                 {
-                    hipLaunchKernelGGL((k_upwind_walk<true>), dim3(ngrid), dim3(BLOCK), 0, s, l, u, unp1, a0, a1, dt, unsigned(nblk));
+                    case 12: SMR_WALK_LAUNCH(false, 1, false); break;
+                    case 13: SMR_WALK_LAUNCH(true, 2, false); break;
+                    case 14: SMR_WALK_LAUNCH(true, 4, false); break;
+                    case 15: SMR_WALK_LAUNCH(true, 8, false); break;
+                    case 16: SMR_WALK_LAUNCH(true, 1, true); break;
+                    case 17: SMR_WALK_LAUNCH(true, 4, true); break;
+                    case 18: SMR_WALK_LAUNCH(true, 8, true); break;
+                    default: SMR_WALK_LAUNCH(true, 1, false); break;
                 }
-                else
-                {
-                    hipLaunchKernelGGL((k_upwind_walk<false>), dim3(ngrid), dim3(BLOCK), 0, s, l, u, unp1, a0, a1, dt, unsigned(nblk));
-                }
+#undef SMR_WALK_LAUNCH
                 return;
             }
             v = kUpwindVariants[5];

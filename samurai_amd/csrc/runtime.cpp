@@ -112,16 +112,18 @@ namespace smr
         dev_free(d_, stream_);
     }
 
-    size_t Blob::reserve(size_t n)
+    size_t Blob::scratch(size_t n)
     {
-        size_t off = (host_.size() + 255) & ~size_t(255);
-        host_.resize(off + std::max<size_t>(n, 8), 0);
+        const size_t off = (scratch_ + 255) & ~size_t(255);
+        scratch_         = off + std::max<size_t>(n, 8);
         return off;
     }
 
     size_t Blob::add(const void* src, size_t n)
     {
-        size_t off = reserve(n);
+        const size_t off = (host_.size() + 255) & ~size_t(255);
+        host_.resize(off + std::max<size_t>(n, 8));
+        host_size_ = host_.size();
         if (n)
         {
             std::memcpy(host_.data() + off, src, n);
@@ -142,15 +144,72 @@ namespace smr
     void Blob::alloc(hipStream_t s)
     {
         stream_ = s;
-        d_      = static_cast<char*>(dev_alloc(host_.size(), s));
+        d_      = static_cast<char*>(dev_alloc(bytes(), s));
     }
 
     void Blob::copy()
     {
-        if (!host_.empty())
+        if (host_size_)
         {
-            hip_check(hipMemcpyAsync(d_, host_.data(), host_.size(), hipMemcpyHostToDevice, stream_), "hipMemcpyAsync(blob)");
+            hip_check(hipMemcpyAsync(d_, staged_ ? staged_ : host_.data(), host_size_, hipMemcpyHostToDevice, stream_),
+                      "hipMemcpyAsync(blob)");
         }
+    }
+
+    void Blob::stage(char* dst)
+    {
+        if (host_size_)
+        {
+            std::memcpy(dst, host_.data(), host_size_);
+        }
+        staged_ = dst;
+        std::vector<char>().swap(host_);
+    }
+
+    PinnedStaging::~PinnedStaging()
+    {
+        if (pending_)
+        {
+            (void)hipEventSynchronize(done_);
+        }
+        if (h_)
+        {
+            (void)hipHostFree(h_);
+        }
+        if (done_)
+        {
+            (void)hipEventDestroy(done_);
+        }
+    }
+
+    char* PinnedStaging::acquire(size_t n)
+    {
+        if (pending_)
+        {
+            hip_check(hipEventSynchronize(done_), "hipEventSynchronize(staging)");
+            pending_ = false;
+        }
+        if (n > cap_)
+        {
+            if (h_)
+            {
+                hip_check(hipHostFree(h_), "hipHostFree(staging)");
+                h_ = nullptr;
+            }
+            cap_ = std::max(n, cap_ + cap_ / 2);
+            hip_check(hipHostMalloc(reinterpret_cast<void**>(&h_), cap_, hipHostMallocDefault), "hipHostMalloc(staging)");
+        }
+        return h_;
+    }
+
+    void PinnedStaging::release(hipStream_t s)
+    {
+        if (!done_)
+        {
+            hip_check(hipEventCreateWithFlags(&done_, hipEventDisableTiming), "hipEventCreate(staging)");
+        }
+        hip_check(hipEventRecord(done_, s), "hipEventRecord(staging)");
+        pending_ = true;
     }
 
     // ------------------------------------------------------------------
@@ -372,8 +431,8 @@ namespace smr
         const size_t otl  = b_.add(tiles.data(), tiles.size() * 4);
         const size_t onx  = b_.add(next.data(), next.size() * 4);
         const size_t owk  = b_.add(walks.data(), walks.size() * 4);
-        const size_t owr  = b_.reserve(size_t(nrows_total) * 8);
-        const size_t obas = b_.reserve(size_t(nruns) * w.taps.size() * 8);
+        const size_t owr  = b_.scratch(size_t(nrows_total) * 8);
+        const size_t obas = b_.scratch(size_t(nruns) * w.taps.size() * 8);
         const int64_t ntiles = int64_t(tiles.size() / 2);
         const int64_t nwalks = int64_t(walks.size() / 4);
         const int ntaps      = int(w.taps.size());
@@ -388,9 +447,9 @@ namespace smr
                 l.d.tiles  = reinterpret_cast<const int32_t*>(b->dptr(otl));
                 l.d.next   = nwalks > 0 ? reinterpret_cast<const int32_t*>(b->dptr(onx)) : nullptr;
                 l.d.walks  = nwalks > 0 ? reinterpret_cast<const int32_t*>(b->dptr(owk)) : nullptr;
-                l.d.walk_rows = nwalks > 0 ? reinterpret_cast<int64_t*>(b->dptr(owr)) : nullptr;
+                l.d.walk_rows = nwalks > 0 ? reinterpret_cast<int64_t*>(b->sptr(owr)) : nullptr;
                 l.d.nwalks = nwalks;
-                l.d.base   = reinterpret_cast<int64_t*>(b->dptr(obas));
+                l.d.base   = reinterpret_cast<int64_t*>(b->sptr(obas));
                 l.d.ntiles = ntiles;
                 l.d.nruns  = nruns;
                 l.d.ncells = ncells;

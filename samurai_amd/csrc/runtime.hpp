@@ -28,23 +28,53 @@ namespace smr
         Blob(const Blob&)            = delete;
         Blob& operator=(const Blob&) = delete;
 
-        // reserves space for n bytes copied from src (8-byte aligned); returns the offset
+        // reserves space for n bytes copied from src (256-byte aligned); returns the offset
         size_t add(const void* src, size_t n);
-        size_t reserve(size_t n);
+        // n bytes of device-only memory (no host copy, not uploaded, contents
+        // undefined until a kernel writes them); returns an offset for sptr()
+        size_t scratch(size_t n);
         // stream-ordered allocation + copy; the host staging copy is kept until
         // the blob dies, so no host synchronization is needed here
         void upload(hipStream_t s);
-        // upload in two steps, to patch device pointers into the host copy between them
+        // upload in two steps, to bind device pointers between them
         void alloc(hipStream_t s);
         void copy();
+        // moves the host part into dst (pinned memory of host_bytes() bytes that
+        // stays untouched until the copy has run); copy() then reads dst
+        void stage(char* dst);
         char* dptr(size_t off) const { return d_ + off; }
-        char* hptr(size_t off) { return host_.data() + off; }
-        size_t bytes() const { return host_.size(); }
+        char* sptr(size_t off) const { return d_ + scratch_base() + off; }
+        size_t host_bytes() const { return host_size_; }
+        size_t bytes() const { return scratch_base() + scratch_; }
 
       private:
+        size_t scratch_base() const { return (host_size_ + 255) & ~size_t(255); }
         std::vector<char> host_;
+        size_t host_size_    = 0;
+        const char* staged_  = nullptr;
+        size_t scratch_      = 0;
         char* d_           = nullptr;
         hipStream_t stream_ = nullptr;
+    };
+
+    // Pinned host staging reused across uploads: acquire() returns at least n
+    // bytes once the copies issued from its previous contents have run;
+    // release() marks the copies just issued on s
+    class PinnedStaging
+    {
+      public:
+        PinnedStaging() = default;
+        ~PinnedStaging();
+        PinnedStaging(const PinnedStaging&)            = delete;
+        PinnedStaging& operator=(const PinnedStaging&) = delete;
+        char* acquire(size_t n);
+        void release(hipStream_t s);
+
+      private:
+        char* h_          = nullptr;
+        size_t cap_       = 0;
+        hipEvent_t done_  = nullptr;
+        bool pending_     = false;
     };
 
     // stream-ordered device memory from the device's default pool, which is
