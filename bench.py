@@ -383,7 +383,7 @@ def bench_adjoint(sa, ctx, args, eps, reg):
         "forward_cell_updates_per_s": ncells * steps / t_fwd,
         "backward_cell_updates_per_s": ncells * steps / t_bwd,
         "fd_u0_rel_err": abs(fd_u - ad_u) / abs(ad_u), "fd_a_rel_err": abs(fd_a - ga[0, 0]) / abs(ga[0, 0]),
-        "fd_h": h, "tuning": "block 256, 1 cell per thread, cooperative fused programs",
+        "fd_h": h, "tuning": "block 256, 1 cell per thread, stage launches replayed as HIP graphs",
         "setup_s": setup, "cpu_baseline_forward": cpu,
     }
 

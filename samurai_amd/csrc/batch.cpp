@@ -697,6 +697,7 @@ namespace smr
             std::vector<Rebuild> rb(na);
             {
                 ScopedTimer t(ctx_, "mesh update");
+                std::optional<ScopedTimer> tt(std::in_place, ctx_, "tags to host");
                 // which instances' tags change their cell array (graduated meshes
                 // whose tags change nothing are at the fixed point already)
                 hip_check(hipMemcpyAsync(host_flags_, flags_, na * sizeof(int), hipMemcpyDeviceToHost, s), "hipMemcpyAsync(flags)");
@@ -715,6 +716,7 @@ namespace smr
                               "hipMemcpyAsync(tags)");
                 }
                 ctx_->sync();
+                tt.reset();
                 parallel_for(int(work.size()),
                              [&](int q)
                              {
@@ -836,13 +838,21 @@ namespace smr
                     uf_prog_.clear();
                     std::vector<Chain> chains(nc);
                     std::vector<double*> nu(nc), nv(nc);
+                    {
+                        ScopedTimer ta(ctx_, "field storage");
+                        for (size_t j = 0; j < nc; ++j)
+                        {
+                            const int64_t n = rb[changed[j]].nm.nref;
+                            nu[j]           = static_cast<double*>(dev_alloc(size_t(std::max<int64_t>(n, 1)) * 8, s));
+                            nv[j]           = static_cast<double*>(dev_alloc(size_t(std::max<int64_t>(n, 1)) * 8, s));
+                        }
+                    }
+                    std::optional<ScopedTimer> tp(std::in_place, ctx_, "update_fields program");
                     for (size_t j = 0; j < nc; ++j)
                     {
                         const BatchInstance& b = inst_[size_t(act[changed[j]])];
                         const MeshConfig& c    = b.u->mesh->host.cfg;
                         const int64_t n        = rb[changed[j]].nm.nref;
-                        nu[j] = static_cast<double*>(dev_alloc(size_t(std::max<int64_t>(n, 1)) * 8, s));
-                        nv[j] = static_cast<double*>(dev_alloc(size_t(std::max<int64_t>(n, 1)) * 8, s));
                         Chain ch(1);
                         dev::Segment z0 = make_seg(dev::SEG_FILL_F64, c);
                         z0.out          = nu[j];
@@ -863,6 +873,8 @@ namespace smr
                     uf_prog_.append(chains);
                     uf_prog_.upload();
                     uf_prog_.run(tuning_);
+                    tp.reset();
+                    ScopedTimer tr(ctx_, "old storage release");
                     for (size_t j = 0; j < nc; ++j)
                     {
                         BatchInstance& b = inst_[size_t(act[changed[j]])];

@@ -161,12 +161,59 @@ namespace smr::dev
         // row above), read by every lane with readlane, so the walk carries no
         // dependent loads; walking up, the row above becomes the centre row and
         // the centre row the row below, so each row is fetched once per walk.
-        // G rows are fetched ahead of their arithmetic (G loads in flight per
-        // wave).  SHIFT: the x - 1 / x + 1 neighbours come from the centre row's
-        // registers of the neighbouring lanes, and only the two strip ends are
-        // loaded (by one lane each); else each lane loads them.  Same arithmetic
-        // as k_upwind.
-        template <bool NT, int G, bool SHIFT>
+        // Rows are fetched G at a time ahead of their arithmetic; PIPE: the next
+        // G rows are fetched before the arithmetic of the current G.  SHIFT: the
+        // x - 1 / x + 1 neighbours come from the centre row's registers of the
+        // neighbouring lanes; the two values beyond the strip are loaded by two
+        // lanes (`ee`): lane 0 the left one, lane min(last + 1, 63) the right one
+        // (last: the last active lane).  Else each lane loads both neighbours.
+        // Same arithmetic as k_upwind.
+        template <int G, bool SHIFT>
+        struct WalkRows
+        {
+            double uu[G], el[G], er[SHIFT ? 1 : G];
+
+            // centre rows k + 1 .. k + G of the walk, rows above k + 2 .. k + G + 1
+            __device__ __forceinline__ void load(const double* __restrict__ u, int64_t rb, int32_t k, int32_t n, int32_t x, int lane,
+                                                 int last, bool active)
+            {
+                const int edge = min(last + 1, 63);
+#pragma unroll
+                for (int g = 0; g < G; ++g)
+                {
+                    uu[g] = 0;
+                    el[g] = 0;
+                    if constexpr (!SHIFT)
+                    {
+                        er[g] = 0;
+                    }
+                    if (G == 1 || k + g < n)
+                    {
+                        const int64_t bu = readlane64(rb, k + g + 2);
+                        const int64_t bc = readlane64(rb, k + g + 1);
+                        if (active)
+                        {
+                            uu[g] = u[bu + x];
+                        }
+                        if constexpr (SHIFT)
+                        {
+                            // x - 1 of lane 0, or x + 1 of the last active lane
+                            if (lane == 0 || lane == edge)
+                            {
+                                el[g] = u[bc + (lane == 0 ? x - 1 : x - lane + last + 1)];
+                            }
+                        }
+                        else if (active)
+                        {
+                            el[g] = u[bc + x - 1];
+                            er[g] = u[bc + x + 1];
+                        }
+                    }
+                }
+            }
+        };
+
+        template <bool NT, int G, bool SHIFT, bool PIPE>
         __global__ __launch_bounds__(BLOCK) void k_upwind_walk(List l, const double* __restrict__ u, double* __restrict__ unp1, double a0,
                                                                double a1, double dt, unsigned nlogical)
         {
@@ -202,40 +249,24 @@ namespace smr::dev
                 uc = u[bc0 + x];
             }
 
+            WalkRows<G, SHIFT> cur;
+            if constexpr (PIPE)
+            {
+                cur.load(u, rb, 0, n, x, lane, last, active);
+            }
             for (int32_t k = 0; k < n; k += G)
             {
-                double uu[G], el[G], er[G];
-#pragma unroll
-                for (int g = 0; g < G; ++g)
+                WalkRows<G, SHIFT> nxt;
+                if constexpr (PIPE)
                 {
-                    uu[g] = 0;
-                    el[g] = 0;
-                    er[g] = 0;
-                    if (G == 1 || k + g < n)
+                    if (k + G < n)
                     {
-                        const int64_t bu = readlane64(rb, k + g + 2);
-                        const int64_t bc = readlane64(rb, k + g + 1);
-                        if (active)
-                        {
-                            uu[g] = u[bu + x];
-                            if constexpr (SHIFT)
-                            {
-                                if (lane == 0)
-                                {
-                                    el[g] = u[bc + x - 1];
-                                }
-                                if (lane == last)
-                                {
-                                    er[g] = u[bc + x + 1];
-                                }
-                            }
-                            else
-                            {
-                                el[g] = u[bc + x - 1];
-                                er[g] = u[bc + x + 1];
-                            }
-                        }
+                        nxt.load(u, rb, k + G, n, x, lane, last, active);
                     }
+                }
+                else
+                {
+                    cur.load(u, rb, k, n, x, lane, last, active);
                 }
 #pragma unroll
                 for (int g = 0; g < G; ++g)
@@ -243,17 +274,24 @@ namespace smr::dev
                     if (G == 1 || k + g < n)
                     {
                         const int64_t bc = readlane64(rb, k + g + 1);
-                        double ul = el[g], ur = er[g];
+                        double ul, ur;
                         if constexpr (SHIFT)
                         {
+                            // lane last + 1 (< 64) holds x + 1 of the last active lane in el
+                            const double ee = cur.el[g];
                             const double sl = wave_shift<0x138>(uc);
-                            const double sr = wave_shift<0x130>(uc);
-                            ul              = lane == 0 ? el[g] : sl;
-                            ur              = lane == last ? er[g] : sr;
+                            const double sr = wave_shift<0x130>(lane == last + 1 ? ee : uc);
+                            ul              = lane == 0 ? ee : sl;
+                            ur              = lane == 63 ? ee : sr;
+                        }
+                        else
+                        {
+                            ul = cur.el[g];
+                            ur = cur.er[g];
                         }
                         if (active)
                         {
-                            const double v = upwind_vals_2d(uc, ul, ur, ud, uu[g], level, ha0, hb0, ha1, hb1, dt);
+                            const double v = upwind_vals_2d(uc, ul, ur, ud, cur.uu[g], level, ha0, hb0, ha1, hb1, dt);
                             if constexpr (NT)
                             {
                                 __builtin_nontemporal_store(v, unp1 + bc + x);
@@ -264,8 +302,12 @@ namespace smr::dev
                             }
                         }
                         ud = uc;
-                        uc = uu[g];
+                        uc = cur.uu[g];
                     }
+                }
+                if constexpr (PIPE)
+                {
+                    cur = nxt;
                 }
             }
         }
@@ -349,14 +391,17 @@ namespace smr::dev
             bool remap, nt;
             int group  = 1;
             bool shift = false;
+            bool pipe  = false;
         };
         constexpr UpwindVariant kUpwindVariants[] = {
             {0, 1, false, false},       {1, 1, true, false},        {2, 2, false, false},       {3, 2, true, false},
             {4, 4, false, false},       {5, 4, true, true},         {6, 1, false, true},        {7, 8, false, false},
             {8, 0, true, true},         {9, 0, true, false},        {10, -1, true, true},       {11, -2, true, true},
             {12, -2, true, false},      {13, -2, true, true, 2},    {14, -2, true, true, 4},    {15, -2, true, true, 8},
-            {16, -2, true, true, 1, true}, {17, -2, true, true, 4, true}, {18, -2, true, true, 8, true}};
-        constexpr int kNumUpwindVariants    = 19;
+            {16, -2, true, true, 1, true}, {17, -2, true, true, 4, true}, {18, -2, true, true, 8, true},
+            {19, -2, true, true, 16, true}, {20, -2, true, false, 8, true}, {21, -2, true, true, 4, true, true},
+            {22, -2, true, true, 8, true, true}, {23, -2, true, true, 2, true, true}};
+        constexpr int kNumUpwindVariants    = 24;
         constexpr int kDefaultUpwindVariant = 10;
         int g_upwind_variant                = -1;
 
@@ -428,18 +473,23 @@ namespace smr::dev
             {
                 const int64_t nblk   = (l.nwalks + BLOCK / 64 - 1) / (BLOCK / 64);
                 const unsigned ngrid = unsigned((nblk + 7) / 8 * 8);
-#define SMR_WALK_LAUNCH(N, G, S) \
-    hipLaunchKernelGGL((k_upwind_walk<N, G, S>), dim3(ngrid), dim3(BLOCK), 0, s, l, u, unp1, a0, a1, dt, unsigned(nblk))
+#define SMR_WALK_LAUNCH(N, G, S, P) \
+    hipLaunchKernelGGL((k_upwind_walk<N, G, S, P>), dim3(ngrid), dim3(BLOCK), 0, s, l, u, unp1, a0, a1, dt, unsigned(nblk))
                 switch (v.id)
                 {
-                    case 12: SMR_WALK_LAUNCH(false, 1, false); break;
-                    case 13: SMR_WALK_LAUNCH(true, 2, false); break;
-                    case 14: SMR_WALK_LAUNCH(true, 4, false); break;
-                    case 15: SMR_WALK_LAUNCH(true, 8, false); break;
-                    case 16: SMR_WALK_LAUNCH(true, 1, true); break;
-                    case 17: SMR_WALK_LAUNCH(true, 4, true); break;
-                    case 18: SMR_WALK_LAUNCH(true, 8, true); break;
-                    default: SMR_WALK_LAUNCH(true, 1, false); break;
+                    case 12: SMR_WALK_LAUNCH(false, 1, false, false); break;
+                    case 13: SMR_WALK_LAUNCH(true, 2, false, false); break;
+                    case 14: SMR_WALK_LAUNCH(true, 4, false, false); break;
+                    case 15: SMR_WALK_LAUNCH(true, 8, false, false); break;
+                    case 16: SMR_WALK_LAUNCH(true, 1, true, false); break;
+                    case 17: SMR_WALK_LAUNCH(true, 4, true, false); break;
+                    case 18: SMR_WALK_LAUNCH(true, 8, true, false); break;
+                    case 19: SMR_WALK_LAUNCH(true, 16, true, false); break;
+                    case 20: SMR_WALK_LAUNCH(false, 8, true, false); break;
+                    case 21: SMR_WALK_LAUNCH(true, 4, true, true); break;
+                    case 22: SMR_WALK_LAUNCH(true, 8, true, true); break;
+                    case 23: SMR_WALK_LAUNCH(true, 2, true, true); break;
+                    default: SMR_WALK_LAUNCH(true, 1, false, false); break;
                 }
 #undef SMR_WALK_LAUNCH
                 return;
