@@ -167,8 +167,9 @@ namespace smr::dev
         // neighbouring lanes; the two values beyond the strip are loaded by two
         // lanes (`ee`): lane 0 the left one, lane min(last + 1, 63) the right one
         // (last: the last active lane).  Else each lane loads both neighbours.
-        // Same arithmetic as k_upwind.
-        template <int G, bool SHIFT>
+        // NTL: the rows above are read with non-temporal loads.  Same arithmetic
+        // as k_upwind.
+        template <int G, bool SHIFT, bool NTL = false>
         struct WalkRows
         {
             double uu[G], el[G], er[SHIFT ? 1 : G];
@@ -193,7 +194,14 @@ namespace smr::dev
                         const int64_t bc = readlane64(rb, k + g + 1);
                         if (active)
                         {
-                            uu[g] = u[bu + x];
+                            if constexpr (NTL)
+                            {
+                                uu[g] = __builtin_nontemporal_load(u + bu + x);
+                            }
+                            else
+                            {
+                                uu[g] = u[bu + x];
+                            }
                         }
                         if constexpr (SHIFT)
                         {
@@ -213,7 +221,7 @@ namespace smr::dev
             }
         };
 
-        template <bool NT, int G, bool SHIFT, bool PIPE>
+        template <bool NT, int G, bool SHIFT, bool PIPE, bool NTL = false>
         __global__ __launch_bounds__(BLOCK) void k_upwind_walk(List l, const double* __restrict__ u, double* __restrict__ unp1, double a0,
                                                                double a1, double dt, unsigned nlogical)
         {
@@ -249,14 +257,14 @@ namespace smr::dev
                 uc = u[bc0 + x];
             }
 
-            WalkRows<G, SHIFT> cur;
+            WalkRows<G, SHIFT, NTL> cur;
             if constexpr (PIPE)
             {
                 cur.load(u, rb, 0, n, x, lane, last, active);
             }
             for (int32_t k = 0; k < n; k += G)
             {
-                WalkRows<G, SHIFT> nxt;
+                WalkRows<G, SHIFT, NTL> nxt;
                 if constexpr (PIPE)
                 {
                     if (k + G < n)
@@ -400,8 +408,8 @@ namespace smr::dev
             {12, -2, true, false},      {13, -2, true, true, 2},    {14, -2, true, true, 4},    {15, -2, true, true, 8},
             {16, -2, true, true, 1, true}, {17, -2, true, true, 4, true}, {18, -2, true, true, 8, true},
             {19, -2, true, true, 16, true}, {20, -2, true, false, 8, true}, {21, -2, true, true, 4, true, true},
-            {22, -2, true, true, 8, true, true}, {23, -2, true, true, 2, true, true}};
-        constexpr int kNumUpwindVariants    = 24;
+            {22, -2, true, true, 8, true, true}, {23, -2, true, true, 2, true, true}, {24, -2, true, true, 16, true}};
+        constexpr int kNumUpwindVariants    = 25;
         constexpr int kDefaultUpwindVariant = 10;
         int g_upwind_variant                = -1;
 
@@ -465,7 +473,7 @@ namespace smr::dev
         if (v.cpt == -1)
         {
             const bool dense = 4 * l.ncells >= 3 * 64 * l.ntiles;
-            v                = !dense ? kUpwindVariants[5] : (dim == 2 && l.nwalks > 0 ? kUpwindVariants[11] : kUpwindVariants[8]);
+            v                = !dense ? kUpwindVariants[5] : (dim == 2 && l.nwalks > 0 ? kUpwindVariants[19] : kUpwindVariants[8]);
         }
         if (v.cpt == -2)
         {
@@ -489,6 +497,10 @@ namespace smr::dev
                     case 21: SMR_WALK_LAUNCH(true, 4, true, true); break;
                     case 22: SMR_WALK_LAUNCH(true, 8, true, true); break;
                     case 23: SMR_WALK_LAUNCH(true, 2, true, true); break;
+                    case 24:
+                        hipLaunchKernelGGL((k_upwind_walk<true, 16, true, false, true>), dim3(ngrid), dim3(BLOCK), 0, s, l, u, unp1, a0, a1,
+                                           dt, unsigned(nblk));
+                        break;
                     default: SMR_WALK_LAUNCH(true, 1, false, false); break;
                 }
 #undef SMR_WALK_LAUNCH
