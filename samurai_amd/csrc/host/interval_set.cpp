@@ -447,6 +447,43 @@ namespace smr
             return out;
         }
         const int k = a.level - level;
+        if (a.dim == 2)
+        {
+            // the fine rows of one coarse row are consecutive: merge their
+            // coarsened runs (each row's already sorted by start) row group by
+            // row group
+            std::vector<std::array<int32_t, 2>> buf;
+            size_t r0 = 0;
+            while (r0 < a.nrows())
+            {
+                const int32_t y = floor_shift(a.ry[r0], k);
+                size_t r1       = r0 + 1;
+                while (r1 < a.nrows() && floor_shift(a.ry[r1], k) == y)
+                {
+                    ++r1;
+                }
+                buf.clear();
+                for (size_t r = r0; r < r1; ++r)
+                {
+                    const size_t mid = buf.size();
+                    for (int64_t q = a.rp[r]; q < a.rp[r + 1]; ++q)
+                    {
+                        const Run& run = a.runs[size_t(q)];
+                        buf.push_back({floor_shift(run.s, k), floor_shift(run.e - 1, k) + 1});
+                    }
+                    if (mid > 0)
+                    {
+                        std::inplace_merge(buf.begin(), buf.begin() + std::ptrdiff_t(mid), buf.end());
+                    }
+                }
+                for (const auto& b : buf)
+                {
+                    out.push(y, 0, b[0], b[1]);
+                }
+                r0 = r1;
+            }
+            return out;
+        }
         std::vector<std::array<int32_t, 4>> q;
         q.reserve(a.runs.size());
         a.for_each_run(
