@@ -1,13 +1,15 @@
 #include "runtime.hpp"
 
-#include <atomic>
-
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <unordered_map>
 
 namespace smr
 {
@@ -92,19 +94,121 @@ namespace smr
     // ------------------------------------------------------------------
     // Blob
     // ------------------------------------------------------------------
+    namespace
+    {
+        struct DevCache
+        {
+            std::mutex m;
+            // block -> (size, stream it was allocated on)
+            std::unordered_map<void*, std::pair<size_t, hipStream_t>> live;
+            // per stream: free blocks by size, and their total
+            std::unordered_map<hipStream_t, std::pair<std::multimap<size_t, void*>, size_t>> free;
+        };
+
+        DevCache& dev_cache()
+        {
+            // never destroyed: blocks may be freed by static destructors
+            static DevCache* c = new DevCache;
+            return *c;
+        }
+
+        // 256-byte multiples up to 4 KiB, then 8 size classes per power of two
+        size_t block_size(size_t n)
+        {
+            n = std::max<size_t>(n, 256);
+            if (n <= 4096)
+            {
+                return (n + 255) & ~size_t(255);
+            }
+            const int b       = 63 - __builtin_clzll(uint64_t(n));
+            const size_t step = size_t(1) << (b - 3);
+            return (n + step - 1) & ~(step - 1);
+        }
+
+        // frees the cached blocks of s, largest first, down to `keep` bytes (lock held)
+        void trim_locked(DevCache& c, hipStream_t s, size_t keep)
+        {
+            auto it = c.free.find(s);
+            if (it == c.free.end())
+            {
+                return;
+            }
+            auto& [blocks, total] = it->second;
+            while (total > keep && !blocks.empty())
+            {
+                auto last = std::prev(blocks.end());
+                (void)hipFreeAsync(last->second, s);
+                total -= last->first;
+                blocks.erase(last);
+            }
+        }
+    } // namespace
+
     void* dev_alloc(size_t bytes, hipStream_t s)
     {
-        void* p = nullptr;
-        hip_check(hipMallocAsync(&p, std::max<size_t>(bytes, 256), s), "hipMallocAsync");
+        const size_t n = block_size(bytes);
+        DevCache& c    = dev_cache();
+        std::lock_guard<std::mutex> lk(c.m);
+        auto& [blocks, total] = c.free[s];
+        // best fit, at most 1/4 larger than asked
+        auto it = blocks.lower_bound(n);
+        if (it != blocks.end() && it->first <= n + n / 4)
+        {
+            void* p = it->second;
+            c.live.emplace(p, std::make_pair(it->first, s));
+            total -= it->first;
+            blocks.erase(it);
+            return p;
+        }
+        void* p       = nullptr;
+        hipError_t e  = hipMallocAsync(&p, n, s);
+        if (e == hipErrorOutOfMemory && total > 0)
+        {
+            (void)hipGetLastError();
+            trim_locked(c, s, 0);
+            e = hipMallocAsync(&p, n, s);
+        }
+        hip_check(e, "hipMallocAsync");
+        c.live.emplace(p, std::make_pair(n, s));
         return p;
     }
 
     void dev_free(void* p, hipStream_t s)
     {
-        if (p)
+        if (!p)
         {
-            (void)hipFreeAsync(p, s);
+            return;
         }
+        DevCache& c = dev_cache();
+        std::lock_guard<std::mutex> lk(c.m);
+        auto it = c.live.find(p);
+        if (it == c.live.end() || it->second.second != s)
+        {
+            // not from dev_alloc on s: no reuse is ordered after its last use
+            if (it != c.live.end())
+            {
+                c.live.erase(it);
+            }
+            (void)hipFreeAsync(p, s);
+            return;
+        }
+        const size_t n = it->second.first;
+        c.live.erase(it);
+        auto& [blocks, total] = c.free[s];
+        blocks.emplace(n, p);
+        total += n;
+        if (total > kDevCacheLimit)
+        {
+            trim_locked(c, s, kDevCacheLimit / 2);
+        }
+    }
+
+    void dev_cache_release(hipStream_t s)
+    {
+        DevCache& c = dev_cache();
+        std::lock_guard<std::mutex> lk(c.m);
+        trim_locked(c, s, 0);
+        c.free.erase(s);
     }
 
     Blob::~Blob()
@@ -223,6 +327,9 @@ namespace smr
         }
         if (stream)
         {
+            (void)hipStreamSynchronize(stream);
+            dev_cache_release(stream);
+            (void)hipStreamSynchronize(stream);
             (void)hipStreamDestroy(stream);
         }
     }
