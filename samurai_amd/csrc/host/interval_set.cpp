@@ -39,35 +39,6 @@ namespace smr
         return true;
     }
 
-    void LevelSet::push(int32_t y, int32_t z, int32_t s, int32_t e)
-    {
-        if (e <= s)
-        {
-            return;
-        }
-        if (!ry.empty() && ry.back() == y && rz.back() == z)
-        {
-            Run& b = runs.back();
-            if (s <= b.e)
-            {
-                if (e > b.e)
-                {
-                    b.e = e;
-                }
-                return;
-            }
-            runs.push_back({s, e});
-            rp.back() = int64_t(runs.size());
-        }
-        else
-        {
-            ry.push_back(y);
-            rz.push_back(z);
-            runs.push_back({s, e});
-            rp.push_back(int64_t(runs.size()));
-        }
-    }
-
     int64_t LevelSet::find_row(int32_t y, int32_t z) const
     {
         int64_t lo = 0, hi = int64_t(ry.size());

@@ -54,7 +54,35 @@ namespace smr
         }
 
         // appends a run; rows must come in (z, y) order and runs in x order
-        void push(int32_t y, int32_t z, int32_t s, int32_t e);
+        // (a run overlapping or touching the row's last run extends it)
+        void push(int32_t y, int32_t z, int32_t s, int32_t e)
+        {
+            if (e <= s)
+            {
+                return;
+            }
+            if (!ry.empty() && ry.back() == y && rz.back() == z)
+            {
+                Run& b = runs.back();
+                if (s <= b.e)
+                {
+                    if (e > b.e)
+                    {
+                        b.e = e;
+                    }
+                    return;
+                }
+                runs.push_back({s, e});
+                rp.back() = int64_t(runs.size());
+            }
+            else
+            {
+                ry.push_back(y);
+                rz.push_back(z);
+                runs.push_back({s, e});
+                rp.push_back(int64_t(runs.size()));
+            }
+        }
 
         // row index of (y, z) or -1 (binary search)
         int64_t find_row(int32_t y, int32_t z) const;
