@@ -875,6 +875,8 @@ namespace smr
                     uf_prog_.run(tuning_);
                     tp.reset();
                     ScopedTimer tr(ctx_, "old storage release");
+                    std::vector<std::unique_ptr<DevMesh>> old_dev(nc);
+                    std::vector<MRMeshData> old_host(nc);
                     for (size_t j = 0; j < nc; ++j)
                     {
                         BatchInstance& b = inst_[size_t(act[changed[j]])];
@@ -883,6 +885,8 @@ namespace smr
                         // old storage and old image are released in stream order after the program
                         dev_free(b.u->d, s);
                         dev_free(b.unp1->d, s);
+                        old_host[j] = std::move(m.host);
+                        old_dev[j]  = std::move(m.dev);
                         m.replace(std::move(rb[changed[j]].nm), std::move(ndev[j]));
                         b.u->d          = nu[j];
                         b.unp1->d       = nv[j];
@@ -891,6 +895,16 @@ namespace smr
                         b.u->version    = m.version;
                         b.unp1->version = m.version;
                     }
+                    // the host halves of the old versions and of the update_fields
+                    // lists are freed in parallel
+                    parallel_for(int(nc),
+                                 [&](int q)
+                                 {
+                                     old_dev[size_t(q)].reset();
+                                     old_host[size_t(q)] = MRMeshData();
+                                     ufl[size_t(q)]      = {};
+                                     rb[changed[size_t(q)]] = Rebuild();
+                                 });
                 }
             }
             std::vector<int> next;
