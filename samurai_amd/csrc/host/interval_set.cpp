@@ -487,11 +487,78 @@ namespace smr
         return out;
     }
 
+    namespace
+    {
+        // 2D: on(expand(a, w), a.level - k) for k >= 0 in one sweep over the
+        // rows.  Coarse row Y collects the input rows y with
+        // Y 2^k - w <= y <= Y 2^k + 2^k - 1 + w (consecutive in a); each maps
+        // its runs to [floor((s - w) / 2^k), floor((e - 1 + w) / 2^k) + 1),
+        // sorted per row, so a merge of the window's rows gives the row.
+        LevelSet expand_on_2d(const LevelSet& a, int w, int k)
+        {
+            LevelSet out(2, a.level - k);
+            const size_t n = a.nrows();
+            if (n == 0)
+            {
+                return out;
+            }
+            const int64_t f = int64_t(1) << k;
+            std::vector<std::array<int32_t, 2>> buf;
+            size_t lo = 0, hi = 0;
+            int64_t y = floor_shift(int32_t(a.ry[0] - w), k);
+            for (;;)
+            {
+                const int64_t ylo = y * f - w, yhi = y * f + f - 1 + w;
+                while (hi < n && a.ry[hi] <= yhi)
+                {
+                    ++hi;
+                }
+                while (lo < hi && a.ry[lo] < ylo)
+                {
+                    ++lo;
+                }
+                if (lo == hi)
+                {
+                    if (hi == n)
+                    {
+                        break;
+                    }
+                    y = floor_shift(int32_t(a.ry[hi] - w), k);
+                    continue;
+                }
+                buf.clear();
+                for (size_t r = lo; r < hi; ++r)
+                {
+                    const size_t mid = buf.size();
+                    for (int64_t q = a.rp[r]; q < a.rp[r + 1]; ++q)
+                    {
+                        const Run& run = a.runs[size_t(q)];
+                        buf.push_back({floor_shift(run.s - w, k), floor_shift(run.e - 1 + w, k) + 1});
+                    }
+                    if (mid > 0)
+                    {
+                        std::inplace_merge(buf.begin(), buf.begin() + std::ptrdiff_t(mid), buf.end());
+                    }
+                }
+                for (const auto& b : buf)
+                {
+                    out.push(int32_t(y), 0, b[0], b[1]);
+                }
+                ++y;
+            }
+            return out;
+        }
+    } // namespace
+
     LevelSet expand(const LevelSet& a, int w)
     {
         if (w == 0 || a.empty())
         {
             return a;
+        }
+        if (a.dim == 2)
+        {
+            return expand_on_2d(a, w, 0);
         }
         std::vector<std::array<int32_t, 4>> q;
         const int wz = a.dim == 3 ? w : 0;
@@ -516,6 +583,10 @@ namespace smr
         if (k < 0)
         {
             return on(expand(a, w), level);
+        }
+        if (a.dim == 2)
+        {
+            return expand_on_2d(a, w, k);
         }
         // the box dilation of a run projects onto a box of coarse cells
         std::vector<std::array<int32_t, 4>> q;

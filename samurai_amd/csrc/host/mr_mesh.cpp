@@ -424,48 +424,29 @@ namespace smr
         {
             ++nit;
             CellArray remove = empty_cell_array(dim);
-            // list_interval_to_refine_for_graduation (graduation.hpp:245-330)
+            // list_interval_to_refine_for_graduation (graduation.hpp:245-330):
+            // for every fine level f and every coarse level c <= f - 2, the
+            // cells of c under on(expand(cells[f], 2w), c).  Coarsening
+            // distributes over unions and composes, so the union over f of
+            // those projections at level c is P_c = on(P_{c+1}, c) |
+            // on(expand(cells[c + 2], 2w), c): one pass from the finest level
+            // gives the same sets as the per-fine-level cascades
             const int fmax = ca_max_level(ca) + 1;
             const int cmin = ca_min_level(ca);
-            // the cascade of each fine level is independent: run them in
-            // parallel, then merge the coarse cells they flag level by level
-            std::vector<CellArray> part(MAXL);
-            parallel_for(MAXL,
-                         [&](int fine_level)
-                         {
-                             if (fine_level > fmax || fine_level <= cmin + 1 || ca[size_t(fine_level)].empty())
-                             {
-                                 return;
-                             }
-                             CellArray& out = part[size_t(fine_level)];
-                             out            = empty_cell_array(dim);
-                             LevelSet proj  = expand_on(ca[size_t(fine_level)], 2 * grad_width, fine_level - 2);
-                             for (int coarse_level = fine_level - 2;; --coarse_level)
-                             {
-                                 if (!proj.empty())
-                                 {
-                                     out[size_t(coarse_level)] = set_intersection(proj, ca[size_t(coarse_level)]);
-                                 }
-                                 if (coarse_level == cmin || proj.empty())
-                                 {
-                                     break;
-                                 }
-                                 proj = on(proj, coarse_level - 1);
-                             }
-                         });
-            for (int fine_level = MAXL - 1; fine_level >= 0; --fine_level)
+            LevelSet proj(dim, std::max(fmax - 1, 0));
+            for (int c = fmax - 2; c >= cmin; --c)
             {
-                if (part[size_t(fine_level)].empty())
+                LevelSet p = proj.empty() ? LevelSet(dim, c) : on(proj, c);
+                if (c + 2 <= fmax && !ca[size_t(c + 2)].empty())
                 {
-                    continue;
+                    LevelSet e = expand_on(ca[size_t(c + 2)], 2 * grad_width, c);
+                    p          = p.empty() ? std::move(e) : set_union(p, e);
                 }
-                for (int l = 0; l < MAXL; ++l)
+                if (!p.empty())
                 {
-                    if (!part[size_t(fine_level)][size_t(l)].empty())
-                    {
-                        remove[size_t(l)] = set_union(remove[size_t(l)], part[size_t(fine_level)][size_t(l)]);
-                    }
+                    remove[size_t(c)] = set_intersection(p, ca[size_t(c)]);
                 }
+                proj = std::move(p);
             }
             bool any_change   = false;
             CellArray add_p   = empty_cell_array(dim);
