@@ -609,6 +609,53 @@ namespace smr
         return from_runs(a.dim, level, q);
     }
 
+    namespace
+    {
+        bool row_in_box(const LevelSet& a, size_t r, const std::array<int, 3>& lo, const std::array<int, 3>& hi)
+        {
+            return a.ry[r] >= lo[1] && a.ry[r] < hi[1] && (a.dim == 2 || (a.rz[r] >= lo[2] && a.rz[r] < hi[2]));
+        }
+    } // namespace
+
+    LevelSet clip_box(const LevelSet& a, const std::array<int, 3>& lo, const std::array<int, 3>& hi)
+    {
+        LevelSet out(a.dim, a.level);
+        for (size_t r = 0; r < a.nrows(); ++r)
+        {
+            if (!row_in_box(a, r, lo, hi))
+            {
+                continue;
+            }
+            for (int64_t k = a.rp[r]; k < a.rp[r + 1]; ++k)
+            {
+                const Run& run = a.runs[size_t(k)];
+                out.push(a.ry[r], a.rz[r], std::max(run.s, int32_t(lo[0])), std::min(run.e, int32_t(hi[0])));
+            }
+        }
+        return out;
+    }
+
+    LevelSet minus_box(const LevelSet& a, const std::array<int, 3>& lo, const std::array<int, 3>& hi)
+    {
+        LevelSet out(a.dim, a.level);
+        for (size_t r = 0; r < a.nrows(); ++r)
+        {
+            const bool in = row_in_box(a, r, lo, hi);
+            for (int64_t k = a.rp[r]; k < a.rp[r + 1]; ++k)
+            {
+                const Run& run = a.runs[size_t(k)];
+                if (!in)
+                {
+                    out.push(a.ry[r], a.rz[r], run.s, run.e);
+                    continue;
+                }
+                out.push(a.ry[r], a.rz[r], run.s, std::min(run.e, int32_t(lo[0])));
+                out.push(a.ry[r], a.rz[r], std::max(run.s, int32_t(hi[0])), run.e);
+            }
+        }
+        return out;
+    }
+
     LevelSet box(int dim, int level, const std::array<int, 3>& lo, const std::array<int, 3>& hi)
     {
         LevelSet out(dim, level);

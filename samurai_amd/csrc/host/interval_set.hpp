@@ -117,6 +117,9 @@ namespace smr
     // on(expand(a, w), level) in one pass
     LevelSet expand_on(const LevelSet& a, int w, int level);
     LevelSet box(int dim, int level, const std::array<int, 3>& lo, const std::array<int, 3>& hi);
+    // a n box(lo, hi) and a \ box(lo, hi) in one pass over a (z bounds ignored in 2D)
+    LevelSet clip_box(const LevelSet& a, const std::array<int, 3>& lo, const std::array<int, 3>& hi);
+    LevelSet minus_box(const LevelSet& a, const std::array<int, 3>& lo, const std::array<int, 3>& hi);
 
     // (y, z, s, e) quadruples in any order -> canonical set
     LevelSet from_runs(int dim, int level, std::vector<std::array<int32_t, 4>>& q);

@@ -198,8 +198,10 @@ namespace smr
                                  }
                                  if (task % 2 == 0)
                                  {
-                                     below1[size_t(level)] = set_intersection(expand(on(m.ids[CELLS_AND_GHOSTS][size_t(level)], level - 1), s),
-                                                                              expand(m.domain[size_t(level - 1)], s));
+                                     // n expand(domain(level - 1), s): the box [-s, 2^(level-1) + s)
+                                     const int n           = 1 << (level - 1);
+                                     below1[size_t(level)] = clip_box(expand(on(m.ids[CELLS_AND_GHOSTS][size_t(level)], level - 1), s),
+                                                                      {-s, -s, -s}, {n + s, n + s, n + s});
                                  }
                                  else if (level - 1 > 0)
                                  {

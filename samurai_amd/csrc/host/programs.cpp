@@ -2,6 +2,7 @@
 #include "parallel.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <numeric>
 #include <stdexcept>
 #include <unordered_map>
@@ -154,11 +155,38 @@ namespace smr
             }
         }
 
-        LevelSet domain_layer(const MRMeshData& m, int level, const Dir& dir)
+        // the box of domain(level) shifted by t: [t, 2^level + t) on each axis
+        struct Bounds
         {
-            // cells of domain(level) whose neighbour in `dir` is outside the box
-            const LevelSet& dom = m.domain[size_t(level)];
-            return set_difference(dom, translate(dom, {-dir[0], -dir[1], -dir[2]}));
+            std::array<int, 3> lo, hi;
+        };
+        Bounds domain_bounds(int level, const Dir& t = {0, 0, 0})
+        {
+            const int n = 1 << level;
+            return {{t[0], t[1], t[2]}, {n + t[0], n + t[1], n + t[2]}};
+        }
+
+        // the cells of domain(level) whose neighbour in `dir` is outside it:
+        // the face layer x = n - 1 for dir x = +1, x = 0 for -1 (each axis)
+        Bounds domain_layer(int level, const Dir& dir)
+        {
+            if (std::abs(dir[0]) + std::abs(dir[1]) + std::abs(dir[2]) != 1)
+            {
+                throw std::logic_error("domain_layer: a cartesian direction is expected");
+            }
+            Bounds b = domain_bounds(level);
+            for (int i = 0; i < 3; ++i)
+            {
+                if (dir[size_t(i)] > 0)
+                {
+                    b.lo[size_t(i)] = b.hi[size_t(i)] - 1;
+                }
+                else if (dir[size_t(i)] < 0)
+                {
+                    b.hi[size_t(i)] = b.lo[size_t(i)] + 1;
+                }
+            }
+            return b;
         }
 
         // update_outer_ghosts(level) (algorithm/update_outer_ghost.hpp:338-432)
@@ -224,7 +252,8 @@ namespace smr
                 if (level < m.cfg.max_level)
                 {
                     // project_bc, layer 1 (algorithm/update_outer_ghost.hpp:20-152)
-                    LevelSet outside = set_difference(translate(m.ids[UNION_CELLS][size_t(level)], dir), m.domain[size_t(level)]);
+                    const Bounds dom = domain_bounds(level);
+                    LevelSet outside = minus_box(translate(m.ids[UNION_CELLS][size_t(level)], dir), dom.lo, dom.hi);
                     LevelSet ghosts  = set_intersection(outside, ref[size_t(level)]);
                     std::vector<int64_t> ch;
                     ghosts.for_each_run(
@@ -264,7 +293,8 @@ namespace smr
                 if (level >= m.cfg.min_level)
                 {
                     // Dirichlet<1> on the boundary cells (bc/apply_field_bc.hpp:53-101)
-                    LevelSet bdry = set_intersection(cells[size_t(level)], domain_layer(m, level, dir));
+                    const Bounds layer = domain_layer(level, dir);
+                    LevelSet bdry      = clip_box(cells[size_t(level)], layer.lo, layer.hi);
                     bdry.for_each_run(
                         [&](int32_t x0, int32_t x1, int32_t y, int32_t z)
                         {
@@ -278,9 +308,10 @@ namespace smr
                 if (level < m.cfg.max_level && level >= m.cfg.min_level)
                 {
                     // predict_bc, order 0 (algorithm/update_outer_ghost.hpp:154-262)
-                    const LevelSet& dom = m.domain[size_t(level)];
-                    LevelSet inner      = set_difference(cells[size_t(level)], translate(dom, {-dir[0], -dir[1], -dir[2]}));
-                    LevelSet bc_ghosts  = set_difference(translate(inner, dir), dom);
+                    const Bounds dom     = domain_bounds(level);
+                    const Bounds shifted = domain_bounds(level, {-dir[0], -dir[1], -dir[2]});
+                    LevelSet inner       = minus_box(cells[size_t(level)], shifted.lo, shifted.hi);
+                    LevelSet bc_ghosts   = minus_box(translate(inner, dir), dom.lo, dom.hi);
                     LevelSet ghosts     = set_intersection(bc_ghosts, ref[size_t(level + 1)]);
                     ghosts.for_each_run(
                         [&](int32_t x0, int32_t x1, int32_t y, int32_t z)
@@ -334,8 +365,9 @@ namespace smr
             {
                 const LevelSet& ref  = m.ids[REFERENCE][size_t(level)];
                 LevelSet pred_ghosts = set_difference(ref, set_union(m.ids[CELLS][size_t(level)], m.ids[PROJ_CELLS][size_t(level)]));
+                const Bounds dom = domain_bounds(level);
                 sets.pred[size_t(level)] =
-                    set_intersection(set_intersection(pred_ghosts, m.domain[size_t(level)]), m.ids[REFERENCE][size_t(level - 1)]);
+                    set_intersection(clip_box(pred_ghosts, dom.lo, dom.hi), m.ids[REFERENCE][size_t(level - 1)]);
             }
         }
 
