@@ -356,49 +356,119 @@ namespace smr
     CellArray update_cell_array_from_tag(const MRMeshData& mesh, const uint8_t* tags)
     {
         const int dim = mesh.cfg.dim;
-        std::vector<std::vector<std::array<int32_t, 4>>> add_m(MAXL), rm_m(MAXL), add_p(MAXL), rm_p(MAXL);
+        // per level: cells added by coarsening (parents) and by refinement
+        // (children), cells removed by coarsening and by refinement
+        CellArray add_m = empty_cell_array(dim), rm_m = empty_cell_array(dim), add_p = empty_cell_array(dim),
+                  rm_p = empty_cell_array(dim);
         const CellArray& cells = mesh.ids[CELLS];
         const int start = ca_min_level(cells), end = ca_max_level(cells) + 1;
-        // level l only appends to add_m[l-1], rm_m[l], add_p[l+1] and rm_p[l]:
-        // the levels are independent
+        // level l only writes add_m[l-1], rm_m[l], add_p[l+1] and rm_p[l]: the
+        // levels are independent.  Cells are visited in (z, y, x) order and
+        // their actions gathered in stretches of equal action, so every set
+        // except the 3D children comes out in canonical order and is pushed
+        // directly; the 3D children (rows 2y, 2y+1 of planes 2z, 2z+1) are
+        // collected and ordered by from_runs.
         parallel_for(std::max(end - start, 0), [&](int li)
         {
             const int level    = start + li;
             const LevelSet& cl = cells[size_t(level)];
+            const bool can_refine = level < mesh.cfg.max_level, can_coarsen = level > mesh.cfg.min_level;
+            LevelSet& rmm = rm_m[size_t(level)];
+            LevelSet& rmp = rm_p[size_t(level)];
+            LevelSet& amm = add_m[size_t(level > 0 ? level - 1 : 0)];
+            LevelSet kids2(dim, level + 1);
+            std::vector<std::array<int32_t, 4>> kids3;
+            std::vector<std::array<int32_t, 2>> row_kids;
+            rmm = LevelSet(dim, level);
+            rmp = LevelSet(dim, level);
+            if (level > 0)
+            {
+                amm = LevelSet(dim, level - 1);
+            }
             for (size_t r = 0; r < cl.nrows(); ++r)
             {
                 const int32_t y = cl.ry[r], z = cl.rz[r];
                 const bool yz_even = (y % 2 == 0) && (dim == 2 || z % 2 == 0);
+                row_kids.clear();
+                // action of a stretch: 1 refine, 2 coarsen, 0 none
+                auto flush = [&](int act, int32_t xa, int32_t xb)
+                {
+                    if (act == 1)
+                    {
+                        rmp.push(y, z, xa, xb);
+                        row_kids.push_back({2 * xa, 2 * xb});
+                    }
+                    else if (act == 2)
+                    {
+                        rmm.push(y, z, xa, xb);
+                        if (yz_even)
+                        {
+                            // parents of the even x of [xa, xb)
+                            const int32_t e0 = xa + (xa & 1), e1 = (xb - 1) - ((xb - 1) & 1);
+                            if (e0 <= e1)
+                            {
+                                amm.push(y >> 1, dim == 3 ? (z >> 1) : 0, e0 >> 1, (e1 >> 1) + 1);
+                            }
+                        }
+                    }
+                };
                 for (int64_t k = cl.rp[r]; k < cl.rp[r + 1]; ++k)
                 {
-                    const Run run = cl.runs[size_t(k)];
-                    int64_t base  = mesh.index(level, run.s, y, z) - run.s;
+                    const Run run      = cl.runs[size_t(k)];
+                    const int64_t base = mesh.index(level, run.s, y, z) - run.s;
+                    int act = -1;
+                    int32_t xa = run.s;
                     for (int32_t x = run.s; x < run.e; ++x)
                     {
-                        const uint8_t t   = tags[base + x];
-                        const bool refine = t & FLAG_REFINE;
-                        const bool cnk    = (t & FLAG_COARSEN) && !(t & FLAG_KEEP);
-                        if (refine && level < mesh.cfg.max_level)
+                        const uint8_t t = tags[base + x];
+                        int a           = 0;
+                        if ((t & FLAG_REFINE) && can_refine)
                         {
-                            rm_p[size_t(level)].push_back({y, z, x, x + 1});
-                            for (int dz = 0; dz < (dim == 3 ? 2 : 1); ++dz)
+                            a = 1;
+                        }
+                        else if ((t & FLAG_COARSEN) && !(t & FLAG_KEEP) && can_coarsen)
+                        {
+                            a = 2;
+                        }
+                        if (a != act)
+                        {
+                            flush(act, xa, x);
+                            act = a;
+                            xa  = x;
+                        }
+                    }
+                    flush(act, xa, run.e);
+                }
+                if (!row_kids.empty())
+                {
+                    if (dim == 2)
+                    {
+                        for (int dy = 0; dy < 2; ++dy)
+                        {
+                            for (const auto& kx : row_kids)
                             {
-                                for (int dy = 0; dy < 2; ++dy)
+                                kids2.push(2 * y + dy, 0, kx[0], kx[1]);
+                            }
+                        }
+                    }
+                    else
+                    {
+                        for (int dz = 0; dz < 2; ++dz)
+                        {
+                            for (int dy = 0; dy < 2; ++dy)
+                            {
+                                for (const auto& kx : row_kids)
                                 {
-                                    add_p[size_t(level + 1)].push_back({2 * y + dy, dim == 3 ? 2 * z + dz : 0, 2 * x, 2 * x + 2});
+                                    kids3.push_back({2 * y + dy, 2 * z + dz, kx[0], kx[1]});
                                 }
                             }
                         }
-                        else if (cnk && level > mesh.cfg.min_level)
-                        {
-                            if (x % 2 == 0 && yz_even)
-                            {
-                                add_m[size_t(level - 1)].push_back({y >> 1, dim == 3 ? (z >> 1) : 0, x >> 1, (x >> 1) + 1});
-                            }
-                            rm_m[size_t(level)].push_back({y, z, x, x + 1});
-                        }
                     }
                 }
+            }
+            if (level + 1 < MAXL)
+            {
+                add_p[size_t(level + 1)] = dim == 2 ? std::move(kids2) : from_runs(dim, level + 1, kids3);
             }
         });
         CellArray out = empty_cell_array(dim);
@@ -406,12 +476,9 @@ namespace smr
         parallel_for(lmax - lmin + 1,
                      [&](int li)
                      {
-                         const int level    = lmin + li;
-                         LevelSet am        = from_runs(dim, level, add_m[size_t(level)]);
-                         LevelSet ap        = from_runs(dim, level, add_p[size_t(level)]);
-                         LevelSet rm        = from_runs(dim, level, rm_m[size_t(level)]);
-                         LevelSet rp        = from_runs(dim, level, rm_p[size_t(level)]);
-                         out[size_t(level)] = set_difference(set_union(cells[size_t(level)], am, ap), set_union(rm, rp));
+                         const int level = lmin + li;
+                         const size_t l  = size_t(level);
+                         out[l] = set_difference(set_union(cells[l], add_m[l], add_p[l]), set_union(rm_m[l], rm_p[l]));
                      });
         return out;
     }
