@@ -188,8 +188,10 @@ namespace smr
                 const double h  = 1.0 / double(1 << level);
                 const double fa = h_factor(dim, h, h);
                 double F[2]     = {fc[0] * fa, fc[1] * fa};
-                const LevelSet& dom = m.domain[size_t(level)];
-                const LevelSet B    = set_difference(cl, translate(dom, me));
+                // cells whose neighbour across the face is outside the domain:
+                // cl minus the domain box shifted by -e (then by +e)
+                const int n       = 1 << level;
+                const LevelSet B  = minus_box(cl, me, {n + me[0], n + me[1], n + me[2]});
                 B.for_each_run(
                     [&](int32_t x0, int32_t x1, int32_t y, int32_t z)
                     {
@@ -206,7 +208,7 @@ namespace smr
                     });
                 F[0] *= -1;
                 F[1] *= -1;
-                const LevelSet Bo = set_difference(cl, translate(dom, e));
+                const LevelSet Bo = minus_box(cl, e, {n + e[0], n + e[1], n + e[2]});
                 Bo.for_each_run(
                     [&](int32_t x0, int32_t x1, int32_t y, int32_t z)
                     {

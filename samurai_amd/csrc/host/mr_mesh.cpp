@@ -261,7 +261,7 @@ namespace smr
             }
             m.corners.clear();
             m.corner_dirs.clear();
-            const LevelSet& dom = m.domain[size_t(m.cfg.max_level)];
+            const LevelSet dom = domain_at(m.cfg.dim, m.cfg.max_level);
             for (const Dir& d : diagonal_directions(m.cfg.dim))
             {
                 LevelSet u(m.cfg.dim, m.cfg.max_level);
@@ -325,10 +325,6 @@ namespace smr
         }
         MRMeshData m;
         m.cfg = cfg;
-        for (int l = 0; l <= cfg.max_level; ++l)
-        {
-            m.domain.push_back(domain_at(cfg.dim, l));
-        }
         for (auto& v : m.ids)
         {
             v = empty_cell_array(cfg.dim);

@@ -55,7 +55,6 @@ namespace smr
     {
         MeshConfig cfg;
         CellArray ids[N_MESH_IDS];
-        std::vector<LevelSet> domain; // levels 0..max_level
         std::vector<LevelSet> corners;
         std::vector<Dir> corner_dirs;
         // reference storage: value index of x in run k of level l is ridx[l][k] + x
