@@ -27,16 +27,26 @@ namespace smr
     void WorkList::add(const LevelSet& s)
     {
         begin_level(s.level);
-        s.for_each_run(
-            [&](int32_t x0, int32_t x1, int32_t y, int32_t z)
+        const size_t r0 = size_t(nruns()), nr = s.runs.size();
+        iv.resize(iv.size() + 5 * nr);
+        pref.resize(pref.size() + nr);
+        int32_t* v = iv.data() + 5 * r0;
+        int64_t* p = pref.data() + r0;
+        for (size_t r = 0; r < s.nrows(); ++r)
+        {
+            for (int64_t k = s.rp[r]; k < s.rp[r + 1]; ++k)
             {
-                iv.push_back(x0);
-                iv.push_back(x1);
-                iv.push_back(y);
-                iv.push_back(z);
-                iv.push_back(s.level);
-                pref.push_back(pref.back() + (x1 - x0));
-            });
+                const Run& run = s.runs[size_t(k)];
+                v[0] = run.s;
+                v[1] = run.e;
+                v[2] = s.ry[r];
+                v[3] = s.rz[r];
+                v[4] = s.level;
+                v += 5;
+                p[1] = p[0] + (run.e - run.s);
+                ++p;
+            }
+        }
     }
 
     void WorkList::finish()
