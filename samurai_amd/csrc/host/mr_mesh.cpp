@@ -496,23 +496,36 @@ namespace smr
             // those projections at level c is P_c = on(P_{c+1}, c) |
             // on(expand(cells[c + 2], 2w), c): one pass from the finest level
             // gives the same sets as the per-fine-level cascades
+            // (the projections of each level and the intersections run in
+            // parallel over the levels; only the chain of unions is serial)
             const int fmax = ca_max_level(ca) + 1;
             const int cmin = ca_min_level(ca);
-            LevelSet proj(dim, std::max(fmax - 1, 0));
-            for (int c = fmax - 2; c >= cmin; --c)
+            std::vector<LevelSet> proj(MAXL);
+            parallel_for(MAXL,
+                         [&](int c)
+                         {
+                             if (c >= cmin && c + 2 <= fmax && !ca[size_t(c + 2)].empty())
+                             {
+                                 proj[size_t(c)] = expand_on(ca[size_t(c + 2)], 2 * grad_width, c);
+                             }
+                         });
+            for (int c = fmax - 3; c >= cmin; --c)
             {
-                LevelSet p = proj.empty() ? LevelSet(dim, c) : on(proj, c);
-                if (c + 2 <= fmax && !ca[size_t(c + 2)].empty())
+                const LevelSet& finer = proj[size_t(c + 1)];
+                if (!finer.empty())
                 {
-                    LevelSet e = expand_on(ca[size_t(c + 2)], 2 * grad_width, c);
-                    p          = p.empty() ? std::move(e) : set_union(p, e);
+                    LevelSet p      = on(finer, c);
+                    proj[size_t(c)] = proj[size_t(c)].empty() ? std::move(p) : set_union(proj[size_t(c)], p);
                 }
-                if (!p.empty())
-                {
-                    remove[size_t(c)] = set_intersection(p, ca[size_t(c)]);
-                }
-                proj = std::move(p);
             }
+            parallel_for(MAXL,
+                         [&](int c)
+                         {
+                             if (c >= cmin && !proj[size_t(c)].empty())
+                             {
+                                 remove[size_t(c)] = set_intersection(proj[size_t(c)], ca[size_t(c)]);
+                             }
+                         });
             bool any_change   = false;
             CellArray add_p   = empty_cell_array(dim);
             for (int level = min_level; level < max_level + 1; ++level)
