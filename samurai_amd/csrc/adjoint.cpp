@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <iterator>
 #include <stdexcept>
 
 namespace smr
@@ -86,7 +87,107 @@ namespace smr
             }
             w.host.finish();
         }
+
+        // storage slots of the ghost neighbours in direction off of the cells of w
+        std::vector<int64_t> scatter_targets(const DevWorkList& w, const MRMeshData& m, const std::array<int, 3>& off)
+        {
+            std::vector<int64_t> t;
+            t.reserve(size_t(w.host.ncells()));
+            const auto& iv = w.host.iv;
+            for (size_t r = 0; r + 4 < iv.size(); r += 5)
+            {
+                const int32_t x0 = iv[r], x1 = iv[r + 1], y = iv[r + 2], z = iv[r + 3], level = iv[r + 4];
+                const int64_t b  = m.index(level, x0 + off[0], y + off[1], z + off[2]) - x0;
+                for (int32_t x = x0; x < x1; ++x)
+                {
+                    t.push_back(b + x);
+                }
+            }
+            std::sort(t.begin(), t.end());
+            return t;
+        }
     } // namespace
+
+    std::vector<std::vector<std::pair<int, int>>> group_reverse_stages(const std::vector<LinearStage>& ghost)
+    {
+        auto sorted_unique = [](std::vector<int64_t> v)
+        {
+            std::sort(v.begin(), v.end());
+            v.erase(std::unique(v.begin(), v.end()), v.end());
+            return v;
+        };
+        auto overlap = [](const std::vector<int64_t>& a, const std::vector<int64_t>& b)
+        {
+            size_t i = 0, j = 0;
+            while (i < a.size() && j < b.size())
+            {
+                if (a[i] == b[j])
+                {
+                    return true;
+                }
+                if (a[i] < b[j])
+                {
+                    ++i;
+                }
+                else
+                {
+                    ++j;
+                }
+            }
+            return false;
+        };
+        auto merged = [](const std::vector<int64_t>& a, const std::vector<int64_t>& b)
+        {
+            std::vector<int64_t> out;
+            out.reserve(a.size() + b.size());
+            std::set_union(a.begin(), a.end(), b.begin(), b.end(), std::back_inserter(out));
+            return out;
+        };
+        std::vector<std::vector<std::pair<int, int>>> groups;
+        std::vector<int64_t> rs, ws; // slots read / written by the open group
+        // the reverse sequence: gather of stage k then zeroing of its outputs,
+        // for k from the last forward stage down; an item joins the open
+        // group when it neither reads nor writes a slot the group writes and
+        // writes no slot the group reads, else it opens the next group
+        for (size_t k = ghost.size(); k-- > 0;)
+        {
+            const LinearStage& g = ghost[k];
+            for (int kind = 0; kind < 2; ++kind)
+            {
+                std::vector<int64_t> r, w;
+                if (kind == 0)
+                {
+                    if (g.tgt.empty())
+                    {
+                        continue;
+                    }
+                    r = sorted_unique(g.src);
+                    w = sorted_unique(g.tgt);
+                }
+                else
+                {
+                    if (g.dst.empty())
+                    {
+                        continue;
+                    }
+                    w = sorted_unique(g.dst);
+                }
+                if (!groups.empty() && !overlap(ws, r) && !overlap(ws, w) && !overlap(rs, w))
+                {
+                    groups.back().emplace_back(int(k), kind);
+                    rs = merged(rs, r);
+                    ws = merged(ws, w);
+                }
+                else
+                {
+                    groups.push_back({{int(k), kind}});
+                    rs = std::move(r);
+                    ws = std::move(w);
+                }
+            }
+        }
+        return groups;
+    }
 
     std::vector<LinearStage> ghost_linear_stages(const MRMeshData& m, const DevMesh& dm, bool has_bc)
     {
@@ -278,11 +379,34 @@ namespace smr
                          Instance& in  = inst_[size_t(i)];
                          const auto& m = in.mesh->host;
                          in.ghost      = ghost_linear_stages(m, *in.mesh->dev, in.has_bc);
+                         in.adj_groups = group_reverse_stages(in.ghost);
                          const std::array<std::array<int, 3>, 6> offs = {
                              {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}}};
+                         // directions whose ghost targets are disjoint share a launch
+                         std::vector<int64_t> open;
+                         in.scatter_groups.clear();
                          for (int d = 0; d < 2 * m.cfg.dim; ++d)
                          {
                              add_scatter_list(in.wdir[size_t(d)], m, in.mesh->dev->cells.host, offs[size_t(d)]);
+                             if (in.wdir[size_t(d)].host.ncells() == 0)
+                             {
+                                 continue;
+                             }
+                             std::vector<int64_t> t = scatter_targets(in.wdir[size_t(d)], m, offs[size_t(d)]);
+                             std::vector<int64_t> both;
+                             std::set_intersection(open.begin(), open.end(), t.begin(), t.end(), std::back_inserter(both));
+                             if (!in.scatter_groups.empty() && both.empty())
+                             {
+                                 in.scatter_groups.back().push_back(d);
+                                 std::vector<int64_t> u;
+                                 std::set_union(open.begin(), open.end(), t.begin(), t.end(), std::back_inserter(u));
+                                 open.swap(u);
+                             }
+                             else
+                             {
+                                 in.scatter_groups.push_back({d});
+                                 open.swap(t);
+                             }
                          }
                      });
         hipStream_t st = ctx_->stream;
@@ -466,54 +590,58 @@ namespace smr
                         u.a2           = c.dim == 3 ? in.a[2] : 0.0;
                         u.dt           = in.dt;
                         // the gather writes cells only and the scatters ghost slots only, so
-                        // the first scatter shares the gather's stage; scatters in different
-                        // directions may hit the same ghost and get stages of their own
-                        std::vector<dev::Segment> sc;
-                        for (int d = 0; d < 2 * c.dim; ++d)
+                        // the first group of scatters shares the gather's stage; the
+                        // directions of a group hit disjoint ghosts
+                        Stage st{u};
+                        for (size_t q = 0; q < in.scatter_groups.size(); ++q)
                         {
-                            DevWorkList& w = in.wdir[size_t(d)];
-                            dev::Segment g = list_seg(dev::SEG_ADJ_SCATTER, c, w, 0, w.d.ncells);
-                            g.pad          = d;
-                            g.u            = lam;
-                            g.out          = mu;
-                            g.a0           = u.a0;
-                            g.a1           = u.a1;
-                            g.a2           = u.a2;
-                            g.dt           = in.dt;
-                            if (g.c1 > g.c0)
+                            if (q > 0)
                             {
-                                sc.push_back(g);
+                                ch.push_back(std::move(st));
+                                st.clear();
                             }
-                        }
-                        Stage first{u};
-                        if (!sc.empty())
-                        {
-                            first.push_back(sc[0]);
-                        }
-                        ch.push_back(std::move(first));
-                        for (size_t q = 1; q < sc.size(); ++q)
-                        {
-                            ch.push_back(Stage{sc[q]});
-                        }
-                        for (size_t k = in.dghost.size(); k-- > 0;)
-                        {
-                            const Instance::DevStage& ds = in.dghost[k];
-                            if (ds.nrows > 0)
+                            for (int d : in.scatter_groups[q])
                             {
-                                dev::Segment g = make_seg(dev::SEG_ADJ_GATHER, c);
-                                g.rdst         = ds.tgt;
-                                g.rsp          = ds.ptr;
-                                g.rsrc         = ds.src;
-                                g.thr          = ds.w;
+                                DevWorkList& w = in.wdir[size_t(d)];
+                                dev::Segment g = list_seg(dev::SEG_ADJ_SCATTER, c, w, 0, w.d.ncells);
+                                g.pad          = d;
+                                g.u            = lam;
                                 g.out          = mu;
-                                g.c1           = ds.nrows;
-                                push_stage(ch, g);
+                                g.a0           = u.a0;
+                                g.a1           = u.a1;
+                                g.a2           = u.a2;
+                                g.dt           = in.dt;
+                                st.push_back(g);
                             }
-                            dev::Segment zr = make_seg(dev::SEG_ADJ_ZERO, c);
-                            zr.rdst         = ds.dst;
-                            zr.out          = mu;
-                            zr.c1           = ds.ndst;
-                            push_stage(ch, zr);
+                        }
+                        ch.push_back(std::move(st));
+                        for (const auto& grp : in.adj_groups)
+                        {
+                            Stage st;
+                            for (const auto& [k, kind] : grp)
+                            {
+                                const Instance::DevStage& ds = in.dghost[size_t(k)];
+                                if (kind == 0)
+                                {
+                                    dev::Segment g = make_seg(dev::SEG_ADJ_GATHER, c);
+                                    g.rdst         = ds.tgt;
+                                    g.rsp          = ds.ptr;
+                                    g.rsrc         = ds.src;
+                                    g.thr          = ds.w;
+                                    g.out          = mu;
+                                    g.c1           = ds.nrows;
+                                    st.push_back(g);
+                                }
+                                else
+                                {
+                                    dev::Segment zr = make_seg(dev::SEG_ADJ_ZERO, c);
+                                    zr.rdst         = ds.dst;
+                                    zr.out          = mu;
+                                    zr.c1           = ds.ndst;
+                                    st.push_back(zr);
+                                }
+                            }
+                            ch.push_back(std::move(st));
                         }
                     }));
             }

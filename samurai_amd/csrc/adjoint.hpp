@@ -39,6 +39,12 @@ namespace smr
     // G of one mesh as linear stages, in the forward order of ghost_chain
     std::vector<LinearStage> ghost_linear_stages(const MRMeshData& m, const DevMesh& dm, bool has_bc);
 
+    // G^T as launches: the reverse stages (gather, then zeroing, of each
+    // forward stage from the last) in order, consecutive items that touch
+    // disjoint slots sharing a launch; item (k, 0) gathers stage k, (k, 1)
+    // zeroes its outputs
+    std::vector<std::vector<std::pair<int, int>>> group_reverse_stages(const std::vector<LinearStage>& ghost);
+
     class Adjoint
     {
       public:
@@ -85,6 +91,12 @@ namespace smr
                 int64_t ndst, nrows;
             };
             std::vector<DevStage> dghost;
+            // the reverse ghost stages grouped into launches: item (k, 0) is the
+            // gather of forward stage k, (k, 1) its zeroing; items of a group
+            // touch disjoint slots (no read-write or write-write overlap)
+            std::vector<std::vector<std::pair<int, int>>> adj_groups;
+            // scatter directions grouped into launches (disjoint ghost targets)
+            std::vector<std::vector<int>> scatter_groups;
             double* states = nullptr; // (every + 1) * nref
             double* ckpt   = nullptr; // nseg * nref
             double* lam    = nullptr; // 2 * nref
