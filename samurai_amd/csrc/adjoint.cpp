@@ -456,10 +456,20 @@ namespace smr
             in.states      = static_cast<double*>(dev_alloc(n * size_t(every_ + 1) * 8, st));
             in.ckpt        = static_cast<double*>(dev_alloc(n * size_t(nseg_) * 8, st));
             in.lam         = static_cast<double*>(dev_alloc(n * 2 * 8, st));
-            in.ga          = static_cast<double*>(dev_alloc(size_t(std::max<int64_t>(in.ncells, 1)) * size_t(dim) * 8, st));
             // defined contents everywhere (ghost slots of the states are read before
             // the first ghost update writes them only by the copies, never by operators)
             hip_check(hipMemsetAsync(in.states, 0, n * size_t(every_ + 1) * 8, st), "hipMemsetAsync(states)");
+        }
+        // the per-cell velocity-gradient terms of all instances, one array
+        ga_off_.assign(inst_.size() + 1, 0);
+        for (size_t i = 0; i < inst_.size(); ++i)
+        {
+            ga_off_[i + 1] = ga_off_[i] + inst_[i].ncells * inst_[i].mesh->host.cfg.dim;
+        }
+        ga_arena_ = static_cast<double*>(dev_alloc(size_t(std::max<int64_t>(ga_off_.back(), 1)) * 8, st));
+        for (size_t i = 0; i < inst_.size(); ++i)
+        {
+            inst_[i].ga = ga_arena_ + ga_off_[i];
         }
         ctx_->check_resolve();
     }
@@ -469,7 +479,7 @@ namespace smr
         hipStream_t s = ctx_->stream;
         for (Instance& in : inst_)
         {
-            for (double* p : {in.states, in.ckpt, in.lam, in.ga})
+            for (double* p : {in.states, in.ckpt, in.lam})
             {
                 if (p)
                 {
@@ -477,6 +487,7 @@ namespace smr
                 }
             }
         }
+        dev_free(ga_arena_, s);
         (void)hipStreamSynchronize(s);
     }
 
@@ -783,30 +794,35 @@ namespace smr
             }
         }
         grad_out_->run(tuning_);
-        // velocity gradient: per-cell terms summed on the host in cell order
-        for (size_t i = 0; i < inst_.size(); ++i)
+        // velocity gradient: per-cell terms of all instances in one copy, summed
+        // on the host per instance in cell order
+        const size_t total = size_t(ga_off_.back());
+        const double* h    = reinterpret_cast<const double*>(ga_host_.acquire(std::max<size_t>(total, 1) * 8));
+        if (total > 0)
         {
-            const Instance& in = inst_[i];
-            const int dim      = in.mesh->host.cfg.dim;
-            std::vector<double> h(size_t(in.ncells) * size_t(dim));
-            if (!h.empty())
-            {
-                hip_check(hipMemcpyAsync(h.data(), in.ga, h.size() * 8, hipMemcpyDeviceToHost, ctx_->stream), "hipMemcpyAsync(ga)");
-            }
-            ctx_->sync();
-            double acc[3] = {0, 0, 0};
-            for (int64_t c = 0; c < in.ncells; ++c)
-            {
-                for (int d = 0; d < dim; ++d)
-                {
-                    acc[d] += h[size_t(c * dim + d)];
-                }
-            }
-            for (int d = 0; d < 3; ++d)
-            {
-                ga[3 * i + size_t(d)] = d < dim ? acc[d] : 0.0;
-            }
+            hip_check(hipMemcpyAsync(const_cast<double*>(h), ga_arena_, total * 8, hipMemcpyDeviceToHost, ctx_->stream),
+                      "hipMemcpyAsync(ga)");
         }
         ctx_->sync();
+        parallel_for(int(inst_.size()),
+                     [&](int q)
+                     {
+                         const size_t i     = size_t(q);
+                         const Instance& in = inst_[i];
+                         const int dim      = in.mesh->host.cfg.dim;
+                         const double* hi   = h + ga_off_[i];
+                         double acc[3]      = {0, 0, 0};
+                         for (int64_t c = 0; c < in.ncells; ++c)
+                         {
+                             for (int d = 0; d < dim; ++d)
+                             {
+                                 acc[d] += hi[c * dim + d];
+                             }
+                         }
+                         for (int d = 0; d < 3; ++d)
+                         {
+                             ga[3 * i + size_t(d)] = d < dim ? acc[d] : 0.0;
+                         }
+                     });
     }
 } // namespace smr

@@ -116,6 +116,10 @@ namespace smr
         const double* bound_gT_ = nullptr;
         double* bound_g0_       = nullptr;
         std::vector<std::unique_ptr<ProgramBuf>> fwd_, save_, restore_, adj_[2];
+        // per-cell velocity-gradient terms of all instances (instance i at ga_off_[i])
+        double* ga_arena_ = nullptr;
+        std::vector<int64_t> ga_off_;
+        PinnedStaging ga_host_;
         std::unique_ptr<ProgramBuf> roll_, in_, out_, lam_in_, grad_out_;
 
         void check_meshes() const;
