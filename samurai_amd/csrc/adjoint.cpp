@@ -539,11 +539,11 @@ namespace smr
     void Adjoint::build_programs()
     {
         const size_t n = inst_.size();
-        fwd_.clear();
-        save_.clear();
-        restore_.clear();
-        adj_[0].clear();
-        adj_[1].clear();
+        fseg_.clear();
+        bseg_.clear();
+        // the chains of every piece (one forward step, one adjoint step of each
+        // parity, checkpoint copies), then one program per checkpoint segment
+        std::vector<std::vector<Chain>> fwd, adj[2], save, restore;
         auto per = [&](auto&& f)
         {
             std::vector<Chain> chains(n);
@@ -551,7 +551,7 @@ namespace smr
             {
                 f(inst_[i], chains[i]);
             }
-            return make_program(chains);
+            return chains;
         };
         auto copy_seg = [](const Instance& in, double* dst, const double* src)
         {
@@ -563,7 +563,7 @@ namespace smr
         };
         for (int j = 0; j < every_; ++j)
         {
-            fwd_.push_back(per(
+            fwd.push_back(per(
                 [&](Instance& in, Chain& ch)
                 {
                     DevMesh& dm         = *in.mesh->dev;
@@ -580,7 +580,7 @@ namespace smr
                 }));
             for (int p = 0; p < 2; ++p)
             {
-                adj_[p].push_back(per(
+                adj[p].push_back(per(
                     [&](Instance& in, Chain& ch)
                     {
                         DevMesh& dm         = *in.mesh->dev;
@@ -659,11 +659,41 @@ namespace smr
         }
         for (int sg = 0; sg < nseg_; ++sg)
         {
-            save_.push_back(per([&](Instance& in, Chain& ch) { push_stage(ch, copy_seg(in, in.ckpt + int64_t(sg) * in.nref, in.state(0))); }));
-            restore_.push_back(
+            save.push_back(per([&](Instance& in, Chain& ch) { push_stage(ch, copy_seg(in, in.ckpt + int64_t(sg) * in.nref, in.state(0))); }));
+            restore.push_back(
                 per([&](Instance& in, Chain& ch) { push_stage(ch, copy_seg(in, in.state(0), in.ckpt + int64_t(sg) * in.nref)); }));
         }
-        roll_           = per([&](Instance& in, Chain& ch) { push_stage(ch, copy_seg(in, in.state(0), in.state(every_))); });
+        const std::vector<Chain> roll = per([&](Instance& in, Chain& ch) { push_stage(ch, copy_seg(in, in.state(0), in.state(every_))); });
+        for (int sg = 0; sg < nseg_; ++sg)
+        {
+            // forward: checkpoint, `every` steps, the last state becomes the first
+            auto f = std::make_unique<ProgramBuf>(ctx_);
+            f->append(save[size_t(sg)]);
+            for (int j = 0; j < every_; ++j)
+            {
+                f->append(fwd[size_t(j)]);
+            }
+            f->append(roll);
+            f->upload();
+            fseg_.push_back(std::move(f));
+            // backward: restore, recompute the states, then the adjoint steps
+            // backwards; lam and mu swap every step, from the parity of the
+            // number of adjoint steps of the later segments
+            auto b = std::make_unique<ProgramBuf>(ctx_);
+            b->append(restore[size_t(sg)]);
+            for (int j = 0; j < every_; ++j)
+            {
+                b->append(fwd[size_t(j)]);
+            }
+            int done = (nseg_ - 1 - sg) * every_;
+            for (int j = every_ - 1; j >= 0; --j)
+            {
+                b->append(adj[done % 2][size_t(j)]);
+                ++done;
+            }
+            b->upload();
+            bseg_.push_back(std::move(b));
+        }
         programs_ready_ = true;
         bound_u0_       = nullptr;
         bound_uT_       = nullptr;
@@ -758,12 +788,7 @@ namespace smr
         in_->run(tuning_);
         for (int sg = 0; sg < nseg_; ++sg)
         {
-            save_[size_t(sg)]->run(tuning_);
-            for (int j = 0; j < every_; ++j)
-            {
-                fwd_[size_t(j)]->run(tuning_);
-            }
-            roll_->run(tuning_);
+            fseg_[size_t(sg)]->run(tuning_);
         }
         out_->run(tuning_);
         ctx_->sync();
@@ -779,19 +804,9 @@ namespace smr
         bind_io(nullptr, nullptr, gT, g0);
         ScopedTimer t(ctx_, "adjoint backward");
         lam_in_->run(tuning_);
-        int done = 0;
         for (int sg = nseg_ - 1; sg >= 0; --sg)
         {
-            restore_[size_t(sg)]->run(tuning_);
-            for (int j = 0; j < every_; ++j)
-            {
-                fwd_[size_t(j)]->run(tuning_);
-            }
-            for (int j = every_ - 1; j >= 0; --j)
-            {
-                adj_[done % 2][size_t(j)]->run(tuning_);
-                ++done;
-            }
+            bseg_[size_t(sg)]->run(tuning_);
         }
         grad_out_->run(tuning_);
         // velocity gradient: per-cell terms of all instances in one copy, summed

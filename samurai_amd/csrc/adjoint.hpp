@@ -115,12 +115,14 @@ namespace smr
         double* bound_uT_       = nullptr;
         const double* bound_gT_ = nullptr;
         double* bound_g0_       = nullptr;
-        std::vector<std::unique_ptr<ProgramBuf>> fwd_, save_, restore_, adj_[2];
+        // per checkpoint segment: the forward program (checkpoint, steps, roll)
+        // and the backward one (restore, recomputed steps, adjoint steps)
+        std::vector<std::unique_ptr<ProgramBuf>> fseg_, bseg_;
+        std::unique_ptr<ProgramBuf> in_, out_, lam_in_, grad_out_;
         // per-cell velocity-gradient terms of all instances (instance i at ga_off_[i])
         double* ga_arena_ = nullptr;
         std::vector<int64_t> ga_off_;
         PinnedStaging ga_host_;
-        std::unique_ptr<ProgramBuf> roll_, in_, out_, lam_in_, grad_out_;
 
         void check_meshes() const;
         void build_programs();

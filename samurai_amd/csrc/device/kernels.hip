@@ -221,20 +221,11 @@ namespace smr::dev
             }
         };
 
-        template <bool NT, int G, bool SHIFT, bool PIPE, bool NTL = false>
-        __global__ __launch_bounds__(BLOCK) void k_upwind_walk(List l, const double* __restrict__ u, double* __restrict__ unp1, double a0,
-                                                               double a1, double dt, unsigned nlogical)
+        // one column walk t (wave-uniform)
+        template <bool NT, int G, bool SHIFT, bool PIPE, bool NTL>
+        __device__ __forceinline__ void upwind_walk(const List& l, const double* __restrict__ u, double* __restrict__ unp1, double a0,
+                                                    double a1, double dt, int64_t t)
         {
-            const int64_t blk = xcd_tile(nlogical);
-            if (blk < 0)
-            {
-                return;
-            }
-            const int64_t t = blk * (BLOCK / 64) + (threadIdx.x >> 6);
-            if (t >= l.nwalks)
-            {
-                return;
-            }
             const int32_t* wk = l.walks + 4 * t;
             const int32_t r0  = __builtin_amdgcn_readfirstlane(wk[0]);
             const int32_t xs  = __builtin_amdgcn_readfirstlane(wk[1]);
@@ -320,6 +311,43 @@ namespace smr::dev
             }
         }
 
+        // one wave per walk
+        template <bool NT, int G, bool SHIFT, bool PIPE, bool NTL = false>
+        __global__ __launch_bounds__(BLOCK) void k_upwind_walk(List l, const double* __restrict__ u, double* __restrict__ unp1, double a0,
+                                                               double a1, double dt, unsigned nlogical)
+        {
+            const int64_t blk = xcd_tile(nlogical);
+            if (blk < 0)
+            {
+                return;
+            }
+            const int64_t t = blk * (BLOCK / 64) + (threadIdx.x >> 6);
+            if (t >= l.nwalks)
+            {
+                return;
+            }
+            upwind_walk<NT, G, SHIFT, PIPE, NTL>(l, u, unp1, a0, a1, dt, t);
+        }
+
+        // a grid sized to the resident waves: logical wave w (the blocks of an
+        // XCD numbered consecutively) runs the walks [w n / W, (w + 1) n / W)
+        // one after the other, so neighbouring strips stay in one wave and one
+        // L2, and no tail of short-lived waves is left
+        template <bool NT, int G, bool SHIFT, bool NTL>
+        __global__ __launch_bounds__(BLOCK) void k_upwind_walk_chunks(List l, const double* __restrict__ u, double* __restrict__ unp1,
+                                                                      double a0, double a1, double dt)
+        {
+            const unsigned per = gridDim.x >> 3;
+            const int64_t lb   = int64_t((blockIdx.x & 7u) * per + (blockIdx.x >> 3));
+            const int64_t w    = lb * (BLOCK / 64) + (threadIdx.x >> 6);
+            const int64_t nw   = int64_t(gridDim.x) * (BLOCK / 64);
+            const int64_t t0 = w * l.nwalks / nw, t1 = (w + 1) * l.nwalks / nw;
+            for (int64_t t = t0; t < t1; ++t)
+            {
+                upwind_walk<NT, G, SHIFT, false, NTL>(l, u, unp1, a0, a1, dt, t);
+            }
+        }
+
         __global__ void k_walk_rows(List l)
         {
             const int64_t w = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -388,6 +416,21 @@ namespace smr::dev
             return unsigned((n + BLOCK - 1) / BLOCK);
         }
 
+        int cu_count()
+        {
+            static int n = -1;
+            if (n < 0)
+            {
+                int dev = 0;
+                (void)hipGetDevice(&dev);
+                if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+                {
+                    n = 256;
+                }
+            }
+            return n;
+        }
+
         // launch shapes of the upwind kernel: cells per thread (0: wave tiles,
         // -2: column walks (2D), -1: column walks (2D) or wave tiles (3D) when at
         // least 3/4 of the lanes hold a cell, else 4 cells per thread), XCD
@@ -408,8 +451,10 @@ namespace smr::dev
             {12, -2, true, false},      {13, -2, true, true, 2},    {14, -2, true, true, 4},    {15, -2, true, true, 8},
             {16, -2, true, true, 1, true}, {17, -2, true, true, 4, true}, {18, -2, true, true, 8, true},
             {19, -2, true, true, 16, true}, {20, -2, true, false, 8, true}, {21, -2, true, true, 4, true, true},
-            {22, -2, true, true, 8, true, true}, {23, -2, true, true, 2, true, true}, {24, -2, true, true, 16, true}};
-        constexpr int kNumUpwindVariants    = 25;
+            {22, -2, true, true, 8, true, true}, {23, -2, true, true, 2, true, true}, {24, -2, true, true, 16, true},
+            {25, -2, true, true, 16, true},   {26, -2, true, true, 16, true},   {27, -2, true, true, 16, true},
+            {28, -2, true, true, 16, true}};
+        constexpr int kNumUpwindVariants    = 29;
         constexpr int kDefaultUpwindVariant = 10;
         int g_upwind_variant                = -1;
 
@@ -501,6 +546,28 @@ namespace smr::dev
                         hipLaunchKernelGGL((k_upwind_walk<true, 16, true, false, true>), dim3(ngrid), dim3(BLOCK), 0, s, l, u, unp1, a0, a1,
                                            dt, unsigned(nblk));
                         break;
+                    case 25:
+                    case 26:
+                    case 27:
+                    case 28:
+                    {
+                        // resident blocks per CU: 5 (VGPR-limited occupancy), 4, 8; 28: 5 without
+                        // non-temporal loads
+                        const int per_cu = v.id == 26 ? 4 : v.id == 27 ? 8 : 5;
+                        int64_t nb       = std::min<int64_t>(nblk, int64_t(cu_count()) * per_cu);
+                        nb               = std::max<int64_t>(8, (nb + 7) / 8 * 8);
+                        if (v.id == 28)
+                        {
+                            hipLaunchKernelGGL((k_upwind_walk_chunks<true, 16, true, false>), dim3(unsigned(nb)), dim3(BLOCK), 0, s, l, u, unp1,
+                                               a0, a1, dt);
+                        }
+                        else
+                        {
+                            hipLaunchKernelGGL((k_upwind_walk_chunks<true, 16, true, true>), dim3(unsigned(nb)), dim3(BLOCK), 0, s, l, u, unp1,
+                                               a0, a1, dt);
+                        }
+                        break;
+                    }
                     default: SMR_WALK_LAUNCH(true, 1, false, false); break;
                 }
 #undef SMR_WALK_LAUNCH
