@@ -564,7 +564,7 @@ def main():
                    "mean_leaves": updates / args.steps, "final_leaves": final_cells},
         "roofline": {"bound": "hbm", "achieved": uniform_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": uniform_gbs / HBM_PEAK_GBS, "traffic": MEASURED_UPWIND_TRAFFIC_BYTES,
-                     "kernel": "upwind (column-walk kernel k_upwind_walk: 16 rows fetched per wave, x neighbours by lane shifts, non-temporal stores) on the uniform level-12 mesh (16,777,216 leaves)",
+                     "kernel": "upwind (column-walk kernel k_upwind_walk: 16 rows fetched per wave, x neighbours by lane shifts, non-temporal loads and stores) on the uniform level-12 mesh (16,777,216 leaves)",
                      "bytes_per_launch": ubytes, "avg_launch_ms": u_ms,
                      "cell_updates_per_s": uniform_cups, "measured_copy_GBs": copy_gbs,
                      "frac_of_measured_copy": uniform_gbs / copy_gbs},

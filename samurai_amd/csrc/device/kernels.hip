@@ -329,25 +329,6 @@ namespace smr::dev
             upwind_walk<NT, G, SHIFT, PIPE, NTL>(l, u, unp1, a0, a1, dt, t);
         }
 
-        // a grid sized to the resident waves: logical wave w (the blocks of an
-        // XCD numbered consecutively) runs the walks [w n / W, (w + 1) n / W)
-        // one after the other, so neighbouring strips stay in one wave and one
-        // L2, and no tail of short-lived waves is left
-        template <bool NT, int G, bool SHIFT, bool NTL>
-        __global__ __launch_bounds__(BLOCK) void k_upwind_walk_chunks(List l, const double* __restrict__ u, double* __restrict__ unp1,
-                                                                      double a0, double a1, double dt)
-        {
-            const unsigned per = gridDim.x >> 3;
-            const int64_t lb   = int64_t((blockIdx.x & 7u) * per + (blockIdx.x >> 3));
-            const int64_t w    = lb * (BLOCK / 64) + (threadIdx.x >> 6);
-            const int64_t nw   = int64_t(gridDim.x) * (BLOCK / 64);
-            const int64_t t0 = w * l.nwalks / nw, t1 = (w + 1) * l.nwalks / nw;
-            for (int64_t t = t0; t < t1; ++t)
-            {
-                upwind_walk<NT, G, SHIFT, false, NTL>(l, u, unp1, a0, a1, dt, t);
-            }
-        }
-
         __global__ void k_walk_rows(List l)
         {
             const int64_t w = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -416,21 +397,6 @@ namespace smr::dev
             return unsigned((n + BLOCK - 1) / BLOCK);
         }
 
-        int cu_count()
-        {
-            static int n = -1;
-            if (n < 0)
-            {
-                int dev = 0;
-                (void)hipGetDevice(&dev);
-                if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
-                {
-                    n = 256;
-                }
-            }
-            return n;
-        }
-
         // launch shapes of the upwind kernel: cells per thread (0: wave tiles,
         // -2: column walks (2D), -1: column walks (2D) or wave tiles (3D) when at
         // least 3/4 of the lanes hold a cell, else 4 cells per thread), XCD
@@ -451,10 +417,8 @@ namespace smr::dev
             {12, -2, true, false},      {13, -2, true, true, 2},    {14, -2, true, true, 4},    {15, -2, true, true, 8},
             {16, -2, true, true, 1, true}, {17, -2, true, true, 4, true}, {18, -2, true, true, 8, true},
             {19, -2, true, true, 16, true}, {20, -2, true, false, 8, true}, {21, -2, true, true, 4, true, true},
-            {22, -2, true, true, 8, true, true}, {23, -2, true, true, 2, true, true}, {24, -2, true, true, 16, true},
-            {25, -2, true, true, 16, true},   {26, -2, true, true, 16, true},   {27, -2, true, true, 16, true},
-            {28, -2, true, true, 16, true}};
-        constexpr int kNumUpwindVariants    = 29;
+            {22, -2, true, true, 8, true, true}, {23, -2, true, true, 2, true, true}, {24, -2, true, true, 16, true}};
+        constexpr int kNumUpwindVariants    = 25;
         constexpr int kDefaultUpwindVariant = 10;
         int g_upwind_variant                = -1;
 
@@ -518,7 +482,7 @@ namespace smr::dev
         if (v.cpt == -1)
         {
             const bool dense = 4 * l.ncells >= 3 * 64 * l.ntiles;
-            v                = !dense ? kUpwindVariants[5] : (dim == 2 && l.nwalks > 0 ? kUpwindVariants[19] : kUpwindVariants[8]);
+            v                = !dense ? kUpwindVariants[5] : (dim == 2 && l.nwalks > 0 ? kUpwindVariants[24] : kUpwindVariants[8]);
         }
         if (v.cpt == -2)
         {
@@ -546,28 +510,6 @@ namespace smr::dev
                         hipLaunchKernelGGL((k_upwind_walk<true, 16, true, false, true>), dim3(ngrid), dim3(BLOCK), 0, s, l, u, unp1, a0, a1,
                                            dt, unsigned(nblk));
                         break;
-                    case 25:
-                    case 26:
-                    case 27:
-                    case 28:
-                    {
-                        // resident blocks per CU: 5 (VGPR-limited occupancy), 4, 8; 28: 5 without
-                        // non-temporal loads
-                        const int per_cu = v.id == 26 ? 4 : v.id == 27 ? 8 : 5;
-                        int64_t nb       = std::min<int64_t>(nblk, int64_t(cu_count()) * per_cu);
-                        nb               = std::max<int64_t>(8, (nb + 7) / 8 * 8);
-                        if (v.id == 28)
-                        {
-                            hipLaunchKernelGGL((k_upwind_walk_chunks<true, 16, true, false>), dim3(unsigned(nb)), dim3(BLOCK), 0, s, l, u, unp1,
-                                               a0, a1, dt);
-                        }
-                        else
-                        {
-                            hipLaunchKernelGGL((k_upwind_walk_chunks<true, 16, true, true>), dim3(unsigned(nb)), dim3(BLOCK), 0, s, l, u, unp1,
-                                               a0, a1, dt);
-                        }
-                        break;
-                    }
                     default: SMR_WALK_LAUNCH(true, 1, false, false); break;
                 }
 #undef SMR_WALK_LAUNCH
