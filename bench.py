@@ -35,10 +35,10 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the uniform level-12 upwind, from two rocprofv3 --pmc
 # runs of this program (FETCH_SIZE, WRITE_SIZE), each scaled by the counter's
 # reading for the copy kernel of known size measured in the same run
-# (profiles/upwind_walk_uniform_L12.md): FETCH 67,672 KiB x 2.0 + WRITE
-# 140,574 KiB x 1.0 for k_upwind_walk<true, 16, true, false>.  None when not
-# measured for the current kernel.
-MEASURED_UPWIND_TRAFFIC_BYTES = 282540544
+# (profiles/upwind_walk_uniform_L12.md): FETCH 68,596 KiB x 2.0 + WRITE
+# 141,262 KiB x 1.0 for k_upwind_walk<true, 16, true, false, true>.  None when
+# not measured for the current kernel.
+MEASURED_UPWIND_TRAFFIC_BYTES = 285136312
 
 
 def progress(msg):
