@@ -116,7 +116,33 @@ namespace smr::dev
             }
         }
 
-        // the cells of one stage, grid-strided; CPT consecutive cells per thread
+        // the last segment s of [sb, se) with pref[s] - base <= w (w uniform
+        // over the wave, all 64 lanes active): 64 prefixes per round trip,
+        // narrowed by a ballot, instead of one load per bisection step
+        __device__ __forceinline__ int32_t find_segment_wave(const int64_t* pref, int32_t sb, int32_t se, int64_t base, int64_t w)
+        {
+            const int lane = int(threadIdx.x & 63);
+            int32_t lo     = sb;
+            int32_t n      = se - sb; // pref[lo] - base <= w throughout
+            while (n > 64)
+            {
+                const int32_t step = (n + 63) >> 6;
+                const int32_t idx  = lo + lane * step;
+                const bool ok      = idx < lo + n && pref[idx] - base <= w;
+                const uint64_t m   = __ballot(ok);
+                const int32_t nlo  = lo + (63 - __clzll(m)) * step;
+                n                  = min(step, lo + n - nlo);
+                lo                 = nlo;
+            }
+            const bool ok    = lane < n && pref[lo + lane] - base <= w;
+            const uint64_t m = __ballot(ok);
+            return __builtin_amdgcn_readfirstlane(lo + (63 - __clzll(m)));
+        }
+
+        // the cells of one stage, grid-strided; CPT consecutive cells per
+        // thread.  The waves step through the stage together (the loop bound
+        // is the wave's first cell), so the segment search can use the whole
+        // wave; each lane then walks forward from the wave's first segment.
         __device__ __forceinline__ void run_stage(const Program& p, int k, int64_t first_thread, int64_t nthreads, int cpt)
         {
             const int32_t sb = p.stage_seg[k], se = p.stage_seg[k + 1];
@@ -126,23 +152,11 @@ namespace smr::dev
             }
             const int64_t base  = p.seg_pref[sb];
             const int64_t total = p.seg_pref[se] - base;
-            for (int64_t i0 = first_thread * cpt; i0 < total; i0 += nthreads * cpt)
+            const int lane      = int(threadIdx.x & 63);
+            for (int64_t w0 = (first_thread - lane) * cpt; w0 < total; w0 += nthreads * cpt)
             {
-                // segment of the first cell, then walk forward
-                int32_t lo = sb, hi = se - 1;
-                while (lo < hi)
-                {
-                    const int32_t mid = (lo + hi + 1) >> 1;
-                    if (p.seg_pref[mid] - base <= i0)
-                    {
-                        lo = mid;
-                    }
-                    else
-                    {
-                        hi = mid - 1;
-                    }
-                }
-                int32_t seg = lo;
+                int32_t seg      = find_segment_wave(p.seg_pref, sb, se, base, w0);
+                const int64_t i0 = w0 + int64_t(lane) * cpt;
                 for (int j = 0; j < cpt; ++j)
                 {
                     const int64_t i = i0 + j;
