@@ -311,17 +311,17 @@ namespace smr::dev
             }
         }
 
-        // one wave per walk
-        template <bool NT, int G, bool SHIFT, bool PIPE, bool NTL = false>
-        __global__ __launch_bounds__(BLOCK) void k_upwind_walk(List l, const double* __restrict__ u, double* __restrict__ unp1, double a0,
-                                                               double a1, double dt, unsigned nlogical)
+        // one wave per walk, WB waves per block
+        template <bool NT, int G, bool SHIFT, bool PIPE, bool NTL = false, int WB = BLOCK / 64>
+        __global__ __launch_bounds__(WB * 64) void k_upwind_walk(List l, const double* __restrict__ u, double* __restrict__ unp1, double a0,
+                                                                 double a1, double dt, unsigned nlogical)
         {
             const int64_t blk = xcd_tile(nlogical);
             if (blk < 0)
             {
                 return;
             }
-            const int64_t t = blk * (BLOCK / 64) + (threadIdx.x >> 6);
+            const int64_t t = blk * WB + (threadIdx.x >> 6);
             if (t >= l.nwalks)
             {
                 return;
@@ -417,8 +417,9 @@ namespace smr::dev
             {12, -2, true, false},      {13, -2, true, true, 2},    {14, -2, true, true, 4},    {15, -2, true, true, 8},
             {16, -2, true, true, 1, true}, {17, -2, true, true, 4, true}, {18, -2, true, true, 8, true},
             {19, -2, true, true, 16, true}, {20, -2, true, false, 8, true}, {21, -2, true, true, 4, true, true},
-            {22, -2, true, true, 8, true, true}, {23, -2, true, true, 2, true, true}, {24, -2, true, true, 16, true}};
-        constexpr int kNumUpwindVariants    = 25;
+            {22, -2, true, true, 8, true, true}, {23, -2, true, true, 2, true, true}, {24, -2, true, true, 16, true},
+            {25, -2, true, true, 16, true},   {26, -2, true, true, 16, true},   {27, -2, true, true, 16, true}};
+        constexpr int kNumUpwindVariants    = 28;
         constexpr int kDefaultUpwindVariant = 10;
         int g_upwind_variant                = -1;
 
@@ -510,6 +511,30 @@ namespace smr::dev
                         hipLaunchKernelGGL((k_upwind_walk<true, 16, true, false, true>), dim3(ngrid), dim3(BLOCK), 0, s, l, u, unp1, a0, a1,
                                            dt, unsigned(nblk));
                         break;
+                    case 25: // 1, 2 or 8 waves per block
+                    case 26:
+                    case 27:
+                    {
+                        const int wb        = v.id == 25 ? 1 : v.id == 26 ? 2 : 8;
+                        const int64_t nb    = (l.nwalks + wb - 1) / wb;
+                        const unsigned ng   = unsigned((nb + 7) / 8 * 8);
+                        if (wb == 1)
+                        {
+                            hipLaunchKernelGGL((k_upwind_walk<true, 16, true, false, true, 1>), dim3(ng), dim3(64), 0, s, l, u, unp1, a0, a1,
+                                               dt, unsigned(nb));
+                        }
+                        else if (wb == 2)
+                        {
+                            hipLaunchKernelGGL((k_upwind_walk<true, 16, true, false, true, 2>), dim3(ng), dim3(128), 0, s, l, u, unp1, a0, a1,
+                                               dt, unsigned(nb));
+                        }
+                        else
+                        {
+                            hipLaunchKernelGGL((k_upwind_walk<true, 16, true, false, true, 8>), dim3(ng), dim3(512), 0, s, l, u, unp1, a0, a1,
+                                               dt, unsigned(nb));
+                        }
+                        break;
+                    }
                     default: SMR_WALK_LAUNCH(true, 1, false, false); break;
                 }
 #undef SMR_WALK_LAUNCH
