@@ -486,7 +486,9 @@ namespace smr::dev
         const uint8_t t  = tag[bases(l, cell.run)[0] + cell.x];
         const bool ref   = (t & 4) && cell.level < max_level;
         const bool crs   = (t & 2) && !(t & 1) && cell.level > min_level;
-        if (ref || crs)
+        // one store per wave: the lowest active lane that saw a change
+        const uint64_t m = __ballot(ref || crs);
+        if (m != 0 && int(threadIdx.x & 63) == __ffsll(static_cast<unsigned long long>(m)) - 1)
         {
             flag[0] = 1;
         }

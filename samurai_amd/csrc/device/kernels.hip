@@ -311,22 +311,143 @@ namespace smr::dev
             }
         }
 
-        // one wave per walk, WB waves per block
-        template <bool NT, int G, bool SHIFT, bool PIPE, bool NTL = false, int WB = BLOCK / 64>
-        __global__ __launch_bounds__(WB * 64) void k_upwind_walk(List l, const double* __restrict__ u, double* __restrict__ unp1, double a0,
-                                                                 double a1, double dt, unsigned nlogical)
+        // one wave per walk
+        template <bool NT, int G, bool SHIFT, bool PIPE, bool NTL = false>
+        __global__ __launch_bounds__(BLOCK) void k_upwind_walk(List l, const double* __restrict__ u, double* __restrict__ unp1, double a0,
+                                                               double a1, double dt, unsigned nlogical)
         {
             const int64_t blk = xcd_tile(nlogical);
             if (blk < 0)
             {
                 return;
             }
-            const int64_t t = blk * WB + (threadIdx.x >> 6);
+            const int64_t t = blk * (BLOCK / 64) + (threadIdx.x >> 6);
             if (t >= l.nwalks)
             {
                 return;
             }
             upwind_walk<NT, G, SHIFT, PIPE, NTL>(l, u, unp1, a0, a1, dt, t);
+        }
+
+        __device__ __forceinline__ double readlane_f64(double v, int lane)
+        {
+            return __longlong_as_double(readlane64(__double_as_longlong(v), lane));
+        }
+
+        // Two adjacent strips of one band (walks 2p and 2p + 1: same runs, x
+        // ranges [xs, xs + 64) and [xs + 64, ...)) as one 128-column walk: lane
+        // k holds columns xs + k (half A) and xs + 64 + k (half B).  The
+        // neighbours across the halves' seam come from a readlane, so only the
+        // two ends of the 128 columns are loaded apart (one load, two lanes)
+        // and the strip edge between the halves costs nothing.  A walk with no
+        // adjacent partner runs as a single strip.  G rows ahead, non-temporal
+        // loads of the rows above and non-temporal stores, as k_upwind_walk.
+        template <int G>
+        __global__ __launch_bounds__(BLOCK) void k_upwind_walk_pair(List l, const double* __restrict__ u, double* __restrict__ unp1,
+                                                                    double a0, double a1, double dt, unsigned nlogical)
+        {
+            const int64_t blk = xcd_tile(nlogical);
+            if (blk < 0)
+            {
+                return;
+            }
+            const int64_t t = 2 * (blk * (BLOCK / 64) + (threadIdx.x >> 6));
+            if (t >= l.nwalks)
+            {
+                return;
+            }
+            const int32_t* wk = l.walks + 4 * t;
+            const int32_t r0  = __builtin_amdgcn_readfirstlane(wk[0]);
+            const int32_t xs  = __builtin_amdgcn_readfirstlane(wk[1]);
+            const int32_t n   = __builtin_amdgcn_readfirstlane(wk[2]);
+            const int32_t ro  = __builtin_amdgcn_readfirstlane(wk[3]);
+            bool pair         = false;
+            if (t + 1 < l.nwalks)
+            {
+                pair = __builtin_amdgcn_readfirstlane(wk[4]) == r0 && __builtin_amdgcn_readfirstlane(wk[5]) == xs + 64 &&
+                       __builtin_amdgcn_readfirstlane(wk[6]) == n;
+            }
+            if (!pair)
+            {
+                upwind_walk<true, 16, true, false, true>(l, u, unp1, a0, a1, dt, t);
+                if (t + 1 < l.nwalks)
+                {
+                    upwind_walk<true, 16, true, false, true>(l, u, unp1, a0, a1, dt, t + 1);
+                }
+                return;
+            }
+            const int lane    = int(threadIdx.x & 63);
+            const int64_t rb  = lane < n + 2 ? l.walk_rows[ro + lane] : 0;
+            const int32_t* iv = l.iv + 5 * int64_t(r0);
+            const int32_t xe  = __builtin_amdgcn_readfirstlane(iv[1]);
+            const int32_t xa = xs + lane, xb = xs + 64 + lane; // half A is whole: walk 2p + 1 exists
+            const bool act_b  = xb < xe;
+            const int last_b  = min(63, xe - 1 - xs - 64); // last active lane of half B
+            const int edge    = min(last_b + 1, 63);       // lane loading x + 1 of half B's last lane
+            const int level   = iv[4];
+            const double ha0 = .5 * a0, hb0 = .5 * fabs(a0), ha1 = .5 * a1, hb1 = .5 * fabs(a1);
+            const int64_t bd0 = readlane64(rb, 0), bc0 = readlane64(rb, 1);
+            double uda = u[bd0 + xa], uca = u[bc0 + xa];
+            double udb = 0, ucb = 0;
+            if (act_b)
+            {
+                udb = u[bd0 + xb];
+                ucb = u[bc0 + xb];
+            }
+            for (int32_t k = 0; k < n; k += G)
+            {
+                double ua[G], ub[G], ee[G];
+#pragma unroll
+                for (int g = 0; g < G; ++g)
+                {
+                    ua[g] = 0;
+                    ub[g] = 0;
+                    ee[g] = 0;
+                    if (k + g < n)
+                    {
+                        const int64_t bu = readlane64(rb, k + g + 2);
+                        const int64_t bc = readlane64(rb, k + g + 1);
+                        ua[g]            = __builtin_nontemporal_load(u + bu + xa);
+                        if (act_b)
+                        {
+                            ub[g] = __builtin_nontemporal_load(u + bu + xb);
+                        }
+                        // x - 1 of half A's lane 0, or x + 1 of half B's last lane
+                        if (lane == 0 || lane == edge)
+                        {
+                            ee[g] = u[bc + (lane == 0 ? xs - 1 : xs + 64 + last_b + 1)];
+                        }
+                    }
+                }
+#pragma unroll
+                for (int g = 0; g < G; ++g)
+                {
+                    if (k + g < n)
+                    {
+                        const int64_t bc  = readlane64(rb, k + g + 1);
+                        const double a63  = readlane_f64(uca, 63);
+                        const double b0   = readlane_f64(ucb, 0);
+                        const double sla  = wave_shift<0x138>(uca);
+                        const double sra  = wave_shift<0x130>(uca);
+                        const double slb  = wave_shift<0x138>(ucb);
+                        const double srb  = wave_shift<0x130>(lane == last_b + 1 ? ee[g] : ucb);
+                        const double ula  = lane == 0 ? ee[g] : sla;
+                        const double ura  = lane == 63 ? b0 : sra;
+                        const double ulb  = lane == 0 ? a63 : slb;
+                        const double urb  = lane == 63 ? ee[g] : srb;
+                        __builtin_nontemporal_store(upwind_vals_2d(uca, ula, ura, uda, ua[g], level, ha0, hb0, ha1, hb1, dt), unp1 + bc + xa);
+                        if (act_b)
+                        {
+                            __builtin_nontemporal_store(upwind_vals_2d(ucb, ulb, urb, udb, ub[g], level, ha0, hb0, ha1, hb1, dt),
+                                                        unp1 + bc + xb);
+                        }
+                        uda = uca;
+                        uca = ua[g];
+                        udb = ucb;
+                        ucb = ub[g];
+                    }
+                }
+            }
         }
 
         __global__ void k_walk_rows(List l)
@@ -511,27 +632,24 @@ namespace smr::dev
                         hipLaunchKernelGGL((k_upwind_walk<true, 16, true, false, true>), dim3(ngrid), dim3(BLOCK), 0, s, l, u, unp1, a0, a1,
                                            dt, unsigned(nblk));
                         break;
-                    case 25: // 1, 2 or 8 waves per block
+                    case 25: // adjacent strips in pairs, 8, 4 or 16 rows ahead
                     case 26:
                     case 27:
                     {
-                        const int wb        = v.id == 25 ? 1 : v.id == 26 ? 2 : 8;
-                        const int64_t nb    = (l.nwalks + wb - 1) / wb;
-                        const unsigned ng   = unsigned((nb + 7) / 8 * 8);
-                        if (wb == 1)
+                        const int64_t np  = (l.nwalks + 1) / 2;
+                        const int64_t nb  = (np + BLOCK / 64 - 1) / (BLOCK / 64);
+                        const unsigned ng = unsigned((nb + 7) / 8 * 8);
+                        if (v.id == 25)
                         {
-                            hipLaunchKernelGGL((k_upwind_walk<true, 16, true, false, true, 1>), dim3(ng), dim3(64), 0, s, l, u, unp1, a0, a1,
-                                               dt, unsigned(nb));
+                            hipLaunchKernelGGL((k_upwind_walk_pair<8>), dim3(ng), dim3(BLOCK), 0, s, l, u, unp1, a0, a1, dt, unsigned(nb));
                         }
-                        else if (wb == 2)
+                        else if (v.id == 26)
                         {
-                            hipLaunchKernelGGL((k_upwind_walk<true, 16, true, false, true, 2>), dim3(ng), dim3(128), 0, s, l, u, unp1, a0, a1,
-                                               dt, unsigned(nb));
+                            hipLaunchKernelGGL((k_upwind_walk_pair<4>), dim3(ng), dim3(BLOCK), 0, s, l, u, unp1, a0, a1, dt, unsigned(nb));
                         }
                         else
                         {
-                            hipLaunchKernelGGL((k_upwind_walk<true, 16, true, false, true, 8>), dim3(ng), dim3(512), 0, s, l, u, unp1, a0, a1,
-                                               dt, unsigned(nb));
+                            hipLaunchKernelGGL((k_upwind_walk_pair<16>), dim3(ng), dim3(BLOCK), 0, s, l, u, unp1, a0, a1, dt, unsigned(nb));
                         }
                         break;
                     }
