@@ -342,7 +342,7 @@ namespace smr::dev
         // and the strip edge between the halves costs nothing.  A walk with no
         // adjacent partner runs as a single strip.  G rows ahead, non-temporal
         // loads of the rows above and non-temporal stores, as k_upwind_walk.
-        template <int G>
+        template <int G, int GF = 16>
         __global__ __launch_bounds__(BLOCK) void k_upwind_walk_pair(List l, const double* __restrict__ u, double* __restrict__ unp1,
                                                                     double a0, double a1, double dt, unsigned nlogical)
         {
@@ -369,10 +369,10 @@ namespace smr::dev
             }
             if (!pair)
             {
-                upwind_walk<true, 16, true, false, true>(l, u, unp1, a0, a1, dt, t);
+                upwind_walk<true, GF, true, false, true>(l, u, unp1, a0, a1, dt, t);
                 if (t + 1 < l.nwalks)
                 {
-                    upwind_walk<true, 16, true, false, true>(l, u, unp1, a0, a1, dt, t + 1);
+                    upwind_walk<true, GF, true, false, true>(l, u, unp1, a0, a1, dt, t + 1);
                 }
                 return;
             }
@@ -539,8 +539,9 @@ namespace smr::dev
             {16, -2, true, true, 1, true}, {17, -2, true, true, 4, true}, {18, -2, true, true, 8, true},
             {19, -2, true, true, 16, true}, {20, -2, true, false, 8, true}, {21, -2, true, true, 4, true, true},
             {22, -2, true, true, 8, true, true}, {23, -2, true, true, 2, true, true}, {24, -2, true, true, 16, true},
-            {25, -2, true, true, 16, true},   {26, -2, true, true, 16, true},   {27, -2, true, true, 16, true}};
-        constexpr int kNumUpwindVariants    = 28;
+            {25, -2, true, true, 16, true},   {26, -2, true, true, 16, true},   {27, -2, true, true, 16, true},
+            {28, -2, true, true, 16, true},   {29, -2, true, true, 16, true}};
+        constexpr int kNumUpwindVariants    = 30;
         constexpr int kDefaultUpwindVariant = 10;
         int g_upwind_variant                = -1;
 
@@ -632,9 +633,11 @@ namespace smr::dev
                         hipLaunchKernelGGL((k_upwind_walk<true, 16, true, false, true>), dim3(ngrid), dim3(BLOCK), 0, s, l, u, unp1, a0, a1,
                                            dt, unsigned(nblk));
                         break;
-                    case 25: // adjacent strips in pairs, 8, 4 or 16 rows ahead
+                    case 25: // adjacent strips in pairs, 8, 4, 16, 16 (8 for single strips), 12 rows ahead
                     case 26:
                     case 27:
+                    case 28:
+                    case 29:
                     {
                         const int64_t np  = (l.nwalks + 1) / 2;
                         const int64_t nb  = (np + BLOCK / 64 - 1) / (BLOCK / 64);
@@ -647,9 +650,17 @@ namespace smr::dev
                         {
                             hipLaunchKernelGGL((k_upwind_walk_pair<4>), dim3(ng), dim3(BLOCK), 0, s, l, u, unp1, a0, a1, dt, unsigned(nb));
                         }
-                        else
+                        else if (v.id == 27)
                         {
                             hipLaunchKernelGGL((k_upwind_walk_pair<16>), dim3(ng), dim3(BLOCK), 0, s, l, u, unp1, a0, a1, dt, unsigned(nb));
+                        }
+                        else if (v.id == 28)
+                        {
+                            hipLaunchKernelGGL((k_upwind_walk_pair<16, 8>), dim3(ng), dim3(BLOCK), 0, s, l, u, unp1, a0, a1, dt, unsigned(nb));
+                        }
+                        else
+                        {
+                            hipLaunchKernelGGL((k_upwind_walk_pair<12>), dim3(ng), dim3(BLOCK), 0, s, l, u, unp1, a0, a1, dt, unsigned(nb));
                         }
                         break;
                     }
