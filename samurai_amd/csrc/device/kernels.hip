@@ -450,132 +450,6 @@ namespace smr::dev
             }
         }
 
-        // NH adjacent strips of one band (walks NH q .. NH q + NH - 1) as one
-        // walk of 64 NH columns: lane k holds columns xs + 64 h + k of half h.
-        // The seams between halves use readlane; the two outer ends are loaded
-        // by two lanes.  Groups that are not NH adjacent strips run strip by
-        // strip.  Rows G at a time ahead, non-temporal loads and stores.
-        template <int G, int NH, int GF>
-        __global__ __launch_bounds__(BLOCK) void k_upwind_walk_group(List l, const double* __restrict__ u, double* __restrict__ unp1,
-                                                                     double a0, double a1, double dt, unsigned nlogical)
-        {
-            const int64_t blk = xcd_tile(nlogical);
-            if (blk < 0)
-            {
-                return;
-            }
-            const int64_t t = NH * (blk * (BLOCK / 64) + (threadIdx.x >> 6));
-            if (t >= l.nwalks)
-            {
-                return;
-            }
-            const int32_t* wk = l.walks + 4 * t;
-            const int32_t r0  = __builtin_amdgcn_readfirstlane(wk[0]);
-            const int32_t xs  = __builtin_amdgcn_readfirstlane(wk[1]);
-            const int32_t n   = __builtin_amdgcn_readfirstlane(wk[2]);
-            const int32_t ro  = __builtin_amdgcn_readfirstlane(wk[3]);
-            bool group        = t + NH <= l.nwalks;
-            for (int h = 1; h < NH && group; ++h)
-            {
-                group = __builtin_amdgcn_readfirstlane(wk[4 * h]) == r0 && __builtin_amdgcn_readfirstlane(wk[4 * h + 1]) == xs + 64 * h &&
-                        __builtin_amdgcn_readfirstlane(wk[4 * h + 2]) == n;
-            }
-            if (!group)
-            {
-                for (int h = 0; h < NH && t + h < l.nwalks; ++h)
-                {
-                    upwind_walk<true, GF, true, false, true>(l, u, unp1, a0, a1, dt, t + h);
-                }
-                return;
-            }
-            const int lane    = int(threadIdx.x & 63);
-            const int64_t rb  = lane < n + 2 ? l.walk_rows[ro + lane] : 0;
-            const int32_t* iv = l.iv + 5 * int64_t(r0);
-            const int32_t xe  = __builtin_amdgcn_readfirstlane(iv[1]);
-            // halves 0 .. NH-2 are whole (the next strip exists); the last may be partial
-            const int32_t xl  = xs + 64 * (NH - 1) + lane;
-            const bool act_l  = xl < xe;
-            const int last_l  = min(63, xe - 1 - xs - 64 * (NH - 1));
-            const int edge    = min(last_l + 1, 63);
-            const int level   = iv[4];
-            const double ha0 = .5 * a0, hb0 = .5 * fabs(a0), ha1 = .5 * a1, hb1 = .5 * fabs(a1);
-            const int64_t bd0 = readlane64(rb, 0), bc0 = readlane64(rb, 1);
-            double ud[NH], uc[NH];
-#pragma unroll
-            for (int h = 0; h < NH; ++h)
-            {
-                ud[h] = 0;
-                uc[h] = 0;
-                if (h < NH - 1 || act_l)
-                {
-                    ud[h] = u[bd0 + xs + 64 * h + lane];
-                    uc[h] = u[bc0 + xs + 64 * h + lane];
-                }
-            }
-            for (int32_t k = 0; k < n; k += G)
-            {
-                double ua[G][NH], ee[G];
-#pragma unroll
-                for (int g = 0; g < G; ++g)
-                {
-                    ee[g] = 0;
-#pragma unroll
-                    for (int h = 0; h < NH; ++h)
-                    {
-                        ua[g][h] = 0;
-                    }
-                    if (k + g < n)
-                    {
-                        const int64_t bu = readlane64(rb, k + g + 2);
-                        const int64_t bc = readlane64(rb, k + g + 1);
-#pragma unroll
-                        for (int h = 0; h < NH; ++h)
-                        {
-                            if (h < NH - 1 || act_l)
-                            {
-                                ua[g][h] = __builtin_nontemporal_load(u + bu + xs + 64 * h + lane);
-                            }
-                        }
-                        // x - 1 of the first column, or x + 1 of the last active one
-                        if (lane == 0 || lane == edge)
-                        {
-                            ee[g] = u[bc + (lane == 0 ? xs - 1 : xs + 64 * (NH - 1) + last_l + 1)];
-                        }
-                    }
-                }
-#pragma unroll
-                for (int g = 0; g < G; ++g)
-                {
-                    if (k + g < n)
-                    {
-                        const int64_t bc = readlane64(rb, k + g + 1);
-                        double ul[NH], ur[NH];
-#pragma unroll
-                        for (int h = 0; h < NH; ++h)
-                        {
-                            const double sl = wave_shift<0x138>(uc[h]);
-                            const double sr = wave_shift<0x130>(h == NH - 1 && lane == last_l + 1 ? ee[g] : uc[h]);
-                            const double lo = h == 0 ? ee[g] : readlane_f64(uc[h > 0 ? h - 1 : 0], 63);
-                            const double hi = h == NH - 1 ? ee[g] : readlane_f64(uc[h < NH - 1 ? h + 1 : h], 0);
-                            ul[h]           = lane == 0 ? lo : sl;
-                            ur[h]           = lane == 63 ? hi : sr;
-                        }
-#pragma unroll
-                        for (int h = 0; h < NH; ++h)
-                        {
-                            if (h < NH - 1 || act_l)
-                            {
-                                __builtin_nontemporal_store(upwind_vals_2d(uc[h], ul[h], ur[h], ud[h], ua[g][h], level, ha0, hb0, ha1, hb1, dt),
-                                                            unp1 + bc + xs + 64 * h + lane);
-                            }
-                            ud[h] = uc[h];
-                            uc[h] = ua[g][h];
-                        }
-                    }
-                }
-            }
-        }
-
         __global__ void k_walk_rows(List l)
         {
             const int64_t w = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -666,9 +540,8 @@ namespace smr::dev
             {19, -2, true, true, 16, true}, {20, -2, true, false, 8, true}, {21, -2, true, true, 4, true, true},
             {22, -2, true, true, 8, true, true}, {23, -2, true, true, 2, true, true}, {24, -2, true, true, 16, true},
             {25, -2, true, true, 16, true},   {26, -2, true, true, 16, true},   {27, -2, true, true, 16, true},
-            {28, -2, true, true, 16, true},   {29, -2, true, true, 16, true},   {30, -2, true, true, 8, true},
-            {31, -2, true, true, 16, true},   {32, -2, true, true, 4, true}};
-        constexpr int kNumUpwindVariants    = 33;
+            {28, -2, true, true, 16, true},   {29, -2, true, true, 16, true}};
+        constexpr int kNumUpwindVariants    = 30;
         constexpr int kDefaultUpwindVariant = 10;
         int g_upwind_variant                = -1;
 
@@ -732,7 +605,7 @@ namespace smr::dev
         if (v.cpt == -1)
         {
             const bool dense = 4 * l.ncells >= 3 * 64 * l.ntiles;
-            v                = !dense ? kUpwindVariants[5] : (dim == 2 && l.nwalks > 0 ? kUpwindVariants[24] : kUpwindVariants[8]);
+            v                = !dense ? kUpwindVariants[5] : (dim == 2 && l.nwalks > 0 ? kUpwindVariants[28] : kUpwindVariants[8]);
         }
         if (v.cpt == -2)
         {
@@ -788,27 +661,6 @@ namespace smr::dev
                         else
                         {
                             hipLaunchKernelGGL((k_upwind_walk_pair<12>), dim3(ng), dim3(BLOCK), 0, s, l, u, unp1, a0, a1, dt, unsigned(nb));
-                        }
-                        break;
-                    }
-                    case 30: // groups of 4 adjacent strips (256 columns), 8 / 16 / 4 rows ahead
-                    case 31:
-                    case 32:
-                    {
-                        const int64_t nq  = (l.nwalks + 3) / 4;
-                        const int64_t nb  = (nq + BLOCK / 64 - 1) / (BLOCK / 64);
-                        const unsigned ng = unsigned((nb + 7) / 8 * 8);
-                        if (v.id == 30)
-                        {
-                            hipLaunchKernelGGL((k_upwind_walk_group<8, 4, 8>), dim3(ng), dim3(BLOCK), 0, s, l, u, unp1, a0, a1, dt, unsigned(nb));
-                        }
-                        else if (v.id == 31)
-                        {
-                            hipLaunchKernelGGL((k_upwind_walk_group<16, 4, 8>), dim3(ng), dim3(BLOCK), 0, s, l, u, unp1, a0, a1, dt, unsigned(nb));
-                        }
-                        else
-                        {
-                            hipLaunchKernelGGL((k_upwind_walk_group<4, 4, 8>), dim3(ng), dim3(BLOCK), 0, s, l, u, unp1, a0, a1, dt, unsigned(nb));
                         }
                         break;
                     }
