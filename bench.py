@@ -35,10 +35,10 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the uniform level-12 upwind, from two rocprofv3 --pmc
 # runs of this program (FETCH_SIZE, WRITE_SIZE), each scaled by the counter's
 # reading for the copy kernel of known size measured in the same run
-# (profiles/upwind_walk_uniform_L12.md): FETCH 68,596 KiB x 2.0 + WRITE
-# 141,262 KiB x 1.0 for k_upwind_walk<true, 16, true, false, true>.  None when
-# not measured for the current kernel.
-MEASURED_UPWIND_TRAFFIC_BYTES = None
+# (profiles/upwind_walk_uniform_L12.md): FETCH 68,191 KiB x 2.0 + WRITE
+# 140,538 KiB x 1.0 for k_upwind_walk_pair<16, 8>.  None when not measured for
+# the current kernel.
+MEASURED_UPWIND_TRAFFIC_BYTES = 283564787
 
 
 def progress(msg):
