@@ -212,6 +212,9 @@ namespace smr::dev
         int32_t r         = wk[0];
         const int32_t n   = wk[2];
         int64_t* rows     = l.walk_rows + wk[3];
+        const int32_t* iv = l.iv + 5 * int64_t(r);
+        rows[l.walk_stride - 2] = int64_t((uint64_t(uint32_t(wk[1])) << 32) | uint32_t(n));
+        rows[l.walk_stride - 1] = int64_t((uint64_t(uint32_t(iv[1])) << 32) | uint32_t(iv[4]));
         rows[0]           = l.base[int64_t(r) * l.ntaps + 1];
         for (int32_t k = 0; k < n; ++k)
         {

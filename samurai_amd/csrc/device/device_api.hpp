@@ -32,12 +32,15 @@ namespace smr::dev
         const int32_t* tiles; // (run, x0) of every 64-cell tile of a run, runs in order
         // column walks (2D cells list): (first run, x0, number of runs n, first
         // row slot) per walk; next[r] is the run right above run r with the same
-        // x range, or -1; walk_rows holds, per walk, the storage bases of the n + 2
-        // rows it touches (the row below, the n rows, the row above), filled on
-        // the device from the resolved bases
+        // x range, or -1; walk_rows holds, per walk, walk_stride slots: the
+        // storage bases of the n + 2 rows it touches (the row below, the n rows,
+        // the row above) from slot 0, then (x0 << 32 | n) and (x1 << 32 | level)
+        // in the last two slots, filled on the device from the resolved bases,
+        // so one load of a walk's slots gives a wave everything it needs
         const int32_t* walks;
         const int32_t* next;
         int64_t* walk_rows;
+        int32_t walk_stride;
         int64_t nruns;
         int64_t ncells;
         int64_t ntiles;

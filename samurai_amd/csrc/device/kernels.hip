@@ -226,19 +226,16 @@ namespace smr::dev
         __device__ __forceinline__ void upwind_walk(const List& l, const double* __restrict__ u, double* __restrict__ unp1, double a0,
                                                     double a1, double dt, int64_t t)
         {
-            const int32_t* wk = l.walks + 4 * t;
-            const int32_t r0  = __builtin_amdgcn_readfirstlane(wk[0]);
-            const int32_t xs  = __builtin_amdgcn_readfirstlane(wk[1]);
-            const int32_t n   = __builtin_amdgcn_readfirstlane(wk[2]);
-            const int32_t ro  = __builtin_amdgcn_readfirstlane(wk[3]);
+            // one load: the walk's row bases and, in its last two slots, its data
+            const int st      = l.walk_stride;
             const int lane    = int(threadIdx.x & 63);
-            const int64_t rb  = lane < n + 2 ? l.walk_rows[ro + lane] : 0;
-            const int32_t* iv = l.iv + 5 * int64_t(r0);
-            const int32_t xe  = __builtin_amdgcn_readfirstlane(iv[1]);
+            const int64_t rb  = lane < st ? l.walk_rows[t * st + lane] : 0;
+            const int64_t m0  = readlane64(rb, st - 2), m1 = readlane64(rb, st - 1);
+            const int32_t xs  = int32_t(m0 >> 32), n = int32_t(m0);
+            const int32_t xe  = int32_t(m1 >> 32), level = int32_t(m1);
             const int32_t x   = xs + lane;
             const bool active = x < xe; // every run of the walk has the same x range
             const int last    = min(63, xe - 1 - xs); // last active lane
-            const int level   = iv[4];
             const double ha0 = .5 * a0, hb0 = .5 * fabs(a0), ha1 = .5 * a1, hb1 = .5 * fabs(a1);
             const int64_t bd0 = readlane64(rb, 0), bc0 = readlane64(rb, 1);
             double ud = 0, uc = 0;
@@ -356,16 +353,23 @@ namespace smr::dev
             {
                 return;
             }
-            const int32_t* wk = l.walks + 4 * t;
-            const int32_t r0  = __builtin_amdgcn_readfirstlane(wk[0]);
-            const int32_t xs  = __builtin_amdgcn_readfirstlane(wk[1]);
-            const int32_t n   = __builtin_amdgcn_readfirstlane(wk[2]);
-            const int32_t ro  = __builtin_amdgcn_readfirstlane(wk[3]);
-            bool pair         = false;
-            if (t + 1 < l.nwalks)
+            // the slots of both walks in one load each (walk t + 1 only if it exists)
+            const int st      = l.walk_stride;
+            const int lane    = int(threadIdx.x & 63);
+            const bool has2   = t + 1 < l.nwalks;
+            const int64_t rb  = lane < st ? l.walk_rows[t * st + lane] : 0;
+            const int64_t rb2 = has2 && lane < st ? l.walk_rows[(t + 1) * st + lane] : 0;
+            const int64_t m0  = readlane64(rb, st - 2), m1 = readlane64(rb, st - 1);
+            const int32_t xs  = int32_t(m0 >> 32), n = int32_t(m0);
+            const int32_t xe  = int32_t(m1 >> 32), level = int32_t(m1);
+            // adjacent strips of one stack: walk t + 1 starts 64 columns on, has
+            // the same rows (same first row base, same x range end, same count)
+            bool pair = false;
+            if (has2)
             {
-                pair = __builtin_amdgcn_readfirstlane(wk[4]) == r0 && __builtin_amdgcn_readfirstlane(wk[5]) == xs + 64 &&
-                       __builtin_amdgcn_readfirstlane(wk[6]) == n;
+                const int64_t p0 = readlane64(rb2, st - 2), p1 = readlane64(rb2, st - 1);
+                pair = int32_t(p0 >> 32) == xs + 64 && int32_t(p0) == n && int32_t(p1 >> 32) == xe &&
+                       readlane64(rb2, 1) == readlane64(rb, 1);
             }
             if (!pair)
             {
@@ -376,15 +380,10 @@ namespace smr::dev
                 }
                 return;
             }
-            const int lane    = int(threadIdx.x & 63);
-            const int64_t rb  = lane < n + 2 ? l.walk_rows[ro + lane] : 0;
-            const int32_t* iv = l.iv + 5 * int64_t(r0);
-            const int32_t xe  = __builtin_amdgcn_readfirstlane(iv[1]);
             const int32_t xa = xs + lane, xb = xs + 64 + lane; // half A is whole: walk 2p + 1 exists
             const bool act_b  = xb < xe;
             const int last_b  = min(63, xe - 1 - xs - 64); // last active lane of half B
             const int edge    = min(last_b + 1, 63);       // lane loading x + 1 of half B's last lane
-            const int level   = iv[4];
             const double ha0 = .5 * a0, hb0 = .5 * fabs(a0), ha1 = .5 * a1, hb1 = .5 * fabs(a1);
             const int64_t bd0 = readlane64(rb, 0), bc0 = readlane64(rb, 1);
             double uda = u[bd0 + xa], uca = u[bc0 + xa];

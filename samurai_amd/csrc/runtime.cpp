@@ -458,16 +458,19 @@ namespace smr
         // 64-column strips of at most kWalkRows runs
         std::vector<int32_t> next, walks;
         int64_t nrows_total = 0;
+        int32_t walk_stride = 0;
         if (w.walk)
         {
-            // rows per walk: n + 2 row bases fit one wavefront (n <= 62); shorter
-            // walks give more wavefronts to hide latency with
+            // rows per walk: n + 2 row bases and 2 slots of walk data fit one
+            // wavefront (n <= 60); shorter walks give more wavefronts to hide
+            // latency with
             static const int kWalkRows = []
             {
                 const char* e = std::getenv("SMR_WALK_ROWS");
                 const int v   = e ? std::atoi(e) : 16;
-                return std::max(1, std::min(v, 62));
+                return std::max(1, std::min(v, 60));
             }();
+            walk_stride = kWalkRows + 4;
             next.assign(size_t(nruns), -1);
             std::vector<uint8_t> below(size_t(nruns), 0);
             // runs are sorted by level, then (z, y), then x: the run above r with
@@ -526,7 +529,7 @@ namespace smr
                     for (int32_t xs = x0; xs < x1; xs += 64)
                     {
                         walks.insert(walks.end(), {int32_t(first), xs, n, int32_t(nrows_total)});
-                        nrows_total += n + 2;
+                        nrows_total += walk_stride;
                     }
                 }
             }
@@ -555,6 +558,7 @@ namespace smr
                 l.d.next   = nwalks > 0 ? reinterpret_cast<const int32_t*>(b->dptr(onx)) : nullptr;
                 l.d.walks  = nwalks > 0 ? reinterpret_cast<const int32_t*>(b->dptr(owk)) : nullptr;
                 l.d.walk_rows = nwalks > 0 ? reinterpret_cast<int64_t*>(b->sptr(owr)) : nullptr;
+                l.d.walk_stride = walk_stride;
                 l.d.nwalks = nwalks;
                 l.d.base   = reinterpret_cast<int64_t*>(b->sptr(obas));
                 l.d.ntiles = ntiles;
