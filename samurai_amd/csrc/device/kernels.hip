@@ -339,8 +339,8 @@ namespace smr::dev
         // and the strip edge between the halves costs nothing.  A walk with no
         // adjacent partner runs as a single strip.  G rows ahead, non-temporal
         // loads of the rows above and non-temporal stores, as k_upwind_walk.
-        template <int G, int GF = 16>
-        __global__ __launch_bounds__(BLOCK) void k_upwind_walk_pair(List l, const double* __restrict__ u, double* __restrict__ unp1,
+        template <int G, int GF = 16, int MINW = 1>
+        __global__ __launch_bounds__(BLOCK, MINW) void k_upwind_walk_pair(List l, const double* __restrict__ u, double* __restrict__ unp1,
                                                                     double a0, double a1, double dt, unsigned nlogical)
         {
             const int64_t blk = xcd_tile(nlogical);
@@ -632,7 +632,7 @@ namespace smr::dev
                         hipLaunchKernelGGL((k_upwind_walk<true, 16, true, false, true>), dim3(ngrid), dim3(BLOCK), 0, s, l, u, unp1, a0, a1,
                                            dt, unsigned(nblk));
                         break;
-                    case 25: // adjacent strips in pairs, 8, 4, 16, 16 (8 for single strips), 12 rows ahead
+                    case 25: // adjacent strips in pairs, 8, 4, 16, 16 (8 for single strips), 16 (8) in 128 registers
                     case 26:
                     case 27:
                     case 28:
@@ -659,7 +659,9 @@ namespace smr::dev
                         }
                         else
                         {
-                            hipLaunchKernelGGL((k_upwind_walk_pair<12>), dim3(ng), dim3(BLOCK), 0, s, l, u, unp1, a0, a1, dt, unsigned(nb));
+                            // at most 128 registers: four waves per SIMD
+                            hipLaunchKernelGGL((k_upwind_walk_pair<16, 8, 4>), dim3(ng), dim3(BLOCK), 0, s, l, u, unp1, a0, a1, dt,
+                                               unsigned(nb));
                         }
                         break;
                     }
