@@ -339,8 +339,8 @@ namespace smr::dev
         // and the strip edge between the halves costs nothing.  A walk with no
         // adjacent partner runs as a single strip.  G rows ahead, non-temporal
         // loads of the rows above and non-temporal stores, as k_upwind_walk.
-        template <int G, int GF = 16, int MINW = 1>
-        __global__ __launch_bounds__(BLOCK, MINW) void k_upwind_walk_pair(List l, const double* __restrict__ u, double* __restrict__ unp1,
+        template <int G, int GF = 16, bool ALIGN = false>
+        __global__ __launch_bounds__(BLOCK) void k_upwind_walk_pair(List l, const double* __restrict__ u, double* __restrict__ unp1,
                                                                     double a0, double a1, double dt, unsigned nlogical)
         {
             const int64_t blk = xcd_tile(nlogical);
@@ -434,11 +434,39 @@ namespace smr::dev
                         const double ura  = lane == 63 ? b0 : sra;
                         const double ulb  = lane == 0 ? a63 : slb;
                         const double urb  = lane == 63 ? ee[g] : srb;
-                        __builtin_nontemporal_store(upwind_vals_2d(uca, ula, ura, uda, ua[g], level, ha0, hb0, ha1, hb1, dt), unp1 + bc + xa);
-                        if (act_b)
+                        const double va = upwind_vals_2d(uca, ula, ura, uda, ua[g], level, ha0, hb0, ha1, hb1, dt);
+                        const double vb = upwind_vals_2d(ucb, ulb, urb, udb, ub[g], level, ha0, hb0, ha1, hb1, dt);
+                        if constexpr (ALIGN)
                         {
-                            __builtin_nontemporal_store(upwind_vals_2d(ucb, ulb, urb, udb, ub[g], level, ha0, hb0, ha1, hb1, dt),
-                                                        unp1 + bc + xb);
+                            // stores that start on 64-byte sectors: the s columns before
+                            // the first sector boundary, then 64 columns from it, then
+                            // the rest; lane k takes column s + k (and 64 + s + k)
+                            const int64_t dst = bc + xs;
+                            const int sh      = int((8 - (dst & 7)) & 7);
+                            const int ncol    = xe - xs;
+                            const int src     = (lane + sh) & 63;
+                            const double pa   = __shfl(va, src, 64);
+                            const double pb   = __shfl(vb, src, 64);
+                            if (lane < sh)
+                            {
+                                __builtin_nontemporal_store(va, unp1 + dst + lane);
+                            }
+                            if (sh + lane < ncol)
+                            {
+                                __builtin_nontemporal_store(lane + sh < 64 ? pa : pb, unp1 + dst + sh + lane);
+                            }
+                            if (lane + sh < 64 && 64 + sh + lane < ncol)
+                            {
+                                __builtin_nontemporal_store(pb, unp1 + dst + 64 + sh + lane);
+                            }
+                        }
+                        else
+                        {
+                            __builtin_nontemporal_store(va, unp1 + bc + xa);
+                            if (act_b)
+                            {
+                                __builtin_nontemporal_store(vb, unp1 + bc + xb);
+                            }
                         }
                         uda = uca;
                         uca = ua[g];
@@ -632,7 +660,7 @@ namespace smr::dev
                         hipLaunchKernelGGL((k_upwind_walk<true, 16, true, false, true>), dim3(ngrid), dim3(BLOCK), 0, s, l, u, unp1, a0, a1,
                                            dt, unsigned(nblk));
                         break;
-                    case 25: // adjacent strips in pairs, 8, 4, 16, 16 (8 for single strips), 16 (8) in 128 registers
+                    case 25: // adjacent strips in pairs, 8, 4, 16, 16 (8 for single strips), 16 (8) with sector-aligned stores
                     case 26:
                     case 27:
                     case 28:
@@ -659,8 +687,7 @@ namespace smr::dev
                         }
                         else
                         {
-                            // at most 128 registers: four waves per SIMD
-                            hipLaunchKernelGGL((k_upwind_walk_pair<16, 8, 4>), dim3(ng), dim3(BLOCK), 0, s, l, u, unp1, a0, a1, dt,
+                            hipLaunchKernelGGL((k_upwind_walk_pair<16, 8, true>), dim3(ng), dim3(BLOCK), 0, s, l, u, unp1, a0, a1, dt,
                                                unsigned(nb));
                         }
                         break;
