@@ -339,7 +339,7 @@ namespace smr::dev
         // and the strip edge between the halves costs nothing.  A walk with no
         // adjacent partner runs as a single strip.  G rows ahead, non-temporal
         // loads of the rows above and non-temporal stores, as k_upwind_walk.
-        template <int G, int GF = 16, bool ALIGN = false>
+        template <int G, int GF = 16>
         __global__ __launch_bounds__(BLOCK) void k_upwind_walk_pair(List l, const double* __restrict__ u, double* __restrict__ unp1,
                                                                     double a0, double a1, double dt, unsigned nlogical)
         {
@@ -435,38 +435,11 @@ namespace smr::dev
                         const double ulb  = lane == 0 ? a63 : slb;
                         const double urb  = lane == 63 ? ee[g] : srb;
                         const double va = upwind_vals_2d(uca, ula, ura, uda, ua[g], level, ha0, hb0, ha1, hb1, dt);
-                        const double vb = upwind_vals_2d(ucb, ulb, urb, udb, ub[g], level, ha0, hb0, ha1, hb1, dt);
-                        if constexpr (ALIGN)
+                        const double vb = act_b ? upwind_vals_2d(ucb, ulb, urb, udb, ub[g], level, ha0, hb0, ha1, hb1, dt) : 0.0;
+                        __builtin_nontemporal_store(va, unp1 + bc + xa);
+                        if (act_b)
                         {
-                            // stores that start on 64-byte sectors: the s columns before
-                            // the first sector boundary, then 64 columns from it, then
-                            // the rest; lane k takes column s + k (and 64 + s + k)
-                            const int64_t dst = bc + xs;
-                            const int sh      = int((8 - (dst & 7)) & 7);
-                            const int ncol    = xe - xs;
-                            const int src     = (lane + sh) & 63;
-                            const double pa   = __shfl(va, src, 64);
-                            const double pb   = __shfl(vb, src, 64);
-                            if (lane < sh)
-                            {
-                                __builtin_nontemporal_store(va, unp1 + dst + lane);
-                            }
-                            if (sh + lane < ncol)
-                            {
-                                __builtin_nontemporal_store(lane + sh < 64 ? pa : pb, unp1 + dst + sh + lane);
-                            }
-                            if (lane + sh < 64 && 64 + sh + lane < ncol)
-                            {
-                                __builtin_nontemporal_store(pb, unp1 + dst + 64 + sh + lane);
-                            }
-                        }
-                        else
-                        {
-                            __builtin_nontemporal_store(va, unp1 + bc + xa);
-                            if (act_b)
-                            {
-                                __builtin_nontemporal_store(vb, unp1 + bc + xb);
-                            }
+                            __builtin_nontemporal_store(vb, unp1 + bc + xb);
                         }
                         uda = uca;
                         uca = ua[g];
@@ -567,8 +540,8 @@ namespace smr::dev
             {19, -2, true, true, 16, true}, {20, -2, true, false, 8, true}, {21, -2, true, true, 4, true, true},
             {22, -2, true, true, 8, true, true}, {23, -2, true, true, 2, true, true}, {24, -2, true, true, 16, true},
             {25, -2, true, true, 16, true},   {26, -2, true, true, 16, true},   {27, -2, true, true, 16, true},
-            {28, -2, true, true, 16, true},   {29, -2, true, true, 16, true}};
-        constexpr int kNumUpwindVariants    = 30;
+            {28, -2, true, true, 16, true}};
+        constexpr int kNumUpwindVariants    = 29;
         constexpr int kDefaultUpwindVariant = 10;
         int g_upwind_variant                = -1;
 
@@ -660,11 +633,10 @@ namespace smr::dev
                         hipLaunchKernelGGL((k_upwind_walk<true, 16, true, false, true>), dim3(ngrid), dim3(BLOCK), 0, s, l, u, unp1, a0, a1,
                                            dt, unsigned(nblk));
                         break;
-                    case 25: // adjacent strips in pairs, 8, 4, 16, 16 (8 for single strips), 16 (8) with sector-aligned stores
+                    case 25: // adjacent strips in pairs, 8, 4, 16, 16 (8 for single strips) rows ahead
                     case 26:
                     case 27:
                     case 28:
-                    case 29:
                     {
                         const int64_t np  = (l.nwalks + 1) / 2;
                         const int64_t nb  = (np + BLOCK / 64 - 1) / (BLOCK / 64);
@@ -681,14 +653,9 @@ namespace smr::dev
                         {
                             hipLaunchKernelGGL((k_upwind_walk_pair<16>), dim3(ng), dim3(BLOCK), 0, s, l, u, unp1, a0, a1, dt, unsigned(nb));
                         }
-                        else if (v.id == 28)
-                        {
-                            hipLaunchKernelGGL((k_upwind_walk_pair<16, 8>), dim3(ng), dim3(BLOCK), 0, s, l, u, unp1, a0, a1, dt, unsigned(nb));
-                        }
                         else
                         {
-                            hipLaunchKernelGGL((k_upwind_walk_pair<16, 8, true>), dim3(ng), dim3(BLOCK), 0, s, l, u, unp1, a0, a1, dt,
-                                               unsigned(nb));
+                            hipLaunchKernelGGL((k_upwind_walk_pair<16, 8>), dim3(ng), dim3(BLOCK), 0, s, l, u, unp1, a0, a1, dt, unsigned(nb));
                         }
                         break;
                     }
