@@ -38,7 +38,7 @@ HBM_PEAK_GBS = 8000.0
 # (profiles/upwind_walk_uniform_L12.md): FETCH 68,191 KiB x 2.0 + WRITE
 # 140,538 KiB x 1.0 for k_upwind_walk_pair<16, 8>.  None when not measured for
 # the current kernel.
-MEASURED_UPWIND_TRAFFIC_BYTES = 283564787
+MEASURED_UPWIND_TRAFFIC_BYTES = None
 
 
 def progress(msg):
@@ -564,7 +564,7 @@ def main():
                    "mean_leaves": updates / args.steps, "final_leaves": final_cells},
         "roofline": {"bound": "hbm", "achieved": uniform_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": uniform_gbs / HBM_PEAK_GBS, "traffic": MEASURED_UPWIND_TRAFFIC_BYTES,
-                     "kernel": "upwind (column-walk kernel k_upwind_walk_pair: two adjacent 64-column strips per wave, 16 rows fetched ahead, x neighbours by lane shifts, non-temporal loads and stores) on the uniform level-12 mesh (16,777,216 leaves)",
+                     "kernel": "upwind (column-walk kernel k_upwind_walk_pair: two adjacent 64-column strips per wave, walks of 10 rows all fetched ahead, x neighbours by lane shifts, non-temporal loads and stores) on the uniform level-12 mesh (16,777,216 leaves)",
                      "bytes_per_launch": ubytes, "avg_launch_ms": u_ms,
                      "cell_updates_per_s": uniform_cups, "measured_copy_GBs": copy_gbs,
                      "frac_of_measured_copy": uniform_gbs / copy_gbs},

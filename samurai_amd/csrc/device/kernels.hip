@@ -156,7 +156,7 @@ namespace smr::dev
         }
 
         // Upwind (2D) by column walks: a wavefront owns 64 columns of a stack of
-        // up to 62 runs with one x range at rows y, y+1, ...  Lane k holds the
+        // up to 60 runs with one x range at rows y, y+1, ...  Lane k holds the
         // storage base of row k of the walk (the row below, the runs' rows, the
         // row above), read by every lane with readlane, so the walk carries no
         // dependent loads; walking up, the row above becomes the centre row and
@@ -540,8 +540,8 @@ namespace smr::dev
             {19, -2, true, true, 16, true}, {20, -2, true, false, 8, true}, {21, -2, true, true, 4, true, true},
             {22, -2, true, true, 8, true, true}, {23, -2, true, true, 2, true, true}, {24, -2, true, true, 16, true},
             {25, -2, true, true, 16, true},   {26, -2, true, true, 16, true},   {27, -2, true, true, 16, true},
-            {28, -2, true, true, 16, true},   {29, -2, true, true, 16, true},   {30, -2, true, true, 16, true}};
-        constexpr int kNumUpwindVariants    = 31;
+            {28, -2, true, true, 16, true}};
+        constexpr int kNumUpwindVariants    = 29;
         constexpr int kDefaultUpwindVariant = 10;
         int g_upwind_variant                = -1;
 
@@ -637,8 +637,6 @@ namespace smr::dev
                     case 26:
                     case 27:
                     case 28:
-                    case 29: // 28 with 64 KB / 40 KB of LDS per block: at most 2 / 4 blocks per CU
-                    case 30:
                     {
                         const int64_t np  = (l.nwalks + 1) / 2;
                         const int64_t nb  = (np + BLOCK / 64 - 1) / (BLOCK / 64);
@@ -657,9 +655,7 @@ namespace smr::dev
                         }
                         else
                         {
-                            const size_t lds = v.id == 29 ? 65536 : v.id == 30 ? 40960 : 0;
-                            hipLaunchKernelGGL((k_upwind_walk_pair<16, 8>), dim3(ng), dim3(BLOCK), lds, s, l, u, unp1, a0, a1, dt,
-                                               unsigned(nb));
+                            hipLaunchKernelGGL((k_upwind_walk_pair<16, 8>), dim3(ng), dim3(BLOCK), 0, s, l, u, unp1, a0, a1, dt, unsigned(nb));
                         }
                         break;
                     }
