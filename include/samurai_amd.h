@@ -76,6 +76,8 @@ int smr_mesh_create_from_cells(smr_ctx* ctx, const smr_mesh_config* cfg, const i
 int smr_mesh_destroy(smr_mesh* mesh);
 /* mesh.nb_cells(mesh_id) (mesh.hpp:540-551); SMR_REFERENCE gives the storage size */
 int64_t smr_mesh_nb_cells(const smr_mesh* mesh, int mesh_id);
+/* mesh.cfg() (mesh.hpp:162-163, 642-652): the configuration the mesh was built with */
+int smr_mesh_get_config(const smr_mesh* mesh, smr_mesh_config* out);
 /* for_each_cell(mesh[mesh_id]) (algorithm.hpp:319-323): n rows of
  * (level, x, y, z, storage index) in traversal order */
 int smr_mesh_get_cells(const smr_mesh* mesh, int mesh_id, int64_t* out);
@@ -92,6 +94,10 @@ int smr_field_create(smr_mesh* mesh, const char* name, smr_field** out);
 int smr_field_destroy(smr_field* field);
 /* field.resize() after a mesh change (field/access_base.hpp:105-119) */
 int smr_field_resize(smr_field* field);
+/* field.name() (field/field_base.hpp:74-76), or NULL for a null field */
+const char* smr_field_name(const smr_field* field);
+/* 1 if field lives on mesh, 0 if not (&field.mesh() == &mesh) */
+int smr_field_on_mesh(const smr_field* field, const smr_mesh* mesh);
 /* values on mesh[cells] in traversal order (like for_each_cell + u[cell]) */
 int smr_field_set_cells(smr_field* field, const double* values);
 int smr_field_get_cells(smr_field* field, double* values);

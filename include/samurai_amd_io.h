@@ -1,0 +1,59 @@
+/* Restart files of samurai's layout (io/restart.hpp), written and read with
+ * the HDF5 C library: a separate shared library (libsamurai_amd_io.so) on top
+ * of the C ABI in samurai_amd.h, so the compute library does not depend on
+ * HDF5.
+ *
+ * File layout (the reference's, io/restart.hpp:52-187, single process):
+ *   /n_process                 size_t (1)
+ *   /mesh/dim                  size_t
+ *   /mesh/min_level, max_level size_t (the mesh's configured levels)
+ *   /mesh/origin_point         double[dim] (0)
+ *   /mesh/scaling_factor       double (1)
+ *   /mesh/level/{l}/dim/{d}/intervals/{partition,data}
+ *        the LevelCellArray of the leaves of level l, dimension d: intervals
+ *        {start, end, step, index} (int32 x3 + int64, 24 bytes);
+ *        dimension 0 index = storage offset of the run (value = index + x)
+ *   /mesh/level/{l}/dim/{d}/offsets/{partition,data}   d >= 1, size_t
+ *   /fields/{name}/n_comp      size_t (1)
+ *   /fields/{name}/data/{partition,data}  double, for_each_cell order
+ * Levels without leaves and empty arrays are not written (io/restart.hpp:82-85).
+ *
+ * Return values and errors follow samurai_amd.h; the message of a failure is
+ * available from smr_io_last_error().
+ */
+#ifndef SAMURAI_AMD_IO_H
+#define SAMURAI_AMD_IO_H
+
+#include "samurai_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* samurai::dump(path, filename, mesh, fields...) (io/restart.hpp:189-200):
+ * `file` is the full file name (the reference appends ".h5" to filename) */
+int smr_restart_dump(const char* file, const smr_mesh* mesh, smr_field* const* fields, int nfields);
+
+/* samurai::load(path, filename, mesh) (io/restart.hpp:423-462): a new mesh on
+ * ctx from the file's leaves, with `cfg`'s stencil and graduation settings and
+ * the file's min and max levels */
+int smr_restart_load_mesh(const char* file, smr_ctx* ctx, const smr_mesh_config* cfg, smr_mesh** out);
+
+/* load_field(file, mesh, field) (io/restart.hpp:341-386): the values of
+ * /fields/{field name}, for_each_cell order, into a field living on `mesh`
+ * (loaded from the same file) */
+int smr_restart_load_field(const char* file, const smr_mesh* mesh, smr_field* field);
+
+/* samurai::dump / load of a CellArray alone (io/restart.hpp:126-150, 313-337):
+ * n leaves given as rows (level, x, y, z) of int32; load returns the leaf
+ * count, writing at most `capacity` rows (call with capacity 0 to size) */
+int smr_restart_dump_cells(const char* file, int dim, const int32_t* cells, int64_t n);
+int64_t smr_restart_load_cells(const char* file, int* dim, int32_t* cells, int64_t capacity);
+
+const char* smr_io_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -16,6 +16,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsamurai_amd.so")
+IO_LIB_PATH = os.path.join(HERE, "libsamurai_amd_io.so")
 
 CELLS, CELLS_AND_GHOSTS, PROJ_CELLS, UNION_CELLS, REFERENCE = range(5)
 KEEP, COARSEN, REFINE = 1, 2, 4
@@ -61,6 +62,10 @@ def lib():
         _sig(L, "smr_mesh_nb_cells", "restype", i64)
         _sig(L, "smr_mesh_nb_cells", "argtypes", [vp, i32])
         _sig(L, "smr_mesh_get_cells", "argtypes", [vp, i32, P(i64)])
+        _sig(L, "smr_mesh_get_config", "argtypes", [vp, P(MeshConfig)])
+        _sig(L, "smr_field_name", "restype", ctypes.c_char_p)
+        _sig(L, "smr_field_name", "argtypes", [vp])
+        _sig(L, "smr_field_on_mesh", "argtypes", [vp, vp])
         _sig(L, "smr_mesh_rebuild_from_tags", "argtypes", [vp, P(ctypes.c_uint8), i64])
         _sig(L, "smr_mesh_validate", "argtypes", [vp])
         _sig(L, "smr_ctx_stream", "restype", vp)
@@ -512,3 +517,94 @@ class AdjointProblem:
         ga = np.zeros((max(self.n, 1), 3))
         _check(lib().smr_adjoint_backward(self.h, ctypes.c_void_p(gT_ptr), ctypes.c_void_p(g0_ptr), _dp(ga)))
         return ga[:self.n]
+
+
+# ---------------------------------------------------------------------------
+# restart files (io/restart.hpp layout), libsamurai_amd_io.so
+# ---------------------------------------------------------------------------
+_io = None
+
+
+def io_lib():
+    """The restart-file library (HDF5); loaded on first use."""
+    global _io
+    if _io is None:
+        lib()  # the compute library first: the I/O library is linked against it
+        if not os.path.exists(IO_LIB_PATH):
+            raise SamuraiError(f"{IO_LIB_PATH} is missing: build it with python samurai_amd/build.py (needs HDF5)")
+        L = ctypes.CDLL(IO_LIB_PATH)
+        vp, i32, i64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64
+        L.smr_io_last_error.restype = ctypes.c_char_p
+        L.smr_restart_dump.argtypes = [ctypes.c_char_p, vp, ctypes.POINTER(vp), i32]
+        L.smr_restart_load_mesh.argtypes = [ctypes.c_char_p, vp, ctypes.POINTER(MeshConfig), ctypes.POINTER(vp)]
+        L.smr_restart_load_field.argtypes = [ctypes.c_char_p, vp, vp]
+        L.smr_restart_dump_cells.argtypes = [ctypes.c_char_p, i32, ctypes.POINTER(ctypes.c_int32), i64]
+        L.smr_restart_load_cells.restype = i64
+        L.smr_restart_load_cells.argtypes = [ctypes.c_char_p, ctypes.POINTER(i32), ctypes.POINTER(ctypes.c_int32), i64]
+        _io = L
+    return _io
+
+
+def _io_check(rc):
+    if rc < 0:
+        raise SamuraiError(f"samurai_amd restart error {rc}: {io_lib().smr_io_last_error().decode()}")
+    return rc
+
+
+def _h5(filename):
+    # samurai::dump / load take the name without extension and add ".h5"
+    return (os.fspath(filename) + ".h5").encode()
+
+
+def dump(filename, mesh, *fields):
+    """samurai::dump(filename, mesh, fields...) (io/restart.hpp:189-237): the
+    leaves of `mesh` and the named scalar fields into filename + ".h5"."""
+    arr = (ctypes.c_void_p * max(len(fields), 1))(*[f.h for f in fields])
+    _io_check(io_lib().smr_restart_dump(_h5(filename), mesh.h, arr, len(fields)))
+
+
+def load_mesh(filename, ctx, cfg):
+    """The mesh part of samurai::load(filename, mesh) (io/restart.hpp:423-462):
+    a new MRMesh from the file's leaves, with cfg's stencil settings and the
+    file's min and max levels."""
+    h = ctypes.c_void_p()
+    _io_check(io_lib().smr_restart_load_mesh(_h5(filename), ctx.h, ctypes.byref(cfg), ctypes.byref(h)))
+    out = MeshConfig()
+    _check(lib().smr_mesh_get_config(h, ctypes.byref(out)))
+    return MRMesh(ctx, h, out)
+
+
+def load_field(filename, mesh, field):
+    """load_field(file, mesh, field) (io/restart.hpp:341-386): the values of
+    /fields/{field.name} into `field`, which lives on `mesh`."""
+    _io_check(io_lib().smr_restart_load_field(_h5(filename), mesh.h, field.h))
+
+
+def load(filename, ctx, cfg, *names):
+    """samurai::load(filename, mesh, fields...): the mesh, then one scalar field
+    per name; returns (mesh, [fields])."""
+    mesh = load_mesh(filename, ctx, cfg)
+    fields = []
+    for name in names:
+        f = make_scalar_field(name, mesh)
+        load_field(filename, mesh, f)
+        fields.append(f)
+    return mesh, fields
+
+
+def dump_cells(filename, dim, cells):
+    """samurai::dump(filename, CellArray) (io/restart.hpp:126-150): leaves as
+    (n, >=4) rows (level, x, y, z) in for_each_cell order."""
+    c = np.ascontiguousarray(np.asarray(cells).reshape(-1, np.asarray(cells).shape[-1])[:, :4], dtype=np.int32)
+    _io_check(io_lib().smr_restart_dump_cells(_h5(filename), int(dim), c.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                              c.shape[0]))
+
+
+def load_cells(filename):
+    """samurai::load(filename, CellArray): (dim, (n, 4) int32 rows (level, x, y, z))."""
+    dim = ctypes.c_int()
+    n = _io_check(io_lib().smr_restart_load_cells(_h5(filename), ctypes.byref(dim), None, 0))
+    out = np.zeros((n, 4), dtype=np.int32)
+    _io_check(io_lib().smr_restart_load_cells(_h5(filename), ctypes.byref(dim), out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), n))
+    return dim.value, out
+

@@ -57,6 +57,7 @@ def build(verbose=False):
         sys.stderr.write(r.stdout.decode(errors="replace"))
         raise RuntimeError("samurai_amd: link failed")
     os.replace(LIB + ".tmp", LIB)
+    build_io()
     # the C++ drop-in demo, built on include/samurai_amd/samurai.hpp
     demo = os.path.join(HERE, "demos", "advection_2d")
     cmd = ["g++", "-O2", "-std=c++17", "-Wall", f"-I{os.path.join(ROOT, 'include')}", os.path.join(HERE, "demos", "advection_2d.cpp"),
@@ -67,6 +68,33 @@ def build(verbose=False):
         raise RuntimeError("samurai_amd: demo build failed")
     os.replace(demo + ".tmp", demo)
     return LIB
+
+
+HDF5_ROOT = os.environ.get("SMR_HDF5_ROOT", "/opt/conda")
+IO_LIB = os.path.join(HERE, "libsamurai_amd_io.so")
+
+
+def build_io():
+    """libsamurai_amd_io.so: restart files (HDF5) over the C ABI; skipped when
+    the HDF5 headers and library are absent (the compute library does not
+    need it)."""
+    inc, lib = os.path.join(HDF5_ROOT, "include"), os.path.join(HDF5_ROOT, "lib")
+    if not (os.path.exists(os.path.join(inc, "hdf5.h")) and os.path.exists(os.path.join(lib, "libhdf5.so"))):
+        sys.stderr.write(f"samurai_amd: HDF5 not found under {HDF5_ROOT}; restart I/O not built\n")
+        return None
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wextra", "-Wno-unused-parameter",
+           f"-I{os.path.join(ROOT, 'include')}", f"-I{inc}", os.path.join(CSRC, "io", "restart.cpp"), "-o", IO_LIB + ".tmp",
+           f"-L{HERE}", "-lsamurai_amd", f"-L{lib}", "-lhdf5",
+           # the HDF5 directory holds an older libstdc++: this library carries its
+           # own (static) and finds HDF5 through a RUNPATH, which serves its own
+           # dependencies only, so libsamurai_amd and HIP keep the system one
+           "-static-libstdc++", "-static-libgcc", "-Wl,--enable-new-dtags", "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{lib}"]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout.decode(errors="replace"))
+        raise RuntimeError("samurai_amd: restart I/O build failed")
+    os.replace(IO_LIB + ".tmp", IO_LIB)
+    return IO_LIB
 
 
 if __name__ == "__main__":

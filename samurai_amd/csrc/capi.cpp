@@ -257,6 +257,23 @@ extern "C"
         return mesh->m.host.nb_cells(mesh_id);
     }
 
+    int smr_mesh_get_config(const smr_mesh* mesh, smr_mesh_config* out)
+    {
+        return guarded(
+            [&]
+            {
+                require(mesh != nullptr && out != nullptr, "null mesh or output");
+                const smr::MeshConfig& c       = mesh->m.host.cfg;
+                out->dim                       = c.dim;
+                out->min_level                 = c.min_level;
+                out->max_level                 = c.max_level;
+                out->prediction_stencil_radius = c.pred_s;
+                out->max_stencil_radius        = c.radius;
+                out->graduation_width          = c.grad_width;
+                return 0;
+            });
+    }
+
     int smr_mesh_get_cells(const smr_mesh* mesh, int mesh_id, int64_t* out)
     {
         return guarded(
@@ -484,6 +501,26 @@ extern "C"
                 f.mesh->ctx->sync();
                 return 0;
             });
+    }
+
+    const char* smr_field_name(const smr_field* field)
+    {
+        if (!field)
+        {
+            g_error = "null field";
+            return nullptr;
+        }
+        return field->f.name.c_str();
+    }
+
+    int smr_field_on_mesh(const smr_field* field, const smr_mesh* mesh)
+    {
+        if (!field || !mesh)
+        {
+            g_error = "null field or mesh";
+            return SMR_ERR_ARGUMENT;
+        }
+        return field->f.mesh == &mesh->m ? 1 : 0;
     }
 
     int smr_field_get_cells(smr_field* field, double* values)

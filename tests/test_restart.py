@@ -1,0 +1,164 @@
+"""Restart files (io/restart.hpp layout) on the CPU: the reference's round
+trips (tests/test_restart.cpp), the LevelCellArray arrays written for known
+cell lists, derived by hand from the reference's construction rules
+(level_cell_array.hpp:465-522), and the file structure.  Meshes are host-only;
+fields need the GPU and are in test_gpu_restart.py."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import samurai_amd as sa
+from oracle_lib import OracleSim
+from initial_conditions import disc_2d, sphere_3d
+
+H5DUMP = "/opt/conda/bin/h5dump"
+
+pytestmark = pytest.mark.skipif(not os.path.exists(sa.IO_LIB_PATH), reason="restart I/O library not built (no HDF5)")
+
+def cells_of(cl):
+    """CellList-like {level: {(y, z): [(s, e), ...]}} -> rows (level, x, y, z) in for_each_cell order."""
+    rows = []
+    for level in sorted(cl):
+        for (y, z) in sorted(cl[level], key=lambda k: (k[1], k[0])):
+            for s, e in cl[level][(y, z)]:
+                rows += [(level, x, y, z) for x in range(s, e)]
+    return np.array(rows, dtype=np.int32)
+
+
+def read(h5, dset, nfields=1):
+    """A dataset's values, parsed from h5dump's text output: a list of
+    numbers, or of tuples of `nfields` numbers for a compound type."""
+    out = subprocess.run([H5DUMP, "-d", dset, "-m", "%.17g", h5], check=True, capture_output=True, text=True).stdout
+    body = out[out.index("DATA {") + len("DATA {"):]
+    body = re.sub(r"\(\d+(,\d+)*\):", " ", body)
+    vals = [float(v) if ("." in v or "e" in v) else int(v) for v in re.findall(r"-?\d+(?:\.\d*)?(?:e[-+]?\d+)?", body)]
+    if nfields == 1:
+        return vals
+    return [tuple(vals[i:i + nfields]) for i in range(0, len(vals), nfields)]
+
+
+# tests/test_restart.cpp:51-84: the CellList of each dimension
+CL_2D = {1: {(0, 0): [(2, 8)], (1, 0): [(10, 15)]}, 4: {(6, 0): [(0, 1)]}}
+CL_3D = {1: {(0, 0): [(2, 8)], (1, 0): [(10, 15)]}, 4: {(6, 1): [(0, 1)]}}
+
+
+@pytest.mark.parametrize("dim,cl", [(2, CL_2D), (3, CL_3D)])
+def test_cell_array_round_trip(tmp_path, dim, cl):
+    cells = cells_of(cl)
+    sa.dump_cells(tmp_path / "mesh", dim, cells)
+    d, back = sa.load_cells(tmp_path / "mesh")
+    assert d == dim
+    assert np.array_equal(back, cells)
+
+
+@pytest.mark.skipif(not os.path.exists(H5DUMP), reason="h5dump not available")
+def test_level_cell_array_arrays_2d(tmp_path):
+    """Level 1 has rows y = 0 ([2, 8)) and y = 1 ([10, 15)): one y-interval
+    [0, 2) whose index maps y to row positions 0, 1 (index 0), offsets
+    [0, 1, 2]; the x-intervals number the cells in for_each_cell order
+    (CellArray's update_index, cell_array.hpp:484-493): 0 - 2 and 6 - 10.  Level 4 has row y = 6: y-interval [6, 7) with index -6 (row
+    position 0), offsets [0, 1]."""
+    sa.dump_cells(tmp_path / "mesh", 2, cells_of(CL_2D))
+    h5 = str(tmp_path / "mesh.h5")
+    x1 = read(h5, "/mesh/level/1/dim/0/intervals/data", 4)
+    y1 = read(h5, "/mesh/level/1/dim/1/intervals/data", 4)
+    o1 = read(h5, "/mesh/level/1/dim/1/offsets/data")
+    assert x1 == [(2, 8, 1, -2), (10, 15, 1, -4)]
+    assert y1 == [(0, 2, 1, 0)]
+    assert o1 == [0, 1, 2]
+    y4 = read(h5, "/mesh/level/4/dim/1/intervals/data", 4)
+    assert y4 == [(6, 7, 1, -6)]
+    assert read(h5, "/mesh/level/4/dim/1/offsets/data") == [0, 1]
+    assert read(h5, "/mesh/level/1/dim/0/intervals/partition") == [0, 2]
+    assert read(h5, "/n_process") == [1]
+    assert read(h5, "/mesh/dim") == [2]
+    assert read(h5, "/mesh/min_level") == [1]
+    assert read(h5, "/mesh/max_level") == [4]
+    assert read(h5, "/mesh/origin_point") == [0.0, 0.0]
+    assert read(h5, "/mesh/scaling_factor") == [1.0]
+
+
+@pytest.mark.skipif(not os.path.exists(H5DUMP), reason="h5dump not available")
+def test_level_cell_array_arrays_3d(tmp_path):
+    """3D level 1: plane z = 0 holds rows y = 0, 1 (one y-interval [0, 2),
+    index 0), so one z-interval [0, 1) with index 0 and plane offsets [0, 1].
+    Level 4: plane z = 1, row y = 6: z-interval [1, 2) index -1, y-interval
+    [6, 7) index -6."""
+    sa.dump_cells(tmp_path / "mesh", 3, cells_of(CL_3D))
+    h5 = str(tmp_path / "mesh.h5")
+    z1 = read(h5, "/mesh/level/1/dim/2/intervals/data", 4)
+    y1 = read(h5, "/mesh/level/1/dim/1/intervals/data", 4)
+    assert z1 == [(0, 1, 1, 0)]
+    assert y1 == [(0, 2, 1, 0)]
+    assert read(h5, "/mesh/level/1/dim/2/offsets/data") == [0, 1]
+    assert read(h5, "/mesh/level/1/dim/1/offsets/data") == [0, 1, 2]
+    z4 = read(h5, "/mesh/level/4/dim/2/intervals/data", 4)
+    y4 = read(h5, "/mesh/level/4/dim/1/intervals/data", 4)
+    assert z4 == [(1, 2, 1, -1)]
+    assert y4 == [(6, 7, 1, -6)]
+
+
+@pytest.mark.skipif(not os.path.exists(H5DUMP), reason="h5dump not available")
+def test_interval_index_is_the_storage_offset(tmp_path, host_ctx):
+    """The x-intervals of an MRMesh's leaves carry the storage offset of their
+    run (value of x at index + x), as the reference's mesh[cells] does after
+    renumbering (mesh.hpp:893-911)."""
+    m = sa.make_mesh(host_ctx, sa.mesh_config(2, 2, 5, 1))
+    sa.dump(tmp_path / "mesh", m)
+    cells = m.cells()
+    xi = read(str(tmp_path / "mesh.h5"), "/mesh/level/5/dim/0/intervals/data", 4)
+    first = {(int(c[2]), int(c[1])): int(c[4]) for c in cells}
+    for i in xi:
+        row = None
+        for (y, x), st in first.items():
+            if x == i[0] and st - x == i[3]:
+                row = y
+                break
+        assert row is not None, i
+
+
+def adapted(ctx, dim, lmin, lmax):
+    """A graduated multi-level mesh: the oracle's adaptation of a disc
+    (sphere) on the box mesh, rebuilt by the product from its leaves."""
+    o = OracleSim(dim, lmin, lmax, pred_s=1)
+    c = o.cells()
+    o.set_values(disc_2d(c) if dim == 2 else sphere_3d(c))
+    o.set_dirichlet(0.0)
+    o.adapt(1e-3, 1.0)
+    return sa.make_mesh_from_cells(ctx, sa.mesh_config(dim, lmin, lmax, 1), o.cells())
+
+
+@pytest.mark.parametrize("dim,lmin,lmax", [(2, 2, 5), (2, 3, 7), (3, 2, 4)])
+def test_mesh_round_trip(tmp_path, host_ctx, dim, lmin, lmax):
+    """restart_mrmesh (tests/test_restart.cpp:96-103) on box and adapted
+    meshes: the loaded mesh has the same leaves, ids and storage numbering."""
+    cfg = sa.mesh_config(dim, lmin, lmax, 1)
+    for m in (sa.make_mesh(host_ctx, cfg), adapted(host_ctx, dim, lmin, lmax)):
+        sa.dump(tmp_path / "mesh", m)
+        m2 = sa.load_mesh(tmp_path / "mesh", host_ctx, sa.mesh_config(dim, 1, 2, 1))
+        assert (m2.min_level, m2.max_level) == (lmin, lmax)
+        for mid in (sa.CELLS, sa.CELLS_AND_GHOSTS, sa.PROJ_CELLS, sa.REFERENCE):
+            assert np.array_equal(m.cells(mid), m2.cells(mid)), mid
+
+
+def test_load_errors(tmp_path, host_ctx):
+    sa.dump_cells(tmp_path / "mesh", 2, cells_of(CL_2D))
+    with pytest.raises(sa.SamuraiError, match="dimension"):
+        sa.load_mesh(tmp_path / "mesh", host_ctx, sa.mesh_config(3, 1, 4, 1))
+    with pytest.raises(sa.SamuraiError, match="cannot open"):
+        sa.load_cells(tmp_path / "absent")
+
+
+def test_io_abi_exports_every_declared_symbol():
+    import ctypes
+    import re
+    header = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "samurai_amd_io.h")
+    names = re.findall(r"\b(smr_\w+)\s*\(", open(header).read())
+    assert names
+    sa.io_lib()
+    L = ctypes.CDLL(sa.IO_LIB_PATH)
+    for n in names:
+        assert hasattr(L, n), n
