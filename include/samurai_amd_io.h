@@ -1,9 +1,9 @@
-/* Restart files of samurai's layout (io/restart.hpp), written and read with
- * the HDF5 C library: a separate shared library (libsamurai_amd_io.so) on top
- * of the C ABI in samurai_amd.h, so the compute library does not depend on
- * HDF5.
+/* Restart files of samurai's layout (io/restart.hpp) and samurai's HDF5/XDMF
+ * output (io/hdf5.hpp), written and read with the HDF5 C library: a separate
+ * shared library (libsamurai_amd_io.so) on top of the C ABI in samurai_amd.h,
+ * so the compute library does not depend on HDF5.
  *
- * File layout (the reference's, io/restart.hpp:52-187, single process):
+ * Restart file layout (the reference's, io/restart.hpp:52-187, single process):
  *   /n_process                 size_t (1)
  *   /mesh/dim                  size_t
  *   /mesh/min_level, max_level size_t (the mesh's configured levels)
@@ -49,6 +49,27 @@ int smr_restart_load_field(const char* file, const smr_mesh* mesh, smr_field* fi
  * count, writing at most `capacity` rows (call with capacity 0 to size) */
 int smr_restart_dump_cells(const char* file, int dim, const int32_t* cells, int64_t n);
 int64_t smr_restart_load_cells(const char* file, int* dim, int32_t* cells, int64_t capacity);
+
+/* samurai::save(path, filename, options, mesh, fields...) (io/hdf5.hpp:1280-1329,
+ * one process): {path}/{filename}.h5 with the leaves as an unstructured grid
+ * (/mesh/points: double [n_points][3], /mesh/connectivity: uint64
+ * [n_cells][2^dim], points numbered in the order the cells' corners are first
+ * met, io/hdf5.hpp:84-145), one dataset per scalar field
+ * (/mesh/fields/{name}), and {path}/{filename}.xdmf describing them.  `path`
+ * is created if missing.  options: 0, or Hdf5Options's by_level
+ * (SMR_SAVE_BY_LEVEL: one grid per level, /level/{l}/mesh, from min_level - 2)
+ * and by_mesh_id (SMR_SAVE_BY_MESH_ID: one grid per mesh id, /mesh/{id name},
+ * fields on every cell of the id, ghosts included) (io/hdf5.hpp:146-158, 428-522) */
+#define SMR_SAVE_BY_LEVEL 1
+#define SMR_SAVE_BY_MESH_ID 2
+int smr_save(const char* path, const char* filename, const smr_mesh* mesh, smr_field* const* fields, int nfields, int options);
+
+/* samurai::save(path, filename, options, cell_array, fields...) for a CellArray
+ * (Hdf5_CellArray, io/hdf5.hpp:958-1000): n leaves as rows (level, x, y, z)
+ * of int32 in for_each_cell order, and nfields host arrays of n values with
+ * their names; the sub-mesh of by_mesh_id is "cell_array" */
+int smr_save_cell_array(const char* path, const char* filename, int dim, const int32_t* cells, int64_t n, const double* const* values,
+                        const char* const* names, int nfields, int options);
 
 const char* smr_io_last_error(void);
 

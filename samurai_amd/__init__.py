@@ -541,6 +541,9 @@ def io_lib():
         L.smr_restart_dump_cells.argtypes = [ctypes.c_char_p, i32, ctypes.POINTER(ctypes.c_int32), i64]
         L.smr_restart_load_cells.restype = i64
         L.smr_restart_load_cells.argtypes = [ctypes.c_char_p, ctypes.POINTER(i32), ctypes.POINTER(ctypes.c_int32), i64]
+        L.smr_save.argtypes = [ctypes.c_char_p, ctypes.c_char_p, vp, ctypes.POINTER(vp), i32, i32]
+        L.smr_save_cell_array.argtypes = [ctypes.c_char_p, ctypes.c_char_p, i32, ctypes.POINTER(ctypes.c_int32), i64,
+                                          ctypes.POINTER(ctypes.POINTER(ctypes.c_double)), ctypes.POINTER(ctypes.c_char_p), i32, i32]
         _io = L
     return _io
 
@@ -608,3 +611,36 @@ def load_cells(filename):
     _io_check(io_lib().smr_restart_load_cells(_h5(filename), ctypes.byref(dim), out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), n))
     return dim.value, out
 
+
+
+SAVE_BY_LEVEL, SAVE_BY_MESH_ID = 1, 2
+
+
+def _save_options(by_level, by_mesh_id):
+    return (SAVE_BY_LEVEL if by_level else 0) | (SAVE_BY_MESH_ID if by_mesh_id else 0)
+
+
+def save(path, filename, mesh, *fields, by_level=False, by_mesh_id=False):
+    """samurai::save(path, filename, Hdf5Options(by_level, by_mesh_id), mesh,
+    fields...) (io/hdf5.hpp:1280-1329): path/filename.h5 and .xdmf."""
+    arr = (ctypes.c_void_p * max(len(fields), 1))(*[f.h for f in fields])
+    _io_check(io_lib().smr_save(os.fspath(path).encode(), filename.encode(), mesh.h, arr, len(fields),
+                                _save_options(by_level, by_mesh_id)))
+
+
+def save_cell_array(path, filename, dim, cells, fields=None, by_level=False, by_mesh_id=False):
+    """samurai::save of a CellArray with fields (io/hdf5.hpp:958-1000): cells as
+    (n, >=4) rows (level, x, y, z) in for_each_cell order, fields a mapping
+    name -> n values."""
+    c = np.ascontiguousarray(np.asarray(cells).reshape(-1, np.asarray(cells).shape[-1])[:, :4], dtype=np.int32)
+    fields = dict(fields or {})
+    vals = [np.ascontiguousarray(v, dtype=np.float64) for v in fields.values()]
+    for v in vals:
+        if v.shape != (c.shape[0],):
+            raise SamuraiError(f"a field has {v.size} values for {c.shape[0]} cells")
+    dp = ctypes.POINTER(ctypes.c_double)
+    varr = (dp * max(len(vals), 1))(*[v.ctypes.data_as(dp) for v in vals])
+    narr = (ctypes.c_char_p * max(len(vals), 1))(*[n.encode() for n in fields])
+    _io_check(io_lib().smr_save_cell_array(os.fspath(path).encode(), filename.encode(), int(dim),
+                                           c.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), c.shape[0], varr, narr, len(vals),
+                                           _save_options(by_level, by_mesh_id)))

@@ -75,7 +75,7 @@ IO_LIB = os.path.join(HERE, "libsamurai_amd_io.so")
 
 
 def build_io():
-    """libsamurai_amd_io.so: restart files (HDF5) over the C ABI; skipped when
+    """libsamurai_amd_io.so: restart files and HDF5/XDMF output over the C ABI; skipped when
     the HDF5 headers and library are absent (the compute library does not
     need it)."""
     inc, lib = os.path.join(HDF5_ROOT, "include"), os.path.join(HDF5_ROOT, "lib")
@@ -83,7 +83,8 @@ def build_io():
         sys.stderr.write(f"samurai_amd: HDF5 not found under {HDF5_ROOT}; restart I/O not built\n")
         return None
     cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wextra", "-Wno-unused-parameter",
-           f"-I{os.path.join(ROOT, 'include')}", f"-I{inc}", os.path.join(CSRC, "io", "restart.cpp"), "-o", IO_LIB + ".tmp",
+           f"-I{os.path.join(ROOT, 'include')}", f"-I{inc}", os.path.join(CSRC, "io", "restart.cpp"), os.path.join(CSRC, "io", "xdmf.cpp"),
+           "-o", IO_LIB + ".tmp",
            f"-L{HERE}", "-lsamurai_amd", f"-L{lib}", "-lhdf5",
            # the HDF5 directory holds an older libstdc++: this library carries its
            # own (static) and finds HDF5 through a RUNPATH, which serves its own

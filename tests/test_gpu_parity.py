@@ -115,7 +115,9 @@ def test_harten_matches_oracle(gpu_ctx, dim, lmin, lmax, s, ic):
             break
 
 
-def run_demo(gpu_ctx, s, n_steps=None):
+def run_demo(gpu_ctx, s, n_steps=None, on_init=None, keep=False):
+    """demos/FiniteVolume/advection_2d.cpp to Tf = 0.01; on_init(mesh, u) runs
+    where the demo saves its initial state; keep adds (mesh, u) to the result."""
     cfg = sa.mesh_config(2, 4, 10, s)
     mesh = sa.make_mesh(gpu_ctx, cfg)
     u = sa.make_scalar_field("u", mesh)
@@ -125,6 +127,8 @@ def run_demo(gpu_ctx, s, n_steps=None):
     adapt = sa.make_MRAdapt(u)
     adapt(2e-4, 1.0)
     init = (mesh.cells(), u.cells())
+    if on_init is not None:
+        on_init(mesh, u)
     dt, Tf, t, n = 0.5 * (1.0 / (1 << 10)), 0.01, 0.0, 0
     a = np.array([1.0, 1.0])
     while t != Tf and (n_steps is None or n < n_steps):
@@ -138,6 +142,8 @@ def run_demo(gpu_ctx, s, n_steps=None):
         sa.upwind(a, dt, u, unp1)
         sa.swap(u, unp1)
         n += 1
+    if keep:
+        return init, (mesh.cells(), u.cells()), n, (mesh, u)
     return init, (mesh.cells(), u.cells()), n
 
 

@@ -81,3 +81,28 @@ def test_restarted_run_matches_the_uninterrupted_run(tmp_path, gpu_ctx):
     advance(m2, u2, ad2, 8, dt, a)
     assert np.array_equal(mesh.cells(), m2.cells())
     assert np.array_equal(bits(u.cells()), bits(u2.cells()))
+
+
+@pytest.mark.parametrize("s", [0, 1])
+def test_save_of_the_advection_demo_matches_reference_files(tmp_path, gpu_ctx, s):
+    """samurai::save of the advection_2d demo (demos/FiniteVolume/advection_2d.cpp:
+    48-57, 124, 151) run on the GPU: the initial and final files have the
+    reference's points, connectivity and XDMF text, and its values within
+    1e-12 (the tolerance of the demo's field parity)."""
+    say("test_save_of_the_advection_demo_matches_reference_files", s)
+    from test_gpu_parity import run_demo
+    from test_save import STEM, check_against_reference, fixture_cells
+    saved = []
+
+    def on_init(mesh, u):
+        sa.save(tmp_path, STEM.format(s=s, suffix="_init"), mesh, u)
+        saved.append("_init")
+
+    (_, _), (_, _), n, (mesh, u) = run_demo(gpu_ctx, s, on_init=on_init, keep=True)
+    assert n == 21
+    sa.save(tmp_path, STEM.format(s=s, suffix=""), mesh, u)
+    for when, name in (("_init", f"advection_2d_pred{s}_init.npz"), ("", f"advection_2d_pred{s}_t0.01.npz")):
+        got = check_against_reference(tmp_path, STEM.format(s=s, suffix=when))
+        _, u_ref = fixture_cells(name)
+        assert np.max(np.abs(got - u_ref)) <= 1e-12, name
+    assert saved == ["_init"]
