@@ -73,6 +73,16 @@ const char* smr_last_error(void);
 int smr_mesh_create_box(smr_ctx* ctx, const smr_mesh_config* cfg, smr_mesh** out);
 /* MRMesh(ca, cfg) (mesh.hpp:386-396): leaves given as n rows (level, x, y, z) */
 int smr_mesh_create_from_cells(smr_ctx* ctx, const smr_mesh_config* cfg, const int32_t* cells, int64_t n, smr_mesh** out);
+/* mesh = MRMesh(ca, cfg) in place, cfg being the mesh's own with the given
+ * min and max levels: what samurai::load does to the mesh it is given
+ * (io/restart.hpp:449-455, std::swap(mesh, new_mesh)).  Leaves as rows
+ * (level, x, y, z); fields on the mesh must be resized afterwards. */
+int smr_mesh_assign_cells(smr_mesh* mesh, int min_level, int max_level, const int32_t* cells, int64_t n);
+/* std::swap(mesh_a, mesh_b) of two meshes of one context: the contents are
+ * exchanged while fields stay attached to their mesh object, as after the
+ * reference's `mesh = mra::make_mesh(...)` or load (io/restart.hpp:454); fields
+ * on either mesh must be resized before their next use. */
+int smr_mesh_swap(smr_mesh* a, smr_mesh* b);
 int smr_mesh_destroy(smr_mesh* mesh);
 /* mesh.nb_cells(mesh_id) (mesh.hpp:540-551); SMR_REFERENCE gives the storage size */
 int64_t smr_mesh_nb_cells(const smr_mesh* mesh, int mesh_id);

@@ -189,6 +189,42 @@ namespace samurai_amd
             h_.reset(m, [](smr_mesh* p) { smr_mesh_destroy(p); });
             cfg_ = cfg;
         }
+        // mra::make_empty_mesh(cfg): no cells, to be assigned or loaded
+        struct empty_tag
+        {
+        };
+        MRMesh(const smr_mesh_config& cfg, empty_tag)
+        {
+            smr_mesh* m = nullptr;
+            check(smr_mesh_create_from_cells(context(), &cfg, nullptr, 0, &m));
+            h_.reset(m, [](smr_mesh* p) { smr_mesh_destroy(p); });
+            cfg_ = cfg;
+        }
+        MRMesh(const MRMesh&)            = default;
+        MRMesh(MRMesh&&)                 = default;
+        MRMesh& operator=(const MRMesh&) = delete;
+        // mesh = mra::make_mesh(...): the new contents are swapped into this
+        // mesh, so the fields made on it stay attached to it (as in samurai,
+        // where fields hold a reference to the mesh object); resize them
+        MRMesh& operator=(MRMesh&& o)
+        {
+            if (h_ && o.h_)
+            {
+                check(smr_mesh_swap(h_.get(), o.h_.get()));
+                std::swap(cfg_, o.cfg_);
+            }
+            else
+            {
+                h_   = std::move(o.h_);
+                cfg_ = o.cfg_;
+            }
+            return *this;
+        }
+        // re-read the configuration after the mesh was replaced through the C ABI
+        void refresh()
+        {
+            check(smr_mesh_get_config(h_.get(), &cfg_));
+        }
         smr_mesh* handle() const
         {
             return h_.get();
@@ -252,6 +288,12 @@ namespace samurai_amd
                 }
             }
             return MRMesh<dim>(cfg.raw());
+        }
+
+        template <class Config>
+        MRMesh<Config::dim> make_empty_mesh(const Config& cfg)
+        {
+            return MRMesh<Config::dim>(cfg.raw(), typename MRMesh<Config::dim>::empty_tag{});
         }
     } // namespace mra
 

@@ -44,6 +44,13 @@ int smr_restart_load_mesh(const char* file, smr_ctx* ctx, const smr_mesh_config*
  * (loaded from the same file) */
 int smr_restart_load_field(const char* file, const smr_mesh* mesh, smr_field* field);
 
+/* samurai::load(path, filename, mesh, fields...) (io/restart.hpp:423-462): the
+ * mesh is replaced in place by the file's leaves (its configuration kept, the
+ * levels taken from the file), then every field on it is resized and given
+ * the values of /fields/{field name}.  Every field must be in the file; the
+ * mesh is left unchanged when one is missing or the dimensions differ. */
+int smr_restart_load(const char* file, smr_mesh* mesh, smr_field* const* fields, int nfields);
+
 /* samurai::dump / load of a CellArray alone (io/restart.hpp:126-150, 313-337):
  * n leaves given as rows (level, x, y, z) of int32; load returns the leaf
  * count, writing at most `capacity` rows (call with capacity 0 to size) */

@@ -58,6 +58,7 @@ def lib():
         _sig(L, "smr_ctx_synchronize", "argtypes", [vp])
         _sig(L, "smr_mesh_create_box", "argtypes", [vp, P(MeshConfig), ctypes.POINTER(vp)])
         _sig(L, "smr_mesh_create_from_cells", "argtypes", [vp, P(MeshConfig), P(ctypes.c_int32), i64, ctypes.POINTER(vp)])
+        _sig(L, "smr_mesh_assign_cells", "argtypes", [vp, i32, i32, P(ctypes.c_int32), i64])
         _sig(L, "smr_mesh_destroy", "argtypes", [vp])
         _sig(L, "smr_mesh_nb_cells", "restype", i64)
         _sig(L, "smr_mesh_nb_cells", "argtypes", [vp, i32])
@@ -200,6 +201,20 @@ class MRMesh:
     def validate(self):
         """Host check that every kernel access of this mesh's programs is in bounds."""
         return _check(lib().smr_mesh_validate(self.h))
+
+    def refresh_config(self):
+        """Re-read the configuration (the levels change when a restart file is loaded into the mesh)."""
+        cfg = MeshConfig()
+        _check(lib().smr_mesh_get_config(self.h, ctypes.byref(cfg)))
+        self.cfg = cfg
+
+    def assign_cells(self, cells, min_level=None, max_level=None):
+        """mesh = MRMesh(ca, cfg) in place, from (n, >=4) rows (level, x, y, z)."""
+        c = np.ascontiguousarray(np.asarray(cells).reshape(-1, np.asarray(cells).shape[-1])[:, :4], dtype=np.int32)
+        lo = self.min_level if min_level is None else min_level
+        hi = self.max_level if max_level is None else max_level
+        _check(lib().smr_mesh_assign_cells(self.h, lo, hi, c.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), c.shape[0]))
+        self.refresh_config()
 
     def rebuild_from_tags(self, tags):
         tags = np.ascontiguousarray(tags, dtype=np.uint8)
@@ -538,6 +553,7 @@ def io_lib():
         L.smr_restart_dump.argtypes = [ctypes.c_char_p, vp, ctypes.POINTER(vp), i32]
         L.smr_restart_load_mesh.argtypes = [ctypes.c_char_p, vp, ctypes.POINTER(MeshConfig), ctypes.POINTER(vp)]
         L.smr_restart_load_field.argtypes = [ctypes.c_char_p, vp, vp]
+        L.smr_restart_load.argtypes = [ctypes.c_char_p, vp, ctypes.POINTER(vp), i32]
         L.smr_restart_dump_cells.argtypes = [ctypes.c_char_p, i32, ctypes.POINTER(ctypes.c_int32), i64]
         L.smr_restart_load_cells.restype = i64
         L.smr_restart_load_cells.argtypes = [ctypes.c_char_p, ctypes.POINTER(i32), ctypes.POINTER(ctypes.c_int32), i64]
@@ -593,6 +609,15 @@ def load(filename, ctx, cfg, *names):
         load_field(filename, mesh, f)
         fields.append(f)
     return mesh, fields
+
+
+def load_into(filename, mesh, *fields):
+    """samurai::load(filename, mesh, fields...) (io/restart.hpp:423-462): `mesh`
+    is replaced in place by the file's leaves and levels, and each field on it
+    gets the file's values of its name."""
+    arr = (ctypes.c_void_p * max(len(fields), 1))(*[f.h for f in fields])
+    _io_check(io_lib().smr_restart_load(_h5(filename), mesh.h, arr, len(fields)))
+    mesh.refresh_config()
 
 
 def dump_cells(filename, dim, cells):

@@ -57,11 +57,15 @@ def build(verbose=False):
         sys.stderr.write(r.stdout.decode(errors="replace"))
         raise RuntimeError("samurai_amd: link failed")
     os.replace(LIB + ".tmp", LIB)
-    build_io()
-    # the C++ drop-in demo, built on include/samurai_amd/samurai.hpp
+    io = build_io()
+    # the C++ drop-in demo, built on include/samurai_amd/samurai.hpp (and io.hpp
+    # for its output and restart files when the I/O library is there)
     demo = os.path.join(HERE, "demos", "advection_2d")
     cmd = ["g++", "-O2", "-std=c++17", "-Wall", f"-I{os.path.join(ROOT, 'include')}", os.path.join(HERE, "demos", "advection_2d.cpp"),
-           "-o", demo + ".tmp", f"-L{HERE}", "-lsamurai_amd", "-Wl,-rpath,$ORIGIN/.."]
+           "-o", demo + ".tmp", f"-L{HERE}"]
+    if io:
+        cmd += ["-DSAMURAI_AMD_WITH_IO", "-lsamurai_amd_io", f"-Wl,-rpath-link,{os.path.join(HDF5_ROOT, 'lib')}"]
+    cmd += ["-lsamurai_amd", "-Wl,-rpath,$ORIGIN/.."]
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
     if r.returncode != 0:
         sys.stderr.write(r.stdout.decode(errors="replace"))

@@ -90,6 +90,42 @@ namespace
         return cfg;
     }
 
+    smr_mesh_config from_cfg(const smr::MeshConfig& c)
+    {
+        smr_mesh_config o;
+        o.dim                       = c.dim;
+        o.min_level                 = c.min_level;
+        o.max_level                 = c.max_level;
+        o.prediction_stencil_radius = c.pred_s;
+        o.max_stencil_radius        = c.radius;
+        o.graduation_width          = c.grad_width;
+        return o;
+    }
+
+    // MRMesh(ca, cfg) (mr/mesh.hpp:386-396) from leaves given as rows
+    // (level, x, y, z); graduated is recorded, not enforced
+    smr::MRMeshData mesh_from_cells(const smr::MeshConfig& c, const int32_t* cells, int64_t n)
+    {
+        require(n >= 0, "negative cell count");
+        std::vector<std::vector<std::array<int32_t, 4>>> q(smr::MAXL);
+        for (int64_t k = 0; k < n; ++k)
+        {
+            const int32_t* r = cells + 4 * k;
+            require(r[0] >= 0 && r[0] < smr::MAXL, "cell level out of range");
+            q[size_t(r[0])].push_back({r[2], c.dim == 3 ? r[3] : 0, r[1], r[1] + 1});
+        }
+        smr::CellArray ca = smr::empty_cell_array(c.dim);
+        for (int l = 0; l < smr::MAXL; ++l)
+        {
+            ca[size_t(l)] = smr::from_runs(c.dim, l, q[size_t(l)]);
+        }
+        smr::CellArray g = ca;
+        smr::make_graduation(g, c.dim, c.grad_width);
+        smr::MRMeshData m = smr::build_mesh(c, ca);
+        m.graduated       = smr::ca_equal(g, ca);
+        return m;
+    }
+
     std::vector<int64_t> cell_storage_index(const smr::MRMeshData& m)
     {
         std::vector<int64_t> idx;
@@ -205,34 +241,45 @@ extern "C"
             [&]
             {
                 require(ctx && out && (cells || n == 0), "null argument");
-                smr::MeshConfig c = to_cfg(cfg);
-                std::vector<std::vector<std::array<int32_t, 4>>> q(smr::MAXL);
-                for (int64_t k = 0; k < n; ++k)
-                {
-                    const int32_t* r = cells + 4 * k;
-                    require(r[0] >= 0 && r[0] < smr::MAXL, "cell level out of range");
-                    q[size_t(r[0])].push_back({r[2], c.dim == 3 ? r[3] : 0, r[1], r[1] + 1});
-                }
-                smr::CellArray ca = smr::empty_cell_array(c.dim);
-                for (int l = 0; l < smr::MAXL; ++l)
-                {
-                    ca[size_t(l)] = smr::from_runs(c.dim, l, q[size_t(l)]);
-                }
-                auto* m  = new smr_mesh();
+                auto m   = std::make_unique<smr_mesh>();
                 m->m.ctx = &ctx->c;
-                try
+                m->m.host = mesh_from_cells(to_cfg(cfg), cells, n);
+                *out      = m.release();
+                return 0;
+            });
+    }
+
+    int smr_mesh_assign_cells(smr_mesh* mesh, int min_level, int max_level, const int32_t* cells, int64_t n)
+    {
+        return guarded(
+            [&]
+            {
+                require(mesh && (cells || n == 0), "null argument");
+                smr_mesh_config c = from_cfg(mesh->m.host.cfg);
+                c.min_level       = min_level;
+                c.max_level       = max_level;
+                mesh->m.replace(mesh_from_cells(to_cfg(&c), cells, n), nullptr);
+                return 0;
+            });
+    }
+
+    int smr_mesh_swap(smr_mesh* a, smr_mesh* b)
+    {
+        return guarded(
+            [&]
+            {
+                require(a && b, "null mesh");
+                require(a->m.ctx == b->m.ctx, "swap of meshes of different contexts");
+                if (a == b)
                 {
-                    smr::CellArray g = ca;
-                    smr::make_graduation(g, c.dim, c.grad_width);
-                    m->m.host           = smr::build_mesh(c, ca);
-                    m->m.host.graduated = smr::ca_equal(g, ca);
+                    return 0;
                 }
-                catch (...)
-                {
-                    delete m;
-                    throw;
-                }
-                *out = m;
+                // device tables are rebuilt from the host data on next use;
+                // replace() moves both meshes to a new version, so fields on
+                // either must be resized before their next use
+                smr::MRMeshData ha = std::move(a->m.host);
+                a->m.replace(std::move(b->m.host), nullptr);
+                b->m.replace(std::move(ha), nullptr);
                 return 0;
             });
     }
@@ -263,13 +310,7 @@ extern "C"
             [&]
             {
                 require(mesh != nullptr && out != nullptr, "null mesh or output");
-                const smr::MeshConfig& c       = mesh->m.host.cfg;
-                out->dim                       = c.dim;
-                out->min_level                 = c.min_level;
-                out->max_level                 = c.max_level;
-                out->prediction_stencil_radius = c.pred_s;
-                out->max_stencil_radius        = c.radius;
-                out->graduation_width          = c.grad_width;
+                *out = from_cfg(mesh->m.host.cfg);
                 return 0;
             });
     }

@@ -256,6 +256,32 @@ namespace
         return out;
     }
 
+    // load_field (io/restart.hpp:341-386): the values of /fields/{name}, for_each_cell
+    // order, into a field living on `mesh`
+    void load_field(const File& f, const smr_mesh* mesh, smr_field* field)
+    {
+        const int on = smr_field_on_mesh(field, mesh);
+        check_smr(on, "field");
+        require(on == 1, "the field does not live on the given mesh");
+        const char* name = smr_field_name(field);
+        require(name && *name, "The field has no name.");
+        const std::string base = std::string("/fields/") + name;
+        require(f.exists(base), std::string("The field ") + name + " does not exist in the file.");
+        const unsigned long n_comp = f.read_scalar<unsigned long>(base + "/n_comp", H5T_NATIVE_ULONG);
+        require(n_comp == 1, "the field in the file has " + std::to_string(n_comp) + " components, a scalar field has 1");
+        check_smr(smr_field_resize(field), "field resize");
+        std::vector<double> v;
+        if (f.exists(base + "/data"))
+        {
+            v = f.read_partitioned<double>(base + "/data", H5T_NATIVE_DOUBLE);
+        }
+        const int64_t n = smr_mesh_nb_cells(mesh, SMR_CELLS);
+        check_smr(n, "mesh size");
+        require(int64_t(v.size()) == n, "the field in the file has " + std::to_string(v.size()) + " values, the mesh " + std::to_string(n) +
+                                            " cells");
+        check_smr(smr_field_set_cells(field, v.data()), "field values");
+    }
+
     struct Silence
     {
         Silence() { H5Eset_auto2(H5E_DEFAULT, nullptr, nullptr); }
@@ -337,27 +363,45 @@ extern "C"
             [&]
             {
                 require(file && mesh && field, "null argument");
-                const int on = smr_field_on_mesh(field, mesh);
-                check_smr(on, "field");
-                require(on == 1, "the field does not live on the given mesh");
-                const char* name = smr_field_name(field);
-                require(name && *name, "The field has no name.");
                 File f(file, false);
-                const std::string base = std::string("/fields/") + name;
-                require(f.exists(base), std::string("The field ") + name + " does not exist in the file.");
-                const unsigned long n_comp = f.read_scalar<unsigned long>(base + "/n_comp", H5T_NATIVE_ULONG);
-                require(n_comp == 1, "the field in the file has " + std::to_string(n_comp) + " components, a scalar field has 1");
-                check_smr(smr_field_resize(field), "field resize");
-                std::vector<double> v;
-                if (f.exists(base + "/data"))
+                load_field(f, mesh, field);
+            });
+    }
+
+    int smr_restart_load(const char* file, smr_mesh* mesh, smr_field* const* fields, int nfields)
+    {
+        return guarded(
+            [&]
+            {
+                require(file && mesh && nfields >= 0 && (nfields == 0 || fields), "null argument");
+                for (int k = 0; k < nfields; ++k)
                 {
-                    v = f.read_partitioned<double>(base + "/data", H5T_NATIVE_DOUBLE);
+                    require(fields[k] != nullptr, "null field");
+                    const int on = smr_field_on_mesh(fields[k], mesh);
+                    check_smr(on, "field");
+                    require(on == 1, "a field does not live on the given mesh");
+                    const char* name = smr_field_name(fields[k]);
+                    require(name && *name, "The field has no name.");
                 }
-                const int64_t n = smr_mesh_nb_cells(mesh, SMR_CELLS);
-                check_smr(n, "mesh size");
-                require(int64_t(v.size()) == n, "the field in the file has " + std::to_string(v.size()) + " values, the mesh " +
-                                                    std::to_string(n) + " cells");
-                check_smr(smr_field_set_cells(field, v.data()), "field values");
+                File f(file, false);
+                smr_mesh_config cfg;
+                check_smr(smr_mesh_get_config(mesh, &cfg), "mesh config");
+                int dim = 0, min_level = 0, max_level = 0;
+                const auto leaves = load_leaves(f, dim, min_level, max_level);
+                require(dim == cfg.dim, "The dimension of the mesh is not the same as the one of the mesh to be loaded. " +
+                                            std::to_string(dim) + " != " + std::to_string(cfg.dim));
+                // every field is in the file before the mesh is replaced
+                for (int k = 0; k < nfields; ++k)
+                {
+                    const std::string name = smr_field_name(fields[k]);
+                    require(f.exists("/fields/" + name), "The field " + name + " does not exist in the file.");
+                }
+                check_smr(smr_mesh_assign_cells(mesh, min_level, max_level, leaves.data(), int64_t(leaves.size() / 4)),
+                          "mesh from the restart file");
+                for (int k = 0; k < nfields; ++k)
+                {
+                    load_field(f, mesh, fields[k]);
+                }
             });
     }
 

@@ -9,6 +9,8 @@ import sys
 import numpy as np
 import pytest
 
+import samurai_amd as sa
+
 pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 GOLDEN = os.path.join(ROOT, "tests", "golden")
@@ -20,7 +22,8 @@ def test_cpp_demo_reproduces_reference_fixtures(tmp_path, s):
     print("cpp demo, prediction order", s, flush=True)
     assert os.path.exists(DEMO), "build the library first (python samurai_amd/build.py)"
     prefix = str(tmp_path / f"adv{s}")
-    r = subprocess.run([DEMO, "--prediction-order", str(s), "--out", prefix], capture_output=True, text=True, timeout=120)
+    r = subprocess.run([DEMO, "--prediction-order", str(s), "--out", prefix, "--path", str(tmp_path)], capture_output=True, text=True,
+                       timeout=120)
     sys.stdout.write(r.stdout)
     sys.stderr.write(r.stderr)
     assert r.returncode == 0
@@ -33,3 +36,33 @@ def test_cpp_demo_reproduces_reference_fixtures(tmp_path, s):
         assert np.array_equal(got[:, 1].astype(np.int64), fx["i"])
         assert np.array_equal(got[:, 2].astype(np.int64), fx["j"])
         assert np.max(np.abs(got[:, 3] - fx["u"])) <= 1e-12, name
+
+
+@pytest.mark.skipif(not os.path.exists(sa.IO_LIB_PATH), reason="I/O library not built (no HDF5)")
+@pytest.mark.parametrize("s", [0, 1])
+def test_cpp_demo_writes_the_reference_files_and_restarts(tmp_path, s):
+    """The demo's samurai::save output (demos/FiniteVolume/advection_2d.cpp:
+    47-59, 124, 151) is the reference's own regression files: same points,
+    connectivity and XDMF text, values within 1e-12.  Restarted from its
+    restart file of the initial state (--restart-file, :105-113), it ends on
+    the same final file."""
+    from test_save import check_against_reference, fixture_cells
+    name = "test_finite_volume_advection_2d_finite-volume-advection-2d-0.01"
+    stem = f"{name}_pred_{s}"
+    for restart in (False, True):
+        print("cpp demo files, prediction order", s, "restart", restart, flush=True)
+        out = tmp_path / ("restarted" if restart else "fresh")
+        out.mkdir()
+        args = [DEMO, "--prediction-order", str(s), "--path", str(out), "--filename", name]
+        if restart:
+            args += ["--restart-file", str(tmp_path / "fresh" / f"{stem}_restart_init")]
+        r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+        sys.stdout.write(r.stdout)
+        sys.stderr.write(r.stderr)
+        assert r.returncode == 0
+        assert "steps 21 " in r.stdout
+        for suffix, fx in (("_init", f"advection_2d_pred{s}_init.npz"), ("", f"advection_2d_pred{s}_t0.01.npz")):
+            got = check_against_reference(out, stem + suffix)
+            _, u_ref = fixture_cells(fx)
+            assert np.max(np.abs(got - u_ref)) <= 1e-12, (restart, fx)
+        assert os.path.exists(out / f"{stem}_restart.h5")

@@ -50,6 +50,17 @@ def test_mesh_and_fields_round_trip(tmp_path, gpu_ctx, dim, lmin, lmax):
     assert np.array_equal(bits(v.cells()), bits(v2.cells()))
     with pytest.raises(sa.SamuraiError, match="field w does not exist"):
         sa.load(tmp_path / "restart", gpu_ctx, sa.mesh_config(dim, 1, 2, 1), "w")
+    # in place, with fields made on the mesh beforehand; a missing field
+    # leaves mesh and fields as they were
+    m3 = sa.make_mesh(gpu_ctx, sa.mesh_config(dim, 1, 2, 1))
+    u3, v3, w3 = (sa.make_scalar_field(n, m3) for n in ("u", "v", "w"))
+    with pytest.raises(sa.SamuraiError, match="field w does not exist"):
+        sa.load_into(tmp_path / "restart", m3, u3, w3)
+    assert (m3.min_level, m3.max_level) == (1, 2) and u3.size() == m3.nb_cells(sa.REFERENCE)
+    sa.load_into(tmp_path / "restart", m3, v3, u3)
+    assert np.array_equal(mesh.cells(sa.REFERENCE), m3.cells(sa.REFERENCE))
+    assert np.array_equal(bits(u.cells()), bits(u3.cells()))
+    assert np.array_equal(bits(v.cells()), bits(v3.cells()))
 
 
 def advance(mesh, u, ad, n, dt, a):
@@ -62,11 +73,14 @@ def advance(mesh, u, ad, n, dt, a):
         sa.swap(u, unp1)
 
 
-def test_restarted_run_matches_the_uninterrupted_run(tmp_path, gpu_ctx):
+@pytest.mark.parametrize("into", [False, True])
+def test_restarted_run_matches_the_uninterrupted_run(tmp_path, gpu_ctx, into):
     """Advection of the disc on levels 4..9: 8 steps, dump, 8 more; the run
     loaded from the dump and advanced 8 steps ends on the same leaves with the
-    same values, bit for bit."""
-    say("test_restarted_run_matches_the_uninterrupted_run")
+    same values, bit for bit.  into: loaded as the reference demo does, into an
+    empty mesh whose field was made before the load (demos/FiniteVolume/
+    advection_2d.cpp:102-114)."""
+    say("test_restarted_run_matches_the_uninterrupted_run", into)
     lmin, lmax = 4, 9
     a = np.array([1.0, 1.0])
     dt = 0.5 / (1 << lmax)
@@ -75,7 +89,12 @@ def test_restarted_run_matches_the_uninterrupted_run(tmp_path, gpu_ctx):
     sa.dump(tmp_path / "advection_2d_restart", mesh, u)
     advance(mesh, u, ad, 8, dt, a)
 
-    m2, (u2,) = sa.load(tmp_path / "advection_2d_restart", gpu_ctx, sa.mesh_config(2, lmin, lmax, 1), "u")
+    if into:
+        m2 = sa.make_mesh_from_cells(gpu_ctx, sa.mesh_config(2, lmin, lmax, 1), np.zeros((0, 4), dtype=np.int32))
+        u2 = sa.make_scalar_field("u", m2)
+        sa.load_into(tmp_path / "advection_2d_restart", m2, u2)
+    else:
+        m2, (u2,) = sa.load(tmp_path / "advection_2d_restart", gpu_ctx, sa.mesh_config(2, lmin, lmax, 1), "u")
     sa.make_bc_dirichlet(u2, 0.0)
     ad2 = sa.make_MRAdapt(u2)
     advance(m2, u2, ad2, 8, dt, a)

@@ -162,3 +162,31 @@ def test_io_abi_exports_every_declared_symbol():
     L = ctypes.CDLL(sa.IO_LIB_PATH)
     for n in names:
         assert hasattr(L, n), n
+
+
+def test_load_into_replaces_the_mesh_in_place(tmp_path, host_ctx):
+    """samurai::load(file, mesh) into an empty mesh made with other levels
+    (the reference demo's make_empty_mesh + load, demos/FiniteVolume/
+    advection_2d.cpp:102-113): the mesh object takes the file's leaves and
+    levels; a file of another dimension leaves it unchanged."""
+    src = adapted(host_ctx, 2, 3, 7)
+    sa.dump(tmp_path / "r", src)
+    m = sa.make_mesh_from_cells(host_ctx, sa.mesh_config(2, 1, 2, 1), np.zeros((0, 4), dtype=np.int32))
+    assert m.nb_cells() == 0
+    sa.load_into(tmp_path / "r", m)
+    assert (m.min_level, m.max_level) == (3, 7)
+    for mid in (sa.CELLS, sa.CELLS_AND_GHOSTS, sa.PROJ_CELLS, sa.UNION_CELLS, sa.REFERENCE):
+        assert np.array_equal(src.cells(mid), m.cells(mid)), mid
+    sa.dump_cells(tmp_path / "c3", 3, cells_of(CL_3D))
+    before = m.cells(sa.REFERENCE)
+    with pytest.raises(sa.SamuraiError, match="dimension"):
+        sa.load_into(tmp_path / "c3", m)
+    assert np.array_equal(before, m.cells(sa.REFERENCE))
+
+
+def test_assign_cells_matches_a_new_mesh(host_ctx):
+    src = adapted(host_ctx, 2, 2, 6)
+    m = sa.make_mesh(host_ctx, sa.mesh_config(2, 2, 6, 1))
+    m.assign_cells(src.cells())
+    for mid in (sa.CELLS, sa.CELLS_AND_GHOSTS, sa.PROJ_CELLS, sa.UNION_CELLS, sa.REFERENCE):
+        assert np.array_equal(src.cells(mid), m.cells(mid)), mid
