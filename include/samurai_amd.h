@@ -188,10 +188,18 @@ int smr_upwind_hand_loop(double dt, smr_field* u, smr_field* unp1);
  * updated first (update_ghost_mr, as update_ghosts_if_needed does);
  * velocity: dim values; u and out: two fields on one mesh. */
 int smr_convection_upwind(const double* velocity, smr_field* u, smr_field* out);
+/* make_convection_upwind<Field>()(u) for a scalar field
+ * (schemes/fv/operators/convection_nonlin.hpp:24-79): the non-linear upwind
+ * of f(u) = u * u, the flux at an interface being f of the side picked by the
+ * sign of the mean of the two values, applied explicitly
+ * (explicit_flux_based_scheme__nonlin.hpp:35-86) over the same interfaces as
+ * smr_convection_upwind; u's ghosts are updated first. */
+int smr_convection_upwind_nonlinear(smr_field* u, smr_field* out);
 /* Diagnostic: the ordered contribution lists behind smr_convection_upwind,
  * computed on the host.  Call with null arrays to get *nrows and *nterms;
  * row: nrows storage slots, ptr: nrows+1, src: 2*nterms (second -1 for a
- * single product), coef: nterms. */
+ * single product), coef: nterms.  With a null velocity, the lists behind
+ * smr_convection_upwind_nonlinear: term k = coef[k] * F(u[src[2k]], u[src[2k+1]]). */
 int smr_mesh_convection_terms(const smr_mesh* mesh, const double* velocity, int64_t* nrows, int64_t* nterms, int64_t* row, int64_t* ptr,
                               int64_t* src, double* coef);
 

@@ -531,6 +531,26 @@ namespace samurai_amd
       private:
         std::array<double, Field::dim> v_;
     };
+    // make_convection_upwind<Field>() (convection_nonlin.hpp:24-79), scalar fields
+    template <class Field>
+    class ConvectionUpwindNonLinear
+    {
+      public:
+        Field operator()(Field& u) const
+        {
+            Field out("convection(" + u.name() + ")", u.mesh());
+            check(smr_convection_upwind_nonlinear(u.handle(), out.handle()));
+            u.touched();
+            out.touched();
+            return out;
+        }
+    };
+    template <class Field>
+    ConvectionUpwindNonLinear<Field> make_convection_upwind()
+    {
+        return {};
+    }
+
     template <class Field, class Vec>
     ConvectionUpwind<Field> make_convection_upwind(const Vec& v)
     {

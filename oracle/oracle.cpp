@@ -1430,6 +1430,129 @@ namespace oracle
         return out;
     }
 
+    // the scalar flux of the non-linear make_convection_upwind<Field>()
+    // (schemes/fv/operators/convection_nonlin.hpp:24-79): the upwind side of
+    // f(u) = u * u, chosen by the sign of the mean of the two values
+    inline double burgers_flux(double ul, double ur)
+    {
+        const double v = 0.5 * (ul + ur);
+        return v >= 0 ? ul * ul : ur * ur;
+    }
+
+    // make_convection_upwind<Field>()(u) for a scalar field, applied explicitly
+    // (explicit_flux_based_scheme__nonlin.hpp:35-86): for each direction, the
+    // same-level interfaces of every level, then the level jumps in the
+    // direction and in the opposite direction, with the fine ghost inside the
+    // coarse cell as stencil value (flux_based_scheme__nonlin.hpp:408-528,
+    // copy_stencil_values, :179-186), then the boundary interfaces
+    // (:530-590).  Each interface adds flux * left_factor to its left cell and
+    // (-flux) * right_factor to its right cell, in that order
+    // (process_interior_interfaces, :334-364; process_boundary_interfaces,
+    // :366-398); the output starts at zero.
+    std::vector<double> convection_upwind_nonlinear(const Sim& s)
+    {
+        const Mesh& m = s.mesh;
+        const int dim = m.dim;
+        std::vector<double> out(size_t(m.nref), 0.0);
+        auto idx = [&](int l, int x, int y, int z) { return size_t(m.index(l, x, y, z)); };
+        const auto& cells = m.ids[CELLS];
+        const int cmin = m.ca_min_level(CELLS), cmax = m.ca_max_level(CELLS);
+        const int minl = std::max(m.min_level, cmin - 1), maxl = std::min(m.max_level, cmax + 1);
+        auto interface = [&](size_t left, size_t right, double ul, double ur, double lf, double rf)
+        {
+            const double f = burgers_flux(ul, ur);
+            out[left] += f * lf;
+            out[right] += (-f) * rf;
+        };
+        for (int d = 0; d < dim; ++d)
+        {
+            std::array<int, 3> e{0, 0, 0};
+            e[size_t(d)] = 1;
+            const std::array<int, 3> me{-e[0], -e[1], -e[2]};
+            for (int level = minl; level <= maxl; ++level)
+            {
+                const double h  = 1.0 / (1 << level);
+                const double fa = h_factor(dim, h, h);
+                const DSet I    = intersection(cells[size_t(level)], translate(cells[size_t(level)], me));
+                for_each_run(I,
+                             [&](int x0, int x1, int y, int z)
+                             {
+                                 for (int x = x0; x < x1; ++x)
+                                 {
+                                     const size_t p = idx(level, x, y, z), q = idx(level, x + e[0], y + e[1], z + e[2]);
+                                     interface(p, q, s.u[p], s.u[q], fa, fa);
+                                 }
+                             });
+            }
+            for (int level = minl; level < maxl; ++level)
+            {
+                if (level >= m.max_level)
+                {
+                    break;
+                }
+                const int lf    = level + 1;
+                const double hl = 1.0 / (1 << level);
+                const double hf = 1.0 / (1 << lf);
+                const int zsh   = dim == 3 ? 1 : 0;
+                auto coarse     = [&](int x, int y, int z) { return idx(level, x >> 1, y >> 1, z >> zsh); };
+                // coarse cell on the left: stencil {p (fine ghost in the coarse cell), p + e}
+                {
+                    const double lfac = h_factor(dim, hf, hl), rfac = h_factor(dim, hf, hf);
+                    const DSet J      = on(intersection(cells[size_t(level)], translate(cells[size_t(lf)], me)), lf);
+                    for_each_run(J,
+                                 [&](int x0, int x1, int y, int z)
+                                 {
+                                     for (int x = x0; x < x1; ++x)
+                                     {
+                                         const size_t g = idx(lf, x, y, z), f = idx(lf, x + e[0], y + e[1], z + e[2]);
+                                         interface(coarse(x, y, z), f, s.u[g], s.u[f], lfac, rfac);
+                                     }
+                                 });
+                }
+                // fine cell on the left: stencil {p - e, p (fine ghost in the coarse cell)}
+                {
+                    const double lfac = h_factor(dim, hf, hf), rfac = h_factor(dim, hf, hl);
+                    const DSet J      = on(intersection(cells[size_t(level)], translate(cells[size_t(lf)], e)), lf);
+                    for_each_run(J,
+                                 [&](int x0, int x1, int y, int z)
+                                 {
+                                     for (int x = x0; x < x1; ++x)
+                                     {
+                                         const size_t f = idx(lf, x - e[0], y - e[1], z - e[2]), g = idx(lf, x, y, z);
+                                         interface(f, coarse(x, y, z), s.u[f], s.u[g], lfac, rfac);
+                                     }
+                                 });
+                }
+            }
+            // boundary interfaces: the outer ghost holds the boundary value
+            for (int level = cmin; level <= cmax; ++level)
+            {
+                const double h  = 1.0 / (1 << level);
+                const double fa = h_factor(dim, h, h);
+                const DSet& dom = m.domain[size_t(level)];
+                for_each_run(difference(cells[size_t(level)], translate(dom, me)),
+                             [&](int x0, int x1, int y, int z)
+                             {
+                                 for (int x = x0; x < x1; ++x)
+                                 {
+                                     const size_t p = idx(level, x, y, z);
+                                     out[p] += burgers_flux(s.u[p], s.u[idx(level, x + e[0], y + e[1], z + e[2])]) * fa;
+                                 }
+                             });
+                for_each_run(difference(cells[size_t(level)], translate(dom, e)),
+                             [&](int x0, int x1, int y, int z)
+                             {
+                                 for (int x = x0; x < x1; ++x)
+                                 {
+                                     const size_t p = idx(level, x, y, z);
+                                     out[p] += (-burgers_flux(s.u[idx(level, x - e[0], y - e[1], z - e[2])], s.u[p])) * fa;
+                                 }
+                             });
+            }
+        }
+        return out;
+    }
+
     Sim* make_box_sim(int dim, int min_level, int max_level, int pred_s, int radius, int grad_width)
     {
         auto* s           = new Sim();
@@ -1685,6 +1808,19 @@ extern "C"
         ORC_TRY({
             update_ghost_mr(*s);
             std::vector<double> r = convection_upwind(*s, v);
+            std::copy(r.begin(), r.end(), out);
+        });
+        return 0;
+    }
+
+    // out (storage size) = make_convection_upwind<Field>()(u) (scalar,
+    // non-linear) after update_ghost_mr(u)
+    int orc_convection_upwind_nonlinear(void* p, double* out)
+    {
+        auto* s = static_cast<Sim*>(p);
+        ORC_TRY({
+            update_ghost_mr(*s);
+            std::vector<double> r = convection_upwind_nonlinear(*s);
             std::copy(r.begin(), r.end(), out);
         });
         return 0;

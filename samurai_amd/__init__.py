@@ -110,6 +110,7 @@ def lib():
         _sig(L, "smr_batch_autotune", "argtypes", [vp, i32, ctypes.c_char_p])
         _sig(L, "smr_batch_load_tuning", "argtypes", [vp, ctypes.c_char_p])
         _sig(L, "smr_convection_upwind", "argtypes", [P(dbl), vp, vp])
+        _sig(L, "smr_convection_upwind_nonlinear", "argtypes", [vp, vp])
         _sig(L, "smr_upwind_hand_loop", "argtypes", [dbl, vp, vp])
         _sig(L, "smr_field_sub_scaled", "argtypes", [vp, dbl, vp, vp])
         _sig(L, "smr_mesh_convection_terms", "argtypes", [vp, P(dbl), P(i64), P(i64), P(i64), P(i64), P(i64), P(dbl)])
@@ -338,37 +339,44 @@ def upwind_hand_loop(dt, u, unp1):
 
 
 class ConvectionUpwind:
-    """make_convection_upwind<Field>(velocity) (schemes/fv/operators/convection_lin.hpp:15-89);
-    calling it on u returns make_convection_upwind(velocity)(u) in a new field."""
+    """make_convection_upwind<Field>(velocity) (schemes/fv/operators/convection_lin.hpp:15-89),
+    or with velocity None the non-linear make_convection_upwind<Field>()
+    (convection_nonlin.hpp:24-79, flux f(u) = u * u); calling it on u returns
+    the operator applied to u in a new field (or in `out`)."""
 
-    def __init__(self, velocity):
-        self.velocity = np.ascontiguousarray(np.asarray(velocity, dtype=np.float64).ravel())
+    def __init__(self, velocity=None):
+        self.velocity = None if velocity is None else np.ascontiguousarray(np.asarray(velocity, dtype=np.float64).ravel())
 
     def __call__(self, u, out=None):
         if out is None:
             out = make_scalar_field("convection(" + u.name + ")", u.mesh)
-        _check(lib().smr_convection_upwind(_dp(self.velocity), u.h, out.h))
+        if self.velocity is None:
+            _check(lib().smr_convection_upwind_nonlinear(u.h, out.h))
+        else:
+            _check(lib().smr_convection_upwind(_dp(self.velocity), u.h, out.h))
         return out
 
 
-def make_convection_upwind(velocity):
+def make_convection_upwind(velocity=None):
     return ConvectionUpwind(velocity)
 
 
-def convection_terms(mesh, velocity):
-    """The ordered contribution lists behind make_convection_upwind (host)."""
-    v = np.ascontiguousarray(np.asarray(velocity, dtype=np.float64).ravel())
+def convection_terms(mesh, velocity=None):
+    """The ordered contribution lists behind make_convection_upwind (host); the
+    non-linear operator's with velocity None."""
+    v = None if velocity is None else np.ascontiguousarray(np.asarray(velocity, dtype=np.float64).ravel())
     i64 = ctypes.c_int64
     nr, nt = i64(0), i64(0)
     null = ctypes.POINTER(i64)()
-    _check(lib().smr_mesh_convection_terms(mesh.h, _dp(v), ctypes.byref(nr), ctypes.byref(nt), null, null, null,
+    vp = _dp(v) if v is not None else ctypes.POINTER(ctypes.c_double)()
+    _check(lib().smr_mesh_convection_terms(mesh.h, vp, ctypes.byref(nr), ctypes.byref(nt), null, null, null,
                                            ctypes.POINTER(ctypes.c_double)()))
     row = np.zeros(nr.value, dtype=np.int64)
     ptr = np.zeros(nr.value + 1, dtype=np.int64)
     src = np.zeros(2 * nt.value, dtype=np.int64)
     coef = np.zeros(nt.value)
     P64 = ctypes.POINTER(i64)
-    _check(lib().smr_mesh_convection_terms(mesh.h, _dp(v), ctypes.byref(nr), ctypes.byref(nt), row.ctypes.data_as(P64),
+    _check(lib().smr_mesh_convection_terms(mesh.h, vp, ctypes.byref(nr), ctypes.byref(nt), row.ctypes.data_as(P64),
                                            ptr.ctypes.data_as(P64), src.ctypes.data_as(P64), _dp(coef)))
     return row, ptr, src, coef
 

@@ -963,6 +963,17 @@ extern "C"
             });
     }
 
+    int smr_convection_upwind_nonlinear(smr_field* u, smr_field* out)
+    {
+        return guarded(
+            [&]
+            {
+                require(u && out, "null argument");
+                smr::convection_upwind_nonlinear(u->f, out->f);
+                return 0;
+            });
+    }
+
     int smr_convection_upwind(const double* velocity, smr_field* u, smr_field* out)
     {
         return guarded(
@@ -980,8 +991,9 @@ extern "C"
         return guarded(
             [&]
             {
-                require(mesh && velocity && nrows && nterms, "null argument");
-                const smr::ConvTerms t = smr::build_convection_terms(mesh->m.host, velocity);
+                require(mesh && nrows && nterms, "null argument");
+                const smr::ConvTerms t =
+                    velocity ? smr::build_convection_terms(mesh->m.host, velocity) : smr::build_burgers_terms(mesh->m.host);
                 const bool fill        = row && ptr && src && coef;
                 if (fill)
                 {

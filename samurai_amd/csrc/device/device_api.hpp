@@ -154,6 +154,10 @@ namespace smr::dev
     // out[row[r]] = ordered sum of the terms of row r (flux-based convection)
     void conv_gather(const int64_t* row, const int64_t* ptr, const int64_t* src, const double* coef, int64_t nrows, const double* u,
                      double* out, hipStream_t s);
+    // the same with term k = coef[k] * F(u[src[2k]], u[src[2k+1]]), F the
+    // scalar non-linear upwind flux of f(u) = u * u
+    void conv_burgers_gather(const int64_t* row, const int64_t* ptr, const int64_t* src, const double* coef, int64_t nrows, const double* u,
+                     double* out, hipStream_t s);
     // launch shape of the upwind kernel (cells per thread, XCD remap, stores)
     int upwind_variant_count();
     void set_upwind_variant(int id);

@@ -136,6 +136,31 @@ namespace smr::dev
             out[row[r]] = acc;
         }
 
+        // scalar non-linear upwind flux of f(u) = u * u
+        // (schemes/fv/operators/convection_nonlin.hpp:54-67)
+        __device__ __forceinline__ double burgers_flux(double ul, double ur)
+        {
+            const double v = 0.5 * (ul + ur);
+            return v >= 0 ? ul * ul : ur * ur;
+        }
+
+        __global__ void k_conv_burgers_gather(const int64_t* __restrict__ row, const int64_t* __restrict__ ptr,
+                                              const int64_t* __restrict__ src, const double* __restrict__ coef, int64_t nrows,
+                                              const double* __restrict__ u, double* __restrict__ out)
+        {
+            const int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+            if (r >= nrows)
+            {
+                return;
+            }
+            double acc = 0;
+            for (int64_t k = ptr[r]; k < ptr[r + 1]; ++k)
+            {
+                acc += burgers_flux(u[src[2 * k]], u[src[2 * k + 1]]) * coef[k];
+            }
+            out[row[r]] = acc;
+        }
+
         __device__ __forceinline__ int64_t readlane64(int64_t v, int lane)
         {
             const uint64_t w  = uint64_t(v);
@@ -573,6 +598,15 @@ namespace smr::dev
         if (nrows > 0)
         {
             hipLaunchKernelGGL(k_conv_gather, dim3(nblocks(nrows)), dim3(BLOCK), 0, s, row, ptr, src, coef, nrows, u, out);
+        }
+    }
+
+    void conv_burgers_gather(const int64_t* row, const int64_t* ptr, const int64_t* src, const double* coef, int64_t nrows, const double* u,
+                     double* out, hipStream_t s)
+    {
+        if (nrows > 0)
+        {
+            hipLaunchKernelGGL(k_conv_burgers_gather, dim3(nblocks(nrows)), dim3(BLOCK), 0, s, row, ptr, src, coef, nrows, u, out);
         }
     }
 

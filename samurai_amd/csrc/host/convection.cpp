@@ -25,6 +25,49 @@ namespace smr
             return b;
         }
 
+        // group by output slot, keeping the order of each slot's terms
+        ConvTerms group_events(const std::vector<Event>& ev, int64_t n, bool two_sources)
+        {
+            std::vector<int64_t> cnt(size_t(n) + 1, 0);
+            for (const Event& x : ev)
+            {
+                if (x.out < 0 || x.out >= n || x.s0 < 0 || x.s0 >= n || x.s1 >= n || (two_sources && x.s1 < 0))
+                {
+                    throw std::out_of_range("convection: a term is outside the field storage");
+                }
+                ++cnt[size_t(x.out) + 1];
+            }
+            ConvTerms t;
+            std::vector<int64_t> start(size_t(n) + 1, 0);
+            for (int64_t i = 0; i < n; ++i)
+            {
+                start[size_t(i) + 1] = start[size_t(i)] + cnt[size_t(i) + 1];
+                if (cnt[size_t(i) + 1] > 0)
+                {
+                    t.row.push_back(i);
+                }
+            }
+            const size_t nt = ev.size();
+            t.src.assign(2 * nt, -1);
+            t.coef.assign(nt, 0.0);
+            std::vector<int64_t> pos(start.begin(), start.end() - 1);
+            for (const Event& x : ev)
+            {
+                const int64_t k          = pos[size_t(x.out)]++;
+                t.src[size_t(2 * k)]     = x.s0;
+                t.src[size_t(2 * k + 1)] = x.s1;
+                t.coef[size_t(k)]        = x.c;
+            }
+            t.ptr.clear();
+            t.ptr.reserve(t.row.size() + 1);
+            for (int64_t r : t.row)
+            {
+                t.ptr.push_back(start[size_t(r)]);
+            }
+            t.ptr.push_back(int64_t(nt));
+            return t;
+        }
+
         // h_factor (flux_based_scheme__lin_hom.hpp:62-67)
         double h_factor(int dim, double h_face, double h_cell)
         {
@@ -227,45 +270,116 @@ namespace smr
             }
         }
 
-        // group by output slot, keeping the order of each slot's terms
-        const int64_t n = m.nref;
-        std::vector<int64_t> cnt(size_t(n) + 1, 0);
-        for (const Event& x : ev)
+        return group_events(ev, m.nref, false);
+    }
+
+    ConvTerms build_burgers_terms(const MRMeshData& m)
+    {
+        const int dim     = m.cfg.dim;
+        const auto& cells = m.ids[CELLS];
+        const int cmin = ca_min_level(cells), cmax = ca_max_level(cells);
+        const int minl = std::max(m.cfg.min_level, cmin - 1), maxl = std::min(m.cfg.max_level, cmax + 1);
+        const int zs   = dim == 3 ? 1 : 0;
+        std::vector<Event> ev;
+        // one interface: flux * lf into the left cell, (-flux) * rf = flux * (-rf) into the right one
+        auto iface = [&](int64_t left, int64_t right, int64_t sl, int64_t sr, double lf, double rf)
         {
-            if (x.out < 0 || x.out >= n || x.s0 < 0 || x.s0 >= n || x.s1 >= n)
+            ev.push_back({left, sl, sr, lf});
+            ev.push_back({right, sl, sr, -rf});
+        };
+        for (int d = 0; d < dim; ++d)
+        {
+            std::array<int, 3> e{0, 0, 0};
+            e[size_t(d)] = 1;
+            const std::array<int, 3> me{-e[0], -e[1], -e[2]};
+            for (int level = minl; level <= maxl; ++level)
             {
-                throw std::out_of_range("convection: a term is outside the field storage");
+                const double h  = 1.0 / double(1 << level);
+                const double fa = h_factor(dim, h, h);
+                set_intersection(cells[size_t(level)], translate(cells[size_t(level)], me))
+                    .for_each_run(
+                        [&](int32_t x0, int32_t x1, int32_t y, int32_t z)
+                        {
+                            const int64_t bp = base_of(m, level, x0, y, z);
+                            const int64_t bq = base_of(m, level, x0 + e[0], y + e[1], z + e[2]) + e[0];
+                            for (int32_t x = x0; x < x1; ++x)
+                            {
+                                iface(bp + x, bq + x, bp + x, bq + x, fa, fa);
+                            }
+                        });
             }
-            ++cnt[size_t(x.out) + 1];
-        }
-        ConvTerms t;
-        std::vector<int64_t> start(size_t(n) + 1, 0);
-        for (int64_t i = 0; i < n; ++i)
-        {
-            start[size_t(i) + 1] = start[size_t(i)] + cnt[size_t(i) + 1];
-            if (cnt[size_t(i) + 1] > 0)
+            for (int level = minl; level < maxl && level < m.cfg.max_level; ++level)
             {
-                t.row.push_back(i);
+                const int lf     = level + 1;
+                const double hl  = 1.0 / double(1 << level);
+                const double hf  = 1.0 / double(1 << lf);
+                auto coarse_base = [&](int32_t x0, int32_t y, int32_t z) { return base_of(m, level, x0 >> 1, y >> 1, z >> zs); };
+                // coarse cell on the left: stencil {p (fine ghost in the coarse cell), p + e}
+                {
+                    const double lfac = h_factor(dim, hf, hl), rfac = h_factor(dim, hf, hf);
+                    on(set_intersection(cells[size_t(level)], translate(cells[size_t(lf)], me)), lf)
+                        .for_each_run(
+                            [&](int32_t x0, int32_t x1, int32_t y, int32_t z)
+                            {
+                                const int64_t bg = base_of(m, lf, x0, y, z);
+                                const int64_t bf = base_of(m, lf, x0 + e[0], y + e[1], z + e[2]) + e[0];
+                                for (int32_t x = x0; x < x1; ++x)
+                                {
+                                    iface(coarse_base(x, y, z) + (x >> 1), bf + x, bg + x, bf + x, lfac, rfac);
+                                }
+                            });
+                }
+                // fine cell on the left: stencil {p - e, p (fine ghost in the coarse cell)}
+                {
+                    const double lfac = h_factor(dim, hf, hf), rfac = h_factor(dim, hf, hl);
+                    on(set_intersection(cells[size_t(level)], translate(cells[size_t(lf)], e)), lf)
+                        .for_each_run(
+                            [&](int32_t x0, int32_t x1, int32_t y, int32_t z)
+                            {
+                                const int64_t bg = base_of(m, lf, x0, y, z);
+                                const int64_t bf = base_of(m, lf, x0 - e[0], y - e[1], z - e[2]) - e[0];
+                                for (int32_t x = x0; x < x1; ++x)
+                                {
+                                    iface(bf + x, coarse_base(x, y, z) + (x >> 1), bf + x, bg + x, lfac, rfac);
+                                }
+                            });
+                }
+            }
+            // boundary interfaces: the outer ghost holds the boundary value
+            for (int level = cmin; level <= cmax; ++level)
+            {
+                const LevelSet& cl = cells[size_t(level)];
+                if (cl.empty())
+                {
+                    continue;
+                }
+                const double h  = 1.0 / double(1 << level);
+                const double fa = h_factor(dim, h, h);
+                const int n     = 1 << level;
+                minus_box(cl, me, {n + me[0], n + me[1], n + me[2]})
+                    .for_each_run(
+                        [&](int32_t x0, int32_t x1, int32_t y, int32_t z)
+                        {
+                            const int64_t bp = base_of(m, level, x0, y, z);
+                            const int64_t bo = base_of(m, level, x0 + e[0], y + e[1], z + e[2]) + e[0];
+                            for (int32_t x = x0; x < x1; ++x)
+                            {
+                                ev.push_back({bp + x, bp + x, bo + x, fa});
+                            }
+                        });
+                minus_box(cl, e, {n + e[0], n + e[1], n + e[2]})
+                    .for_each_run(
+                        [&](int32_t x0, int32_t x1, int32_t y, int32_t z)
+                        {
+                            const int64_t bp = base_of(m, level, x0, y, z);
+                            const int64_t bo = base_of(m, level, x0 - e[0], y - e[1], z - e[2]) - e[0];
+                            for (int32_t x = x0; x < x1; ++x)
+                            {
+                                ev.push_back({bp + x, bo + x, bp + x, -fa});
+                            }
+                        });
             }
         }
-        const size_t nt = ev.size();
-        t.src.assign(2 * nt, -1);
-        t.coef.assign(nt, 0.0);
-        std::vector<int64_t> pos(start.begin(), start.end() - 1);
-        for (const Event& x : ev)
-        {
-            const int64_t k       = pos[size_t(x.out)]++;
-            t.src[size_t(2 * k)]     = x.s0;
-            t.src[size_t(2 * k + 1)] = x.s1;
-            t.coef[size_t(k)]        = x.c;
-        }
-        t.ptr.clear();
-        t.ptr.reserve(t.row.size() + 1);
-        for (int64_t r : t.row)
-        {
-            t.ptr.push_back(start[size_t(r)]);
-        }
-        t.ptr.push_back(int64_t(nt));
-        return t;
+        return group_events(ev, m.nref, true);
     }
 } // namespace smr

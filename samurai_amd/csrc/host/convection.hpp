@@ -31,4 +31,14 @@ namespace smr
     // Terms with a zero coefficient are left out: adding 0 * u to a sum that
     // started at +0 never changes it (for finite u).
     ConvTerms build_convection_terms(const MRMeshData& m, const double* velocity);
+
+    // make_convection_upwind<Field>()(u) for a scalar field, the non-linear
+    // upwind of f(u) = u * u (schemes/fv/operators/convection_nonlin.hpp:24-79,
+    // explicit_flux_based_scheme__nonlin.hpp:35-86): the same interfaces in the
+    // same order, each adding flux * left_factor to its left cell and
+    // (-flux) * right_factor to its right cell
+    // (flux_based_scheme__nonlin.hpp:334-398, 408-590).  Term k is
+    // coef[k] * F(u[src[2k]], u[src[2k+1]]) with F(l, r) = (0.5 * (l + r) >= 0 ?
+    // l * l : r * r); coef carries the sign, (-F) * rf being F * (-rf) exactly.
+    ConvTerms build_burgers_terms(const MRMeshData& m);
 } // namespace smr

@@ -943,6 +943,47 @@ namespace smr
         dev::conv_gather(dm.conv->row, dm.conv->ptr, dm.conv->src, dm.conv->coef, dm.conv->nrows, u.d, out.d, s);
     }
 
+    namespace
+    {
+        std::unique_ptr<DevMesh::Conv> upload_terms(const ConvTerms& t, hipStream_t s)
+        {
+            auto c            = std::make_unique<DevMesh::Conv>();
+            c->blob           = std::make_unique<Blob>();
+            const size_t orow = c->blob->add(t.row.data(), t.row.size() * 8);
+            const size_t optr = c->blob->add(t.ptr.data(), t.ptr.size() * 8);
+            const size_t osrc = c->blob->add(t.src.data(), t.src.size() * 8);
+            const size_t ocof = c->blob->add(t.coef.data(), t.coef.size() * 8);
+            c->blob->upload(s);
+            c->row   = reinterpret_cast<const int64_t*>(c->blob->dptr(orow));
+            c->ptr   = reinterpret_cast<const int64_t*>(c->blob->dptr(optr));
+            c->src   = reinterpret_cast<const int64_t*>(c->blob->dptr(osrc));
+            c->coef  = reinterpret_cast<const double*>(c->blob->dptr(ocof));
+            c->nrows = int64_t(t.row.size());
+            return c;
+        }
+    } // namespace
+
+    void convection_upwind_nonlinear(Field& u, Field& out)
+    {
+        u.check_current();
+        out.check_current();
+        if (u.mesh != out.mesh || &u == &out)
+        {
+            throw std::invalid_argument("convection: input and output must be two fields on one mesh");
+        }
+        Mesh& m = *u.mesh;
+        update_ghost_mr(u);
+        DevMesh& dm = m.device();
+        ScopedTimer timer(m.ctx, "convection operator");
+        hipStream_t s = m.ctx->stream;
+        if (!dm.burgers)
+        {
+            dm.burgers = upload_terms(build_burgers_terms(m.host), s);
+        }
+        hip_check(hipMemsetAsync(out.d, 0, size_t(out.size) * 8, s), "hipMemsetAsync(convection)");
+        dev::conv_burgers_gather(dm.burgers->row, dm.burgers->ptr, dm.burgers->src, dm.burgers->coef, dm.burgers->nrows, u.d, out.d, s);
+    }
+
     // ------------------------------------------------------------------
     // adaptation (mr/adapt.hpp:147-389)
     // ------------------------------------------------------------------

@@ -130,6 +130,8 @@ namespace smr
             int64_t nrows      = 0;
         };
         std::unique_ptr<Conv> conv;
+        // the non-linear (Burgers) convection terms of this image
+        std::unique_ptr<Conv> burgers;
         // the work lists of a complete image (cells first)
         std::vector<DevWorkList*> lists() { return {&cells, &projection, &prediction, &detail, &criteria, &maximum}; }
     };
@@ -259,6 +261,8 @@ namespace smr
     void upwind_hand_loop(double dt, Field& u, Field& unp1);
     // out = make_convection_upwind(v)(u), after update_ghost_mr(u)
     void convection_upwind(const double* v, Field& u, Field& out);
+    // out = make_convection_upwind<Field>()(u) (scalar, non-linear), after update_ghost_mr(u)
+    void convection_upwind_nonlinear(Field& u, Field& out);
     // Adapt::operator() (mr/adapt.hpp:147-195); returns the number of harten calls
     int adapt(AdaptState& st, double eps, double regularity, int max_iterations = -1);
     // one harten call; true at the fixed point

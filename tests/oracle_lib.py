@@ -45,6 +45,7 @@ def lib():
         L.orc_upwind_step.argtypes = [vp, P(dbl), dbl]
         L.orc_upwind_only.argtypes = [vp, P(dbl), dbl]
         L.orc_convection_upwind.argtypes = [vp, P(dbl), P(dbl)]
+        L.orc_convection_upwind_nonlinear.argtypes = [vp, P(dbl)]
         L.orc_upwind_hand_loop.argtypes = [vp, dbl]
         _lib = L
     return _lib
@@ -139,6 +140,12 @@ class OracleSim:
     def upwind_only(self, a, dt):
         a = np.ascontiguousarray(a, dtype=np.float64)
         _check(lib().orc_upwind_only(self.h, _dp(a), float(dt)))
+
+    def convection_upwind_nonlinear(self):
+        """make_convection_upwind<Field>()(u) (scalar Burgers flux u*u) after update_ghost_mr(u): the output storage."""
+        out = np.zeros(self.nb_cells(REFERENCE))
+        _check(lib().orc_convection_upwind_nonlinear(self.h, _dp(out)))
+        return out
 
     def convection_upwind(self, v):
         """make_convection_upwind(v)(u) after update_ghost_mr(u): the output storage."""

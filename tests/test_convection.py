@@ -123,3 +123,113 @@ def test_host_terms_reproduce_oracle(host_ctx, dim, lmin, lmax, v):
     assert m.nb_cells(sa.REFERENCE) == o.nb_cells(REFERENCE)
     got = evaluate_terms(sa.convection_terms(m, np.array(v)), u, len(u))
     assert np.array_equal(got.view(np.uint64), ref.view(np.uint64))
+
+
+# ---------------------------------------------------------------------------
+# non-linear make_convection_upwind<Field>() (convection_nonlin.hpp:24-79):
+# flux f(u) = u * u of the side picked by the sign of the mean value.  The
+# reference's tests hold no known answer for it; the oracle is pinned by
+# closed forms derived here and by the invariants above.
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("sign", [1.0, -1.0])
+def test_nonlinear_closed_form_on_linear_fields(dim, sign):
+    """u = sign * (3 + sum_d b_d x_d), never zero.  With u > 0 every flux is
+    u_left^2, so on an interior cell the x_d part is (u_i^2 - u_{i-1}^2) / h =
+    b_d (2 u_i - b_d h); with u < 0 it is (u_{i+1}^2 - u_i^2) / h =
+    b'_d (2 u_i + b'_d h), b' = sign * b."""
+    level = 3
+    o = uniform(dim, level)
+    cells = o.cells()
+    b = np.array([d + 2.0 for d in range(dim)])
+    c = centers(cells)[:, :dim]
+    u = sign * (3.0 + c @ b)
+    o.set_values(u)
+    o.set_dirichlet(0.0)
+    out = o.convection_upwind_nonlinear()[cell_slots(o)]
+    h = 1.0 / (1 << level)
+    bs = sign * b
+    expected = sum(bs[d] * (2 * u - sign * bs[d] * h) for d in range(dim))
+    n = 1 << level
+    interior = np.all((cells[:, 1:1 + dim] >= 1) & (cells[:, 1:1 + dim] <= n - 2), axis=1)
+    assert interior.sum() > 0
+    assert np.allclose(out[interior], expected[interior], rtol=0, atol=1e-10)
+
+
+def test_nonlinear_upwind_side_follows_the_mean():
+    """One row of a uniform 2D mesh, u = +1 on x < 1/2 and -3 elsewhere: at
+    the jump the mean is -1 < 0, so the flux is (-3)^2 = 9 and the cell left
+    of the jump (x index 3 of 8) gets (9 - 1) / h in x; a mean >= 0 (u = +3
+    then -1) takes the left value 3^2 = 9 again, giving 0 there."""
+    level = 3
+    h = 1.0 / (1 << level)
+    for left, right, want in ((1.0, -3.0, (9.0 - 1.0) / h), (3.0, -1.0, 0.0)):
+        o = uniform(2, level)
+        cells = o.cells()
+        u = np.where(cells[:, 1] < 4, left, right).astype(np.float64)
+        o.set_values(u)
+        o.set_dirichlet(0.0)
+        out = o.convection_upwind_nonlinear()[cell_slots(o)]
+        sel = (cells[:, 1] == 3) & (cells[:, 2] == 4)
+        # y fluxes cancel on the interior row (u constant along y there)
+        assert out[sel][0] == want
+
+
+@pytest.mark.parametrize("dim,lmin,lmax", [(2, 2, 7), (2, 3, 8), (3, 2, 5)])
+def test_nonlinear_conservation_on_adapted_meshes(dim, lmin, lmax):
+    field = field_2d if dim == 2 else (lambda c: 0.5 * sphere_3d(c, radius=0.2))
+    o = adapted(dim, lmin, lmax, field)
+    cells = o.cells()
+    assert len(np.unique(cells[:, 0])) > 1, "the mesh must hold level jumps"
+    # mixed signs so that both upwind sides occur
+    o.set_values(field(cells) - 0.3 * field(cells[:, [0, 2, 1, 3, 4]]))
+    out = o.convection_upwind_nonlinear()[cell_slots(o)]
+    area = np.ldexp(1.0, -dim * cells[:, 0].astype(np.int64))
+    total = float(np.sum(area * out))
+    scale = float(np.sum(area * np.abs(out)))
+    assert scale > 0
+    assert abs(total) <= 1e-12 * scale
+
+
+@pytest.mark.parametrize("dim,lmin,lmax", [(2, 2, 7), (3, 2, 5)])
+def test_nonlinear_constant_field_gives_zero_away_from_boundary(dim, lmin, lmax):
+    field = field_2d if dim == 2 else (lambda c: 0.5 * sphere_3d(c, radius=0.2))
+    o = adapted(dim, lmin, lmax, field)
+    o.set_values(np.full(o.nb_cells(), -1.0))
+    out = o.convection_upwind_nonlinear()[cell_slots(o)]
+    cells = o.cells()
+    n = np.left_shift(1, cells[:, 0].astype(np.int64))
+    inner = np.all((cells[:, 1:1 + dim] >= 4) & (cells[:, 1:1 + dim] <= (n - 5)[:, None]), axis=1)
+    assert inner.sum() > 0
+    if dim == 2:
+        assert np.all(out[inner] == 0.0)
+    else:
+        scale = np.ldexp(1.0, cells[:, 0].astype(np.int64))
+        assert np.all(np.abs(out[inner]) <= 1e-13 * scale[inner])
+
+
+def evaluate_burgers_terms(terms, u, nref):
+    row, ptr, src, coef = terms
+    out = np.zeros(nref)
+    for r in range(len(row)):
+        acc = 0.0
+        for k in range(ptr[r], ptr[r + 1]):
+            ul, ur = u[src[2 * k]], u[src[2 * k + 1]]
+            f = ul * ul if 0.5 * (ul + ur) >= 0 else ur * ur
+            acc += f * coef[k]
+        out[row[r]] = acc
+    return out
+
+
+@pytest.mark.parametrize("dim,lmin,lmax", [(2, 2, 7), (2, 3, 7), (3, 2, 5)])
+def test_host_burgers_terms_reproduce_oracle(host_ctx, dim, lmin, lmax):
+    field = field_2d if dim == 2 else (lambda c: 0.5 * sphere_3d(c, radius=0.2))
+    o = adapted(dim, lmin, lmax, field)
+    cells = o.cells()
+    o.set_values(field(cells) - 0.4 * field(cells[:, [0, 2, 1, 3, 4]]))
+    ref = o.convection_upwind_nonlinear()
+    u = o.storage()
+    m = sa.make_mesh_from_cells(host_ctx, sa.mesh_config(dim, lmin, lmax, 1), cells)
+    got = evaluate_burgers_terms(sa.convection_terms(m), u, len(u))
+    assert np.array_equal(got.view(np.uint64), ref.view(np.uint64))

@@ -197,6 +197,59 @@ def test_convection_upwind_matches_oracle(gpu_ctx, dim, lmin, lmax, v):
     assert np.array_equal(bits(out2.storage()), bits(ref))
 
 
+def mixed_sign(o, dim):
+    """A field with both signs on the oracle's cells: the disc (sphere) minus
+    a shifted copy, so that both upwind sides of the Burgers flux occur."""
+    c = o.cells()
+    f = disc_2d if dim == 2 else sphere_3d
+    shifted = c.copy()
+    shifted[:, 1] = c[:, 2]
+    shifted[:, 2] = c[:, 1]
+    return f(c) - 0.6 * f(shifted)
+
+
+@pytest.mark.parametrize("dim,lmin,lmax", [(2, 2, 7), (2, 3, 8), (3, 2, 5)])
+def test_convection_nonlinear_matches_oracle(gpu_ctx, dim, lmin, lmax):
+    """make_convection_upwind<Field>()(u) (scalar, flux u * u) on adapted meshes."""
+    say("convection nonlinear", dim, lmin, lmax)
+    o, mesh, u = pair(gpu_ctx, dim, lmin, lmax, 1, adapt_eps=1e-3)
+    v = mixed_sign(o, dim)
+    o.set_values(v)
+    u.set_cells(v)
+    ref = o.convection_upwind_nonlinear()
+    conv = sa.make_convection_upwind()
+    got = conv(u).storage()
+    assert np.array_equal(bits(got), bits(ref))
+    assert np.array_equal(bits(conv(u).storage()), bits(ref))
+
+
+@pytest.mark.parametrize("dim,lmin,lmax", [(2, 3, 8), (3, 2, 5)])
+def test_burgers_steps_match_oracle(gpu_ctx, dim, lmin, lmax):
+    """Adapt + unp1 = u - dt * make_convection_upwind<Field>()(u), four steps,
+    bit for bit (the time loop of demos/FiniteVolume/burgers.cpp, scalar)."""
+    say("burgers steps", dim, lmin, lmax)
+    o, mesh, u = pair(gpu_ctx, dim, lmin, lmax, 1)
+    v = mixed_sign(o, dim)
+    o.set_values(v)
+    u.set_cells(v)
+    ad = sa.make_MRAdapt(u)
+    conv = sa.make_convection_upwind()
+    f = sa.make_scalar_field("f", mesh)
+    unp1 = sa.make_scalar_field("unp1", mesh)
+    dt = 0.25 / (1 << lmax)
+    for step in range(4):
+        assert o.adapt(1e-3, 2.0) == ad(1e-3, 2.0)
+        assert np.array_equal(o.cells(), mesh.cells()), step
+        slots = o.cells()[:, 4].astype(np.int64)
+        o.set_values(o.values() - dt * o.convection_upwind_nonlinear()[slots])
+        f.resize()
+        unp1.resize()
+        conv(u, out=f)
+        sa.sub_scaled(u, dt, f, unp1)
+        sa.swap(u, unp1)
+        assert np.array_equal(bits(o.values()), bits(u.cells())), step
+
+
 @pytest.mark.parametrize("lmin,lmax", [(5, 5), (2, 7)])
 def test_hand_loop_matches_oracle(gpu_ctx, lmin, lmax):
     """The README's for_each_interval upwind loop."""
