@@ -195,6 +195,14 @@ int smr_convection_upwind(const double* velocity, smr_field* u, smr_field* out);
  * (explicit_flux_based_scheme__nonlin.hpp:35-86) over the same interfaces as
  * smr_convection_upwind; u's ghosts are updated first. */
 int smr_convection_upwind_nonlinear(smr_field* u, smr_field* out);
+/* make_convection_upwind<Field>(velocity_field)(u) for a scalar field
+ * (schemes/fv/operators/convection_lin.hpp:184-264): the velocity field is
+ * given as dim scalar fields on u's mesh (component d in velocity[d], each
+ * with its own boundary condition); at an interface the velocity is the mean
+ * of the two stencil cells' values, and the upwind side's value is carried
+ * (explicit_flux_based_scheme__lin_het.hpp:35-110).  The ghosts of the
+ * velocity components, then of u, are updated first (FV_scheme.hpp:176-196). */
+int smr_convection_upwind_field(smr_field* const* velocity, smr_field* u, smr_field* out);
 /* Diagnostic: the ordered contribution lists behind smr_convection_upwind,
  * computed on the host.  Call with null arrays to get *nrows and *nterms;
  * row: nrows storage slots, ptr: nrows+1, src: 2*nterms (second -1 for a

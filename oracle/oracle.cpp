@@ -1553,6 +1553,129 @@ namespace oracle
         return out;
     }
 
+    // make_convection_upwind<Field>(velocity_field)(u) for a scalar field
+    // (schemes/fv/operators/convection_lin.hpp:184-264, the LinearHeterogeneous
+    // flux): at each interface the velocity is 0.5 * (v[left] + v[right]) of the
+    // stencil cells' velocity values (ghosts included, updated beforehand as
+    // FV_scheme.hpp:176-196 does for the parameter field), and the coefficient
+    // of the upwind side is factor * v_d.  Applied explicitly
+    // (explicit_flux_based_scheme__lin_het.hpp:35-110) cell by cell over the
+    // same interfaces as the other operators (flux_based_scheme__lin_het.hpp:
+    // 75-250): the left cell gets (lf * v_d) * u, the right cell
+    // (-(rf * v_d)) * u of the upwind stencil value; the zero coefficient of the
+    // other side adds nothing.  vel[d] is the storage of component d.
+    std::vector<double> convection_upwind_heterogeneous(const Sim& s, const double* const* vel)
+    {
+        const Mesh& m = s.mesh;
+        const int dim = m.dim;
+        std::vector<double> out(size_t(m.nref), 0.0);
+        auto idx = [&](int l, int x, int y, int z) { return size_t(m.index(l, x, y, z)); };
+        const auto& cells = m.ids[CELLS];
+        const int cmin = m.ca_min_level(CELLS), cmax = m.ca_max_level(CELLS);
+        const int minl = std::max(m.min_level, cmin - 1), maxl = std::min(m.max_level, cmax + 1);
+        for (int d = 0; d < dim; ++d)
+        {
+            const double* v = vel[d];
+            // the coefficient of the upwind side and that side's value
+            auto upwind = [&](size_t sl, size_t sr, double factor, double& value)
+            {
+                const double vd = 0.5 * (v[sl] + v[sr]);
+                value           = vd >= 0 ? s.u[sl] : s.u[sr];
+                return factor * vd;
+            };
+            auto interface = [&](size_t left, size_t right, size_t sl, size_t sr, double lf, double rf)
+            {
+                double value = 0;
+                const double cl = upwind(sl, sr, lf, value);
+                const double cr = -upwind(sl, sr, rf, value);
+                out[left] += cl * value;
+                out[right] += cr * value;
+            };
+            std::array<int, 3> e{0, 0, 0};
+            e[size_t(d)] = 1;
+            const std::array<int, 3> me{-e[0], -e[1], -e[2]};
+            for (int level = minl; level <= maxl; ++level)
+            {
+                const double h  = 1.0 / (1 << level);
+                const double fa = h_factor(dim, h, h);
+                for_each_run(intersection(cells[size_t(level)], translate(cells[size_t(level)], me)),
+                             [&](int x0, int x1, int y, int z)
+                             {
+                                 for (int x = x0; x < x1; ++x)
+                                 {
+                                     const size_t p = idx(level, x, y, z), q = idx(level, x + e[0], y + e[1], z + e[2]);
+                                     interface(p, q, p, q, fa, fa);
+                                 }
+                             });
+            }
+            for (int level = minl; level < maxl; ++level)
+            {
+                if (level >= m.max_level)
+                {
+                    break;
+                }
+                const int lf    = level + 1;
+                const double hl = 1.0 / (1 << level);
+                const double hf = 1.0 / (1 << lf);
+                const int zsh   = dim == 3 ? 1 : 0;
+                auto coarse     = [&](int x, int y, int z) { return idx(level, x >> 1, y >> 1, z >> zsh); };
+                {
+                    const double lfac = h_factor(dim, hf, hl), rfac = h_factor(dim, hf, hf);
+                    for_each_run(on(intersection(cells[size_t(level)], translate(cells[size_t(lf)], me)), lf),
+                                 [&](int x0, int x1, int y, int z)
+                                 {
+                                     for (int x = x0; x < x1; ++x)
+                                     {
+                                         const size_t g = idx(lf, x, y, z), f = idx(lf, x + e[0], y + e[1], z + e[2]);
+                                         interface(coarse(x, y, z), f, g, f, lfac, rfac);
+                                     }
+                                 });
+                }
+                {
+                    const double lfac = h_factor(dim, hf, hf), rfac = h_factor(dim, hf, hl);
+                    for_each_run(on(intersection(cells[size_t(level)], translate(cells[size_t(lf)], e)), lf),
+                                 [&](int x0, int x1, int y, int z)
+                                 {
+                                     for (int x = x0; x < x1; ++x)
+                                     {
+                                         const size_t f = idx(lf, x - e[0], y - e[1], z - e[2]), g = idx(lf, x, y, z);
+                                         interface(f, coarse(x, y, z), f, g, lfac, rfac);
+                                     }
+                                 });
+                }
+            }
+            for (int level = cmin; level <= cmax; ++level)
+            {
+                const double h  = 1.0 / (1 << level);
+                const double fa = h_factor(dim, h, h);
+                const DSet& dom = m.domain[size_t(level)];
+                for_each_run(difference(cells[size_t(level)], translate(dom, me)),
+                             [&](int x0, int x1, int y, int z)
+                             {
+                                 for (int x = x0; x < x1; ++x)
+                                 {
+                                     const size_t p = idx(level, x, y, z);
+                                     double value   = 0;
+                                     const double c = upwind(p, idx(level, x + e[0], y + e[1], z + e[2]), fa, value);
+                                     out[p] += c * value;
+                                 }
+                             });
+                for_each_run(difference(cells[size_t(level)], translate(dom, e)),
+                             [&](int x0, int x1, int y, int z)
+                             {
+                                 for (int x = x0; x < x1; ++x)
+                                 {
+                                     const size_t p = idx(level, x, y, z);
+                                     double value   = 0;
+                                     const double c = upwind(idx(level, x - e[0], y - e[1], z - e[2]), p, -fa, value);
+                                     out[p] += c * value;
+                                 }
+                             });
+            }
+        }
+        return out;
+    }
+
     Sim* make_box_sim(int dim, int min_level, int max_level, int pred_s, int radius, int grad_width)
     {
         auto* s           = new Sim();
@@ -1821,6 +1944,20 @@ extern "C"
         ORC_TRY({
             update_ghost_mr(*s);
             std::vector<double> r = convection_upwind_nonlinear(*s);
+            std::copy(r.begin(), r.end(), out);
+        });
+        return 0;
+    }
+
+    // out = make_convection_upwind<Field>(velocity_field)(u) after
+    // update_ghost_mr(u); vel: dim storages of the velocity components, ghosts
+    // already updated
+    int orc_convection_upwind_heterogeneous(void* p, const double* const* vel, double* out)
+    {
+        auto* s = static_cast<Sim*>(p);
+        ORC_TRY({
+            update_ghost_mr(*s);
+            std::vector<double> r = convection_upwind_heterogeneous(*s, vel);
             std::copy(r.begin(), r.end(), out);
         });
         return 0;

@@ -111,6 +111,7 @@ def lib():
         _sig(L, "smr_batch_load_tuning", "argtypes", [vp, ctypes.c_char_p])
         _sig(L, "smr_convection_upwind", "argtypes", [P(dbl), vp, vp])
         _sig(L, "smr_convection_upwind_nonlinear", "argtypes", [vp, vp])
+        _sig(L, "smr_convection_upwind_field", "argtypes", [ctypes.POINTER(vp), vp, vp])
         _sig(L, "smr_upwind_hand_loop", "argtypes", [dbl, vp, vp])
         _sig(L, "smr_field_sub_scaled", "argtypes", [vp, dbl, vp, vp])
         _sig(L, "smr_mesh_convection_terms", "argtypes", [vp, P(dbl), P(i64), P(i64), P(i64), P(i64), P(i64), P(dbl)])
@@ -340,17 +341,27 @@ def upwind_hand_loop(dt, u, unp1):
 
 class ConvectionUpwind:
     """make_convection_upwind<Field>(velocity) (schemes/fv/operators/convection_lin.hpp:15-89),
+    with a list of component fields the velocity-field form (:184-264),
     or with velocity None the non-linear make_convection_upwind<Field>()
     (convection_nonlin.hpp:24-79, flux f(u) = u * u); calling it on u returns
     the operator applied to u in a new field (or in `out`)."""
 
     def __init__(self, velocity=None):
-        self.velocity = None if velocity is None else np.ascontiguousarray(np.asarray(velocity, dtype=np.float64).ravel())
+        self.fields = None
+        if velocity is not None and len(velocity) > 0 and isinstance(velocity[0], ScalarField):
+            # make_convection_upwind<Field>(velocity_field) (convection_lin.hpp:184-264):
+            # one scalar field per velocity component
+            self.fields, self.velocity = list(velocity), None
+        else:
+            self.velocity = None if velocity is None else np.ascontiguousarray(np.asarray(velocity, dtype=np.float64).ravel())
 
     def __call__(self, u, out=None):
         if out is None:
             out = make_scalar_field("convection(" + u.name + ")", u.mesh)
-        if self.velocity is None:
+        if self.fields is not None:
+            arr = (ctypes.c_void_p * len(self.fields))(*[f.h for f in self.fields])
+            _check(lib().smr_convection_upwind_field(arr, u.h, out.h))
+        elif self.velocity is None:
             _check(lib().smr_convection_upwind_nonlinear(u.h, out.h))
         else:
             _check(lib().smr_convection_upwind(_dp(self.velocity), u.h, out.h))

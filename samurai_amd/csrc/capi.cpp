@@ -974,6 +974,24 @@ extern "C"
             });
     }
 
+    int smr_convection_upwind_field(smr_field* const* velocity, smr_field* u, smr_field* out)
+    {
+        return guarded(
+            [&]
+            {
+                require(velocity && u && out, "null argument");
+                const int dim = u->f.mesh->host.cfg.dim;
+                smr::Field* v[3] = {nullptr, nullptr, nullptr};
+                for (int d = 0; d < dim; ++d)
+                {
+                    require(velocity[d] != nullptr, "null velocity component");
+                    v[d] = &velocity[d]->f;
+                }
+                smr::convection_upwind_heterogeneous(v, u->f, out->f);
+                return 0;
+            });
+    }
+
     int smr_convection_upwind(const double* velocity, smr_field* u, smr_field* out)
     {
         return guarded(

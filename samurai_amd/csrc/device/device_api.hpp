@@ -158,6 +158,11 @@ namespace smr::dev
     // scalar non-linear upwind flux of f(u) = u * u
     void conv_burgers_gather(const int64_t* row, const int64_t* ptr, const int64_t* src, const double* coef, int64_t nrows, const double* u,
                      double* out, hipStream_t s);
+    // make_convection_upwind(velocity_field): term k of direction dir[k] is
+    // (coef[k] * w) * u[upwind side], w the mean of velocity[dir[k]] over the
+    // two stencil slots (velocity[d] for d >= dim may be null)
+    void conv_het_gather(const int64_t* row, const int64_t* ptr, const int64_t* src, const double* coef, const int8_t* dir, int64_t nrows,
+                         const double* u, const double* const* velocity, double* out, hipStream_t s);
     // launch shape of the upwind kernel (cells per thread, XCD remap, stores)
     int upwind_variant_count();
     void set_upwind_variant(int id);

@@ -161,6 +161,32 @@ namespace smr::dev
             out[row[r]] = acc;
         }
 
+        // make_convection_upwind(velocity_field): term k of direction dir[k]
+        // is (coef[k] * w) * u[upwind side], w = 0.5 * (v_d[left] + v_d[right])
+        // (convection_lin.hpp:210-226)
+        __global__ void k_conv_het_gather(const int64_t* __restrict__ row, const int64_t* __restrict__ ptr, const int64_t* __restrict__ src,
+                                          const double* __restrict__ coef, const int8_t* __restrict__ dir, int64_t nrows,
+                                          const double* __restrict__ u, const double* __restrict__ v0, const double* __restrict__ v1,
+                                          const double* __restrict__ v2, double* __restrict__ out)
+        {
+            const int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+            if (r >= nrows)
+            {
+                return;
+            }
+            double acc = 0;
+            for (int64_t k = ptr[r]; k < ptr[r + 1]; ++k)
+            {
+                const int d        = dir[k];
+                const double* v    = d == 0 ? v0 : (d == 1 ? v1 : v2);
+                const int64_t sl   = src[2 * k], sr = src[2 * k + 1];
+                const double w     = 0.5 * (v[sl] + v[sr]);
+                const double value = w >= 0 ? u[sl] : u[sr];
+                acc += (coef[k] * w) * value;
+            }
+            out[row[r]] = acc;
+        }
+
         __device__ __forceinline__ int64_t readlane64(int64_t v, int lane)
         {
             const uint64_t w  = uint64_t(v);
@@ -607,6 +633,16 @@ namespace smr::dev
         if (nrows > 0)
         {
             hipLaunchKernelGGL(k_conv_burgers_gather, dim3(nblocks(nrows)), dim3(BLOCK), 0, s, row, ptr, src, coef, nrows, u, out);
+        }
+    }
+
+    void conv_het_gather(const int64_t* row, const int64_t* ptr, const int64_t* src, const double* coef, const int8_t* dir, int64_t nrows,
+                         const double* u, const double* const* velocity, double* out, hipStream_t s)
+    {
+        if (nrows > 0)
+        {
+            hipLaunchKernelGGL(k_conv_het_gather, dim3(nblocks(nrows)), dim3(BLOCK), 0, s, row, ptr, src, coef, dir, nrows, u, velocity[0],
+                               velocity[1], velocity[2], out);
         }
     }
 

@@ -13,6 +13,7 @@ namespace smr
         {
             int64_t out, s0, s1;
             double c;
+            int8_t dir = 0;
         };
 
         int64_t base_of(const MRMeshData& m, int level, int32_t x, int32_t y, int32_t z)
@@ -26,7 +27,7 @@ namespace smr
         }
 
         // group by output slot, keeping the order of each slot's terms
-        ConvTerms group_events(const std::vector<Event>& ev, int64_t n, bool two_sources)
+        ConvTerms group_events(const std::vector<Event>& ev, int64_t n, bool two_sources, bool with_dir = false)
         {
             std::vector<int64_t> cnt(size_t(n) + 1, 0);
             for (const Event& x : ev)
@@ -50,6 +51,10 @@ namespace smr
             const size_t nt = ev.size();
             t.src.assign(2 * nt, -1);
             t.coef.assign(nt, 0.0);
+            if (with_dir)
+            {
+                t.dir.assign(nt, 0);
+            }
             std::vector<int64_t> pos(start.begin(), start.end() - 1);
             for (const Event& x : ev)
             {
@@ -57,6 +62,10 @@ namespace smr
                 t.src[size_t(2 * k)]     = x.s0;
                 t.src[size_t(2 * k + 1)] = x.s1;
                 t.coef[size_t(k)]        = x.c;
+                if (with_dir)
+                {
+                    t.dir[size_t(k)] = x.dir;
+                }
             }
             t.ptr.clear();
             t.ptr.reserve(t.row.size() + 1);
@@ -273,8 +282,15 @@ namespace smr
         return group_events(ev, m.nref, false);
     }
 
-    ConvTerms build_burgers_terms(const MRMeshData& m)
+    namespace
     {
+        // the interfaces of the flux-based operators one by one, in the order
+        // the reference applies them (flux_based_scheme__nonlin.hpp:408-590,
+        // flux_based_scheme__lin_het.hpp:75-250): each interface adds a term for
+        // its left cell and one for its right cell; a boundary interface one
+        // for its inner cell.  Events carry the direction.
+        std::vector<Event> interface_events(const MRMeshData& m)
+        {
         const int dim     = m.cfg.dim;
         const auto& cells = m.ids[CELLS];
         const int cmin = ca_min_level(cells), cmax = ca_max_level(cells);
@@ -282,13 +298,15 @@ namespace smr
         const int zs   = dim == 3 ? 1 : 0;
         std::vector<Event> ev;
         // one interface: flux * lf into the left cell, (-flux) * rf = flux * (-rf) into the right one
+        int8_t dir = 0;
         auto iface = [&](int64_t left, int64_t right, int64_t sl, int64_t sr, double lf, double rf)
         {
-            ev.push_back({left, sl, sr, lf});
-            ev.push_back({right, sl, sr, -rf});
+            ev.push_back({left, sl, sr, lf, dir});
+            ev.push_back({right, sl, sr, -rf, dir});
         };
         for (int d = 0; d < dim; ++d)
         {
+            dir = int8_t(d);
             std::array<int, 3> e{0, 0, 0};
             e[size_t(d)] = 1;
             const std::array<int, 3> me{-e[0], -e[1], -e[2]};
@@ -364,7 +382,7 @@ namespace smr
                             const int64_t bo = base_of(m, level, x0 + e[0], y + e[1], z + e[2]) + e[0];
                             for (int32_t x = x0; x < x1; ++x)
                             {
-                                ev.push_back({bp + x, bp + x, bo + x, fa});
+                                ev.push_back({bp + x, bp + x, bo + x, fa, dir});
                             }
                         });
                 minus_box(cl, e, {n + e[0], n + e[1], n + e[2]})
@@ -375,11 +393,22 @@ namespace smr
                             const int64_t bo = base_of(m, level, x0 - e[0], y - e[1], z - e[2]) - e[0];
                             for (int32_t x = x0; x < x1; ++x)
                             {
-                                ev.push_back({bp + x, bo + x, bp + x, -fa});
+                                ev.push_back({bp + x, bo + x, bp + x, -fa, dir});
                             }
                         });
             }
         }
-        return group_events(ev, m.nref, true);
+        return ev;
+        }
+    } // namespace
+
+    ConvTerms build_burgers_terms(const MRMeshData& m)
+    {
+        return group_events(interface_events(m), m.nref, true);
+    }
+
+    ConvTerms build_heterogeneous_terms(const MRMeshData& m)
+    {
+        return group_events(interface_events(m), m.nref, true, true);
     }
 } // namespace smr

@@ -19,6 +19,7 @@ namespace smr
         std::vector<int64_t> ptr{0};
         std::vector<int64_t> src;
         std::vector<double> coef;
+        std::vector<int8_t> dir; // direction of each term (heterogeneous operator only)
         int64_t nterms() const { return int64_t(coef.size()); }
     };
 
@@ -41,4 +42,11 @@ namespace smr
     // coef[k] * F(u[src[2k]], u[src[2k+1]]) with F(l, r) = (0.5 * (l + r) >= 0 ?
     // l * l : r * r); coef carries the sign, (-F) * rf being F * (-rf) exactly.
     ConvTerms build_burgers_terms(const MRMeshData& m);
+
+    // make_convection_upwind<Field>(velocity_field)(u) for a scalar field
+    // (convection_lin.hpp:184-264, explicit_flux_based_scheme__lin_het.hpp:
+    // 35-110): the same interfaces; term k, of direction d = dir[k], is
+    // (coef[k] * w) * (w >= 0 ? u[s0] : u[s1]) with w = 0.5 * (v_d[s0] + v_d[s1])
+    // (s0, s1 = src[2k], src[2k+1]), v_d the storage of velocity component d.
+    ConvTerms build_heterogeneous_terms(const MRMeshData& m);
 } // namespace smr

@@ -953,11 +953,13 @@ namespace smr
             const size_t optr = c->blob->add(t.ptr.data(), t.ptr.size() * 8);
             const size_t osrc = c->blob->add(t.src.data(), t.src.size() * 8);
             const size_t ocof = c->blob->add(t.coef.data(), t.coef.size() * 8);
+            const size_t odir = t.dir.empty() ? 0 : c->blob->add(t.dir.data(), t.dir.size());
             c->blob->upload(s);
             c->row   = reinterpret_cast<const int64_t*>(c->blob->dptr(orow));
             c->ptr   = reinterpret_cast<const int64_t*>(c->blob->dptr(optr));
             c->src   = reinterpret_cast<const int64_t*>(c->blob->dptr(osrc));
             c->coef  = reinterpret_cast<const double*>(c->blob->dptr(ocof));
+            c->dir   = t.dir.empty() ? nullptr : reinterpret_cast<const int8_t*>(c->blob->dptr(odir));
             c->nrows = int64_t(t.row.size());
             return c;
         }
@@ -982,6 +984,45 @@ namespace smr
         }
         hip_check(hipMemsetAsync(out.d, 0, size_t(out.size) * 8, s), "hipMemsetAsync(convection)");
         dev::conv_burgers_gather(dm.burgers->row, dm.burgers->ptr, dm.burgers->src, dm.burgers->coef, dm.burgers->nrows, u.d, out.d, s);
+    }
+
+    void convection_upwind_heterogeneous(Field* const* velocity, Field& u, Field& out)
+    {
+        u.check_current();
+        out.check_current();
+        if (u.mesh != out.mesh || &u == &out)
+        {
+            throw std::invalid_argument("convection: input and output must be two fields on one mesh");
+        }
+        Mesh& m       = *u.mesh;
+        const int dim = m.host.cfg.dim;
+        const double* vel[3] = {nullptr, nullptr, nullptr};
+        for (int d = 0; d < dim; ++d)
+        {
+            Field* v = velocity[d];
+            if (!v || v->mesh != u.mesh || v == &out)
+            {
+                throw std::invalid_argument("convection: the velocity components must be fields on the mesh of u, other than the output");
+            }
+            v->check_current();
+        }
+        // FV_scheme.hpp:176-196: the parameter field's ghosts, then the input's
+        for (int d = 0; d < dim; ++d)
+        {
+            update_ghost_mr(*velocity[d]);
+            vel[d] = velocity[d]->d;
+        }
+        update_ghost_mr(u);
+        DevMesh& dm = m.device();
+        ScopedTimer timer(m.ctx, "convection operator");
+        hipStream_t s = m.ctx->stream;
+        if (!dm.heterogeneous)
+        {
+            dm.heterogeneous = upload_terms(build_heterogeneous_terms(m.host), s);
+        }
+        hip_check(hipMemsetAsync(out.d, 0, size_t(out.size) * 8, s), "hipMemsetAsync(convection)");
+        const DevMesh::Conv& c = *dm.heterogeneous;
+        dev::conv_het_gather(c.row, c.ptr, c.src, c.coef, c.dir, c.nrows, u.d, vel, out.d, s);
     }
 
     // ------------------------------------------------------------------

@@ -127,11 +127,14 @@ namespace smr
             std::unique_ptr<Blob> blob;
             const int64_t *row = nullptr, *ptr = nullptr, *src = nullptr;
             const double* coef = nullptr;
+            const int8_t* dir  = nullptr;
             int64_t nrows      = 0;
         };
         std::unique_ptr<Conv> conv;
         // the non-linear (Burgers) convection terms of this image
         std::unique_ptr<Conv> burgers;
+        // the heterogeneous (velocity field) convection terms of this image
+        std::unique_ptr<Conv> heterogeneous;
         // the work lists of a complete image (cells first)
         std::vector<DevWorkList*> lists() { return {&cells, &projection, &prediction, &detail, &criteria, &maximum}; }
     };
@@ -263,6 +266,9 @@ namespace smr
     void convection_upwind(const double* v, Field& u, Field& out);
     // out = make_convection_upwind<Field>()(u) (scalar, non-linear), after update_ghost_mr(u)
     void convection_upwind_nonlinear(Field& u, Field& out);
+    // out = make_convection_upwind<Field>(velocity_field)(u): velocity given as
+    // dim scalar component fields on u's mesh; their ghosts and u's are updated
+    void convection_upwind_heterogeneous(Field* const* velocity, Field& u, Field& out);
     // Adapt::operator() (mr/adapt.hpp:147-195); returns the number of harten calls
     int adapt(AdaptState& st, double eps, double regularity, int max_iterations = -1);
     // one harten call; true at the fixed point

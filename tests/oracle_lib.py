@@ -46,6 +46,7 @@ def lib():
         L.orc_upwind_only.argtypes = [vp, P(dbl), dbl]
         L.orc_convection_upwind.argtypes = [vp, P(dbl), P(dbl)]
         L.orc_convection_upwind_nonlinear.argtypes = [vp, P(dbl)]
+        L.orc_convection_upwind_heterogeneous.argtypes = [vp, P(P(dbl)), P(dbl)]
         L.orc_upwind_hand_loop.argtypes = [vp, dbl]
         _lib = L
     return _lib
@@ -115,6 +116,7 @@ class OracleSim:
         lib().orc_set_storage(self.h, _dp(v))
 
     def set_dirichlet(self, value=0.0):
+        self._bc = float(value)
         lib().orc_set_dirichlet(self.h, float(value))
 
     def update_ghost_mr(self):
@@ -145,6 +147,31 @@ class OracleSim:
         """make_convection_upwind<Field>()(u) (scalar Burgers flux u*u) after update_ghost_mr(u): the output storage."""
         out = np.zeros(self.nb_cells(REFERENCE))
         _check(lib().orc_convection_upwind_nonlinear(self.h, _dp(out)))
+        return out
+
+    def ghost_updated(self, values, bc_value):
+        """The storage of another field on this mesh (values on the cells,
+        Dirichlet bc_value) after update_ghost_mr; this sim's own field and BC
+        are left as they were."""
+        keep, keep_bc = self.storage(), (self._bc if hasattr(self, "_bc") else None)
+        st = np.zeros(self.nb_cells(REFERENCE))
+        self.set_storage(st)
+        self.set_values(np.asarray(values, dtype=np.float64))
+        lib().orc_set_dirichlet(self.h, float(bc_value))
+        self.update_ghost_mr()
+        out = self.storage()
+        self.set_storage(keep)
+        if keep_bc is not None:
+            lib().orc_set_dirichlet(self.h, float(keep_bc))
+        return out
+
+    def convection_upwind_heterogeneous(self, velocity_storages):
+        """make_convection_upwind<Field>(velocity_field)(u) after update_ghost_mr(u);
+        velocity_storages: one ghost-updated storage per component."""
+        vs = [np.ascontiguousarray(v, dtype=np.float64) for v in velocity_storages]
+        arr = (ctypes.POINTER(ctypes.c_double) * len(vs))(*[_dp(v) for v in vs])
+        out = np.zeros(self.nb_cells(REFERENCE))
+        _check(lib().orc_convection_upwind_heterogeneous(self.h, arr, _dp(out)))
         return out
 
     def convection_upwind(self, v):

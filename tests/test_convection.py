@@ -233,3 +233,66 @@ def test_host_burgers_terms_reproduce_oracle(host_ctx, dim, lmin, lmax):
     m = sa.make_mesh_from_cells(host_ctx, sa.mesh_config(dim, lmin, lmax, 1), cells)
     got = evaluate_burgers_terms(sa.convection_terms(m), u, len(u))
     assert np.array_equal(got.view(np.uint64), ref.view(np.uint64))
+
+
+# ---------------------------------------------------------------------------
+# make_convection_upwind<Field>(velocity_field) (convection_lin.hpp:184-264):
+# velocity given per cell, as one scalar field per component
+# ---------------------------------------------------------------------------
+
+def velocity_storages(o, comps, bcs):
+    """Ghost-updated storages of the velocity components (values on o's cells)."""
+    return [o.ghost_updated(c, bc) for c, bc in zip(comps, bcs)]
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_velocity_field_known_answer(dim):
+    """tests/test_fv_operators.cpp:640-715: uniform level-3 mesh, velocity
+    component d = d + 1 everywhere (Dirichlet with the same values), u =
+    sum_d (d + 2) x_d with Dirichlet 0: sum_d (d + 1)(d + 2) on interior cells."""
+    level = 3
+    o = uniform(dim, level)
+    cells = o.cells()
+    w = [float(d + 1) for d in range(dim)]
+    b = np.array([d + 2.0 for d in range(dim)])
+    o.set_values(centers(cells)[:, :dim] @ b)
+    o.set_dirichlet(0.0)
+    vel = velocity_storages(o, [np.full(len(cells), wd) for wd in w], w)
+    out = o.convection_upwind_heterogeneous(vel)[cell_slots(o)]
+    n = 1 << level
+    interior = np.all((cells[:, 1:1 + dim] >= 1) & (cells[:, 1:1 + dim] <= n - 2), axis=1)
+    assert interior.sum() > 0
+    assert np.allclose(out[interior], float(np.dot(w, b)), rtol=0, atol=1e-10)
+
+
+@pytest.mark.parametrize("dim,lmin,lmax,v", [(2, 2, 7, (1.0, -0.5)), (2, 3, 8, (-0.7, 1.3)), (3, 2, 5, (1.0, -0.5, 0.25))])
+def test_constant_velocity_field_is_the_linear_operator(dim, lmin, lmax, v):
+    """With a constant velocity field (and its boundary values) the
+    heterogeneous operator is make_convection_upwind(v): same interfaces,
+    factors and upwind sides; the reference adds the terms of a cell in another
+    order (cell by cell instead of interval by interval), so the two agree to
+    rounding."""
+    field = field_2d if dim == 2 else (lambda c: 0.5 * sphere_3d(c, radius=0.2))
+    o = adapted(dim, lmin, lmax, field)
+    n = o.nb_cells()
+    vel = velocity_storages(o, [np.full(n, vd) for vd in v], v)
+    het = o.convection_upwind_heterogeneous(vel)[cell_slots(o)]
+    lin = o.convection_upwind(np.array(v))[cell_slots(o)]
+    scale = np.abs(lin).max()
+    assert scale > 0
+    assert np.max(np.abs(het - lin)) <= 1e-13 * scale
+
+
+@pytest.mark.parametrize("dim,lmin,lmax", [(2, 2, 7), (3, 2, 5)])
+def test_velocity_field_conservation(dim, lmin, lmax):
+    """A rotating velocity field: interior fluxes cancel in sum area * out
+    (the field u vanishes near the boundary)."""
+    field = field_2d if dim == 2 else (lambda c: 0.5 * sphere_3d(c, radius=0.2))
+    o = adapted(dim, lmin, lmax, field)
+    c = centers(o.cells())
+    comps = [-(c[:, 1] - 0.5), c[:, 0] - 0.5] + ([0.3 * np.sin(6 * c[:, 0])] if dim == 3 else [])
+    out = o.convection_upwind_heterogeneous(velocity_storages(o, comps, [0.0] * dim))[cell_slots(o)]
+    area = np.ldexp(1.0, -dim * o.cells()[:, 0].astype(np.int64))
+    total, scale = float(np.sum(area * out)), float(np.sum(area * np.abs(out)))
+    assert scale > 0
+    assert abs(total) <= 1e-12 * scale

@@ -250,6 +250,38 @@ def test_burgers_steps_match_oracle(gpu_ctx, dim, lmin, lmax):
         assert np.array_equal(bits(o.values()), bits(u.cells())), step
 
 
+def rotating_velocity(cells, dim):
+    from initial_conditions import centers
+    c = centers(cells)
+    comps = [-(c[:, 1] - 0.5), c[:, 0] - 0.5]
+    if dim == 3:
+        comps.append(0.3 * np.sin(6 * c[:, 0]) - 0.1)
+    return comps
+
+
+@pytest.mark.parametrize("dim,lmin,lmax", [(2, 2, 7), (2, 3, 8), (3, 2, 5)])
+def test_convection_velocity_field_matches_oracle(gpu_ctx, dim, lmin, lmax):
+    """make_convection_upwind<Field>(velocity_field)(u) with a rotating velocity
+    field (components with Dirichlet values) on adapted meshes."""
+    say("convection velocity field", dim, lmin, lmax)
+    o, mesh, u = pair(gpu_ctx, dim, lmin, lmax, 1, adapt_eps=1e-3)
+    comps = rotating_velocity(o.cells(), dim)
+    bcs = [0.25, -0.5, 0.125][:dim]
+    vel = []
+    for d in range(dim):
+        f = sa.make_scalar_field(f"v{d}", mesh)
+        f.set_cells(comps[d])
+        sa.make_bc_dirichlet(f, bcs[d])
+        vel.append(f)
+    ref = o.convection_upwind_heterogeneous([o.ghost_updated(c, b) for c, b in zip(comps, bcs)])
+    conv = sa.make_convection_upwind(vel)
+    assert np.array_equal(bits(conv(u).storage()), bits(ref))
+    # the velocity components' ghosts were updated in place, as the reference does
+    for d in range(dim):
+        assert np.array_equal(bits(vel[d].storage()), bits(o.ghost_updated(comps[d], bcs[d])))
+    assert np.array_equal(bits(conv(u).storage()), bits(ref))
+
+
 @pytest.mark.parametrize("lmin,lmax", [(5, 5), (2, 7)])
 def test_hand_loop_matches_oracle(gpu_ctx, lmin, lmax):
     """The README's for_each_interval upwind loop."""
