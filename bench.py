@@ -291,6 +291,37 @@ def bench_3d(sa, ctx, args):
             "ms_per_step": wall / args.steps_3d * 1e3, "mean_leaves": updates / args.steps_3d, "setup_s": setup}
 
 
+def bench_convection(sa, ctx, mesh, u, a, reps):
+    """The flux-based convection operators on the final adapted mesh: time per
+    call (ghost update of the inputs + gather of the ordered contribution
+    lists), after one untimed call that builds and uploads the lists."""
+    from initial_conditions import centers
+    c = centers(mesh.cells())
+    vel = []
+    for d, comp in enumerate([-(c[:, 1] - 0.5), c[:, 0] - 0.5]):
+        v = sa.make_scalar_field(f"v{d}", mesh)
+        v.set_cells(comp)
+        sa.make_bc_dirichlet(v, 0.0)
+        vel.append(v)
+    f = sa.make_scalar_field("f", mesh)
+    ops = {"constant_velocity": (sa.make_convection_upwind(a), a),
+           "nonlinear_burgers": (sa.make_convection_upwind(), None),
+           "velocity_field": (sa.make_convection_upwind(vel), None)}
+    res = {}
+    for name, (op, lists_of) in ops.items():
+        op(u, out=f)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            op(u, out=f)
+        ctx.synchronize()
+        res[name] = {"ms_per_call": (time.perf_counter() - t0) / reps * 1e3,
+                     "terms": int(len(sa.convection_terms(mesh, lists_of)[3]))}
+    res["leaves"] = mesh.nb_cells()
+    progress("convection operators: " + ", ".join(f"{k} {v['ms_per_call']:.3f} ms" for k, v in res.items() if isinstance(v, dict)))
+    return res
+
+
 def bench_adjoint(sa, ctx, args, eps, reg):
     """Gradients of L = 0.5 sum area (u_T - u_target)^2 after 100 frozen-mesh
     steps on 32 instances (L2..10, adapted once), w.r.t. u0 and a; u_target is
@@ -408,6 +439,7 @@ def main():
     p.add_argument("--adjoint", type=int, default=32, help="instances of the adjoint workload (0 disables)")
     p.add_argument("--adjoint-steps", type=int, default=100)
     p.add_argument("--adjoint-reps", type=int, default=2)
+    p.add_argument("--convection", type=int, default=1, help="time the flux-based convection operators (0 disables)")
     args = p.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): this build has no multi-GPU path; "
@@ -478,6 +510,7 @@ def main():
         phases = {k: {"ms_per_step": v["seconds"] / args.phase_steps * 1e3, "calls": v["calls"]}
                   for k, v in sa.timers_report().items()}
     final_cells = mesh.nb_cells()
+    convection = bench_convection(sa, ctx, mesh, u, a, 20) if args.convection else None
 
     # ---------------- uniform level-12 roofline check ----------------
     # Four (u, unp1) pairs on the uniform mesh, each ghost-updated once; the
@@ -585,6 +618,7 @@ def main():
         "cpu_baseline": cpu,
         "phases": phases,
         "plain_3d": plain_3d,
+        "convection": convection,
         "batched": batched,
         "adjoint": adjoint,
     }
