@@ -180,6 +180,13 @@ int smr_field_sub_scaled(smr_field* u, double a, smr_field* v, smr_field* out);
  *                                                + u(level, i, j) - u(level, i, j - 1))
  * with dx = cell_length(level), as for_each_interval(mesh, ...) runs it (2D,
  * velocity (1, 1)).  u's ghosts must be up to date (update_ghost_mr). */
+/* unp1 = u - dt * upwind_scalar_burgers(k, u) (stencil_field.hpp:181-255,
+ * demos/FiniteVolume/scalar_burgers_2d.cpp:155-160), two-dimensional: the
+ * finite-volume combination of upwind (-F_left + F_right - F_down + F_up) / dx
+ * with the flux 0.5 * min(|ul|,|ur|)^2 where (k ul < k ur) and ul ur > 0,
+ * 0.5 * max(|ul|,|ur|)^2 where !(k ul < k ur), else 0.  k: two values.  As
+ * for smr_upwind, u's ghosts must have been updated (update_ghost_mr). */
+int smr_upwind_scalar_burgers(const double* k, double dt, smr_field* u, smr_field* unp1);
 int smr_upwind_hand_loop(double dt, smr_field* u, smr_field* unp1);
 /* out = make_convection_upwind<Field>(velocity)(u): the flux-based linear
  * upwind convection operator (schemes/fv/operators/convection_lin.hpp:15-89)

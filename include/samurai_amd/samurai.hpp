@@ -11,6 +11,7 @@
 //   MRadaptation(samurai::mra_config().epsilon(e).regularity(r));
 //   samurai::update_ghost_mr(u);                              update_ghost_mr.hpp:260-270
 //   unp1 = u - dt * samurai::upwind(a, u);                    stencil_field.hpp:175-179
+//   unp1 = u - dt * samurai::upwind_scalar_burgers(k, u);     stencil_field.hpp:181-255
 //   auto conv = samurai::make_convection_upwind<decltype(u)>(v);
 //   unp1 = u - dt * conv(u);                                  convection_lin.hpp:15-89
 //   samurai::swap(u, unp1);                                   field/swap.hpp:12-19
@@ -317,6 +318,7 @@ namespace samurai_amd
     {
         std::array<double, 3> a;
         ScalarField<dim_>* u;
+        bool scalar_burgers = false; // upwind_scalar_burgers(k, u) instead of upwind(a, u)
     };
     template <std::size_t dim_>
     struct ScaledUpwind
@@ -403,7 +405,14 @@ namespace samurai_amd
             {
                 throw std::invalid_argument("u = u - dt * upwind(a, u) needs a second field");
             }
-            check(smr_upwind(st.s.e.a.data(), st.s.dt, st.u->handle(), handle()));
+            if (st.s.e.scalar_burgers)
+            {
+                check(smr_upwind_scalar_burgers(st.s.e.a.data(), st.s.dt, st.u->handle(), handle()));
+            }
+            else
+            {
+                check(smr_upwind(st.s.e.a.data(), st.s.dt, st.u->handle(), handle()));
+            }
             touched();
             return *this;
         }
@@ -487,6 +496,13 @@ namespace samurai_amd
         {
             e.a[d] = a[d];
         }
+        return e;
+    }
+    // upwind_scalar_burgers(k, u) (stencil_field.hpp:181-255), 2D
+    template <class Vec>
+    UpwindExpr<2> upwind_scalar_burgers(const Vec& k, ScalarField<2>& u)
+    {
+        UpwindExpr<2> e{{k[0], k[1], 0}, &u, true};
         return e;
     }
     template <std::size_t dim>

@@ -1186,6 +1186,52 @@ namespace oracle
         s.u.swap(unp1);
     }
 
+    // flux of upwind_scalar_burgers_op (stencil_field.hpp:193-216): 0 by
+    // default; 0.5 * min(|ul|, |ur|)^2 where (a ul < a ur) and ul ur > 0;
+    // 0.5 * max(|ul|, |ur|)^2 where !(a ul < a ur)
+    inline double burgers_upwind_flux(double a, double ul, double ur)
+    {
+        const bool mask1 = (a * ul) < (a * ur);
+        const bool mask2 = (ul * ur) > 0.0;
+        const double mn  = std::min(std::abs(ul), std::abs(ur));
+        const double mx  = std::max(std::abs(ul), std::abs(ur));
+        double out       = 0.0;
+        if (mask1 && mask2)
+        {
+            out = .5 * mn * mn;
+        }
+        if (!mask1)
+        {
+            out = .5 * mx * mx;
+        }
+        return out;
+    }
+
+    // unp1 = u - dt * upwind_scalar_burgers(k, u) over mesh[cells], 2D
+    // (stencil_field.hpp:181-255 with finite_volume, :37-53; demos/FiniteVolume/
+    // scalar_burgers_2d.cpp:155-160), then the swap of the arrays
+    void upwind_scalar_burgers(Sim& s, const double* k, double dt)
+    {
+        const Mesh& m = s.mesh;
+        std::vector<double> unp1 = new_storage(s, m.nref);
+        for (int level = 0; level < MAXL; ++level)
+        {
+            const double dx = 1.0 / (1 << level);
+            m.ids[CELLS][size_t(level)].for_each(
+                [&](int x, int y, int z)
+                {
+                    const double uc = s.u[size_t(m.index(level, x, y, z))];
+                    const double FL = burgers_upwind_flux(k[0], s.u[size_t(m.index(level, x - 1, y))], uc);
+                    const double FR = burgers_upwind_flux(k[0], uc, s.u[size_t(m.index(level, x + 1, y))]);
+                    const double FD = burgers_upwind_flux(k[1], s.u[size_t(m.index(level, x, y - 1))], uc);
+                    const double FU = burgers_upwind_flux(k[1], uc, s.u[size_t(m.index(level, x, y + 1))]);
+                    const double S  = -FL + FR + -FD + FU;
+                    unp1[size_t(m.index(level, x, y, z))] = uc - dt * (S / dx);
+                });
+        }
+        s.u.swap(unp1);
+    }
+
     // README.md:135-154: for_each_interval(mesh, ...) over the cells,
     //   unp1(level, i, j) = u(level, i, j) - dt / dx * (u(level, i, j) - u(level, i - 1, j)
     //                                                 + u(level, i, j) - u(level, i, j - 1));
@@ -1959,6 +2005,21 @@ extern "C"
             update_ghost_mr(*s);
             std::vector<double> r = convection_upwind_heterogeneous(*s, vel);
             std::copy(r.begin(), r.end(), out);
+        });
+        return 0;
+    }
+
+    // update_ghost_mr(u); unp1 = u - dt * upwind_scalar_burgers(k, u); swap (2D)
+    int orc_upwind_scalar_burgers_step(void* p, const double* k, double dt)
+    {
+        auto* s = static_cast<Sim*>(p);
+        ORC_TRY({
+            if (s->mesh.dim != 2)
+            {
+                throw std::invalid_argument("upwind_scalar_burgers is two-dimensional");
+            }
+            update_ghost_mr(*s);
+            upwind_scalar_burgers(*s, k, dt);
         });
         return 0;
     }

@@ -111,6 +111,7 @@ def lib():
         _sig(L, "smr_batch_load_tuning", "argtypes", [vp, ctypes.c_char_p])
         _sig(L, "smr_convection_upwind", "argtypes", [P(dbl), vp, vp])
         _sig(L, "smr_convection_upwind_nonlinear", "argtypes", [vp, vp])
+        _sig(L, "smr_upwind_scalar_burgers", "argtypes", [P(dbl), dbl, vp, vp])
         _sig(L, "smr_convection_upwind_field", "argtypes", [ctypes.POINTER(vp), vp, vp])
         _sig(L, "smr_upwind_hand_loop", "argtypes", [dbl, vp, vp])
         _sig(L, "smr_field_sub_scaled", "argtypes", [vp, dbl, vp, vp])
@@ -327,6 +328,12 @@ def upwind(a, dt, u, unp1):
     """unp1 = u - dt * upwind(a, u) on mesh[cells]."""
     av = np.ascontiguousarray(a, dtype=np.float64)
     _check(lib().smr_upwind(_dp(av), float(dt), u.h, unp1.h))
+
+
+def upwind_scalar_burgers(k, dt, u, unp1):
+    """unp1 = u - dt * upwind_scalar_burgers(k, u) (stencil_field.hpp:181-255), 2D."""
+    k = np.ascontiguousarray(np.asarray(k, dtype=np.float64).ravel())
+    _check(lib().smr_upwind_scalar_burgers(_dp(k), float(dt), u.h, unp1.h))
 
 
 def sub_scaled(u, a, v, out):

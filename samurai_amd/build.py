@@ -58,19 +58,20 @@ def build(verbose=False):
         raise RuntimeError("samurai_amd: link failed")
     os.replace(LIB + ".tmp", LIB)
     io = build_io()
-    # the C++ drop-in demo, built on include/samurai_amd/samurai.hpp (and io.hpp
-    # for its output and restart files when the I/O library is there)
-    demo = os.path.join(HERE, "demos", "advection_2d")
-    cmd = ["g++", "-O2", "-std=c++17", "-Wall", f"-I{os.path.join(ROOT, 'include')}", os.path.join(HERE, "demos", "advection_2d.cpp"),
-           "-o", demo + ".tmp", f"-L{HERE}"]
-    if io:
-        cmd += ["-DSAMURAI_AMD_WITH_IO", "-lsamurai_amd_io", f"-Wl,-rpath-link,{os.path.join(HDF5_ROOT, 'lib')}"]
-    cmd += ["-lsamurai_amd", "-Wl,-rpath,$ORIGIN/.."]
-    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
-    if r.returncode != 0:
-        sys.stderr.write(r.stdout.decode(errors="replace"))
-        raise RuntimeError("samurai_amd: demo build failed")
-    os.replace(demo + ".tmp", demo)
+    # the C++ drop-in demos, built on include/samurai_amd/samurai.hpp (and
+    # io.hpp for their output and restart files when the I/O library is there)
+    for name in ("advection_2d", "scalar_burgers_2d"):
+        demo = os.path.join(HERE, "demos", name)
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", f"-I{os.path.join(ROOT, 'include')}", os.path.join(HERE, "demos", name + ".cpp"),
+               "-o", demo + ".tmp", f"-L{HERE}"]
+        if io:
+            cmd += ["-DSAMURAI_AMD_WITH_IO", "-lsamurai_amd_io", f"-Wl,-rpath-link,{os.path.join(HDF5_ROOT, 'lib')}"]
+        cmd += ["-lsamurai_amd", "-Wl,-rpath,$ORIGIN/.."]
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout.decode(errors="replace"))
+            raise RuntimeError(f"samurai_amd: demo {name} build failed")
+        os.replace(demo + ".tmp", demo)
     return LIB
 
 

@@ -1027,6 +1027,17 @@ extern "C"
             });
     }
 
+    int smr_upwind_scalar_burgers(const double* k, double dt, smr_field* u, smr_field* unp1)
+    {
+        return guarded(
+            [&]
+            {
+                require(k && u && unp1, "null argument");
+                smr::upwind_scalar_burgers(k, dt, u->f, unp1->f);
+                return 0;
+            });
+    }
+
     int smr_upwind_hand_loop(double dt, smr_field* u, smr_field* unp1)
     {
         return guarded(

@@ -169,6 +169,39 @@ namespace smr::dev
         return ha * (ul + ur) + habs * (ul - ur);
     }
 
+    // flux of upwind_scalar_burgers_op (stencil_field.hpp:193-216)
+    __device__ __forceinline__ double burgers_upwind_flux(double a, double ul, double ur)
+    {
+        const bool mask1 = (a * ul) < (a * ur);
+        const bool mask2 = (ul * ur) > 0.0;
+        const double mn  = fmin(fabs(ul), fabs(ur));
+        const double mx  = fmax(fabs(ul), fabs(ur));
+        double out       = 0.0;
+        if (mask1 && mask2)
+        {
+            out = .5 * mn * mn;
+        }
+        if (!mask1)
+        {
+            out = .5 * mx * mx;
+        }
+        return out;
+    }
+
+    // unp1 = u - dt * upwind_scalar_burgers(k, u) (2D; same taps and order as upwind_value<2>)
+    __device__ __forceinline__ double burgers_value(const int64_t* b, int32_t x, int level, const double* __restrict__ u, double k0,
+                                                    double k1, double dt)
+    {
+        const double* uc_p = u + b[0] + x;
+        const double uc    = uc_p[0];
+        const double FL    = burgers_upwind_flux(k0, uc_p[-1], uc);
+        const double FR    = burgers_upwind_flux(k0, uc, uc_p[1]);
+        const double FD    = burgers_upwind_flux(k1, u[b[1] + x], uc);
+        const double FU    = burgers_upwind_flux(k1, uc, u[b[2] + x]);
+        const double S     = -FL + FR + -FD + FU;
+        return uc - dt * (S * double(1 << level));
+    }
+
     // interp_coeffs<3>(+1) = {1/8, 1, -1/8}, interp_coeffs<3>(-1) = {-1/8, 1, 1/8}
     // (numeric/prediction.hpp:31-35), indexed by parity (0 even, 1 odd)
     __device__ __forceinline__ double pcoef(int parity, int k)

@@ -172,6 +172,9 @@ namespace smr::dev
     void sub_scaled(const List& l, int64_t c0, int64_t c1, const double* u, double a, const double* v, double* out, hipStream_t s);
     // the README's for_each_interval upwind loop (2D, a = (1, 1))
     void hand_loop(const List& l, int64_t c0, int64_t c1, const double* u, double* unp1, double dt, hipStream_t s);
+    // unp1 = u - dt * upwind_scalar_burgers(k, u) on cells [c0, c1) of a 2D list
+    void upwind_scalar_burgers(const List& l, int64_t c0, int64_t c1, const double* u, double* unp1, double k0, double k1, double dt,
+                               hipStream_t s);
     void project(int dim, const List& l, int64_t c0, int64_t c1, double* dst, const double* src, hipStream_t s);
     void predict_fine(int dim, int pred_s, const List& l, int64_t c0, int64_t c1, double* u, hipStream_t s);
     void detail(int dim, int pred_s, const List& l, int64_t c0, int64_t c1, const double* u, double* det, hipStream_t s);

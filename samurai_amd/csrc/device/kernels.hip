@@ -545,6 +545,12 @@ namespace smr::dev
                             unp1[b[0] + cell.x] = hand_loop_value(b, cell.x, cell.level, u, dt);
                         }),
                         const double* u, double* unp1, double dt)
+        SMR_CELL_KERNEL(k_upwind_burgers, ({
+                            const Cell cell         = locate(l, c);
+                            const int64_t* b        = bases(l, cell.run);
+                            unp1[b[0] + cell.x] = burgers_value(b, cell.x, cell.level, u, k0, k1, dt);
+                        }),
+                        const double* u, double* unp1, double k0, double k1, double dt)
         SMR_CELL_KERNEL(k_tag_change, (op_tag_change(l, c, tag, min_level, max_level, flag)), const uint8_t* tag, int min_level,
                         int max_level, int* flag)
         SMR_CELL_KERNEL(k_copy, (op_copy(l, c, dst, src)), double* dst, const double* src)
@@ -880,6 +886,15 @@ namespace smr::dev
         if (c1 > c0)
         {
             hipLaunchKernelGGL(k_hand_loop, dim3(nblocks(c1 - c0)), dim3(BLOCK), 0, s, l, c0, c1, u, unp1, dt);
+        }
+    }
+
+    void upwind_scalar_burgers(const List& l, int64_t c0, int64_t c1, const double* u, double* unp1, double k0, double k1, double dt,
+                               hipStream_t s)
+    {
+        if (c1 > c0)
+        {
+            hipLaunchKernelGGL(k_upwind_burgers, dim3(nblocks(c1 - c0)), dim3(BLOCK), 0, s, l, c0, c1, u, unp1, k0, k1, dt);
         }
     }
 

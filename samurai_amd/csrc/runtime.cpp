@@ -866,6 +866,24 @@ namespace smr
         dev::upwind(dim, dm.cells.d, 0, dm.cells.d.ncells, u.d, unp1.d, a[0], a[1], dim == 3 ? a[2] : 0.0, dt, m.ctx->stream);
     }
 
+    void upwind_scalar_burgers(const double* k, double dt, Field& u, Field& unp1)
+    {
+        u.check_current();
+        unp1.check_current();
+        if (u.mesh != unp1.mesh || &u == &unp1)
+        {
+            throw std::invalid_argument("upwind_scalar_burgers: u and unp1 must be two fields on one mesh");
+        }
+        Mesh& m = *u.mesh;
+        if (m.host.cfg.dim != 2)
+        {
+            throw std::invalid_argument("upwind_scalar_burgers is two-dimensional (stencil_field.hpp:218-247)");
+        }
+        DevMesh& dm = m.device();
+        ScopedTimer timer(m.ctx, "field expressions");
+        dev::upwind_scalar_burgers(dm.cells.d, 0, dm.cells.d.ncells, u.d, unp1.d, k[0], k[1], dt, m.ctx->stream);
+    }
+
     void sub_scaled(Field& u, double a, Field& v, Field& out)
     {
         u.check_current();

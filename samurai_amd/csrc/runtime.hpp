@@ -264,6 +264,8 @@ namespace smr
     void upwind_hand_loop(double dt, Field& u, Field& unp1);
     // out = make_convection_upwind(v)(u), after update_ghost_mr(u)
     void convection_upwind(const double* v, Field& u, Field& out);
+    // unp1 = u - dt * upwind_scalar_burgers(k, u), 2D, ghosts of u updated beforehand
+    void upwind_scalar_burgers(const double* k, double dt, Field& u, Field& unp1);
     // out = make_convection_upwind<Field>()(u) (scalar, non-linear), after update_ghost_mr(u)
     void convection_upwind_nonlinear(Field& u, Field& out);
     // out = make_convection_upwind<Field>(velocity_field)(u): velocity given as

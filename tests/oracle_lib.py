@@ -46,6 +46,7 @@ def lib():
         L.orc_upwind_only.argtypes = [vp, P(dbl), dbl]
         L.orc_convection_upwind.argtypes = [vp, P(dbl), P(dbl)]
         L.orc_convection_upwind_nonlinear.argtypes = [vp, P(dbl)]
+        L.orc_upwind_scalar_burgers_step.argtypes = [vp, P(dbl), dbl]
         L.orc_convection_upwind_heterogeneous.argtypes = [vp, P(P(dbl)), P(dbl)]
         L.orc_upwind_hand_loop.argtypes = [vp, dbl]
         _lib = L
@@ -142,6 +143,11 @@ class OracleSim:
     def upwind_only(self, a, dt):
         a = np.ascontiguousarray(a, dtype=np.float64)
         _check(lib().orc_upwind_only(self.h, _dp(a), float(dt)))
+
+    def upwind_scalar_burgers_step(self, k, dt):
+        """update_ghost_mr, unp1 = u - dt * upwind_scalar_burgers(k, u), swap (2D)."""
+        k = np.ascontiguousarray(k, dtype=np.float64)
+        _check(lib().orc_upwind_scalar_burgers_step(self.h, _dp(k), float(dt)))
 
     def convection_upwind_nonlinear(self):
         """make_convection_upwind<Field>()(u) (scalar Burgers flux u*u) after update_ghost_mr(u): the output storage."""

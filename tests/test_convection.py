@@ -296,3 +296,69 @@ def test_velocity_field_conservation(dim, lmin, lmax):
     total, scale = float(np.sum(area * out)), float(np.sum(area * np.abs(out)))
     assert scale > 0
     assert abs(total) <= 1e-12 * scale
+
+
+# ---------------------------------------------------------------------------
+# upwind_scalar_burgers(k, u) (stencil_field.hpp:181-255), the stencil operator
+# of demos/FiniteVolume/scalar_burgers_2d.cpp; no known answer in the
+# reference's tests: pinned by closed forms and the flux's four cases
+# ---------------------------------------------------------------------------
+
+def test_scalar_burgers_closed_form():
+    """u = 3 + 2x + 3y > 0, k = (1, 1): every flux is 0.5 * u_left^2 (the
+    left value is the smaller one, same signs), so an interior cell moves by
+    -dt * sum_d 0.5 b_d (2 u - b_d h)."""
+    level = 4
+    o = uniform(2, level)
+    cells = o.cells()
+    b = np.array([2.0, 3.0])
+    u = 3.0 + centers(cells)[:, :2] @ b
+    o.set_values(u)
+    o.set_dirichlet(0.0)
+    dt, h = 1e-3, 1.0 / (1 << level)
+    o.upwind_scalar_burgers_step(np.array([1.0, 1.0]), dt)
+    got = o.values()
+    expected = u - dt * sum(0.5 * b[d] * (2 * u - b[d] * h) for d in range(2))
+    n = 1 << level
+    interior = np.all((cells[:, 1:3] >= 1) & (cells[:, 1:3] <= n - 2), axis=1)
+    assert np.allclose(got[interior], expected[interior], rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("ul,ur,flux", [(1.0, 2.0, 0.5), (2.0, 1.0, 2.0), (-1.0, 2.0, 0.0), (2.0, -3.0, 4.5), (-2.0, -1.0, 0.5)])
+def test_scalar_burgers_flux_cases(ul, ur, flux):
+    """k = (1, 1), a row of cells at x < 1/2 holding ul and x >= 1/2 holding
+    ur: the interior cell left of the jump sees F_right - F_left = flux -
+    0.5 ul^2 on the x axis (the y fluxes cancel on an interior row):
+    increasing same sign -> 0.5 min^2, decreasing -> 0.5 max^2, rarefaction
+    (ul < 0 < ur) -> 0, shock -> 0.5 max^2."""
+    level = 3
+    o = uniform(2, level)
+    cells = o.cells()
+    o.set_values(np.where(cells[:, 1] < 4, ul, ur).astype(np.float64))
+    o.set_dirichlet(0.0)
+    dt, h = 1e-3, 1.0 / (1 << level)
+    o.upwind_scalar_burgers_step(np.array([1.0, 1.0]), dt)
+    sel = (cells[:, 1] == 3) & (cells[:, 2] == 4)
+    left_flux = 0.5 * ul * ul  # both sides hold ul: 0.5 * ul^2 in every case
+    want = ul - dt * ((flux - left_flux) / h)
+    assert o.values()[sel][0] == want
+
+
+def test_scalar_burgers_agrees_with_nonlinear_convection_on_positive_fields():
+    """For u > 0 and k > 0 both operators take the left value on every
+    interior interface: (u - unp1) / dt = 0.5 * make_convection_upwind<Field>()(u)
+    on interior cells of a uniform mesh, to rounding (two independently pinned
+    operators, different operation order)."""
+    level = 5
+    o = uniform(2, level)
+    cells = o.cells()
+    u = 1.5 + np.sin(3 * centers(cells)[:, 0]) * np.cos(2 * centers(cells)[:, 1]) * 0.5
+    o.set_values(u)
+    o.set_dirichlet(0.0)
+    conv = o.convection_upwind_nonlinear()[cell_slots(o)]
+    dt = 1e-4
+    o.upwind_scalar_burgers_step(np.array([np.sqrt(2) / 2, np.sqrt(2) / 2]), dt)
+    rate = (u - o.values()) / dt
+    n = 1 << level
+    interior = np.all((cells[:, 1:3] >= 1) & (cells[:, 1:3] <= n - 2), axis=1)
+    assert np.allclose(rate[interior], 0.5 * conv[interior], rtol=1e-9, atol=1e-9)

@@ -66,3 +66,48 @@ def test_cpp_demo_writes_the_reference_files_and_restarts(tmp_path, s):
             _, u_ref = fixture_cells(fx)
             assert np.max(np.abs(got - u_ref)) <= 1e-12, (restart, fx)
         assert os.path.exists(out / f"{stem}_restart.h5")
+
+
+def test_cpp_scalar_burgers_demo_matches_oracle(tmp_path):
+    """The scalar_burgers_2d loop written against the C++ drop-in layer
+    (samurai_amd/demos/scalar_burgers_2d.cpp, the loop of
+    demos/FiniteVolume/scalar_burgers_2d.cpp) on levels 3..7 for a few steps:
+    its final leaves and values equal the oracle's run of the same loop bit for
+    bit."""
+    import sys as _sys
+    _sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import OracleSim
+    from test_gpu_parity import two_discs
+    demo = os.path.join(ROOT, "samurai_amd", "demos", "scalar_burgers_2d")
+    assert os.path.exists(demo), "build the library first (python samurai_amd/build.py)"
+    lmin, lmax, Tf, cfl = 3, 7, 0.004, 0.05
+    print("cpp scalar burgers demo", lmin, lmax, Tf, flush=True)
+    r = subprocess.run([demo, "--min-level", str(lmin), "--max-level", str(lmax), "--Tf", str(Tf), "--cfl", str(cfl),
+                        "--path", str(tmp_path), "--out", str(tmp_path / "b")], capture_output=True, text=True, timeout=120)
+    sys.stdout.write(r.stdout)
+    sys.stderr.write(r.stderr)
+    assert r.returncode == 0
+    got = np.loadtxt(tmp_path / "b_final.txt")
+
+    o = OracleSim(2, lmin, lmax, pred_s=1)
+    o.set_poison(False)
+    o.set_values(two_discs(o.cells()))
+    o.set_dirichlet(0.0)
+    k = np.array([np.sqrt(2.0) / 2.0, np.sqrt(2.0) / 2.0])
+    dt, t, n = cfl / (1 << lmax), 0.0, 0
+    o.adapt(2e-4, 1.0)
+    while t != Tf:
+        o.adapt(2e-4, 1.0)
+        t += dt
+        if t > Tf:
+            dt += Tf - t
+            t = Tf
+        o.upwind_scalar_burgers_step(k, dt)
+        n += 1
+    assert f"steps {n} " in r.stdout
+    cells = o.cells()
+    assert got.shape[0] == cells.shape[0]
+    assert np.array_equal(got[:, 0].astype(np.int64), cells[:, 0])
+    assert np.array_equal(got[:, 1].astype(np.int64), cells[:, 1])
+    assert np.array_equal(got[:, 2].astype(np.int64), cells[:, 2])
+    assert np.array_equal(got[:, 3].view(np.uint64), o.values().view(np.uint64))

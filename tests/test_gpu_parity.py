@@ -282,6 +282,43 @@ def test_convection_velocity_field_matches_oracle(gpu_ctx, dim, lmin, lmax):
     assert np.array_equal(bits(conv(u).storage()), bits(ref))
 
 
+def two_discs(cells):
+    """demos/FiniteVolume/scalar_burgers_2d.cpp:20-50: +1 in the disc of radius
+    0.1 at (0.5, 0.5), -1 in the one at (0.2, 0.2), 0 elsewhere."""
+    from initial_conditions import centers
+    c = centers(cells)
+    u = np.where((c[:, 0] - 0.5) ** 2 + (c[:, 1] - 0.5) ** 2 <= 0.01, 1.0, 0.0)
+    return np.where((c[:, 0] - 0.2) ** 2 + (c[:, 1] - 0.2) ** 2 <= 0.01, -1.0, u)
+
+
+@pytest.mark.parametrize("lmin,lmax", [(3, 8), (4, 9)])
+def test_scalar_burgers_steps_match_oracle(gpu_ctx, lmin, lmax):
+    """The scalar_burgers_2d time loop: adapt, update_ghost_mr, unp1 = u - dt *
+    upwind_scalar_burgers(k, u), swap; five steps, bit for bit."""
+    say("scalar burgers steps", lmin, lmax)
+    o = OracleSim(2, lmin, lmax, pred_s=1)
+    o.set_poison(False)
+    o.set_values(two_discs(o.cells()))
+    o.set_dirichlet(0.0)
+    mesh = sa.make_mesh(gpu_ctx, sa.mesh_config(2, lmin, lmax, 1))
+    u = sa.make_scalar_field("u", mesh)
+    u.set_cells(two_discs(mesh.cells()))
+    sa.make_bc_dirichlet(u, 0.0)
+    unp1 = sa.make_scalar_field("unp1", mesh)
+    ad = sa.make_MRAdapt(u)
+    k = np.array([np.sqrt(2.0) / 2.0, np.sqrt(2.0) / 2.0])
+    dt = 0.05 / (1 << lmax)
+    for step in range(5):
+        assert o.adapt(2e-4, 1.0) == ad(2e-4, 1.0)
+        o.upwind_scalar_burgers_step(k, dt)
+        sa.update_ghost_mr(u)
+        unp1.resize()
+        sa.upwind_scalar_burgers(k, dt, u, unp1)
+        sa.swap(u, unp1)
+        assert np.array_equal(o.cells(), mesh.cells()), step
+        assert np.array_equal(bits(o.values()), bits(u.cells())), step
+
+
 @pytest.mark.parametrize("lmin,lmax", [(5, 5), (2, 7)])
 def test_hand_loop_matches_oracle(gpu_ctx, lmin, lmax):
     """The README's for_each_interval upwind loop."""
