@@ -291,114 +291,114 @@ namespace smr
         // for its inner cell.  Events carry the direction.
         std::vector<Event> interface_events(const MRMeshData& m)
         {
-        const int dim     = m.cfg.dim;
-        const auto& cells = m.ids[CELLS];
-        const int cmin = ca_min_level(cells), cmax = ca_max_level(cells);
-        const int minl = std::max(m.cfg.min_level, cmin - 1), maxl = std::min(m.cfg.max_level, cmax + 1);
-        const int zs   = dim == 3 ? 1 : 0;
-        std::vector<Event> ev;
-        // one interface: flux * lf into the left cell, (-flux) * rf = flux * (-rf) into the right one
-        int8_t dir = 0;
-        auto iface = [&](int64_t left, int64_t right, int64_t sl, int64_t sr, double lf, double rf)
-        {
-            ev.push_back({left, sl, sr, lf, dir});
-            ev.push_back({right, sl, sr, -rf, dir});
-        };
-        for (int d = 0; d < dim; ++d)
-        {
-            dir = int8_t(d);
-            std::array<int, 3> e{0, 0, 0};
-            e[size_t(d)] = 1;
-            const std::array<int, 3> me{-e[0], -e[1], -e[2]};
-            for (int level = minl; level <= maxl; ++level)
+            const int dim     = m.cfg.dim;
+            const auto& cells = m.ids[CELLS];
+            const int cmin = ca_min_level(cells), cmax = ca_max_level(cells);
+            const int minl = std::max(m.cfg.min_level, cmin - 1), maxl = std::min(m.cfg.max_level, cmax + 1);
+            const int zs   = dim == 3 ? 1 : 0;
+            std::vector<Event> ev;
+            // one interface: flux * lf into the left cell, (-flux) * rf = flux * (-rf) into the right one
+            int8_t dir = 0;
+            auto iface = [&](int64_t left, int64_t right, int64_t sl, int64_t sr, double lf, double rf)
             {
-                const double h  = 1.0 / double(1 << level);
-                const double fa = h_factor(dim, h, h);
-                set_intersection(cells[size_t(level)], translate(cells[size_t(level)], me))
-                    .for_each_run(
-                        [&](int32_t x0, int32_t x1, int32_t y, int32_t z)
-                        {
-                            const int64_t bp = base_of(m, level, x0, y, z);
-                            const int64_t bq = base_of(m, level, x0 + e[0], y + e[1], z + e[2]) + e[0];
-                            for (int32_t x = x0; x < x1; ++x)
-                            {
-                                iface(bp + x, bq + x, bp + x, bq + x, fa, fa);
-                            }
-                        });
-            }
-            for (int level = minl; level < maxl && level < m.cfg.max_level; ++level)
+                ev.push_back({left, sl, sr, lf, dir});
+                ev.push_back({right, sl, sr, -rf, dir});
+            };
+            for (int d = 0; d < dim; ++d)
             {
-                const int lf     = level + 1;
-                const double hl  = 1.0 / double(1 << level);
-                const double hf  = 1.0 / double(1 << lf);
-                auto coarse_base = [&](int32_t x0, int32_t y, int32_t z) { return base_of(m, level, x0 >> 1, y >> 1, z >> zs); };
-                // coarse cell on the left: stencil {p (fine ghost in the coarse cell), p + e}
+                dir = int8_t(d);
+                std::array<int, 3> e{0, 0, 0};
+                e[size_t(d)] = 1;
+                const std::array<int, 3> me{-e[0], -e[1], -e[2]};
+                for (int level = minl; level <= maxl; ++level)
                 {
-                    const double lfac = h_factor(dim, hf, hl), rfac = h_factor(dim, hf, hf);
-                    on(set_intersection(cells[size_t(level)], translate(cells[size_t(lf)], me)), lf)
+                    const double h  = 1.0 / double(1 << level);
+                    const double fa = h_factor(dim, h, h);
+                    set_intersection(cells[size_t(level)], translate(cells[size_t(level)], me))
                         .for_each_run(
                             [&](int32_t x0, int32_t x1, int32_t y, int32_t z)
                             {
-                                const int64_t bg = base_of(m, lf, x0, y, z);
-                                const int64_t bf = base_of(m, lf, x0 + e[0], y + e[1], z + e[2]) + e[0];
+                                const int64_t bp = base_of(m, level, x0, y, z);
+                                const int64_t bq = base_of(m, level, x0 + e[0], y + e[1], z + e[2]) + e[0];
                                 for (int32_t x = x0; x < x1; ++x)
                                 {
-                                    iface(coarse_base(x, y, z) + (x >> 1), bf + x, bg + x, bf + x, lfac, rfac);
+                                    iface(bp + x, bq + x, bp + x, bq + x, fa, fa);
                                 }
                             });
                 }
-                // fine cell on the left: stencil {p - e, p (fine ghost in the coarse cell)}
+                for (int level = minl; level < maxl && level < m.cfg.max_level; ++level)
                 {
-                    const double lfac = h_factor(dim, hf, hf), rfac = h_factor(dim, hf, hl);
-                    on(set_intersection(cells[size_t(level)], translate(cells[size_t(lf)], e)), lf)
+                    const int lf     = level + 1;
+                    const double hl  = 1.0 / double(1 << level);
+                    const double hf  = 1.0 / double(1 << lf);
+                    auto coarse_base = [&](int32_t x0, int32_t y, int32_t z) { return base_of(m, level, x0 >> 1, y >> 1, z >> zs); };
+                    // coarse cell on the left: stencil {p (fine ghost in the coarse cell), p + e}
+                    {
+                        const double lfac = h_factor(dim, hf, hl), rfac = h_factor(dim, hf, hf);
+                        on(set_intersection(cells[size_t(level)], translate(cells[size_t(lf)], me)), lf)
+                            .for_each_run(
+                                [&](int32_t x0, int32_t x1, int32_t y, int32_t z)
+                                {
+                                    const int64_t bg = base_of(m, lf, x0, y, z);
+                                    const int64_t bf = base_of(m, lf, x0 + e[0], y + e[1], z + e[2]) + e[0];
+                                    for (int32_t x = x0; x < x1; ++x)
+                                    {
+                                        iface(coarse_base(x, y, z) + (x >> 1), bf + x, bg + x, bf + x, lfac, rfac);
+                                    }
+                                });
+                    }
+                    // fine cell on the left: stencil {p - e, p (fine ghost in the coarse cell)}
+                    {
+                        const double lfac = h_factor(dim, hf, hf), rfac = h_factor(dim, hf, hl);
+                        on(set_intersection(cells[size_t(level)], translate(cells[size_t(lf)], e)), lf)
+                            .for_each_run(
+                                [&](int32_t x0, int32_t x1, int32_t y, int32_t z)
+                                {
+                                    const int64_t bg = base_of(m, lf, x0, y, z);
+                                    const int64_t bf = base_of(m, lf, x0 - e[0], y - e[1], z - e[2]) - e[0];
+                                    for (int32_t x = x0; x < x1; ++x)
+                                    {
+                                        iface(bf + x, coarse_base(x, y, z) + (x >> 1), bf + x, bg + x, lfac, rfac);
+                                    }
+                                });
+                    }
+                }
+                // boundary interfaces: the outer ghost holds the boundary value
+                for (int level = cmin; level <= cmax; ++level)
+                {
+                    const LevelSet& cl = cells[size_t(level)];
+                    if (cl.empty())
+                    {
+                        continue;
+                    }
+                    const double h  = 1.0 / double(1 << level);
+                    const double fa = h_factor(dim, h, h);
+                    const int n     = 1 << level;
+                    minus_box(cl, me, {n + me[0], n + me[1], n + me[2]})
                         .for_each_run(
                             [&](int32_t x0, int32_t x1, int32_t y, int32_t z)
                             {
-                                const int64_t bg = base_of(m, lf, x0, y, z);
-                                const int64_t bf = base_of(m, lf, x0 - e[0], y - e[1], z - e[2]) - e[0];
+                                const int64_t bp = base_of(m, level, x0, y, z);
+                                const int64_t bo = base_of(m, level, x0 + e[0], y + e[1], z + e[2]) + e[0];
                                 for (int32_t x = x0; x < x1; ++x)
                                 {
-                                    iface(bf + x, coarse_base(x, y, z) + (x >> 1), bf + x, bg + x, lfac, rfac);
+                                    ev.push_back({bp + x, bp + x, bo + x, fa, dir});
+                                }
+                            });
+                    minus_box(cl, e, {n + e[0], n + e[1], n + e[2]})
+                        .for_each_run(
+                            [&](int32_t x0, int32_t x1, int32_t y, int32_t z)
+                            {
+                                const int64_t bp = base_of(m, level, x0, y, z);
+                                const int64_t bo = base_of(m, level, x0 - e[0], y - e[1], z - e[2]) - e[0];
+                                for (int32_t x = x0; x < x1; ++x)
+                                {
+                                    ev.push_back({bp + x, bo + x, bp + x, -fa, dir});
                                 }
                             });
                 }
             }
-            // boundary interfaces: the outer ghost holds the boundary value
-            for (int level = cmin; level <= cmax; ++level)
-            {
-                const LevelSet& cl = cells[size_t(level)];
-                if (cl.empty())
-                {
-                    continue;
-                }
-                const double h  = 1.0 / double(1 << level);
-                const double fa = h_factor(dim, h, h);
-                const int n     = 1 << level;
-                minus_box(cl, me, {n + me[0], n + me[1], n + me[2]})
-                    .for_each_run(
-                        [&](int32_t x0, int32_t x1, int32_t y, int32_t z)
-                        {
-                            const int64_t bp = base_of(m, level, x0, y, z);
-                            const int64_t bo = base_of(m, level, x0 + e[0], y + e[1], z + e[2]) + e[0];
-                            for (int32_t x = x0; x < x1; ++x)
-                            {
-                                ev.push_back({bp + x, bp + x, bo + x, fa, dir});
-                            }
-                        });
-                minus_box(cl, e, {n + e[0], n + e[1], n + e[2]})
-                    .for_each_run(
-                        [&](int32_t x0, int32_t x1, int32_t y, int32_t z)
-                        {
-                            const int64_t bp = base_of(m, level, x0, y, z);
-                            const int64_t bo = base_of(m, level, x0 - e[0], y - e[1], z - e[2]) - e[0];
-                            for (int32_t x = x0; x < x1; ++x)
-                            {
-                                ev.push_back({bp + x, bo + x, bp + x, -fa, dir});
-                            }
-                        });
-            }
-        }
-        return ev;
+            return ev;
         }
     } // namespace
 
