@@ -190,3 +190,38 @@ def test_assign_cells_matches_a_new_mesh(host_ctx):
     m.assign_cells(src.cells())
     for mid in (sa.CELLS, sa.CELLS_AND_GHOSTS, sa.PROJ_CELLS, sa.UNION_CELLS, sa.REFERENCE):
         assert np.array_equal(src.cells(mid), m.cells(mid)), mid
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("seed", range(6))
+def test_random_cell_arrays_round_trip(tmp_path, dim, seed):
+    """Random sparse cell lists on several levels, negative coordinates
+    included (ghost-like cells), in for_each_cell order: load(dump(ca)) == ca,
+    and the arrays keep the LevelCellArray invariants (offsets ascending from
+    0 to the number of x-intervals; x-interval index = cells before the run -
+    its start, in for_each_cell order)."""
+    rng = np.random.default_rng(seed)
+    rows = []
+    for level in sorted(rng.choice(np.arange(1, 9), size=3, replace=False)):
+        n = 1 << level
+        k = int(rng.integers(1, 60))
+        pts = np.unique(np.stack([rng.integers(-2, n + 2, k), rng.integers(-2, n + 2, k),
+                                  rng.integers(-2, n + 2, k) if dim == 3 else np.zeros(k, dtype=np.int64)], axis=1), axis=0)
+        order = np.lexsort((pts[:, 0], pts[:, 1], pts[:, 2]))  # z, then y, then x
+        rows += [(level, int(p[0]), int(p[1]), int(p[2])) for p in pts[order]]
+    cells = np.array(rows, dtype=np.int32)
+    sa.dump_cells(tmp_path / "r", dim, cells)
+    d, back = sa.load_cells(tmp_path / "r")
+    assert d == dim and np.array_equal(back, cells)
+    if not os.path.exists(H5DUMP):
+        return
+    h5 = str(tmp_path / "r.h5")
+    count = 0
+    for level in sorted(set(cells[:, 0])):
+        xi = read(h5, f"/mesh/level/{level}/dim/0/intervals/data", 4)
+        off = read(h5, f"/mesh/level/{level}/dim/1/offsets/data")
+        assert off[0] == 0 and off[-1] == len(xi) and all(a <= b for a, b in zip(off, off[1:]))
+        for s, e, step, index in xi:
+            assert step == 1 and s < e and index == count - s
+            count += e - s
+    assert count == len(cells)

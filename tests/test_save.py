@@ -142,3 +142,38 @@ def test_io_abi_exports_save_symbols():
     L = ctypes.CDLL(sa.IO_LIB_PATH)
     for n in ("smr_save", "smr_save_cell_array"):
         assert hasattr(L, n), n
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("seed", range(4))
+def test_random_cell_arrays_points_and_connectivity(tmp_path, dim, seed):
+    """Random sparse multi-level cell lists: every cell's corners, read back
+    through the connectivity, are its exact corners in the element order; the
+    points are exactly the distinct corners, numbered at first sight in
+    for_each_cell order."""
+    rng = np.random.default_rng(100 + seed)
+    rows = []
+    for level in sorted(rng.choice(np.arange(1, 7), size=3, replace=False)):
+        n = 1 << level
+        k = int(rng.integers(1, 40))
+        pts = np.unique(np.stack([rng.integers(0, n, k), rng.integers(0, n, k),
+                                  rng.integers(0, n, k) if dim == 3 else np.zeros(k, dtype=np.int64)], axis=1), axis=0)
+        order = np.lexsort((pts[:, 0], pts[:, 1], pts[:, 2]))
+        rows += [(level, int(p[0]), int(p[1]), int(p[2])) for p in pts[order]]
+    cells = np.array(rows, dtype=np.int32)
+    sa.save_cell_array(tmp_path, "rnd", dim, cells)
+    h5 = str(tmp_path / "rnd.h5")
+    npc = 1 << dim
+    con = read(h5, "/mesh/connectivity", "<u8", tmp_path).reshape(-1, npc).astype(np.int64)
+    pts = read(h5, "/mesh/points", "<f8", tmp_path).reshape(-1, 3)
+    elem = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)][:npc]
+    seen = {}
+    for c, row in zip(cells, con):
+        h = np.ldexp(1.0, -int(c[0]))
+        for e, pid in zip(elem, row):
+            corner = tuple((c[1 + d] + e[d]) * h if d < dim else 0.0 for d in range(3))
+            assert tuple(pts[pid]) == corner
+            if corner not in seen:
+                assert pid == len(seen)  # numbered at first sight
+                seen[corner] = pid
+    assert len(seen) == len(pts)
