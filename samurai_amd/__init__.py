@@ -59,6 +59,7 @@ def lib():
         _sig(L, "smr_mesh_create_box", "argtypes", [vp, P(MeshConfig), ctypes.POINTER(vp)])
         _sig(L, "smr_mesh_create_from_cells", "argtypes", [vp, P(MeshConfig), P(ctypes.c_int32), i64, ctypes.POINTER(vp)])
         _sig(L, "smr_mesh_assign_cells", "argtypes", [vp, i32, i32, P(ctypes.c_int32), i64])
+        _sig(L, "smr_mesh_swap", "argtypes", [vp, vp])
         _sig(L, "smr_mesh_destroy", "argtypes", [vp])
         _sig(L, "smr_mesh_nb_cells", "restype", i64)
         _sig(L, "smr_mesh_nb_cells", "argtypes", [vp, i32])
@@ -223,6 +224,15 @@ class MRMesh:
         tags = np.ascontiguousarray(tags, dtype=np.uint8)
         return _check(lib().smr_mesh_rebuild_from_tags(
             self.h, tags.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), tags.size))
+
+
+def swap_meshes(a, b):
+    """std::swap(mesh_a, mesh_b) of two meshes of one context: the contents
+    (and configurations) are exchanged, fields stay with their mesh object and
+    must be resized before their next use."""
+    _check(lib().smr_mesh_swap(a.h, b.h))
+    a.refresh_config()
+    b.refresh_config()
 
 
 def mesh_config(dim, min_level, max_level, prediction_stencil_radius=1, max_stencil_radius=1, graduation_width=1):

@@ -225,3 +225,21 @@ def test_random_cell_arrays_round_trip(tmp_path, dim, seed):
             assert step == 1 and s < e and index == count - s
             count += e - s
     assert count == len(cells)
+
+
+def test_mesh_swap_exchanges_contents(host_ctx):
+    """smr_mesh_swap, the std::swap(mesh, new_mesh) of samurai::load and of
+    `mesh = make_mesh(...)`: contents and configurations change places; meshes
+    of two contexts are refused."""
+    a = sa.make_mesh(host_ctx, sa.mesh_config(2, 2, 4, 1))
+    b = adapted(host_ctx, 2, 3, 7)
+    ca = {mid: a.cells(mid) for mid in (sa.CELLS, sa.REFERENCE)}
+    cb = {mid: b.cells(mid) for mid in (sa.CELLS, sa.REFERENCE)}
+    sa.swap_meshes(a, b)
+    assert (a.min_level, a.max_level, b.min_level, b.max_level) == (3, 7, 2, 4)
+    for mid in (sa.CELLS, sa.REFERENCE):
+        assert np.array_equal(a.cells(mid), cb[mid]) and np.array_equal(b.cells(mid), ca[mid])
+    other = sa.Context(-1)
+    c = sa.make_mesh(other, sa.mesh_config(2, 2, 4, 1))
+    with pytest.raises(sa.SamuraiError, match="different contexts"):
+        sa.swap_meshes(a, c)
