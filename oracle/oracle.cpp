@@ -13,13 +13,17 @@
 // this library.  The product (samurai_amd/) never links or calls it.
 //
 // Pinning: the 2D path is checked against the reference's advection_2d golden
-// fixtures (cells exact, values <= 1e-12); upwind, projection/prediction,
-// graduation and the flux-based convection against closed forms and the
-// reference's own known answers (tests/test_oracle_*.py, tests/test_convection.py).
-// Parity unpinned by reference data: the 3D outer-ghost corner/edge logic
-// (project_corner_below, corner extrapolation in 3D) -- no 3D fixture exists;
-// it is checked only by invariants and by agreement with the product's
-// independent interval-set implementation.
+// fixtures (cells exact, values <= 1e-12).  In 2D and 3D, the operators are
+// checked against the reference's own known answers and the mathematics
+// (tests/test_oracle_pinning.py, tests/test_oracle_invariants.py,
+// tests/test_convection.py): the projection/prediction round trip of
+// per-axis quadratics (s = 0 and 1), quadratic averages reproduced by
+// update_ghost_mr on adapted meshes, details vanishing on quadratics,
+// Dirichlet<1> ghosts on uniform and adapted meshes, corner ghosts as the
+// reflection of the field, the vertex-ghost cascade of project_corner_below,
+// the textbook upwind update in 2D and 3D, graduation and tiling invariants.
+// The 3D edge ghosts below the finest level (project_corner_below along an
+// edge) are covered only by the product's agreement with this file.
 #include "dense_set.hpp"
 
 #include <cmath>
@@ -2038,6 +2042,72 @@ extern "C"
     int orc_upwind_only(void* p, const double* a, double dt)
     {
         ORC_TRY(upwind(*static_cast<Sim*>(p), a, dt));
+        return 0;
+    }
+
+    // One projection or prediction operator on the sim's own mesh, as the
+    // reference's round-trip test applies them
+    // (tests/test_projection_prediction_roundtrip.cpp:139-197), over
+    // set = intersection(cells[level], reference[level-1]):
+    //   op 0: set.on(level-1).apply_op(projection(u))      (numeric/projection.hpp:15-65)
+    //   op 1: set.on(level).apply_op(prediction<s,false>(u)) (numeric/prediction.hpp:259-357)
+    //   op 2: set.on(level-1).apply_op(prediction<s,true>(u)) (numeric/prediction.hpp:107-257)
+    int orc_apply_level_op(void* p, int op, int level)
+    {
+        auto* s = static_cast<Sim*>(p);
+        try
+        {
+            const Mesh& m = s->mesh;
+            if (level < 1 || level >= MAXL || op < 0 || op > 2)
+            {
+                throw std::invalid_argument("bad level or operator");
+            }
+            DSet set = intersection(m.ids[CELLS][size_t(level)], m.ids[REFERENCE][size_t(level - 1)]);
+            if (op == 0)
+            {
+                on(set, level - 1).for_each([&](int x, int y, int z) { U(*s, level - 1, x, y, z) = project_value(*s, level, x, y, z); });
+            }
+            else if (op == 1)
+            {
+                // every value is computed from the coarse level before any fine one is written
+                std::vector<std::pair<int64_t, double>> out;
+                on(set, level).for_each([&](int x, int y, int z) { out.emplace_back(m.index(level, x, y, z), predict_fine(*s, level, x, y, z)); });
+                for (auto& iv : out)
+                {
+                    s->u[size_t(iv.first)] = iv.second;
+                }
+            }
+            else
+            {
+                std::vector<double> dst = s->u;
+                on(set, level - 1).for_each([&](int x, int y, int z) { predict_children(m, s->u, m, dst, level - 1, x, y, z); });
+                s->u = std::move(dst);
+            }
+        }
+        catch (const std::exception& e)
+        {
+            g_err = e.what();
+            return -1;
+        }
+        return 0;
+    }
+
+    // apply_field_bc(u, direction) (bc/apply_field_bc.hpp:256-267): the
+    // Dirichlet<1> ghosts of one Cartesian direction at every level from
+    // min_level to max_level, with the sim's boundary value
+    int orc_apply_field_bc_direction(void* p, int dx, int dy, int dz)
+    {
+        auto* s = static_cast<Sim*>(p);
+        ORC_TRY({
+            if (std::abs(dx) + std::abs(dy) + std::abs(dz) != 1)
+            {
+                throw std::invalid_argument("not a Cartesian direction");
+            }
+            for (int level = s->mesh.min_level; level <= s->mesh.max_level; ++level)
+            {
+                apply_field_bc(*s, level, Dir{dx, dy, dz});
+            }
+        });
         return 0;
     }
 }

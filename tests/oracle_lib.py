@@ -49,6 +49,8 @@ def lib():
         L.orc_upwind_scalar_burgers_step.argtypes = [vp, P(dbl), dbl]
         L.orc_convection_upwind_heterogeneous.argtypes = [vp, P(P(dbl)), P(dbl)]
         L.orc_upwind_hand_loop.argtypes = [vp, dbl]
+        L.orc_apply_level_op.argtypes = [vp, i32, i32]
+        L.orc_apply_field_bc_direction.argtypes = [vp, i32, i32, i32]
         _lib = L
     return _lib
 
@@ -190,3 +192,13 @@ class OracleSim:
     def upwind_hand_loop(self, dt):
         """update_ghost_mr, then the README's for_each_interval upwind loop and swap."""
         _check(lib().orc_upwind_hand_loop(self.h, float(dt)))
+
+    def apply_level_op(self, op, level):
+        """op 0 projection into level-1, 1 prediction<s,false> into level,
+        2 prediction<s,true> from level-1, over cells[level] & reference[level-1]."""
+        _check(lib().orc_apply_level_op(self.h, int(op), int(level)))
+
+    def apply_field_bc_direction(self, direction):
+        """apply_field_bc(u, direction) at every level, with the sim's Dirichlet value."""
+        d = list(direction) + [0] * (3 - len(direction))
+        _check(lib().orc_apply_field_bc_direction(self.h, *[int(v) for v in d]))
