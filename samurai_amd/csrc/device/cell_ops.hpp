@@ -513,18 +513,24 @@ namespace smr::dev
         return uc - k * (uc - u[b[0] + x - 1] + uc - u[b[1] + x]);
     }
 
-    // flag[0] = 1 when the tag of a cell changes the cell array: a refine below
+    // true when the tag of a cell changes the cell array: a refine below
     // max_level or a coarsen without keep above min_level
     // (update_cell_array_from_tag, graduation.hpp:743-843)
+    __device__ __forceinline__ bool tag_changes_mesh(const List& l, int64_t c, const uint8_t* tag, int min_level, int max_level)
+    {
+        const Cell cell = locate(l, c);
+        const uint8_t t = tag[bases(l, cell.run)[0] + cell.x];
+        const bool ref  = (t & 4) && cell.level < max_level;
+        const bool crs  = (t & 2) && !(t & 1) && cell.level > min_level;
+        return ref || crs;
+    }
+
+    // flag[0] = 1 when the cell's tag changes the cell array.  Every lane that
+    // sees a change stores: in a stage program one wave can hold cells of
+    // several segments (instances), each with its own flag.
     __device__ __forceinline__ void op_tag_change(const List& l, int64_t c, const uint8_t* tag, int min_level, int max_level, int* flag)
     {
-        const Cell cell  = locate(l, c);
-        const uint8_t t  = tag[bases(l, cell.run)[0] + cell.x];
-        const bool ref   = (t & 4) && cell.level < max_level;
-        const bool crs   = (t & 2) && !(t & 1) && cell.level > min_level;
-        // one store per wave: the lowest active lane that saw a change
-        const uint64_t m = __ballot(ref || crs);
-        if (m != 0 && int(threadIdx.x & 63) == __ffsll(static_cast<unsigned long long>(m)) - 1)
+        if (tag_changes_mesh(l, c, tag, min_level, max_level))
         {
             flag[0] = 1;
         }

@@ -551,8 +551,16 @@ namespace smr::dev
                             unp1[b[0] + cell.x] = burgers_value(b, cell.x, cell.level, u, k0, k1, dt);
                         }),
                         const double* u, double* unp1, double k0, double k1, double dt)
-        SMR_CELL_KERNEL(k_tag_change, (op_tag_change(l, c, tag, min_level, max_level, flag)), const uint8_t* tag, int min_level,
-                        int max_level, int* flag)
+        // one flag for the whole list: one store per wave, from the lowest
+        // active lane that saw a change
+        SMR_CELL_KERNEL(k_tag_change, ({
+                            const uint64_t m = __ballot(tag_changes_mesh(l, c, tag, min_level, max_level));
+                            if (m != 0 && int(threadIdx.x & 63) == __ffsll(static_cast<unsigned long long>(m)) - 1)
+                            {
+                                flag[0] = 1;
+                            }
+                        }),
+                        const uint8_t* tag, int min_level, int max_level, int* flag)
         SMR_CELL_KERNEL(k_copy, (op_copy(l, c, dst, src)), double* dst, const double* src)
         template <int DIM, int S>
         SMR_CELL_KERNEL(k_predict_children, (op_predict_children<DIM, S>(l, c, dst, src)), double* dst, const double* src)
