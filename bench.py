@@ -623,6 +623,8 @@ def main():
         "adjoint": adjoint,
     }
     print(json.dumps(out), flush=True)
+    # every library object released while the HIP runtime is whole
+    ctx.close()
 
 
 if __name__ == "__main__":

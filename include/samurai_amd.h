@@ -65,6 +65,11 @@ typedef struct smr_mesh_config
 /* samurai::initialize (samurai.hpp:47-113): a context on HIP device `device`,
  * or a host-only context (mesh construction only) when device < 0. */
 int smr_ctx_create(int device, smr_ctx** out);
+/* samurai::finalize (samurai.hpp:100-113).  Every object made from the
+ * context (batches, adjoint problems, adaptations, fields, then meshes) must
+ * be destroyed before it: their destructors use its stream.  Call it before
+ * the process exits, not from a static destructor or an atexit handler that
+ * runs after the HIP runtime has begun to shut down. */
 int smr_ctx_destroy(smr_ctx* ctx);
 int smr_ctx_synchronize(smr_ctx* ctx);
 const char* smr_last_error(void);
@@ -98,6 +103,12 @@ int smr_mesh_get_cells(const smr_mesh* mesh, int mesh_id, int64_t* out);
  * unchanged), 0 if the mesh was rebuilt.  Host only: fields on the mesh must
  * be resized afterwards, their values are not transferred. */
 int smr_mesh_rebuild_from_tags(smr_mesh* mesh, const uint8_t* tags, int64_t n);
+/* smr_mesh_rebuild_from_tags with update_cell_array_from_tag and the
+ * make_graduation fixed point run on the device (per-level bitmaps), from a
+ * device array of tags; the host receives the new leaves only when they
+ * change.  The path every harten call of smr_adapt_run takes.  Returns 1 at
+ * the fixed point, 0 if the mesh was rebuilt. */
+int smr_mesh_rebuild_from_device_tags(smr_mesh* mesh, const uint8_t* tags, int64_t n);
 
 /* make_scalar_field<double>(name, mesh) (field/scalar_field.hpp:171-176) */
 int smr_field_create(smr_mesh* mesh, const char* name, smr_field** out);
@@ -137,6 +148,15 @@ int smr_adapt_run(smr_adapt* adapt, double eps, double regularity);
 /* one Adapt::harten(ite) call (mr/adapt.hpp:276-389): 1 at the fixed point, 0 if
  * the mesh changed */
 int smr_adapt_harten(smr_adapt* adapt, int ite, double eps, double regularity);
+/* The device half of one Adapt::harten(ite) call (mr/adapt.hpp:276-358:
+ * tag init to keep, update_ghost_mr, compute_detail, mr_criteria_op and the
+ * maximum_op tag finalization), without the host mesh update: tags
+ * (CellFlag bits) into `tags`, a device array of
+ * smr_mesh_nb_cells(mesh, SMR_REFERENCE) bytes indexed by the field's
+ * storage, ordered on the context's stream (nothing is copied to the host).
+ * The field's ghosts are updated; the details go to the adaptation's own
+ * buffer.  The mesh is not changed. */
+int smr_adapt_tags(smr_adapt* adapt, int ite, double eps, double regularity, uint8_t* tags);
 /* tags (CellFlag bits, cell_flag.hpp:11-17) and details of the last harten
  * call, indexed by the storage of the mesh they were computed on */
 int64_t smr_adapt_last_size(const smr_adapt* adapt);

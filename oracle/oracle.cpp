@@ -21,9 +21,10 @@
 // update_ghost_mr on adapted meshes, details vanishing on quadratics,
 // Dirichlet<1> ghosts on uniform and adapted meshes, corner ghosts as the
 // reflection of the field, the vertex-ghost cascade of project_corner_below,
-// the textbook upwind update in 2D and 3D, graduation and tiling invariants.
-// The 3D edge ghosts below the finest level (project_corner_below along an
-// edge) are covered only by the product's agreement with this file.
+// the textbook upwind update in 2D and 3D, the edge-ghost cascade of
+// project_corner_below in 3D, graduation and tiling invariants.  Planted
+// mistakes (ORACLE_PLANT below) check that these checks catch a swapped
+// parity coefficient, a dropped flux, a wrong ghost source or sign.
 #include "dense_set.hpp"
 
 #include <cmath>
@@ -33,8 +34,17 @@
 #include <memory>
 #include <string>
 
+// Mistakes planted on purpose, one per value of ORACLE_PLANT, for the mutation
+// test of the pinning checks (tests/test_oracle_mutants.py builds each into a
+// scratch library and expects the checks to fail).  0, the default, plants
+// nothing.
+#ifndef ORACLE_PLANT
+#define ORACLE_PLANT 0
+#endif
+
 namespace oracle
 {
+    constexpr int kPlant = ORACLE_PLANT;
     constexpr int MAXL = 21; // CellArray max_size 20 -> levels 0..20 (samurai_config.hpp:50)
 
     enum MeshId
@@ -388,7 +398,7 @@ namespace oracle
             [&](int x, int y, int z)
             {
                 double uin                                  = U(s, level, x, y, z);
-                U(s, level, x + dir[0], y + dir[1], z + dir[2]) = 2 * val - uin;
+                U(s, level, x + dir[0], y + dir[1], z + dir[2]) = kPlant == 5 ? 2 * val + uin : 2 * val - uin;
             });
     }
 
@@ -404,7 +414,7 @@ namespace oracle
         cells.for_each(
             [&](int x, int y, int z)
             {
-                U(s, level, x + d[0], y + d[1], z + d[2]) = U(s, level, x, y, z);
+                U(s, level, x + d[0], y + d[1], z + d[2]) = kPlant == 4 ? U(s, level, x + d[0], y, z) : U(s, level, x, y, z);
             });
     }
 
@@ -429,7 +439,7 @@ namespace oracle
                 {
                     int cx = (x << dl) + (d[0] == -1 ? last : 0);
                     int cy = (y << dl) + (d[1] == -1 ? last : 0);
-                    int cz = m.dim == 3 ? ((z << dl) + (d[2] == -1 ? last : 0)) : 0;
+                    int cz = m.dim == 3 ? ((z << dl) + (d[2] == -1 ? (kPlant == 6 ? 0 : last) : 0)) : 0;
                     if (m.exists(level, cx, cy, cz))
                     {
                         U(s, pl, x, y, z) = U(s, level, cx, cy, cz);
@@ -590,7 +600,7 @@ namespace oracle
                 {
                     double f     = s.u[size_t(m.index(l - 1, xc + sx - 1, yc + sy - 1))];
                     double coeff = 1.;
-                    coeff *= pair[size_t(px)][size_t(sx)];
+                    coeff *= pair[size_t(kPlant == 8 ? 1 - px : px)][size_t(sx)];
                     coeff *= pair[size_t(py)][size_t(sy)];
                     v += f * coeff;
                 }
@@ -608,7 +618,7 @@ namespace oracle
                         double coeff = 1.;
                         coeff *= pair[size_t(px)][size_t(sx)];
                         coeff *= pair[size_t(py)][size_t(sy)];
-                        coeff *= pair[size_t(pz)][size_t(sz)];
+                        coeff *= pair[size_t(kPlant == 1 ? 1 - pz : pz)][size_t(sz)];
                         v += f * coeff;
                     }
                 }
@@ -712,7 +722,7 @@ namespace oracle
                         double c8[8];
                         for (int c = 0; c < 8; ++c)
                         {
-                            c8[c] = cx[c & 1][ki] * cx[(c >> 1) & 1][kj] * cx[(c >> 2) & 1][kk];
+                            c8[c] = cx[c & 1][ki] * cx[(c >> 1) & 1][kj] * cx[kPlant == 2 ? 1 - ((c >> 2) & 1) : (c >> 2) & 1][kk];
                         }
                         for (int c = 0; c < 8; ++c)
                         {
@@ -1011,7 +1021,7 @@ namespace oracle
                             coeff *= pair[size_t((c >> 1) & 1)][size_t(sy)];
                             if (dim == 3)
                             {
-                                coeff *= pair[size_t((c >> 2) & 1)][size_t(sz)];
+                                coeff *= pair[size_t(kPlant == 7 ? 1 - ((c >> 2) & 1) : (c >> 2) & 1)][size_t(sz)];
                             }
                             vals[c] += f * coeff;
                         }
@@ -1182,7 +1192,7 @@ namespace oracle
                         double FU = flux(a[1], uc, s.u[size_t(m.index(level, x, y + 1, z))]);
                         double FF = flux(a[2], s.u[size_t(m.index(level, x, y, z - 1))], uc);
                         double FB = flux(a[2], uc, s.u[size_t(m.index(level, x, y, z + 1))]);
-                        S         = -FL + FR + -FD + FU + -FF + FB;
+                        S         = kPlant == 3 ? -FL + FR + -FD + FU + FB : -FL + FR + -FD + FU + -FF + FB;
                     }
                     unp1[size_t(m.index(level, x, y, z))] = uc - dt * (S / dx);
                 });

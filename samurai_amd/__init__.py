@@ -9,8 +9,10 @@ Names follow the reference's C++ API for this path (hpc-maths/samurai v0.33.0):
 The library must be built (``python samurai_amd/build.py``); nothing here
 falls back to a CPU implementation.
 """
+import atexit
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -69,6 +71,7 @@ def lib():
         _sig(L, "smr_field_name", "argtypes", [vp])
         _sig(L, "smr_field_on_mesh", "argtypes", [vp, vp])
         _sig(L, "smr_mesh_rebuild_from_tags", "argtypes", [vp, P(ctypes.c_uint8), i64])
+        _sig(L, "smr_mesh_rebuild_from_device_tags", "argtypes", [vp, vp, i64])
         _sig(L, "smr_mesh_validate", "argtypes", [vp])
         _sig(L, "smr_ctx_stream", "restype", vp)
         _sig(L, "smr_ctx_stream", "argtypes", [vp])
@@ -96,6 +99,7 @@ def lib():
         _sig(L, "smr_adapt_destroy", "argtypes", [vp])
         _sig(L, "smr_adapt_run", "argtypes", [vp, dbl, dbl])
         _sig(L, "smr_adapt_harten", "argtypes", [vp, i32, dbl, dbl])
+        _sig(L, "smr_adapt_tags", "argtypes", [vp, i32, dbl, dbl, vp])
         _sig(L, "smr_adapt_last_size", "restype", i64)
         _sig(L, "smr_adapt_last_size", "argtypes", [vp])
         _sig(L, "smr_adapt_last_tags", "argtypes", [vp, P(ctypes.c_uint8), P(dbl)])
@@ -146,6 +150,33 @@ def _dp(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
+class _Handle:
+    """A library object owned by a context.  close() destroys it once; the
+    context closes every object still alive before itself, dependents first
+    (batched and adjoint problems, adaptations, fields, meshes), so no object
+    is destroyed after the stream and device memory it uses."""
+
+    _destroy = None  # name of the smr_*_destroy function
+
+    def _own(self, ctx, h):
+        self.h, self._ctx = h, ctx
+        ctx._children.add(self)
+
+    def close(self):
+        h = getattr(self, "h", None)
+        self.h = None
+        if h and _lib is not None:
+            getattr(_lib, self._destroy)(h)
+
+    def __del__(self):
+        self.close()
+
+
+# the order in which Context.close() destroys the objects still alive
+_CLOSE_ORDER = ("Batch", "AdjointProblem", "Adapt", "ScalarField", "MRMesh")
+_contexts = weakref.WeakSet()
+
+
 class Context:
     """A device (``device >= 0``) or host-only (``device < 0``) context."""
 
@@ -154,6 +185,8 @@ class Context:
         _check(lib().smr_ctx_create(int(device), ctypes.byref(h)))
         self.h = h
         self.device = device
+        self._children = weakref.WeakSet()
+        _contexts.add(self)
 
     def synchronize(self):
         _check(lib().smr_ctx_synchronize(self.h))
@@ -162,23 +195,43 @@ class Context:
         """The hipStream_t (as an integer) all work of this context runs on."""
         return lib().smr_ctx_stream(self.h)
 
+    def close(self):
+        """Destroys every object of this context still alive, then the context
+        (idempotent; also run for every context at interpreter exit)."""
+        if getattr(self, "h", None) is None:
+            return
+        children = list(self._children)
+        for kind in _CLOSE_ORDER:
+            for obj in children:
+                if type(obj).__name__ == kind:
+                    obj.close()
+        for obj in children:
+            obj.close()
+        h, self.h = self.h, None
+        if _lib is not None:
+            _lib.smr_ctx_destroy(h)
+
     def __del__(self):
-        if getattr(self, "h", None) and _lib is not None:
-            lib().smr_ctx_destroy(self.h)
-            self.h = None
+        self.close()
 
 
-class MRMesh:
+@atexit.register
+def _close_contexts():
+    # before the interpreter and the HIP runtime tear down: no library object
+    # is left for a finalizer or a static destructor to release later
+    for ctx in list(_contexts):
+        ctx.close()
+
+
+class MRMesh(_Handle):
     """samurai::MRMesh on the unit square/cube (mr/mesh.hpp)."""
 
-    def __init__(self, ctx, handle, cfg):
-        self.ctx, self.h, self.cfg = ctx, handle, cfg
-        self.dim = cfg.dim
+    _destroy = "smr_mesh_destroy"
 
-    def __del__(self):
-        if getattr(self, "h", None) and _lib is not None:
-            lib().smr_mesh_destroy(self.h)
-            self.h = None
+    def __init__(self, ctx, handle, cfg):
+        self._own(ctx, handle)
+        self.ctx, self.cfg = ctx, cfg
+        self.dim = cfg.dim
 
     @property
     def min_level(self):
@@ -225,6 +278,10 @@ class MRMesh:
         return _check(lib().smr_mesh_rebuild_from_tags(
             self.h, tags.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), tags.size))
 
+    def rebuild_from_device_tags(self, tags_ptr, n):
+        """rebuild_from_tags on the device from n tags at device address tags_ptr."""
+        return _check(lib().smr_mesh_rebuild_from_device_tags(self.h, ctypes.c_void_p(tags_ptr), int(n)))
+
 
 def swap_meshes(a, b):
     """std::swap(mesh_a, mesh_b) of two meshes of one context: the contents
@@ -255,18 +312,16 @@ def make_mesh_from_cells(ctx, cfg, cells):
     return MRMesh(ctx, h, cfg)
 
 
-class ScalarField:
+class ScalarField(_Handle):
     """samurai::ScalarField<mesh, double> stored on the device."""
+
+    _destroy = "smr_field_destroy"
 
     def __init__(self, mesh, name):
         h = ctypes.c_void_p()
         _check(lib().smr_field_create(mesh.h, name.encode(), ctypes.byref(h)))
-        self.mesh, self.h, self.name = mesh, h, name
-
-    def __del__(self):
-        if getattr(self, "h", None) and _lib is not None:
-            lib().smr_field_destroy(self.h)
-            self.h = None
+        self._own(mesh.ctx, h)
+        self.mesh, self.name = mesh, name
 
     def resize(self):
         _check(lib().smr_field_resize(self.h))
@@ -409,18 +464,16 @@ def convection_terms(mesh, velocity=None):
     return row, ptr, src, coef
 
 
-class Adapt:
+class Adapt(_Handle):
     """samurai::Adapt built by make_MRAdapt(u) (mr/adapt.hpp)."""
+
+    _destroy = "smr_adapt_destroy"
 
     def __init__(self, field):
         h = ctypes.c_void_p()
         _check(lib().smr_adapt_create(field.h, ctypes.byref(h)))
-        self.field, self.h = field, h
-
-    def __del__(self):
-        if getattr(self, "h", None) and _lib is not None:
-            lib().smr_adapt_destroy(self.h)
-            self.h = None
+        self._own(field.mesh.ctx, h)
+        self.field = field
 
     def __call__(self, epsilon=1e-4, regularity=1.0):
         """MRadaptation(mra_config().epsilon(eps).regularity(reg)); returns #harten calls."""
@@ -428,6 +481,11 @@ class Adapt:
 
     def harten(self, ite, epsilon, regularity):
         return _check(lib().smr_adapt_harten(self.h, int(ite), float(epsilon), float(regularity)))
+
+    def tags_into(self, ite, epsilon, regularity, tags_ptr):
+        """The device half of harten(ite): tags into the device array at
+        tags_ptr (reference-storage size bytes); the mesh is not changed."""
+        _check(lib().smr_adapt_tags(self.h, int(ite), float(epsilon), float(regularity), ctypes.c_void_p(tags_ptr)))
 
     def last_tags(self):
         n = lib().smr_adapt_last_size(self.h)
@@ -460,13 +518,15 @@ def default_tuning_file():
     return os.path.join(d, "batch_tuning.txt")
 
 
-class Batch:
+class Batch(_Handle):
     """Many independent instances of the advection demo loop advanced together.
 
     instances: list of (u, unp1, a, dt) with u, unp1 ScalarFields on one mesh
     per instance.  step(eps, reg) is, for every instance, MRadaptation(eps,
     reg); unp1.resize(); update_ghost_mr(u); unp1 = u - dt * upwind(a, u);
     swap(u, unp1) -- bit-identical to those per-instance calls."""
+
+    _destroy = "smr_batch_destroy"
 
     def __init__(self, ctx, instances):
         n = len(instances)
@@ -481,12 +541,8 @@ class Batch:
             dt[k] = dk
         h = vp()
         _check(lib().smr_batch_create(ctx.h, n, us, vs, _dp(a), _dp(dt), ctypes.byref(h)))
-        self.ctx, self.instances, self.h = ctx, list(instances), h
-
-    def __del__(self):
-        if getattr(self, "h", None) and _lib is not None:
-            lib().smr_batch_destroy(self.h)
-            self.h = None
+        self._own(ctx, h)
+        self.ctx, self.instances = ctx, list(instances)
 
     def adapt(self, epsilon, regularity):
         return _check(lib().smr_batch_adapt(self.h, float(epsilon), float(regularity)))
@@ -518,13 +574,15 @@ class Batch:
         return bool(_check(lib().smr_batch_load_tuning(self.h, cache_file.encode())))
 
 
-class AdjointProblem:
+class AdjointProblem(_Handle):
     """Frozen-mesh upwind steps of many instances and their adjoint.
 
     meshes: adapted MRMesh objects (not adapted again while this lives);
     dt: per-instance time steps; bc: per-instance Dirichlet values (None: no
     boundary condition).  forward/backward take and return device pointers
     of compact cell vectors (see samurai_amd.torch_adjoint for tensors)."""
+
+    _destroy = "smr_adjoint_destroy"
 
     def __init__(self, ctx, meshes, dt, bc=None, steps=100, checkpoint_every=10):
         n = len(meshes)
@@ -537,13 +595,9 @@ class AdjointProblem:
             bcp = _dp(bc)
         h = vp()
         _check(lib().smr_adjoint_create(ctx.h, n, ms, _dp(dt), bcp, int(steps), int(checkpoint_every), ctypes.byref(h)))
-        self.ctx, self.meshes, self.h = ctx, list(meshes), h
+        self._own(ctx, h)
+        self.ctx, self.meshes = ctx, list(meshes)
         self.n, self.steps, self.checkpoint_every = n, int(steps), int(checkpoint_every)
-
-    def __del__(self):
-        if getattr(self, "h", None) and _lib is not None:
-            lib().smr_adjoint_destroy(self.h)
-            self.h = None
 
     def nb_cells(self, instance=-1):
         return _check(lib().smr_adjoint_nb_cells(self.h, int(instance)))

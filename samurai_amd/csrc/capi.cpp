@@ -345,6 +345,31 @@ extern "C"
             });
     }
 
+    int smr_mesh_rebuild_from_device_tags(smr_mesh* mesh, const uint8_t* tags, int64_t n)
+    {
+        return guarded(
+            [&]
+            {
+                require(mesh && tags, "null argument");
+                smr::Mesh& mm = mesh->m;
+                mm.ctx->require_device();
+                require(n == mm.host.nref, "tag array size differs from the storage size");
+                smr::CellArray ca;
+                if (!smr::device_mesh_update(*mm.ctx, mm.device(), mm.host.cfg, tags, ca))
+                {
+                    return 1;
+                }
+                smr::MRMeshData nm;
+                {
+                    smr::ScopedTimer t3(nullptr, "mesh construction");
+                    nm           = smr::build_mesh(mm.host.cfg, ca);
+                    nm.graduated = true;
+                }
+                mm.replace(std::move(nm), nullptr);
+                return 0;
+            });
+    }
+
     int smr_mesh_rebuild_from_tags(smr_mesh* mesh, const uint8_t* tags, int64_t n)
     {
         return guarded(
@@ -710,6 +735,20 @@ extern "C"
             {
                 require(adapt != nullptr, "null adapt");
                 return smr::harten(adapt->a, ite, eps, regularity) ? 1 : 0;
+            });
+    }
+
+    int smr_adapt_tags(smr_adapt* adapt, int ite, double eps, double regularity, uint8_t* tags)
+    {
+        return guarded(
+            [&]
+            {
+                require(adapt != nullptr, "null adapt");
+                require(tags != nullptr, "null tag array");
+                require(ite >= 0, "negative harten iteration");
+                adapt->a.field->mesh->ctx->require_device();
+                smr::adapt_tags(adapt->a, ite, eps, regularity, tags);
+                return 0;
             });
     }
 

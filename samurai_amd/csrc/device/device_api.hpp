@@ -57,6 +57,17 @@ namespace smr::dev
 
     constexpr int HINT_SHIFT = 8; // 256 cells per hint chunk
 
+    // Per-level dense bitmaps of the unit box at levels [lo, hi] (rebuild.hip):
+    // level l stores each row (y, or (z, y)) in wpr[l] = max(1, 2^l / 64)
+    // 64-bit words from word off[l] (off[lo] = 0, off[hi + 1] = total), x
+    // ascending from bit 0.
+    struct Bits
+    {
+        int lo = 0, hi = -1;
+        int64_t off[22] = {};
+        int64_t wpr[21] = {};
+    };
+
     // One operator applied to a cell range of one instance: the unit of the
     // fused stage program that advances many independent meshes together.
     enum SegmentOp : int32_t
@@ -182,12 +193,38 @@ namespace smr::dev
                   int max_level, hipStream_t s);
     void maximum(int dim, const List& l, int64_t c0, int64_t c1, uint8_t* tag, hipStream_t s);
     void tag_init(const List& l, int64_t c0, int64_t c1, uint8_t* tag, hipStream_t s);
+    // p[0, n) = 0 by a kernel (graph-capturable, unlike hipMemsetAsync)
+    void zero_f64(double* p, int64_t n, hipStream_t s);
+    void zero_u8(uint8_t* p, int64_t n, hipStream_t s);
     // flag[0] = 1 if a cell's tag changes the cell array (cells list)
     void tag_change(const List& l, int64_t c0, int64_t c1, const uint8_t* tag, int min_level, int max_level, int* flag, hipStream_t s);
     void copy_cells(const List& l, int64_t c0, int64_t c1, double* dst, const double* src, hipStream_t s);
     void predict_children(int dim, int pred_s, const List& l, int64_t c0, int64_t c1, double* dst, const double* src, hipStream_t s);
     // one dependency-free stage of outer-ghost recipes
     // (Dirichlet recipes are skipped when the field has no boundary condition)
+    // mesh update of a harten call on bitmaps (rebuild.hip)
+    void bits_zero(const Bits& b, uint64_t* bits, hipStream_t s);
+    // bits = the leaves of a cells work list
+    void bits_from_cells(const List& l, const Bits& b, uint64_t* bits, hipStream_t s);
+    // update_cell_array_from_tag's additions and removals of the leaves of l
+    void bits_tag_marks(int dim, const List& l, const Bits& b, const uint8_t* tag, int min_level, int max_level, uint64_t* add, uint64_t* rem,
+                        hipStream_t s);
+    // out = (old | add) & ~rem
+    void bits_combine(const Bits& b, const uint64_t* old, const uint64_t* add, const uint64_t* rem, uint64_t* out, hipStream_t s);
+    // one pass of make_graduation's fixed point from ca into out (q, o:
+    // scratch bitmaps); *flag = 1 if a cell was refined (flag not cleared)
+    void bits_graduation_pass(int dim, const Bits& b, const uint64_t* ca, uint64_t* q, uint64_t* o, uint64_t* out, int w2, int* flag,
+                              hipStream_t s);
+    // *flag = 1 if a and c differ (flag not cleared)
+    void bits_differ(const Bits& b, const uint64_t* a, const uint64_t* c, int* flag, hipStream_t s);
+    // run-length encoding: counts and exclusive offsets (one per word of
+    // levels [lo, hi]) of the runs of bits, then the runs as (x0, x1, y, z,
+    // level) in (level, z, y, x) order
+    size_t bits_scan_temp_bytes(const Bits& b);
+    void bits_run_offsets(int dim, const Bits& b, const uint64_t* bits, int32_t* counts, int32_t* offsets, void* temp, size_t temp_bytes,
+                          hipStream_t s);
+    void bits_run_emit(int dim, const Bits& b, const uint64_t* bits, const int32_t* offsets, int32_t* runs, hipStream_t s);
+
     void recipes(const int64_t* dst, const int32_t* op, const int64_t* src_ptr, const int64_t* src, int64_t r0, int64_t r1, double* u,
                  bool has_bc, double bc_value, hipStream_t s);
 } // namespace smr::dev

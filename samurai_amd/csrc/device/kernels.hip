@@ -583,6 +583,18 @@ namespace smr::dev
             return unsigned((n + BLOCK - 1) / BLOCK);
         }
 
+        // p[0, n) = 0 (a kernel, not hipMemsetAsync, so that a caller may
+        // capture the launch in a HIP graph)
+        template <class T>
+        __global__ __launch_bounds__(BLOCK) void k_zero(T* p, int64_t n)
+        {
+            const int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
+            if (i < n)
+            {
+                p[i] = T(0);
+            }
+        }
+
         // launch shapes of the upwind kernel: cells per thread (0: wave tiles,
         // -2: column walks (2D), -1: column walks (2D) or wave tiles (3D) when at
         // least 3/4 of the lanes hold a cell, else 4 cells per thread), XCD
@@ -911,6 +923,22 @@ namespace smr::dev
         if (c1 > c0)
         {
             hipLaunchKernelGGL(k_tag_change, dim3(nblocks(c1 - c0)), dim3(BLOCK), 0, s, l, c0, c1, tag, min_level, max_level, flag);
+        }
+    }
+
+    void zero_f64(double* p, int64_t n, hipStream_t s)
+    {
+        if (n > 0)
+        {
+            hipLaunchKernelGGL(k_zero<double>, dim3(nblocks(n)), dim3(BLOCK), 0, s, p, n);
+        }
+    }
+
+    void zero_u8(uint8_t* p, int64_t n, hipStream_t s)
+    {
+        if (n > 0)
+        {
+            hipLaunchKernelGGL(k_zero<uint8_t>, dim3(nblocks(n)), dim3(BLOCK), 0, s, p, n);
         }
     }
 

@@ -321,13 +321,17 @@ namespace smr
     // ------------------------------------------------------------------
     Ctx::~Ctx()
     {
+        // work still queued on the stream may use d_err and the cached blocks
+        if (stream)
+        {
+            (void)hipStreamSynchronize(stream);
+        }
         if (d_err)
         {
             (void)hipFree(d_err);
         }
         if (stream)
         {
-            (void)hipStreamSynchronize(stream);
             dev_cache_release(stream);
             (void)hipStreamSynchronize(stream);
             (void)hipStreamDestroy(stream);
@@ -1092,6 +1096,115 @@ namespace smr
         }
     } // namespace
 
+    dev::Bits make_bits(const MeshConfig& cfg)
+    {
+        dev::Bits b;
+        b.lo        = cfg.min_level;
+        b.hi        = cfg.max_level;
+        int64_t off = 0;
+        for (int l = 0; l <= 20; ++l)
+        {
+            const int64_t n = int64_t(1) << l;
+            b.wpr[l]        = std::max<int64_t>(1, n / 64);
+            b.off[l]        = off;
+            if (l >= b.lo && l <= b.hi)
+            {
+                off += b.wpr[l] * (cfg.dim == 3 ? n * n : n);
+            }
+        }
+        b.off[21] = off;
+        return b;
+    }
+
+    bool device_mesh_update(Ctx& ctx, DevMesh& dm, const MeshConfig& cfg, const uint8_t* tag, CellArray& out)
+    {
+        hipStream_t s       = ctx.stream;
+        const dev::Bits b   = make_bits(cfg);
+        const int64_t nw    = b.off[b.hi + 1];
+        const size_t wbytes = size_t(std::max<int64_t>(nw, 1)) * 8;
+        if (!dm.has_leaf_bits)
+        {
+            dev::bits_from_cells(dm.cells.d, b, static_cast<uint64_t*>(dm.leaf_bits.get(wbytes, s)), s);
+            dm.has_leaf_bits = true;
+        }
+        const uint64_t* old = static_cast<const uint64_t*>(dm.leaf_bits.p);
+        DevBuffer work;
+        uint64_t* w   = static_cast<uint64_t*>(work.get(5 * wbytes, s));
+        uint64_t* add = w;
+        uint64_t* rem = w + nw;
+        uint64_t* cur = w + 2 * nw;
+        uint64_t* nxt = w + 3 * nw;
+        uint64_t* q   = w + 4 * nw;
+        uint64_t* o   = add; // free once the marks are combined
+        int* flag     = ctx.d_flag();
+        int h_flag    = 0;
+        {
+            ScopedTimer t(&ctx, "update_cell_array_from_tag");
+            dev::bits_tag_marks(cfg.dim, dm.cells.d, b, tag, cfg.min_level, cfg.max_level, add, rem, s);
+            dev::bits_combine(b, old, add, rem, cur, s);
+        }
+        {
+            ScopedTimer t(&ctx, "make_graduation");
+            // make_graduation's fixed point (graduation.hpp:623-723): passes
+            // until one refines nothing
+            for (int pass = 0;; ++pass)
+            {
+                if (pass > 4 * 21)
+                {
+                    throw std::logic_error("make_graduation: no fixed point");
+                }
+                hip_check(hipMemsetAsync(flag, 0, sizeof(int), s), "hipMemsetAsync(flag)");
+                dev::bits_graduation_pass(cfg.dim, b, cur, q, o, nxt, 2 * cfg.grad_width, flag, s);
+                hip_check(hipMemcpyAsync(&h_flag, flag, sizeof(int), hipMemcpyDeviceToHost, s), "hipMemcpyAsync(flag)");
+                ctx.sync();
+                if (!h_flag)
+                {
+                    break;
+                }
+                std::swap(cur, nxt);
+            }
+            hip_check(hipMemsetAsync(flag, 0, sizeof(int), s), "hipMemsetAsync(flag)");
+            dev::bits_differ(b, cur, old, flag, s);
+            hip_check(hipMemcpyAsync(&h_flag, flag, sizeof(int), hipMemcpyDeviceToHost, s), "hipMemcpyAsync(flag)");
+            ctx.sync();
+        }
+        if (!h_flag)
+        {
+            return false;
+        }
+        ScopedTimer t(&ctx, "leaves to host");
+        // run-length encoding of the new leaves, in traversal order
+        const int64_t nl      = nw - b.off[b.lo];
+        const size_t tbytes   = dev::bits_scan_temp_bytes(b);
+        DevBuffer scan;
+        char* sp              = static_cast<char*>(scan.get(size_t(2 * std::max<int64_t>(nl, 1)) * 4 + 256 + tbytes, s));
+        int32_t* counts       = reinterpret_cast<int32_t*>(sp);
+        int32_t* offsets      = counts + std::max<int64_t>(nl, 1);
+        void* temp            = sp + ((size_t(2 * std::max<int64_t>(nl, 1)) * 4 + 255) & ~size_t(255));
+        dev::bits_run_offsets(cfg.dim, b, cur, counts, offsets, temp, tbytes, s);
+        int32_t last[2] = {0, 0};
+        hip_check(hipMemcpyAsync(&last[0], offsets + nl - 1, 4, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(runs)");
+        hip_check(hipMemcpyAsync(&last[1], counts + nl - 1, 4, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(runs)");
+        ctx.sync();
+        const int64_t nruns = int64_t(last[0]) + last[1];
+        std::vector<int32_t> runs(size_t(nruns) * 5);
+        if (nruns > 0)
+        {
+            DevBuffer rb;
+            int32_t* dr = static_cast<int32_t*>(rb.get(size_t(nruns) * 20, s));
+            dev::bits_run_emit(cfg.dim, b, cur, offsets, dr, s);
+            hip_check(hipMemcpyAsync(runs.data(), dr, size_t(nruns) * 20, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(runs)");
+            ctx.sync();
+        }
+        out = empty_cell_array(cfg.dim);
+        for (int64_t k = 0; k < nruns; ++k)
+        {
+            const int32_t* r = runs.data() + 5 * k;
+            out[size_t(r[4])].push(r[2], r[3], r[0], r[1]);
+        }
+        return true;
+    }
+
     dev::Thresholds make_thresholds(const MeshConfig& cfg, int ite, double eps, double regularity)
     {
         const int dim = cfg.dim, minl = cfg.min_level, maxl = cfg.max_level;
@@ -1109,21 +1222,19 @@ namespace smr
         return thr;
     }
 
-    bool harten(AdaptState& st, int ite, double eps, double regularity)
+    void harten_tags(Field& f, int ite, double eps, double regularity, uint8_t* tag, double* det)
     {
-        Field& f        = *st.field;
         f.check_current();
-        Mesh& m         = *f.mesh;
-        Ctx& ctx        = *m.ctx;
-        hipStream_t s   = ctx.stream;
+        Mesh& m              = *f.mesh;
+        Ctx& ctx             = *m.ctx;
+        hipStream_t s        = ctx.stream;
         const MeshConfig cfg = m.host.cfg;
         const int dim = cfg.dim, minl = cfg.min_level, maxl = cfg.max_level;
         DevMesh& dm     = m.device_adapt();
         const int64_t n = m.host.nref;
-        ensure_adapt_buffers(st, n);
-        hip_check(hipMemsetAsync(st.detail, 0, size_t(n) * 8, s), "hipMemsetAsync(detail)");
-        hip_check(hipMemsetAsync(st.tag, 0, size_t(n), s), "hipMemsetAsync(tag)");
-        dev::tag_init(dm.cells.d, 0, dm.cells.d.ncells, st.tag, s);
+        dev::zero_f64(det, n, s);
+        dev::zero_u8(tag, n, s);
+        dev::tag_init(dm.cells.d, 0, dm.cells.d.ncells, tag, s);
 
         update_ghost_mr(f);
 
@@ -1132,7 +1243,7 @@ namespace smr
             ScopedTimer timer(&ctx, "detail computation");
             int64_t c0, c1;
             dm.detail.cells_of_levels(minl > 0 ? minl - 1 : 0, maxl - ite, c0, c1);
-            dev::detail(dim, cfg.pred_s, dm.detail.d, c0, c1, f.d, st.detail, s);
+            dev::detail(dim, cfg.pred_s, dm.detail.d, c0, c1, f.d, det, s);
         }
         // tag cells: fine levels [min, max-ite], i.e. coarse levels [min-1, max-ite-1]
         {
@@ -1140,7 +1251,7 @@ namespace smr
             const dev::Thresholds thr = make_thresholds(cfg, ite, eps, regularity);
             int64_t c0, c1;
             dm.criteria.cells_of_levels(minl - 1, maxl - ite, c0, c1);
-            dev::criteria(dim, dm.criteria.d, c0, c1, st.detail, st.tag, thr, minl, maxl, s);
+            dev::criteria(dim, dm.criteria.d, c0, c1, det, tag, thr, minl, maxl, s);
         }
         // tag finalization: levels max..1, sequential
         {
@@ -1149,9 +1260,34 @@ namespace smr
             {
                 int64_t c0, c1;
                 dm.maximum.cells_of_levels(level - 1, level, c0, c1);
-                dev::maximum(dim, dm.maximum.d, c0, c1, st.tag, s);
+                dev::maximum(dim, dm.maximum.d, c0, c1, tag, s);
             }
         }
+    }
+
+    void adapt_tags(AdaptState& st, int ite, double eps, double regularity, uint8_t* tag)
+    {
+        Field& f = *st.field;
+        f.check_current();
+        ensure_adapt_buffers(st, f.mesh->host.nref);
+        harten_tags(f, ite, eps, regularity, tag, st.detail);
+        st.last_size      = 0; // the tags are the caller's, not the adaptation's
+        st.host_tag_valid = false;
+    }
+
+    bool harten(AdaptState& st, int ite, double eps, double regularity)
+    {
+        Field& f        = *st.field;
+        f.check_current();
+        Mesh& m         = *f.mesh;
+        Ctx& ctx        = *m.ctx;
+        hipStream_t s   = ctx.stream;
+        const MeshConfig cfg = m.host.cfg;
+        const int minl = cfg.min_level, maxl = cfg.max_level;
+        const int64_t n = m.host.nref;
+        ensure_adapt_buffers(st, n);
+        harten_tags(f, ite, eps, regularity, st.tag, st.detail);
+        DevMesh& dm = *m.dev;
         std::unique_ptr<DevMesh> ndev;
         MRMeshData nm;
         {
@@ -1173,31 +1309,10 @@ namespace smr
                     return true;
                 }
             }
-            if (n > st.host_tag_capacity)
-            {
-                if (st.host_tag)
-                {
-                    hip_check(hipHostFree(st.host_tag), "hipHostFree(tag)");
-                    st.host_tag = nullptr;
-                }
-                hip_check(hipHostMalloc(reinterpret_cast<void**>(&st.host_tag), size_t(n), hipHostMallocDefault), "hipHostMalloc(tag)");
-                st.host_tag_capacity = n;
-            }
-            hip_check(hipMemcpyAsync(st.host_tag, st.tag, size_t(n), hipMemcpyDeviceToHost, s), "hipMemcpyAsync(tag)");
-            ctx.sync();
-            st.host_tag_valid = true;
-
-            // mesh update on the host (graduation.hpp:743-843, 573-726)
+            // mesh update (graduation.hpp:743-843, 573-726): on the device,
+            // the host receives the new leaves only when they change
             CellArray new_ca;
-            {
-                ScopedTimer t1(nullptr, "update_cell_array_from_tag");
-                new_ca = update_cell_array_from_tag(m.host, st.host_tag);
-            }
-            {
-                ScopedTimer t2(nullptr, "make_graduation");
-                make_graduation(new_ca, dim, cfg.grad_width);
-            }
-            if (ca_equal(m.host.ids[CELLS], new_ca))
+            if (!device_mesh_update(ctx, dm, cfg, st.tag, new_ca))
             {
                 return true;
             }

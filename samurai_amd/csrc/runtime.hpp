@@ -88,6 +88,32 @@ namespace smr
     void dev_cache_release(hipStream_t s);
     constexpr size_t kDevCacheLimit = size_t(4) << 30;
 
+    // Device memory from dev_alloc, released in the order of its stream.
+    struct DevBuffer
+    {
+        void* p         = nullptr;
+        hipStream_t s   = nullptr;
+        DevBuffer()     = default;
+        DevBuffer(const DevBuffer&)            = delete;
+        DevBuffer& operator=(const DevBuffer&) = delete;
+        ~DevBuffer() { reset(); }
+        void reset()
+        {
+            if (p)
+            {
+                dev_free(p, s);
+                p = nullptr;
+            }
+        }
+        void* get(size_t bytes, hipStream_t stream)
+        {
+            reset();
+            s = stream;
+            p = dev_alloc(bytes, stream);
+            return p;
+        }
+    };
+
     struct DevWorkList
     {
         WorkList host;
@@ -135,6 +161,9 @@ namespace smr
         std::unique_ptr<Conv> burgers;
         // the heterogeneous (velocity field) convection terms of this image
         std::unique_ptr<Conv> heterogeneous;
+        // the leaves as per-level bitmaps (device mesh update), built on first use
+        DevBuffer leaf_bits;
+        bool has_leaf_bits = false;
         // the work lists of a complete image (cells first)
         std::vector<DevWorkList*> lists() { return {&cells, &projection, &prediction, &detail, &criteria, &maximum}; }
     };
@@ -253,6 +282,13 @@ namespace smr
     // DevWorkList upload + resolve against the given tables
     void upload_list(Ctx& ctx, DevWorkList& l, const dev::Table& t0, const dev::Table& t1);
 
+    // bitmap layout of the levels of a mesh configuration
+    dev::Bits make_bits(const MeshConfig& cfg);
+    // update_cell_array_from_tag + make_graduation on the device from tags
+    // indexed by dm's storage (graduation.hpp:573-843): false if the leaves
+    // do not change, else true with the new leaves in out
+    bool device_mesh_update(Ctx& ctx, DevMesh& dm, const MeshConfig& cfg, const uint8_t* tag, CellArray& out);
+
     // criteria thresholds of harten iteration `ite` (mr/adapt.hpp:325-339)
     dev::Thresholds make_thresholds(const MeshConfig& cfg, int ite, double eps, double regularity);
 
@@ -275,4 +311,11 @@ namespace smr
     int adapt(AdaptState& st, double eps, double regularity, int max_iterations = -1);
     // one harten call; true at the fixed point
     bool harten(AdaptState& st, int ite, double eps, double regularity);
+    // the device half of a harten call (mr/adapt.hpp:276-358): tag init, ghost
+    // update, details, criteria and tag finalization of iteration ite, into
+    // tag and det (device arrays of the mesh's nref slots); no host copy
+    void harten_tags(Field& f, int ite, double eps, double regularity, uint8_t* tag, double* det);
+    // harten_tags into a caller's device tag array, details in st's buffer
+    void adapt_tags(AdaptState& st, int ite, double eps, double regularity, uint8_t* tag);
+
 } // namespace smr

@@ -16,10 +16,14 @@ def pytest_configure(config):
 @pytest.fixture(scope="session")
 def gpu_ctx():
     import samurai_amd as sa
-    return sa.Context(0)
+    ctx = sa.Context(0)
+    yield ctx
+    ctx.close()
 
 
 @pytest.fixture(scope="session")
 def host_ctx():
     import samurai_amd as sa
-    return sa.Context(-1)
+    ctx = sa.Context(-1)
+    yield ctx
+    ctx.close()

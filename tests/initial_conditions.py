@@ -34,11 +34,35 @@ def sphere_3d(cells, center=(0.5, 0.5, 0.5), radius=0.15):
     return np.where(d <= radius * radius, 1.0, 0.0)
 
 
+# exp(x) from IEEE-754 +, -, * and exact scaling only, so that the seeded
+# fields are bit-identical on every host (the libm and SIMD exp of numpy may
+# differ in the last place between CPUs).  Cody-Waite reduction
+# x = k ln2 + r, |r| <= ln2 / 2, with ln2 split so that k * LN2_HI is exact,
+# then the Taylor polynomial of degree 16 in Horner form (truncation error
+# below 1e-21 relative) and the scaling by 2^k.
+_LN2_HI = 6.93147180369123816490e-01
+_LN2_LO = 1.90821492927058770002e-10
+_INV_LN2 = 1.44269504088896338700e+00
+_TAYLOR = [1.0 / np.prod(np.arange(1, k + 1, dtype=np.float64)) for k in range(17)]
+
+
+def det_exp(x):
+    x = np.asarray(x, dtype=np.float64)
+    xc = np.clip(x, -745.0, 709.0)
+    k = np.rint(xc * _INV_LN2)
+    r = (xc - k * _LN2_HI) - k * _LN2_LO
+    p = np.full_like(r, _TAYLOR[16])
+    for c in _TAYLOR[15::-1]:
+        p = p * r + c
+    return np.where(x < -745.0, 0.0, np.ldexp(p, k.astype(np.int64)))
+
+
 def seeded_2d(cells, seed, n_boxes=3, n_gauss=2):
     """Seeded sum of indicator boxes and Gaussians (the large 2D workload).
 
     Draw order (numpy Generator PCG64(seed)): for each box, centre x, centre y,
-    side; then for each Gaussian, centre x, centre y, sigma, amplitude.
+    side; then for each Gaussian, centre x, centre y, sigma, amplitude.  The
+    Gaussians use det_exp, so the field is the same bit for bit on every host.
     """
     rng = np.random.Generator(np.random.PCG64(seed))
     c = centers(cells)
@@ -53,7 +77,7 @@ def seeded_2d(cells, seed, n_boxes=3, n_gauss=2):
         sigma = rng.uniform(0.02, 0.08)
         amp = rng.uniform(0.5, 1.0)
         r2 = (c[:, 0] - cx) * (c[:, 0] - cx) + (c[:, 1] - cy) * (c[:, 1] - cy)
-        u = u + amp * np.exp(-r2 / (2.0 * sigma * sigma))
+        u = u + amp * det_exp(-r2 / (2.0 * sigma * sigma))
     return u
 
 

@@ -6,7 +6,9 @@ import os
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+# SMR_ORACLE_LIB selects another build of the oracle (the mutation test's
+# libraries with a planted mistake)
+LIB_PATH = os.environ.get("SMR_ORACLE_LIB") or os.path.join(ROOT, "oracle", "liboracle.so")
 
 CELLS, CELLS_AND_GHOSTS, PROJ_CELLS, UNION_CELLS, REFERENCE = range(5)
 

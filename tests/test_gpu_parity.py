@@ -98,13 +98,20 @@ def test_harten_matches_oracle(gpu_ctx, dim, lmin, lmax, s, ic):
     """Flags, details, new mesh and transferred field of every harten call."""
     say("test_harten_matches_oracle", dim, lmin, lmax, s, ic)
     o, mesh, u = pair(gpu_ctx, dim, lmin, lmax, s, ic=ic)
+    import torch
     ad = sa.make_MRAdapt(u)
     eps = 1e-4 if dim == 2 else 1e-3
     for ite in range(lmax - lmin):
+        # the device half alone (smr_adapt_tags) first, into a caller's array
+        dev_tags = torch.full((mesh.nb_cells(sa.REFERENCE),), 0xEE, dtype=torch.uint8, device="cuda")
+        torch.cuda.synchronize()
+        ad.tags_into(ite, eps, 2.0, dev_tags.data_ptr())
+        gpu_ctx.synchronize()
         fo = o.harten(ite, eps, 2.0)
         fp = ad.harten(ite, eps, 2.0)
         to, do = o.last_tags()
         tp, dp = ad.last_tags()
+        assert np.array_equal(to, dev_tags.cpu().numpy()), ite
         assert fo == fp
         assert np.array_equal(to, tp), ite
         assert np.array_equal(bits(do), bits(dp)), ite

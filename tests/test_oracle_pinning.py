@@ -351,3 +351,36 @@ def test_vertex_ghosts_cascade_from_the_corner_leaf(dim):
                 checked += 1
                 deepest = max(deepest, lf - l)
     assert checked >= 2 ** dim and deepest >= 2
+
+
+def test_edge_ghosts_cascade_below_the_finest_level():
+    """project_corner_below along the twelve edges of the cube
+    (algorithm/update_outer_ghost.hpp:264-336): on a uniform level-4 mesh with
+    min_level 2, every edge ghost of the reference mesh at levels 3 and 2
+    (outside the box along exactly two axes) copies the level-4 edge ghost of
+    its corner-most child -- the child next to the box along the two outside
+    axes and the first along the edge -- whose value is the field at the
+    reflection of that child's centre (corner_oracle)."""
+    dim, L = 3, 4
+    f = lambda q: 1.0 + 0.5 * q[:, 0] * q[:, 0] + q[:, 0] + 2.0 * q[:, 1] + 3.0 * q[:, 2]
+    sim = OracleSim(dim, 2, L, pred_s=1)
+    sim.set_values(f(centers(sim.cells())))
+    sim.update_ghost_mr()
+    st = sim.storage()
+    ref = sim.cells(REFERENCE)
+    checked = 0
+    for l in (3, 2):
+        d = L - l
+        rows = ref[ref[:, 0] == l]
+        n = 1 << l
+        for r in rows:
+            g = [int(v) for v in r[1:4]]
+            out = [(-1 if g[k] < 0 else (1 if g[k] >= n else 0)) for k in range(3)]
+            if sum(o != 0 for o in out) != 2 or any(g[k] < -1 or g[k] > n for k in range(3)):
+                continue
+            child = [(g[k] << d) + ((1 << d) - 1 if out[k] == -1 else 0) for k in range(3)]
+            fine = np.array([[L, child[0], child[1], child[2], 0]])
+            expect = corner_oracle(centers(fine), dim, f)[0]
+            assert abs(st[int(r[4])] - expect) <= 1e-11, (l, g)
+            checked += 1
+    assert checked >= 12 * 2
