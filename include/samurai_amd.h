@@ -157,6 +157,17 @@ int smr_adapt_harten(smr_adapt* adapt, int ite, double eps, double regularity);
  * The field's ghosts are updated; the details go to the adaptation's own
  * buffer.  The mesh is not changed. */
 int smr_adapt_tags(smr_adapt* adapt, int ite, double eps, double regularity, uint8_t* tags);
+/* No counterpart in the reference.  smr_adapt_tags for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops): it calls no
+ * hipMalloc, hipFree or allocating routine and clears with kernels, not
+ * hipMemsetAsync.  The details go to a static device array of 2^21 FP64
+ * slots per GPU (16 MiB, held until the process ends), so the mesh's
+ * reference storage must not exceed 2^21 slots (SMR_ERR_ARGUMENT
+ * otherwise), and the mesh version's device programs must already exist:
+ * run smr_adapt_tags or smr_adapt_run once on it first (SMR_ERR_ARGUMENT
+ * otherwise).  Every call on one GPU shares that array: calls that overlap
+ * on one GPU (several contexts or threads) race. */
+int smr_adapt_tags_noalloc(smr_adapt* adapt, int ite, double eps, double regularity, uint8_t* tags);
 /* tags (CellFlag bits, cell_flag.hpp:11-17) and details of the last harten
  * call, indexed by the storage of the mesh they were computed on */
 int64_t smr_adapt_last_size(const smr_adapt* adapt);

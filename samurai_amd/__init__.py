@@ -100,6 +100,7 @@ def lib():
         _sig(L, "smr_adapt_run", "argtypes", [vp, dbl, dbl])
         _sig(L, "smr_adapt_harten", "argtypes", [vp, i32, dbl, dbl])
         _sig(L, "smr_adapt_tags", "argtypes", [vp, i32, dbl, dbl, vp])
+        _sig(L, "smr_adapt_tags_noalloc", "argtypes", [vp, i32, dbl, dbl, vp])
         _sig(L, "smr_adapt_last_size", "restype", i64)
         _sig(L, "smr_adapt_last_size", "argtypes", [vp])
         _sig(L, "smr_adapt_last_tags", "argtypes", [vp, P(ctypes.c_uint8), P(dbl)])
@@ -486,6 +487,11 @@ class Adapt(_Handle):
         """The device half of harten(ite): tags into the device array at
         tags_ptr (reference-storage size bytes); the mesh is not changed."""
         _check(lib().smr_adapt_tags(self.h, int(ite), float(epsilon), float(regularity), ctypes.c_void_p(tags_ptr)))
+
+    def tags_into_noalloc(self, ite, epsilon, regularity, tags_ptr):
+        """tags_into without allocating: details in the library's static
+        per-GPU scratch array (the mesh's programs must exist already)."""
+        _check(lib().smr_adapt_tags_noalloc(self.h, int(ite), float(epsilon), float(regularity), ctypes.c_void_p(tags_ptr)))
 
     def last_tags(self):
         n = lib().smr_adapt_last_size(self.h)

@@ -177,6 +177,7 @@ extern "C"
                         smr::hip_check(hipMalloc(reinterpret_cast<void**>(&c->c.d_err), 2 * sizeof(int)), "hipMalloc(err)");
                         smr::hip_check(hipMemsetAsync(c->c.d_err, 0, 2 * sizeof(int), c->c.stream), "hipMemsetAsync(err)");
                         c->c.sync();
+                        smr::noalloc_init(device);
                     }
                     catch (...)
                     {
@@ -748,6 +749,20 @@ extern "C"
                 require(ite >= 0, "negative harten iteration");
                 adapt->a.field->mesh->ctx->require_device();
                 smr::adapt_tags(adapt->a, ite, eps, regularity, tags);
+                return 0;
+            });
+    }
+
+    int smr_adapt_tags_noalloc(smr_adapt* adapt, int ite, double eps, double regularity, uint8_t* tags)
+    {
+        return guarded(
+            [&]
+            {
+                require(adapt != nullptr, "null adapt");
+                require(tags != nullptr, "null tag array");
+                require(ite >= 0, "negative harten iteration");
+                adapt->a.field->mesh->ctx->require_device();
+                smr::adapt_tags_noalloc(*adapt->a.field, ite, eps, regularity, tags);
                 return 0;
             });
     }

@@ -318,4 +318,13 @@ namespace smr
     // harten_tags into a caller's device tag array, details in st's buffer
     void adapt_tags(AdaptState& st, int ite, double eps, double regularity, uint8_t* tag);
 
+    // slots (doubles) of the static device array adapt_tags_noalloc keeps its
+    // details in (16 MiB per GPU, held until the process ends)
+    constexpr int64_t kNoAllocSlots = int64_t(1) << 21;
+    // takes the device address of that array on `device` (current); once per device
+    void noalloc_init(int device);
+    // harten_tags with the details in the static array: no allocation.  The
+    // mesh's ghost and adapt programs must exist (built by an earlier
+    // allocating call on this mesh version); nref <= kNoAllocSlots
+    void adapt_tags_noalloc(Field& f, int ite, double eps, double regularity, uint8_t* tag);
 } // namespace smr
