@@ -110,6 +110,41 @@ int smr_mesh_rebuild_from_tags(smr_mesh* mesh, const uint8_t* tags, int64_t n);
  * the fixed point, 0 if the mesh was rebuilt. */
 int smr_mesh_rebuild_from_device_tags(smr_mesh* mesh, const uint8_t* tags, int64_t n);
 
+/* The device image of a mesh version, for user kernels that walk the leaves
+ * and look cells up (for_each_interval(mesh, f), algorithm.hpp:75-132, with a
+ * device f; include/samurai_amd/device.hpp).  Pointers are device addresses,
+ * valid until the mesh changes (adaptation, assignment, swap, load).
+ *  - reference interval table (LevelCellArray of mesh[reference] per level,
+ *    level_cell_array.hpp:68-256): rows of level l are
+ *    [row_off[l], row_off[l+1]) with keys ((int64)z << 32) + (y + 2^31),
+ *    ascending; row r owns the x-runs [row_run[r], row_run[r+1]) with
+ *    [run_start, run_end) and storage index run_index + x;
+ *  - the leaves (mesh[cells]) as x-runs (x0, x1, y, z, level), 5 int32 each,
+ *    in traversal order; cell_prefix[r] is the first cell of run r
+ *    (cell_prefix[nb_cell_runs] = nb_cells); cell_hint[j] is the run holding
+ *    cell 256 j; cell_base[r * cell_base_stride] + x is the storage index of
+ *    cell x of run r. */
+typedef struct smr_mesh_view
+{
+    int dim;
+    int min_level;
+    int max_level;
+    const int64_t* row_key;
+    const int64_t* row_off; /* 22 entries */
+    const int64_t* row_run;
+    const int32_t* run_start;
+    const int32_t* run_end;
+    const int64_t* run_index;
+    const int32_t* cell_runs;
+    const int64_t* cell_prefix;
+    const int32_t* cell_hint;
+    const int64_t* cell_base;
+    int64_t cell_base_stride;
+    int64_t nb_cell_runs;
+    int64_t nb_cells;
+} smr_mesh_view;
+int smr_mesh_device_view(smr_mesh* mesh, smr_mesh_view* out);
+
 /* make_scalar_field<double>(name, mesh) (field/scalar_field.hpp:171-176) */
 int smr_field_create(smr_mesh* mesh, const char* name, smr_field** out);
 int smr_field_destroy(smr_field* field);
@@ -186,6 +221,11 @@ void* smr_ctx_stream(smr_ctx* ctx);
  * distinct cell read (the leaves and their face neighbours) plus 8 bytes per
  * leaf written. */
 int64_t smr_mesh_upwind_bytes(const smr_mesh* mesh);
+/* Algorithmic HBM bytes of one update_ghost_mr on the mesh: per projection
+ * ghost 8 * (2^dim + 1) (children read, ghost written), per prediction ghost
+ * 8 written plus 8 per distinct coarse stencil value read, per outer-ghost
+ * recipe 8 * (1 + its sources). */
+int64_t smr_mesh_ghost_update_bytes(const smr_mesh* mesh);
 /* Phase timers under the reference's ScopedTimer names (timers.hpp:979-1001;
  * "mesh adaptation", "ghost update", "detail computation", "tag cells",
  * "tag finalization", "mesh update", "fields update", "field expressions"),

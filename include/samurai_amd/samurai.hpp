@@ -89,6 +89,14 @@ namespace samurai_amd
         return context_slot().get();
     }
 
+    // shared ownership of the context: meshes and fields keep it alive, so
+    // finalize() before they go out of scope destroys it after them
+    inline std::shared_ptr<smr_ctx> context_holder()
+    {
+        context();
+        return context_slot();
+    }
+
     // ------------------------------------------------------------------
     // configuration
     // ------------------------------------------------------------------
@@ -184,24 +192,45 @@ namespace samurai_amd
       public:
         static constexpr std::size_t dim = dim_;
         explicit MRMesh(const smr_mesh_config& cfg)
+            : ctx_(context_holder())
         {
             smr_mesh* m = nullptr;
-            check(smr_mesh_create_box(context(), &cfg, &m));
+            check(smr_mesh_create_box(ctx_.get(), &cfg, &m));
             h_.reset(m, [](smr_mesh* p) { smr_mesh_destroy(p); });
-            cfg_ = cfg;
         }
         // mra::make_empty_mesh(cfg): no cells, to be assigned or loaded
         struct empty_tag
         {
         };
         MRMesh(const smr_mesh_config& cfg, empty_tag)
+            : ctx_(context_holder())
         {
             smr_mesh* m = nullptr;
-            check(smr_mesh_create_from_cells(context(), &cfg, nullptr, 0, &m));
+            check(smr_mesh_create_from_cells(ctx_.get(), &cfg, nullptr, 0, &m));
             h_.reset(m, [](smr_mesh* p) { smr_mesh_destroy(p); });
-            cfg_ = cfg;
         }
-        MRMesh(const MRMesh&)            = default;
+        // a copy is a new mesh with the same cells and configuration (as
+        // samurai's MRMesh copy), independent of the original
+        MRMesh(const MRMesh& o)
+            : ctx_(o.ctx_)
+        {
+            smr_mesh_config cfg{};
+            check(smr_mesh_get_config(o.h_.get(), &cfg));
+            const int64_t n = smr_mesh_nb_cells(o.h_.get(), SMR_CELLS);
+            std::vector<int64_t> raw(size_t(std::max<int64_t>(n, 0)) * 5);
+            check(smr_mesh_get_cells(o.h_.get(), SMR_CELLS, raw.data()));
+            std::vector<int32_t> rows(size_t(std::max<int64_t>(n, 0)) * 4);
+            for (int64_t k = 0; k < n; ++k)
+            {
+                for (int d = 0; d < 4; ++d)
+                {
+                    rows[size_t(4 * k + d)] = int32_t(raw[size_t(5 * k + d)]);
+                }
+            }
+            smr_mesh* m = nullptr;
+            check(smr_mesh_create_from_cells(ctx_.get(), &cfg, rows.data(), n, &m));
+            h_.reset(m, [](smr_mesh* p) { smr_mesh_destroy(p); });
+        }
         MRMesh(MRMesh&&)                 = default;
         MRMesh& operator=(const MRMesh&) = delete;
         // mesh = mra::make_mesh(...): the new contents are swapped into this
@@ -212,35 +241,49 @@ namespace samurai_amd
             if (h_ && o.h_)
             {
                 check(smr_mesh_swap(h_.get(), o.h_.get()));
-                std::swap(cfg_, o.cfg_);
             }
             else
             {
+                ctx_ = o.ctx_;
                 h_   = std::move(o.h_);
-                cfg_ = o.cfg_;
             }
             return *this;
         }
-        // re-read the configuration after the mesh was replaced through the C ABI
+        // kept for callers of the earlier interface: the configuration is
+        // always read from the mesh itself
         void refresh()
         {
-            check(smr_mesh_get_config(h_.get(), &cfg_));
         }
         smr_mesh* handle() const
         {
             return h_.get();
         }
+        // shared ownership of the mesh for the objects that use it
+        const std::shared_ptr<smr_mesh>& shared_handle() const
+        {
+            return h_;
+        }
+        const std::shared_ptr<smr_ctx>& shared_context() const
+        {
+            return ctx_;
+        }
         std::size_t nb_cells(mesh_id_t id = mesh_id_t::cells) const
         {
             return std::size_t(smr_mesh_nb_cells(h_.get(), int(id)));
         }
+        smr_mesh_config config() const
+        {
+            smr_mesh_config c{};
+            check(smr_mesh_get_config(h_.get(), &c));
+            return c;
+        }
         int min_level() const
         {
-            return cfg_.min_level;
+            return config().min_level;
         }
         int max_level() const
         {
-            return cfg_.max_level;
+            return config().max_level;
         }
         double cell_length(int level) const
         {
@@ -248,7 +291,7 @@ namespace samurai_amd
         }
         double min_cell_length() const
         {
-            return cell_length(cfg_.max_level);
+            return cell_length(max_level());
         }
         std::vector<Cell<dim_>> cells(mesh_id_t id = mesh_id_t::cells) const
         {
@@ -272,8 +315,9 @@ namespace samurai_amd
         }
 
       private:
+        // the context outlives the mesh (members are destroyed last to first)
+        std::shared_ptr<smr_ctx> ctx_;
         std::shared_ptr<smr_mesh> h_;
-        smr_mesh_config cfg_{};
     };
 
     namespace mra
@@ -357,6 +401,8 @@ namespace samurai_amd
         ScalarField(std::string name, MRMesh<dim_>& mesh)
             : name_(std::move(name))
             , mesh_(&mesh)
+            , ctx_(mesh.shared_context())
+            , mesh_h_(mesh.shared_handle())
         {
             smr_field* f = nullptr;
             check(smr_field_create(mesh.handle(), name_.c_str(), &f));
@@ -366,6 +412,10 @@ namespace samurai_amd
         {
             flush();
             return h_.get();
+        }
+        const std::shared_ptr<smr_field>& shared_handle() const
+        {
+            return h_;
         }
         MRMesh<dim_>& mesh()
         {
@@ -443,6 +493,9 @@ namespace samurai_amd
         }
         std::string name_;
         MRMesh<dim_>* mesh_;
+        // the context and the mesh outlive the field
+        std::shared_ptr<smr_ctx> ctx_;
+        std::shared_ptr<smr_mesh> mesh_h_;
         std::shared_ptr<smr_field> h_;
         std::vector<double> host_;
         bool dirty_ = false;
@@ -611,6 +664,7 @@ namespace samurai_amd
       public:
         explicit MRAdapt(ScalarField<dim>& u)
             : u_(&u)
+            , field_h_(u.shared_handle())
         {
             smr_adapt* a = nullptr;
             check(smr_adapt_create(u.handle(), &a));
@@ -628,6 +682,8 @@ namespace samurai_amd
 
       private:
         ScalarField<dim>* u_;
+        // the field outlives the adaptation object
+        std::shared_ptr<smr_field> field_h_;
         std::shared_ptr<smr_adapt> h_;
     };
 

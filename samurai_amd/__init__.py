@@ -77,6 +77,8 @@ def lib():
         _sig(L, "smr_ctx_stream", "argtypes", [vp])
         _sig(L, "smr_mesh_upwind_bytes", "restype", i64)
         _sig(L, "smr_mesh_upwind_bytes", "argtypes", [vp])
+        _sig(L, "smr_mesh_ghost_update_bytes", "restype", i64)
+        _sig(L, "smr_mesh_ghost_update_bytes", "argtypes", [vp])
         _sig(L, "smr_timers_enable", "argtypes", [i32])
         _sig(L, "smr_timers_report", "restype", i64)
         _sig(L, "smr_timers_report", "argtypes", [ctypes.c_char_p, i64])
@@ -255,6 +257,10 @@ class MRMesh(_Handle):
     def upwind_bytes(self):
         """Algorithmic HBM bytes of one upwind launch on this mesh."""
         return _check(lib().smr_mesh_upwind_bytes(self.h))
+
+    def ghost_update_bytes(self):
+        """Algorithmic HBM bytes of one update_ghost_mr on this mesh."""
+        return _check(lib().smr_mesh_ghost_update_bytes(self.h))
 
     def validate(self):
         """Host check that every kernel access of this mesh's programs is in bounds."""

@@ -72,6 +72,17 @@ def build(verbose=False):
             sys.stderr.write(r.stdout.decode(errors="replace"))
             raise RuntimeError(f"samurai_amd: demo {name} build failed")
         os.replace(demo + ".tmp", demo)
+    # the README example with its loop as a device lambda
+    # (include/samurai_amd/device.hpp), compiled by hipcc
+    demo = os.path.join(HERE, "demos", "readme_loop_device")
+    cmd = [os.path.join(ROCM, "bin", "hipcc"), "-x", "hip", *[f for f in HIPFLAGS if f != "-fPIC"], f"-I{os.path.join(ROOT, 'include')}",
+           os.path.join(HERE, "demos", "readme_loop_device.cpp"), "-o", demo + ".tmp", f"-L{HERE}", "-lsamurai_amd",
+           "-Wl,-rpath,$ORIGIN/.."]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout.decode(errors="replace"))
+        raise RuntimeError("samurai_amd: demo readme_loop_device build failed")
+    os.replace(demo + ".tmp", demo)
     return LIB
 
 

@@ -459,6 +459,84 @@ extern "C"
         return ctx ? static_cast<void*>(ctx->c.stream) : nullptr;
     }
 
+    int64_t smr_mesh_ghost_update_bytes(const smr_mesh* mesh)
+    {
+        if (!mesh)
+        {
+            g_error = "null mesh";
+            return SMR_ERR_ARGUMENT;
+        }
+        int64_t bytes = 0;
+        const int rc  = guarded(
+            [&]
+            {
+                const smr::MRMeshData& m = mesh->m.host;
+                const int dim            = m.cfg.dim;
+                const smr::GhostProgram g = smr::build_ghost_program(m);
+                // projection: 2^dim fine values read, one written per ghost
+                bytes += 8 * ((int64_t(1) << dim) + 1) * g.projection.ncells();
+                // prediction: one write per ghost, and the distinct coarse
+                // values of the (2s+1)^dim stencils
+                for (int l = 1; l < smr::MAXL; ++l)
+                {
+                    int64_t r0       = 0;
+                    const int64_t nr = g.prediction.runs_of_levels(l, l + 1, r0);
+                    if (nr == 0)
+                    {
+                        continue;
+                    }
+                    smr::LevelSet fine(dim, l);
+                    for (int64_t r = r0; r < r0 + nr; ++r)
+                    {
+                        const int32_t* v = &g.prediction.iv[size_t(5 * r)];
+                        fine.push(v[2], v[3], v[0], v[1]);
+                    }
+                    bytes += 8 * fine.count();
+                    bytes += 8 * smr::expand(smr::on(fine, l - 1), m.cfg.pred_s).count();
+                }
+                // outer ghosts (boundary condition, corners, BC projection and
+                // prediction): each recipe reads its sources and writes one value
+                for (const smr::Recipes& r : g.outer)
+                {
+                    bytes += 8 * (int64_t(r.size()) + int64_t(r.src.size()));
+                }
+                return 0;
+            });
+        return rc < 0 ? rc : bytes;
+    }
+
+    int smr_mesh_device_view(smr_mesh* mesh, smr_mesh_view* out)
+    {
+        return guarded(
+            [&]
+            {
+                require(mesh && out, "null argument");
+                smr::Mesh& mm = mesh->m;
+                mm.ctx->require_device();
+                smr::DevMesh& dm     = mm.device();
+                const smr::dev::List& c = dm.cells.d;
+                smr_mesh_view v{};
+                v.dim              = mm.host.cfg.dim;
+                v.min_level        = mm.host.cfg.min_level;
+                v.max_level        = mm.host.cfg.max_level;
+                v.row_key          = dm.table.row_key;
+                v.row_off          = dm.table.row_off;
+                v.row_run          = dm.table.rp;
+                v.run_start        = dm.table.run_s;
+                v.run_end          = dm.table.run_e;
+                v.run_index        = dm.table.run_idx;
+                v.cell_runs        = c.iv;
+                v.cell_prefix      = c.pref;
+                v.cell_hint        = c.hint;
+                v.cell_base        = c.base;
+                v.cell_base_stride = c.ntaps;
+                v.nb_cell_runs     = c.nruns;
+                v.nb_cells         = c.ncells;
+                *out               = v;
+                return 0;
+            });
+    }
+
     int64_t smr_mesh_upwind_bytes(const smr_mesh* mesh)
     {
         if (!mesh)

@@ -115,6 +115,33 @@ namespace smr::dev
         SEG_WALK_ROWS = 22
     };
 
+    // One stage of a chain of small dependent stages run in order by a single
+    // block (k_sequence): a range of outer-ghost recipes, of projection or
+    // prediction ghosts, or of tag-finalization cells.  Launching each such
+    // stage on its own costs more than its few hundred cells.
+    enum SeqOp : int32_t
+    {
+        SEQ_RECIPES = 0,
+        SEQ_PROJECT = 1,
+        SEQ_PREDICT = 2,
+        SEQ_MAXIMUM = 3
+    };
+    struct SeqItem
+    {
+        int32_t op;
+        int32_t pad;
+        int64_t c0, c1;
+        const int64_t* dst; // recipes of the stage (SEQ_RECIPES)
+        const int32_t* rop;
+        const int64_t* sp;
+        const int64_t* src;
+    };
+    // items [i0, i1) of a device array, in order, by one block of 1024
+    // threads with a block barrier between items; l_proj, l_pred and l_max are
+    // the lists the project, predict and maximum items index
+    void sequence(int dim, int ps, const SeqItem* items, int i0, int i1, const List& l_proj, const List& l_pred, const List& l_max, double* u,
+                  uint8_t* tag, bool has_bc, double bc_value, hipStream_t s);
+
     struct Segment
     {
         int32_t op, dim, pred_s, has_bc;
@@ -211,10 +238,9 @@ namespace smr::dev
                         hipStream_t s);
     // out = (old | add) & ~rem
     void bits_combine(const Bits& b, const uint64_t* old, const uint64_t* add, const uint64_t* rem, uint64_t* out, hipStream_t s);
-    // one pass of make_graduation's fixed point from ca into out (q, o:
-    // scratch bitmaps); *flag = 1 if a cell was refined (flag not cleared)
-    void bits_graduation_pass(int dim, const Bits& b, const uint64_t* ca, uint64_t* q, uint64_t* o, uint64_t* out, int w2, int* flag,
-                              hipStream_t s);
+    // one pass of make_graduation's fixed point from ca into out (p: scratch
+    // bitmap); *flag = 1 if a cell was refined (flag not cleared)
+    void bits_graduation_pass(int dim, const Bits& b, const uint64_t* ca, uint64_t* p, uint64_t* out, int w2, int* flag, hipStream_t s);
     // *flag = 1 if a and c differ (flag not cleared)
     void bits_differ(const Bits& b, const uint64_t* a, const uint64_t* c, int* flag, hipStream_t s);
     // run-length encoding: counts and exclusive offsets (one per word of
@@ -223,7 +249,8 @@ namespace smr::dev
     size_t bits_scan_temp_bytes(const Bits& b);
     void bits_run_offsets(int dim, const Bits& b, const uint64_t* bits, int32_t* counts, int32_t* offsets, void* temp, size_t temp_bytes,
                           hipStream_t s);
-    void bits_run_emit(int dim, const Bits& b, const uint64_t* bits, const int32_t* offsets, int32_t* runs, hipStream_t s);
+    // (runs of rank >= cap are not written)
+    void bits_run_emit(int dim, const Bits& b, const uint64_t* bits, const int32_t* offsets, int32_t* runs, int64_t cap, hipStream_t s);
 
     void recipes(const int64_t* dst, const int32_t* op, const int64_t* src_ptr, const int64_t* src, int64_t r0, int64_t r1, double* u,
                  bool has_bc, double bc_value, hipStream_t s);

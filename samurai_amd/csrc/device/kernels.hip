@@ -583,6 +583,33 @@ namespace smr::dev
             return unsigned((n + BLOCK - 1) / BLOCK);
         }
 
+        constexpr int SEQ_BLOCK = 1024;
+
+        // the items of a chain of small stages in order; the block barrier
+        // orders every write of an item before the reads of the next (global
+        // memory written by the block is visible to the block after it)
+        template <int DIM, int S>
+        __global__ __launch_bounds__(SEQ_BLOCK) void k_sequence(const SeqItem* items, int i0, int i1, List l_proj, List l_pred, List l_max,
+                                                                double* u, uint8_t* tag, int has_bc, double bc_value)
+        {
+            for (int k = i0; k < i1; ++k)
+            {
+                const SeqItem it = items[k];
+                for (int64_t c = it.c0 + threadIdx.x; c < it.c1; c += SEQ_BLOCK)
+                {
+                    switch (it.op)
+                    {
+                        case SEQ_RECIPES: op_recipe(it.dst, it.rop, it.sp, it.src, c, u, has_bc, bc_value); break;
+                        case SEQ_PROJECT: op_project<DIM>(l_proj, c, u, u); break;
+                        case SEQ_PREDICT: op_predict_fine<DIM, S>(l_pred, c, u); break;
+                        case SEQ_MAXIMUM: op_maximum<DIM>(l_max, c, tag); break;
+                        default: break;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+
         // p[0, n) = 0 (a kernel, not hipMemsetAsync, so that a caller may
         // capture the launch in a HIP graph)
         template <class T>
@@ -967,6 +994,32 @@ namespace smr::dev
     }
 #undef SMR_DISPATCH_DIM
 #undef SMR_DISPATCH_DIM_S
+
+    void sequence(int dim, int ps, const SeqItem* items, int i0, int i1, const List& l_proj, const List& l_pred, const List& l_max, double* u,
+                  uint8_t* tag, bool has_bc, double bc_value, hipStream_t s)
+    {
+        if (i1 <= i0)
+        {
+            return;
+        }
+        const dim3 g(1), b(SEQ_BLOCK);
+        if (dim == 2 && ps == 0)
+        {
+            hipLaunchKernelGGL((k_sequence<2, 0>), g, b, 0, s, items, i0, i1, l_proj, l_pred, l_max, u, tag, int(has_bc), bc_value);
+        }
+        else if (dim == 2)
+        {
+            hipLaunchKernelGGL((k_sequence<2, 1>), g, b, 0, s, items, i0, i1, l_proj, l_pred, l_max, u, tag, int(has_bc), bc_value);
+        }
+        else if (ps == 0)
+        {
+            hipLaunchKernelGGL((k_sequence<3, 0>), g, b, 0, s, items, i0, i1, l_proj, l_pred, l_max, u, tag, int(has_bc), bc_value);
+        }
+        else
+        {
+            hipLaunchKernelGGL((k_sequence<3, 1>), g, b, 0, s, items, i0, i1, l_proj, l_pred, l_max, u, tag, int(has_bc), bc_value);
+        }
+    }
 
     void recipes(const int64_t* dst, const int32_t* op, const int64_t* src_ptr, const int64_t* src, int64_t r0, int64_t r1, double* u,
                  bool has_bc, double bc_value, hipStream_t s)

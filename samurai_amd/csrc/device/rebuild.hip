@@ -111,10 +111,10 @@ namespace smr::dev
         template <int DIM>
         __device__ __forceinline__ uint64_t on_parent_word(const Bits& b, const uint64_t* src, int l, int32_t y, int32_t z, int64_t wx)
         {
-            const int lc      = l + 1;
-            const int64_t cw  = b.wpr[lc];
-            uint64_t acc      = 0;
-            const int nz      = DIM == 3 ? 2 : 1;
+            const int lc     = l + 1;
+            const int64_t cw = b.wpr[lc];
+            uint64_t acc     = 0;
+            const int nz     = DIM == 3 ? 2 : 1;
             for (int dz = 0; dz < nz; ++dz)
             {
                 for (int dy = 0; dy < 2; ++dy)
@@ -244,11 +244,13 @@ namespace smr::dev
             }
         }
 
-        // q = nestedExpand(ca, w2) at every level in [b.lo + 2, b.hi]: a bit
-        // is set when a cell of ca lies within w2 cells in every direction
-        // (box dilation, clipped to the domain); other levels get 0
+        // list_interval_to_refine_for_graduation (graduation.hpp:245-330),
+        // first half: every word of nestedExpand(ca[f], w2) (box dilation
+        // clipped to the domain) at a level f in [lo + 2, hi] is projected two
+        // levels down (.on(f - 2): OR over 4x4 blocks) and ORed into p[f - 2]
+        // (p zeroed beforehand; at most 16 fine words share a target word)
         template <int DIM>
-        __global__ __launch_bounds__(RB) void k_dilate(Bits b, const uint64_t* ca, uint64_t* q, int w2)
+        __global__ __launch_bounds__(RB) void k_dilate_project(Bits b, const uint64_t* ca, uint64_t* p, int w2)
         {
             const int64_t w = int64_t(blockIdx.x) * RB + threadIdx.x;
             if (w >= b.off[b.hi + 1] - b.off[b.lo])
@@ -256,25 +258,24 @@ namespace smr::dev
                 return;
             }
             const int64_t gw = b.off[b.lo] + w;
-            const WordPos p  = word_pos(b, DIM, gw);
-            if (p.l < b.lo + 2)
+            const WordPos q  = word_pos(b, DIM, gw);
+            if (q.l < b.lo + 2)
             {
-                q[gw] = 0;
                 return;
             }
-            const int32_t n   = int32_t(1) << p.l;
-            const int64_t W   = b.wpr[p.l];
-            uint64_t acc      = 0;
-            const int zlo = DIM == 3 ? max(0, p.z - w2) : 0, zhi = DIM == 3 ? min(n - 1, p.z + w2) : 0;
-            const int ylo = max(0, p.y - w2), yhi = min(n - 1, p.y + w2);
+            const int32_t n = int32_t(1) << q.l;
+            const int64_t W = b.wpr[q.l];
+            uint64_t acc    = 0;
+            const int zlo = DIM == 3 ? max(0, q.z - w2) : 0, zhi = DIM == 3 ? min(n - 1, q.z + w2) : 0;
+            const int ylo = max(0, q.y - w2), yhi = min(n - 1, q.y + w2);
             for (int zz = zlo; zz <= zhi; ++zz)
             {
                 for (int yy = ylo; yy <= yhi; ++yy)
                 {
-                    const int64_t base = word_of(b, p.l, 0, yy, zz);
-                    const uint64_t cur = ca[base + p.wx];
-                    const uint64_t prv = p.wx > 0 ? ca[base + p.wx - 1] : 0ull;
-                    const uint64_t nxt = p.wx + 1 < W ? ca[base + p.wx + 1] : 0ull;
+                    const int64_t base = word_of(b, q.l, 0, yy, zz);
+                    const uint64_t cur = ca[base + q.wx];
+                    const uint64_t prv = q.wx > 0 ? ca[base + q.wx - 1] : 0ull;
+                    const uint64_t nxt = q.wx + 1 < W ? ca[base + q.wx + 1] : 0ull;
                     uint64_t r         = cur;
                     for (int s = 1; s <= w2; ++s)
                     {
@@ -284,57 +285,68 @@ namespace smr::dev
                     acc |= r;
                 }
             }
-            q[gw] = acc & row_mask(p.l);
-        }
-
-        // one level of the projection chain: o[l] = q[l+1].on(l), q[l] |= o[l]
-        template <int DIM>
-        __global__ __launch_bounds__(RB) void k_chain(Bits b, uint64_t* q, uint64_t* o, int l)
-        {
-            const int64_t n = b.off[l + 1] - b.off[l];
-            const int64_t w = int64_t(blockIdx.x) * RB + threadIdx.x;
-            if (w >= n)
+            acc &= row_mask(q.l);
+            if (acc == 0)
             {
                 return;
             }
-            const int64_t gw = b.off[l] + w;
-            const WordPos p  = word_pos(b, DIM, gw);
-            const uint64_t v = on_parent_word<DIM>(b, q, l, p.y, p.z, p.wx);
-            o[gw]            = v;
-            q[gw] |= v;
+            const uint64_t m     = pair_or_compress(pair_or_compress(acc));
+            const int32_t cx     = int32_t(q.wx * 64) >> 2;
+            uint64_t* target     = p + word_of(b, q.l - 2, cx, q.y >> 2, q.z >> 2);
+            const uint64_t bits  = m << (cx & 63);
+            if ((__atomic_load_n(target, __ATOMIC_RELAXED) & bits) != bits)
+            {
+                atomicOr(reinterpret_cast<unsigned long long*>(target), bits);
+            }
+        }
+
+        // second half, by one block: p[c] |= p[c + 1].on(c) from c = hi - 3
+        // down to lo, a level at a time (these levels hold 1/64 of the words
+        // of the finest one and less), so that p[c] = union over f >= c + 2 of
+        // the dilated fine cells of level f projected on c
+        template <int DIM>
+        __global__ __launch_bounds__(1024) void k_project_cascade(Bits b, uint64_t* p)
+        {
+            for (int c = b.hi - 3; c >= b.lo; --c)
+            {
+                const int64_t n0 = b.off[c], n1 = b.off[c + 1];
+                for (int64_t gw = n0 + threadIdx.x; gw < n1; gw += 1024)
+                {
+                    const WordPos q = word_pos(b, DIM, gw);
+                    const uint64_t v = on_parent_word<DIM>(b, p, c, q.y, q.z, q.wx);
+                    if (v)
+                    {
+                        p[gw] |= v;
+                    }
+                }
+                __syncthreads();
+            }
         }
 
         // one graduation pass (graduation.hpp:629-718): the cells of level c
-        // under P_c = o[c+1].on(c) (the fine cells of every level >= c + 2,
-        // dilated and projected) are replaced by their children:
-        // out[l] = (ca[l] | children(rem[l-1])) & ~rem[l], rem[c] = P_c & ca[c];
+        // under p[c] are replaced by their children:
+        // out[l] = (ca[l] | children(rem[l-1])) & ~rem[l], rem[c] = p[c] & ca[c];
         // *flag = 1 if any cell is refined
         template <int DIM>
-        __global__ __launch_bounds__(RB) void k_grad_apply(Bits b, const uint64_t* ca, const uint64_t* o, uint64_t* out, int* flag)
+        __global__ __launch_bounds__(RB) void k_grad_apply(Bits b, const uint64_t* ca, const uint64_t* p, uint64_t* out, int* flag)
         {
             const int64_t w = int64_t(blockIdx.x) * RB + threadIdx.x;
             if (w >= b.off[b.hi + 1] - b.off[b.lo])
             {
                 return;
             }
-            const int64_t gw = b.off[b.lo] + w;
-            const WordPos p  = word_pos(b, DIM, gw);
-            const uint64_t c = ca[gw];
-            uint64_t rem     = 0;
-            if (p.l + 2 <= b.hi)
+            const int64_t gw   = b.off[b.lo] + w;
+            const WordPos q    = word_pos(b, DIM, gw);
+            const uint64_t c   = ca[gw];
+            const uint64_t rem = p[gw] & c;
+            uint64_t add       = 0;
+            if (q.l - 1 >= b.lo)
             {
-                rem = on_parent_word<DIM>(b, o, p.l, p.y, p.z, p.wx) & c;
-            }
-            uint64_t add = 0;
-            if (p.l - 1 >= b.lo && p.l + 1 <= b.hi)
-            {
-                // the parents of this word at l - 1 that are refined
-                const int pl      = p.l - 1;
-                const int64_t pwx = p.wx >> 1;
-                const int32_t py = p.y >> 1, pz = p.z >> 1;
-                const int64_t pw  = word_of(b, pl, 0, py, pz) + pwx;
-                const uint64_t pr = on_parent_word<DIM>(b, o, pl, py, pz, pwx) & ca[pw];
-                add               = spread(pr >> (32 * (p.wx & 1)));
+                // the refined parents of this word at l - 1
+                const int pl      = q.l - 1;
+                const int64_t pw  = word_of(b, pl, 0, q.y >> 1, q.z >> 1) + (q.wx >> 1);
+                const uint64_t pr = p[pw] & ca[pw];
+                add               = spread(pr >> (32 * (q.wx & 1)));
             }
             out[gw] = (c | add) & ~rem;
             if (rem != 0)
@@ -370,7 +382,7 @@ namespace smr::dev
         // start found in word w is written at offsets[w] + its rank in w; its
         // end is the first clear bit after it in the row
         template <int DIM>
-        __global__ __launch_bounds__(RB) void k_run_emit(Bits b, const uint64_t* bits, const int32_t* offsets, int32_t* runs)
+        __global__ __launch_bounds__(RB) void k_run_emit(Bits b, const uint64_t* bits, const int32_t* offsets, int32_t* runs, int64_t cap)
         {
             const int64_t w = int64_t(blockIdx.x) * RB + threadIdx.x;
             if (w >= b.off[b.hi + 1] - b.off[b.lo])
@@ -409,6 +421,10 @@ namespace smr::dev
                 if (x1 < 0 || x1 > n)
                 {
                     x1 = n;
+                }
+                if (k >= cap)
+                {
+                    break;
                 }
                 int32_t* r = runs + 5 * int64_t(k);
                 r[0]       = x0;
@@ -469,33 +485,31 @@ namespace smr::dev
         }
     }
 
-    void bits_graduation_pass(int dim, const Bits& b, const uint64_t* ca, uint64_t* q, uint64_t* o, uint64_t* out, int w2, int* flag,
-                              hipStream_t s)
+    void bits_graduation_pass(int dim, const Bits& b, const uint64_t* ca, uint64_t* p, uint64_t* out, int w2, int* flag, hipStream_t s)
     {
         const int64_t n = b.off[b.hi + 1] - b.off[b.lo];
         if (n <= 0)
         {
             return;
         }
+        bits_zero(b, p, s);
         if (dim == 2)
         {
-            hipLaunchKernelGGL(k_dilate<2>, dim3(blocks_for(n)), dim3(RB), 0, s, b, ca, q, w2);
-            for (int l = b.hi - 1; l >= b.lo + 1; --l)
+            hipLaunchKernelGGL(k_dilate_project<2>, dim3(blocks_for(n)), dim3(RB), 0, s, b, ca, p, w2);
+            if (b.hi - 3 >= b.lo)
             {
-                const int64_t nl = b.off[l + 1] - b.off[l];
-                hipLaunchKernelGGL(k_chain<2>, dim3(blocks_for(nl)), dim3(RB), 0, s, b, q, o, l);
+                hipLaunchKernelGGL(k_project_cascade<2>, dim3(1), dim3(1024), 0, s, b, p);
             }
-            hipLaunchKernelGGL(k_grad_apply<2>, dim3(blocks_for(n)), dim3(RB), 0, s, b, ca, o, out, flag);
+            hipLaunchKernelGGL(k_grad_apply<2>, dim3(blocks_for(n)), dim3(RB), 0, s, b, ca, p, out, flag);
         }
         else
         {
-            hipLaunchKernelGGL(k_dilate<3>, dim3(blocks_for(n)), dim3(RB), 0, s, b, ca, q, w2);
-            for (int l = b.hi - 1; l >= b.lo + 1; --l)
+            hipLaunchKernelGGL(k_dilate_project<3>, dim3(blocks_for(n)), dim3(RB), 0, s, b, ca, p, w2);
+            if (b.hi - 3 >= b.lo)
             {
-                const int64_t nl = b.off[l + 1] - b.off[l];
-                hipLaunchKernelGGL(k_chain<3>, dim3(blocks_for(nl)), dim3(RB), 0, s, b, q, o, l);
+                hipLaunchKernelGGL(k_project_cascade<3>, dim3(1), dim3(1024), 0, s, b, p);
             }
-            hipLaunchKernelGGL(k_grad_apply<3>, dim3(blocks_for(n)), dim3(RB), 0, s, b, ca, o, out, flag);
+            hipLaunchKernelGGL(k_grad_apply<3>, dim3(blocks_for(n)), dim3(RB), 0, s, b, ca, p, out, flag);
         }
     }
 
@@ -536,7 +550,7 @@ namespace smr::dev
         (void)hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, counts, offsets, int(n), s);
     }
 
-    void bits_run_emit(int dim, const Bits& b, const uint64_t* bits, const int32_t* offsets, int32_t* runs, hipStream_t s)
+    void bits_run_emit(int dim, const Bits& b, const uint64_t* bits, const int32_t* offsets, int32_t* runs, int64_t cap, hipStream_t s)
     {
         const int64_t n = b.off[b.hi + 1] - b.off[b.lo];
         if (n <= 0)
@@ -545,11 +559,11 @@ namespace smr::dev
         }
         if (dim == 2)
         {
-            hipLaunchKernelGGL(k_run_emit<2>, dim3(blocks_for(n)), dim3(RB), 0, s, b, bits, offsets, runs);
+            hipLaunchKernelGGL(k_run_emit<2>, dim3(blocks_for(n)), dim3(RB), 0, s, b, bits, offsets, runs, cap);
         }
         else
         {
-            hipLaunchKernelGGL(k_run_emit<3>, dim3(blocks_for(n)), dim3(RB), 0, s, b, bits, offsets, runs);
+            hipLaunchKernelGGL(k_run_emit<3>, dim3(blocks_for(n)), dim3(RB), 0, s, b, bits, offsets, runs, cap);
         }
     }
 } // namespace smr::dev

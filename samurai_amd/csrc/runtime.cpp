@@ -822,6 +822,115 @@ namespace smr
     // ------------------------------------------------------------------
     // update_ghost_mr (algorithm/update_ghost_mr.hpp:194-237)
     // ------------------------------------------------------------------
+    void SeqPlan::build(std::vector<dev::SeqItem>&& all, hipStream_t s)
+    {
+        items = std::move(all);
+        steps.clear();
+        for (int k = 0; k < int(items.size());)
+        {
+            if (items[size_t(k)].c1 - items[size_t(k)].c0 > kSeqSmall)
+            {
+                steps.push_back({false, k, k + 1});
+                ++k;
+                continue;
+            }
+            int e = k + 1;
+            while (e < int(items.size()) && items[size_t(e)].c1 - items[size_t(e)].c0 <= kSeqSmall)
+            {
+                ++e;
+            }
+            steps.push_back({true, k, e});
+            k = e;
+        }
+        if (!items.empty())
+        {
+            void* d = d_items.get(items.size() * sizeof(dev::SeqItem), s);
+            hip_check(hipMemcpyAsync(d, items.data(), items.size() * sizeof(dev::SeqItem), hipMemcpyHostToDevice, s), "hipMemcpyAsync(plan)");
+        }
+        built = true;
+    }
+
+    namespace
+    {
+        dev::SeqItem seq_item(int32_t op, int64_t c0, int64_t c1)
+        {
+            dev::SeqItem it{};
+            it.op = op;
+            it.c0 = c0;
+            it.c1 = c1;
+            return it;
+        }
+
+        // update_ghost_mr's stages in order (algorithm/update_ghost_mr.hpp:
+        // 194-237): per level from the finest, the outer-ghost recipe stages
+        // then the projection into the level below; then the predictions from
+        // the coarsest level up
+        SeqPlan& ghost_plan(Ctx& ctx, DevMesh& dm, const MeshConfig& cfg)
+        {
+            if (!dm.ghost_plan.built)
+            {
+                std::vector<dev::SeqItem> items;
+                for (int level = cfg.max_level; level >= 0; --level)
+                {
+                    const DevRecipes& r = dm.outer[size_t(level)];
+                    const auto& sp      = r.host.stage_ptr;
+                    for (size_t k = 0; k + 1 < sp.size(); ++k)
+                    {
+                        if (sp[k + 1] > sp[k])
+                        {
+                            dev::SeqItem it = seq_item(dev::SEQ_RECIPES, sp[k], sp[k + 1]);
+                            it.dst          = r.dst;
+                            it.rop          = r.op;
+                            it.sp           = r.sp;
+                            it.src          = r.src;
+                            items.push_back(it);
+                        }
+                    }
+                    if (level > 0)
+                    {
+                        int64_t c0, c1;
+                        dm.projection.cells_of_levels(level - 1, level, c0, c1);
+                        if (c1 > c0)
+                        {
+                            items.push_back(seq_item(dev::SEQ_PROJECT, c0, c1));
+                        }
+                    }
+                }
+                for (int level = 1; level <= cfg.max_level; ++level)
+                {
+                    int64_t c0, c1;
+                    dm.prediction.cells_of_levels(level, level + 1, c0, c1);
+                    if (c1 > c0)
+                    {
+                        items.push_back(seq_item(dev::SEQ_PREDICT, c0, c1));
+                    }
+                }
+                dm.ghost_plan.build(std::move(items), ctx.stream);
+            }
+            return dm.ghost_plan;
+        }
+
+        // maximum_op from the finest level down (mr/adapt.hpp:348-358)
+        SeqPlan& max_plan(Ctx& ctx, DevMesh& dm, const MeshConfig& cfg)
+        {
+            if (!dm.max_plan.built)
+            {
+                std::vector<dev::SeqItem> items;
+                for (int level = cfg.max_level; level > 0; --level)
+                {
+                    int64_t c0, c1;
+                    dm.maximum.cells_of_levels(level - 1, level, c0, c1);
+                    if (c1 > c0)
+                    {
+                        items.push_back(seq_item(dev::SEQ_MAXIMUM, c0, c1));
+                    }
+                }
+                dm.max_plan.build(std::move(items), ctx.stream);
+            }
+            return dm.max_plan;
+        }
+    } // namespace
+
     void update_ghost_mr(Field& f)
     {
         f.check_current();
@@ -830,28 +939,25 @@ namespace smr
         ScopedTimer timer(m.ctx, "ghost update");
         hipStream_t s   = m.ctx->stream;
         const int dim   = m.host.cfg.dim;
-        const int maxl  = m.host.cfg.max_level;
+        const int ps    = m.host.cfg.pred_s;
         const double bc = f.bc_value;
-        for (int level = maxl; level >= 0; --level)
+        const SeqPlan& plan = ghost_plan(*m.ctx, dm, m.host.cfg);
+        const auto* d_items = static_cast<const dev::SeqItem*>(plan.d_items.p);
+        for (const SeqPlan::Step& st : plan.steps)
         {
-            DevRecipes& r  = dm.outer[size_t(level)];
-            const auto& sp = r.host.stage_ptr;
-            for (size_t k = 0; k + 1 < sp.size(); ++k)
+            if (st.seq)
             {
-                dev::recipes(r.dst, r.op, r.sp, r.src, sp[k], sp[k + 1], f.d, f.has_bc, bc, s);
+                dev::sequence(dim, ps, d_items, st.i0, st.i1, dm.projection.d, dm.prediction.d, dev::List{}, f.d, nullptr, f.has_bc, bc, s);
+                continue;
             }
-            if (level > 0)
+            const dev::SeqItem& it = plan.items[size_t(st.i0)];
+            switch (it.op)
             {
-                int64_t c0, c1;
-                dm.projection.cells_of_levels(level - 1, level, c0, c1);
-                dev::project(dim, dm.projection.d, c0, c1, f.d, f.d, s);
+                case dev::SEQ_RECIPES: dev::recipes(it.dst, it.rop, it.sp, it.src, it.c0, it.c1, f.d, f.has_bc, bc, s); break;
+                case dev::SEQ_PROJECT: dev::project(dim, dm.projection.d, it.c0, it.c1, f.d, f.d, s); break;
+                case dev::SEQ_PREDICT: dev::predict_fine(dim, ps, dm.prediction.d, it.c0, it.c1, f.d, s); break;
+                default: break;
             }
-        }
-        for (int level = 1; level <= maxl; ++level)
-        {
-            int64_t c0, c1;
-            dm.prediction.cells_of_levels(level, level + 1, c0, c1);
-            dev::predict_fine(dim, m.host.cfg.pred_s, dm.prediction.d, c0, c1, f.d, s);
         }
     }
 
@@ -1128,72 +1234,91 @@ namespace smr
             dm.has_leaf_bits = true;
         }
         const uint64_t* old = static_cast<const uint64_t*>(dm.leaf_bits.p);
+        // graduation passes launched per readback of their flags: most
+        // updates reach the fixed point within two, and the third confirms it
+        constexpr int kPasses = 3;
         DevBuffer work;
-        uint64_t* w   = static_cast<uint64_t*>(work.get(5 * wbytes, s));
+        uint64_t* w   = static_cast<uint64_t*>(work.get(5 * wbytes + 256, s));
         uint64_t* add = w;
         uint64_t* rem = w + nw;
-        uint64_t* cur = w + 2 * nw;
-        uint64_t* nxt = w + 3 * nw;
+        uint64_t* buf[2] = {w + 2 * nw, w + 3 * nw};
         uint64_t* q   = w + 4 * nw;
-        uint64_t* o   = add; // free once the marks are combined
-        int* flag     = ctx.d_flag();
-        int h_flag    = 0;
+        int* flags    = reinterpret_cast<int*>(w + 5 * nw); // kPasses pass flags, then the differ flag
+        int h_flags[kPasses + 1];
         {
             ScopedTimer t(&ctx, "update_cell_array_from_tag");
             dev::bits_tag_marks(cfg.dim, dm.cells.d, b, tag, cfg.min_level, cfg.max_level, add, rem, s);
-            dev::bits_combine(b, old, add, rem, cur, s);
+            dev::bits_combine(b, old, add, rem, buf[0], s);
         }
+        int cur = 0;
         {
             ScopedTimer t(&ctx, "make_graduation");
             // make_graduation's fixed point (graduation.hpp:623-723): passes
-            // until one refines nothing
-            for (int pass = 0;; ++pass)
+            // until one refines nothing.  A pass that refines nothing copies
+            // its input, so the passes after it leave the leaves as they are
+            // and the last buffer holds the fixed point; the difference with
+            // the old leaves is taken in the same round trip.
+            for (int batch = 0;; ++batch)
             {
-                if (pass > 4 * 21)
+                if (batch > 21)
                 {
                     throw std::logic_error("make_graduation: no fixed point");
                 }
-                hip_check(hipMemsetAsync(flag, 0, sizeof(int), s), "hipMemsetAsync(flag)");
-                dev::bits_graduation_pass(cfg.dim, b, cur, q, o, nxt, 2 * cfg.grad_width, flag, s);
-                hip_check(hipMemcpyAsync(&h_flag, flag, sizeof(int), hipMemcpyDeviceToHost, s), "hipMemcpyAsync(flag)");
+                hip_check(hipMemsetAsync(flags, 0, sizeof(int) * (kPasses + 1), s), "hipMemsetAsync(flags)");
+                for (int p = 0; p < kPasses; ++p)
+                {
+                    dev::bits_graduation_pass(cfg.dim, b, buf[cur], q, buf[1 - cur], 2 * cfg.grad_width, flags + p, s);
+                    cur = 1 - cur;
+                }
+                dev::bits_differ(b, buf[cur], old, flags + kPasses, s);
+                hip_check(hipMemcpyAsync(h_flags, flags, sizeof(h_flags), hipMemcpyDeviceToHost, s), "hipMemcpyAsync(flags)");
                 ctx.sync();
-                if (!h_flag)
+                bool fixed = false;
+                for (int p = 0; p < kPasses; ++p)
+                {
+                    fixed = fixed || h_flags[p] == 0;
+                }
+                if (fixed)
                 {
                     break;
                 }
-                std::swap(cur, nxt);
             }
-            hip_check(hipMemsetAsync(flag, 0, sizeof(int), s), "hipMemsetAsync(flag)");
-            dev::bits_differ(b, cur, old, flag, s);
-            hip_check(hipMemcpyAsync(&h_flag, flag, sizeof(int), hipMemcpyDeviceToHost, s), "hipMemcpyAsync(flag)");
-            ctx.sync();
         }
-        if (!h_flag)
+        if (!h_flags[kPasses])
         {
             return false;
         }
         ScopedTimer t(&ctx, "leaves to host");
-        // run-length encoding of the new leaves, in traversal order
-        const int64_t nl      = nw - b.off[b.lo];
-        const size_t tbytes   = dev::bits_scan_temp_bytes(b);
+        // run-length encoding of the new leaves, in traversal order, into a
+        // buffer sized from the old leaves' runs; one round trip brings the run
+        // count and the runs (a second one only if the buffer was too small)
+        const uint64_t* leaves = buf[cur];
+        const int64_t nl       = nw - b.off[b.lo];
+        const size_t tbytes    = dev::bits_scan_temp_bytes(b);
+        const int64_t cap      = dm.cells.d.nruns + dm.cells.d.nruns / 4 + 1024;
+        const size_t cbytes    = (size_t(2 * std::max<int64_t>(nl, 1)) * 4 + 255) & ~size_t(255);
         DevBuffer scan;
-        char* sp              = static_cast<char*>(scan.get(size_t(2 * std::max<int64_t>(nl, 1)) * 4 + 256 + tbytes, s));
-        int32_t* counts       = reinterpret_cast<int32_t*>(sp);
-        int32_t* offsets      = counts + std::max<int64_t>(nl, 1);
-        void* temp            = sp + ((size_t(2 * std::max<int64_t>(nl, 1)) * 4 + 255) & ~size_t(255));
-        dev::bits_run_offsets(cfg.dim, b, cur, counts, offsets, temp, tbytes, s);
+        char* sp          = static_cast<char*>(scan.get(cbytes + tbytes + size_t(cap) * 20, s));
+        int32_t* counts   = reinterpret_cast<int32_t*>(sp);
+        int32_t* offsets  = counts + std::max<int64_t>(nl, 1);
+        void* temp        = sp + cbytes;
+        int32_t* dr       = reinterpret_cast<int32_t*>(sp + cbytes + tbytes);
+        dev::bits_run_offsets(cfg.dim, b, leaves, counts, offsets, temp, tbytes, s);
+        dev::bits_run_emit(cfg.dim, b, leaves, offsets, dr, cap, s);
         int32_t last[2] = {0, 0};
+        std::vector<int32_t> runs(size_t(cap) * 5);
         hip_check(hipMemcpyAsync(&last[0], offsets + nl - 1, 4, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(runs)");
         hip_check(hipMemcpyAsync(&last[1], counts + nl - 1, 4, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(runs)");
+        hip_check(hipMemcpyAsync(runs.data(), dr, size_t(cap) * 20, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(runs)");
         ctx.sync();
         const int64_t nruns = int64_t(last[0]) + last[1];
-        std::vector<int32_t> runs(size_t(nruns) * 5);
-        if (nruns > 0)
+        if (nruns > cap)
         {
             DevBuffer rb;
-            int32_t* dr = static_cast<int32_t*>(rb.get(size_t(nruns) * 20, s));
-            dev::bits_run_emit(cfg.dim, b, cur, offsets, dr, s);
-            hip_check(hipMemcpyAsync(runs.data(), dr, size_t(nruns) * 20, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(runs)");
+            int32_t* big = static_cast<int32_t*>(rb.get(size_t(nruns) * 20, s));
+            dev::bits_run_emit(cfg.dim, b, leaves, offsets, big, nruns, s);
+            runs.resize(size_t(nruns) * 5);
+            hip_check(hipMemcpyAsync(runs.data(), big, size_t(nruns) * 20, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(runs)");
             ctx.sync();
         }
         out = empty_cell_array(cfg.dim);
@@ -1256,11 +1381,19 @@ namespace smr
         // tag finalization: levels max..1, sequential
         {
             ScopedTimer timer(&ctx, "tag finalization");
-            for (int level = maxl; level > 0; --level)
+            const SeqPlan& plan = max_plan(ctx, dm, cfg);
+            const auto* d_items = static_cast<const dev::SeqItem*>(plan.d_items.p);
+            for (const SeqPlan::Step& st : plan.steps)
             {
-                int64_t c0, c1;
-                dm.maximum.cells_of_levels(level - 1, level, c0, c1);
-                dev::maximum(dim, dm.maximum.d, c0, c1, tag, s);
+                if (st.seq)
+                {
+                    dev::sequence(dim, cfg.pred_s, d_items, st.i0, st.i1, dev::List{}, dev::List{}, dm.maximum.d, nullptr, tag, false, 0.0, s);
+                }
+                else
+                {
+                    const dev::SeqItem& it = plan.items[size_t(st.i0)];
+                    dev::maximum(dim, dm.maximum.d, it.c0, it.c1, tag, s);
+                }
             }
         }
     }
@@ -1294,23 +1427,9 @@ namespace smr
             ScopedTimer timer(&ctx, "mesh update");
             st.last_size      = n;
             st.host_tag_valid = false;
-            if (m.host.graduated)
-            {
-                // fixed point without the host: no tag changes the cell array of
-                // a graduated mesh, so update_cell_array_from_tag and
-                // make_graduation would return the cells unchanged
-                hip_check(hipMemsetAsync(ctx.d_flag(), 0, sizeof(int), s), "hipMemsetAsync(flag)");
-                dev::tag_change(dm.cells.d, 0, dm.cells.d.ncells, st.tag, minl, maxl, ctx.d_flag(), s);
-                int changed = 1;
-                hip_check(hipMemcpyAsync(&changed, ctx.d_flag(), sizeof(int), hipMemcpyDeviceToHost, s), "hipMemcpyAsync(flag)");
-                ctx.sync();
-                if (!changed)
-                {
-                    return true;
-                }
-            }
             // mesh update (graduation.hpp:743-843, 573-726): on the device,
-            // the host receives the new leaves only when they change
+            // the host receives the new leaves only when they change (one
+            // round trip when they do not)
             CellArray new_ca;
             if (!device_mesh_update(ctx, dm, cfg, st.tag, new_ca))
             {

@@ -134,6 +134,24 @@ namespace smr
         const int64_t* src = nullptr;
     };
 
+    // A chain of dependent stages launched as: the large stages one launch
+    // each, every run of consecutive small stages (at most kSeqSmall cells
+    // each) one single-block launch of dev::sequence.
+    struct SeqPlan
+    {
+        struct Step
+        {
+            bool seq;
+            int i0, i1;
+        };
+        std::vector<dev::SeqItem> items; // kept for the asynchronous upload
+        std::vector<Step> steps;
+        DevBuffer d_items;
+        bool built = false;
+        void build(std::vector<dev::SeqItem>&& all, hipStream_t s);
+    };
+    constexpr int64_t kSeqSmall = 2048;
+
     // Device image of one mesh version.
     struct DevMesh
     {
@@ -161,6 +179,8 @@ namespace smr
         std::unique_ptr<Conv> burgers;
         // the heterogeneous (velocity field) convection terms of this image
         std::unique_ptr<Conv> heterogeneous;
+        // launch plans of update_ghost_mr and of the tag finalization
+        SeqPlan ghost_plan, max_plan;
         // the leaves as per-level bitmaps (device mesh update), built on first use
         DevBuffer leaf_bits;
         bool has_leaf_bits = false;

@@ -111,3 +111,47 @@ def test_cpp_scalar_burgers_demo_matches_oracle(tmp_path):
     assert np.array_equal(got[:, 1].astype(np.int64), cells[:, 1])
     assert np.array_equal(got[:, 2].astype(np.int64), cells[:, 2])
     assert np.array_equal(got[:, 3].view(np.uint64), o.values().view(np.uint64))
+
+
+README_DEMO = os.path.join(ROOT, "samurai_amd", "demos", "readme_loop_device")
+
+
+def test_readme_loop_as_device_lambda_matches_the_hand_loop(gpu_ctx, tmp_path):
+    """README.md:86-154 with its for_each_interval loop written as a device
+    lambda over include/samurai_amd/device.hpp field views: 20 steps of
+    MRadaptation(1e-4, 2), update_ghost_mr, the loop and swap, bit-identical
+    to the same steps through smr_upwind_hand_loop (the built-in kernel of
+    that loop, itself checked against the oracle)."""
+    from initial_conditions import box
+    assert os.path.exists(README_DEMO), "build the library first (python samurai_amd/build.py)"
+    out = str(tmp_path / "readme.txt")
+    r = subprocess.run([README_DEMO, "--steps", "20", "--out", out], capture_output=True, text=True, timeout=120)
+    sys.stdout.write(r.stdout)
+    sys.stderr.write(r.stderr)
+    assert r.returncode == 0
+    got = np.loadtxt(out)
+    mesh = sa.make_mesh(gpu_ctx, sa.mesh_config(2, 2, 8, 1))
+    u = sa.make_scalar_field("u", mesh)
+    u.set_cells(box(mesh.cells(), 2))
+    sa.make_bc_dirichlet(u, 0.0)
+    unp1 = sa.make_scalar_field("u", mesh)
+    adapt = sa.make_MRAdapt(u)
+    dt = 0.5 * 2.0 ** -8
+    for _ in range(20):
+        adapt(1e-4, 2.0)
+        sa.update_ghost_mr(u)
+        unp1.resize()
+        sa.upwind_hand_loop(dt, u, unp1)
+        sa.swap(u, unp1)
+    cells = mesh.cells()
+    assert np.array_equal(got[:, :3].astype(np.int64), cells[:, :3])
+    assert np.array_equal(got[:, 3].view(np.uint64), u.cells().view(np.uint64))
+
+
+def test_device_lookup_of_an_absent_cell_raises_out_of_range():
+    """u(level, i - 3, j) outside the reference mesh: for_each_interval raises
+    std::out_of_range (level_cell_array.hpp:674-680)."""
+    r = subprocess.run([README_DEMO, "--steps", "2", "--probe-missing"], capture_output=True, text=True, timeout=120)
+    sys.stdout.write(r.stdout)
+    assert r.returncode == 3
+    assert "out_of_range" in r.stdout
