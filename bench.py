@@ -87,39 +87,25 @@ def make_problem(sa, ctx, min_level, max_level, seed):
     return mesh, u, unp1
 
 
-def cpu_baseline(cells, values, a, dt, eps, reg, budget_s):
-    """The oracle (single-threaded C++ restatement) on the current adapted mesh
-    and field of the GPU run, for as many steps as fit in the budget."""
-    from oracle_lib import OracleSim
+def _cpu_worker(job):
+    """One instance on a CPU path for `budget` seconds, in a fresh process with
+    one host thread: "interval" is the single-core interval-algebra CPU
+    baseline (oracle/cpu_step.cpp: the reference's algorithm on interval
+    sets), "dense" the dense-bitmap oracle.  Imports nothing that touches the
+    GPU."""
+    os.environ["SMR_HOST_THREADS"] = "1"
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    kind, cells, values, min_level, max_level, a, dt, eps, reg, adapt, budget = job
     t0 = time.perf_counter()
-    o = OracleSim(2, 4, 12, pred_s=1, cells=cells)
+    if kind == "interval":
+        from cpu_step_lib import CpuSim
+        o = CpuSim(2, min_level, max_level, cells)
+    else:
+        from oracle_lib import OracleSim
+        o = OracleSim(2, min_level, max_level, pred_s=1, cells=cells)
     o.set_values(values)
     o.set_dirichlet(0.0)
     setup = time.perf_counter() - t0
-    steps, updates, spent = 0, 0, 0.0
-    while steps == 0 or spent < budget_s:
-        t = time.perf_counter()
-        o.adapt(eps, reg)
-        n = o.nb_cells()
-        o.upwind_step(a, dt)
-        spent += time.perf_counter() - t
-        steps += 1
-        updates += n
-    return {"value": updates / spent, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-            "sample": f"{steps} oracle step(s) (MRadaptation + update_ghost_mr + upwind) on the GPU run's final "
-                      f"adapted L4..12 mesh ({len(cells)} leaves), {spent:.1f} s of CPU time, mesh setup "
-                      f"{setup:.1f} s untimed; single thread, dense-set oracle"}
-
-
-def _oracle_worker(job):
-    """One instance on the oracle for `budget` seconds (a worker process of
-    cpu_baseline_parallel; imports nothing that touches the GPU)."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from oracle_lib import OracleSim
-    cells, values, min_level, max_level, a, dt, eps, reg, adapt, budget = job
-    o = OracleSim(2, min_level, max_level, pred_s=1, cells=cells)
-    o.set_values(values)
-    o.set_dirichlet(0.0)
     steps, updates, spent = 0, 0, 0.0
     while steps == 0 or spent < budget:
         t = time.perf_counter()
@@ -130,24 +116,47 @@ def _oracle_worker(job):
         spent += time.perf_counter() - t
         steps += 1
         updates += n
-    return updates, spent, steps
+    return updates, spent, steps, setup
+
+
+def _run_workers(jobs):
+    import multiprocessing as mp
+    t0 = time.perf_counter()
+    with mp.get_context("spawn").Pool(len(jobs)) as pool:
+        res = pool.map(_cpu_worker, jobs)
+    return res, time.perf_counter() - t0
+
+
+def cpu_baseline(cells, values, a, dt, eps, reg, budget_s):
+    """The single-core CPU baseline (interval algebra, the reference's
+    algorithm) on the GPU run's final adapted mesh and field for about
+    budget_s seconds, and one step of the dense-bitmap oracle beside it."""
+    job = ("interval", cells, values, 4, 12, a, dt, eps, reg, True, budget_s)
+    (res,), _ = _run_workers([job])
+    updates, spent, steps, setup = res
+    (dres,), _ = _run_workers([("dense",) + job[1:-1] + (0.0,)])
+    return {"value": updates / spent, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": f"{steps} steps (MRadaptation + update_ghost_mr + upwind) of the interval-algebra CPU path "
+                      f"(oracle/cpu_step.cpp: the product's host interval algebra and operator bodies on one core, "
+                      f"-O3 -ffp-contract=off, bit-identical to the oracle) on the GPU run's final adapted L4..12 mesh "
+                      f"({len(cells)} leaves), {spent:.1f} s of CPU time, setup {setup:.1f} s untimed",
+            "dense_oracle": {"value": dres[0] / dres[1], "unit": "cell-updates/s", "cores": 1,
+                             "sample": f"{dres[2]} step(s) of the dense-bitmap oracle on the same mesh, {dres[1]:.1f} s"}}
 
 
 def cpu_baseline_parallel(jobs, label):
-    """The oracle on several instances at once, one process per instance
-    (processes started fresh, not forked from this GPU process)."""
-    import multiprocessing as mp
+    """The interval-algebra CPU path on several instances at once, one
+    single-threaded process per instance (processes started fresh, not
+    forked from this GPU process)."""
     workers = max(1, min(len(jobs), 16, os.cpu_count() or 1))
-    jobs = jobs[:workers]
-    t0 = time.perf_counter()
-    with mp.get_context("spawn").Pool(workers) as pool:
-        res = pool.map(_oracle_worker, jobs)
-    wall = time.perf_counter() - t0
+    jobs = [("interval",) + tuple(j) for j in jobs[:workers]]
+    res, wall = _run_workers(jobs)
     updates = sum(r[0] for r in res)
     busy = max(r[1] for r in res)
     return {"value": updates / busy, "unit": "cell-updates/s", "cores": workers, "kind": "port",
-            "sample": f"{label}: {workers} instances, one oracle process each, {sum(r[2] for r in res)} steps in "
-                      f"{busy:.1f} s of concurrent CPU time ({wall:.1f} s wall with process start); dense-set oracle"}
+            "sample": f"{label}: {workers} instances, one single-threaded interval-algebra CPU process each "
+                      f"(oracle/cpu_step.cpp), {sum(r[2] for r in res)} steps in {busy:.1f} s of concurrent CPU time "
+                      f"({wall:.1f} s wall with process start)"}
 
 
 def make_batch_set(sa, ctx, n, levels, seed0):

@@ -1,6 +1,8 @@
 // Per-cell bodies of the MR-step operators, shared by the standalone kernels
 // (kernels.hip) and the fused stage-program kernel (program.hip), so both
-// paths execute the same arithmetic in the same order.
+// paths execute the same arithmetic in the same order.  They are host
+// callable too: the single-core CPU baseline (oracle/cpu_step.cpp) runs the
+// same bodies over host copies of the lists.
 //
 // A cell is addressed by its index c in a work list (List): its run is found
 // between two chunk hints, and every storage position it touches is
@@ -22,7 +24,7 @@ namespace smr::dev
         int level;
     };
 
-    __device__ __forceinline__ Cell locate(const List& l, int64_t c)
+    __host__ __device__ __forceinline__ Cell locate(const List& l, int64_t c)
     {
         const int64_t j = c >> HINT_SHIFT;
         int64_t lo = l.hint[j], hi = l.hint[j + 1];
@@ -47,12 +49,12 @@ namespace smr::dev
         return cell;
     }
 
-    __device__ __forceinline__ const int64_t* bases(const List& l, int64_t run)
+    __host__ __device__ __forceinline__ const int64_t* bases(const List& l, int64_t run)
     {
         return l.base + run * l.ntaps;
     }
 
-    __device__ __forceinline__ int64_t row_key(int32_t y, int32_t z)
+    __host__ __device__ __forceinline__ int64_t row_key(int32_t y, int32_t z)
     {
         return (int64_t(z) << 32) + (int64_t(y) + 2147483648LL);
     }
@@ -60,7 +62,7 @@ namespace smr::dev
     // LevelCellArray::get_interval/find (level_cell_array.hpp:669-743,
     // algorithm.hpp:391-467) as two binary searches: the row among the
     // level's rows, then the run holding x among the row's runs.
-    __device__ __forceinline__ bool find_base(const Table& t, int level, int32_t x, int32_t y, int32_t z, int64_t& base, int32_t& rs, int32_t& re)
+    __host__ __device__ __forceinline__ bool find_base(const Table& t, int level, int32_t x, int32_t y, int32_t z, int64_t& base, int32_t& rs, int32_t& re)
     {
         if (level < 0 || level > 20)
         {
@@ -115,7 +117,7 @@ namespace smr::dev
     // for the whole run is not inside the single reference run found: the
     // host then refuses to launch the operator kernels, so their accesses
     // are always inside the field storage.
-    __device__ __forceinline__ void op_resolve(const List& l, int64_t gid, const Table& t0, const Table& t1, int* err)
+    __host__ __device__ __forceinline__ void op_resolve(const List& l, int64_t gid, const Table& t0, const Table& t1, int* err)
     {
         const int64_t r   = gid / l.ntaps;
         const int tp      = int(gid - r * l.ntaps);
@@ -164,13 +166,13 @@ namespace smr::dev
     // flux of upwind_op (stencil_field.hpp:91-97),
     //   .5 * a * (ul + ur) + .5 * |a| * (ul - ur),
     // with the scalar factors (0.5*a, 0.5*|a|) formed once per launch
-    __device__ __forceinline__ double flux_h(double ha, double habs, double ul, double ur)
+    __host__ __device__ __forceinline__ double flux_h(double ha, double habs, double ul, double ur)
     {
         return ha * (ul + ur) + habs * (ul - ur);
     }
 
     // flux of upwind_scalar_burgers_op (stencil_field.hpp:193-216)
-    __device__ __forceinline__ double burgers_upwind_flux(double a, double ul, double ur)
+    __host__ __device__ __forceinline__ double burgers_upwind_flux(double a, double ul, double ur)
     {
         const bool mask1 = (a * ul) < (a * ur);
         const bool mask2 = (ul * ur) > 0.0;
@@ -189,7 +191,7 @@ namespace smr::dev
     }
 
     // unp1 = u - dt * upwind_scalar_burgers(k, u) (2D; same taps and order as upwind_value<2>)
-    __device__ __forceinline__ double burgers_value(const int64_t* b, int32_t x, int level, const double* __restrict__ u, double k0,
+    __host__ __device__ __forceinline__ double burgers_value(const int64_t* b, int32_t x, int level, const double* __restrict__ u, double k0,
                                                     double k1, double dt)
     {
         const double* uc_p = u + b[0] + x;
@@ -204,7 +206,7 @@ namespace smr::dev
 
     // interp_coeffs<3>(+1) = {1/8, 1, -1/8}, interp_coeffs<3>(-1) = {-1/8, 1, 1/8}
     // (numeric/prediction.hpp:31-35), indexed by parity (0 even, 1 odd)
-    __device__ __forceinline__ double pcoef(int parity, int k)
+    __host__ __device__ __forceinline__ double pcoef(int parity, int k)
     {
         return k == 1 ? 1.0 : ((parity == 0) == (k == 0) ? 1. / 8. : -1. / 8.);
     }
@@ -214,7 +216,7 @@ namespace smr::dev
     // dx = 2^-level is computed as S * 2^level: both are the exact value
     // S*2^level, so the result is bit-identical.
     template <int DIM>
-    __device__ __forceinline__ double upwind_value(const int64_t* b, int32_t x, int level, const double* __restrict__ u, double ha0,
+    __host__ __device__ __forceinline__ double upwind_value(const int64_t* b, int32_t x, int level, const double* __restrict__ u, double ha0,
                                                    double hb0, double ha1, double hb1, double ha2, double hb2, double dt)
     {
         const double* uc_p = u + b[0] + x;
@@ -239,7 +241,7 @@ namespace smr::dev
 
     // storage bases of the rows of column walk w: the row below its first run,
     // the rows of its n runs, the row above its last run
-    __device__ __forceinline__ void op_walk_rows(const List& l, int64_t w)
+    __host__ __device__ __forceinline__ void op_walk_rows(const List& l, int64_t w)
     {
         const int32_t* wk = l.walks + 4 * w;
         int32_t r         = wk[0];
@@ -265,7 +267,7 @@ namespace smr::dev
 
     // upwind_value<2> on values already loaded: left, right, down, up neighbours
     // of uc (same arithmetic, same order)
-    __device__ __forceinline__ double upwind_vals_2d(double uc, double ul, double ur, double ud, double uu, int level, double ha0,
+    __host__ __device__ __forceinline__ double upwind_vals_2d(double uc, double ul, double ur, double ud, double uu, int level, double ha0,
                                                      double hb0, double ha1, double hb1, double dt)
     {
         const double FL = flux_h(ha0, hb0, ul, uc);
@@ -278,7 +280,7 @@ namespace smr::dev
 
     // projection_op_ (numeric/projection.hpp:15-65): taps dst, then the 2^(dim-1) fine rows
     template <int DIM>
-    __device__ __forceinline__ void op_project(const List& l, int64_t c, double* dst, const double* src)
+    __host__ __device__ __forceinline__ void op_project(const List& l, int64_t c, double* dst, const double* src)
     {
         const Cell cell  = locate(l, c);
         const int64_t* b = bases(l, cell.run);
@@ -297,7 +299,7 @@ namespace smr::dev
     // prediction_op <s,false> (numeric/prediction.hpp:259-357): taps dst, then
     // the coarse rows (z outer, y inner) starting at x = (x0>>1) - s
     template <int DIM, int S>
-    __device__ __forceinline__ void op_predict_fine(const List& l, int64_t c, double* u)
+    __host__ __device__ __forceinline__ void op_predict_fine(const List& l, int64_t c, double* u)
     {
         const Cell cell  = locate(l, c);
         const int64_t* b = bases(l, cell.run);
@@ -341,7 +343,7 @@ namespace smr::dev
     // compute_detail_op (mr/operators.hpp:139-540): taps = the 2^(dim-1) fine
     // rows, then the coarse stencil rows (z outer, y inner)
     template <int DIM, int S>
-    __device__ __forceinline__ void op_detail(const List& l, int64_t c, const double* u, double* det)
+    __host__ __device__ __forceinline__ void op_detail(const List& l, int64_t c, const double* u, double* det)
     {
         const Cell cell  = locate(l, c);
         const int64_t* b = bases(l, cell.run);
@@ -403,7 +405,7 @@ namespace smr::dev
     // mr_criteria_op (mr/criteria.hpp:13-122): taps coarse, then fine rows;
     // thr holds eps, fine and coarse thresholds per fine level (3 x 21)
     template <int DIM>
-    __device__ __forceinline__ void op_criteria(const List& l, int64_t c, const double* det, uint8_t* tag, const double* thr, int min_level,
+    __host__ __device__ __forceinline__ void op_criteria(const List& l, int64_t c, const double* det, uint8_t* tag, const double* thr, int min_level,
                                                 int max_level)
     {
         const Cell cell  = locate(l, c);
@@ -450,7 +452,7 @@ namespace smr::dev
 
     // maximum_op (mr/operators.hpp:22-96): taps coarse, then fine rows
     template <int DIM>
-    __device__ __forceinline__ void op_maximum(const List& l, int64_t c, uint8_t* tag)
+    __host__ __device__ __forceinline__ void op_maximum(const List& l, int64_t c, uint8_t* tag)
     {
         const Cell cell  = locate(l, c);
         const int64_t* b = bases(l, cell.run);
@@ -497,7 +499,7 @@ namespace smr::dev
         }
     }
 
-    __device__ __forceinline__ void op_tag_init(const List& l, int64_t c, uint8_t* tag)
+    __host__ __device__ __forceinline__ void op_tag_init(const List& l, int64_t c, uint8_t* tag)
     {
         const Cell cell = locate(l, c);
         tag[bases(l, cell.run)[0] + cell.x] = 1;
@@ -506,7 +508,7 @@ namespace smr::dev
     // README.md:135-154 hand loop over for_each_interval (2D, a = (1, 1)):
     //   unp1 = u - dt / dx * (u - u(i-1) + u - u(j-1)),
     // with dt / dx = dt * 2^level exactly (dx = 2^-level)
-    __device__ __forceinline__ double hand_loop_value(const int64_t* b, int32_t x, int level, const double* __restrict__ u, double dt)
+    __host__ __device__ __forceinline__ double hand_loop_value(const int64_t* b, int32_t x, int level, const double* __restrict__ u, double dt)
     {
         const double uc = u[b[0] + x];
         const double k  = dt / ldexp(1.0, -level);
@@ -516,7 +518,7 @@ namespace smr::dev
     // true when the tag of a cell changes the cell array: a refine below
     // max_level or a coarsen without keep above min_level
     // (update_cell_array_from_tag, graduation.hpp:743-843)
-    __device__ __forceinline__ bool tag_changes_mesh(const List& l, int64_t c, const uint8_t* tag, int min_level, int max_level)
+    __host__ __device__ __forceinline__ bool tag_changes_mesh(const List& l, int64_t c, const uint8_t* tag, int min_level, int max_level)
     {
         const Cell cell = locate(l, c);
         const uint8_t t = tag[bases(l, cell.run)[0] + cell.x];
@@ -528,7 +530,7 @@ namespace smr::dev
     // flag[0] = 1 when the cell's tag changes the cell array.  Every lane that
     // sees a change stores: in a stage program one wave can hold cells of
     // several segments (instances), each with its own flag.
-    __device__ __forceinline__ void op_tag_change(const List& l, int64_t c, const uint8_t* tag, int min_level, int max_level, int* flag)
+    __host__ __device__ __forceinline__ void op_tag_change(const List& l, int64_t c, const uint8_t* tag, int min_level, int max_level, int* flag)
     {
         if (tag_changes_mesh(l, c, tag, min_level, max_level))
         {
@@ -536,7 +538,7 @@ namespace smr::dev
         }
     }
 
-    __device__ __forceinline__ void op_copy(const List& l, int64_t c, double* dst, const double* src)
+    __host__ __device__ __forceinline__ void op_copy(const List& l, int64_t c, double* dst, const double* src)
     {
         const Cell cell  = locate(l, c);
         const int64_t* b = bases(l, cell.run);
@@ -546,7 +548,7 @@ namespace smr::dev
     // prediction_op <s,true> (numeric/prediction.hpp:107-257): taps = the
     // 2^(dim-1) destination rows (children), then the source stencil rows
     template <int DIM, int S>
-    __device__ __forceinline__ void op_predict_children(const List& l, int64_t c, double* dst, const double* src)
+    __host__ __device__ __forceinline__ void op_predict_children(const List& l, int64_t c, double* dst, const double* src)
     {
         const Cell cell  = locate(l, c);
         const int64_t* b = bases(l, cell.run);
@@ -607,7 +609,7 @@ namespace smr::dev
 
     // one outer-ghost recipe: copy, Dirichlet<1> (bc/dirichlet.hpp:29), or the BC
     // projection average (algorithm/update_outer_ghost.hpp:78-131)
-    __device__ __forceinline__ void op_recipe(const int64_t* __restrict__ dst, const int32_t* __restrict__ op, const int64_t* __restrict__ sp,
+    __host__ __device__ __forceinline__ void op_recipe(const int64_t* __restrict__ dst, const int32_t* __restrict__ op, const int64_t* __restrict__ sp,
                                               const int64_t* __restrict__ src, int64_t r, double* u, int has_bc, double bc_value)
     {
         const int o = op[r];
@@ -649,14 +651,14 @@ namespace smr::dev
     // (lam is zero outside the cells).  The velocity derivative uses
     //   dF/da = .5 (ul + ur) + .5 sign(a) (ul - ur).
     // ------------------------------------------------------------------
-    __device__ __forceinline__ double dflux_da(double a, double ul, double ur)
+    __host__ __device__ __forceinline__ double dflux_da(double a, double ul, double ur)
     {
         const double sg = a > 0 ? 1.0 : (a < 0 ? -1.0 : 0.0);
         return .5 * (ul + ur) + .5 * sg * (ul - ur);
     }
 
     template <int DIM>
-    __device__ __forceinline__ void op_adj_upwind(const List& l, int64_t c, const double* lam, double* mu, const double* un, double* ga,
+    __host__ __device__ __forceinline__ void op_adj_upwind(const List& l, int64_t c, const double* lam, double* mu, const double* un, double* ga,
                                                   double a0, double a1, double a2, double dt)
     {
         const Cell cell  = locate(l, c);
@@ -689,7 +691,7 @@ namespace smr::dev
 
     // dir: 0 x-1, 1 x+1, 2 y-1, 3 y+1, 4 z-1, 5 z+1 (the neighbour is a ghost)
     template <int DIM>
-    __device__ __forceinline__ void op_adj_scatter(const List& l, int64_t c, int dir, const double* lam, double* mu, double a0, double a1,
+    __host__ __device__ __forceinline__ void op_adj_scatter(const List& l, int64_t c, int dir, const double* lam, double* mu, double a0, double a1,
                                                    double a2, double dt)
     {
         const Cell cell  = locate(l, c);
@@ -709,7 +711,7 @@ namespace smr::dev
         }
     }
 
-    __device__ __forceinline__ void op_adj_gather(const int64_t* __restrict__ tgt, const int64_t* __restrict__ ptr,
+    __host__ __device__ __forceinline__ void op_adj_gather(const int64_t* __restrict__ tgt, const int64_t* __restrict__ ptr,
                                                   const int64_t* __restrict__ src, const double* __restrict__ w, int64_t r, double* mu)
     {
         double acc = 0;
@@ -720,13 +722,13 @@ namespace smr::dev
         mu[tgt[r]] += acc;
     }
 
-    __device__ __forceinline__ void op_cells_to_compact(const List& l, int64_t c, double* out, const double* u)
+    __host__ __device__ __forceinline__ void op_cells_to_compact(const List& l, int64_t c, double* out, const double* u)
     {
         const Cell cell = locate(l, c);
         out[c]          = u[bases(l, cell.run)[0] + cell.x];
     }
 
-    __device__ __forceinline__ void op_cells_from_compact(const List& l, int64_t c, double* u, const double* in)
+    __host__ __device__ __forceinline__ void op_cells_from_compact(const List& l, int64_t c, double* u, const double* in)
     {
         const Cell cell                        = locate(l, c);
         u[bases(l, cell.run)[0] + cell.x] = in[c];
