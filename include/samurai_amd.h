@@ -176,6 +176,16 @@ int smr_upwind(const double* a, double dt, smr_field* u, smr_field* unp1);
 
 /* make_MRAdapt(u) (mr/adapt.hpp:391-397) */
 int smr_adapt_create(smr_field* field, smr_adapt** out);
+/* make_MRAdapt(u, v, ...) (mr/adapt.hpp:138-146, 391-397): n fields on one
+ * mesh adapted together -- the details of every field join the criteria
+ * (mr/criteria.hpp:47-105: a coarse detail of any field above the coarse
+ * threshold blocks the coarsening, which needs every field's children below
+ * eps; a child is refined when any field's detail is above the fine
+ * threshold) and every field is transferred to each new mesh
+ * (update_fields, algorithm/update_fields.hpp:27-127). */
+int smr_adapt_create_fields(smr_field* const* fields, int n, smr_adapt** out);
+/* the number of fields an adaptation adapts */
+int smr_adapt_nb_fields(const smr_adapt* adapt);
 int smr_adapt_destroy(smr_adapt* adapt);
 /* MRadaptation(mra_config().epsilon(eps).regularity(reg)) (mr/adapt.hpp:147-195):
  * returns the number of harten iterations run (>= 0) */
@@ -197,14 +207,16 @@ int smr_adapt_tags(smr_adapt* adapt, int ite, double eps, double regularity, uin
  * hipMalloc, hipFree or allocating routine and clears with kernels, not
  * hipMemsetAsync.  The details go to a static device array of 2^21 FP64
  * slots per GPU (16 MiB, held until the process ends), so the mesh's
- * reference storage must not exceed 2^21 slots (SMR_ERR_ARGUMENT
- * otherwise), and the mesh version's device programs must already exist:
+ * reference storage times the number of adapted fields must not exceed 2^21
+ * slots (SMR_ERR_ARGUMENT otherwise), and the mesh version's device programs must already exist:
  * run smr_adapt_tags or smr_adapt_run once on it first (SMR_ERR_ARGUMENT
  * otherwise).  Every call on one GPU shares that array: calls that overlap
  * on one GPU (several contexts or threads) race. */
 int smr_adapt_tags_noalloc(smr_adapt* adapt, int ite, double eps, double regularity, uint8_t* tags);
 /* tags (CellFlag bits, cell_flag.hpp:11-17) and details of the last harten
- * call, indexed by the storage of the mesh they were computed on */
+ * call, indexed by the storage of the mesh they were computed on; details
+ * holds smr_adapt_nb_fields arrays of smr_adapt_last_size values, one per
+ * field in the order given */
 int64_t smr_adapt_last_size(const smr_adapt* adapt);
 int smr_adapt_last_tags(const smr_adapt* adapt, uint8_t* tags, double* details);
 

@@ -533,6 +533,14 @@ namespace samurai_amd
         u.touched();
     }
 
+    // update_ghost_mr(u, v...) (algorithm/update_ghost_mr.hpp:260-270)
+    template <std::size_t dim, class... Fields>
+    void update_ghost_mr(ScalarField<dim>& u, ScalarField<dim>& v, Fields&... others)
+    {
+        update_ghost_mr(u);
+        update_ghost_mr(v, others...);
+    }
+
     template <std::size_t dim>
     void swap(ScalarField<dim>& u, ScalarField<dim>& v)
     {
@@ -658,16 +666,22 @@ namespace samurai_amd
         double eps_ = 1e-4, reg_ = 1.0; // mr/config.hpp defaults
     };
 
+    // Adapt (mr/adapt.hpp:96-146) over one or more fields on one mesh
     template <std::size_t dim>
     class MRAdapt
     {
       public:
-        explicit MRAdapt(ScalarField<dim>& u)
-            : u_(&u)
-            , field_h_(u.shared_handle())
+        explicit MRAdapt(std::vector<ScalarField<dim>*> fields)
+            : fields_(std::move(fields))
         {
+            std::vector<smr_field*> hs;
+            for (ScalarField<dim>* f : fields_)
+            {
+                hs.push_back(f->handle());
+                field_h_.push_back(f->shared_handle());
+            }
             smr_adapt* a = nullptr;
-            check(smr_adapt_create(u.handle(), &a));
+            check(smr_adapt_create_fields(hs.data(), int(hs.size()), &a));
             h_.reset(a, [](smr_adapt* p) { smr_adapt_destroy(p); });
         }
         void operator()(const mra_config& cfg)
@@ -676,21 +690,29 @@ namespace samurai_amd
         }
         void operator()(double eps, double regularity)
         {
+            for (ScalarField<dim>* f : fields_)
+            {
+                f->handle(); // host-side writes reach the device first
+            }
             check(smr_adapt_run(h_.get(), eps, regularity));
-            u_->touched();
+            for (ScalarField<dim>* f : fields_)
+            {
+                f->touched();
+            }
         }
 
       private:
-        ScalarField<dim>* u_;
-        // the field outlives the adaptation object
-        std::shared_ptr<smr_field> field_h_;
+        std::vector<ScalarField<dim>*> fields_;
+        // the fields outlive the adaptation object
+        std::vector<std::shared_ptr<smr_field>> field_h_;
         std::shared_ptr<smr_adapt> h_;
     };
 
-    template <std::size_t dim>
-    MRAdapt<dim> make_MRAdapt(ScalarField<dim>& u)
+    // make_MRAdapt(u, v...) (mr/adapt.hpp:391-397)
+    template <std::size_t dim, class... Fields>
+    MRAdapt<dim> make_MRAdapt(ScalarField<dim>& u, Fields&... others)
     {
-        return MRAdapt<dim>(u);
+        return MRAdapt<dim>(std::vector<ScalarField<dim>*>{&u, &others...});
     }
 } // namespace samurai_amd
 

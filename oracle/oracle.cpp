@@ -362,6 +362,9 @@ namespace oracle
         std::vector<double> u;
         std::vector<double> detail;
         std::vector<uint8_t> tag;
+        // fields adapted together with u (make_MRAdapt(u, v...),
+        // mr/adapt.hpp:138-146): their storage and their details
+        std::vector<std::vector<double>> extra, extra_detail;
         bool has_bc     = false;
         double bc_value = 0.0;
         bool poison     = true; // NaN-fill storage nobody wrote (SAMURAI_CHECK_NAN idea)
@@ -751,17 +754,35 @@ namespace oracle
         {
             fidx[c] = m.index(fine_level, 2 * x + (c & 1), 2 * y + ((c >> 1) & 1), m.dim == 3 ? 2 * z + ((c >> 2) & 1) : 0);
         }
+        // the details of every adapted field (the n_comp loops of
+        // criteria.hpp:47-56, 63-72, 96-105)
+        std::vector<const std::vector<double>*> comps{&s.detail};
+        for (const auto& d : s.extra_detail)
+        {
+            comps.push_back(&d);
+        }
         if (fine_level > m.min_level)
         {
-            bool cond = std::abs(s.detail[size_t(m.index(level, x, y, z))]) > coarse_eps;
+            bool cond = false;
+            for (const auto* d : comps)
+            {
+                if (std::abs((*d)[size_t(m.index(level, x, y, z))]) > coarse_eps)
+                {
+                    cond = true;
+                    break;
+                }
+            }
             if (!cond)
             {
                 bool coarsen_ok = true;
                 for (int c = 0; c < nch; ++c)
                 {
-                    if (std::abs(s.detail[size_t(fidx[c])]) > eps)
+                    for (const auto* d : comps)
                     {
-                        coarsen_ok = false;
+                        if (std::abs((*d)[size_t(fidx[c])]) > eps)
+                        {
+                            coarsen_ok = false;
+                        }
                     }
                 }
                 if (coarsen_ok)
@@ -777,7 +798,12 @@ namespace oracle
         {
             for (int c = 0; c < nch; ++c)
             {
-                bool r = std::abs(s.detail[size_t(fidx[c])]) > fine_eps;
+                bool r = kPlant == 9; // plant 9: refine only when every field asks
+                for (const auto* d : comps)
+                {
+                    const bool above = std::abs((*d)[size_t(fidx[c])]) > fine_eps;
+                    r                = kPlant == 9 ? (r && above) : (r || above);
+                }
                 s.tag[size_t(fidx[c])] |= uint8_t(r) * REFINE;
             }
         }
@@ -1083,18 +1109,34 @@ namespace oracle
                     s.tag[size_t(m.index(l, x, y, z))] = KEEP;
                 });
         }
-        update_ghost_mr(s);
-
-        for (int level = (min_level > 0 ? min_level - 1 : 0); level < max_level - ite; ++level)
+        // update_ghost_mr(m_fields) and compute_detail(m_detail, m_fields)
+        // (mr/adapt.hpp:292, 309-318): each extra field is swapped into u and
+        // its detail array into detail while its turn lasts
+        s.extra_detail.assign(s.extra.size(), std::vector<double>(size_t(m.nref), 0.0));
+        for (size_t k = 0; k <= s.extra.size(); ++k)
         {
-            DSet below = on(intersection(m.ids[REFERENCE][size_t(level)],
-                                         union_(m.ids[CELLS][size_t(level + 1)], m.ids[CELLS][size_t(std::min(level + 2, MAXL - 1))])),
-                            level);
-            below.for_each(
-                [&](int x, int y, int z)
-                {
-                    compute_detail(s, level, x, y, z);
-                });
+            if (k > 0)
+            {
+                std::swap(s.u, s.extra[k - 1]);
+                std::swap(s.detail, s.extra_detail[k - 1]);
+            }
+            update_ghost_mr(s);
+            for (int level = (min_level > 0 ? min_level - 1 : 0); level < max_level - ite; ++level)
+            {
+                DSet below = on(intersection(m.ids[REFERENCE][size_t(level)],
+                                             union_(m.ids[CELLS][size_t(level + 1)], m.ids[CELLS][size_t(std::min(level + 2, MAXL - 1))])),
+                                level);
+                below.for_each(
+                    [&](int x, int y, int z)
+                    {
+                        compute_detail(s, level, x, y, z);
+                    });
+            }
+            if (k > 0)
+            {
+                std::swap(s.u, s.extra[k - 1]);
+                std::swap(s.detail, s.extra_detail[k - 1]);
+            }
         }
         for (int level = min_level; level <= max_level - ite; ++level)
         {
@@ -1128,7 +1170,18 @@ namespace oracle
             return true;
         }
         Mesh nm = make_mesh_from_cells(m, new_ca);
-        update_fields(s, nm);
+        for (size_t k = 0; k <= s.extra.size(); ++k)
+        {
+            if (k > 0)
+            {
+                std::swap(s.u, s.extra[k - 1]);
+            }
+            update_fields(s, nm);
+            if (k > 0)
+            {
+                std::swap(s.u, s.extra[k - 1]);
+            }
+        }
         s.mesh = std::move(nm);
         return false;
     }
@@ -1915,6 +1968,41 @@ extern "C"
                     });
             }
         });
+        return 0;
+    }
+
+    // another field adapted with the main one (make_MRAdapt(u, v...)), zero
+    // storage; returns its number k >= 0
+    int orc_add_field(void* p)
+    {
+        auto* s = static_cast<Sim*>(p);
+        s->extra.emplace_back(s->u.size(), 0.0);
+        return int(s->extra.size()) - 1;
+    }
+
+    // values of extra field k on the leaves / its whole storage
+    int orc_set_extra_values(void* p, int k, const double* v)
+    {
+        auto* s = static_cast<Sim*>(p);
+        std::swap(s->u, s->extra[size_t(k)]);
+        const int rc = orc_set_cells_values(p, v);
+        std::swap(s->u, s->extra[size_t(k)]);
+        return rc;
+    }
+
+    int orc_get_extra_values(void* p, int k, double* v)
+    {
+        auto* s = static_cast<Sim*>(p);
+        std::swap(s->u, s->extra[size_t(k)]);
+        const int rc = orc_get_cells_values(p, v);
+        std::swap(s->u, s->extra[size_t(k)]);
+        return rc;
+    }
+
+    int orc_get_extra_storage(void* p, int k, double* v)
+    {
+        auto* s = static_cast<Sim*>(p);
+        std::memcpy(v, s->extra[size_t(k)].data(), s->extra[size_t(k)].size() * sizeof(double));
         return 0;
     }
 

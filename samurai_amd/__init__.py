@@ -99,6 +99,8 @@ def lib():
         _sig(L, "smr_upwind", "argtypes", [P(dbl), dbl, vp, vp])
         _sig(L, "smr_adapt_create", "argtypes", [vp, ctypes.POINTER(vp)])
         _sig(L, "smr_adapt_destroy", "argtypes", [vp])
+        _sig(L, "smr_adapt_create_fields", "argtypes", [ctypes.POINTER(vp), i32, ctypes.POINTER(vp)])
+        _sig(L, "smr_adapt_nb_fields", "argtypes", [vp])
         _sig(L, "smr_adapt_run", "argtypes", [vp, dbl, dbl])
         _sig(L, "smr_adapt_harten", "argtypes", [vp, i32, dbl, dbl])
         _sig(L, "smr_adapt_tags", "argtypes", [vp, i32, dbl, dbl, vp])
@@ -392,8 +394,10 @@ def swap(u, v):
     _check(lib().smr_field_swap(u.h, v.h))
 
 
-def update_ghost_mr(field):
-    _check(lib().smr_update_ghost_mr(field.h))
+def update_ghost_mr(*fields):
+    """update_ghost_mr(u, v, ...) (algorithm/update_ghost_mr.hpp:260-270)."""
+    for field in fields:
+        _check(lib().smr_update_ghost_mr(field.h))
 
 
 def upwind(a, dt, u, unp1):
@@ -472,15 +476,22 @@ def convection_terms(mesh, velocity=None):
 
 
 class Adapt(_Handle):
-    """samurai::Adapt built by make_MRAdapt(u) (mr/adapt.hpp)."""
+    """samurai::Adapt built by make_MRAdapt(u, v, ...) (mr/adapt.hpp): the
+    fields (one mesh) are adapted together."""
 
     _destroy = "smr_adapt_destroy"
 
-    def __init__(self, field):
+    def __init__(self, *fields):
+        if not fields:
+            raise ValueError("make_MRAdapt needs at least one field")
         h = ctypes.c_void_p()
-        _check(lib().smr_adapt_create(field.h, ctypes.byref(h)))
-        self._own(field.mesh.ctx, h)
-        self.field = field
+        if len(fields) == 1:
+            _check(lib().smr_adapt_create(fields[0].h, ctypes.byref(h)))
+        else:
+            arr = (ctypes.c_void_p * len(fields))(*[f.h for f in fields])
+            _check(lib().smr_adapt_create_fields(arr, len(fields), ctypes.byref(h)))
+        self._own(fields[0].mesh.ctx, h)
+        self.field, self.fields = fields[0], list(fields)
 
     def __call__(self, epsilon=1e-4, regularity=1.0):
         """MRadaptation(mra_config().epsilon(eps).regularity(reg)); returns #harten calls."""
@@ -500,15 +511,17 @@ class Adapt(_Handle):
         _check(lib().smr_adapt_tags_noalloc(self.h, int(ite), float(epsilon), float(regularity), ctypes.c_void_p(tags_ptr)))
 
     def last_tags(self):
+        """Tags and details of the last harten call (details: one array per
+        field, concatenated; a single field gives one array)."""
         n = lib().smr_adapt_last_size(self.h)
         t = np.zeros(max(n, 0), dtype=np.uint8)
-        d = np.zeros(max(n, 0))
+        d = np.zeros(max(n, 0) * len(self.fields))
         _check(lib().smr_adapt_last_tags(self.h, t.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), _dp(d)))
         return t, d
 
 
-def make_MRAdapt(field):  # noqa: N802 (reference name)
-    return Adapt(field)
+def make_MRAdapt(*fields):  # noqa: N802 (reference name)
+    return Adapt(*fields)
 
 
 def upwind_variant_count():

@@ -782,9 +782,41 @@ extern "C"
                 require(field && out, "null argument");
                 auto* a    = new smr_adapt();
                 a->a.field = &field->f;
+                a->a.fields.push_back(&field->f);
                 *out       = a;
                 return 0;
             });
+    }
+
+    int smr_adapt_create_fields(smr_field* const* fields, int n, smr_adapt** out)
+    {
+        return guarded(
+            [&]
+            {
+                require(fields && out && n >= 1, "null argument or no field");
+                for (int k = 0; k < n; ++k)
+                {
+                    require(fields[k] != nullptr, "null field");
+                    require(fields[k]->f.mesh == fields[0]->f.mesh, "make_MRAdapt: the fields must live on one mesh");
+                    for (int q = 0; q < k; ++q)
+                    {
+                        require(fields[q] != fields[k], "make_MRAdapt: a field is given twice");
+                    }
+                }
+                auto* a    = new smr_adapt();
+                a->a.field = &fields[0]->f;
+                for (int k = 0; k < n; ++k)
+                {
+                    a->a.fields.push_back(&fields[k]->f);
+                }
+                *out = a;
+                return 0;
+            });
+    }
+
+    int smr_adapt_nb_fields(const smr_adapt* adapt)
+    {
+        return adapt ? adapt->a.ncomp() : SMR_ERR_ARGUMENT;
     }
 
     int smr_adapt_destroy(smr_adapt* adapt)
@@ -840,7 +872,7 @@ extern "C"
                 require(tags != nullptr, "null tag array");
                 require(ite >= 0, "negative harten iteration");
                 adapt->a.field->mesh->ctx->require_device();
-                smr::adapt_tags_noalloc(*adapt->a.field, ite, eps, regularity, tags);
+                smr::adapt_tags_noalloc(adapt->a, ite, eps, regularity, tags);
                 return 0;
             });
     }

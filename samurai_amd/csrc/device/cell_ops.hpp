@@ -403,10 +403,15 @@ namespace smr::dev
     }
 
     // mr_criteria_op (mr/criteria.hpp:13-122): taps coarse, then fine rows;
-    // thr holds eps, fine and coarse thresholds per fine level (3 x 21)
+    // thr holds eps, fine and coarse thresholds per fine level (3 x 21).  The
+    // details of ncomp components (one per adapted field) are stored one array
+    // after the other, cstride apart: a coarse detail of any component above
+    // coarse_eps blocks the coarsening, which needs every component of every
+    // child at most eps; a child is refined when any of its components is
+    // above fine_eps (criteria.hpp:47-56, 63-72, 96-105)
     template <int DIM>
     __host__ __device__ __forceinline__ void op_criteria(const List& l, int64_t c, const double* det, uint8_t* tag, const double* thr, int min_level,
-                                                int max_level)
+                                                int max_level, int ncomp = 1, int64_t cstride = 0)
     {
         const Cell cell  = locate(l, c);
         const int64_t* b = bases(l, cell.run);
@@ -422,13 +427,21 @@ namespace smr::dev
         const double eps = thr[fl], fine_eps = thr[21 + fl], coarse_eps = thr[42 + fl];
         if (fl > min_level)
         {
-            if (!(fabs(det[b[0] + x]) > coarse_eps))
+            bool cond = false;
+            for (int n = 0; n < ncomp && !cond; ++n)
+            {
+                cond = fabs(det[n * cstride + b[0] + x]) > coarse_eps;
+            }
+            if (!cond)
             {
                 bool ok = true;
 #pragma unroll
                 for (int k = 0; k < NC; ++k)
                 {
-                    ok = ok && !(fabs(det[fi[k]]) > eps);
+                    for (int n = 0; n < ncomp; ++n)
+                    {
+                        ok = ok && !(fabs(det[n * cstride + fi[k]]) > eps);
+                    }
                 }
                 if (ok)
                 {
@@ -445,7 +458,12 @@ namespace smr::dev
 #pragma unroll
             for (int k = 0; k < NC; ++k)
             {
-                tag[fi[k]] |= uint8_t(fabs(det[fi[k]]) > fine_eps) * uint8_t(4);
+                bool refine = false;
+                for (int n = 0; n < ncomp; ++n)
+                {
+                    refine = refine || fabs(det[n * cstride + fi[k]]) > fine_eps;
+                }
+                tag[fi[k]] |= uint8_t(refine) * uint8_t(4);
             }
         }
     }

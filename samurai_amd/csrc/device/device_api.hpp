@@ -216,8 +216,9 @@ namespace smr::dev
     void project(int dim, const List& l, int64_t c0, int64_t c1, double* dst, const double* src, hipStream_t s);
     void predict_fine(int dim, int pred_s, const List& l, int64_t c0, int64_t c1, double* u, hipStream_t s);
     void detail(int dim, int pred_s, const List& l, int64_t c0, int64_t c1, const double* u, double* det, hipStream_t s);
+    // ncomp detail arrays (one per adapted field), cstride apart
     void criteria(int dim, const List& l, int64_t c0, int64_t c1, const double* det, uint8_t* tag, const Thresholds& thr, int min_level,
-                  int max_level, hipStream_t s);
+                  int max_level, hipStream_t s, int ncomp = 1, int64_t cstride = 0);
     void maximum(int dim, const List& l, int64_t c0, int64_t c1, uint8_t* tag, hipStream_t s);
     void tag_init(const List& l, int64_t c0, int64_t c1, uint8_t* tag, hipStream_t s);
     // p[0, n) = 0 by a kernel (graph-capturable, unlike hipMemsetAsync)

@@ -528,8 +528,8 @@ namespace smr::dev
         template <int DIM, int S>
         SMR_CELL_KERNEL(k_detail, (op_detail<DIM, S>(l, c, u, det)), const double* u, double* det)
         template <int DIM>
-        SMR_CELL_KERNEL(k_criteria, (op_criteria<DIM>(l, c, det, tag, thr.eps, min_level, max_level)), const double* det, uint8_t* tag,
-                        Thresholds thr, int min_level, int max_level)
+        SMR_CELL_KERNEL(k_criteria, (op_criteria<DIM>(l, c, det, tag, thr.eps, min_level, max_level, ncomp, cstride)), const double* det,
+                        uint8_t* tag, Thresholds thr, int min_level, int max_level, int ncomp, int64_t cstride)
         template <int DIM>
         SMR_CELL_KERNEL(k_maximum, (op_maximum<DIM>(l, c, tag)), uint8_t* tag)
         SMR_CELL_KERNEL(k_tag_init, (op_tag_init(l, c, tag)), uint8_t* tag)
@@ -904,11 +904,11 @@ namespace smr::dev
     }
 
     void criteria(int dim, const List& l, int64_t c0, int64_t c1, const double* det, uint8_t* tag, const Thresholds& thr, int min_level,
-                  int max_level, hipStream_t s)
+                  int max_level, hipStream_t s, int ncomp, int64_t cstride)
     {
         if (c1 > c0)
         {
-            SMR_DISPATCH_DIM(k_criteria, det, tag, thr, min_level, max_level)
+            SMR_DISPATCH_DIM(k_criteria, det, tag, thr, min_level, max_level, ncomp, cstride)
         }
     }
 

@@ -48,8 +48,9 @@ namespace smr
         g_scratch[device] = static_cast<double*>(p);
     }
 
-    void adapt_tags_noalloc(Field& f, int ite, double eps, double regularity, uint8_t* tag)
+    void adapt_tags_noalloc(AdaptState& st, int ite, double eps, double regularity, uint8_t* tag)
     {
+        Field& f = *st.field;
         f.check_current();
         Mesh& m         = *f.mesh;
         const int dev   = m.ctx->device;
@@ -62,16 +63,23 @@ namespace smr
         {
             throw std::runtime_error("adapt_tags_noalloc: no scratch address for this device (context not initialised)");
         }
-        if (m.host.nref > kNoAllocSlots)
+        if (m.host.nref * st.ncomp() > kNoAllocSlots)
         {
-            throw std::invalid_argument("adapt_tags_noalloc: " + std::to_string(m.host.nref) + " storage slots exceed the scratch of "
-                                        + std::to_string(kNoAllocSlots));
+            throw std::invalid_argument("adapt_tags_noalloc: " + std::to_string(m.host.nref) + " storage slots x " + std::to_string(st.ncomp())
+                                        + " fields exceed the scratch of " + std::to_string(kNoAllocSlots));
         }
         if (!m.dev || !m.dev->has_ghost || !m.dev->has_adapt)
         {
             throw std::invalid_argument("adapt_tags_noalloc: the mesh's device programs are not built; run the allocating "
                                         "entry point (smr_adapt_tags or smr_adapt_run) once on this mesh version first");
         }
-        harten_tags(f, ite, eps, regularity, tag, det);
+        if (st.fields.empty())
+        {
+            harten_tags(f, ite, eps, regularity, tag, det);
+        }
+        else
+        {
+            harten_tags(st.fields.data(), int(st.fields.size()), ite, eps, regularity, tag, det);
+        }
     }
 } // namespace smr

@@ -814,7 +814,8 @@ namespace smr
         }
         if (details && last_size > 0)
         {
-            hip_check(hipMemcpyAsync(details, detail, size_t(last_size) * 8, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(detail)");
+            hip_check(hipMemcpyAsync(details, detail, size_t(last_size) * size_t(ncomp()) * 8, hipMemcpyDeviceToHost, s),
+                      "hipMemcpyAsync(detail)");
             hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
         }
     }
@@ -1158,8 +1159,15 @@ namespace smr
     // ------------------------------------------------------------------
     namespace
     {
+        std::vector<Field*> adapted_fields(AdaptState& st)
+        {
+            return st.fields.empty() ? std::vector<Field*>{st.field} : st.fields;
+        }
+
+        // detail arrays of all adapted fields (ncomp x n) and the tags (n)
         void ensure_adapt_buffers(AdaptState& st, int64_t n)
         {
+            n *= st.ncomp();
             if (n > st.capacity)
             {
                 if (st.detail)
@@ -1177,9 +1185,9 @@ namespace smr
         }
 
         // update_fields (algorithm/update_fields.hpp:27-127) into a new mesh image
-        void update_fields(Field& f, MRMeshData& nm, DevMesh& ndev)
+        void update_fields(const std::vector<Field*>& fs, MRMeshData& nm, DevMesh& ndev)
         {
-            Mesh& m         = *f.mesh;
+            Mesh& m         = *fs[0]->mesh;
             Ctx& ctx        = *m.ctx;
             ScopedTimer timer(&ctx, "fields update");
             hipStream_t s   = ctx.stream;
@@ -1190,15 +1198,18 @@ namespace smr
             upload_list(ctx, coarsen, ndev.table, odev.table);
             upload_list(ctx, refine, ndev.table, odev.table);
             ctx.check_resolve();
-            double* nd = static_cast<double*>(dev_alloc(size_t(std::max<int64_t>(nm.nref, 1)) * 8, s));
-            hip_check(hipMemsetAsync(nd, 0, size_t(nm.nref) * 8, s), "hipMemsetAsync(new field)");
-            dev::copy_cells(copy.d, 0, copy.d.ncells, nd, f.d, s);
-            dev::project(nm.cfg.dim, coarsen.d, 0, coarsen.d.ncells, nd, f.d, s);
-            dev::predict_children(nm.cfg.dim, nm.cfg.pred_s, refine.d, 0, refine.d.ncells, nd, f.d, s);
-            // the lists and the old storage are released in stream order after these kernels
-            dev_free(f.d, s);
-            f.d    = nd;
-            f.size = nm.nref;
+            for (Field* f : fs)
+            {
+                double* nd = static_cast<double*>(dev_alloc(size_t(std::max<int64_t>(nm.nref, 1)) * 8, s));
+                hip_check(hipMemsetAsync(nd, 0, size_t(nm.nref) * 8, s), "hipMemsetAsync(new field)");
+                dev::copy_cells(copy.d, 0, copy.d.ncells, nd, f->d, s);
+                dev::project(nm.cfg.dim, coarsen.d, 0, coarsen.d.ncells, nd, f->d, s);
+                dev::predict_children(nm.cfg.dim, nm.cfg.pred_s, refine.d, 0, refine.d.ncells, nd, f->d, s);
+                // the lists and the old storage are released in stream order after these kernels
+                dev_free(f->d, s);
+                f->d    = nd;
+                f->size = nm.nref;
+            }
         }
     } // namespace
 
@@ -1349,26 +1360,49 @@ namespace smr
 
     void harten_tags(Field& f, int ite, double eps, double regularity, uint8_t* tag, double* det)
     {
-        f.check_current();
+        Field* fs[1] = {&f};
+        harten_tags(fs, 1, ite, eps, regularity, tag, det);
+    }
+
+    void harten_tags(Field* const* fs, int nf, int ite, double eps, double regularity, uint8_t* tag, double* det)
+    {
+        Field& f = *fs[0];
+        for (int k = 0; k < nf; ++k)
+        {
+            Field* g = fs[k];
+            g->check_current();
+            if (g->mesh != f.mesh)
+            {
+                throw std::invalid_argument("make_MRAdapt: the fields must live on one mesh");
+            }
+        }
         Mesh& m              = *f.mesh;
         Ctx& ctx             = *m.ctx;
         hipStream_t s        = ctx.stream;
         const MeshConfig cfg = m.host.cfg;
         const int dim = cfg.dim, minl = cfg.min_level, maxl = cfg.max_level;
         DevMesh& dm     = m.device_adapt();
-        const int64_t n = m.host.nref;
-        dev::zero_f64(det, n, s);
+        const int64_t n   = m.host.nref;
+        const int ncomp   = nf;
+        dev::zero_f64(det, n * ncomp, s);
         dev::zero_u8(tag, n, s);
         dev::tag_init(dm.cells.d, 0, dm.cells.d.ncells, tag, s);
 
-        update_ghost_mr(f);
+        // update_ghost_mr(m_fields) (mr/adapt.hpp:292)
+        for (int k = 0; k < nf; ++k)
+        {
+            update_ghost_mr(*fs[k]);
+        }
 
-        // detail computation: levels [min-1, max-ite)
+        // detail computation: levels [min-1, max-ite), one detail array per field
         {
             ScopedTimer timer(&ctx, "detail computation");
             int64_t c0, c1;
             dm.detail.cells_of_levels(minl > 0 ? minl - 1 : 0, maxl - ite, c0, c1);
-            dev::detail(dim, cfg.pred_s, dm.detail.d, c0, c1, f.d, det, s);
+            for (int k = 0; k < ncomp; ++k)
+            {
+                dev::detail(dim, cfg.pred_s, dm.detail.d, c0, c1, fs[size_t(k)]->d, det + k * n, s);
+            }
         }
         // tag cells: fine levels [min, max-ite], i.e. coarse levels [min-1, max-ite-1]
         {
@@ -1376,7 +1410,7 @@ namespace smr
             const dev::Thresholds thr = make_thresholds(cfg, ite, eps, regularity);
             int64_t c0, c1;
             dm.criteria.cells_of_levels(minl - 1, maxl - ite, c0, c1);
-            dev::criteria(dim, dm.criteria.d, c0, c1, det, tag, thr, minl, maxl, s);
+            dev::criteria(dim, dm.criteria.d, c0, c1, det, tag, thr, minl, maxl, s, ncomp, n);
         }
         // tag finalization: levels max..1, sequential
         {
@@ -1403,7 +1437,8 @@ namespace smr
         Field& f = *st.field;
         f.check_current();
         ensure_adapt_buffers(st, f.mesh->host.nref);
-        harten_tags(f, ite, eps, regularity, tag, st.detail);
+        const std::vector<Field*> fs = adapted_fields(st);
+        harten_tags(fs.data(), int(fs.size()), ite, eps, regularity, tag, st.detail);
         st.last_size      = 0; // the tags are the caller's, not the adaptation's
         st.host_tag_valid = false;
     }
@@ -1419,7 +1454,8 @@ namespace smr
         const int minl = cfg.min_level, maxl = cfg.max_level;
         const int64_t n = m.host.nref;
         ensure_adapt_buffers(st, n);
-        harten_tags(f, ite, eps, regularity, st.tag, st.detail);
+        const std::vector<Field*> fs = adapted_fields(st);
+        harten_tags(fs.data(), int(fs.size()), ite, eps, regularity, st.tag, st.detail);
         DevMesh& dm = *m.dev;
         std::unique_ptr<DevMesh> ndev;
         MRMeshData nm;
@@ -1451,9 +1487,12 @@ namespace smr
             ScopedTimer timer(&ctx, "device upload");
             ndev = upload_image(ctx, nm, std::move(img), false);
         }
-        update_fields(f, nm, *ndev);
+        update_fields(fs, nm, *ndev);
         m.replace(std::move(nm), std::move(ndev));
-        f.version = m.version;
+        for (Field* g : fs)
+        {
+            g->version = m.version;
+        }
         return false;
     }
 

@@ -234,6 +234,11 @@ namespace smr
     struct AdaptState
     {
         Field* field   = nullptr;
+        // every adapted field, `field` first (make_MRAdapt(u, v...),
+        // mr/adapt.hpp:391-397): their details join the criteria and they are
+        // transferred to every new mesh
+        std::vector<Field*> fields;
+        int ncomp() const { return fields.empty() ? 1 : int(fields.size()); }
         double* detail = nullptr;
         uint8_t* tag   = nullptr;
         int64_t capacity = 0;
@@ -335,6 +340,9 @@ namespace smr
     // update, details, criteria and tag finalization of iteration ite, into
     // tag and det (device arrays of the mesh's nref slots); no host copy
     void harten_tags(Field& f, int ite, double eps, double regularity, uint8_t* tag, double* det);
+    // harten_tags over nf fields on one mesh (fs[0] first): ghosts of each,
+    // details of field k into det + k * nref, criteria over all of them
+    void harten_tags(Field* const* fs, int nf, int ite, double eps, double regularity, uint8_t* tag, double* det);
     // harten_tags into a caller's device tag array, details in st's buffer
     void adapt_tags(AdaptState& st, int ite, double eps, double regularity, uint8_t* tag);
 
@@ -346,5 +354,5 @@ namespace smr
     // harten_tags with the details in the static array: no allocation.  The
     // mesh's ghost and adapt programs must exist (built by an earlier
     // allocating call on this mesh version); nref <= kNoAllocSlots
-    void adapt_tags_noalloc(Field& f, int ite, double eps, double regularity, uint8_t* tag);
+    void adapt_tags_noalloc(AdaptState& st, int ite, double eps, double regularity, uint8_t* tag);
 } // namespace smr

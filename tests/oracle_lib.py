@@ -53,6 +53,10 @@ def lib():
         L.orc_upwind_hand_loop.argtypes = [vp, dbl]
         L.orc_apply_level_op.argtypes = [vp, i32, i32]
         L.orc_apply_field_bc_direction.argtypes = [vp, i32, i32, i32]
+        L.orc_add_field.argtypes = [vp]
+        L.orc_set_extra_values.argtypes = [vp, i32, P(dbl)]
+        L.orc_get_extra_values.argtypes = [vp, i32, P(dbl)]
+        L.orc_get_extra_storage.argtypes = [vp, i32, P(dbl)]
         _lib = L
     return _lib
 
@@ -119,6 +123,25 @@ class OracleSim:
         v = np.ascontiguousarray(v, dtype=np.float64)
         assert v.size == self.nb_cells(REFERENCE)
         lib().orc_set_storage(self.h, _dp(v))
+
+    def add_field(self, values):
+        """Another field on this mesh adapted together with the main one
+        (make_MRAdapt(u, v...)); values on the leaves.  Returns its number."""
+        k = _check(lib().orc_add_field(self.h))
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        assert v.size == self.nb_cells(CELLS)
+        _check(lib().orc_set_extra_values(self.h, k, _dp(v)))
+        return k
+
+    def extra_values(self, k):
+        out = np.zeros(self.nb_cells(CELLS))
+        _check(lib().orc_get_extra_values(self.h, k, _dp(out)))
+        return out
+
+    def extra_storage(self, k):
+        out = np.zeros(self.nb_cells(REFERENCE))
+        _check(lib().orc_get_extra_storage(self.h, k, _dp(out)))
+        return out
 
     def set_dirichlet(self, value=0.0):
         self._bc = float(value)

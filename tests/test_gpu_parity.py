@@ -359,3 +359,42 @@ def test_every_upwind_launch_shape_matches_oracle(gpu_ctx, dim, lmin, lmax):
             assert np.array_equal(ref, bits(unp1.cells())), f"variant {variant}"
     finally:
         sa.set_upwind_variant(-1)
+
+
+@pytest.mark.parametrize("dim,lmin,lmax", [(2, 2, 7), (3, 2, 5)])
+def test_several_fields_adapt_like_the_oracle(gpu_ctx, dim, lmin, lmax):
+    """make_MRAdapt(u, v) (mr/adapt.hpp:138-146, 391-397): the details of both
+    fields drive one mesh, and both fields are transferred to every new mesh;
+    u is advected, v rides along.  Cells and both fields bit for bit against
+    the oracle, at every step."""
+    say("test_several_fields_adapt_like_the_oracle", dim, lmin, lmax)
+    fu = disc_2d if dim == 2 else sphere_3d
+    fv = (lambda c: seeded_2d(c, 4)) if dim == 2 else (lambda c: sphere_3d(c, center=(0.3, 0.6, 0.4), radius=0.12))
+    eps = 1e-4 if dim == 2 else 1e-3
+    a = np.array([1.0, 0.5, 0.25][:dim])
+    dt = 0.4 / 3 * 2.0 ** -lmax
+    o = OracleSim(dim, lmin, lmax, pred_s=1)
+    o.set_poison(False)
+    o.set_values(fu(o.cells()))
+    o.set_dirichlet(0.0)
+    o.add_field(fv(o.cells()))
+    mesh = sa.make_mesh(gpu_ctx, sa.mesh_config(dim, lmin, lmax, 1))
+    u = sa.make_scalar_field("u", mesh)
+    v = sa.make_scalar_field("v", mesh)
+    u.set_cells(fu(mesh.cells()))
+    v.set_cells(fv(mesh.cells()))
+    sa.make_bc_dirichlet(u, 0.0)
+    sa.make_bc_dirichlet(v, 0.0)
+    unp1 = sa.make_scalar_field("unp1", mesh)
+    adapt = sa.make_MRAdapt(u, v)
+    for step in range(4):
+        assert adapt(eps, 2.0) == o.adapt(eps, 2.0), step
+        assert np.array_equal(o.cells(), mesh.cells()), step
+        assert np.array_equal(bits(o.values()), bits(u.cells())), step
+        assert np.array_equal(bits(o.extra_values(0)), bits(v.cells())), step
+        sa.update_ghost_mr(u)
+        unp1.resize()
+        sa.upwind(a, dt, u, unp1)
+        sa.swap(u, unp1)
+        o.upwind_step(a, dt)
+        assert np.array_equal(bits(o.values()), bits(u.cells())), step

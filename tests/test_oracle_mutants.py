@@ -3,7 +3,8 @@ oracle/oracle.cpp (ORACLE_PLANT = k: a swapped z parity of the 3D prediction,
 detail and on-level prediction coefficients, the 3D upwind sum without
 +(-F_F), a corner ghost copied from the wrong cell, the Dirichlet<1> ghost with
 the wrong sign, the wrong child in the 3D edge/vertex cascade, a swapped x
-parity of the 2D prediction) is built into a scratch library, and the
+parity of the 2D prediction, a refinement of several adapted fields that
+needs every field) is built into a scratch library, and the
 known-answer checks (golden fixtures, pinning and invariant tests) run on it
 must fail.  A check that passes on a mutant pins nothing about that part."""
 import os
@@ -17,7 +18,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SRC = os.path.join(ROOT, "oracle", "oracle.cpp")
 CHECKS = ["tests/test_oracle_pinning.py", "tests/test_oracle_invariants.py", "tests/test_oracle_golden.py"]
 PLANTS = {1: "3D prediction z parity", 2: "3D detail z parity", 3: "3D upwind without +(-F_F)", 4: "corner ghost source",
-          5: "Dirichlet sign", 6: "3D cascade child offset", 7: "3D on-level prediction z parity", 8: "2D prediction x parity"}
+          5: "Dirichlet sign", 6: "3D cascade child offset", 7: "3D on-level prediction z parity", 8: "2D prediction x parity",
+          9: "several fields: refinement only when every field asks"}
 
 
 @pytest.fixture(scope="module")

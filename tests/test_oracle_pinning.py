@@ -384,3 +384,42 @@ def test_edge_ghosts_cascade_below_the_finest_level():
             assert abs(st[int(r[4])] - expect) <= 1e-11, (l, g)
             checked += 1
     assert checked >= 12 * 2
+
+
+@pytest.mark.parametrize("dim,lmin,lmax", [(2, 2, 7), (3, 2, 5)])
+def test_several_fields_reduce_to_one(dim, lmin, lmax):
+    """make_MRAdapt(u, v) (mr/adapt.hpp:138-146, criteria.hpp's n_comp loops):
+    with v = u the first adaptation is the one of u alone (same leaves and
+    values, and v carried along as u); with v = 0 every step (adaptation +
+    upwind of u) is the one of u alone.  A refinement that needed every field
+    instead of any, or a coarsening allowed by any field instead of all,
+    fails the v = 0 case."""
+    from initial_conditions import disc_2d, sphere_3d
+    f = disc_2d if dim == 2 else sphere_3d
+    a = np.array([1.0, 0.5, 0.25][:dim])
+    dt = 0.4 / 3 * 2.0 ** -lmax
+    eps = 1e-4 if dim == 2 else 1e-3
+    sims = []
+    for second in (None, "same", "zero"):
+        o = OracleSim(dim, lmin, lmax, pred_s=1)
+        o.set_poison(False)
+        c = o.cells()
+        o.set_values(f(c))
+        o.set_dirichlet(0.0)
+        if second is not None:
+            o.add_field(f(c) if second == "same" else np.zeros(len(c)))
+        sims.append(o)
+    calls = [o.adapt(eps, 2.0) for o in sims]
+    assert calls[0] == calls[1] == calls[2]
+    for o in sims[1:]:
+        assert np.array_equal(o.cells(), sims[0].cells())
+        assert np.array_equal(o.values().view(np.uint64), sims[0].values().view(np.uint64))
+    assert np.array_equal(sims[1].extra_values(0).view(np.uint64), sims[1].values().view(np.uint64))
+    one, zero = sims[0], sims[2]
+    for step in range(3):
+        one.upwind_step(a, dt)
+        zero.upwind_step(a, dt)
+        assert one.adapt(eps, 2.0) == zero.adapt(eps, 2.0), step
+        assert np.array_equal(one.cells(), zero.cells()), step
+        assert np.array_equal(one.values().view(np.uint64), zero.values().view(np.uint64)), step
+        assert not np.any(zero.extra_values(0)), step
