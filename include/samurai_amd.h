@@ -324,8 +324,12 @@ int smr_batch_advance(smr_batch* batch);
 int smr_batch_step(smr_batch* batch, double eps, double regularity);
 int64_t smr_batch_nb_cells(const smr_batch* batch);
 /* launch shape of the stage programs: block size (multiple of 64, <= 1024),
- * consecutive cells per thread, mode (0 one cooperative launch per program,
- * 1 one launch per stage, 2 stage launches replayed from a hipGraph) */
+ * consecutive cells per thread (0: one precomputed 64-index wave item per
+ * wavefront, blocks of 256), mode (0 one cooperative launch per program,
+ * 1 one launch per stage, 2 stage launches replayed from a hipGraph, 3 for
+ * programs of independent instances: every run of small stages one launch
+ * with one block per instance, large stages grid-wide, replayed from a
+ * hipGraph; a program without independent instances runs mode 3 as 2) */
 int smr_batch_set_tuning(smr_batch* batch, int block, int cells_per_thread, int mode);
 int smr_batch_get_tuning(const smr_batch* batch, int* block, int* cells_per_thread, int* mode);
 /* times every launch shape on the advance program of the current state and

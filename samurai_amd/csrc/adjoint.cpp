@@ -668,6 +668,7 @@ namespace smr
         {
             // forward: checkpoint, `every` steps, the last state becomes the first
             auto f = std::make_unique<ProgramBuf>(ctx_);
+            f->set_independent_chains(true); // chain i is instance i throughout
             f->append(save[size_t(sg)]);
             for (int j = 0; j < every_; ++j)
             {
@@ -680,6 +681,7 @@ namespace smr
             // backwards; lam and mu swap every step, from the parity of the
             // number of adjoint steps of the later segments
             auto b = std::make_unique<ProgramBuf>(ctx_);
+            b->set_independent_chains(true);
             b->append(restore[size_t(sg)]);
             for (int j = 0; j < every_; ++j)
             {

@@ -109,7 +109,7 @@ namespace smr
         int steps_, every_, nseg_;
         // the programs are built once per velocity and replayed: graph replays of
         // the stage launches by default
-        Tuning tuning_{256, 1, MODE_GRAPH};
+        Tuning tuning_{256, 0, MODE_BLOCKS};
         bool programs_ready_ = false;
         const double* bound_u0_ = nullptr;
         double* bound_uT_       = nullptr;

@@ -5,7 +5,9 @@
 #include <algorithm>
 #include <array>
 #include <chrono>
+#include <climits>
 #include <cstddef>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <map>
@@ -24,6 +26,20 @@ namespace smr
         {
             return (n + a - 1) / a * a;
         }
+
+        // MODE_BLOCKS: a stage where some chain has more cells than this runs
+        // grid-wide; the others run one block per chain (SMR_BLOCK_STAGE_MAX)
+        int64_t block_stage_max()
+        {
+            static const int64_t v = []
+            {
+                const char* e = std::getenv("SMR_BLOCK_STAGE_MAX");
+                const long long x = e ? std::atoll(e) : 0;
+                return x > 0 ? int64_t(x) : int64_t(4096);
+            }();
+            return v;
+        }
+        constexpr int kBlockThreads = 1024;
     } // namespace
 
     dev::Segment make_seg(int op, const MeshConfig& cfg)
@@ -164,7 +180,7 @@ namespace smr
             {
                 case dev::SEG_RECIPE: ok = ok && g.rdst && g.rop && g.rsp && g.rsrc && g.u; break;
                 case dev::SEG_RESOLVE:
-                    ok = ok && g.list.base && g.c1 <= g.list.nruns * g.list.ntaps && (g.tabs || (g.aux & AUX_TABS)) && g.err;
+                    ok = ok && g.list.base && g.c1 <= dev::resolve_range(g.list) && (g.tabs || (g.aux & AUX_TABS)) && g.err;
                     break;
                 case dev::SEG_FILL_F64: ok = ok && g.out; break;
                 case dev::SEG_FILL_U8: ok = ok && g.tag; break;
@@ -247,12 +263,33 @@ namespace smr
         seg_pref_.assign(1, 0);
         stage_seg_.assign(1, 0);
         stage_cells_.clear();
+        stage_max_chain_.clear();
+        chain_items_.clear();
+        nchains_           = -1;
+        chains_consistent_ = true;
+        plan_.clear();
+        blk_.clear();
+        witems_.clear();
+        stage_item_.assign(1, 0);
+        stage_nitems_.clear();
         aux_.clear();
         uploaded_ = false;
     }
 
     void ProgramBuf::append(const std::vector<Chain>& chains)
     {
+        if (nchains_ < 0)
+        {
+            nchains_ = int(chains.size());
+        }
+        else if (nchains_ != int(chains.size()))
+        {
+            chains_consistent_ = false;
+        }
+        if (chain_items_.size() < chains.size())
+        {
+            chain_items_.resize(chains.size());
+        }
         size_t depth = 0;
         for (const Chain& c : chains)
         {
@@ -260,13 +297,16 @@ namespace smr
         }
         for (size_t k = 0; k < depth; ++k)
         {
-            int64_t cells = 0;
-            for (const Chain& c : chains)
+            int64_t cells = 0, most = 0;
+            const int32_t stage = int32_t(stage_cells_.size());
+            for (size_t i = 0; i < chains.size(); ++i)
             {
+                const Chain& c = chains[i];
                 if (k >= c.size())
                 {
                     continue;
                 }
+                int64_t mine = 0;
                 for (const dev::Segment& g : c[k])
                 {
                     if (g.c1 <= g.c0)
@@ -274,18 +314,181 @@ namespace smr
                         continue;
                     }
                     check_segment(g);
+                    chain_items_[i].emplace_back(int32_t(segs_.size()), stage);
                     segs_.push_back(g);
                     seg_pref_.push_back(seg_pref_.back() + (g.c1 - g.c0));
                     cells += g.c1 - g.c0;
+                    mine += g.c1 - g.c0;
                 }
+                most = std::max(most, mine);
             }
             if (cells > 0)
             {
                 stage_seg_.push_back(int32_t(segs_.size()));
                 stage_cells_.push_back(cells);
+                stage_max_chain_.push_back(most);
             }
         }
         uploaded_ = false;
+    }
+
+    // consecutive stages where no chain has more than block_stage_max() cells
+    // become one block launch; every other stage is a grid-wide launch
+    void ProgramBuf::build_block_plan()
+    {
+        plan_.clear();
+        blk_.clear();
+        if (!blocks_capable())
+        {
+            return;
+        }
+        const int ns        = nstages();
+        const int64_t limit = block_stage_max();
+        const size_t nch    = size_t(nchains_);
+        std::vector<std::vector<int32_t>> per(nch); // per block step and chain
+        std::vector<size_t> cur(nch, 0);
+        std::vector<int32_t> items;
+        std::vector<int32_t> offs;
+        for (int k = 0; k < ns;)
+        {
+            if (stage_max_chain_[size_t(k)] > limit)
+            {
+                plan_.push_back({false, k, k + 1, 0});
+                for (size_t i = 0; i < nch; ++i)
+                {
+                    while (cur[i] < chain_items_[i].size() && chain_items_[i][cur[i]].second == k)
+                    {
+                        ++cur[i];
+                    }
+                }
+                ++k;
+                continue;
+            }
+            int k1 = k;
+            while (k1 < ns && stage_max_chain_[size_t(k1)] <= limit)
+            {
+                ++k1;
+            }
+            plan_.push_back({true, k, k1, offs.size()});
+            for (size_t i = 0; i < nch; ++i)
+            {
+                offs.push_back(int32_t(items.size()));
+                const auto& ci = chain_items_[i];
+                while (cur[i] < ci.size() && ci[cur[i]].second < k1)
+                {
+                    const auto [seg, stage] = ci[cur[i]];
+                    ++cur[i];
+                    // a barrier when the chain's next item in this launch is of a later stage
+                    const bool barrier = cur[i] < ci.size() && ci[cur[i]].second < k1 && ci[cur[i]].second != stage;
+                    items.push_back(int32_t(seg) << 1 | (barrier ? 1 : 0));
+                }
+            }
+            offs.push_back(int32_t(items.size()));
+            k = k1;
+        }
+        // chain offsets are relative to the items: shift them past the offsets
+        blk_items_ = offs.size();
+        blk_       = std::move(offs);
+        for (int32_t& o : blk_)
+        {
+            o += int32_t(blk_items_);
+        }
+        blk_.insert(blk_.end(), items.begin(), items.end());
+    }
+
+    void ProgramBuf::launch_block_plan(const Tuning& t, hipStream_t s) const
+    {
+        for (const PlanStep& st : plan_)
+        {
+            if (st.blocks)
+            {
+                dev::run_program_blocks(prog_, dblk_ + st.off, dblk_, nchains_, kBlockThreads, s);
+            }
+            else
+            {
+                launch_stage(t, st.k0, s);
+            }
+        }
+    }
+
+    void ProgramBuf::launch_stage(const Tuning& t, int k, hipStream_t s) const
+    {
+        if (t.cpt <= 0)
+        {
+            dev::run_program_stage(prog_, k, stage_nitems_[size_t(k)], s);
+        }
+        else
+        {
+            dev::run_program_stage_cells(prog_, k, stage_cells_[size_t(k)], t.block, t.cpt, s);
+        }
+    }
+
+    void ProgramBuf::launch_stagewise(const Tuning& t, hipStream_t s) const
+    {
+        for (int k = 0; k < nstages(); ++k)
+        {
+            launch_stage(t, k, s);
+        }
+    }
+
+    // the segments of a stage packed densely in wave items of 64 consecutive
+    // indices, each with the segment its first index falls in; a large
+    // segment of a streaming operator gets items of kStreamChunks chunks
+    void ProgramBuf::build_wave_items()
+    {
+        constexpr int kStreamChunks = 16;
+        witems_.clear();
+        stage_item_.assign(1, 0);
+        stage_nitems_.clear();
+        auto streaming = [&](int32_t sg)
+        {
+            const dev::Segment& g = segs_[size_t(sg)];
+            return dev::is_streaming_op(g.op) && g.c1 - g.c0 >= 64 * kStreamChunks;
+        };
+        for (size_t k = 0; k + 1 < stage_seg_.size(); ++k)
+        {
+            const int32_t se = stage_seg_[k + 1];
+            for (int32_t sb = stage_seg_[k]; sb < se;)
+            {
+                if (streaming(sb))
+                {
+                    const int64_t n = segs_[size_t(sb)].c1 - segs_[size_t(sb)].c0;
+                    for (int64_t off = 0; off < n; off += 64 * kStreamChunks)
+                    {
+                        if (off > int64_t(INT32_MAX))
+                        {
+                            throw std::length_error("stage program: segment too large");
+                        }
+                        witems_.push_back({sb, int32_t(off), sb + 1, kStreamChunks});
+                    }
+                    ++sb;
+                    continue;
+                }
+                // a pack: the consecutive segments [sb, end) that are not streamed
+                int32_t end = sb + 1;
+                while (end < se && !streaming(end))
+                {
+                    ++end;
+                }
+                int32_t cur = sb;
+                for (int64_t pos = seg_pref_[size_t(sb)]; pos < seg_pref_[size_t(end)]; pos += 64)
+                {
+                    while (seg_pref_[size_t(cur) + 1] <= pos)
+                    {
+                        ++cur;
+                    }
+                    const int64_t off = pos - seg_pref_[size_t(cur)];
+                    if (off > int64_t(INT32_MAX))
+                    {
+                        throw std::length_error("stage program: segment too large");
+                    }
+                    witems_.push_back({cur, int32_t(off), end, 1});
+                }
+                sb = end;
+            }
+            stage_nitems_.push_back(int64_t(witems_.size()) - stage_item_.back());
+            stage_item_.push_back(int64_t(witems_.size()));
+        }
     }
 
     size_t ProgramBuf::add_aux(const void* src, size_t n)
@@ -324,7 +527,12 @@ namespace smr
         const size_t o_pref  = align_up(o_segs + nseg * sizeof(dev::Segment), 256);
         const size_t o_stage = align_up(o_pref + (nseg + 1) * sizeof(int64_t), 256);
         const size_t o_aux   = align_up(o_stage + stage_seg_.size() * sizeof(int32_t), 256);
-        const size_t total   = std::max<size_t>(o_aux + aux_.size(), 256);
+        build_block_plan();
+        build_wave_items();
+        const size_t o_blk   = align_up(o_aux + aux_.size(), 256);
+        const size_t o_sit   = align_up(o_blk + blk_.size() * sizeof(int32_t), 256);
+        const size_t o_wit   = align_up(o_sit + stage_item_.size() * sizeof(int64_t), 256);
+        const size_t total   = std::max<size_t>(o_wit + witems_.size() * sizeof(dev::WaveItem), 256);
         drop_graph();
         if (!copied_)
         {
@@ -375,12 +583,24 @@ namespace smr
         {
             std::memcpy(hbuf_ + o_aux, aux_.data(), aux_.size());
         }
+        if (!blk_.empty())
+        {
+            std::memcpy(hbuf_ + o_blk, blk_.data(), blk_.size() * sizeof(int32_t));
+        }
+        std::memcpy(hbuf_ + o_sit, stage_item_.data(), stage_item_.size() * sizeof(int64_t));
+        if (!witems_.empty())
+        {
+            std::memcpy(hbuf_ + o_wit, witems_.data(), witems_.size() * sizeof(dev::WaveItem));
+        }
         hip_check(hipMemcpyAsync(dbuf_, hbuf_, total, hipMemcpyHostToDevice, s), "hipMemcpyAsync(program)");
         hip_check(hipEventRecord(copied_, s), "hipEventRecord");
         prog_.segs      = reinterpret_cast<const dev::Segment*>(dbuf_ + o_segs);
         prog_.seg_pref  = reinterpret_cast<const int64_t*>(dbuf_ + o_pref);
         prog_.stage_seg = reinterpret_cast<const int32_t*>(dbuf_ + o_stage);
         prog_.nstages   = int(stage_cells_.size());
+        dblk_           = reinterpret_cast<const int32_t*>(dbuf_ + o_blk);
+        prog_.stage_item = reinterpret_cast<const int64_t*>(dbuf_ + o_sit);
+        prog_.items      = reinterpret_cast<const dev::WaveItem*>(dbuf_ + o_wit);
         uploaded_       = true;
     }
 
@@ -399,38 +619,50 @@ namespace smr
         {
             // the grid barrier costs grow with the number of blocks that arrive
             // at it: at most one block per CU, the rest is grid-strided
+            // the fused launch walks cells: cpt 0 (wave items) walks one per thread
+            const int cpt         = std::max(t.cpt, 1);
             const int maxb        = std::min(max_blocks_cached(t.block), cu_count());
-            const int64_t per     = int64_t(t.block) * t.cpt;
+            const int64_t per     = int64_t(t.block) * cpt;
             const int64_t needed  = (max_stage_cells() + per - 1) / per;
             const int nblocks     = int(std::max<int64_t>(1, std::min<int64_t>(needed, maxb)));
-            if (maxb > 0 && dev::run_program_fused(prog_, t.block, nblocks, t.cpt, s))
+            if (maxb > 0 && dev::run_program_fused(prog_, t.block, nblocks, cpt, s))
             {
                 return;
             }
             // the runtime refused the cooperative launch: same program, one launch per stage
             (void)hipGetLastError();
-            dev::run_program_stagewise(prog_, stage_cells_, t.block, t.cpt, s);
+            launch_stagewise(t, s);
             return;
         }
-        if (t.mode == MODE_GRAPH)
+        if (t.mode == MODE_GRAPH || t.mode == MODE_BLOCKS)
         {
-            if (!gexec_ || g_block_ != t.block || g_cpt_ != t.cpt)
+            const bool blocks = t.mode == MODE_BLOCKS && blocks_capable();
+            const int gmode   = blocks ? MODE_BLOCKS : MODE_GRAPH;
+            if (!gexec_ || g_block_ != t.block || g_cpt_ != t.cpt || g_mode_ != gmode)
             {
                 drop_graph();
                 hipGraph_t g = nullptr;
                 hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-                dev::run_program_stagewise(prog_, stage_cells_, t.block, t.cpt, s);
+                if (blocks)
+                {
+                    launch_block_plan(t, s);
+                }
+                else
+                {
+                    launch_stagewise(t, s);
+                }
                 hip_check(hipStreamEndCapture(s, &g), "hipStreamEndCapture");
                 const hipError_t e = hipGraphInstantiate(&gexec_, g, nullptr, nullptr, 0);
                 (void)hipGraphDestroy(g);
                 hip_check(e, "hipGraphInstantiate");
                 g_block_ = t.block;
                 g_cpt_   = t.cpt;
+                g_mode_  = gmode;
             }
             hip_check(hipGraphLaunch(gexec_, s), "hipGraphLaunch");
             return;
         }
-        dev::run_program_stagewise(prog_, stage_cells_, t.block, t.cpt, s);
+        launch_stagewise(t, s);
     }
 
     // ------------------------------------------------------------------
@@ -522,11 +754,11 @@ namespace smr
         {
             throw std::invalid_argument("block size must be a multiple of 64 in [64, 1024]");
         }
-        if (t.cpt < 1 || t.cpt > 64)
+        if (t.cpt < 0 || t.cpt > 64)
         {
-            throw std::invalid_argument("cells per thread must be in [1, 64]");
+            throw std::invalid_argument("cells per thread must be in [0, 64] (0: wave items)");
         }
-        if (t.mode < MODE_FUSED || t.mode > MODE_GRAPH)
+        if (t.mode < MODE_FUSED || t.mode > MODE_BLOCKS)
         {
             throw std::invalid_argument("unknown program mode");
         }
@@ -801,7 +1033,7 @@ namespace smr
                         Stage st;
                         auto add = [&](DevWorkList& l, size_t h)
                         {
-                            dev::Segment g = list_seg(dev::SEG_RESOLVE, c, l, 0, l.d.nruns * l.d.ntaps);
+                            dev::Segment g = list_seg(dev::SEG_RESOLVE, c, l, 0, dev::resolve_range(l.d));
                             g.tabs         = reinterpret_cast<const dev::Table*>(uintptr_t(h));
                             g.err          = ctx_->d_err;
                             g.aux          = AUX_TABS;
@@ -1046,8 +1278,14 @@ namespace smr
                 {
                     continue;
                 }
-                for (int cpt : {1, 2, 4, 8})
+                for (int cpt : {0, 1, 2, 4, 8})
                 {
+                    // wave items (cpt 0) are launched in blocks of 256 whatever the block
+                    // size, and the fused launch walks cells
+                    if (cpt == 0 && (block != 256 || mode == MODE_FUSED))
+                    {
+                        continue;
+                    }
                     const Tuning t{block, cpt, mode};
                     p.upload();
                     p.run(t); // warm-up

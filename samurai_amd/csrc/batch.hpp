@@ -19,14 +19,22 @@ namespace smr
     {
         MODE_FUSED  = 0, // one cooperative launch, grid barriers between stages
         MODE_STAGES = 1, // one launch per stage
-        MODE_GRAPH  = 2  // MODE_STAGES captured in a hipGraph, replayed while the program is unchanged
+        MODE_GRAPH  = 2, // MODE_STAGES captured in a hipGraph, replayed while the program is unchanged
+        // chains of independent instances: each run of consecutive small stages
+        // is one launch with one block per chain (block barriers between the
+        // stages), each large stage a grid-wide launch; captured in a hipGraph.
+        // Programs whose chains are not declared independent run as MODE_GRAPH.
+        MODE_BLOCKS = 3
     };
 
     struct Tuning
     {
         int block = 256;
-        int cpt   = 1; // consecutive cells per thread (interval chunk length)
-        int mode  = MODE_FUSED;
+        // grid-wide stage launches: consecutive cells per thread (interval chunk
+        // length), each wave searching for its segment; 0: one precomputed wave
+        // item (64 indices, its first segment known) per wavefront
+        int cpt  = 1;
+        int mode = MODE_FUSED;
     };
 
     using Stage = std::vector<dev::Segment>;
@@ -55,6 +63,12 @@ namespace smr
         void clear();
         // appends the chains side by side: new stage k holds stage k of every chain
         void append(const std::vector<Chain>& chains);
+        // declares that chain i of every append is the same instance and that
+        // no chain reads or writes another chain's data (what MODE_BLOCKS needs)
+        void set_independent_chains(bool on) { independent_ = on; }
+        bool blocks_capable() const { return independent_ && chains_consistent_ && nchains_ > 0; }
+        // launches of one run in MODE_BLOCKS (block launches + grid stages)
+        int block_plan_launches() const { return int(plan_.size()); }
         // bytes copied with the program; returns their offset, which a segment
         // stores in its thr / tabs pointer with the matching `aux` bit set
         size_t add_aux(const void* src, size_t n);
@@ -71,6 +85,25 @@ namespace smr
         std::vector<int64_t> seg_pref_{0}; // cumulative cells over all segments (+ total)
         std::vector<int32_t> stage_seg_{0};
         std::vector<int64_t> stage_cells_;
+        std::vector<int64_t> stage_max_chain_; // per stage: the most cells one chain has in it
+        std::vector<dev::WaveItem> witems_;     // wave items of all stages, stage by stage
+        std::vector<int64_t> stage_item_{0};    // first wave item of each stage (+ total)
+        std::vector<int64_t> stage_nitems_;     // wave items per stage
+        // per chain: (segment, stage) in program order
+        std::vector<std::vector<std::pair<int32_t, int32_t>>> chain_items_;
+        int nchains_            = -1;
+        bool chains_consistent_ = true;
+        bool independent_       = false;
+        struct PlanStep
+        {
+            bool blocks;   // one block per chain over stages [k0, k1), else stage k0 grid-wide
+            int k0, k1;
+            size_t off;    // blocks: index of the chain offsets in blk_
+        };
+        std::vector<PlanStep> plan_;
+        std::vector<int32_t> blk_; // chain offsets (nchains + 1 per block step), then the items
+        size_t blk_items_ = 0;     // index of the first item in blk_
+        const int32_t* dblk_ = nullptr;
         std::vector<char> aux_;
         char* dbuf_ = nullptr;
         size_t dcap_ = 0;
@@ -80,7 +113,12 @@ namespace smr
         dev::Program prog_{};
         // graph of the stage-wise launches of the uploaded program
         hipGraphExec_t gexec_ = nullptr;
-        int g_block_ = 0, g_cpt_ = 0;
+        int g_block_ = 0, g_cpt_ = 0, g_mode_ = -1;
+        void build_block_plan();
+        void build_wave_items();
+        void launch_block_plan(const Tuning& t, hipStream_t s) const;
+        void launch_stage(const Tuning& t, int k, hipStream_t s) const;
+        void launch_stagewise(const Tuning& t, hipStream_t s) const;
         bool uploaded_ = false;
         void drop_graph();
     };

@@ -985,6 +985,9 @@ extern "C"
             [&]
             {
                 require(batch != nullptr, "null batch");
+                require(block >= 64 && block <= 1024 && block % 64 == 0, "block size must be a multiple of 64 in [64, 1024]");
+                require(cells_per_thread >= 0 && cells_per_thread <= 64, "cells per thread must be in [0, 64] (0: wave items)");
+                require(mode >= 0 && mode <= 3, "unknown program mode");
                 batch->b->set_tuning(smr::Tuning{block, cells_per_thread, mode});
                 return 0;
             });
@@ -1120,8 +1123,8 @@ extern "C"
             {
                 require(adj != nullptr, "null adjoint");
                 require(block >= 64 && block <= 1024 && block % 64 == 0, "block size must be a multiple of 64 in [64, 1024]");
-                require(cells_per_thread >= 1 && cells_per_thread <= 64, "cells per thread must be in [1, 64]");
-                require(mode >= 0 && mode <= 2, "unknown program mode");
+                require(cells_per_thread >= 0 && cells_per_thread <= 64, "cells per thread must be in [0, 64] (0: wave items)");
+                require(mode >= 0 && mode <= 3, "unknown program mode");
                 adj->a->set_tuning(smr::Tuning{block, cells_per_thread, mode});
                 return 0;
             });

@@ -24,7 +24,8 @@ namespace smr::dev
         int level;
     };
 
-    __host__ __device__ __forceinline__ Cell locate(const List& l, int64_t c)
+    // the run holding cell c: a bisection between the runs of two hints
+    __host__ __device__ __forceinline__ int64_t find_run(const List& l, int64_t c)
     {
         const int64_t j = c >> HINT_SHIFT;
         int64_t lo = l.hint[j], hi = l.hint[j + 1];
@@ -40,6 +41,20 @@ namespace smr::dev
                 hi = mid - 1;
             }
         }
+        return lo;
+    }
+
+    // the run map of the list (one entry per cell), written by the resolve
+    __host__ __device__ __forceinline__ void op_cellrun(const List& l, int64_t c)
+    {
+        l.cellrun[c] = int32_t(find_run(l, c));
+    }
+
+    // cell c of the list: its run from the run map (one load) once the list
+    // is resolved, else by bisection
+    __host__ __device__ __forceinline__ Cell locate(const List& l, int64_t c)
+    {
+        const int64_t lo = l.cellrun ? int64_t(l.cellrun[c]) : find_run(l, c);
         Cell cell;
         cell.run   = lo;
         cell.x     = l.iv[5 * lo + 0] + int32_t(c - l.pref[lo]);
@@ -280,9 +295,8 @@ namespace smr::dev
 
     // projection_op_ (numeric/projection.hpp:15-65): taps dst, then the 2^(dim-1) fine rows
     template <int DIM>
-    __host__ __device__ __forceinline__ void op_project(const List& l, int64_t c, double* dst, const double* src)
+    __host__ __device__ __forceinline__ void op_project_at(const List& l, int64_t c, const Cell& cell, double* dst, const double* src)
     {
-        const Cell cell  = locate(l, c);
         const int64_t* b = bases(l, cell.run);
         constexpr int NR = 1 << (DIM - 1);
         constexpr double inv = 1.0 / double(1 << DIM);
@@ -296,12 +310,17 @@ namespace smr::dev
         dst[b[0] + cell.x] = sum * inv;
     }
 
+    template <int DIM>
+    __host__ __device__ __forceinline__ void op_project(const List& l, int64_t c, double* dst, const double* src)
+    {
+        op_project_at<DIM>(l, c, locate(l, c), dst, src);
+    }
+
     // prediction_op <s,false> (numeric/prediction.hpp:259-357): taps dst, then
     // the coarse rows (z outer, y inner) starting at x = (x0>>1) - s
     template <int DIM, int S>
-    __host__ __device__ __forceinline__ void op_predict_fine(const List& l, int64_t c, double* u)
+    __host__ __device__ __forceinline__ void op_predict_fine_at(const List& l, int64_t c, const Cell& cell, double* u)
     {
-        const Cell cell  = locate(l, c);
         const int64_t* b = bases(l, cell.run);
         const int32_t xc = cell.x >> 1;
         double v;
@@ -340,12 +359,17 @@ namespace smr::dev
         u[b[0] + cell.x] = v;
     }
 
+    template <int DIM, int S>
+    __host__ __device__ __forceinline__ void op_predict_fine(const List& l, int64_t c, double* u)
+    {
+        op_predict_fine_at<DIM, S>(l, c, locate(l, c), u);
+    }
+
     // compute_detail_op (mr/operators.hpp:139-540): taps = the 2^(dim-1) fine
     // rows, then the coarse stencil rows (z outer, y inner)
     template <int DIM, int S>
-    __host__ __device__ __forceinline__ void op_detail(const List& l, int64_t c, const double* u, double* det)
+    __host__ __device__ __forceinline__ void op_detail_at(const List& l, int64_t c, const Cell& cell, const double* u, double* det)
     {
-        const Cell cell  = locate(l, c);
         const int64_t* b = bases(l, cell.run);
         constexpr int NR = 1 << (DIM - 1);
         constexpr int NC = 1 << DIM;
@@ -402,6 +426,12 @@ namespace smr::dev
         }
     }
 
+    template <int DIM, int S>
+    __host__ __device__ __forceinline__ void op_detail(const List& l, int64_t c, const double* u, double* det)
+    {
+        op_detail_at<DIM, S>(l, c, locate(l, c), u, det);
+    }
+
     // mr_criteria_op (mr/criteria.hpp:13-122): taps coarse, then fine rows;
     // thr holds eps, fine and coarse thresholds per fine level (3 x 21).  The
     // details of ncomp components (one per adapted field) are stored one array
@@ -410,10 +440,9 @@ namespace smr::dev
     // child at most eps; a child is refined when any of its components is
     // above fine_eps (criteria.hpp:47-56, 63-72, 96-105)
     template <int DIM>
-    __host__ __device__ __forceinline__ void op_criteria(const List& l, int64_t c, const double* det, uint8_t* tag, const double* thr, int min_level,
+    __host__ __device__ __forceinline__ void op_criteria_at(const List& l, int64_t c, const Cell& cell, const double* det, uint8_t* tag, const double* thr, int min_level,
                                                 int max_level, int ncomp = 1, int64_t cstride = 0)
     {
-        const Cell cell  = locate(l, c);
         const int64_t* b = bases(l, cell.run);
         constexpr int NC = 1 << DIM;
         const int fl     = cell.level + 1;
@@ -468,11 +497,17 @@ namespace smr::dev
         }
     }
 
+    template <int DIM>
+    __host__ __device__ __forceinline__ void op_criteria(const List& l, int64_t c, const double* det, uint8_t* tag, const double* thr, int min_level,
+                                                int max_level, int ncomp = 1, int64_t cstride = 0)
+    {
+        op_criteria_at<DIM>(l, c, locate(l, c), det, tag, thr, min_level, max_level, ncomp, cstride);
+    }
+
     // maximum_op (mr/operators.hpp:22-96): taps coarse, then fine rows
     template <int DIM>
-    __host__ __device__ __forceinline__ void op_maximum(const List& l, int64_t c, uint8_t* tag)
+    __host__ __device__ __forceinline__ void op_maximum_at(const List& l, int64_t c, const Cell& cell, uint8_t* tag)
     {
-        const Cell cell  = locate(l, c);
         const int64_t* b = bases(l, cell.run);
         constexpr int NC = 1 << DIM;
         const int32_t x  = cell.x;
@@ -517,10 +552,20 @@ namespace smr::dev
         }
     }
 
+    template <int DIM>
+    __host__ __device__ __forceinline__ void op_maximum(const List& l, int64_t c, uint8_t* tag)
+    {
+        op_maximum_at<DIM>(l, c, locate(l, c), tag);
+    }
+
+    __host__ __device__ __forceinline__ void op_tag_init_at(const List& l, int64_t c, const Cell& cell, uint8_t* tag)
+    {
+        tag[bases(l, cell.run)[0] + cell.x] = 1;
+    }
+
     __host__ __device__ __forceinline__ void op_tag_init(const List& l, int64_t c, uint8_t* tag)
     {
-        const Cell cell = locate(l, c);
-        tag[bases(l, cell.run)[0] + cell.x] = 1;
+        op_tag_init_at(l, c, locate(l, c), tag);
     }
 
     // README.md:135-154 hand loop over for_each_interval (2D, a = (1, 1)):
@@ -536,18 +581,31 @@ namespace smr::dev
     // true when the tag of a cell changes the cell array: a refine below
     // max_level or a coarsen without keep above min_level
     // (update_cell_array_from_tag, graduation.hpp:743-843)
-    __host__ __device__ __forceinline__ bool tag_changes_mesh(const List& l, int64_t c, const uint8_t* tag, int min_level, int max_level)
+    __host__ __device__ __forceinline__ bool tag_changes_mesh_at(const List& l, int64_t c, const Cell& cell, const uint8_t* tag, int min_level, int max_level)
     {
-        const Cell cell = locate(l, c);
         const uint8_t t = tag[bases(l, cell.run)[0] + cell.x];
         const bool ref  = (t & 4) && cell.level < max_level;
         const bool crs  = (t & 2) && !(t & 1) && cell.level > min_level;
         return ref || crs;
     }
 
+    __host__ __device__ __forceinline__ bool tag_changes_mesh(const List& l, int64_t c, const uint8_t* tag, int min_level, int max_level)
+    {
+        return tag_changes_mesh_at(l, c, locate(l, c), tag, min_level, max_level);
+    }
+
     // flag[0] = 1 when the cell's tag changes the cell array.  Every lane that
     // sees a change stores: in a stage program one wave can hold cells of
     // several segments (instances), each with its own flag.
+    __host__ __device__ __forceinline__ void op_tag_change_at(const List& l, int64_t c, const Cell& cell, const uint8_t* tag, int min_level,
+                                                     int max_level, int* flag)
+    {
+        if (tag_changes_mesh_at(l, c, cell, tag, min_level, max_level))
+        {
+            flag[0] = 1;
+        }
+    }
+
     __host__ __device__ __forceinline__ void op_tag_change(const List& l, int64_t c, const uint8_t* tag, int min_level, int max_level, int* flag)
     {
         if (tag_changes_mesh(l, c, tag, min_level, max_level))
@@ -556,19 +614,22 @@ namespace smr::dev
         }
     }
 
-    __host__ __device__ __forceinline__ void op_copy(const List& l, int64_t c, double* dst, const double* src)
+    __host__ __device__ __forceinline__ void op_copy_at(const List& l, int64_t c, const Cell& cell, double* dst, const double* src)
     {
-        const Cell cell  = locate(l, c);
         const int64_t* b = bases(l, cell.run);
         dst[b[0] + cell.x] = src[b[1] + cell.x];
+    }
+
+    __host__ __device__ __forceinline__ void op_copy(const List& l, int64_t c, double* dst, const double* src)
+    {
+        op_copy_at(l, c, locate(l, c), dst, src);
     }
 
     // prediction_op <s,true> (numeric/prediction.hpp:107-257): taps = the
     // 2^(dim-1) destination rows (children), then the source stencil rows
     template <int DIM, int S>
-    __host__ __device__ __forceinline__ void op_predict_children(const List& l, int64_t c, double* dst, const double* src)
+    __host__ __device__ __forceinline__ void op_predict_children_at(const List& l, int64_t c, const Cell& cell, double* dst, const double* src)
     {
-        const Cell cell  = locate(l, c);
         const int64_t* b = bases(l, cell.run);
         constexpr int NR = 1 << (DIM - 1);
         constexpr int NC = 1 << DIM;
@@ -625,6 +686,12 @@ namespace smr::dev
         }
     }
 
+    template <int DIM, int S>
+    __host__ __device__ __forceinline__ void op_predict_children(const List& l, int64_t c, double* dst, const double* src)
+    {
+        op_predict_children_at<DIM, S>(l, c, locate(l, c), dst, src);
+    }
+
     // one outer-ghost recipe: copy, Dirichlet<1> (bc/dirichlet.hpp:29), or the BC
     // projection average (algorithm/update_outer_ghost.hpp:78-131)
     __host__ __device__ __forceinline__ void op_recipe(const int64_t* __restrict__ dst, const int32_t* __restrict__ op, const int64_t* __restrict__ sp,
@@ -676,10 +743,9 @@ namespace smr::dev
     }
 
     template <int DIM>
-    __host__ __device__ __forceinline__ void op_adj_upwind(const List& l, int64_t c, const double* lam, double* mu, const double* un, double* ga,
+    __host__ __device__ __forceinline__ void op_adj_upwind_at(const List& l, int64_t c, const Cell& cell, const double* lam, double* mu, const double* un, double* ga,
                                                   double a0, double a1, double a2, double dt)
     {
-        const Cell cell  = locate(l, c);
         const int64_t* b = bases(l, cell.run);
         const int32_t x  = cell.x;
         const double k   = dt * ldexp(1.0, cell.level);
@@ -707,12 +773,18 @@ namespace smr::dev
         }
     }
 
+    template <int DIM>
+    __host__ __device__ __forceinline__ void op_adj_upwind(const List& l, int64_t c, const double* lam, double* mu, const double* un, double* ga,
+                                                  double a0, double a1, double a2, double dt)
+    {
+        op_adj_upwind_at<DIM>(l, c, locate(l, c), lam, mu, un, ga, a0, a1, a2, dt);
+    }
+
     // dir: 0 x-1, 1 x+1, 2 y-1, 3 y+1, 4 z-1, 5 z+1 (the neighbour is a ghost)
     template <int DIM>
-    __host__ __device__ __forceinline__ void op_adj_scatter(const List& l, int64_t c, int dir, const double* lam, double* mu, double a0, double a1,
+    __host__ __device__ __forceinline__ void op_adj_scatter_at(const List& l, int64_t c, const Cell& cell, int dir, const double* lam, double* mu, double a0, double a1,
                                                    double a2, double dt)
     {
-        const Cell cell  = locate(l, c);
         const int64_t* b = bases(l, cell.run);
         const int32_t x  = cell.x;
         const double k   = dt * ldexp(1.0, cell.level);
@@ -729,6 +801,13 @@ namespace smr::dev
         }
     }
 
+    template <int DIM>
+    __host__ __device__ __forceinline__ void op_adj_scatter(const List& l, int64_t c, int dir, const double* lam, double* mu, double a0, double a1,
+                                                   double a2, double dt)
+    {
+        op_adj_scatter_at<DIM>(l, c, locate(l, c), dir, lam, mu, a0, a1, a2, dt);
+    }
+
     __host__ __device__ __forceinline__ void op_adj_gather(const int64_t* __restrict__ tgt, const int64_t* __restrict__ ptr,
                                                   const int64_t* __restrict__ src, const double* __restrict__ w, int64_t r, double* mu)
     {
@@ -740,15 +819,23 @@ namespace smr::dev
         mu[tgt[r]] += acc;
     }
 
+    __host__ __device__ __forceinline__ void op_cells_to_compact_at(const List& l, int64_t c, const Cell& cell, double* out, const double* u)
+    {
+        out[c]          = u[bases(l, cell.run)[0] + cell.x];
+    }
+
     __host__ __device__ __forceinline__ void op_cells_to_compact(const List& l, int64_t c, double* out, const double* u)
     {
-        const Cell cell = locate(l, c);
-        out[c]          = u[bases(l, cell.run)[0] + cell.x];
+        op_cells_to_compact_at(l, c, locate(l, c), out, u);
+    }
+
+    __host__ __device__ __forceinline__ void op_cells_from_compact_at(const List& l, int64_t c, const Cell& cell, double* u, const double* in)
+    {
+        u[bases(l, cell.run)[0] + cell.x] = in[c];
     }
 
     __host__ __device__ __forceinline__ void op_cells_from_compact(const List& l, int64_t c, double* u, const double* in)
     {
-        const Cell cell                        = locate(l, c);
-        u[bases(l, cell.run)[0] + cell.x] = in[c];
+        op_cells_from_compact_at(l, c, locate(l, c), u, in);
     }
 } // namespace smr::dev

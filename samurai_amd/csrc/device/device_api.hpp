@@ -28,6 +28,7 @@ namespace smr::dev
         const int64_t* pref;
         const int32_t* hint; // run holding cell 256*j, j = 0..ceil(ncells/256)
         int64_t* base;       // nruns * ntaps
+        int32_t* cellrun;    // the run of every cell, filled by the resolve (null: cells are searched for)
         const int8_t* taps;  // 7 per tap: table, dl, dx, dy, dz, wlo, whi
         const int32_t* tiles; // (run, x0) of every 64-cell tile of a run, runs in order
         // column walks (2D cells list): (first run, x0, number of runs n, first
@@ -115,6 +116,16 @@ namespace smr::dev
         SEG_WALK_ROWS = 22
     };
 
+    // operators whose index c is a cell of the segment's work list (the
+    // others index recipes, rows, runs x taps, walks or storage slots)
+    __host__ __device__ constexpr bool is_cell_list_op(int op)
+    {
+        return op == SEG_PROJECT || op == SEG_PREDICT_FINE || op == SEG_DETAIL || op == SEG_CRITERIA || op == SEG_MAXIMUM ||
+               op == SEG_TAG_INIT || op == SEG_UPWIND || op == SEG_COPY || op == SEG_PROJECT_TO || op == SEG_PREDICT_CHILDREN ||
+               op == SEG_CELLS_TO_COMPACT || op == SEG_CELLS_FROM_COMPACT || op == SEG_ADJ_UPWIND || op == SEG_ADJ_SCATTER ||
+               op == SEG_TAG_CHANGE;
+    }
+
     // One stage of a chain of small dependent stages run in order by a single
     // block (k_sequence): a range of outer-ghost recipes, of projection or
     // prediction ghosts, or of tag-finalization cells.  Launching each such
@@ -171,21 +182,52 @@ namespace smr::dev
     // (segment i covers [seg_pref[i], seg_pref[i+1]), nsegs + 1 entries).
     // Stages are separated by grid barriers (cooperative launch) or, in
     // stage-wise mode, by kernel boundaries.
+    // The work of one wavefront in a stage: n chunks of 64 consecutive
+    // indices of the segments [seg, end) (consecutive in the program) from
+    // index `off` of segment seg on, lanes stepping into the next segments as
+    // they pass their ends.  Segments of a stage are packed densely, and a
+    // wave knows its first segment without searching for it; n > 1 only
+    // within one segment (large fills and copies).
+    struct WaveItem
+    {
+        int32_t seg, off, end, n;
+    };
+
+    // operators cheap enough per index that a wave takes several chunks
+    __host__ __device__ constexpr bool is_streaming_op(int op)
+    {
+        return op == SEG_FILL_F64 || op == SEG_FILL_U8 || op == SEG_COPY_RAW;
+    }
+
     struct Program
     {
         const Segment* segs;
         const int64_t* seg_pref;
         const int32_t* stage_seg;
         int nstages;
+        // wave items of stage k: [stage_item[k], stage_item[k + 1])
+        const WaveItem* items;
+        const int64_t* stage_item;
     };
 
     // maximum co-resident blocks of the fused kernel for a block size
     int program_max_blocks(int block);
     // one cooperative launch; returns false if the runtime refused it
     bool run_program_fused(const Program& p, int block, int nblocks, int cpt, hipStream_t s);
-    // one launch per stage (capturable in a hipGraph); cells per stage given by the host
-    void run_program_stagewise(const Program& p, const std::vector<int64_t>& stage_cells, int block, int cpt, hipStream_t s);
+    // stage k alone, grid-wide, one wave item per wavefront
+    void run_program_stage(const Program& p, int k, int64_t nitems, hipStream_t s);
+    // stage k alone, grid-wide, cpt consecutive cells per thread in blocks of `block`
+    void run_program_stage_cells(const Program& p, int k, int64_t cells, int block, int cpt, hipStream_t s);
+    // one block of `threads` per chain, each running its chain's items
+    // items[off[b] .. off[b+1]): item = segment index << 1 | (block barrier after it)
+    void run_program_blocks(const Program& p, const int32_t* off, const int32_t* items, int nchains, int threads, hipStream_t s);
 
+    // threads of a resolve: the (run, tap) pairs, and the cells for the run map
+    inline int64_t resolve_range(const List& l)
+    {
+        const int64_t pairs = l.nruns * l.ntaps;
+        return l.cellrun && l.ncells > pairs ? l.ncells : pairs;
+    }
     void resolve(const List& l, const Table& t0, const Table& t1, int* err, hipStream_t s);
     // fills l.walk_rows from the resolved bases (after resolve; no-op without walks)
     void walk_rows(const List& l, hipStream_t s);

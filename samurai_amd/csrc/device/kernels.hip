@@ -13,12 +13,17 @@ namespace smr::dev
     {
         constexpr int BLOCK = 256;
 
+        // the (run, tap) bases and, in the same launch, the run map
         __global__ void k_resolve(List l, Table t0, Table t1, int* err)
         {
             const int64_t gid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
             if (gid < l.nruns * l.ntaps)
             {
                 op_resolve(l, gid, t0, t1, err);
+            }
+            if (l.cellrun && gid < l.ncells)
+            {
+                op_cellrun(l, gid);
             }
         }
 
@@ -709,7 +714,7 @@ namespace smr::dev
 
     void resolve(const List& l, const Table& t0, const Table& t1, int* err, hipStream_t s)
     {
-        const int64_t n = l.nruns * l.ntaps;
+        const int64_t n = resolve_range(l);
         if (n > 0)
         {
             hipLaunchKernelGGL(k_resolve, dim3(nblocks(n)), dim3(BLOCK), 0, s, l, t0, t1, err);

@@ -14,61 +14,73 @@ namespace smr::dev
 {
     namespace
     {
+        // a cell-list operator on cell c of the list, located at `cell`
         template <int DIM>
-        __device__ __forceinline__ void run_cell_dim(const Segment& g, int64_t c)
+        __device__ __forceinline__ void run_located_dim(const Segment& g, int64_t c, const Cell& cell)
         {
             switch (g.op)
             {
-                case SEG_PROJECT: op_project<DIM>(g.list, c, g.u, g.u); break;
+                case SEG_PROJECT: op_project_at<DIM>(g.list, c, cell, g.u, g.u); break;
                 case SEG_PREDICT_FINE:
                     if (g.pred_s == 0)
                     {
-                        op_predict_fine<DIM, 0>(g.list, c, g.u);
+                        op_predict_fine_at<DIM, 0>(g.list, c, cell, g.u);
                     }
                     else
                     {
-                        op_predict_fine<DIM, 1>(g.list, c, g.u);
+                        op_predict_fine_at<DIM, 1>(g.list, c, cell, g.u);
                     }
                     break;
                 case SEG_DETAIL:
                     if (g.pred_s == 0)
                     {
-                        op_detail<DIM, 0>(g.list, c, g.u, g.out);
+                        op_detail_at<DIM, 0>(g.list, c, cell, g.u, g.out);
                     }
                     else
                     {
-                        op_detail<DIM, 1>(g.list, c, g.u, g.out);
+                        op_detail_at<DIM, 1>(g.list, c, cell, g.u, g.out);
                     }
                     break;
-                case SEG_CRITERIA: op_criteria<DIM>(g.list, c, g.out, g.tag, g.thr, g.min_level, g.max_level); break;
-                case SEG_MAXIMUM: op_maximum<DIM>(g.list, c, g.tag); break;
-                case SEG_TAG_INIT: op_tag_init(g.list, c, g.tag); break;
-                case SEG_TAG_CHANGE: op_tag_change(g.list, c, g.tag, g.min_level, g.max_level, g.err); break;
-                case SEG_COPY: op_copy(g.list, c, g.out, g.u); break;
-                case SEG_PROJECT_TO: op_project<DIM>(g.list, c, g.out, g.u); break;
+                case SEG_CRITERIA: op_criteria_at<DIM>(g.list, c, cell, g.out, g.tag, g.thr, g.min_level, g.max_level); break;
+                case SEG_MAXIMUM: op_maximum_at<DIM>(g.list, c, cell, g.tag); break;
+                case SEG_TAG_INIT: op_tag_init_at(g.list, c, cell, g.tag); break;
+                case SEG_TAG_CHANGE: op_tag_change_at(g.list, c, cell, g.tag, g.min_level, g.max_level, g.err); break;
+                case SEG_COPY: op_copy_at(g.list, c, cell, g.out, g.u); break;
+                case SEG_PROJECT_TO: op_project_at<DIM>(g.list, c, cell, g.out, g.u); break;
                 case SEG_PREDICT_CHILDREN:
                     if (g.pred_s == 0)
                     {
-                        op_predict_children<DIM, 0>(g.list, c, g.out, g.u);
+                        op_predict_children_at<DIM, 0>(g.list, c, cell, g.out, g.u);
                     }
                     else
                     {
-                        op_predict_children<DIM, 1>(g.list, c, g.out, g.u);
+                        op_predict_children_at<DIM, 1>(g.list, c, cell, g.out, g.u);
                     }
                     break;
-                case SEG_CELLS_TO_COMPACT: op_cells_to_compact(g.list, c, g.out, g.u); break;
-                case SEG_CELLS_FROM_COMPACT: op_cells_from_compact(g.list, c, g.out, g.in2); break;
-                case SEG_ADJ_UPWIND: op_adj_upwind<DIM>(g.list, c, g.u, g.out, g.in2, g.out2, g.a0, g.a1, g.a2, g.dt); break;
-                case SEG_ADJ_SCATTER: op_adj_scatter<DIM>(g.list, c, g.pad, g.u, g.out, g.a0, g.a1, g.a2, g.dt); break;
+                case SEG_CELLS_TO_COMPACT: op_cells_to_compact_at(g.list, c, cell, g.out, g.u); break;
+                case SEG_CELLS_FROM_COMPACT: op_cells_from_compact_at(g.list, c, cell, g.out, g.in2); break;
+                case SEG_ADJ_UPWIND: op_adj_upwind_at<DIM>(g.list, c, cell, g.u, g.out, g.in2, g.out2, g.a0, g.a1, g.a2, g.dt); break;
+                case SEG_ADJ_SCATTER: op_adj_scatter_at<DIM>(g.list, c, cell, g.pad, g.u, g.out, g.a0, g.a1, g.a2, g.dt); break;
                 case SEG_UPWIND:
                 {
-                    const Cell cell  = locate(g.list, c);
                     const int64_t* b = bases(g.list, cell.run);
                     g.out[b[0] + cell.x] = upwind_value<DIM>(b, cell.x, cell.level, g.u, .5 * g.a0, .5 * fabs(g.a0), .5 * g.a1,
                                                              .5 * fabs(g.a1), .5 * g.a2, .5 * fabs(g.a2), g.dt);
                     break;
                 }
                 default: break;
+            }
+        }
+
+        __device__ __forceinline__ void run_located(const Segment& g, int64_t c, const Cell& cell)
+        {
+            if (g.dim == 2)
+            {
+                run_located_dim<2>(g, c, cell);
+            }
+            else
+            {
+                run_located_dim<3>(g, c, cell);
             }
         }
 
@@ -80,7 +92,14 @@ namespace smr::dev
             }
             else if (g.op == SEG_RESOLVE)
             {
-                op_resolve(g.list, c, g.tabs[0], g.tabs[1], g.err);
+                if (c < g.list.nruns * g.list.ntaps)
+                {
+                    op_resolve(g.list, c, g.tabs[0], g.tabs[1], g.err);
+                }
+                if (g.list.cellrun && c < g.list.ncells)
+                {
+                    op_cellrun(g.list, c);
+                }
             }
             else if (g.op == SEG_FILL_F64)
             {
@@ -106,14 +125,42 @@ namespace smr::dev
             {
                 g.out[g.rdst[c]] = 0.0;
             }
-            else if (g.dim == 2)
+            else if (is_cell_list_op(g.op))
             {
-                run_cell_dim<2>(g, c);
+                run_located(g, c, locate(g.list, c));
             }
-            else
+        }
+
+        // wave item `it` of a program: 64 consecutive indices of the segments
+        // [it.seg, it.end) from index it.off of segment it.seg on
+        __device__ __forceinline__ void run_item(const Program& p, WaveItem it, int lane)
+        {
+            if (it.n > 1)
             {
-                run_cell_dim<3>(g, c);
+                // chunks of one segment of a streaming operator
+                const Segment& g = p.segs[it.seg];
+                const int64_t c  = g.c0 + it.off + lane;
+                for (int q = 0; q < it.n; ++q)
+                {
+                    if (c + 64 * q < g.c1)
+                    {
+                        run_cell(g, c + 64 * q);
+                    }
+                }
+                return;
             }
+            const int64_t pos = p.seg_pref[it.seg] + it.off + lane;
+            if (pos >= p.seg_pref[it.end])
+            {
+                return;
+            }
+            int32_t sg = it.seg;
+            while (p.seg_pref[sg + 1] <= pos)
+            {
+                ++sg;
+            }
+            const Segment& g = p.segs[sg];
+            run_cell(g, g.c0 + (pos - p.seg_pref[sg]));
         }
 
         // the last segment s of [sb, se) with pref[s] - base <= w (w uniform
@@ -186,13 +233,75 @@ namespace smr::dev
             }
         }
 
-        __global__ void k_program_stage(Program p, int k, int cpt)
+        // stage k, cpt consecutive cells per thread, each wave searching for
+        // its first segment
+        __global__ void k_program_stage_cells(Program p, int k, int cpt)
         {
             const int64_t tid      = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
             const int64_t nthreads = int64_t(gridDim.x) * blockDim.x;
             run_stage(p, k, tid, nthreads, cpt);
         }
+
+        // stage k, one wave item per wavefront (blocks of 256 threads)
+        __global__ __launch_bounds__(256) void k_program_stage(Program p, int k)
+        {
+            const int64_t w = int64_t(blockIdx.x) * 4 + __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+            const int64_t i = p.stage_item[k] + w;
+            if (i >= p.stage_item[k + 1])
+            {
+                return;
+            }
+            run_item(p, p.items[i], int(threadIdx.x & 63));
+        }
+
+        // One block per chain: block b runs the segments items[off[b] ..
+        // off[b+1]) in order, all its threads on one segment at a time, with
+        // a block barrier after the items whose low bit is set (the last
+        // segment of the chain in a stage).  The item list is uniform over
+        // the block, so the segment descriptors are scalar loads and every
+        // barrier is reached by all threads.  Chains must not touch each
+        // other's data: blocks are not ordered among themselves.
+        __global__ __launch_bounds__(1024) void k_program_blocks(Program p, const int32_t* __restrict__ off,
+                                                                 const int32_t* __restrict__ items)
+        {
+            const int32_t t0   = off[blockIdx.x], t1 = off[blockIdx.x + 1];
+            const int wave     = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+            const int nwaves   = int(blockDim.x >> 6);
+            const int lane     = int(threadIdx.x & 63);
+            for (int32_t t = t0; t < t1; ++t)
+            {
+                const int32_t v  = items[t];
+                const int32_t sg = v >> 1;
+                const Segment& g = p.segs[sg];
+                const int32_t n  = int32_t(g.c1 - g.c0);
+                for (int32_t q = wave * 64; q < n; q += nwaves * 64)
+                {
+                    run_item(p, WaveItem{sg, q, sg + 1, 1}, lane);
+                }
+                if (v & 1)
+                {
+                    __syncthreads();
+                }
+            }
+        }
     } // namespace
+
+    void run_program_blocks(const Program& p, const int32_t* off, const int32_t* items, int nchains, int threads, hipStream_t s)
+    {
+        if (nchains > 0)
+        {
+            hipLaunchKernelGGL(k_program_blocks, dim3(unsigned(nchains)), dim3(unsigned(threads)), 0, s, p, off, items);
+        }
+    }
+
+    void run_program_stage(const Program& p, int k, int64_t nitems, hipStream_t s)
+    {
+        if (nitems <= 0)
+        {
+            return;
+        }
+        hipLaunchKernelGGL(k_program_stage, dim3(unsigned((nitems + 3) / 4)), dim3(256), 0, s, p, k);
+    }
 
     int program_max_blocks(int block)
     {
@@ -222,18 +331,14 @@ namespace smr::dev
                hipSuccess;
     }
 
-    void run_program_stagewise(const Program& p, const std::vector<int64_t>& stage_cells, int block, int cpt, hipStream_t s)
+    void run_program_stage_cells(const Program& p, int k, int64_t cells, int block, int cpt, hipStream_t s)
     {
-        for (int k = 0; k < p.nstages; ++k)
+        if (cells <= 0)
         {
-            const int64_t n = stage_cells[size_t(k)];
-            if (n <= 0)
-            {
-                continue;
-            }
-            const int64_t threads = (n + cpt - 1) / cpt;
-            const unsigned grid   = unsigned((threads + block - 1) / block);
-            hipLaunchKernelGGL(k_program_stage, dim3(grid), dim3(block), 0, s, p, k, cpt);
+            return;
         }
+        const int64_t threads = (cells + cpt - 1) / cpt;
+        const unsigned grid   = unsigned((threads + block - 1) / block);
+        hipLaunchKernelGGL(k_program_stage_cells, dim3(grid), dim3(block), 0, s, p, k, cpt);
     }
 } // namespace smr::dev

@@ -548,6 +548,7 @@ namespace smr
         const size_t owk  = b_.add(walks.data(), walks.size() * 4);
         const size_t owr  = b_.scratch(size_t(nrows_total) * 8);
         const size_t obas = b_.scratch(size_t(nruns) * w.taps.size() * 8);
+        const size_t ocr  = b_.scratch(size_t(ncells) * 4);
         const int64_t ntiles = int64_t(tiles.size() / 2);
         const int64_t nwalks = int64_t(walks.size() / 4);
         const int ntaps      = int(w.taps.size());
@@ -566,6 +567,7 @@ namespace smr
                 l.d.walk_stride = walk_stride;
                 l.d.nwalks = nwalks;
                 l.d.base   = reinterpret_cast<int64_t*>(b->sptr(obas));
+                l.d.cellrun = ncells > 0 ? reinterpret_cast<int32_t*>(b->sptr(ocr)) : nullptr;
                 l.d.ntiles = ntiles;
                 l.d.nruns  = nruns;
                 l.d.ncells = ncells;

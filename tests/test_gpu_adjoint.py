@@ -157,14 +157,16 @@ def test_checkpoint_interval_does_not_change_gradients(gpu_ctx):
     assert np.array_equal(a1, a2)
 
 
-@pytest.mark.parametrize("mode", [1, 2])
-def test_adjoint_launch_modes_agree(gpu_ctx, mode):
+# (block, cells per thread, mode): the default is (256, 0, 3), one block per
+# instance for runs of small stages and wave items for the others
+@pytest.mark.parametrize("shape", [(128, 2, 1), (128, 2, 2), (256, 0, 1), (256, 0, 2), (512, 4, 3), (256, 1, 0)])
+def test_adjoint_launch_modes_agree(gpu_ctx, shape):
     import samurai_amd as sa
     import torch
     inst, p, u0, area, target = setup(sa, gpu_ctx, [93, 94, 95], (4, 6), steps=6, every=3)
     uT0, g00, ga0 = gradients(torch, p, u0, area, target)
-    p.set_tuning(128, 2, mode)
-    for rep in range(2):  # mode 2: the second pass replays the captured graphs
+    p.set_tuning(*shape)
+    for rep in range(2):  # modes 2 and 3: the second pass replays the captured graphs
         uT, g0, ga = gradients(torch, p, u0, area, target)
         assert np.array_equal(uT, uT0) and np.array_equal(g0, g00) and np.array_equal(ga, ga0), rep
 

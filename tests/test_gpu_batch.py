@@ -54,12 +54,13 @@ def setup_pair(sa, ctx, seeds, levels):
     return plain, bat, batch
 
 
-@pytest.mark.parametrize("mode", [0, 1, 2])
-def test_batch_steps_match_plain(gpu_ctx, mode):
+# (cells per thread, mode); cells per thread 0: precomputed wave items
+@pytest.mark.parametrize("cpt,mode", [(1, 0), (4, 1), (1, 2), (0, 1), (0, 2)])
+def test_batch_steps_match_plain(gpu_ctx, cpt, mode):
     import samurai_amd as sa
     seeds = [11, 12, 13, 14, 15, 16]
     plain, bat, batch = setup_pair(sa, gpu_ctx, seeds, (4, 7))
-    batch.set_tuning(256, 1 if mode != 1 else 4, mode)
+    batch.set_tuning(256, cpt, mode)
     for step in range(4):
         for x in plain:
             plain_step(sa, x)
