@@ -963,7 +963,7 @@ namespace smr
                                      r.converged = true;
                                      return;
                                  }
-                                 r.nm           = build_mesh(c, ca);
+                                 r.nm           = build_mesh(c, std::move(ca));
                                  r.nm.graduated = true;
                                  r.img = build_host_image(r.nm);
                                  build_update_fields_lists(m.host, r.nm, r.copy, r.coarsen, r.refine);

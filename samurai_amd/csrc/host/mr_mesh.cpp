@@ -166,79 +166,94 @@ namespace smr
             }
         }
 
-        // MRMesh::update_sub_mesh_impl (mr/mesh.hpp:221-455), serial and non-periodic
+        // MRMesh::update_sub_mesh_impl (mr/mesh.hpp:221-455), non-periodic, with
+        // Mesh_base::construct_union (mesh.hpp:1231-1262).  The level sets are
+        // built as a task graph: the union chain runs beside the per-level
+        // ghost expansions; every level of the reference before the
+        // projection ghosts is cells_and_ghosts[t] U below1[t+1] U below2[t+2]
+        // (the prediction ghosts one and two levels below the cells); the
+        // projection ghosts are added level by level, each level depending on
+        // the one below.
         void update_sub_mesh(MRMeshData& m)
         {
             const int r = m.cfg.radius, s = m.cfg.pred_s, dim = m.cfg.dim;
-            CellArray cell_list = empty_cell_array(dim);
-            parallel_for(MAXL,
-                         [&](int l)
+            const bool mr = m.cfg.max_level != m.cfg.min_level;
+            const CellArray& cells = m.ids[CELLS];
+            CellArray cag = empty_cell_array(dim), below1 = empty_cell_array(dim), below2 = empty_cell_array(dim);
+            m.ids[UNION_CELLS] = empty_cell_array(dim);
+            // phase 1: the union chain | cells_and_ghosts[l] | below2[l]
+            parallel_for(1 + 2 * MAXL,
+                         [&](int task)
                          {
-                             if (!m.ids[CELLS][size_t(l)].empty())
+                             if (task == 0)
                              {
-                                 cell_list[size_t(l)] = expand(m.ids[CELLS][size_t(l)], r);
+                                 construct_union(m);
+                                 return;
+                             }
+                             const int level    = (task - 1) / 2;
+                             const LevelSet& cl = cells[size_t(level)];
+                             if (cl.empty())
+                             {
+                                 return;
+                             }
+                             if ((task - 1) % 2 == 0)
+                             {
+                                 cag[size_t(level)] = expand(cl, r);
+                             }
+                             else if (mr && level - 1 > 0)
+                             {
+                                 below2[size_t(level)] = expand(on(cl, level - 2), s);
                              }
                          });
-            m.ids[CELLS_AND_GHOSTS] = cell_list;
-            m.ids[PROJ_CELLS]       = empty_cell_array(dim);
-            if (m.cfg.max_level != m.cfg.min_level)
+            if (mr)
             {
-                // ghosts for the prediction one and two levels below each level
-                // with cells; the sets are built per level in parallel and merged
-                // into the cell list in level order
-                CellArray below1 = empty_cell_array(dim), below2 = empty_cell_array(dim);
-                parallel_for(2 * MAXL,
-                             [&](int task)
+                // phase 2: below1[l] = expand(cells_and_ghosts[l] on l-1, s) within
+                // the box [-s, 2^(l-1) + s)
+                parallel_for(MAXL,
+                             [&](int level)
                              {
-                                 const int level    = task / 2;
-                                 const LevelSet& cl = m.ids[CELLS][size_t(level)];
-                                 if (level < 1 || cl.empty())
+                                 if (level < 1 || cells[size_t(level)].empty())
                                  {
                                      return;
                                  }
-                                 if (task % 2 == 0)
-                                 {
-                                     // n expand(domain(level - 1), s): the box [-s, 2^(level-1) + s)
-                                     const int n           = 1 << (level - 1);
-                                     below1[size_t(level)] = clip_box(expand(on(m.ids[CELLS_AND_GHOSTS][size_t(level)], level - 1), s),
-                                                                      {-s, -s, -s}, {n + s, n + s, n + s});
-                                 }
-                                 else if (level - 1 > 0)
-                                 {
-                                     below2[size_t(level)] = expand(on(cl, level - 2), s);
-                                 }
+                                 const int n           = 1 << (level - 1);
+                                 below1[size_t(level)] = clip_box(expand(on(cag[size_t(level)], level - 1), s), {-s, -s, -s},
+                                                                  {n + s, n + s, n + s});
                              });
-                for (int level = 1; level < MAXL; ++level)
+            }
+            CellArray ref = empty_cell_array(dim);
+            // phase 3: the reference before the projection ghosts, per level
+            parallel_for(MAXL,
+                         [&](int t)
+                         {
+                             LevelSet x = cag[size_t(t)];
+                             if (mr && t + 1 < MAXL && !below1[size_t(t + 1)].empty())
+                             {
+                                 x = set_union(x, below1[size_t(t + 1)]);
+                             }
+                             if (mr && t + 2 < MAXL && !below2[size_t(t + 2)].empty())
+                             {
+                                 x = set_union(x, below2[size_t(t + 2)]);
+                             }
+                             ref[size_t(t)] = std::move(x);
+                         });
+            m.ids[CELLS_AND_GHOSTS] = std::move(cag);
+            m.ids[PROJ_CELLS]       = empty_cell_array(dim);
+            if (mr)
+            {
+                // phase 4: projection ghosts, level by level on the growing reference
+                for (int level = 0; level < MAXL - 1 && level <= ca_max_level(ref); ++level)
                 {
-                    if (m.ids[CELLS][size_t(level)].empty())
+                    if (ref[size_t(level)].empty())
                     {
                         continue;
                     }
-                    cell_list[size_t(level - 1)] = set_union(cell_list[size_t(level - 1)], below1[size_t(level)]);
-                    if (level - 1 > 0)
-                    {
-                        cell_list[size_t(level - 2)] = set_union(cell_list[size_t(level - 2)], below2[size_t(level)]);
-                    }
-                }
-                m.ids[REFERENCE] = cell_list;
-                // projection ghosts, level by level on the growing reference
-                for (int level = 0; level < MAXL - 1; ++level)
-                {
-                    if (level > ca_max_level(m.ids[REFERENCE]))
-                    {
-                        break;
-                    }
-                    if (m.ids[REFERENCE][size_t(level)].empty())
-                    {
-                        continue;
-                    }
-                    LevelSet expr = set_intersection(m.ids[REFERENCE][size_t(level)], m.ids[UNION_CELLS][size_t(level)]);
-                    cell_list[size_t(level + 1)] = set_union(cell_list[size_t(level + 1)], on(expr, level + 1));
-                    m.ids[REFERENCE][size_t(level + 1)] = cell_list[size_t(level + 1)];
-                    m.ids[PROJ_CELLS][size_t(level)]    = expr;
+                    LevelSet expr = set_intersection(ref[size_t(level)], m.ids[UNION_CELLS][size_t(level)]);
+                    ref[size_t(level + 1)]           = set_union(ref[size_t(level + 1)], on(expr, level + 1));
+                    m.ids[PROJ_CELLS][size_t(level)] = std::move(expr);
                 }
             }
-            m.ids[REFERENCE] = cell_list;
+            m.ids[REFERENCE] = std::move(ref);
         }
 
         // Mesh_base::construct_corners (mesh.hpp:913-997).  The corners depend on
@@ -307,6 +322,11 @@ namespace smr
 
     MRMeshData build_mesh(const MeshConfig& cfg, const CellArray& cells)
     {
+        return build_mesh(cfg, CellArray(cells));
+    }
+
+    MRMeshData build_mesh(const MeshConfig& cfg, CellArray&& cells)
+    {
         if (cfg.radius != 1)
         {
             throw std::invalid_argument("only a scheme stencil radius of 1 is supported");
@@ -329,11 +349,11 @@ namespace smr
         {
             v = empty_cell_array(cfg.dim);
         }
-        for (int l = 0; l < MAXL; ++l)
+        if (cells.size() != size_t(MAXL))
         {
-            m.ids[CELLS][size_t(l)] = cells[size_t(l)];
+            throw std::invalid_argument("a cell array holds one level set per level");
         }
-        construct_union(m);
+        m.ids[CELLS] = std::move(cells);
         update_sub_mesh(m);
         construct_corners(m);
         renumbering(m);

@@ -75,6 +75,7 @@ namespace smr
 
     // Mesh_base(ca, cfg) -> finalize_mesh (mesh.hpp:386-439, mr/mesh.hpp:221-455)
     MRMeshData build_mesh(const MeshConfig& cfg, const CellArray& cells);
+    MRMeshData build_mesh(const MeshConfig& cfg, CellArray&& cells);
     // mra::make_mesh(unit box, cfg): all cells at max_level (mr/mesh.hpp:510-518)
     MRMeshData build_box_mesh(const MeshConfig& cfg);
 

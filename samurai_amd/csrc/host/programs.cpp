@@ -397,53 +397,67 @@ namespace smr
             }
         }
 
-        void assemble_ghost(const MRMeshData& m, GhostProgram& p, ProgramSets& sets)
-        {
-            const int dim = m.cfg.dim, s = m.cfg.pred_s, maxl = m.cfg.max_level;
-            // projection (numeric/projection.hpp:15-65): runs at the coarse level
-            p.projection.taps.push_back({0, 0, 0, 0, 0, 0, 0});
-            add_row_taps(p.projection, 0, 1, 0, 0, 1, dim, 0, 1);
-            // prediction <s,false> (numeric/prediction.hpp:259-357)
-            p.prediction.taps.push_back({0, 0, 0, 0, 0, 0, 0});
-            add_row_taps(p.prediction, 0, -1, int8_t(-s), -s, s, dim, 0, int8_t(2 * s));
-            for (int level = 1; level <= maxl; ++level)
-            {
-                p.projection.add(sets.proj[size_t(level)]);
-                p.prediction.add(sets.pred[size_t(level)]);
-            }
-            p.projection.finish();
-            p.prediction.finish();
-        }
-
-        void assemble_adapt(const MRMeshData& m, AdaptProgram& p, ProgramSets& sets)
+        // the five work lists of the ghost and adapt programs from their
+        // per-level sets; the lists are independent and built in parallel
+        void assemble(const MRMeshData& m, GhostProgram* gp, AdaptProgram* ap, ProgramSets& sets)
         {
             const int dim = m.cfg.dim, s = m.cfg.pred_s;
             const int minl = m.cfg.min_level, maxl = m.cfg.max_level;
             const int l0   = minl > 0 ? minl - 1 : 0;
-            // compute_detail (mr/operators.hpp:139-540): fine rows, then coarse stencil rows
-            add_row_taps(p.detail, 0, 1, 0, 0, 1, dim, 0, 1);
-            add_row_taps(p.detail, 0, 0, int8_t(-s), -s, s, dim, 0, int8_t(2 * s));
-            for (int level = l0; level < maxl; ++level)
-            {
-                p.detail.add(sets.det[size_t(level)]);
-            }
-            p.detail.finish();
-            // mr_criteria / maximum (mr/criteria.hpp:13-122, mr/operators.hpp:22-96): the
-            // criteria sets are those of the fine levels [min, max], the maximum sets
-            // those of [1, max]; both are (cells[l] n reference[l-1]).on(l-1)
-            p.criteria.taps.push_back({0, 0, 0, 0, 0, 0, 0});
-            add_row_taps(p.criteria, 0, 1, 0, 0, 1, dim, 0, 1);
-            p.maximum.taps = p.criteria.taps;
-            for (int level = 1; level <= maxl; ++level)
-            {
-                if (level >= minl)
-                {
-                    p.criteria.add(sets.crit[size_t(level)]);
-                }
-                p.maximum.add(sets.crit[size_t(level)]);
-            }
-            p.criteria.finish();
-            p.maximum.finish();
+            parallel_for(5,
+                         [&](int task)
+                         {
+                             if (task == 0 && gp)
+                             {
+                                 // projection (numeric/projection.hpp:15-65): runs at the coarse level
+                                 WorkList& w = gp->projection;
+                                 w.taps.push_back({0, 0, 0, 0, 0, 0, 0});
+                                 add_row_taps(w, 0, 1, 0, 0, 1, dim, 0, 1);
+                                 for (int level = 1; level <= maxl; ++level)
+                                 {
+                                     w.add(sets.proj[size_t(level)]);
+                                 }
+                                 w.finish();
+                             }
+                             else if (task == 1 && gp)
+                             {
+                                 // prediction <s,false> (numeric/prediction.hpp:259-357)
+                                 WorkList& w = gp->prediction;
+                                 w.taps.push_back({0, 0, 0, 0, 0, 0, 0});
+                                 add_row_taps(w, 0, -1, int8_t(-s), -s, s, dim, 0, int8_t(2 * s));
+                                 for (int level = 1; level <= maxl; ++level)
+                                 {
+                                     w.add(sets.pred[size_t(level)]);
+                                 }
+                                 w.finish();
+                             }
+                             else if (task == 2 && ap)
+                             {
+                                 // compute_detail (mr/operators.hpp:139-540): fine rows, then coarse stencil rows
+                                 WorkList& w = ap->detail;
+                                 add_row_taps(w, 0, 1, 0, 0, 1, dim, 0, 1);
+                                 add_row_taps(w, 0, 0, int8_t(-s), -s, s, dim, 0, int8_t(2 * s));
+                                 for (int level = l0; level < maxl; ++level)
+                                 {
+                                     w.add(sets.det[size_t(level)]);
+                                 }
+                                 w.finish();
+                             }
+                             else if (task >= 3 && ap)
+                             {
+                                 // mr_criteria / maximum (mr/criteria.hpp:13-122, mr/operators.hpp:22-96):
+                                 // the criteria sets are those of the fine levels [min, max], the
+                                 // maximum sets those of [1, max]; both are (cells[l] n reference[l-1]).on(l-1)
+                                 WorkList& w = task == 3 ? ap->criteria : ap->maximum;
+                                 w.taps.push_back({0, 0, 0, 0, 0, 0, 0});
+                                 add_row_taps(w, 0, 1, 0, 0, 1, dim, 0, 1);
+                                 for (int level = task == 3 ? std::max(1, minl) : 1; level <= maxl; ++level)
+                                 {
+                                     w.add(sets.crit[size_t(level)]);
+                                 }
+                                 w.finish();
+                             }
+                         });
         }
 
         ProgramSets make_sets(const MRMeshData& m)
@@ -465,7 +479,7 @@ namespace smr
         p.outer.resize(size_t(m.cfg.max_level) + 1);
         ProgramSets sets = make_sets(m);
         parallel_for(ghost_tasks(m), [&](int task) { ghost_task(m, cart, task, p, sets); });
-        assemble_ghost(m, p, sets);
+        assemble(m, &p, nullptr, sets);
         return p;
     }
 
@@ -474,7 +488,7 @@ namespace smr
         AdaptProgram p;
         ProgramSets sets = make_sets(m);
         parallel_for(adapt_tasks(m), [&](int task) { adapt_task(m, task, sets); });
-        assemble_adapt(m, p, sets);
+        assemble(m, nullptr, &p, sets);
         return p;
     }
 
@@ -503,8 +517,7 @@ namespace smr
                              cells = build_cells_list(m);
                          }
                      });
-        assemble_ghost(m, ghost, sets);
-        assemble_adapt(m, adapt, sets);
+        assemble(m, &ghost, &adapt, sets);
     }
 
     WorkList build_cells_list(const MRMeshData& m)

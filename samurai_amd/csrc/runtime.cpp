@@ -1,5 +1,7 @@
 #include "runtime.hpp"
 
+#include "host/parallel.hpp"
+
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -225,14 +227,35 @@ namespace smr
 
     size_t Blob::add(const void* src, size_t n)
     {
-        const size_t off = (host_.size() + 255) & ~size_t(255);
-        host_.resize(off + std::max<size_t>(n, 8));
-        host_size_ = host_.size();
+        const size_t off = (host_size_ + 255) & ~size_t(255);
+        host_size_       = off + std::max<size_t>(n, 8);
         if (n)
         {
-            std::memcpy(host_.data() + off, src, n);
+            parts_.push_back({src, n, off});
         }
         return off;
+    }
+
+    void Blob::write_parts(char* dst)
+    {
+        auto one = [&](int i)
+        {
+            const Part& p = parts_[size_t(i)];
+            std::memcpy(dst + p.off, p.src, p.n);
+        };
+        if (host_size_ >= (size_t(1) << 20))
+        {
+            parallel_for(int(parts_.size()), one);
+        }
+        else
+        {
+            for (int i = 0; i < int(parts_.size()); ++i)
+            {
+                one(i);
+            }
+        }
+        std::vector<Part>().swap(parts_);
+        std::vector<std::shared_ptr<void>>().swap(keep_);
     }
 
     void Blob::upload(hipStream_t s)
@@ -253,21 +276,26 @@ namespace smr
 
     void Blob::copy()
     {
-        if (host_size_)
+        if (!host_size_)
         {
-            hip_check(hipMemcpyAsync(d_, staged_ ? staged_ : host_.data(), host_size_, hipMemcpyHostToDevice, stream_),
-                      "hipMemcpyAsync(blob)");
+            return;
         }
+        if (!staged_)
+        {
+            host_.reset(new char[host_size_]);
+            write_parts(host_.get());
+            staged_ = host_.get();
+        }
+        hip_check(hipMemcpyAsync(d_, staged_, host_size_, hipMemcpyHostToDevice, stream_), "hipMemcpyAsync(blob)");
     }
 
     void Blob::stage(char* dst)
     {
         if (host_size_)
         {
-            std::memcpy(dst, host_.data(), host_size_);
+            write_parts(dst);
         }
         staged_ = dst;
-        std::vector<char>().swap(host_);
     }
 
     PinnedStaging::~PinnedStaging()
@@ -326,6 +354,7 @@ namespace smr
         {
             (void)hipStreamSynchronize(stream);
         }
+        staging.reset();
         if (d_err)
         {
             (void)hipFree(d_err);
@@ -391,9 +420,10 @@ namespace smr
             }
         }
         row_off[MAXL] = int64_t(key.size());
-        const size_t o_key = b_.add(key.data(), key.size() * 8), o_off = b_.add(row_off.data(), row_off.size() * 8);
-        const size_t o_rp = b_.add(rp.data(), rp.size() * 8), o_rs = b_.add(rs.data(), rs.size() * 4);
-        const size_t o_re = b_.add(re.data(), re.size() * 4), o_idx = b_.add(idx.data(), idx.size() * 8);
+        // the blob keeps these vectors until it is staged
+        const size_t o_key = b_.add(std::move(key)), o_off = b_.add(std::move(row_off));
+        const size_t o_rp = b_.add(std::move(rp)), o_rs = b_.add(std::move(rs));
+        const size_t o_re = b_.add(std::move(re)), o_idx = b_.add(std::move(idx));
         Blob* b = &b_;
         binds_.push_back(
             [=, &t]
@@ -542,15 +572,16 @@ namespace smr
         const size_t oiv  = b_.add(w.iv.data(), w.iv.size() * 4);
         const size_t opf  = b_.add(w.pref.data(), w.pref.size() * 8);
         const size_t oh   = b_.add(l.hint.data(), l.hint.size() * 4);
-        const size_t ot   = b_.add(taps.data(), taps.size());
-        const size_t otl  = b_.add(tiles.data(), tiles.size() * 4);
-        const size_t onx  = b_.add(next.data(), next.size() * 4);
-        const size_t owk  = b_.add(walks.data(), walks.size() * 4);
+        const int64_t ntiles = int64_t(tiles.size() / 2);
+        const int64_t nwalks = int64_t(walks.size() / 4);
+        // the blob keeps the vectors built here until it is staged
+        const size_t ot   = b_.add(std::move(taps));
+        const size_t otl  = b_.add(std::move(tiles));
+        const size_t onx  = b_.add(std::move(next));
+        const size_t owk  = b_.add(std::move(walks));
         const size_t owr  = b_.scratch(size_t(nrows_total) * 8);
         const size_t obas = b_.scratch(size_t(nruns) * w.taps.size() * 8);
         const size_t ocr  = b_.scratch(size_t(ncells) * 4);
-        const int64_t ntiles = int64_t(tiles.size() / 2);
-        const int64_t nwalks = int64_t(walks.size() / 4);
         const int ntaps      = int(w.taps.size());
         Blob* b              = &b_;
         binds_.push_back(
@@ -605,12 +636,21 @@ namespace smr
         c1            = host.pref[size_t(r0 + nr)];
     }
 
+    // the blob's host arrays through the context's pinned staging, then its
+    // allocation, pointer binds and copy
+    void finish_staged(Ctx& ctx, Packer& p, Blob& b)
+    {
+        b.stage(ctx.staging->acquire(b.host_bytes()));
+        p.finish(ctx.stream);
+        ctx.staging->release(ctx.stream);
+    }
+
     void upload_list(Ctx& ctx, DevWorkList& l, const dev::Table& t0, const dev::Table& t1)
     {
         l.blob = std::make_unique<Blob>();
         Packer p(*l.blob);
         p.list(l);
-        p.finish(ctx.stream);
+        finish_staged(ctx, p, *l.blob);
         dev::resolve(l.d, t0, t1, ctx.d_err, ctx.stream);
         dev::walk_rows(l.d, ctx.stream);
     }
@@ -624,7 +664,7 @@ namespace smr
         p.table(m, dm->table);
         dm->cells.host = build_cells_list(m);
         p.list(dm->cells);
-        p.finish(ctx.stream);
+        finish_staged(ctx, p, *dm->table_blob);
         dev::resolve(dm->cells.d, dm->table, dm->table, ctx.d_err, ctx.stream);
         dev::walk_rows(dm->cells.d, ctx.stream);
         ctx.check_resolve();
@@ -671,7 +711,7 @@ namespace smr
         dm->table_blob = std::make_unique<Blob>();
         Packer p(*dm->table_blob);
         pack_image(p, *dm, m, std::move(h));
-        p.finish(ctx.stream);
+        finish_staged(ctx, p, *dm->table_blob);
         for (DevWorkList* l : dm->lists())
         {
             dev::resolve(l->d, dm->table, dm->table, ctx.d_err, ctx.stream);
@@ -1474,7 +1514,7 @@ namespace smr
                 return true;
             }
             ScopedTimer t3(nullptr, "mesh construction");
-            nm           = build_mesh(cfg, new_ca);
+            nm           = build_mesh(cfg, std::move(new_ca));
             nm.graduated = true;
         }
         {

@@ -20,6 +20,8 @@ namespace smr
     void hip_check(hipError_t e, const char* what);
 
     // One device allocation holding several host arrays (uploaded together).
+    // The arrays are laid out when added and copied once, when the blob is
+    // staged (into pinned memory) or copied from a host buffer of its own.
     class Blob
     {
       public:
@@ -28,8 +30,18 @@ namespace smr
         Blob(const Blob&)            = delete;
         Blob& operator=(const Blob&) = delete;
 
-        // reserves space for n bytes copied from src (256-byte aligned); returns the offset
+        // reserves space for n bytes of src (256-byte aligned) and returns the
+        // offset; src must stay valid and unchanged until stage() or copy()
         size_t add(const void* src, size_t n);
+        // the same for a vector the blob keeps until it is staged or copied
+        template <class T>
+        size_t add(std::vector<T>&& v)
+        {
+            auto kept        = std::make_shared<std::vector<T>>(std::move(v));
+            const size_t off = add(kept->data(), kept->size() * sizeof(T));
+            keep_.push_back(std::move(kept));
+            return off;
+        }
         // n bytes of device-only memory (no host copy, not uploaded, contents
         // undefined until a kernel writes them); returns an offset for sptr()
         size_t scratch(size_t n);
@@ -39,8 +51,8 @@ namespace smr
         // upload in two steps, to bind device pointers between them
         void alloc(hipStream_t s);
         void copy();
-        // moves the host part into dst (pinned memory of host_bytes() bytes that
-        // stays untouched until the copy has run); copy() then reads dst
+        // writes the host arrays into dst (pinned memory of host_bytes() bytes
+        // that stays untouched until the copy has run); copy() then reads dst
         void stage(char* dst);
         char* dptr(size_t off) const { return d_ + off; }
         char* sptr(size_t off) const { return d_ + scratch_base() + off; }
@@ -48,8 +60,16 @@ namespace smr
         size_t bytes() const { return scratch_base() + scratch_; }
 
       private:
+        struct Part
+        {
+            const void* src;
+            size_t n, off;
+        };
         size_t scratch_base() const { return (host_size_ + 255) & ~size_t(255); }
-        std::vector<char> host_;
+        void write_parts(char* dst);
+        std::vector<Part> parts_;
+        std::vector<std::shared_ptr<void>> keep_;
+        std::unique_ptr<char[]> host_; // when copied without staging
         size_t host_size_    = 0;
         const char* staged_  = nullptr;
         size_t scratch_      = 0;
@@ -257,6 +277,8 @@ namespace smr
         int device         = -1;
         hipStream_t stream = nullptr;
         int* d_err         = nullptr; // two ints: resolve error, then a scratch flag
+        // pinned host memory the mesh images are staged in before their copy
+        std::unique_ptr<PinnedStaging> staging = std::make_unique<PinnedStaging>();
         ~Ctx();
         int* d_flag() const { return d_err + 1; }
         bool has_device() const { return device >= 0; }
