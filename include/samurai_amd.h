@@ -103,11 +103,13 @@ int smr_mesh_get_cells(const smr_mesh* mesh, int mesh_id, int64_t* out);
  * unchanged), 0 if the mesh was rebuilt.  Host only: fields on the mesh must
  * be resized afterwards, their values are not transferred. */
 int smr_mesh_rebuild_from_tags(smr_mesh* mesh, const uint8_t* tags, int64_t n);
-/* smr_mesh_rebuild_from_tags with update_cell_array_from_tag and the
- * make_graduation fixed point run on the device (per-level bitmaps), from a
- * device array of tags; the host receives the new leaves only when they
- * change.  The path every harten call of smr_adapt_run takes.  Returns 1 at
- * the fixed point, 0 if the mesh was rebuilt. */
+/* smr_mesh_rebuild_from_tags with update_cell_array_from_tag, the
+ * make_graduation fixed point and the new mesh's ghost layers and union
+ * (MRMesh::update_sub_mesh_impl, mr/mesh.hpp:221-455) run on the device
+ * (per-level bitmaps), from a device array of tags; the host receives the
+ * new mesh ids as runs only when the leaves change, and renumbers them.  The
+ * path every harten call of smr_adapt_run takes.  Returns 1 at the fixed
+ * point, 0 if the mesh was rebuilt. */
 int smr_mesh_rebuild_from_device_tags(smr_mesh* mesh, const uint8_t* tags, int64_t n);
 
 /* The device image of a mesh version, for user kernels that walk the leaves

@@ -356,11 +356,13 @@ extern "C"
                 mm.ctx->require_device();
                 require(n == mm.host.nref, "tag array size differs from the storage size");
                 smr::CellArray ca;
-                if (!smr::device_mesh_update(*mm.ctx, mm.device(), mm.host.cfg, tags, ca))
+                smr::MRMeshData nm;
+                nm.nref = -1; // set when the device builds the whole mesh
+                if (!smr::device_mesh_update(*mm.ctx, mm.device(), mm.host.cfg, tags, ca, &mm.host, &nm))
                 {
                     return 1;
                 }
-                smr::MRMeshData nm;
+                if (nm.nref < 0)
                 {
                     smr::ScopedTimer t3(nullptr, "mesh construction");
                     nm           = smr::build_mesh(mm.host.cfg, ca);

@@ -69,6 +69,62 @@ namespace smr::dev
         int64_t wpr[21] = {};
     };
 
+    // Per-level dense bitmaps of levels [0, hi] padded by `pad` cells on every
+    // side (submesh.hip): level l has n[l] = 2^l + 2 pad cells per axis, a
+    // row (Y, or (Z, Y) in 3D, padded indices) holds wpr[l] 64-bit words, x
+    // ascending from bit 0 at unpadded x = -pad; level l starts at word off[l].
+    struct PBits
+    {
+        int dim, hi, pad;
+        int64_t n[21];
+        int64_t wpr[21];
+        int64_t off[22];
+    };
+
+    inline PBits make_pbits(int dim, int hi, int pad)
+    {
+        PBits p{};
+        p.dim       = dim;
+        p.hi        = hi;
+        p.pad       = pad;
+        int64_t off = 0;
+        for (int l = 0; l <= 20; ++l)
+        {
+            p.n[l]   = (int64_t(1) << l) + 2 * pad;
+            p.wpr[l] = (p.n[l] + 63) / 64;
+            p.off[l] = off;
+            if (l <= hi)
+            {
+                off += p.wpr[l] * (dim == 3 ? p.n[l] * p.n[l] : p.n[l]);
+            }
+        }
+        p.off[21] = off;
+        return p;
+    }
+
+    // the sets of the device mesh construction, in the order of the mesh ids
+    enum SubmeshSet : int
+    {
+        SM_CELLS            = 0,
+        SM_CELLS_AND_GHOSTS = 1,
+        SM_PROJ_CELLS       = 2,
+        SM_UNION_CELLS      = 3,
+        SM_REFERENCE        = 4,
+        SM_NSETS            = 5
+    };
+
+    // the five ids of MRMesh::update_sub_mesh_impl from the leaves of levels
+    // [b.lo, b.hi] (unpadded), into sets (SM_NSETS x p.off[p.hi + 1] words);
+    // r: scheme stencil radius, s: prediction radius, mr: max_level > min_level
+    void submesh_build(const Bits& b, const uint64_t* leaves, const PBits& p, int r, int s, bool mr, uint64_t* sets, hipStream_t st);
+    size_t submesh_scan_temp_bytes(int64_t n);
+    // per-word run counts of nsets consecutive sets and their exclusive sum
+    void submesh_run_offsets(const PBits& p, int nsets, const uint64_t* sets, int32_t* counts, int32_t* offsets, void* temp, size_t temp_bytes,
+                             hipStream_t s);
+    // the runs (x0, x1, y, z, level, set), 6 int32 each, of at most cap runs
+    void submesh_run_emit(const PBits& p, int nsets, const uint64_t* sets, const int32_t* offsets, int32_t* runs, int64_t cap, hipStream_t s);
+
+
     // One operator applied to a cell range of one instance: the unit of the
     // fused stage program that advances many independent meshes together.
     enum SegmentOp : int32_t

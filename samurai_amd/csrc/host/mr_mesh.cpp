@@ -320,6 +320,12 @@ namespace smr
         }
     } // namespace
 
+    void complete_mesh(MRMeshData& m)
+    {
+        construct_corners(m);
+        renumbering(m);
+    }
+
     MRMeshData build_mesh(const MeshConfig& cfg, const CellArray& cells)
     {
         return build_mesh(cfg, CellArray(cells));

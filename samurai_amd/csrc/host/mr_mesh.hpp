@@ -76,6 +76,9 @@ namespace smr
     // Mesh_base(ca, cfg) -> finalize_mesh (mesh.hpp:386-439, mr/mesh.hpp:221-455)
     MRMeshData build_mesh(const MeshConfig& cfg, const CellArray& cells);
     MRMeshData build_mesh(const MeshConfig& cfg, CellArray&& cells);
+    // corners and renumbering of a mesh whose five ids are set (the ids built
+    // elsewhere, e.g. on the device, as build_mesh would build them)
+    void complete_mesh(MRMeshData& m);
     // mra::make_mesh(unit box, cfg): all cells at max_level (mr/mesh.hpp:510-518)
     MRMeshData build_box_mesh(const MeshConfig& cfg);
 

@@ -345,7 +345,65 @@ namespace smr
         struct ProgramSets
         {
             std::vector<LevelSet> proj, pred, det, crit;
+            // update_fields from an older mesh: copy, coarsen, refine per level
+            std::vector<LevelSet> cp, co, re;
         };
+
+        // task t of the update_fields sets (algorithm/update_fields.hpp:27-127)
+        // from the mesh om to the mesh nm
+        void uf_task(const MRMeshData& om, const MRMeshData& nm, int task, ProgramSets& sets)
+        {
+            const int minl = om.cfg.min_level;
+            const int kind = task % 3, level = task / 3;
+            if (kind == 0 && level >= minl)
+            {
+                sets.cp[size_t(level)] = set_intersection(om.ids[REFERENCE][size_t(level)], nm.ids[CELLS][size_t(level)]);
+            }
+            else if (kind == 1 && level >= minl + 1)
+            {
+                sets.co[size_t(level)] = set_intersection(on(om.ids[CELLS][size_t(level)], level - 1), nm.ids[CELLS][size_t(level - 1)]);
+            }
+            else if (kind == 2 && level >= minl + 1)
+            {
+                sets.re[size_t(level)] = set_intersection(on(nm.ids[CELLS][size_t(level)], level - 1), om.ids[CELLS][size_t(level - 1)]);
+            }
+        }
+
+        // list `which` (0 copy, 1 coarsen, 2 refine) of update_fields from its sets
+        void uf_assemble(const MRMeshData& om, int which, WorkList& w, const ProgramSets& sets)
+        {
+            const int dim = om.cfg.dim, s = om.cfg.pred_s;
+            const int minl = om.cfg.min_level, maxl = om.cfg.max_level;
+            w = WorkList();
+            if (which == 0)
+            {
+                w.taps.push_back({0, 0, 0, 0, 0, 0, 0});
+                w.taps.push_back({1, 0, 0, 0, 0, 0, 0});
+                for (int level = minl; level <= maxl; ++level)
+                {
+                    w.add(sets.cp[size_t(level)]);
+                }
+            }
+            else if (which == 1)
+            {
+                w.taps.push_back({0, 0, 0, 0, 0, 0, 0});
+                add_row_taps(w, 1, 1, 0, 0, 1, dim, 0, 1);
+                for (int level = minl + 1; level <= maxl; ++level)
+                {
+                    w.add(sets.co[size_t(level)]);
+                }
+            }
+            else
+            {
+                add_row_taps(w, 0, 1, 0, 0, 1, dim, 0, 1);
+                add_row_taps(w, 1, 0, int8_t(-s), -s, s, dim, 0, int8_t(2 * s));
+                for (int level = minl + 1; level <= maxl; ++level)
+                {
+                    w.add(sets.re[size_t(level)]);
+                }
+            }
+            w.finish();
+        }
 
         int ghost_tasks(const MRMeshData& m)
         {
@@ -399,15 +457,20 @@ namespace smr
 
         // the five work lists of the ghost and adapt programs from their
         // per-level sets; the lists are independent and built in parallel
-        void assemble(const MRMeshData& m, GhostProgram* gp, AdaptProgram* ap, ProgramSets& sets)
+        void assemble(const MRMeshData& m, GhostProgram* gp, AdaptProgram* ap, ProgramSets& sets, const MRMeshData* old = nullptr,
+                      WorkList* uf = nullptr)
         {
             const int dim = m.cfg.dim, s = m.cfg.pred_s;
             const int minl = m.cfg.min_level, maxl = m.cfg.max_level;
             const int l0   = minl > 0 ? minl - 1 : 0;
-            parallel_for(5,
+            parallel_for(old && uf ? 8 : 5,
                          [&](int task)
                          {
-                             if (task == 0 && gp)
+                             if (task >= 5)
+                             {
+                                 uf_assemble(*old, task - 5, uf[task - 5], sets);
+                             }
+                             else if (task == 0 && gp)
                              {
                                  // projection (numeric/projection.hpp:15-65): runs at the coarse level
                                  WorkList& w = gp->projection;
@@ -443,7 +506,7 @@ namespace smr
                                  }
                                  w.finish();
                              }
-                             else if (task >= 3 && ap)
+                             else if ((task == 3 || task == 4) && ap)
                              {
                                  // mr_criteria / maximum (mr/criteria.hpp:13-122, mr/operators.hpp:22-96):
                                  // the criteria sets are those of the fine levels [min, max], the
@@ -468,6 +531,9 @@ namespace smr
             sets.pred.resize(n);
             sets.det.resize(n);
             sets.crit.resize(n);
+            sets.cp.resize(n);
+            sets.co.resize(n);
+            sets.re.resize(n);
             return sets;
         }
     } // namespace
@@ -492,7 +558,8 @@ namespace smr
         return p;
     }
 
-    void build_programs(const MRMeshData& m, WorkList& cells, GhostProgram& ghost, AdaptProgram& adapt)
+    void build_programs(const MRMeshData& m, WorkList& cells, GhostProgram& ghost, AdaptProgram& adapt, const MRMeshData* old,
+                        WorkList* uf)
     {
         ghost            = GhostProgram();
         adapt            = AdaptProgram();
@@ -500,8 +567,14 @@ namespace smr
         ghost.outer.resize(size_t(m.cfg.max_level) + 1);
         ProgramSets sets = make_sets(m);
         const int ng = ghost_tasks(m), na = adapt_tasks(m);
-        // one pool of tasks for both programs and the cells list
-        parallel_for(ng + na + 1,
+        const int nu = old && uf ? 3 * (m.cfg.max_level + 1) : 0;
+        if (nu && (old->cfg.max_level != m.cfg.max_level || old->cfg.min_level != m.cfg.min_level))
+        {
+            throw std::invalid_argument("update_fields between meshes of different level ranges");
+        }
+        // one pool of tasks for both programs, the cells list and the
+        // update_fields sets from the older mesh
+        parallel_for(ng + na + 1 + nu,
                      [&](int task)
                      {
                          if (task < ng)
@@ -512,12 +585,16 @@ namespace smr
                          {
                              adapt_task(m, task - ng, sets);
                          }
-                         else
+                         else if (task == ng + na)
                          {
                              cells = build_cells_list(m);
                          }
+                         else
+                         {
+                             uf_task(*old, m, task - ng - na - 1, sets);
+                         }
                      });
-        assemble(m, &ghost, &adapt, sets);
+        assemble(m, &ghost, &adapt, sets, nu ? old : nullptr, uf);
     }
 
     WorkList build_cells_list(const MRMeshData& m)
@@ -545,49 +622,10 @@ namespace smr
 
     void build_update_fields_lists(const MRMeshData& om, const MRMeshData& nm, WorkList& copy, WorkList& coarsen, WorkList& refine)
     {
-        const int dim = om.cfg.dim, s = om.cfg.pred_s;
-        const int minl = om.cfg.min_level, maxl = om.cfg.max_level;
-        copy    = WorkList();
-        coarsen = WorkList();
-        refine  = WorkList();
-        std::vector<LevelSet> cp(size_t(maxl) + 1), co(size_t(maxl) + 1), re(size_t(maxl) + 1);
-        parallel_for(3 * (maxl + 1),
-                     [&](int task)
-                     {
-                         const int kind = task % 3, level = task / 3;
-                         if (kind == 0 && level >= minl)
-                         {
-                             cp[size_t(level)] = set_intersection(om.ids[REFERENCE][size_t(level)], nm.ids[CELLS][size_t(level)]);
-                         }
-                         else if (kind == 1 && level >= minl + 1)
-                         {
-                             co[size_t(level)] =
-                                 set_intersection(on(om.ids[CELLS][size_t(level)], level - 1), nm.ids[CELLS][size_t(level - 1)]);
-                         }
-                         else if (kind == 2 && level >= minl + 1)
-                         {
-                             re[size_t(level)] =
-                                 set_intersection(on(nm.ids[CELLS][size_t(level)], level - 1), om.ids[CELLS][size_t(level - 1)]);
-                         }
-                     });
-        copy.taps.push_back({0, 0, 0, 0, 0, 0, 0});
-        copy.taps.push_back({1, 0, 0, 0, 0, 0, 0});
-        for (int level = minl; level <= maxl; ++level)
-        {
-            copy.add(cp[size_t(level)]);
-        }
-        copy.finish();
-        coarsen.taps.push_back({0, 0, 0, 0, 0, 0, 0});
-        add_row_taps(coarsen, 1, 1, 0, 0, 1, dim, 0, 1);
-        add_row_taps(refine, 0, 1, 0, 0, 1, dim, 0, 1);
-        add_row_taps(refine, 1, 0, int8_t(-s), -s, s, dim, 0, int8_t(2 * s));
-        for (int level = minl + 1; level <= maxl; ++level)
-        {
-            coarsen.add(co[size_t(level)]);
-            refine.add(re[size_t(level)]);
-        }
-        coarsen.finish();
-        refine.finish();
+        ProgramSets sets = make_sets(om);
+        parallel_for(3 * (om.cfg.max_level + 1), [&](int task) { uf_task(om, nm, task, sets); });
+        WorkList* out[3] = {&copy, &coarsen, &refine};
+        parallel_for(3, [&](int k) { uf_assemble(om, k, *out[k], sets); });
     }
 } // namespace smr
 

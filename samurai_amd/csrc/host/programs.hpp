@@ -97,7 +97,10 @@ namespace smr
     AdaptProgram build_adapt_program(const MRMeshData& m);
     WorkList build_cells_list(const MRMeshData& m);             // mesh[cells], taps for upwind
     // the three above together, their per-level work in one parallel loop
-    void build_programs(const MRMeshData& m, WorkList& cells, GhostProgram& ghost, AdaptProgram& adapt);
+    // with old and uf: also the update_fields lists (copy, coarsen, refine)
+    // from the mesh old to m into uf[0..2], in the same pool of tasks
+    void build_programs(const MRMeshData& m, WorkList& cells, GhostProgram& ghost, AdaptProgram& adapt, const MRMeshData* old = nullptr,
+                        WorkList* uf = nullptr);
     // update_fields (algorithm/update_fields.hpp:27-127): table 0 = new mesh, 1 = old
     void build_update_fields_lists(const MRMeshData& old_m, const MRMeshData& new_m, WorkList& copy, WorkList& coarsen,
                                    WorkList& refine);

@@ -355,6 +355,7 @@ namespace smr
             (void)hipStreamSynchronize(stream);
         }
         staging.reset();
+        readback.reset();
         if (d_err)
         {
             (void)hipFree(d_err);
@@ -671,10 +672,11 @@ namespace smr
         return dm;
     }
 
-    HostImage build_host_image(const MRMeshData& m)
+    HostImage build_host_image(const MRMeshData& m, const MRMeshData* old)
     {
         HostImage h;
-        build_programs(m, h.cells, h.ghost, h.adapt);
+        build_programs(m, h.cells, h.ghost, h.adapt, old, old ? h.uf : nullptr);
+        h.has_uf = old != nullptr;
         return h;
     }
 
@@ -701,11 +703,24 @@ namespace smr
                 p.list(*l);
             }
         }
+        if (h.has_uf)
+        {
+            for (int k = 0; k < 3; ++k)
+            {
+                dm.uf[k].host = std::move(h.uf[k]);
+                p.list(dm.uf[k]);
+            }
+            dm.has_uf = true;
+        }
         dm.has_ghost = dm.has_adapt = true;
     }
 
-    std::unique_ptr<DevMesh> upload_image(Ctx& ctx, const MRMeshData& m, HostImage&& h, bool check)
+    std::unique_ptr<DevMesh> upload_image(Ctx& ctx, const MRMeshData& m, HostImage&& h, bool check, const dev::Table* old_table)
     {
+        if (h.has_uf && !old_table)
+        {
+            throw std::logic_error("update_fields lists need the table of the mesh they come from");
+        }
         ctx.require_device();
         auto dm        = std::make_unique<DevMesh>();
         dm->table_blob = std::make_unique<Blob>();
@@ -715,6 +730,13 @@ namespace smr
         for (DevWorkList* l : dm->lists())
         {
             dev::resolve(l->d, dm->table, dm->table, ctx.d_err, ctx.stream);
+        }
+        if (dm->has_uf)
+        {
+            for (DevWorkList& l : dm->uf)
+            {
+                dev::resolve(l.d, dm->table, *old_table, ctx.d_err, ctx.stream);
+            }
         }
         dev::walk_rows(dm->cells.d, ctx.stream);
         if (check)
@@ -1234,12 +1256,21 @@ namespace smr
             ScopedTimer timer(&ctx, "fields update");
             hipStream_t s   = ctx.stream;
             DevMesh& odev   = *m.dev;
-            DevWorkList copy, coarsen, refine;
-            build_update_fields_lists(m.host, nm, copy.host, coarsen.host, refine.host);
-            upload_list(ctx, copy, ndev.table, odev.table);
-            upload_list(ctx, coarsen, ndev.table, odev.table);
-            upload_list(ctx, refine, ndev.table, odev.table);
+            DevWorkList local[3];
+            if (!ndev.has_uf)
+            {
+                // lists built and uploaded here (the image came without them)
+                build_update_fields_lists(m.host, nm, local[0].host, local[1].host, local[2].host);
+                for (DevWorkList& l : local)
+                {
+                    upload_list(ctx, l, ndev.table, odev.table);
+                }
+            }
+            // the resolve of the image's lists and of these is checked once
             ctx.check_resolve();
+            const DevWorkList& copy    = ndev.has_uf ? ndev.uf[0] : local[0];
+            const DevWorkList& coarsen = ndev.has_uf ? ndev.uf[1] : local[1];
+            const DevWorkList& refine  = ndev.has_uf ? ndev.uf[2] : local[2];
             for (Field* f : fs)
             {
                 double* nd = static_cast<double*>(dev_alloc(size_t(std::max<int64_t>(nm.nref, 1)) * 8, s));
@@ -1275,7 +1306,124 @@ namespace smr
         return b;
     }
 
-    bool device_mesh_update(Ctx& ctx, DevMesh& dm, const MeshConfig& cfg, const uint8_t* tag, CellArray& out)
+    namespace
+    {
+        bool host_submesh()
+        {
+            static const bool v = []
+            {
+                const char* e = std::getenv("SMR_HOST_SUBMESH");
+                return e && std::atoi(e) != 0;
+            }();
+            return v;
+        }
+
+        // the five ids of the mesh with the leaves `leaves` (unpadded bitmaps
+        // of b), built on the device and read back as runs; the run buffer
+        // is sized from the current mesh's runs, with a second round trip
+        // only if that is too small
+        MRMeshData device_submesh(Ctx& ctx, const MeshConfig& cfg, const dev::Bits& b, const uint64_t* leaves, const MRMeshData& current)
+        {
+            hipStream_t s          = ctx.stream;
+            const int pad          = std::max(cfg.radius, cfg.pred_s);
+            const dev::PBits p     = dev::make_pbits(cfg.dim, cfg.max_level, pad);
+            const int64_t total    = p.off[p.hi + 1];
+            const int64_t nwords   = total * dev::SM_NSETS;
+            int64_t cur_runs       = 0;
+            for (const CellArray& ca : current.ids)
+            {
+                for (const LevelSet& ls : ca)
+                {
+                    cur_runs += int64_t(ls.runs.size());
+                }
+            }
+            const int64_t cap     = cur_runs + cur_runs / 4 + 4096;
+            const size_t tbytes   = dev::submesh_scan_temp_bytes(nwords);
+            const size_t sbytes   = size_t(std::max<int64_t>(nwords, 1)) * 8;
+            const size_t cbytes   = (size_t(2 * std::max<int64_t>(nwords, 1)) * 4 + 255) & ~size_t(255);
+            DevBuffer work;
+            char* wp         = static_cast<char*>(work.get(sbytes + cbytes + tbytes + size_t(cap) * 24 + 256, s));
+            uint64_t* sets   = reinterpret_cast<uint64_t*>(wp);
+            int32_t* counts  = reinterpret_cast<int32_t*>(wp + sbytes);
+            int32_t* offsets = counts + std::max<int64_t>(nwords, 1);
+            void* temp       = wp + sbytes + cbytes;
+            int32_t* dr      = reinterpret_cast<int32_t*>(wp + sbytes + cbytes + tbytes);
+            dev::submesh_build(b, leaves, p, cfg.radius, cfg.pred_s, cfg.max_level != cfg.min_level, sets, s);
+            dev::submesh_run_offsets(p, dev::SM_NSETS, sets, counts, offsets, temp, tbytes, s);
+            dev::submesh_run_emit(p, dev::SM_NSETS, sets, offsets, dr, cap, s);
+            // the run count (last offset + last count), then the runs
+            char* h           = ctx.readback->acquire(8 + size_t(cap) * 24);
+            int32_t* last     = reinterpret_cast<int32_t*>(h);
+            int32_t* hr       = reinterpret_cast<int32_t*>(h + 8);
+            hip_check(hipMemcpyAsync(&last[0], offsets + nwords - 1, 4, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(submesh)");
+            hip_check(hipMemcpyAsync(&last[1], counts + nwords - 1, 4, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(submesh)");
+            hip_check(hipMemcpyAsync(hr, dr, size_t(cap) * 24, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(submesh)");
+            ctx.sync();
+            int64_t nruns = int64_t(last[0]) + last[1];
+            if (nruns > cap)
+            {
+                DevBuffer rb;
+                int32_t* big = static_cast<int32_t*>(rb.get(size_t(nruns) * 24, s));
+                dev::submesh_run_emit(p, dev::SM_NSETS, sets, offsets, big, nruns, s);
+                h  = ctx.readback->acquire(8 + size_t(nruns) * 24);
+                hr = reinterpret_cast<int32_t*>(h + 8);
+                hip_check(hipMemcpyAsync(hr, big, size_t(nruns) * 24, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(submesh)");
+                ctx.sync();
+            }
+            MRMeshData m;
+            m.cfg = cfg;
+            for (auto& ca : m.ids)
+            {
+                ca = empty_cell_array(cfg.dim);
+            }
+            // the runs come in (id, level, z, y, x) order: every level set is
+            // one segment, found by bisection and filled by a task of its own
+            const int nl = cfg.max_level + 1;
+            auto first   = [&](int id, int level)
+            {
+                int64_t lo = 0, hi = nruns;
+                while (lo < hi)
+                {
+                    const int64_t mid = (lo + hi) >> 1;
+                    const int32_t* r  = hr + 6 * mid;
+                    if (r[5] < id || (r[5] == id && r[4] < level))
+                    {
+                        lo = mid + 1;
+                    }
+                    else
+                    {
+                        hi = mid;
+                    }
+                }
+                return lo;
+            };
+            parallel_for(dev::SM_NSETS * nl,
+                         [&](int task)
+                         {
+                             const int id = task / nl, level = task % nl;
+                             const int64_t a = first(id, level), e = first(id, level + 1);
+                             if (e <= a)
+                             {
+                                 return;
+                             }
+                             LevelSet& ls = m.ids[size_t(id)][size_t(level)];
+                             ls.runs.reserve(size_t(e - a));
+                             ls.ry.reserve(size_t(e - a));
+                             ls.rz.reserve(size_t(e - a));
+                             ls.rp.reserve(size_t(e - a) + 1);
+                             for (int64_t k = a; k < e; ++k)
+                             {
+                                 const int32_t* r = hr + 6 * k;
+                                 ls.push(r[2], r[3], r[0], r[1]);
+                             }
+                         });
+            complete_mesh(m);
+            return m;
+        }
+    } // namespace
+
+    bool device_mesh_update(Ctx& ctx, DevMesh& dm, const MeshConfig& cfg, const uint8_t* tag, CellArray& out, const MRMeshData* current,
+                            MRMeshData* mesh)
     {
         hipStream_t s       = ctx.stream;
         const dev::Bits b   = make_bits(cfg);
@@ -1340,6 +1488,13 @@ namespace smr
         if (!h_flags[kPasses])
         {
             return false;
+        }
+        if (mesh && current && !host_submesh())
+        {
+            ScopedTimer t(&ctx, "mesh construction");
+            *mesh           = device_submesh(ctx, cfg, b, buf[cur], *current);
+            mesh->graduated = true;
+            return true;
         }
         ScopedTimer t(&ctx, "leaves to host");
         // run-length encoding of the new leaves, in traversal order, into a
@@ -1509,13 +1664,17 @@ namespace smr
             // the host receives the new leaves only when they change (one
             // round trip when they do not)
             CellArray new_ca;
-            if (!device_mesh_update(ctx, dm, cfg, st.tag, new_ca))
+            nm.nref = -1; // set when the device builds the whole mesh
+            if (!device_mesh_update(ctx, dm, cfg, st.tag, new_ca, &m.host, &nm))
             {
                 return true;
             }
-            ScopedTimer t3(nullptr, "mesh construction");
-            nm           = build_mesh(cfg, std::move(new_ca));
-            nm.graduated = true;
+            if (nm.nref < 0)
+            {
+                ScopedTimer t3(nullptr, "mesh construction");
+                nm           = build_mesh(cfg, std::move(new_ca));
+                nm.graduated = true;
+            }
         }
         {
             // the complete image of the new mesh (cells, ghost and adapt
@@ -1523,11 +1682,12 @@ namespace smr
             // the update_fields lists
             HostImage img;
             {
+                // the programs and the update_fields lists from the current mesh
                 ScopedTimer t4(nullptr, "program build");
-                img = build_host_image(nm);
+                img = build_host_image(nm, &m.host);
             }
             ScopedTimer timer(&ctx, "device upload");
-            ndev = upload_image(ctx, nm, std::move(img), false);
+            ndev = upload_image(ctx, nm, std::move(img), false, &dm.table);
         }
         update_fields(fs, nm, *ndev);
         m.replace(std::move(nm), std::move(ndev));

@@ -204,6 +204,10 @@ namespace smr
         // the leaves as per-level bitmaps (device mesh update), built on first use
         DevBuffer leaf_bits;
         bool has_leaf_bits = false;
+        // update_fields lists (copy, coarsen, refine) from the mesh this image
+        // replaced, packed with the image when it was built for a mesh change
+        bool has_uf = false;
+        DevWorkList uf[3];
         // the work lists of a complete image (cells first)
         std::vector<DevWorkList*> lists() { return {&cells, &projection, &prediction, &detail, &criteria, &maximum}; }
     };
@@ -279,6 +283,8 @@ namespace smr
         int* d_err         = nullptr; // two ints: resolve error, then a scratch flag
         // pinned host memory the mesh images are staged in before their copy
         std::unique_ptr<PinnedStaging> staging = std::make_unique<PinnedStaging>();
+        // pinned host memory the runs of a new mesh are read back into
+        std::unique_ptr<PinnedStaging> readback = std::make_unique<PinnedStaging>();
         ~Ctx();
         int* d_flag() const { return d_err + 1; }
         bool has_device() const { return device >= 0; }
@@ -320,12 +326,19 @@ namespace smr
         WorkList cells;
         GhostProgram ghost;
         AdaptProgram adapt;
+        // update_fields (copy, coarsen, refine) from an older mesh, when given
+        bool has_uf = false;
+        WorkList uf[3];
     };
-    HostImage build_host_image(const MRMeshData& m);
+    // with old: also the update_fields lists from old to m
+    HostImage build_host_image(const MRMeshData& m, const MRMeshData* old = nullptr);
     // packs a complete image (table, cells, ghost and adapt programs) into p
     void pack_image(Packer& p, DevMesh& dm, const MRMeshData& m, HostImage&& h);
     // device half: upload and resolve (check = wait for and verify the resolve)
-    std::unique_ptr<DevMesh> upload_image(Ctx& ctx, const MRMeshData& m, HostImage&& h, bool check);
+    // old_table: the table of the mesh the image replaces (resolves the
+    // update_fields lists of an image built with them)
+    std::unique_ptr<DevMesh> upload_image(Ctx& ctx, const MRMeshData& m, HostImage&& h, bool check,
+                                          const dev::Table* old_table = nullptr);
     // DevWorkList upload + resolve against the given tables
     void upload_list(Ctx& ctx, DevWorkList& l, const dev::Table& t0, const dev::Table& t1);
 
@@ -334,7 +347,12 @@ namespace smr
     // update_cell_array_from_tag + make_graduation on the device from tags
     // indexed by dm's storage (graduation.hpp:573-843): false if the leaves
     // do not change, else true with the new leaves in out
-    bool device_mesh_update(Ctx& ctx, DevMesh& dm, const MeshConfig& cfg, const uint8_t* tag, CellArray& out);
+    // With mesh: the new mesh's ids are built on the device too
+    // (update_sub_mesh_impl, mr/mesh.hpp:221-455) and *mesh is the complete
+    // new mesh (out is then left empty); SMR_HOST_SUBMESH=1 builds them on the
+    // host from out instead.
+    bool device_mesh_update(Ctx& ctx, DevMesh& dm, const MeshConfig& cfg, const uint8_t* tag, CellArray& out, const MRMeshData* current = nullptr,
+                            MRMeshData* mesh = nullptr);
 
     // criteria thresholds of harten iteration `ite` (mr/adapt.hpp:325-339)
     dev::Thresholds make_thresholds(const MeshConfig& cfg, int ite, double eps, double regularity);
