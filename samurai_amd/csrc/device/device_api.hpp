@@ -285,6 +285,18 @@ namespace smr::dev
         return l.cellrun && l.ncells > pairs ? l.ncells : pairs;
     }
     void resolve(const List& l, const Table& t0, const Table& t1, int* err, hipStream_t s);
+    // the resolves of up to kResolveJobs lists in one launch: list k against
+    // (t_new, t_new), or (t_new, t_old) when old[k] is set
+    constexpr int kResolveJobs = 12;
+    struct ResolveBatch
+    {
+        List lists[kResolveJobs];
+        int64_t start[kResolveJobs + 1]; // first thread of each list
+        int8_t old[kResolveJobs];
+        int n;
+        Table t_new, t_old;
+    };
+    void resolve_batch(const ResolveBatch& b, int* err, hipStream_t s);
     // fills l.walk_rows from the resolved bases (after resolve; no-op without walks)
     void walk_rows(const List& l, hipStream_t s);
     // out[row[r]] = ordered sum of the terms of row r (flux-based convection)

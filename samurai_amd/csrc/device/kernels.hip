@@ -13,6 +13,30 @@ namespace smr::dev
     {
         constexpr int BLOCK = 256;
 
+        __global__ void k_resolve_batch(ResolveBatch b, int* err)
+        {
+            const int64_t gid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+            if (gid >= b.start[b.n])
+            {
+                return;
+            }
+            int k = 0;
+            while (k + 1 < b.n && gid >= b.start[k + 1])
+            {
+                ++k;
+            }
+            const List& l   = b.lists[k];
+            const int64_t i = gid - b.start[k];
+            if (i < l.nruns * l.ntaps)
+            {
+                op_resolve(l, i, b.t_new, b.old[k] ? b.t_old : b.t_new, err);
+            }
+            if (l.cellrun && i < l.ncells)
+            {
+                op_cellrun(l, i);
+            }
+        }
+
         // the (run, tap) bases and, in the same launch, the run map
         __global__ void k_resolve(List l, Table t0, Table t1, int* err)
         {
@@ -718,6 +742,15 @@ namespace smr::dev
         if (n > 0)
         {
             hipLaunchKernelGGL(k_resolve, dim3(nblocks(n)), dim3(BLOCK), 0, s, l, t0, t1, err);
+        }
+    }
+
+    void resolve_batch(const ResolveBatch& b, int* err, hipStream_t s)
+    {
+        const int64_t n = b.n > 0 ? b.start[b.n] : 0;
+        if (n > 0)
+        {
+            hipLaunchKernelGGL(k_resolve_batch, dim3(nblocks(n)), dim3(BLOCK), 0, s, b, err);
         }
     }
 

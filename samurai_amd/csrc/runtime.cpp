@@ -455,7 +455,9 @@ namespace smr
             });
     }
 
-    void Packer::list(DevWorkList& l)
+    // the host arrays of a work list besides its runs: chunk hints (into l),
+    // taps, 64-cell tiles and column walks (thread-safe across lists)
+    Packer::ListPrep Packer::prepare_list(DevWorkList& l)
     {
         const WorkList& w     = l.host;
         const int64_t nruns   = w.nruns();
@@ -570,6 +572,27 @@ namespace smr
                 }
             }
         }
+        ListPrep prep;
+        prep.taps        = std::move(taps);
+        prep.tiles       = std::move(tiles);
+        prep.next        = std::move(next);
+        prep.walks       = std::move(walks);
+        prep.nrows_total = nrows_total;
+        prep.walk_stride = walk_stride;
+        return prep;
+    }
+
+    void Packer::commit_list(DevWorkList& l, ListPrep&& prep)
+    {
+        const WorkList& w    = l.host;
+        const int64_t nruns  = w.nruns();
+        const int64_t ncells = w.ncells();
+        std::vector<int8_t>& taps    = prep.taps;
+        std::vector<int32_t>& tiles  = prep.tiles;
+        std::vector<int32_t>& next   = prep.next;
+        std::vector<int32_t>& walks  = prep.walks;
+        const int64_t nrows_total    = prep.nrows_total;
+        const int32_t walk_stride    = prep.walk_stride;
         const size_t oiv  = b_.add(w.iv.data(), w.iv.size() * 4);
         const size_t opf  = b_.add(w.pref.data(), w.pref.size() * 8);
         const size_t oh   = b_.add(l.hint.data(), l.hint.size() * 4);
@@ -605,6 +628,21 @@ namespace smr
                 l.d.ncells = ncells;
                 l.d.ntaps  = ntaps;
             });
+    }
+
+    void Packer::list(DevWorkList& l)
+    {
+        commit_list(l, prepare_list(l));
+    }
+
+    void Packer::lists(const std::vector<DevWorkList*>& ls)
+    {
+        std::vector<ListPrep> preps(ls.size());
+        parallel_for(int(ls.size()), [&](int i) { preps[size_t(i)] = prepare_list(*ls[size_t(i)]); });
+        for (size_t i = 0; i < ls.size(); ++i)
+        {
+            commit_list(*ls[i], std::move(preps[i]));
+        }
     }
 
     void Packer::finish(hipStream_t s)
@@ -683,35 +721,29 @@ namespace smr
     void pack_image(Packer& p, DevMesh& dm, const MRMeshData& m, HostImage&& h)
     {
         p.table(m, dm.table);
-        dm.cells.host = std::move(h.cells);
-        p.list(dm.cells);
         dm.outer.resize(h.ghost.outer.size());
         for (size_t l = 0; l < h.ghost.outer.size(); ++l)
         {
             dm.outer[l].host = std::move(h.ghost.outer[l]);
             p.recipes(dm.outer[l]);
         }
+        dm.cells.host      = std::move(h.cells);
         dm.projection.host = std::move(h.ghost.projection);
         dm.prediction.host = std::move(h.ghost.prediction);
         dm.detail.host     = std::move(h.adapt.detail);
         dm.criteria.host   = std::move(h.adapt.criteria);
         dm.maximum.host    = std::move(h.adapt.maximum);
-        for (DevWorkList* l : dm.lists())
-        {
-            if (l != &dm.cells)
-            {
-                p.list(*l);
-            }
-        }
+        std::vector<DevWorkList*> ls = dm.lists();
         if (h.has_uf)
         {
             for (int k = 0; k < 3; ++k)
             {
                 dm.uf[k].host = std::move(h.uf[k]);
-                p.list(dm.uf[k]);
+                ls.push_back(&dm.uf[k]);
             }
             dm.has_uf = true;
         }
+        p.lists(ls);
         dm.has_ghost = dm.has_adapt = true;
     }
 
@@ -727,17 +759,34 @@ namespace smr
         Packer p(*dm->table_blob);
         pack_image(p, *dm, m, std::move(h));
         finish_staged(ctx, p, *dm->table_blob);
+        // every list of the image (and its update_fields lists) in one launch
+        dev::ResolveBatch rb{};
+        rb.t_new = dm->table;
+        rb.t_old = old_table ? *old_table : dm->table;
+        rb.n     = 0;
+        auto add = [&](const DevWorkList& l, bool old)
+        {
+            if (rb.n >= dev::kResolveJobs)
+            {
+                throw std::logic_error("resolve batch: too many lists");
+            }
+            rb.lists[rb.n] = l.d;
+            rb.old[rb.n]   = old ? 1 : 0;
+            rb.start[rb.n + 1] = rb.start[rb.n] + dev::resolve_range(l.d);
+            ++rb.n;
+        };
         for (DevWorkList* l : dm->lists())
         {
-            dev::resolve(l->d, dm->table, dm->table, ctx.d_err, ctx.stream);
+            add(*l, false);
         }
         if (dm->has_uf)
         {
-            for (DevWorkList& l : dm->uf)
+            for (const DevWorkList& l : dm->uf)
             {
-                dev::resolve(l.d, dm->table, *old_table, ctx.d_err, ctx.stream);
+                add(l, true);
             }
         }
+        dev::resolve_batch(rb, ctx.d_err, ctx.stream);
         dev::walk_rows(dm->cells.d, ctx.stream);
         if (check)
         {

@@ -220,10 +220,21 @@ namespace smr
         explicit Packer(Blob& b) : b_(b) {}
         void table(const MRMeshData& m, dev::Table& t);
         void list(DevWorkList& l);
+        // several lists, their host arrays prepared in parallel
+        void lists(const std::vector<DevWorkList*>& ls);
         void recipes(DevRecipes& r);
         void finish(hipStream_t s);
 
       private:
+        struct ListPrep
+        {
+            std::vector<int8_t> taps;
+            std::vector<int32_t> tiles, next, walks;
+            int64_t nrows_total = 0;
+            int32_t walk_stride = 0;
+        };
+        static ListPrep prepare_list(DevWorkList& l);
+        void commit_list(DevWorkList& l, ListPrep&& prep);
         Blob& b_;
         std::vector<std::function<void()>> binds_;
     };
