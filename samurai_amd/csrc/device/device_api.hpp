@@ -114,9 +114,11 @@ namespace smr::dev
     };
 
     // the five ids of MRMesh::update_sub_mesh_impl from the leaves of levels
-    // [b.lo, b.hi] (unpadded), into sets (SM_NSETS x p.off[p.hi + 1] words);
-    // r: scheme stencil radius, s: prediction radius, mr: max_level > min_level
-    void submesh_build(const Bits& b, const uint64_t* leaves, const PBits& p, int r, int s, bool mr, uint64_t* sets, hipStream_t st);
+    // [b.lo, b.hi] (unpadded), into sets (SM_NSETS x p.off[p.hi + 1] words),
+    // with scratch of 2 x p.off[p.hi + 1] words; r: scheme stencil radius, s:
+    // prediction radius, mr: max_level > min_level
+    void submesh_build(const Bits& b, const uint64_t* leaves, const PBits& p, int r, int s, bool mr, uint64_t* sets, uint64_t* scratch,
+                       hipStream_t st);
     size_t submesh_scan_temp_bytes(int64_t n);
     // per-word run counts of nsets consecutive sets and their exclusive sum
     void submesh_run_offsets(const PBits& p, int nsets, const uint64_t* sets, int32_t* counts, int32_t* offsets, void* temp, size_t temp_bytes,

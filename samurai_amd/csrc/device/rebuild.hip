@@ -305,19 +305,37 @@ namespace smr::dev
         // of the finest one and less), so that p[c] = union over f >= c + 2 of
         // the dilated fine cells of level f projected on c
         template <int DIM>
-        __global__ __launch_bounds__(1024) void k_project_cascade(Bits b, uint64_t* p)
+        __device__ __forceinline__ void cascade_word(const Bits& b, uint64_t* p, int c, int64_t gw)
         {
-            for (int c = b.hi - 3; c >= b.lo; --c)
+            const WordPos q  = word_pos(b, DIM, gw);
+            const uint64_t v = on_parent_word<DIM>(b, p, c, q.y, q.z, q.wx);
+            if (v)
+            {
+                p[gw] |= v;
+            }
+        }
+
+        // one level c of the cascade, grid-wide (the large levels)
+        template <int DIM>
+        __global__ __launch_bounds__(RB) void k_project_cascade_level(Bits b, uint64_t* p, int c)
+        {
+            const int64_t gw = b.off[c] + int64_t(blockIdx.x) * RB + threadIdx.x;
+            if (gw < b.off[c + 1])
+            {
+                cascade_word<DIM>(b, p, c, gw);
+            }
+        }
+
+        // levels c0 down to lo of the cascade by one block
+        template <int DIM>
+        __global__ __launch_bounds__(1024) void k_project_cascade(Bits b, uint64_t* p, int c0)
+        {
+            for (int c = c0; c >= b.lo; --c)
             {
                 const int64_t n0 = b.off[c], n1 = b.off[c + 1];
                 for (int64_t gw = n0 + threadIdx.x; gw < n1; gw += 1024)
                 {
-                    const WordPos q = word_pos(b, DIM, gw);
-                    const uint64_t v = on_parent_word<DIM>(b, p, c, q.y, q.z, q.wx);
-                    if (v)
-                    {
-                        p[gw] |= v;
-                    }
+                    cascade_word<DIM>(b, p, c, gw);
                 }
                 __syncthreads();
             }
@@ -493,21 +511,33 @@ namespace smr::dev
             return;
         }
         bits_zero(b, p, s);
+        // the cascade: a grid-wide launch per level of more than kSmall words,
+        // then one block for the rest
+        constexpr int64_t kSmall = 1024;
+        int c = b.hi - 3;
         if (dim == 2)
         {
             hipLaunchKernelGGL(k_dilate_project<2>, dim3(blocks_for(n)), dim3(RB), 0, s, b, ca, p, w2);
-            if (b.hi - 3 >= b.lo)
+            for (; c >= b.lo && b.off[c + 1] - b.off[c] > kSmall; --c)
             {
-                hipLaunchKernelGGL(k_project_cascade<2>, dim3(1), dim3(1024), 0, s, b, p);
+                hipLaunchKernelGGL(k_project_cascade_level<2>, dim3(blocks_for(b.off[c + 1] - b.off[c])), dim3(RB), 0, s, b, p, c);
+            }
+            if (c >= b.lo)
+            {
+                hipLaunchKernelGGL(k_project_cascade<2>, dim3(1), dim3(1024), 0, s, b, p, c);
             }
             hipLaunchKernelGGL(k_grad_apply<2>, dim3(blocks_for(n)), dim3(RB), 0, s, b, ca, p, out, flag);
         }
         else
         {
             hipLaunchKernelGGL(k_dilate_project<3>, dim3(blocks_for(n)), dim3(RB), 0, s, b, ca, p, w2);
-            if (b.hi - 3 >= b.lo)
+            for (; c >= b.lo && b.off[c + 1] - b.off[c] > kSmall; --c)
             {
-                hipLaunchKernelGGL(k_project_cascade<3>, dim3(1), dim3(1024), 0, s, b, p);
+                hipLaunchKernelGGL(k_project_cascade_level<3>, dim3(blocks_for(b.off[c + 1] - b.off[c])), dim3(RB), 0, s, b, p, c);
+            }
+            if (c >= b.lo)
+            {
+                hipLaunchKernelGGL(k_project_cascade<3>, dim3(1), dim3(1024), 0, s, b, p, c);
             }
             hipLaunchKernelGGL(k_grad_apply<3>, dim3(blocks_for(n)), dim3(RB), 0, s, b, ca, p, out, flag);
         }

@@ -255,9 +255,9 @@ namespace smr::dev
             }
         }
 
-        // reference before the projection ghosts: cells_and_ghosts[t] | the
-        // prediction ghosts of level t + 1 (clipped) and of level t + 2
-        __global__ __launch_bounds__(SB) void k_sm_reference(PBits p, const uint64_t* C, const uint64_t* G, uint64_t* R, int s, int mr)
+        // K1[t] = cells_and_ghosts[t + 1].on(t), K2[t] = cells[t + 2].on(t):
+        // the coarse maps the prediction ghosts are dilated from
+        __global__ __launch_bounds__(SB) void k_sm_coarse(PBits p, const uint64_t* C, const uint64_t* G, uint64_t* K1, uint64_t* K2)
         {
             const int64_t gw = int64_t(blockIdx.x) * SB + threadIdx.x;
             if (gw >= p.off[p.hi + 1])
@@ -266,39 +266,53 @@ namespace smr::dev
             }
             const PWord q  = pword(p, gw);
             const int t    = q.l;
-            uint64_t acc   = G[gw];
-            const int sz   = p.dim == 3 ? s : 0;
+            const uint64_t m = tail_mask(p, t, q.wx);
+            K1[gw] = t + 1 <= p.hi ? coarse_word(p, G, t, 1, q.Y, q.Z, 64 * q.wx) & m : 0ull;
+            K2[gw] = t + 2 <= p.hi ? coarse_word(p, C, t, 2, q.Y, q.Z, 64 * q.wx) & m : 0ull;
+        }
+
+        // reference before the projection ghosts: cells_and_ghosts[t] |
+        // nestedExpand(K1[t], s) clipped to [-s, 2^t + s) | nestedExpand(K2[t], s)
+        __global__ __launch_bounds__(SB) void k_sm_reference(PBits p, const uint64_t* G, const uint64_t* K1, const uint64_t* K2, uint64_t* R, int s,
+                                                             int mr)
+        {
+            const int64_t gw = int64_t(blockIdx.x) * SB + threadIdx.x;
+            if (gw >= p.off[p.hi + 1])
+            {
+                return;
+            }
+            const PWord q = pword(p, gw);
+            const int t   = q.l;
+            uint64_t acc  = G[gw];
             if (mr)
             {
+                const int sz    = p.dim == 3 ? s : 0;
                 const int64_t n = int64_t(1) << t;
-                for (int k = 1; k <= 2; ++k)
+                uint64_t b1 = 0, b2 = 0;
+                for (int dz = -sz; dz <= sz; ++dz)
                 {
-                    if (t + k > p.hi)
+                    for (int dy = -s; dy <= s; ++dy)
                     {
-                        break;
-                    }
-                    const uint64_t* src = k == 1 ? G : C;
-                    uint64_t b          = 0;
-                    for (int dz = -sz; dz <= sz; ++dz)
-                    {
-                        for (int dy = -s; dy <= s; ++dy)
+                        const uint64_t* r1 = prow(p, K1, t, q.Y + dy, q.Z + dz);
+                        const uint64_t* r2 = prow(p, K2, t, q.Y + dy, q.Z + dz);
+                        for (int dx = -s; dx <= s; ++dx)
                         {
-                            const int64_t Y = q.Y + dy, Z = q.Z + dz;
-                            for (int dx = -s; dx <= s; ++dx)
+                            if (r1)
                             {
-                                b |= coarse_word(p, src, t, k, Y, Z, 64 * q.wx + dx);
+                                b1 |= load64(r1, p.wpr[t], 64 * q.wx + dx);
+                            }
+                            if (r2)
+                            {
+                                b2 |= load64(r2, p.wpr[t], 64 * q.wx + dx);
                             }
                         }
                     }
-                    if (k == 1)
-                    {
-                        // clip_box(..., [-s, 2^t + s)) along every axis
-                        const int64_t y = q.Y - p.pad, z = q.Z - p.pad;
-                        const bool in_row = y >= -s && y < n + s && (p.dim == 2 || (z >= -s && z < n + s));
-                        b = in_row ? (b & range_mask(p.pad, q.wx, -s, n + s)) : 0ull;
-                    }
-                    acc |= b;
                 }
+                // clip_box(..., [-s, 2^t + s)) along every axis
+                const int64_t y = q.Y - p.pad, z = q.Z - p.pad;
+                const bool in_row = y >= -s && y < n + s && (p.dim == 2 || (z >= -s && z < n + s));
+                b1 = in_row ? (b1 & range_mask(p.pad, q.wx, -s, n + s)) : 0ull;
+                acc |= b1 | b2;
             }
             R[gw] = acc & tail_mask(p, t, q.wx);
         }
@@ -440,7 +454,8 @@ namespace smr::dev
         }
     } // namespace
 
-    void submesh_build(const Bits& b, const uint64_t* leaves, const PBits& p, int r, int s, bool mr, uint64_t* sets, hipStream_t st)
+    void submesh_build(const Bits& b, const uint64_t* leaves, const PBits& p, int r, int s, bool mr, uint64_t* sets, uint64_t* scratch,
+                       hipStream_t st)
     {
         const int64_t total = p.off[p.hi + 1];
         if (total <= 0)
@@ -456,7 +471,7 @@ namespace smr::dev
         hipLaunchKernelGGL(k_sm_dilate, dim3(sm_blocks(total)), dim3(SB), 0, st, p, C, G, r);
         // the union chain from the finest level down (union[hi] is empty): a
         // launch per large level, then one block for the small ones
-        constexpr int64_t kSmall = 8192; // words of a level a single block takes
+        constexpr int64_t kSmall = 1024; // words of a level a single block takes
         hipLaunchKernelGGL(k_sm_zero, dim3(sm_blocks(total)), dim3(SB), 0, st, U, total);
         int l = p.hi;
         for (; l >= 1 && p.off[l] - p.off[l - 1] > kSmall; --l)
@@ -468,7 +483,13 @@ namespace smr::dev
         {
             hipLaunchKernelGGL(k_sm_union_cascade, dim3(1), dim3(1024), 0, st, p, C, U, l);
         }
-        hipLaunchKernelGGL(k_sm_reference, dim3(sm_blocks(total)), dim3(SB), 0, st, p, C, G, R, s, mr ? 1 : 0);
+        uint64_t* K1 = scratch;
+        uint64_t* K2 = scratch + total;
+        if (mr)
+        {
+            hipLaunchKernelGGL(k_sm_coarse, dim3(sm_blocks(total)), dim3(SB), 0, st, p, C, G, K1, K2);
+        }
+        hipLaunchKernelGGL(k_sm_reference, dim3(sm_blocks(total)), dim3(SB), 0, st, p, G, K1, K2, R, s, mr ? 1 : 0);
         hipLaunchKernelGGL(k_sm_zero, dim3(sm_blocks(total)), dim3(SB), 0, st, J, total);
         if (mr)
         {

@@ -1388,7 +1388,7 @@ namespace smr
             }
             const int64_t cap     = cur_runs + cur_runs / 4 + 4096;
             const size_t tbytes   = dev::submesh_scan_temp_bytes(nwords);
-            const size_t sbytes   = size_t(std::max<int64_t>(nwords, 1)) * 8;
+            const size_t sbytes   = size_t(std::max<int64_t>(nwords + 2 * total, 1)) * 8; // the sets, then 2 scratch maps
             const size_t cbytes   = (size_t(2 * std::max<int64_t>(nwords, 1)) * 4 + 255) & ~size_t(255);
             DevBuffer work;
             char* wp         = static_cast<char*>(work.get(sbytes + cbytes + tbytes + size_t(cap) * 24 + 256, s));
@@ -1397,7 +1397,7 @@ namespace smr
             int32_t* offsets = counts + std::max<int64_t>(nwords, 1);
             void* temp       = wp + sbytes + cbytes;
             int32_t* dr      = reinterpret_cast<int32_t*>(wp + sbytes + cbytes + tbytes);
-            dev::submesh_build(b, leaves, p, cfg.radius, cfg.pred_s, cfg.max_level != cfg.min_level, sets, s);
+            dev::submesh_build(b, leaves, p, cfg.radius, cfg.pred_s, cfg.max_level != cfg.min_level, sets, sets + nwords, s);
             dev::submesh_run_offsets(p, dev::SM_NSETS, sets, counts, offsets, temp, tbytes, s);
             dev::submesh_run_emit(p, dev::SM_NSETS, sets, offsets, dr, cap, s);
             // the run count (last offset + last count), then the runs
