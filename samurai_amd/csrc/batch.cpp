@@ -223,6 +223,193 @@ namespace smr
             HostImage img;
             WorkList copy, coarsen, refine;
         };
+
+        bool batch_host_rebuild()
+        {
+            static const bool v = []
+            {
+                const char* e = std::getenv("SMR_BATCH_HOST_REBUILD");
+                return e && std::atoi(e) != 0;
+            }();
+            return v;
+        }
+
+        // update_cell_array_from_tag and the make_graduation fixed point of
+        // the meshes ms (tags[j] indexed by mesh j's storage) on the device,
+        // all meshes in each launch (rebuild_batch.hip).  Returns, per mesh,
+        // whether its leaves change, and the new leaves of those that do.
+        std::vector<bool> device_update_batch(Ctx& ctx, const std::vector<const Mesh*>& ms, const std::vector<const uint8_t*>& tags,
+                                              std::vector<CellArray>& out)
+        {
+            hipStream_t s = ctx.stream;
+            const int n   = int(ms.size());
+            std::vector<dev::MeshItem> items(static_cast<size_t>(n));
+            std::vector<int64_t> wpref(size_t(n) + 1, 0), cpref(size_t(n) + 1, 0), rpref(size_t(n) + 1, 0);
+            int w2 = -1;
+            for (int j = 0; j < n; ++j)
+            {
+                const Mesh& m       = *ms[size_t(j)];
+                const MeshConfig& c = m.host.cfg;
+                const dev::Bits b   = make_bits(c);
+                if (w2 >= 0 && w2 != 2 * c.grad_width)
+                {
+                    throw std::invalid_argument("batched mesh update: instances of different graduation widths");
+                }
+                w2                   = 2 * c.grad_width;
+                dev::MeshItem& it    = items[size_t(j)];
+                it.b                 = b;
+                it.cells             = m.dev->cells.d;
+                it.tag               = tags[size_t(j)];
+                it.w0                = wpref[size_t(j)];
+                it.dim               = c.dim;
+                it.min_level         = c.min_level;
+                it.max_level         = c.max_level;
+                wpref[size_t(j) + 1] = wpref[size_t(j)] + b.off[b.hi + 1];
+                cpref[size_t(j) + 1] = cpref[size_t(j)] + it.cells.ncells;
+                rpref[size_t(j) + 1] = rpref[size_t(j)] + it.cells.nruns;
+            }
+            const int64_t W       = wpref[size_t(n)];
+            constexpr int kPasses = 3;
+            const int fs          = kPasses + 1;
+            // the item table and its prefixes, one upload
+            const size_t ib = (size_t(n) * sizeof(dev::MeshItem) + 255) & ~size_t(255);
+            const size_t pb = size_t(n + 1) * 8;
+            const size_t hb = ib + 3 * pb;
+            char* h         = ctx.staging->acquire(hb);
+            std::memcpy(h, items.data(), size_t(n) * sizeof(dev::MeshItem));
+            std::memcpy(h + ib, wpref.data(), pb);
+            std::memcpy(h + ib + pb, cpref.data(), pb);
+            std::memcpy(h + ib + 2 * pb, rpref.data(), pb);
+            DevBuffer tab;
+            char* dt = static_cast<char*>(tab.get(hb, s));
+            hip_check(hipMemcpyAsync(dt, h, hb, hipMemcpyHostToDevice, s), "hipMemcpyAsync(mesh items)");
+            ctx.staging->release(s);
+            const auto* d_items = reinterpret_cast<const dev::MeshItem*>(dt);
+            const auto* d_wpref = reinterpret_cast<const int64_t*>(dt + ib);
+            const auto* d_cpref = reinterpret_cast<const int64_t*>(dt + ib + pb);
+            const auto* d_rpref = reinterpret_cast<const int64_t*>(dt + ib + 2 * pb);
+            // bitmaps: old, add, rem, two pass buffers, the projected marks; flags
+            DevBuffer work;
+            uint64_t* w = static_cast<uint64_t*>(work.get(size_t(6 * std::max<int64_t>(W, 1)) * 8 + size_t(n) * fs * 4 + 256, s));
+            uint64_t* old = w, *add = w + W, *rem = w + 2 * W, *q = w + 5 * W;
+            uint64_t* buf[2] = {w + 3 * W, w + 4 * W};
+            int* flags       = reinterpret_cast<int*>(w + 6 * W);
+            dev::mesh_batch_zero(w, 3 * W, s);
+            dev::mesh_batch_leaves(d_items, d_rpref, rpref[size_t(n)], n, old, s);
+            dev::mesh_batch_tag_marks(d_items, d_cpref, cpref[size_t(n)], n, add, rem, s);
+            dev::mesh_batch_combine(old, add, rem, buf[0], W, s);
+            int cur = 0;
+            std::vector<int> hf(size_t(n) * fs);
+            for (int batch = 0;; ++batch)
+            {
+                if (batch > 21)
+                {
+                    throw std::logic_error("make_graduation: no fixed point");
+                }
+                hip_check(hipMemsetAsync(flags, 0, size_t(n) * fs * 4, s), "hipMemsetAsync(flags)");
+                for (int p = 0; p < kPasses; ++p)
+                {
+                    dev::mesh_batch_graduation_pass(d_items, d_wpref, W, n, buf[cur], q, buf[1 - cur], w2, flags, fs, p, s);
+                    cur = 1 - cur;
+                }
+                dev::mesh_batch_differ(d_wpref, W, n, buf[cur], old, flags, fs, kPasses, s);
+                hip_check(hipMemcpyAsync(hf.data(), flags, hf.size() * 4, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(flags)");
+                ctx.sync();
+                bool all = true;
+                for (int j = 0; j < n && all; ++j)
+                {
+                    bool fixed = false;
+                    for (int p = 0; p < kPasses; ++p)
+                    {
+                        fixed = fixed || hf[size_t(j) * fs + p] == 0;
+                    }
+                    all = fixed;
+                }
+                if (all)
+                {
+                    break;
+                }
+            }
+            std::vector<bool> changed(static_cast<size_t>(n));
+            int64_t cap = 1024;
+            for (int j = 0; j < n; ++j)
+            {
+                changed[size_t(j)] = hf[size_t(j) * fs + kPasses] != 0;
+                if (changed[size_t(j)])
+                {
+                    cap += items[size_t(j)].cells.nruns + items[size_t(j)].cells.nruns / 4;
+                }
+            }
+            out.assign(size_t(n), CellArray());
+            if (cap == 1024 && std::none_of(changed.begin(), changed.end(), [](bool c) { return c; }))
+            {
+                return changed;
+            }
+            // the new leaves of the changed meshes as runs, one round trip
+            const size_t tbytes = dev::mesh_batch_scan_temp_bytes(W);
+            const size_t cbytes = (size_t(2 * std::max<int64_t>(W, 1)) * 4 + 255) & ~size_t(255);
+            DevBuffer scan;
+            char* sp         = static_cast<char*>(scan.get(cbytes + tbytes + size_t(cap) * 24, s));
+            int32_t* counts  = reinterpret_cast<int32_t*>(sp);
+            int32_t* offsets = counts + std::max<int64_t>(W, 1);
+            void* temp       = sp + cbytes;
+            int32_t* dr      = reinterpret_cast<int32_t*>(sp + cbytes + tbytes);
+            const uint64_t* leaves = buf[cur];
+            dev::mesh_batch_run_offsets(d_items, d_wpref, W, n, leaves, flags, fs, kPasses, counts, offsets, temp, tbytes, s);
+            dev::mesh_batch_run_emit(d_items, d_wpref, W, n, leaves, flags, fs, kPasses, offsets, dr, cap, s);
+            char* hr         = ctx.readback->acquire(8 + size_t(cap) * 24);
+            int32_t* last    = reinterpret_cast<int32_t*>(hr);
+            int32_t* runs    = reinterpret_cast<int32_t*>(hr + 8);
+            hip_check(hipMemcpyAsync(&last[0], offsets + W - 1, 4, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(runs)");
+            hip_check(hipMemcpyAsync(&last[1], counts + W - 1, 4, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(runs)");
+            hip_check(hipMemcpyAsync(runs, dr, size_t(cap) * 24, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(runs)");
+            ctx.sync();
+            const int64_t nruns = int64_t(last[0]) + last[1];
+            if (nruns > cap)
+            {
+                DevBuffer big;
+                int32_t* bd = static_cast<int32_t*>(big.get(size_t(nruns) * 24, s));
+                dev::mesh_batch_run_emit(d_items, d_wpref, W, n, leaves, flags, fs, kPasses, offsets, bd, nruns, s);
+                hr   = ctx.readback->acquire(8 + size_t(nruns) * 24);
+                runs = reinterpret_cast<int32_t*>(hr + 8);
+                hip_check(hipMemcpyAsync(runs, bd, size_t(nruns) * 24, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(runs)");
+                ctx.sync();
+            }
+            // the runs come in mesh order: each changed mesh is one segment
+            auto first = [&](int j)
+            {
+                int64_t lo = 0, hi = nruns;
+                while (lo < hi)
+                {
+                    const int64_t mid = (lo + hi) >> 1;
+                    if (runs[6 * mid + 5] < j)
+                    {
+                        lo = mid + 1;
+                    }
+                    else
+                    {
+                        hi = mid;
+                    }
+                }
+                return lo;
+            };
+            parallel_for(n,
+                         [&](int j)
+                         {
+                             if (!changed[size_t(j)])
+                             {
+                                 return;
+                             }
+                             CellArray& ca = out[size_t(j)];
+                             ca            = empty_cell_array(items[size_t(j)].dim);
+                             for (int64_t k = first(j), e = first(j + 1); k < e; ++k)
+                             {
+                                 const int32_t* r = runs + 6 * k;
+                                 ca[size_t(r[4])].push(r[2], r[3], r[0], r[1]);
+                             }
+                         });
+            return changed;
+        }
     } // namespace
 
     // ------------------------------------------------------------------
@@ -944,30 +1131,74 @@ namespace smr
                         continue;
                     }
                     work.push_back(k);
-                    hip_check(hipMemcpyAsync(host_tags_ + off[k], tag_arena_ + off[k], size_t(m.host.nref), hipMemcpyDeviceToHost, s),
-                              "hipMemcpyAsync(tags)");
                 }
-                ctx_->sync();
-                tt.reset();
-                parallel_for(int(work.size()),
-                             [&](int q)
-                             {
-                                 const size_t k      = work[size_t(q)];
-                                 const Mesh& m       = *inst_[size_t(act[k])].u->mesh;
-                                 const MeshConfig& c = m.host.cfg;
-                                 Rebuild& r          = rb[k];
-                                 CellArray ca        = update_cell_array_from_tag(m.host, host_tags_ + off[k]);
-                                 make_graduation(ca, c.dim, c.grad_width);
-                                 if (ca_equal(m.host.ids[CELLS], ca))
+                if (!batch_host_rebuild())
+                {
+                    // the mesh updates on the device, all instances per launch;
+                    // the host builds the changed meshes and their programs
+                    tt.reset();
+                    std::vector<const Mesh*> ms;
+                    std::vector<const uint8_t*> tg;
+                    for (size_t k : work)
+                    {
+                        ms.push_back(inst_[size_t(act[k])].u->mesh);
+                        tg.push_back(tag_arena_ + off[k]);
+                    }
+                    std::vector<CellArray> cas;
+                    std::vector<bool> ch;
+                    {
+                        ScopedTimer td(ctx_, "device mesh update");
+                        ch = ms.empty() ? std::vector<bool>() : device_update_batch(*ctx_, ms, tg, cas);
+                    }
+                    ScopedTimer th(ctx_, "host mesh and programs");
+                    parallel_for(int(work.size()),
+                                 [&](int q)
                                  {
-                                     r.converged = true;
-                                     return;
-                                 }
-                                 r.nm           = build_mesh(c, std::move(ca));
-                                 r.nm.graduated = true;
-                                 r.img = build_host_image(r.nm);
-                                 build_update_fields_lists(m.host, r.nm, r.copy, r.coarsen, r.refine);
-                             });
+                                     const size_t k      = work[size_t(q)];
+                                     const Mesh& m       = *inst_[size_t(act[k])].u->mesh;
+                                     const MeshConfig& c = m.host.cfg;
+                                     Rebuild& r          = rb[k];
+                                     if (!ch[size_t(q)])
+                                     {
+                                         r.converged = true;
+                                         return;
+                                     }
+                                     r.nm           = build_mesh(c, std::move(cas[size_t(q)]));
+                                     r.nm.graduated = true;
+                                     r.img          = build_host_image(r.nm);
+                                     build_update_fields_lists(m.host, r.nm, r.copy, r.coarsen, r.refine);
+                                 });
+                }
+                else
+                {
+                    for (size_t k : work)
+                    {
+                        const Mesh& m = *inst_[size_t(act[k])].u->mesh;
+                        hip_check(hipMemcpyAsync(host_tags_ + off[k], tag_arena_ + off[k], size_t(m.host.nref), hipMemcpyDeviceToHost, s),
+                                  "hipMemcpyAsync(tags)");
+                    }
+                    ctx_->sync();
+                    tt.reset();
+                    parallel_for(int(work.size()),
+                                 [&](int q)
+                                 {
+                                     const size_t k      = work[size_t(q)];
+                                     const Mesh& m       = *inst_[size_t(act[k])].u->mesh;
+                                     const MeshConfig& c = m.host.cfg;
+                                     Rebuild& r          = rb[k];
+                                     CellArray ca        = update_cell_array_from_tag(m.host, host_tags_ + off[k]);
+                                     make_graduation(ca, c.dim, c.grad_width);
+                                     if (ca_equal(m.host.ids[CELLS], ca))
+                                     {
+                                         r.converged = true;
+                                         return;
+                                     }
+                                     r.nm           = build_mesh(c, std::move(ca));
+                                     r.nm.graduated = true;
+                                     r.img          = build_host_image(r.nm);
+                                     build_update_fields_lists(m.host, r.nm, r.copy, r.coarsen, r.refine);
+                                 });
+                }
             }
             std::vector<size_t> changed;
             for (size_t k = 0; k < na; ++k)

@@ -69,6 +69,40 @@ namespace smr::dev
         int64_t wpr[21] = {};
     };
 
+    // One mesh of a batched device mesh update (rebuild_batch.hip): its bitmap
+    // layout, its resolved cells list, its tags, and the first word of its
+    // bitmaps in the concatenated arrays of the batch.
+    struct MeshItem
+    {
+        Bits b;
+        List cells;
+        const uint8_t* tag;
+        int64_t w0;
+        int dim, min_level, max_level;
+    };
+
+    void mesh_batch_zero(uint64_t* p, int64_t n, hipStream_t s);
+    // out = (old | add) & ~rem over n words (graduation.hpp:832-840)
+    void mesh_batch_combine(const uint64_t* old, const uint64_t* add, const uint64_t* rem, uint64_t* out, int64_t n, hipStream_t s);
+    // the leaves of every item's cells list (rpref: prefix of their runs)
+    void mesh_batch_leaves(const MeshItem* items, const int64_t* rpref, int64_t nruns, int n, uint64_t* bits, hipStream_t s);
+    // update_cell_array_from_tag's marks of every item (cpref: prefix of cells)
+    void mesh_batch_tag_marks(const MeshItem* items, const int64_t* cpref, int64_t ncells, int n, uint64_t* add, uint64_t* rem, hipStream_t s);
+    // one graduation pass of every item (wpref: prefix of bitmap words);
+    // flags[k * fstride + pass] = 1 when item k refines a cell
+    void mesh_batch_graduation_pass(const MeshItem* items, const int64_t* wpref, int64_t nwords, int n, const uint64_t* ca, uint64_t* p,
+                                    uint64_t* out, int w2, int* flags, int fstride, int pass, hipStream_t s);
+    // flags[k * fstride + slot] = 1 when item k's bitmaps a and c differ
+    void mesh_batch_differ(const int64_t* wpref, int64_t nwords, int n, const uint64_t* a, const uint64_t* c, int* flags, int fstride, int slot,
+                           hipStream_t s);
+    size_t mesh_batch_scan_temp_bytes(int64_t nwords);
+    // runs of the items whose flag `slot` is set: counts and their exclusive sum,
+    // then (x0, x1, y, z, level, item) records, at most cap
+    void mesh_batch_run_offsets(const MeshItem* items, const int64_t* wpref, int64_t nwords, int n, const uint64_t* bits, const int* flags,
+                                int fstride, int slot, int32_t* counts, int32_t* offsets, void* temp, size_t temp_bytes, hipStream_t s);
+    void mesh_batch_run_emit(const MeshItem* items, const int64_t* wpref, int64_t nwords, int n, const uint64_t* bits, const int* flags,
+                             int fstride, int slot, const int32_t* offsets, int32_t* runs, int64_t cap, hipStream_t s);
+
     // Per-level dense bitmaps of levels [0, hi] padded by `pad` cells on every
     // side (submesh.hip): level l has n[l] = 2^l + 2 pad cells per axis, a
     // row (Y, or (Z, Y) in 3D, padded indices) holds wpr[l] 64-bit words, x
