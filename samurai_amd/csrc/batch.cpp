@@ -234,12 +234,132 @@ namespace smr
             return v;
         }
 
+        // update_sub_mesh_impl (mr/mesh.hpp:221-455) of the meshes ms[j] with
+        // changed[j], from their new leaves (the concatenated unpadded bitmaps
+        // laid out by items), all meshes in each launch (submesh_batch.hip);
+        // the five ids come back as runs in one round trip (a second one only
+        // if the buffer, sized from the current meshes' runs, is too small),
+        // and meshes[j] is the complete new mesh
+        void construct_batch(Ctx& ctx, const std::vector<const Mesh*>& ms, const std::vector<dev::MeshItem>& items, const std::vector<bool>& changed,
+                             const uint64_t* leaves, std::vector<MRMeshData>& meshes)
+        {
+            ScopedTimer t(&ctx, "mesh construction");
+            hipStream_t s = ctx.stream;
+            std::vector<int> sel;
+            for (size_t j = 0; j < changed.size(); ++j)
+            {
+                if (changed[j])
+                {
+                    sel.push_back(int(j));
+                }
+            }
+            const int n = int(sel.size());
+            if (n == 0)
+            {
+                return;
+            }
+            std::vector<dev::SubmeshItem> si(static_cast<size_t>(n));
+            std::vector<int64_t> pp(size_t(n) + 1, 0);
+            int64_t cap = 4096;
+            for (int q = 0; q < n; ++q)
+            {
+                const size_t j        = size_t(sel[size_t(q)]);
+                const MeshConfig& c   = ms[j]->host.cfg;
+                dev::SubmeshItem& it  = si[size_t(q)];
+                it.b                  = items[j].b;
+                it.lw0                = items[j].w0;
+                it.p                  = dev::make_pbits(c.dim, c.max_level, std::max(c.radius, c.pred_s));
+                it.pw0                = pp[size_t(q)];
+                it.r                  = c.radius;
+                it.s                  = c.pred_s;
+                it.mr                 = c.max_level != c.min_level ? 1 : 0;
+                pp[size_t(q) + 1]     = pp[size_t(q)] + it.p.off[it.p.hi + 1];
+                int64_t cur           = 0;
+                for (const CellArray& ca : ms[j]->host.ids)
+                {
+                    for (const LevelSet& ls : ca)
+                    {
+                        cur += int64_t(ls.runs.size());
+                    }
+                }
+                cap += cur + cur / 8;
+            }
+            const int64_t PW = pp[size_t(n)];
+            const int64_t nw = dev::SM_NSETS * PW;
+            std::optional<ScopedTimer> td(std::in_place, &ctx, "mesh ids on device");
+            // the item table and its prefix, one upload
+            const size_t ib = (size_t(n) * sizeof(dev::SubmeshItem) + 255) & ~size_t(255);
+            const size_t pb = size_t(n + 1) * 8;
+            char* h         = ctx.staging->acquire(ib + pb);
+            std::memcpy(h, si.data(), size_t(n) * sizeof(dev::SubmeshItem));
+            std::memcpy(h + ib, pp.data(), pb);
+            DevBuffer tab;
+            char* dt = static_cast<char*>(tab.get(ib + pb, s));
+            hip_check(hipMemcpyAsync(dt, h, ib + pb, hipMemcpyHostToDevice, s), "hipMemcpyAsync(submesh items)");
+            ctx.staging->release(s);
+            const auto* d_items = reinterpret_cast<const dev::SubmeshItem*>(dt);
+            const auto* d_pp    = reinterpret_cast<const int64_t*>(dt + ib);
+            // the sets, two scratch maps per mesh, run counts and offsets, scan
+            // temp, the segment table, runs
+            const size_t tbytes = (dev::submesh_scan_temp_bytes(nw) + 255) & ~size_t(255);
+            const size_t sbytes = size_t(nw + 2 * PW) * 8;
+            const size_t cbytes = (size_t(2 * nw) * 4 + 255) & ~size_t(255);
+            const size_t gb     = (size_t(n) * dev::SM_NSETS * dev::SM_SEG * 4 + 255) & ~size_t(255);
+            const size_t rbytes = size_t(dev::SM_RUN) * 4;
+            DevBuffer work;
+            char* wp         = static_cast<char*>(work.get(sbytes + cbytes + tbytes + gb + size_t(cap) * rbytes + 256, s));
+            uint64_t* sets   = reinterpret_cast<uint64_t*>(wp);
+            int32_t* counts  = reinterpret_cast<int32_t*>(wp + sbytes);
+            int32_t* offsets = counts + nw;
+            void* temp       = wp + sbytes + cbytes;
+            int32_t* dseg    = reinterpret_cast<int32_t*>(wp + sbytes + cbytes + tbytes);
+            int32_t* dr      = reinterpret_cast<int32_t*>(wp + sbytes + cbytes + tbytes + gb);
+            dev::submesh_batch_build(d_items, d_pp, PW, n, leaves, sets, sets + nw, s);
+            dev::submesh_batch_run_offsets(d_items, d_pp, PW, n, sets, counts, offsets, temp, tbytes, s);
+            dev::submesh_batch_run_emit(d_items, d_pp, PW, n, sets, offsets, dr, cap, s);
+            dev::submesh_batch_segments(d_items, d_pp, n, offsets, counts, dseg, s);
+            // the segment table (its last entry is the run count), then the runs
+            char* hr      = ctx.readback->acquire(gb + size_t(cap) * rbytes);
+            int32_t* hs   = reinterpret_cast<int32_t*>(hr);
+            int32_t* runs = reinterpret_cast<int32_t*>(hr + gb);
+            hip_check(hipMemcpyAsync(hs, dseg, gb, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(submesh runs)");
+            hip_check(hipMemcpyAsync(runs, dr, size_t(cap) * rbytes, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(submesh runs)");
+            ctx.sync();
+            const int64_t nruns = hs[int64_t(n) * dev::SM_NSETS * dev::SM_SEG - 1];
+            if (nruns > cap)
+            {
+                DevBuffer big;
+                int32_t* bd = static_cast<int32_t*>(big.get(size_t(nruns) * rbytes, s));
+                dev::submesh_batch_run_emit(d_items, d_pp, PW, n, sets, offsets, bd, nruns, s);
+                hr   = ctx.readback->acquire(gb + size_t(nruns) * rbytes);
+                hs   = reinterpret_cast<int32_t*>(hr);
+                runs = reinterpret_cast<int32_t*>(hr + gb);
+                hip_check(hipMemcpyAsync(hs, dseg, gb, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(submesh runs)");
+                hip_check(hipMemcpyAsync(runs, bd, size_t(nruns) * rbytes, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(submesh runs)");
+                ctx.sync();
+            }
+            td.reset();
+            ScopedTimer tc(&ctx, "mesh completion");
+            std::vector<const MeshConfig*> cfgs(static_cast<size_t>(n));
+            std::vector<MRMeshData*> out(static_cast<size_t>(n));
+            for (int q = 0; q < n; ++q)
+            {
+                const size_t j  = size_t(sel[size_t(q)]);
+                cfgs[size_t(q)] = &ms[j]->host.cfg;
+                out[size_t(q)]  = &meshes[j];
+            }
+            meshes_from_runs(cfgs, runs, hs, out);
+        }
+
         // update_cell_array_from_tag and the make_graduation fixed point of
         // the meshes ms (tags[j] indexed by mesh j's storage) on the device,
         // all meshes in each launch (rebuild_batch.hip).  Returns, per mesh,
-        // whether its leaves change, and the new leaves of those that do.
+        // whether its leaves change, and the new leaves of those that do; with
+        // meshes, the changed meshes are built on the device too
+        // (construct_batch) and meshes[j] is the complete new mesh of mesh j
+        // (out is then left empty).
         std::vector<bool> device_update_batch(Ctx& ctx, const std::vector<const Mesh*>& ms, const std::vector<const uint8_t*>& tags,
-                                              std::vector<CellArray>& out)
+                                              std::vector<CellArray>& out, std::vector<MRMeshData>* meshes)
         {
             hipStream_t s = ctx.stream;
             const int n   = int(ms.size());
@@ -343,6 +463,12 @@ namespace smr
             out.assign(size_t(n), CellArray());
             if (cap == 1024 && std::none_of(changed.begin(), changed.end(), [](bool c) { return c; }))
             {
+                return changed;
+            }
+            if (meshes)
+            {
+                meshes->assign(size_t(n), MRMeshData());
+                construct_batch(ctx, ms, items, changed, buf[cur], *meshes);
                 return changed;
             }
             // the new leaves of the changed meshes as runs, one round trip
@@ -1134,8 +1260,10 @@ namespace smr
                 }
                 if (!batch_host_rebuild())
                 {
-                    // the mesh updates on the device, all instances per launch;
-                    // the host builds the changed meshes and their programs
+                    // the mesh update and the changed meshes' ids on the device,
+                    // all instances per launch; the host completes those meshes
+                    // (corners, renumbering) and builds their programs
+                    // (SMR_HOST_SUBMESH=1: their ids on the host)
                     tt.reset();
                     std::vector<const Mesh*> ms;
                     std::vector<const uint8_t*> tg;
@@ -1145,10 +1273,12 @@ namespace smr
                         tg.push_back(tag_arena_ + off[k]);
                     }
                     std::vector<CellArray> cas;
+                    std::vector<MRMeshData> nms;
                     std::vector<bool> ch;
+                    const bool dev_ids = !host_submesh();
                     {
                         ScopedTimer td(ctx_, "device mesh update");
-                        ch = ms.empty() ? std::vector<bool>() : device_update_batch(*ctx_, ms, tg, cas);
+                        ch = ms.empty() ? std::vector<bool>() : device_update_batch(*ctx_, ms, tg, cas, dev_ids ? &nms : nullptr);
                     }
                     ScopedTimer th(ctx_, "host mesh and programs");
                     parallel_for(int(work.size()),
@@ -1163,7 +1293,7 @@ namespace smr
                                          r.converged = true;
                                          return;
                                      }
-                                     r.nm           = build_mesh(c, std::move(cas[size_t(q)]));
+                                     r.nm           = dev_ids ? std::move(nms[size_t(q)]) : build_mesh(c, std::move(cas[size_t(q)]));
                                      r.nm.graduated = true;
                                      r.img          = build_host_image(r.nm);
                                      build_update_fields_lists(m.host, r.nm, r.copy, r.coarsen, r.refine);

@@ -136,6 +136,11 @@ namespace smr::dev
         return p;
     }
 
+    // int32 per run (x0, x1, y, z) and segment starts per set (levels 0..21)
+    // of the device mesh construction's output
+    constexpr int SM_RUN = 4;
+    constexpr int SM_SEG = 22;
+
     // the sets of the device mesh construction, in the order of the mesh ids
     enum SubmeshSet : int
     {
@@ -157,8 +162,43 @@ namespace smr::dev
     // per-word run counts of nsets consecutive sets and their exclusive sum
     void submesh_run_offsets(const PBits& p, int nsets, const uint64_t* sets, int32_t* counts, int32_t* offsets, void* temp, size_t temp_bytes,
                              hipStream_t s);
-    // the runs (x0, x1, y, z, level, set), 6 int32 each, of at most cap runs
+    // the runs (x0, x1, y, z), SM_RUN int32 each, of at most cap runs, in
+    // (set, level, z, y, x) order
     void submesh_run_emit(const PBits& p, int nsets, const uint64_t* sets, const int32_t* offsets, int32_t* runs, int64_t cap, hipStream_t s);
+    // seg[set * SM_SEG + l] = first run of level l of the set (levels past
+    // p.hi: the end of the set), for the nsets sets
+    void submesh_segments(const PBits& p, int nsets, const int32_t* offsets, const int32_t* counts, int32_t* seg, hipStream_t s);
+
+    // One mesh of a batched device mesh construction (submesh_batch.hip): its
+    // new leaves (unpadded layout b, from word lw0 of the concatenated
+    // leaves), its padded layout p (its SM_NSETS sets from word SM_NSETS * pw0
+    // of the concatenated sets, its two scratch maps from word 2 * pw0 of the
+    // scratch), its stencil radius r, prediction radius s and mr = max_level >
+    // min_level.
+    struct SubmeshItem
+    {
+        Bits b;
+        PBits p;
+        int64_t lw0, pw0;
+        int r, s, mr;
+    };
+
+    // submesh_build of every item (ppref: prefix of the items' padded words,
+    // pwords = ppref[n])
+    void submesh_batch_build(const SubmeshItem* items, const int64_t* ppref, int64_t pwords, int n, const uint64_t* leaves, uint64_t* sets,
+                             uint64_t* scratch, hipStream_t st);
+    // per-word run counts of the SM_NSETS * pwords words of the sets and
+    // their exclusive sum
+    void submesh_batch_run_offsets(const SubmeshItem* items, const int64_t* ppref, int64_t pwords, int n, const uint64_t* sets, int32_t* counts,
+                                   int32_t* offsets, void* temp, size_t temp_bytes, hipStream_t st);
+    // the runs (x0, x1, y, z), SM_RUN int32 each, of at most cap runs, in
+    // (item, set, level, z, y, x) order
+    void submesh_batch_run_emit(const SubmeshItem* items, const int64_t* ppref, int64_t pwords, int n, const uint64_t* sets, const int32_t* offsets,
+                                int32_t* runs, int64_t cap, hipStream_t st);
+    // seg[(SM_NSETS * item + set) * SM_SEG + l] = first run of level l of
+    // that set of that item (levels past its p.hi: the end of the set)
+    void submesh_batch_segments(const SubmeshItem* items, const int64_t* ppref, int n, const int32_t* offsets, const int32_t* counts, int32_t* seg,
+                                hipStream_t st);
 
 
     // One operator applied to a cell range of one instance: the unit of the

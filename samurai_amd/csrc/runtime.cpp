@@ -1357,16 +1357,6 @@ namespace smr
 
     namespace
     {
-        bool host_submesh()
-        {
-            static const bool v = []
-            {
-                const char* e = std::getenv("SMR_HOST_SUBMESH");
-                return e && std::atoi(e) != 0;
-            }();
-            return v;
-        }
-
         // the five ids of the mesh with the leaves `leaves` (unpadded bitmaps
         // of b), built on the device and read back as runs; the run buffer
         // is sized from the current mesh's runs, with a second round trip
@@ -1386,90 +1376,193 @@ namespace smr
                     cur_runs += int64_t(ls.runs.size());
                 }
             }
-            const int64_t cap     = cur_runs + cur_runs / 4 + 4096;
-            const size_t tbytes   = dev::submesh_scan_temp_bytes(nwords);
+            const int64_t cap     = cur_runs + cur_runs / 8 + 4096;
+            const size_t tbytes   = (dev::submesh_scan_temp_bytes(nwords) + 255) & ~size_t(255);
             const size_t sbytes   = size_t(std::max<int64_t>(nwords + 2 * total, 1)) * 8; // the sets, then 2 scratch maps
             const size_t cbytes   = (size_t(2 * std::max<int64_t>(nwords, 1)) * 4 + 255) & ~size_t(255);
+            constexpr size_t gb   = (size_t(dev::SM_NSETS * dev::SM_SEG) * 4 + 255) & ~size_t(255);
+            const size_t rbytes   = size_t(dev::SM_RUN) * 4;
             DevBuffer work;
-            char* wp         = static_cast<char*>(work.get(sbytes + cbytes + tbytes + size_t(cap) * 24 + 256, s));
+            char* wp         = static_cast<char*>(work.get(sbytes + cbytes + tbytes + gb + size_t(cap) * rbytes + 256, s));
             uint64_t* sets   = reinterpret_cast<uint64_t*>(wp);
             int32_t* counts  = reinterpret_cast<int32_t*>(wp + sbytes);
             int32_t* offsets = counts + std::max<int64_t>(nwords, 1);
             void* temp       = wp + sbytes + cbytes;
-            int32_t* dr      = reinterpret_cast<int32_t*>(wp + sbytes + cbytes + tbytes);
+            int32_t* dseg    = reinterpret_cast<int32_t*>(wp + sbytes + cbytes + tbytes);
+            int32_t* dr      = reinterpret_cast<int32_t*>(wp + sbytes + cbytes + tbytes + gb);
             dev::submesh_build(b, leaves, p, cfg.radius, cfg.pred_s, cfg.max_level != cfg.min_level, sets, sets + nwords, s);
             dev::submesh_run_offsets(p, dev::SM_NSETS, sets, counts, offsets, temp, tbytes, s);
             dev::submesh_run_emit(p, dev::SM_NSETS, sets, offsets, dr, cap, s);
-            // the run count (last offset + last count), then the runs
-            char* h           = ctx.readback->acquire(8 + size_t(cap) * 24);
-            int32_t* last     = reinterpret_cast<int32_t*>(h);
-            int32_t* hr       = reinterpret_cast<int32_t*>(h + 8);
-            hip_check(hipMemcpyAsync(&last[0], offsets + nwords - 1, 4, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(submesh)");
-            hip_check(hipMemcpyAsync(&last[1], counts + nwords - 1, 4, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(submesh)");
-            hip_check(hipMemcpyAsync(hr, dr, size_t(cap) * 24, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(submesh)");
+            dev::submesh_segments(p, dev::SM_NSETS, offsets, counts, dseg, s);
+            // the segment table (its last entry is the run count), then the runs
+            char* h      = ctx.readback->acquire(gb + size_t(cap) * rbytes);
+            int32_t* hs  = reinterpret_cast<int32_t*>(h);
+            int32_t* hr  = reinterpret_cast<int32_t*>(h + gb);
+            hip_check(hipMemcpyAsync(hs, dseg, gb, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(submesh)");
+            hip_check(hipMemcpyAsync(hr, dr, size_t(cap) * rbytes, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(submesh)");
             ctx.sync();
-            int64_t nruns = int64_t(last[0]) + last[1];
+            const int64_t nruns = hs[dev::SM_NSETS * dev::SM_SEG - 1];
             if (nruns > cap)
             {
                 DevBuffer rb;
-                int32_t* big = static_cast<int32_t*>(rb.get(size_t(nruns) * 24, s));
+                int32_t* big = static_cast<int32_t*>(rb.get(size_t(nruns) * rbytes, s));
                 dev::submesh_run_emit(p, dev::SM_NSETS, sets, offsets, big, nruns, s);
-                h  = ctx.readback->acquire(8 + size_t(nruns) * 24);
-                hr = reinterpret_cast<int32_t*>(h + 8);
-                hip_check(hipMemcpyAsync(hr, big, size_t(nruns) * 24, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(submesh)");
+                h  = ctx.readback->acquire(gb + size_t(nruns) * rbytes);
+                hs = reinterpret_cast<int32_t*>(h);
+                hr = reinterpret_cast<int32_t*>(h + gb);
+                hip_check(hipMemcpyAsync(hs, dseg, gb, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(submesh)");
+                hip_check(hipMemcpyAsync(hr, big, size_t(nruns) * rbytes, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(submesh)");
                 ctx.sync();
             }
             MRMeshData m;
+            meshes_from_runs({&cfg}, hr, hs, {&m});
+            return m;
+        }
+    } // namespace
+
+    bool host_submesh()
+    {
+        static const bool v = []
+        {
+            const char* e = std::getenv("SMR_HOST_SUBMESH");
+            return e && std::atoi(e) != 0;
+        }();
+        return v;
+    }
+
+    namespace
+    {
+        // level set `ls` from the runs [a, e): the runs are maximal and in
+        // (z, y, x) order, so the rows are counted first and every array is
+        // stored at its final size
+        void fill_level_set(LevelSet& ls, const int32_t* runs, int64_t a, int64_t e)
+        {
+            const int32_t* r = runs + dev::SM_RUN * a;
+            const int64_t nr = e - a;
+            int64_t rows     = 1;
+            for (int64_t k = 1; k < nr; ++k)
+            {
+                const int32_t* u = r + dev::SM_RUN * k;
+                rows += (u[2] != u[2 - dev::SM_RUN]) | (u[3] != u[3 - dev::SM_RUN]);
+            }
+            ls.runs.resize(size_t(nr));
+            ls.ry.resize(size_t(rows));
+            ls.rz.resize(size_t(rows));
+            ls.rp.resize(size_t(rows) + 1);
+            Run* ru    = ls.runs.data();
+            int32_t* y = ls.ry.data();
+            int32_t* z = ls.rz.data();
+            int64_t* o = ls.rp.data();
+            int64_t w  = -1;
+            for (int64_t k = 0; k < nr; ++k)
+            {
+                const int32_t* u = r + dev::SM_RUN * k;
+                if (w < 0 || u[2] != y[w] || u[3] != z[w])
+                {
+                    ++w;
+                    y[w] = u[2];
+                    z[w] = u[3];
+                    o[w] = k;
+                }
+                ru[k] = Run{u[0], u[1]};
+            }
+            o[rows] = nr;
+        }
+
+        void init_mesh(MRMeshData& m, const MeshConfig& cfg)
+        {
+            m     = MRMeshData();
             m.cfg = cfg;
             for (auto& ca : m.ids)
             {
                 ca = empty_cell_array(cfg.dim);
             }
-            // the runs come in (id, level, z, y, x) order: every level set is
-            // one segment, found by bisection and filled by a task of its own
-            const int nl = cfg.max_level + 1;
-            auto first   = [&](int id, int level)
-            {
-                int64_t lo = 0, hi = nruns;
-                while (lo < hi)
-                {
-                    const int64_t mid = (lo + hi) >> 1;
-                    const int32_t* r  = hr + 6 * mid;
-                    if (r[5] < id || (r[5] == id && r[4] < level))
-                    {
-                        lo = mid + 1;
-                    }
-                    else
-                    {
-                        hi = mid;
-                    }
-                }
-                return lo;
-            };
-            parallel_for(dev::SM_NSETS * nl,
-                         [&](int task)
-                         {
-                             const int id = task / nl, level = task % nl;
-                             const int64_t a = first(id, level), e = first(id, level + 1);
-                             if (e <= a)
-                             {
-                                 return;
-                             }
-                             LevelSet& ls = m.ids[size_t(id)][size_t(level)];
-                             ls.runs.reserve(size_t(e - a));
-                             ls.ry.reserve(size_t(e - a));
-                             ls.rz.reserve(size_t(e - a));
-                             ls.rp.reserve(size_t(e - a) + 1);
-                             for (int64_t k = a; k < e; ++k)
-                             {
-                                 const int32_t* r = hr + 6 * k;
-                                 ls.push(r[2], r[3], r[0], r[1]);
-                             }
-                         });
-            complete_mesh(m);
-            return m;
         }
     } // namespace
+
+    void meshes_from_runs(const std::vector<const MeshConfig*>& cfgs, const int32_t* runs, const int32_t* seg, const std::vector<MRMeshData*>& out)
+    {
+        const int n        = int(cfgs.size());
+        const int nthreads = HostPool::get().threads();
+        auto segment       = [&](int q, int id, int level)
+        {
+            const int32_t* sg = seg + (int64_t(q) * dev::SM_NSETS + id) * dev::SM_SEG;
+            return std::make_pair(int64_t(sg[level]), int64_t(sg[level + 1]));
+        };
+        auto levels = [&](int q) { return std::min(cfgs[size_t(q)]->max_level + 1, dev::SM_SEG - 1); };
+        if (n >= 2 * nthreads)
+        {
+            // many meshes: one task per mesh, the largest first
+            std::vector<int> order(static_cast<size_t>(n));
+            for (int q = 0; q < n; ++q)
+            {
+                order[size_t(q)] = q;
+            }
+            auto size = [&](int q) { return seg[(int64_t(q) + 1) * dev::SM_NSETS * dev::SM_SEG - 1] - seg[int64_t(q) * dev::SM_NSETS * dev::SM_SEG]; };
+            std::sort(order.begin(), order.end(), [&](int a, int b) { return size(a) > size(b); });
+            parallel_for(n,
+                         [&](int t)
+                         {
+                             const int q   = order[size_t(t)];
+                             MRMeshData& m = *out[size_t(q)];
+                             init_mesh(m, *cfgs[size_t(q)]);
+                             for (int id = 0; id < dev::SM_NSETS; ++id)
+                             {
+                                 for (int level = 0; level < levels(q); ++level)
+                                 {
+                                     const auto [a, e] = segment(q, id, level);
+                                     if (e > a)
+                                     {
+                                         fill_level_set(m.ids[size_t(id)][size_t(level)], runs, a, e);
+                                     }
+                                 }
+                             }
+                             complete_mesh(m);
+                         });
+            return;
+        }
+        // few meshes: every non-empty level set is a segment of the runs; the
+        // segments are split in chunks of about equal run counts, one task each
+        // (a task per segment would contend on the pool's task counter)
+        parallel_for(n, [&](int q) { init_mesh(*out[size_t(q)], *cfgs[size_t(q)]); });
+        struct Seg
+        {
+            int q, id, level;
+            int64_t a, e;
+        };
+        std::vector<Seg> segs;
+        std::vector<int64_t> work{0};
+        for (int q = 0; q < n; ++q)
+        {
+            for (int id = 0; id < dev::SM_NSETS; ++id)
+            {
+                for (int level = 0; level < levels(q); ++level)
+                {
+                    const auto [a, e] = segment(q, id, level);
+                    if (e > a)
+                    {
+                        segs.push_back({q, id, level, a, e});
+                        work.push_back(work.back() + (e - a) + 16);
+                    }
+                }
+            }
+        }
+        const int nchunks = int(std::min<size_t>(segs.size(), size_t(8 * nthreads)));
+        parallel_for(nchunks,
+                     [&](int c)
+                     {
+                         // the segments whose work prefix starts in [lo, hi)
+                         const int64_t lo = work.back() * c / nchunks, hi = work.back() * (c + 1) / nchunks;
+                         const size_t s0  = size_t(std::lower_bound(work.begin(), work.end() - 1, lo) - work.begin());
+                         const size_t s1  = size_t(std::lower_bound(work.begin(), work.end() - 1, hi) - work.begin());
+                         for (size_t t = s0; t < s1; ++t)
+                         {
+                             const Seg& g = segs[t];
+                             fill_level_set(out[size_t(g.q)]->ids[size_t(g.id)][size_t(g.level)], runs, g.a, g.e);
+                         }
+                     });
+        parallel_for(n, [&](int q) { complete_mesh(*out[size_t(q)]); });
+    }
 
     bool device_mesh_update(Ctx& ctx, DevMesh& dm, const MeshConfig& cfg, const uint8_t* tag, CellArray& out, const MRMeshData* current,
                             MRMeshData* mesh)

@@ -364,6 +364,13 @@ namespace smr
     // host from out instead.
     bool device_mesh_update(Ctx& ctx, DevMesh& dm, const MeshConfig& cfg, const uint8_t* tag, CellArray& out, const MRMeshData* current = nullptr,
                             MRMeshData* mesh = nullptr);
+    // SMR_HOST_SUBMESH=1: the mesh ids are built on the host from the leaves
+    bool host_submesh();
+    // the meshes whose ids are the runs (x0, x1, y, z; dev::SM_RUN int32
+    // each) of a device mesh construction: id `id` of mesh q at level l is
+    // the runs [seg[b + l], seg[b + l + 1]), b = (SM_NSETS q + id) SM_SEG;
+    // each mesh is completed with its corners and renumbering
+    void meshes_from_runs(const std::vector<const MeshConfig*>& cfgs, const int32_t* runs, const int32_t* seg, const std::vector<MRMeshData*>& out);
 
     // criteria thresholds of harten iteration `ite` (mr/adapt.hpp:325-339)
     dev::Thresholds make_thresholds(const MeshConfig& cfg, int ite, double eps, double regularity);
