@@ -216,6 +216,14 @@ namespace smr
             LevelSet out(A.dim, L);
             size_t i = 0, j = 0;
             const size_t na = a->nrows(), nb = b->nrows();
+            if (op == Op::Intersection)
+            {
+                out.reserve(std::min(na, nb), a->runs.size() + b->runs.size());
+            }
+            else
+            {
+                out.reserve(op == Op::Union ? na + nb : na, a->runs.size() + b->runs.size());
+            }
             while (i < na || j < nb)
             {
                 int which;
@@ -391,6 +399,7 @@ namespace smr
             const int k     = level - a.level;
             const int32_t f = int32_t(1) << k;
             const int32_t nz = a.dim == 3 ? f : 1;
+            out.reserve(a.nrows() * size_t(f) * size_t(nz), a.runs.size() * size_t(f) * size_t(nz));
             size_t r0        = 0;
             while (r0 < a.nrows())
             {
@@ -620,6 +629,7 @@ namespace smr
     LevelSet clip_box(const LevelSet& a, const std::array<int, 3>& lo, const std::array<int, 3>& hi)
     {
         LevelSet out(a.dim, a.level);
+        out.reserve(a.nrows(), a.runs.size());
         for (size_t r = 0; r < a.nrows(); ++r)
         {
             if (!row_in_box(a, r, lo, hi))
@@ -638,6 +648,7 @@ namespace smr
     LevelSet minus_box(const LevelSet& a, const std::array<int, 3>& lo, const std::array<int, 3>& hi)
     {
         LevelSet out(a.dim, a.level);
+        out.reserve(a.nrows(), 2 * a.runs.size());
         for (size_t r = 0; r < a.nrows(); ++r)
         {
             const bool in = row_in_box(a, r, lo, hi);

@@ -41,6 +41,15 @@ namespace smr
         }
 
         size_t nrows() const { return ry.size(); }
+        // room for `rows` rows and `n` runs, so that pushing them does not
+        // reallocate
+        void reserve(size_t rows, size_t n)
+        {
+            ry.reserve(rows);
+            rz.reserve(rows);
+            rp.reserve(rows + 1);
+            runs.reserve(n);
+        }
         bool empty() const { return runs.empty(); }
         int64_t count() const;
         bool operator==(const LevelSet& o) const;
