@@ -228,6 +228,13 @@ int smr_adapt_last_tags(const smr_adapt* adapt, uint8_t* tags, double* details);
  * the device performs before any operator kernel runs.  0 if valid,
  * SMR_ERR_RANGE otherwise. */
 int smr_mesh_validate(const smr_mesh* mesh);
+/* Diagnostics (no counterpart in the reference): after a mesh change made by
+ * the device, compares the work lists the device built for the mesh's image
+ * (projection, prediction, detail, criteria, maximum: runs, cell prefixes,
+ * chunk hints, per-level offsets) with those the host builds from the same
+ * mesh.  0 if identical, 1 if the image has no device-built lists,
+ * SMR_ERR_RUNTIME if they differ. */
+int smr_mesh_check_device_lists(smr_mesh* mesh);
 /* The HIP stream (hipStream_t) the context orders its work on, for timing
  * with HIP events; NULL for a host-only context. */
 void* smr_ctx_stream(smr_ctx* ctx);

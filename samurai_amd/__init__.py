@@ -77,6 +77,7 @@ def lib():
         _sig(L, "smr_ctx_stream", "argtypes", [vp])
         _sig(L, "smr_mesh_upwind_bytes", "restype", i64)
         _sig(L, "smr_mesh_upwind_bytes", "argtypes", [vp])
+        _sig(L, "smr_mesh_check_device_lists", "argtypes", [vp])
         _sig(L, "smr_mesh_ghost_update_bytes", "restype", i64)
         _sig(L, "smr_mesh_ghost_update_bytes", "argtypes", [vp])
         _sig(L, "smr_timers_enable", "argtypes", [i32])
@@ -267,6 +268,10 @@ class MRMesh(_Handle):
     def validate(self):
         """Host check that every kernel access of this mesh's programs is in bounds."""
         return _check(lib().smr_mesh_validate(self.h))
+
+    def check_device_lists(self):
+        """0 when the work lists the device built for this mesh equal the host's, 1 when it built none."""
+        return _check(lib().smr_mesh_check_device_lists(self.h))
 
     def refresh_config(self):
         """Re-read the configuration (the levels change when a restart file is loaded into the mesh)."""

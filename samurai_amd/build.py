@@ -17,7 +17,7 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 HOST_SOURCES = ["host/interval_set.cpp", "host/mr_mesh.cpp", "host/programs.cpp", "host/convection.cpp", "runtime.cpp", "batch.cpp", "adjoint.cpp", "capi.cpp"]
 HIP_SOURCES = ["device/kernels.hip", "device/program.hip", "device/noalloc.hip", "device/rebuild.hip", "device/rebuild_batch.hip", "device/submesh.hip",
-               "device/submesh_batch.hip"]
+               "device/submesh_batch.hip", "device/lists.hip"]
 
 CXXFLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall", "-Wextra",
             "-Wno-unused-parameter", f"-I{ROCM}/include", "-D__HIP_PLATFORM_AMD__"]

@@ -364,7 +364,7 @@ namespace smr
             in.ncells  = in.mesh->host.nb_cells(CELLS);
             in.off     = off;
             off += in.ncells;
-            in.mesh->device_ghost();
+            in.mesh->ensure_host_lists();
         }
         std::vector<const Mesh*> sorted(meshes.begin(), meshes.end());
         std::sort(sorted.begin(), sorted.end());

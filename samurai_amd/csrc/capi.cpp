@@ -580,6 +580,16 @@ extern "C"
             });
     }
 
+    int smr_mesh_check_device_lists(smr_mesh* mesh)
+    {
+        return guarded(
+            [&]
+            {
+                require(mesh != nullptr, "null mesh");
+                return smr::check_device_lists(mesh->m);
+            });
+    }
+
     int smr_field_create(smr_mesh* mesh, const char* name, smr_field** out)
     {
         return guarded(

@@ -169,6 +169,50 @@ namespace smr::dev
     // p.hi: the end of the set), for the nsets sets
     void submesh_segments(const PBits& p, int nsets, const int32_t* offsets, const int32_t* counts, int32_t* seg, hipStream_t s);
 
+    // The work lists of build_programs (host/programs.cpp) built on the device
+    // from the padded sets of a mesh (SM_NSETS x p.off[p.hi + 1] words) and,
+    // for the update_fields lists, of the mesh it replaces (lists.hip): each
+    // list is a per-level bitmap expression of the mesh ids, run-length
+    // encoded in (level, z, y, x) order into the arrays the host would build.
+    enum DevListKind : int
+    {
+        DL_PROJECTION = 0,
+        DL_PREDICTION = 1,
+        DL_DETAIL     = 2,
+        DL_CRITERIA   = 3,
+        DL_MAXIMUM    = 4,
+        DL_UF_COPY    = 5,
+        DL_UF_COARSEN = 6,
+        DL_UF_REFINE  = 7,
+        DL_N          = 8
+    };
+
+    // where list k's runs go: iv (5 int32 per run), pref (nruns + 1), hint
+    // (one per 2^HINT_SHIFT cells, plus one); run0, cell0: the list's first
+    // run and cell in the scanned counts
+    struct ListOut
+    {
+        int32_t* iv;
+        int64_t* pref;
+        int32_t* hint;
+        int64_t run0, cell0, nruns, ncells;
+    };
+    struct ListOuts
+    {
+        ListOut l[DL_N];
+    };
+
+    size_t lists_scan_temp_bytes(int64_t n);
+    // the bitmaps of the first nlists lists (nlists x total words; old_sets
+    // may be null when nlists <= DL_UF_COPY), per-word packed counts (runs <<
+    // 32 | cells) and their exclusive sum, and table[k * SM_SEG + l]: the
+    // packed scan at the first word of level l of list k (levels past p.hi:
+    // the list's end)
+    void lists_build(const PBits& p, int minl, int maxl, const uint64_t* sets, const uint64_t* old_sets, int nlists, uint64_t* bits,
+                     uint64_t* counts, uint64_t* scan, void* temp, size_t temp_bytes, uint64_t* table, hipStream_t s);
+    // writes the runs, prefixes and hints of every list into outs
+    void lists_emit(const PBits& p, int nlists, const uint64_t* bits, const uint64_t* scan, const ListOuts& outs, hipStream_t s);
+
     // One mesh of a batched device mesh construction (submesh_batch.hip): its
     // new leaves (unpadded layout b, from word lw0 of the concatenated
     // leaves), its padded layout p (its SM_NSETS sets from word SM_NSETS * pw0

@@ -55,6 +55,18 @@ namespace smr::dev::smops
             return x | (x << 1);
         }
 
+        // the 64 fine bits whose parents are the bits of v from bit 0 on, the
+        // first fine bit being the second child of its parent when o = 1
+        __device__ __forceinline__ uint64_t spread_parents(uint64_t v, int o)
+        {
+            uint64_t c = spread32(v) >> o;
+            if (o)
+            {
+                c |= ((v >> 32) & 1ull) << 63;
+            }
+            return c;
+        }
+
         struct PWord
         {
             int l;
@@ -286,16 +298,23 @@ namespace smr::dev::smops
                     // target bit j (unpadded x = 64 wx - pad + j) has the parent (x >> 1)
                     const int64_t x0 = 64 * q.wx - pad;
                     const int64_t p0 = x0 >> 1;
-                    const int o      = int(x0 & 1);
                     const uint64_t v = load64(r, p.wpr[t], p0 + pad) & load64(u, p.wpr[t], p0 + pad);
-                    uint64_t c       = spread32(v) >> o;
-                    if (o)
-                    {
-                        c |= ((v >> 32) & 1ull) << 63;
-                    }
-                    R[p.off[t + 1] + i] |= c & tail_mask(p, t + 1, q.wx);
+                    R[p.off[t + 1] + i] |= spread_parents(v, int(x0 & 1)) & tail_mask(p, t + 1, q.wx);
                 }
             }
+        }
+
+        // set[t - 1].on(t) for the 64 bits of word wx of padded row (Y, Z) at level t
+        __device__ __forceinline__ uint64_t refine_word(const PBits& p, const uint64_t* set, int t, int64_t Y, int64_t Z, int64_t wx)
+        {
+            const int pad     = p.pad;
+            const uint64_t* r = prow(p, set, t - 1, ((Y - pad) >> 1) + pad, p.dim == 3 ? ((Z - pad) >> 1) + pad : 0);
+            if (!r)
+            {
+                return 0ull;
+            }
+            const int64_t x0 = 64 * wx - pad;
+            return spread_parents(load64(r, p.wpr[t - 1], (x0 >> 1) + pad), int(x0 & 1));
         }
 
         // projection steps t = 0 .. t1 - 1 by the threads of one block

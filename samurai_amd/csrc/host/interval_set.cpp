@@ -629,18 +629,65 @@ namespace smr
     LevelSet clip_box(const LevelSet& a, const std::array<int, 3>& lo, const std::array<int, 3>& hi)
     {
         LevelSet out(a.dim, a.level);
-        out.reserve(a.nrows(), a.runs.size());
-        for (size_t r = 0; r < a.nrows(); ++r)
+        const int32_t zlo = a.dim == 3 ? int32_t(lo[2]) : 0, zhi = a.dim == 3 ? int32_t(hi[2]) : 1;
+        if (lo[0] >= hi[0] || lo[1] >= hi[1] || zlo >= zhi)
         {
-            if (!row_in_box(a, r, lo, hi))
+            return out;
+        }
+        // first row at or after (y, z) in (z, y) order
+        auto first_row = [&](size_t from, int32_t y, int32_t z)
+        {
+            size_t l = from, h = a.nrows();
+            while (l < h)
             {
+                const size_t mid = (l + h) / 2;
+                if (row_less(a.ry[mid], a.rz[mid], y, z))
+                {
+                    l = mid + 1;
+                }
+                else
+                {
+                    h = mid;
+                }
+            }
+            return l;
+        };
+        // the rows of the box: one range of rows per plane, found by bisection
+        size_t r = first_row(0, int32_t(lo[1]), zlo);
+        while (r < a.nrows() && a.rz[r] < zhi)
+        {
+            if (a.ry[r] < lo[1])
+            {
+                r = first_row(r, int32_t(lo[1]), a.rz[r]);
                 continue;
             }
-            for (int64_t k = a.rp[r]; k < a.rp[r + 1]; ++k)
+            if (a.ry[r] >= hi[1])
+            {
+                r = first_row(r, int32_t(lo[1]), a.rz[r] + 1);
+                continue;
+            }
+            // the runs of the row that meet [lo[0], hi[0]): the first run
+            // ending after lo[0] by bisection, then until a run starts at hi[0]
+            int64_t k = a.rp[r], ke = a.rp[r + 1];
+            int64_t kh = ke;
+            while (k < kh)
+            {
+                const int64_t mid = (k + kh) / 2;
+                if (a.runs[size_t(mid)].e <= lo[0])
+                {
+                    k = mid + 1;
+                }
+                else
+                {
+                    kh = mid;
+                }
+            }
+            for (; k < ke && a.runs[size_t(k)].s < hi[0]; ++k)
             {
                 const Run& run = a.runs[size_t(k)];
                 out.push(a.ry[r], a.rz[r], std::max(run.s, int32_t(lo[0])), std::min(run.e, int32_t(hi[0])));
             }
+            ++r;
         }
         return out;
     }

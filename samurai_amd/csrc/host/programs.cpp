@@ -199,7 +199,46 @@ namespace smr
             return b;
         }
 
-        // update_outer_ghosts(level) (algorithm/update_outer_ghost.hpp:338-432)
+        // the smallest box holding a (lo = hi when a is empty)
+        Bounds set_bounds(const LevelSet& a)
+        {
+            Bounds b{{0, 0, 0}, {0, 0, 0}};
+            if (a.empty())
+            {
+                return b;
+            }
+            b.lo = {a.runs[0].s, a.ry[0], a.rz[0]};
+            b.hi = {a.runs[0].e, a.ry[0] + 1, a.rz[0] + 1};
+            for (size_t r = 0; r < a.nrows(); ++r)
+            {
+                b.lo[0] = std::min(b.lo[0], int(a.runs[size_t(a.rp[r])].s));
+                b.hi[0] = std::max(b.hi[0], int(a.runs[size_t(a.rp[r + 1] - 1)].e));
+                b.lo[1] = std::min(b.lo[1], int(a.ry[r]));
+                b.hi[1] = std::max(b.hi[1], int(a.ry[r]) + 1);
+                b.lo[2] = std::min(b.lo[2], int(a.rz[r]));
+                b.hi[2] = std::max(b.hi[2], int(a.rz[r]) + 1);
+            }
+            return b;
+        }
+
+        // a n b for a set a much smaller than b: b is clipped to the box of a
+        // first (the same set, without a pass over all of b)
+        LevelSet small_intersection(const LevelSet& a, const LevelSet& b)
+        {
+            if (a.level != b.level)
+            {
+                throw std::logic_error("small_intersection: sets of different levels");
+            }
+            const Bounds box = set_bounds(a);
+            return set_intersection(a, clip_box(b, box.lo, box.hi));
+        }
+
+        // update_outer_ghosts(level) (algorithm/update_outer_ghost.hpp:338-432).
+        // Every set below involves only cells next to the boundary: the
+        // leaves, union cells and reference sets are clipped to the boundary
+        // layer (or to the box of a corner) before the set algebra, which gives
+        // the same sets as the whole-level expressions (the leaves and union
+        // cells lie in the domain) at the cost of the boundary cells only.
         Recipes outer_recipes(const MRMeshData& m, int level, const std::vector<Dir>& cart)
         {
             Recipes rec;
@@ -214,7 +253,7 @@ namespace smr
                     // corner extrapolation, ghost width 1 (bc/apply_field_bc.hpp:315-466):
                     // the diagonal ghost copies the corner cell
                     LevelSet corner = on(m.corners[c], level);
-                    LevelSet cc     = set_intersection(cells[size_t(level)], corner);
+                    LevelSet cc     = small_intersection(corner, cells[size_t(level)]);
                     cc.for_each_run(
                         [&](int32_t x0, int32_t x1, int32_t y, int32_t z)
                         {
@@ -227,11 +266,11 @@ namespace smr
                     // project_corner_below (algorithm/update_outer_ghost.hpp:267-336)
                     if (level > 0)
                     {
-                        LevelSet fine_outer = set_intersection(translate(corner, d), ref[size_t(level)]);
+                        LevelSet fine_outer = small_intersection(translate(corner, d), ref[size_t(level)]);
                         for (int dl = 1; dl <= 2; ++dl)
                         {
                             const int pl   = level - dl;
-                            LevelSet ghost = set_intersection(on(fine_outer, pl), ref[size_t(pl)]);
+                            LevelSet ghost = small_intersection(on(fine_outer, pl), ref[size_t(pl)]);
                             const int last = (1 << dl) - 1;
                             ghost.for_each_run(
                                 [&](int32_t x0, int32_t x1, int32_t y, int32_t z)
@@ -262,9 +301,10 @@ namespace smr
                 if (level < m.cfg.max_level)
                 {
                     // project_bc, layer 1 (algorithm/update_outer_ghost.hpp:20-152)
-                    const Bounds dom = domain_bounds(level);
-                    LevelSet outside = minus_box(translate(m.ids[UNION_CELLS][size_t(level)], dir), dom.lo, dom.hi);
-                    LevelSet ghosts  = set_intersection(outside, ref[size_t(level)]);
+                    const Bounds dom  = domain_bounds(level);
+                    const Bounds face = domain_layer(level, dir);
+                    LevelSet outside  = minus_box(translate(clip_box(m.ids[UNION_CELLS][size_t(level)], face.lo, face.hi), dir), dom.lo, dom.hi);
+                    LevelSet ghosts   = small_intersection(outside, ref[size_t(level)]);
                     std::vector<int64_t> ch;
                     ghosts.for_each_run(
                         [&](int32_t x0, int32_t x1, int32_t y, int32_t z)
@@ -318,11 +358,13 @@ namespace smr
                 if (level < m.cfg.max_level && level >= m.cfg.min_level)
                 {
                     // predict_bc, order 0 (algorithm/update_outer_ghost.hpp:154-262)
-                    const Bounds dom     = domain_bounds(level);
-                    const Bounds shifted = domain_bounds(level, {-dir[0], -dir[1], -dir[2]});
-                    LevelSet inner       = minus_box(cells[size_t(level)], shifted.lo, shifted.hi);
-                    LevelSet bc_ghosts   = minus_box(translate(inner, dir), dom.lo, dom.hi);
-                    LevelSet ghosts     = set_intersection(bc_ghosts, ref[size_t(level + 1)]);
+                    // the leaves outside the domain shifted by -dir are those of
+                    // the boundary layer in direction dir
+                    const Bounds dom   = domain_bounds(level);
+                    const Bounds face  = domain_layer(level, dir);
+                    LevelSet inner     = clip_box(cells[size_t(level)], face.lo, face.hi);
+                    LevelSet bc_ghosts = minus_box(translate(inner, dir), dom.lo, dom.hi);
+                    LevelSet ghosts    = small_intersection(on(bc_ghosts, level + 1), ref[size_t(level + 1)]);
                     ghosts.for_each_run(
                         [&](int32_t x0, int32_t x1, int32_t y, int32_t z)
                         {
@@ -374,11 +416,10 @@ namespace smr
         {
             const int dim = om.cfg.dim, s = om.cfg.pred_s;
             const int minl = om.cfg.min_level, maxl = om.cfg.max_level;
-            w = WorkList();
+            w      = WorkList();
+            w.taps = list_taps(LIST_UF_COPY + which, dim, s);
             if (which == 0)
             {
-                w.taps.push_back({0, 0, 0, 0, 0, 0, 0});
-                w.taps.push_back({1, 0, 0, 0, 0, 0, 0});
                 for (int level = minl; level <= maxl; ++level)
                 {
                     w.add(sets.cp[size_t(level)]);
@@ -386,8 +427,6 @@ namespace smr
             }
             else if (which == 1)
             {
-                w.taps.push_back({0, 0, 0, 0, 0, 0, 0});
-                add_row_taps(w, 1, 1, 0, 0, 1, dim, 0, 1);
                 for (int level = minl + 1; level <= maxl; ++level)
                 {
                     w.add(sets.co[size_t(level)]);
@@ -395,8 +434,6 @@ namespace smr
             }
             else
             {
-                add_row_taps(w, 0, 1, 0, 0, 1, dim, 0, 1);
-                add_row_taps(w, 1, 0, int8_t(-s), -s, s, dim, 0, int8_t(2 * s));
                 for (int level = minl + 1; level <= maxl; ++level)
                 {
                     w.add(sets.re[size_t(level)]);
@@ -474,8 +511,7 @@ namespace smr
                              {
                                  // projection (numeric/projection.hpp:15-65): runs at the coarse level
                                  WorkList& w = gp->projection;
-                                 w.taps.push_back({0, 0, 0, 0, 0, 0, 0});
-                                 add_row_taps(w, 0, 1, 0, 0, 1, dim, 0, 1);
+                                 w.taps      = list_taps(LIST_PROJECTION, dim, s);
                                  for (int level = 1; level <= maxl; ++level)
                                  {
                                      w.add(sets.proj[size_t(level)]);
@@ -486,8 +522,7 @@ namespace smr
                              {
                                  // prediction <s,false> (numeric/prediction.hpp:259-357)
                                  WorkList& w = gp->prediction;
-                                 w.taps.push_back({0, 0, 0, 0, 0, 0, 0});
-                                 add_row_taps(w, 0, -1, int8_t(-s), -s, s, dim, 0, int8_t(2 * s));
+                                 w.taps      = list_taps(LIST_PREDICTION, dim, s);
                                  for (int level = 1; level <= maxl; ++level)
                                  {
                                      w.add(sets.pred[size_t(level)]);
@@ -498,8 +533,7 @@ namespace smr
                              {
                                  // compute_detail (mr/operators.hpp:139-540): fine rows, then coarse stencil rows
                                  WorkList& w = ap->detail;
-                                 add_row_taps(w, 0, 1, 0, 0, 1, dim, 0, 1);
-                                 add_row_taps(w, 0, 0, int8_t(-s), -s, s, dim, 0, int8_t(2 * s));
+                                 w.taps      = list_taps(LIST_DETAIL, dim, s);
                                  for (int level = l0; level < maxl; ++level)
                                  {
                                      w.add(sets.det[size_t(level)]);
@@ -512,8 +546,7 @@ namespace smr
                                  // the criteria sets are those of the fine levels [min, max], the
                                  // maximum sets those of [1, max]; both are (cells[l] n reference[l-1]).on(l-1)
                                  WorkList& w = task == 3 ? ap->criteria : ap->maximum;
-                                 w.taps.push_back({0, 0, 0, 0, 0, 0, 0});
-                                 add_row_taps(w, 0, 1, 0, 0, 1, dim, 0, 1);
+                                 w.taps      = list_taps(task == 3 ? LIST_CRITERIA : LIST_MAXIMUM, dim, s);
                                  for (int level = task == 3 ? std::max(1, minl) : 1; level <= maxl; ++level)
                                  {
                                      w.add(sets.crit[size_t(level)]);
@@ -537,6 +570,72 @@ namespace smr
             return sets;
         }
     } // namespace
+
+    std::vector<Tap> list_taps(int kind, int dim, int s)
+    {
+        WorkList w;
+        switch (kind)
+        {
+            case LIST_PROJECTION:
+                // numeric/projection.hpp:15-65: the cell, then the fine rows
+                w.taps.push_back({0, 0, 0, 0, 0, 0, 0});
+                add_row_taps(w, 0, 1, 0, 0, 1, dim, 0, 1);
+                break;
+            case LIST_PREDICTION:
+                // prediction <s,false> (numeric/prediction.hpp:259-357): the
+                // ghost, then the coarse stencil rows
+                w.taps.push_back({0, 0, 0, 0, 0, 0, 0});
+                add_row_taps(w, 0, -1, int8_t(-s), -s, s, dim, 0, int8_t(2 * s));
+                break;
+            case LIST_DETAIL:
+                // compute_detail (mr/operators.hpp:139-540): fine rows, then coarse stencil rows
+                add_row_taps(w, 0, 1, 0, 0, 1, dim, 0, 1);
+                add_row_taps(w, 0, 0, int8_t(-s), -s, s, dim, 0, int8_t(2 * s));
+                break;
+            case LIST_CRITERIA:
+            case LIST_MAXIMUM:
+                // mr_criteria / maximum (mr/criteria.hpp:13-122, mr/operators.hpp:22-96)
+                w.taps.push_back({0, 0, 0, 0, 0, 0, 0});
+                add_row_taps(w, 0, 1, 0, 0, 1, dim, 0, 1);
+                break;
+            case LIST_UF_COPY:
+                // update_fields (algorithm/update_fields.hpp:27-127): new cell, old cell
+                w.taps.push_back({0, 0, 0, 0, 0, 0, 0});
+                w.taps.push_back({1, 0, 0, 0, 0, 0, 0});
+                break;
+            case LIST_UF_COARSEN:
+                w.taps.push_back({0, 0, 0, 0, 0, 0, 0});
+                add_row_taps(w, 1, 1, 0, 0, 1, dim, 0, 1);
+                break;
+            case LIST_UF_REFINE:
+                add_row_taps(w, 0, 1, 0, 0, 1, dim, 0, 1);
+                add_row_taps(w, 1, 0, int8_t(-s), -s, s, dim, 0, int8_t(2 * s));
+                break;
+            default: throw std::logic_error("list_taps: unknown list");
+        }
+        return std::move(w.taps);
+    }
+
+    void build_recipes_and_cells(const MRMeshData& m, std::vector<Recipes>& outer, WorkList& cells)
+    {
+        const auto cart = cartesian_directions(m.cfg.dim);
+        const int nl    = m.cfg.max_level + 1;
+        outer.assign(size_t(nl), Recipes());
+        parallel_for(nl + 1,
+                     [&](int task)
+                     {
+                         if (task == nl)
+                         {
+                             cells = build_cells_list(m);
+                         }
+                         else
+                         {
+                             // the finest levels first: they hold the most boundary cells
+                             const int level   = nl - 1 - task;
+                             outer[size_t(level)] = outer_recipes(m, level, cart);
+                         }
+                     });
+    }
 
     GhostProgram build_ghost_program(const MRMeshData& m)
     {

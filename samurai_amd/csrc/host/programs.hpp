@@ -93,6 +93,24 @@ namespace smr
         WorkList maximum;  // same sets as criteria
     };
 
+    // the work lists of the programs, in the order of the device-built lists
+    enum ListKind : int
+    {
+        LIST_PROJECTION = 0,
+        LIST_PREDICTION = 1,
+        LIST_DETAIL     = 2,
+        LIST_CRITERIA   = 3,
+        LIST_MAXIMUM    = 4,
+        LIST_UF_COPY    = 5,
+        LIST_UF_COARSEN = 6,
+        LIST_UF_REFINE  = 7
+    };
+    // the taps of a list of that kind (dim, prediction radius s)
+    std::vector<Tap> list_taps(int kind, int dim, int s);
+    // update_outer_ghosts' recipes of every level and the cells list (the
+    // parts of the programs the device does not build)
+    void build_recipes_and_cells(const MRMeshData& m, std::vector<Recipes>& outer, WorkList& cells);
+
     GhostProgram build_ghost_program(const MRMeshData& m);
     AdaptProgram build_adapt_program(const MRMeshData& m);
     WorkList build_cells_list(const MRMeshData& m);             // mesh[cells], taps for upwind

@@ -635,6 +635,34 @@ namespace smr
         commit_list(l, prepare_list(l));
     }
 
+    void Packer::device_list(DevWorkList& l, const int32_t* iv, const int64_t* pref, const int32_t* hint, int64_t nruns, int64_t ncells)
+    {
+        std::vector<int8_t> taps;
+        for (const Tap& t : l.host.taps)
+        {
+            taps.insert(taps.end(), {t.table, t.dl, t.dx, t.dy, t.dz, t.wlo, t.whi});
+        }
+        const int ntaps   = int(l.host.taps.size());
+        const size_t ot   = b_.add(std::move(taps));
+        const size_t obas = b_.scratch(size_t(nruns) * size_t(ntaps) * 8);
+        const size_t ocr  = b_.scratch(size_t(ncells) * 4);
+        Blob* b           = &b_;
+        binds_.push_back(
+            [=, &l]
+            {
+                l.d         = dev::List{};
+                l.d.iv      = iv;
+                l.d.pref    = pref;
+                l.d.hint    = hint;
+                l.d.taps    = reinterpret_cast<const int8_t*>(b->dptr(ot));
+                l.d.base    = reinterpret_cast<int64_t*>(b->sptr(obas));
+                l.d.cellrun = ncells > 0 ? reinterpret_cast<int32_t*>(b->sptr(ocr)) : nullptr;
+                l.d.nruns   = nruns;
+                l.d.ncells  = ncells;
+                l.d.ntaps   = ntaps;
+            });
+    }
+
     void Packer::lists(const std::vector<DevWorkList*>& ls)
     {
         std::vector<ListPrep> preps(ls.size());
@@ -669,6 +697,18 @@ namespace smr
 
     void DevWorkList::cells_of_levels(int l0, int l1, int64_t& c0, int64_t& c1) const
     {
+        if (device_built)
+        {
+            l0 = std::max(l0, 0);
+            l1 = std::min(l1, MAXL);
+            c0 = c1 = 0;
+            if (l1 > l0)
+            {
+                c0 = level_cell[size_t(l0)];
+                c1 = level_cell[size_t(l1)];
+            }
+            return;
+        }
         int64_t r0    = 0;
         int64_t nr    = host.runs_of_levels(l0, l1, r0);
         c0            = host.pref[size_t(r0)];
@@ -793,6 +833,171 @@ namespace smr
             ctx.check_resolve();
         }
         return dm;
+    }
+
+    std::unique_ptr<DevMesh> upload_image_device_lists(Ctx& ctx, const MRMeshData& m, std::vector<Recipes>&& outer, WorkList&& cells,
+                                                       DeviceLists&& dl, const dev::Table* old_table)
+    {
+        const bool uf = dl.nlists > dev::DL_UF_COPY;
+        if (uf && !old_table)
+        {
+            throw std::logic_error("update_fields lists need the table of the mesh they come from");
+        }
+        ctx.require_device();
+        auto dm        = std::make_unique<DevMesh>();
+        dm->table_blob = std::make_unique<Blob>();
+        Packer p(*dm->table_blob);
+        p.table(m, dm->table);
+        dm->outer.resize(outer.size());
+        for (size_t l = 0; l < outer.size(); ++l)
+        {
+            dm->outer[l].host = std::move(outer[l]);
+            p.recipes(dm->outer[l]);
+        }
+        dm->cells.host = std::move(cells);
+        p.list(dm->cells);
+        DevWorkList* dls[dev::DL_N] = {&dm->projection, &dm->prediction, &dm->detail, &dm->criteria, &dm->maximum, &dm->uf[0], &dm->uf[1], &dm->uf[2]};
+        for (int k = 0; k < dl.nlists; ++k)
+        {
+            const DeviceLists::One& o = dl.l[k];
+            DevWorkList& w            = *dls[k];
+            w.host                    = WorkList();
+            w.host.taps               = list_taps(k, m.cfg.dim, m.cfg.pred_s);
+            w.host.level_run          = o.level_run;
+            w.level_cell              = o.level_cell;
+            w.device_built            = true;
+            p.device_list(w, o.iv, o.pref, o.hint, o.nruns, o.ncells);
+        }
+        dm->list_mem      = std::move(dl.mem);
+        dm->mesh_sets     = std::move(dl.sets);
+        dm->has_mesh_sets = dm->mesh_sets.p != nullptr;
+        dm->has_ghost = dm->has_adapt = true;
+        dm->has_uf    = uf;
+        finish_staged(ctx, p, *dm->table_blob);
+        // every list of the image (and its update_fields lists) in one launch
+        dev::ResolveBatch rb{};
+        rb.t_new = dm->table;
+        rb.t_old = old_table ? *old_table : dm->table;
+        rb.n     = 0;
+        auto add = [&](const DevWorkList& l, bool old)
+        {
+            rb.lists[rb.n]     = l.d;
+            rb.old[rb.n]       = old ? 1 : 0;
+            rb.start[rb.n + 1] = rb.start[rb.n] + dev::resolve_range(l.d);
+            ++rb.n;
+        };
+        for (DevWorkList* l : dm->lists())
+        {
+            add(*l, false);
+        }
+        if (uf)
+        {
+            for (const DevWorkList& l : dm->uf)
+            {
+                add(l, true);
+            }
+        }
+        dev::resolve_batch(rb, ctx.d_err, ctx.stream);
+        dev::walk_rows(dm->cells.d, ctx.stream);
+        return dm;
+    }
+
+    int check_device_lists(Mesh& m)
+    {
+        if (!m.dev || !m.dev->projection.device_built)
+        {
+            return 1;
+        }
+        DevMesh& dm = *m.dev;
+        WorkList cells;
+        GhostProgram gp;
+        AdaptProgram ap;
+        build_programs(m.host, cells, gp, ap);
+        const char* names[5] = {"projection", "prediction", "detail", "criteria", "maximum"};
+        DevWorkList* d[5]    = {&dm.projection, &dm.prediction, &dm.detail, &dm.criteria, &dm.maximum};
+        const WorkList* h[5] = {&gp.projection, &gp.prediction, &ap.detail, &ap.criteria, &ap.maximum};
+        for (int k = 0; k < 5; ++k)
+        {
+            const WorkList& w = *h[k];
+            const dev::List& l = d[k]->d;
+            auto fail = [&](const char* what) { throw std::logic_error(std::string("device-built ") + names[k] + " list: " + what + " differs"); };
+            if (l.nruns != w.nruns() || l.ncells != w.ncells())
+            {
+                fail("size");
+            }
+            if (d[k]->host.taps.size() != w.taps.size())
+            {
+                fail("taps");
+            }
+            std::vector<int32_t> iv(w.iv.size());
+            std::vector<int64_t> pref(w.pref.size());
+            const int64_t nchunks = (w.ncells() + (int64_t(1) << dev::HINT_SHIFT) - 1) >> dev::HINT_SHIFT;
+            std::vector<int32_t> hint(size_t(nchunks) + 1);
+            hip_check(hipMemcpyAsync(iv.data(), l.iv, iv.size() * 4, hipMemcpyDeviceToHost, m.ctx->stream), "hipMemcpyAsync(check)");
+            hip_check(hipMemcpyAsync(pref.data(), l.pref, pref.size() * 8, hipMemcpyDeviceToHost, m.ctx->stream), "hipMemcpyAsync(check)");
+            hip_check(hipMemcpyAsync(hint.data(), l.hint, hint.size() * 4, hipMemcpyDeviceToHost, m.ctx->stream), "hipMemcpyAsync(check)");
+            m.ctx->sync();
+            if (iv != w.iv)
+            {
+                fail("runs");
+            }
+            if (pref != w.pref)
+            {
+                fail("cell prefix");
+            }
+            int64_t r = 0;
+            for (int64_t j = 0; j < nchunks; ++j)
+            {
+                while (w.pref[size_t(r + 1)] <= (j << dev::HINT_SHIFT))
+                {
+                    ++r;
+                }
+                if (hint[size_t(j)] != r)
+                {
+                    fail("hint");
+                }
+            }
+            if (hint[size_t(nchunks)] != int32_t(std::max<int64_t>(w.nruns() - 1, 0)))
+            {
+                fail("last hint");
+            }
+            for (int l0 = 0; l0 < MAXL; ++l0)
+            {
+                int64_t a, b, r0;
+                d[k]->cells_of_levels(l0, l0 + 1, a, b);
+                const int64_t nr = w.runs_of_levels(l0, l0 + 1, r0);
+                if (a != w.pref[size_t(r0)] || b != w.pref[size_t(r0 + nr)] || d[k]->host.level_run[size_t(l0)] != w.level_run[size_t(l0)])
+                {
+                    fail("level offsets");
+                }
+            }
+        }
+        return 0;
+    }
+
+    void Mesh::ensure_host_lists()
+    {
+        DevMesh& dm = device_adapt();
+        if (!dm.projection.device_built && !dm.detail.device_built)
+        {
+            return;
+        }
+        WorkList cells;
+        GhostProgram gp;
+        AdaptProgram ap;
+        build_programs(host, cells, gp, ap);
+        DevWorkList* d[5]    = {&dm.projection, &dm.prediction, &dm.detail, &dm.criteria, &dm.maximum};
+        WorkList* h[5]       = {&gp.projection, &gp.prediction, &ap.detail, &ap.criteria, &ap.maximum};
+        for (int k = 0; k < 5; ++k)
+        {
+            if (h[k]->nruns() != d[k]->d.nruns || h[k]->ncells() != d[k]->d.ncells)
+            {
+                throw std::logic_error("device-built work list differs from the host's");
+            }
+            d[k]->host         = std::move(*h[k]);
+            d[k]->device_built = false;
+            d[k]->level_cell.clear();
+        }
     }
 
     DevMesh::DevMesh()
@@ -1361,7 +1566,69 @@ namespace smr
         // of b), built on the device and read back as runs; the run buffer
         // is sized from the current mesh's runs, with a second round trip
         // only if that is too small
-        MRMeshData device_submesh(Ctx& ctx, const MeshConfig& cfg, const dev::Bits& b, const uint64_t* leaves, const MRMeshData& current)
+        // allocates the arrays of the device-built lists from their table
+        // (packed runs << 32 | cells scans at the first word of every level)
+        // and launches the kernel that writes them
+        void emit_lists(Ctx& ctx, const dev::PBits& p, int nlists, const uint64_t* bits, const uint64_t* scan, const std::vector<uint64_t>& table,
+                        DeviceLists& dl)
+        {
+            auto runs_of  = [](uint64_t v) { return int64_t(v >> 32); };
+            auto cells_of = [](uint64_t v) { return int64_t(v & 0xFFFFFFFFull); };
+            auto align    = [](size_t n) { return (n + 255) & ~size_t(255); };
+            dl.nlists     = nlists;
+            size_t bytes  = 0;
+            std::vector<size_t> off(size_t(nlists) * 3);
+            for (int k = 0; k < nlists; ++k)
+            {
+                const uint64_t* t  = table.data() + size_t(k) * dev::SM_SEG;
+                DeviceLists::One& o = dl.l[k];
+                o.nruns  = runs_of(t[dev::SM_SEG - 1]) - runs_of(t[0]);
+                o.ncells = cells_of(t[dev::SM_SEG - 1]) - cells_of(t[0]);
+                o.level_run.resize(dev::SM_SEG);
+                o.level_cell.resize(dev::SM_SEG);
+                for (int l = 0; l < dev::SM_SEG; ++l)
+                {
+                    o.level_run[size_t(l)]  = runs_of(t[l]) - runs_of(t[0]);
+                    o.level_cell[size_t(l)] = cells_of(t[l]) - cells_of(t[0]);
+                }
+                const int64_t nchunks = (o.ncells + (int64_t(1) << dev::HINT_SHIFT) - 1) >> dev::HINT_SHIFT;
+                off[size_t(3 * k)]     = bytes;
+                bytes += align(size_t(std::max<int64_t>(o.nruns, 1)) * 20);
+                off[size_t(3 * k + 1)] = bytes;
+                bytes += align(size_t(o.nruns + 1) * 8);
+                off[size_t(3 * k + 2)] = bytes;
+                bytes += align(size_t(nchunks + 1) * 4);
+            }
+            char* d = static_cast<char*>(dl.mem.get(bytes, ctx.stream));
+            dev::ListOuts outs{};
+            for (int k = 0; k < nlists; ++k)
+            {
+                const uint64_t* t  = table.data() + size_t(k) * dev::SM_SEG;
+                DeviceLists::One& o = dl.l[k];
+                dev::ListOut& w    = outs.l[k];
+                w.iv               = reinterpret_cast<int32_t*>(d + off[size_t(3 * k)]);
+                w.pref             = reinterpret_cast<int64_t*>(d + off[size_t(3 * k + 1)]);
+                w.hint             = reinterpret_cast<int32_t*>(d + off[size_t(3 * k + 2)]);
+                w.run0             = runs_of(t[0]);
+                w.cell0            = cells_of(t[0]);
+                w.nruns            = o.nruns;
+                w.ncells           = o.ncells;
+                o.iv               = w.iv;
+                o.pref             = w.pref;
+                o.hint             = w.hint;
+            }
+            dev::lists_emit(p, nlists, bits, scan, outs, ctx.stream);
+        }
+
+        // the five ids of the mesh with the leaves `leaves` (unpadded bitmaps
+        // of b), built on the device and read back as runs; the run buffer
+        // is sized from the current mesh's runs, with a second round trip
+        // only if that is too small.  With dl: the work lists of the new mesh
+        // too (and its update_fields lists from old_sets, the padded ids of
+        // the current mesh, when given), their per-level offsets read back in
+        // the same round trip; dl keeps the new padded ids.
+        MRMeshData device_submesh(Ctx& ctx, const MeshConfig& cfg, const dev::Bits& b, const uint64_t* leaves, const MRMeshData& current,
+                                  const uint64_t* old_sets, DeviceLists* dl)
         {
             hipStream_t s          = ctx.stream;
             const int pad          = std::max(cfg.radius, cfg.pred_s);
@@ -1378,29 +1645,62 @@ namespace smr
             }
             const int64_t cap     = cur_runs + cur_runs / 8 + 4096;
             const size_t tbytes   = (dev::submesh_scan_temp_bytes(nwords) + 255) & ~size_t(255);
-            const size_t sbytes   = size_t(std::max<int64_t>(nwords + 2 * total, 1)) * 8; // the sets, then 2 scratch maps
+            const size_t kbytes   = size_t(std::max<int64_t>(2 * total, 1)) * 8; // 2 scratch maps
             const size_t cbytes   = (size_t(2 * std::max<int64_t>(nwords, 1)) * 4 + 255) & ~size_t(255);
             constexpr size_t gb   = (size_t(dev::SM_NSETS * dev::SM_SEG) * 4 + 255) & ~size_t(255);
             const size_t rbytes   = size_t(dev::SM_RUN) * 4;
+            DevBuffer sets_buf;
+            uint64_t* sets = static_cast<uint64_t*>(sets_buf.get(size_t(std::max<int64_t>(nwords, 1)) * 8, s));
             DevBuffer work;
-            char* wp         = static_cast<char*>(work.get(sbytes + cbytes + tbytes + gb + size_t(cap) * rbytes + 256, s));
-            uint64_t* sets   = reinterpret_cast<uint64_t*>(wp);
-            int32_t* counts  = reinterpret_cast<int32_t*>(wp + sbytes);
+            char* wp         = static_cast<char*>(work.get(kbytes + cbytes + tbytes + gb + size_t(cap) * rbytes + 256, s));
+            uint64_t* scr    = reinterpret_cast<uint64_t*>(wp);
+            int32_t* counts  = reinterpret_cast<int32_t*>(wp + kbytes);
             int32_t* offsets = counts + std::max<int64_t>(nwords, 1);
-            void* temp       = wp + sbytes + cbytes;
-            int32_t* dseg    = reinterpret_cast<int32_t*>(wp + sbytes + cbytes + tbytes);
-            int32_t* dr      = reinterpret_cast<int32_t*>(wp + sbytes + cbytes + tbytes + gb);
-            dev::submesh_build(b, leaves, p, cfg.radius, cfg.pred_s, cfg.max_level != cfg.min_level, sets, sets + nwords, s);
+            void* temp       = wp + kbytes + cbytes;
+            int32_t* dseg    = reinterpret_cast<int32_t*>(wp + kbytes + cbytes + tbytes);
+            int32_t* dr      = reinterpret_cast<int32_t*>(wp + kbytes + cbytes + tbytes + gb);
+            dev::submesh_build(b, leaves, p, cfg.radius, cfg.pred_s, cfg.max_level != cfg.min_level, sets, scr, s);
             dev::submesh_run_offsets(p, dev::SM_NSETS, sets, counts, offsets, temp, tbytes, s);
             dev::submesh_run_emit(p, dev::SM_NSETS, sets, offsets, dr, cap, s);
             dev::submesh_segments(p, dev::SM_NSETS, offsets, counts, dseg, s);
-            // the segment table (its last entry is the run count), then the runs
-            char* h      = ctx.readback->acquire(gb + size_t(cap) * rbytes);
-            int32_t* hs  = reinterpret_cast<int32_t*>(h);
-            int32_t* hr  = reinterpret_cast<int32_t*>(h + gb);
+            // the work lists: bitmaps, packed counts, their scan, per-level table
+            const int nlists     = dl ? (old_sets ? int(dev::DL_N) : int(dev::DL_UF_COPY)) : 0;
+            const int64_t lwords = total * nlists;
+            const size_t ltb     = nlists ? (dev::lists_scan_temp_bytes(lwords) + 255) & ~size_t(255) : 0;
+            const size_t lgb     = (size_t(std::max(nlists, 1)) * dev::SM_SEG * 8 + 255) & ~size_t(255);
+            DevBuffer lwork;
+            uint64_t *lbits = nullptr, *lscan = nullptr, *ltable = nullptr;
+            if (nlists)
+            {
+                char* lp           = static_cast<char*>(lwork.get(size_t(3 * lwords) * 8 + ltb + lgb + 256, s));
+                lbits              = reinterpret_cast<uint64_t*>(lp);
+                uint64_t* lcounts  = lbits + lwords;
+                lscan              = lcounts + lwords;
+                void* ltemp        = lp + size_t(3 * lwords) * 8;
+                ltable             = reinterpret_cast<uint64_t*>(lp + size_t(3 * lwords) * 8 + ltb);
+                dev::lists_build(p, cfg.min_level, cfg.max_level, sets, old_sets, nlists, lbits, lcounts, lscan, ltemp, ltb, ltable, s);
+            }
+            // the list table, the segment table (its last entry is the run
+            // count), then the runs
+            char* h      = ctx.readback->acquire(lgb + gb + size_t(cap) * rbytes);
+            uint64_t* ht = reinterpret_cast<uint64_t*>(h);
+            int32_t* hs  = reinterpret_cast<int32_t*>(h + lgb);
+            int32_t* hr  = reinterpret_cast<int32_t*>(h + lgb + gb);
+            if (nlists)
+            {
+                hip_check(hipMemcpyAsync(ht, ltable, size_t(nlists) * dev::SM_SEG * 8, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(lists)");
+            }
             hip_check(hipMemcpyAsync(hs, dseg, gb, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(submesh)");
             hip_check(hipMemcpyAsync(hr, dr, size_t(cap) * rbytes, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(submesh)");
             ctx.sync();
+            if (nlists)
+            {
+                // the lists' sizes and levels; their runs are written while the
+                // host builds the mesh
+                const std::vector<uint64_t> table(ht, ht + size_t(nlists) * dev::SM_SEG);
+                emit_lists(ctx, p, nlists, lbits, lscan, table, *dl);
+                dl->sets = std::move(sets_buf);
+            }
             const int64_t nruns = hs[dev::SM_NSETS * dev::SM_SEG - 1];
             if (nruns > cap)
             {
@@ -1419,6 +1719,16 @@ namespace smr
             return m;
         }
     } // namespace
+
+    bool host_lists()
+    {
+        static const bool v = []
+        {
+            const char* e = std::getenv("SMR_HOST_LISTS");
+            return e && std::atoi(e) != 0;
+        }();
+        return v;
+    }
 
     bool host_submesh()
     {
@@ -1565,7 +1875,7 @@ namespace smr
     }
 
     bool device_mesh_update(Ctx& ctx, DevMesh& dm, const MeshConfig& cfg, const uint8_t* tag, CellArray& out, const MRMeshData* current,
-                            MRMeshData* mesh)
+                            MRMeshData* mesh, DeviceLists* lists)
     {
         hipStream_t s       = ctx.stream;
         const dev::Bits b   = make_bits(cfg);
@@ -1634,7 +1944,22 @@ namespace smr
         if (mesh && current && !host_submesh())
         {
             ScopedTimer t(&ctx, "mesh construction");
-            *mesh           = device_submesh(ctx, cfg, b, buf[cur], *current);
+            DevBuffer old_buf;
+            const uint64_t* old_sets = nullptr;
+            if (lists && dm.has_mesh_sets)
+            {
+                old_sets = static_cast<const uint64_t*>(dm.mesh_sets.p);
+            }
+            else if (lists)
+            {
+                // the padded ids of the current mesh, from its leaves
+                const dev::PBits p  = dev::make_pbits(cfg.dim, cfg.max_level, std::max(cfg.radius, cfg.pred_s));
+                const int64_t total = p.off[p.hi + 1];
+                uint64_t* o = static_cast<uint64_t*>(old_buf.get(size_t(std::max<int64_t>((dev::SM_NSETS + 2) * total, 1)) * 8, s));
+                dev::submesh_build(b, old, p, cfg.radius, cfg.pred_s, cfg.max_level != cfg.min_level, o, o + dev::SM_NSETS * total, s);
+                old_sets = o;
+            }
+            *mesh           = device_submesh(ctx, cfg, b, buf[cur], *current, old_sets, lists);
             mesh->graduated = true;
             return true;
         }
@@ -1798,16 +2123,18 @@ namespace smr
         DevMesh& dm = *m.dev;
         std::unique_ptr<DevMesh> ndev;
         MRMeshData nm;
+        DeviceLists dl;
         {
             ScopedTimer timer(&ctx, "mesh update");
             st.last_size      = n;
             st.host_tag_valid = false;
             // mesh update (graduation.hpp:743-843, 573-726): on the device,
             // the host receives the new leaves only when they change (one
-            // round trip when they do not)
+            // round trip when they do not); the new mesh's ids and work lists
+            // are built on the device too
             CellArray new_ca;
             nm.nref = -1; // set when the device builds the whole mesh
-            if (!device_mesh_update(ctx, dm, cfg, st.tag, new_ca, &m.host, &nm))
+            if (!device_mesh_update(ctx, dm, cfg, st.tag, new_ca, &m.host, &nm, host_lists() ? nullptr : &dl))
             {
                 return true;
             }
@@ -1818,6 +2145,21 @@ namespace smr
                 nm.graduated = true;
             }
         }
+        if (dl.nlists == int(dev::DL_N))
+        {
+            // the outer-ghost recipes and the cells list on the host, then one
+            // allocation for the image; the resolve is checked with the
+            // update_fields lists
+            std::vector<Recipes> outer;
+            WorkList cells;
+            {
+                ScopedTimer t4(nullptr, "program build");
+                build_recipes_and_cells(nm, outer, cells);
+            }
+            ScopedTimer timer(&ctx, "device upload");
+            ndev = upload_image_device_lists(ctx, nm, std::move(outer), std::move(cells), std::move(dl), &dm.table);
+        }
+        else
         {
             // the complete image of the new mesh (cells, ghost and adapt
             // programs) in one allocation; its resolve is checked together with

@@ -116,6 +116,18 @@ namespace smr
         DevBuffer()     = default;
         DevBuffer(const DevBuffer&)            = delete;
         DevBuffer& operator=(const DevBuffer&) = delete;
+        DevBuffer(DevBuffer&& o) noexcept : p(o.p), s(o.s) { o.p = nullptr; }
+        DevBuffer& operator=(DevBuffer&& o) noexcept
+        {
+            if (this != &o)
+            {
+                reset();
+                p   = o.p;
+                s   = o.s;
+                o.p = nullptr;
+            }
+            return *this;
+        }
         ~DevBuffer() { reset(); }
         void reset()
         {
@@ -140,6 +152,10 @@ namespace smr
         dev::List d{};
         std::vector<int32_t> hint;
         std::unique_ptr<Blob> blob;
+        // built on the device: host holds the taps and the first run of every
+        // level only, level_cell the first cell of every level (MAXL + 1)
+        bool device_built = false;
+        std::vector<int64_t> level_cell;
         // cell range [c0, c1) of the runs of levels [l0, l1)
         void cells_of_levels(int l0, int l1, int64_t& c0, int64_t& c1) const;
     };
@@ -208,6 +224,12 @@ namespace smr
         // replaced, packed with the image when it was built for a mesh change
         bool has_uf = false;
         DevWorkList uf[3];
+        // the arrays of the work lists built on the device, and the padded
+        // bitmaps of the mesh ids of this version (the old mesh of the next
+        // device-built update_fields lists)
+        DevBuffer list_mem;
+        DevBuffer mesh_sets;
+        bool has_mesh_sets = false;
         // the work lists of a complete image (cells first)
         std::vector<DevWorkList*> lists() { return {&cells, &projection, &prediction, &detail, &criteria, &maximum}; }
     };
@@ -223,6 +245,9 @@ namespace smr
         // several lists, their host arrays prepared in parallel
         void lists(const std::vector<DevWorkList*>& ls);
         void recipes(DevRecipes& r);
+        // a list whose runs, prefixes and hints the device built (at iv,
+        // pref, hint); its taps come from l.host
+        void device_list(DevWorkList& l, const int32_t* iv, const int64_t* pref, const int32_t* hint, int64_t nruns, int64_t ncells);
         void finish(hipStream_t s);
 
       private:
@@ -249,6 +274,9 @@ namespace smr
         DevMesh& device();          // table + cells list (uploaded and resolved)
         DevMesh& device_ghost();    // + ghost program
         DevMesh& device_adapt();    // + adapt program
+        // the host arrays of work lists the device built (for host code that
+        // reads the lists' runs)
+        void ensure_host_lists();
         void replace(MRMeshData&& m, std::unique_ptr<DevMesh>&& d);
     };
 
@@ -353,6 +381,8 @@ namespace smr
     // DevWorkList upload + resolve against the given tables
     void upload_list(Ctx& ctx, DevWorkList& l, const dev::Table& t0, const dev::Table& t1);
 
+    struct DeviceLists;
+
     // bitmap layout of the levels of a mesh configuration
     dev::Bits make_bits(const MeshConfig& cfg);
     // update_cell_array_from_tag + make_graduation on the device from tags
@@ -362,10 +392,42 @@ namespace smr
     // (update_sub_mesh_impl, mr/mesh.hpp:221-455) and *mesh is the complete
     // new mesh (out is then left empty); SMR_HOST_SUBMESH=1 builds them on the
     // host from out instead.
+    // With lists too: the work lists of the new mesh are built on the device
+    // (with the update_fields lists from dm's mesh) into *lists.
     bool device_mesh_update(Ctx& ctx, DevMesh& dm, const MeshConfig& cfg, const uint8_t* tag, CellArray& out, const MRMeshData* current = nullptr,
-                            MRMeshData* mesh = nullptr);
+                            MRMeshData* mesh = nullptr, DeviceLists* lists = nullptr);
     // SMR_HOST_SUBMESH=1: the mesh ids are built on the host from the leaves
     bool host_submesh();
+    // SMR_HOST_LISTS=1: the work lists of a mesh change are built on the host
+    bool host_lists();
+
+    // the work lists of a new mesh built on the device with its ids: the
+    // arrays of every list (runs, prefixes, hints) in mem, per list its counts
+    // and the first run and cell of every level; sets: the padded ids
+    struct DeviceLists
+    {
+        struct One
+        {
+            const int32_t* iv   = nullptr;
+            const int64_t* pref = nullptr;
+            const int32_t* hint = nullptr;
+            int64_t nruns = 0, ncells = 0;
+            std::vector<int64_t> level_run, level_cell;
+        };
+        int nlists = 0;
+        One l[dev::DL_N];
+        DevBuffer mem;
+        DevBuffer sets;
+    };
+    // the image of a mesh whose projection, prediction, detail, criteria,
+    // maximum and update_fields lists the device built (dl); the host gives
+    // the recipes and the cells list
+    // 0 when the projection, prediction, detail, criteria and maximum lists
+    // the device built for m's image equal the host's (throws where they
+    // differ), 1 when the image has no device-built lists
+    int check_device_lists(Mesh& m);
+    std::unique_ptr<DevMesh> upload_image_device_lists(Ctx& ctx, const MRMeshData& m, std::vector<Recipes>&& outer, WorkList&& cells,
+                                                       DeviceLists&& dl, const dev::Table* old_table);
     // the meshes whose ids are the runs (x0, x1, y, z; dev::SM_RUN int32
     // each) of a device mesh construction: id `id` of mesh q at level l is
     // the runs [seg[b + l], seg[b + l + 1]), b = (SM_NSETS q + id) SM_SEG;

@@ -36,9 +36,11 @@ def plain_step(sa, inst, adapt=True):
 
 
 def assert_same(p, b, what):
+    import samurai_amd as sa
     for k, (x, y) in enumerate(zip(p, b)):
-        cx, cy = x["mesh"].cells(), y["mesh"].cells()
-        assert np.array_equal(cx, cy), f"{what}: instance {k} cells differ"
+        for mid in (sa.CELLS, sa.CELLS_AND_GHOSTS, sa.PROJ_CELLS, sa.UNION_CELLS, sa.REFERENCE):
+            cx, cy = x["mesh"].cells(mid), y["mesh"].cells(mid)
+            assert np.array_equal(cx, cy), f"{what}: instance {k} mesh id {mid} differs"
         for name in ("u", "unp1"):
             sx, sy = x[name].storage(), y[name].storage()
             assert sx.shape == sy.shape, f"{what}: instance {k} {name} storage size"
@@ -68,6 +70,22 @@ def test_batch_steps_match_plain(gpu_ctx, cpt, mode):
         gpu_ctx.synchronize()
         assert_same(plain, bat, f"mode {mode} step {step}")
     assert batch.nb_cells() == sum(x["mesh"].nb_cells() for x in plain)
+
+
+def test_batch_mesh_ids_match_host_construction(gpu_ctx):
+    """The batched device construction (update_sub_mesh_impl of every changed
+    instance per launch) against the host's construction of the same leaves:
+    all five ids and the storage numbering, after each of four steps."""
+    import samurai_amd as sa
+    plain, bat, batch = setup_pair(sa, gpu_ctx, [61, 62, 63, 64, 65], (4, 8))
+    for step in range(4):
+        batch.step(EPS, REG)
+        gpu_ctx.synchronize()
+        for k, y in enumerate(bat):
+            m = y["mesh"]
+            host = sa.make_mesh_from_cells(gpu_ctx, m.cfg, m.cells())
+            for mid in (sa.CELLS, sa.CELLS_AND_GHOSTS, sa.PROJ_CELLS, sa.UNION_CELLS, sa.REFERENCE):
+                assert np.array_equal(host.cells(mid), m.cells(mid)), (step, k, mid)
 
 
 @pytest.mark.parametrize("block,cpt", [(64, 1), (128, 8), (1024, 2)])

@@ -77,3 +77,42 @@ def test_device_mesh_update_at_the_fixed_point(gpu_ctx):
     before = m.cells(sa.REFERENCE)
     assert m.rebuild_from_device_tags(t.data_ptr(), n) == 1
     assert np.array_equal(before, m.cells(sa.REFERENCE))
+
+
+@pytest.mark.parametrize("dim,lmin,lmax,seed", [(2, 2, 8, 4), (2, 4, 10, 1), (3, 2, 6, 0)])
+def test_device_built_work_lists_match_host(gpu_ctx, dim, lmin, lmax, seed):
+    """A harten call that changes the mesh builds the new mesh's work lists on
+    the device (projection, prediction, detail, criteria, maximum and the
+    update_fields lists): each must equal the host's build_programs list of
+    the same mesh, run for run, with the same cell prefix, chunk hints and
+    per-level offsets, and the time steps must stay those of the oracle."""
+    print("device lists", dim, lmin, lmax, seed, flush=True)
+    cfg = sa.mesh_config(dim, lmin, lmax, 1)
+    mesh = sa.make_mesh(gpu_ctx, cfg)
+    u = sa.make_scalar_field("u", mesh)
+    c = mesh.cells()
+    u.set_cells(seeded_2d(c, seed) if dim == 2 else sphere_3d(c, radius=0.25))
+    sa.make_bc_dirichlet(u, 0.0)
+    o = OracleSim(dim, lmin, lmax, pred_s=1)
+    o.set_poison(False)
+    oc = o.cells()
+    o.set_values(seeded_2d(oc, seed) if dim == 2 else sphere_3d(oc, radius=0.25))
+    o.set_dirichlet(0.0)
+    unp1 = sa.make_scalar_field("unp1", mesh)
+    adapt = sa.make_MRAdapt(u)
+    a = np.array([1.0, 0.5, 0.25][:dim])
+    dt = 0.4 / dim / (1 << lmax)
+    checked = 0
+    for step in range(3):
+        assert o.adapt(1e-3, 2.0) == adapt(1e-3, 2.0), step
+        rc = mesh.check_device_lists()
+        assert rc in (0, 1), rc
+        checked += rc == 0
+        o.upwind_step(a, dt)
+        sa.update_ghost_mr(u)
+        unp1.resize()
+        sa.upwind(a, dt, u, unp1)
+        sa.swap(u, unp1)
+        assert np.array_equal(o.cells(), mesh.cells()), step
+        assert np.array_equal(o.values().view(np.uint64), u.cells().view(np.uint64)), step
+    assert checked > 0
