@@ -222,6 +222,12 @@ namespace smr
             MRMeshData nm;
             HostImage img;
             WorkList copy, coarsen, refine;
+            // work lists built on the device: the host adds the recipes and
+            // the cells list
+            bool device_lists = false;
+            std::array<DeviceLists::One, dev::DL_N> dl;
+            std::vector<Recipes> outer;
+            WorkList cells;
         };
 
         bool batch_host_rebuild()
@@ -234,14 +240,104 @@ namespace smr
             return v;
         }
 
+        // the arrays of the device-built lists of the meshes sel[q] from their
+        // table (packed runs << 32 | cells scans at the first word of every
+        // level of every list), one allocation for all, and the launch that
+        // writes them
+        void emit_batch_lists(Ctx& ctx, const dev::SubmeshItem* d_items, const int64_t* d_pp, int64_t PW, int n, const uint64_t* bits,
+                              const uint64_t* scan, const uint64_t* ht, const std::vector<int>& sel,
+                              std::vector<std::array<DeviceLists::One, dev::DL_N>>& lists, std::shared_ptr<DevBuffer>& mem)
+        {
+            hipStream_t s = ctx.stream;
+            auto runs_of  = [](uint64_t v) { return int64_t(v >> 32); };
+            auto cells_of = [](uint64_t v) { return int64_t(v & 0xFFFFFFFFull); };
+            auto align    = [](size_t b) { return (b + 255) & ~size_t(255); };
+            const std::vector<uint64_t> table(ht, ht + size_t(n) * dev::DL_N * dev::SM_SEG);
+            std::vector<dev::ListOut> outs(size_t(n) * dev::DL_N);
+            std::vector<size_t> off(size_t(n) * dev::DL_N * 3);
+            size_t bytes = 0;
+            for (int q = 0; q < n; ++q)
+            {
+                std::array<DeviceLists::One, dev::DL_N>& ls = lists[size_t(sel[size_t(q)])];
+                for (int k = 0; k < dev::DL_N; ++k)
+                {
+                    const size_t e       = size_t(q) * dev::DL_N + size_t(k);
+                    const uint64_t* t    = table.data() + e * dev::SM_SEG;
+                    DeviceLists::One& o  = ls[size_t(k)];
+                    o.nruns  = runs_of(t[dev::SM_SEG - 1]) - runs_of(t[0]);
+                    o.ncells = cells_of(t[dev::SM_SEG - 1]) - cells_of(t[0]);
+                    o.level_run.resize(dev::SM_SEG);
+                    o.level_cell.resize(dev::SM_SEG);
+                    for (int l = 0; l < dev::SM_SEG; ++l)
+                    {
+                        o.level_run[size_t(l)]  = runs_of(t[l]) - runs_of(t[0]);
+                        o.level_cell[size_t(l)] = cells_of(t[l]) - cells_of(t[0]);
+                    }
+                    for (int l = 1; l < dev::SM_SEG; ++l)
+                    {
+                        if (o.level_run[size_t(l)] < o.level_run[size_t(l - 1)] || o.level_cell[size_t(l)] < o.level_cell[size_t(l - 1)])
+                        {
+                            throw std::logic_error("device work lists: inconsistent counts (mesh " + std::to_string(q) + ", list " +
+                                                   std::to_string(k) + ", level " + std::to_string(l) + ": " + std::to_string(t[0]) + " " +
+                                                   std::to_string(t[l - 1]) + " " + std::to_string(t[l]) + ")");
+                        }
+                    }
+                    const int64_t nchunks = (o.ncells + (int64_t(1) << dev::HINT_SHIFT) - 1) >> dev::HINT_SHIFT;
+                    off[3 * e] = bytes;
+                    bytes += align(size_t(std::max<int64_t>(o.nruns, 1)) * 20);
+                    off[3 * e + 1] = bytes;
+                    bytes += align(size_t(o.nruns + 1) * 8);
+                    off[3 * e + 2] = bytes;
+                    bytes += align(size_t(nchunks + 1) * 4);
+                    dev::ListOut& w = outs[e];
+                    w.run0          = runs_of(t[0]);
+                    w.cell0         = cells_of(t[0]);
+                    w.nruns         = o.nruns;
+                    w.ncells        = o.ncells;
+                }
+            }
+            mem     = std::make_shared<DevBuffer>();
+            char* d = static_cast<char*>(mem->get(bytes, s));
+            for (int q = 0; q < n; ++q)
+            {
+                std::array<DeviceLists::One, dev::DL_N>& ls = lists[size_t(sel[size_t(q)])];
+                for (int k = 0; k < dev::DL_N; ++k)
+                {
+                    const size_t e      = size_t(q) * dev::DL_N + size_t(k);
+                    dev::ListOut& w     = outs[e];
+                    w.iv                = reinterpret_cast<int32_t*>(d + off[3 * e]);
+                    w.pref              = reinterpret_cast<int64_t*>(d + off[3 * e + 1]);
+                    w.hint              = reinterpret_cast<int32_t*>(d + off[3 * e + 2]);
+                    ls[size_t(k)].iv    = w.iv;
+                    ls[size_t(k)].pref  = w.pref;
+                    ls[size_t(k)].hint  = w.hint;
+                }
+            }
+            // the destinations to the device, then the runs
+            const size_t ob = outs.size() * sizeof(dev::ListOut);
+            char* h         = ctx.staging->acquire(ob);
+            std::memcpy(h, outs.data(), ob);
+            DevBuffer dout;
+            auto* d_outs = static_cast<dev::ListOut*>(dout.get(ob, s));
+            hip_check(hipMemcpyAsync(d_outs, h, ob, hipMemcpyHostToDevice, s), "hipMemcpyAsync(list destinations)");
+            ctx.staging->release(s);
+            dev::lists_batch_emit(d_items, d_pp, PW, n, bits, scan, d_outs, s);
+        }
+
         // update_sub_mesh_impl (mr/mesh.hpp:221-455) of the meshes ms[j] with
         // changed[j], from their new leaves (the concatenated unpadded bitmaps
         // laid out by items), all meshes in each launch (submesh_batch.hip);
         // the five ids come back as runs in one round trip (a second one only
         // if the buffer, sized from the current meshes' runs, is too small),
         // and meshes[j] is the complete new mesh
+        //
+        // With lists: the work lists of those meshes too (lists.hip, all
+        // meshes per launch; the update_fields lists from the current meshes,
+        // whose padded ids are built from old_leaves), (*lists)[j] those of
+        // mesh j, their arrays in *list_mem.
         void construct_batch(Ctx& ctx, const std::vector<const Mesh*>& ms, const std::vector<dev::MeshItem>& items, const std::vector<bool>& changed,
-                             const uint64_t* leaves, std::vector<MRMeshData>& meshes)
+                             const uint64_t* leaves, std::vector<MRMeshData>& meshes, const uint64_t* old_leaves = nullptr,
+                             std::vector<std::array<DeviceLists::One, dev::DL_N>>* lists = nullptr, std::shared_ptr<DevBuffer>* list_mem = nullptr)
         {
             ScopedTimer t(&ctx, "mesh construction");
             hipStream_t s = ctx.stream;
@@ -273,6 +369,7 @@ namespace smr
                 it.r                  = c.radius;
                 it.s                  = c.pred_s;
                 it.mr                 = c.max_level != c.min_level ? 1 : 0;
+                it.minl               = c.min_level;
                 pp[size_t(q) + 1]     = pp[size_t(q)] + it.p.off[it.p.hi + 1];
                 int64_t cur           = 0;
                 for (const CellArray& ca : ms[j]->host.ids)
@@ -302,7 +399,7 @@ namespace smr
             // the sets, two scratch maps per mesh, run counts and offsets, scan
             // temp, the segment table, runs
             const size_t tbytes = (dev::submesh_scan_temp_bytes(nw) + 255) & ~size_t(255);
-            const size_t sbytes = size_t(nw + 2 * PW) * 8;
+            const size_t sbytes = (size_t(nw + 2 * PW) * 8 + 255) & ~size_t(255);
             const size_t cbytes = (size_t(2 * nw) * 4 + 255) & ~size_t(255);
             const size_t gb     = (size_t(n) * dev::SM_NSETS * dev::SM_SEG * 4 + 255) & ~size_t(255);
             const size_t rbytes = size_t(dev::SM_RUN) * 4;
@@ -318,13 +415,46 @@ namespace smr
             dev::submesh_batch_run_offsets(d_items, d_pp, PW, n, sets, counts, offsets, temp, tbytes, s);
             dev::submesh_batch_run_emit(d_items, d_pp, PW, n, sets, offsets, dr, cap, s);
             dev::submesh_batch_segments(d_items, d_pp, n, offsets, counts, dseg, s);
-            // the segment table (its last entry is the run count), then the runs
-            char* hr      = ctx.readback->acquire(gb + size_t(cap) * rbytes);
-            int32_t* hs   = reinterpret_cast<int32_t*>(hr);
-            int32_t* runs = reinterpret_cast<int32_t*>(hr + gb);
+            // the work lists: the current meshes' padded ids from their leaves,
+            // then the lists' bitmaps, packed counts, scan and per-level table
+            const bool with_lists = lists && old_leaves;
+            const int64_t lw      = with_lists ? int64_t(dev::DL_N) * PW : 0;
+            const size_t ltb      = with_lists ? (dev::lists_scan_temp_bytes(lw) + 255) & ~size_t(255) : 0;
+            const size_t lgb      = with_lists ? (size_t(n) * dev::DL_N * dev::SM_SEG * 8 + 255) & ~size_t(255) : 0;
+            DevBuffer lwork;
+            uint64_t *lbits = nullptr, *lscan = nullptr, *ltable = nullptr;
+            if (with_lists)
+            {
+                // the scan's temporary storage 256-byte aligned (its look-back
+                // state is read and written with wide atomics)
+                const size_t lsz  = (size_t(nw + 2 * PW + 3 * lw) * 8 + 255) & ~size_t(255);
+                char* lp          = static_cast<char*>(lwork.get(lsz + ltb + lgb + 256, s));
+                uint64_t* osets   = reinterpret_cast<uint64_t*>(lp);
+                lbits             = osets + nw + 2 * PW;
+                uint64_t* lcounts = lbits + lw;
+                lscan             = lcounts + lw;
+                void* ltemp       = lp + lsz;
+                ltable            = reinterpret_cast<uint64_t*>(lp + lsz + ltb);
+                dev::submesh_batch_build(d_items, d_pp, PW, n, old_leaves, osets, osets + nw, s);
+                dev::lists_batch_build(d_items, d_pp, PW, n, sets, osets, lbits, lcounts, lscan, ltemp, ltb, ltable, s);
+            }
+            // the list table, the segment table (its last entry is the run
+            // count), then the runs
+            char* hr      = ctx.readback->acquire(lgb + gb + size_t(cap) * rbytes);
+            uint64_t* ht  = reinterpret_cast<uint64_t*>(hr);
+            int32_t* hs   = reinterpret_cast<int32_t*>(hr + lgb);
+            int32_t* runs = reinterpret_cast<int32_t*>(hr + lgb + gb);
+            if (with_lists)
+            {
+                hip_check(hipMemcpyAsync(ht, ltable, size_t(n) * dev::DL_N * dev::SM_SEG * 8, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(lists)");
+            }
             hip_check(hipMemcpyAsync(hs, dseg, gb, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(submesh runs)");
             hip_check(hipMemcpyAsync(runs, dr, size_t(cap) * rbytes, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(submesh runs)");
             ctx.sync();
+            if (with_lists)
+            {
+                emit_batch_lists(ctx, d_items, d_pp, PW, n, lbits, lscan, ht, sel, *lists, *list_mem);
+            }
             const int64_t nruns = hs[int64_t(n) * dev::SM_NSETS * dev::SM_SEG - 1];
             if (nruns > cap)
             {
@@ -357,9 +487,11 @@ namespace smr
         // whether its leaves change, and the new leaves of those that do; with
         // meshes, the changed meshes are built on the device too
         // (construct_batch) and meshes[j] is the complete new mesh of mesh j
-        // (out is then left empty).
+        // (out is then left empty); with lists, their work lists too.
         std::vector<bool> device_update_batch(Ctx& ctx, const std::vector<const Mesh*>& ms, const std::vector<const uint8_t*>& tags,
-                                              std::vector<CellArray>& out, std::vector<MRMeshData>* meshes)
+                                              std::vector<CellArray>& out, std::vector<MRMeshData>* meshes,
+                                              std::vector<std::array<DeviceLists::One, dev::DL_N>>* lists = nullptr,
+                                              std::shared_ptr<DevBuffer>* list_mem = nullptr)
         {
             hipStream_t s = ctx.stream;
             const int n   = int(ms.size());
@@ -468,7 +600,11 @@ namespace smr
             if (meshes)
             {
                 meshes->assign(size_t(n), MRMeshData());
-                construct_batch(ctx, ms, items, changed, buf[cur], *meshes);
+                if (lists)
+                {
+                    lists->assign(size_t(n), {});
+                }
+                construct_batch(ctx, ms, items, changed, buf[cur], *meshes, old, lists, list_mem);
                 return changed;
             }
             // the new leaves of the changed meshes as runs, one round trip
@@ -1240,6 +1376,8 @@ namespace smr
 
             // tags to the host, then every instance's mesh update in parallel
             std::vector<Rebuild> rb(na);
+            // the arrays of the device-built lists of this pass's new meshes
+            std::shared_ptr<DevBuffer> batch_list_mem;
             {
                 ScopedTimer t(ctx_, "mesh update");
                 std::optional<ScopedTimer> tt(std::in_place, ctx_, "tags to host");
@@ -1274,11 +1412,15 @@ namespace smr
                     }
                     std::vector<CellArray> cas;
                     std::vector<MRMeshData> nms;
+                    std::vector<std::array<DeviceLists::One, dev::DL_N>> dls;
                     std::vector<bool> ch;
-                    const bool dev_ids = !host_submesh();
+                    const bool dev_ids   = !host_submesh();
+                    const bool dev_lists = dev_ids && !host_lists();
                     {
                         ScopedTimer td(ctx_, "device mesh update");
-                        ch = ms.empty() ? std::vector<bool>() : device_update_batch(*ctx_, ms, tg, cas, dev_ids ? &nms : nullptr);
+                        ch = ms.empty() ? std::vector<bool>()
+                                        : device_update_batch(*ctx_, ms, tg, cas, dev_ids ? &nms : nullptr, dev_lists ? &dls : nullptr,
+                                                              &batch_list_mem);
                     }
                     ScopedTimer th(ctx_, "host mesh and programs");
                     parallel_for(int(work.size()),
@@ -1295,7 +1437,14 @@ namespace smr
                                      }
                                      r.nm           = dev_ids ? std::move(nms[size_t(q)]) : build_mesh(c, std::move(cas[size_t(q)]));
                                      r.nm.graduated = true;
-                                     r.img          = build_host_image(r.nm);
+                                     if (dev_lists)
+                                     {
+                                         r.device_lists = true;
+                                         r.dl           = std::move(dls[size_t(q)]);
+                                         build_recipes_and_cells(r.nm, r.outer, r.cells);
+                                         return;
+                                     }
+                                     r.img = build_host_image(r.nm);
                                      build_update_fields_lists(m.host, r.nm, r.copy, r.coarsen, r.refine);
                                  });
                 }
@@ -1358,6 +1507,17 @@ namespace smr
                                      ndev[j]             = std::make_unique<DevMesh>();
                                      ndev[j]->table_blob = std::make_unique<Blob>();
                                      packers[j]          = std::make_unique<Packer>(*ndev[j]->table_blob);
+                                     if (r.device_lists)
+                                     {
+                                         pack_device_lists(*packers[j], *ndev[j], r.nm, std::move(r.outer), std::move(r.cells), r.dl.data(),
+                                                           batch_list_mem);
+                                         for (int k = 0; k < 3; ++k)
+                                         {
+                                             pack_device_list(*packers[j], ufl[j][size_t(k)], dev::DL_UF_COPY + k, r.nm.cfg,
+                                                              r.dl[size_t(dev::DL_UF_COPY + k)]);
+                                         }
+                                         return;
+                                     }
                                      pack_image(*packers[j], *ndev[j], r.nm, std::move(r.img));
                                      ufl[j][0].host = std::move(r.copy);
                                      ufl[j][1].host = std::move(r.coarsen);

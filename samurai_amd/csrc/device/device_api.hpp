@@ -218,14 +218,27 @@ namespace smr::dev
     // leaves), its padded layout p (its SM_NSETS sets from word SM_NSETS * pw0
     // of the concatenated sets, its two scratch maps from word 2 * pw0 of the
     // scratch), its stencil radius r, prediction radius s and mr = max_level >
-    // min_level.
+    // min_level, and its min_level.
     struct SubmeshItem
     {
         Bits b;
         PBits p;
         int64_t lw0, pw0;
-        int r, s, mr;
+        int r, s, mr, minl;
     };
+
+    // all DL_N lists of every item (its min_level minl, its max_level p.hi)
+    // from the items' padded ids (sets) and the ids of the meshes they replace
+    // (old_sets), both laid out like submesh_batch_build's sets: bitmaps
+    // (item k's list j from word DL_N * pw0 + j * words), packed counts and
+    // their scan (DL_N * pwords words each), and table[(DL_N * k + j) *
+    // SM_SEG + l]: the packed scan at the first word of level l of list j of
+    // item k (levels past p.hi: the list's end)
+    void lists_batch_build(const SubmeshItem* items, const int64_t* pp, int64_t pwords, int n, const uint64_t* sets, const uint64_t* old_sets,
+                           uint64_t* bits, uint64_t* counts, uint64_t* scan, void* temp, size_t temp_bytes, uint64_t* table, hipStream_t s);
+    // the runs of every list; outs[DL_N * k + j]: list j of item k (device memory)
+    void lists_batch_emit(const SubmeshItem* items, const int64_t* pp, int64_t pwords, int n, const uint64_t* bits, const uint64_t* scan,
+                          const ListOut* outs, hipStream_t s);
 
     // submesh_build of every item (ppref: prefix of the items' padded words,
     // pwords = ppref[n])

@@ -122,35 +122,27 @@ namespace smr::dev
             table[i]            = w < n ? scan[w] : scan[n - 1] + counts[n - 1];
         }
 
-        __global__ __launch_bounds__(SB) void k_list_emit(PBits p, int nlists, const uint64_t* bits, const uint64_t* scan, ListOuts outs)
+        // the runs of list o that start in word gw of its bitmap set (scan_i:
+        // the packed scan at that word); the thread of word 0 writes the list's
+        // trailer (the cell count and the last hint)
+        __device__ __forceinline__ void emit_list_word(const PBits& p, const uint64_t* set, int64_t gw, uint64_t scan_i, const ListOut& o)
         {
-            const int64_t total = p.off[p.hi + 1];
-            const int64_t i     = int64_t(blockIdx.x) * SB + threadIdx.x;
-            if (i >= total * nlists)
-            {
-                return;
-            }
-            const int k         = int(i / total);
-            const int64_t gw    = i % total;
-            const ListOut& o    = outs.l[k];
             const int64_t nchunks = (o.ncells + (int64_t(1) << HINT_SHIFT) - 1) >> HINT_SHIFT;
             if (gw == 0)
             {
-                // the list's trailer: the cell count and the last hint
                 o.pref[o.nruns] = o.ncells;
                 o.hint[nchunks] = int32_t(o.nruns > 0 ? o.nruns - 1 : 0);
             }
-            const uint64_t* set = bits + int64_t(k) * total;
-            const PWord q       = pword(p, gw);
-            uint64_t st         = sm_run_starts(set, gw, q.wx);
+            const PWord q = pword(p, gw);
+            uint64_t st   = sm_run_starts(set, gw, q.wx);
             if (!st)
             {
                 return;
             }
-            int64_t r         = int64_t(scan[i] >> 32) - o.run0;
-            const int64_t c0  = int64_t(scan[i] & 0xFFFFFFFFull) - o.cell0;
-            const int64_t W   = p.wpr[q.l];
-            const int pad     = p.pad;
+            int64_t r        = int64_t(scan_i >> 32) - o.run0;
+            const int64_t c0 = int64_t(scan_i & 0xFFFFFFFFull) - o.cell0;
+            const int64_t W  = p.wpr[q.l];
+            const int pad    = p.pad;
             while (st)
             {
                 const int b = __ffsll(static_cast<unsigned long long>(st)) - 1;
@@ -196,6 +188,76 @@ namespace smr::dev
                 ++r;
             }
         }
+
+        __global__ __launch_bounds__(SB) void k_list_emit(PBits p, int nlists, const uint64_t* bits, const uint64_t* scan, ListOuts outs)
+        {
+            const int64_t total = p.off[p.hi + 1];
+            const int64_t i     = int64_t(blockIdx.x) * SB + threadIdx.x;
+            if (i < total * nlists)
+            {
+                const int k = int(i / total);
+                emit_list_word(p, bits + int64_t(k) * total, i % total, scan[i], outs.l[k]);
+            }
+        }
+
+        // ---- the lists of many meshes per launch: mesh k's DL_N bitmaps from
+        // word DL_N * pw0 of the concatenated bitmaps (list j from DL_N * pw0 +
+        // j * words), its ids and old ids from word SM_NSETS * pw0 ----
+
+        __global__ __launch_bounds__(SB) void k_blist_words(const SubmeshItem* items, const int64_t* pp, int n, const uint64_t* sets,
+                                                            const uint64_t* old_sets, uint64_t* bits)
+        {
+            for_word(items, pp, n, DL_N,
+                     [&](const SubmeshItem& it, int, int64_t j)
+                     {
+                         const int64_t total = item_words(it);
+                         bits[DL_N * it.pw0 + j] = list_word(it.p, it.minl, it.p.hi, sets + SM_NSETS * it.pw0, old_sets + SM_NSETS * it.pw0,
+                                                             int(j / total), j % total);
+                     });
+        }
+
+        __global__ __launch_bounds__(SB) void k_blist_counts(const SubmeshItem* items, const int64_t* pp, int n, const uint64_t* bits,
+                                                             uint64_t* counts)
+        {
+            for_word(items, pp, n, DL_N,
+                     [&](const SubmeshItem& it, int, int64_t j)
+                     {
+                         const int64_t total = item_words(it), gw = j % total;
+                         const uint64_t* set = bits + DL_N * it.pw0 + (j / total) * total;
+                         counts[DL_N * it.pw0 + j] = (uint64_t(run_count_word(it.p, set, gw)) << 32) | uint64_t(__popcll(set[gw]));
+                     });
+        }
+
+        // one thread per (mesh, list, level)
+        __global__ __launch_bounds__(SB) void k_blist_table(const SubmeshItem* items, const int64_t* pp, int n, const uint64_t* counts,
+                                                            const uint64_t* scan, uint64_t* table)
+        {
+            const int64_t i = int64_t(blockIdx.x) * SB + threadIdx.x;
+            if (i >= int64_t(n) * DL_N * SM_SEG)
+            {
+                return;
+            }
+            const int k           = int(i / (DL_N * SM_SEG));
+            const int j           = int(i / SM_SEG % DL_N);
+            const int l           = int(i % SM_SEG);
+            const SubmeshItem& it = items[k];
+            const PBits& p        = it.p;
+            const int64_t N       = DL_N * pp[n];
+            const int64_t w       = DL_N * it.pw0 + j * item_words(it) + p.off[l <= p.hi + 1 ? l : p.hi + 1];
+            table[i]              = w < N ? scan[w] : scan[N - 1] + counts[N - 1];
+        }
+
+        __global__ __launch_bounds__(SB) void k_blist_emit(const SubmeshItem* items, const int64_t* pp, int n, const uint64_t* bits,
+                                                           const uint64_t* scan, const ListOut* outs)
+        {
+            for_word(items, pp, n, DL_N,
+                     [&](const SubmeshItem& it, int k, int64_t j)
+                     {
+                         const int64_t total = item_words(it);
+                         const int list      = int(j / total);
+                         emit_list_word(it.p, bits + DL_N * it.pw0 + list * total, j % total, scan[DL_N * it.pw0 + j], outs[k * DL_N + list]);
+                     });
+        }
     } // namespace
 
     size_t lists_scan_temp_bytes(int64_t n)
@@ -226,6 +288,30 @@ namespace smr::dev
         if (n > 0)
         {
             hipLaunchKernelGGL(k_list_emit, dim3(sm_blocks(n)), dim3(SB), 0, s, p, nlists, bits, scan, outs);
+        }
+    }
+
+    void lists_batch_build(const SubmeshItem* items, const int64_t* pp, int64_t pwords, int n, const uint64_t* sets, const uint64_t* old_sets,
+                           uint64_t* bits, uint64_t* counts, uint64_t* scan, void* temp, size_t temp_bytes, uint64_t* table, hipStream_t s)
+    {
+        const int64_t nw = DL_N * pwords;
+        if (nw <= 0 || n <= 0)
+        {
+            return;
+        }
+        hipLaunchKernelGGL(k_blist_words, dim3(sm_blocks(nw)), dim3(SB), 0, s, items, pp, n, sets, old_sets, bits);
+        hipLaunchKernelGGL(k_blist_counts, dim3(sm_blocks(nw)), dim3(SB), 0, s, items, pp, n, bits, counts);
+        (void)hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, counts, scan, int(nw), s);
+        hipLaunchKernelGGL(k_blist_table, dim3(sm_blocks(int64_t(n) * DL_N * SM_SEG)), dim3(SB), 0, s, items, pp, n, counts, scan, table);
+    }
+
+    void lists_batch_emit(const SubmeshItem* items, const int64_t* pp, int64_t pwords, int n, const uint64_t* bits, const uint64_t* scan,
+                          const ListOut* outs, hipStream_t s)
+    {
+        const int64_t nw = DL_N * pwords;
+        if (nw > 0 && n > 0)
+        {
+            hipLaunchKernelGGL(k_blist_emit, dim3(sm_blocks(nw)), dim3(SB), 0, s, items, pp, n, bits, scan, outs);
         }
     }
 } // namespace smr::dev

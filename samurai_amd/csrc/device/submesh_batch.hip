@@ -21,84 +21,6 @@ namespace smr::dev
     {
         constexpr int CB = 1024; // threads of a chain block
 
-        static_assert(sizeof(SubmeshItem) % 8 == 0, "items are staged in LDS as 64-bit words");
-
-        // the item holding global index i of the prefix pref over n items
-        __device__ __forceinline__ int find_item(const int64_t* pref, int n, int64_t i)
-        {
-            int lo = 0, hi = n - 1;
-            while (lo < hi)
-            {
-                const int mid = (lo + hi + 1) >> 1;
-                if (pref[mid] <= i)
-                {
-                    lo = mid;
-                }
-                else
-                {
-                    hi = mid - 1;
-                }
-            }
-            return lo;
-        }
-
-        // the item of every index of this block, or -1 when they span two
-        __device__ __forceinline__ int block_item(const int64_t* pref, int n)
-        {
-            const int64_t first = int64_t(blockIdx.x) * SB;
-            const int64_t last  = (first + SB < pref[n] ? first + SB : pref[n]) - 1;
-            const int a         = find_item(pref, n, first);
-            return a == find_item(pref, n, last) ? a : -1;
-        }
-
-        constexpr int kItemWords = int(sizeof(SubmeshItem) / 8);
-
-        // copies src into the LDS words dst by the block's threads and returns
-        // the copy (read after a barrier)
-        __device__ __forceinline__ const SubmeshItem& stage_item(uint64_t* dst, const SubmeshItem& src)
-        {
-            const uint64_t* s = reinterpret_cast<const uint64_t*>(&src);
-            for (int i = threadIdx.x; i < kItemWords; i += blockDim.x)
-            {
-                dst[i] = s[i];
-            }
-            return *reinterpret_cast<const SubmeshItem*>(dst);
-        }
-
-        // f(item, k, gw) for the padded word of global index blockIdx.x * SB
-        // + threadIdx.x (its item k, its word gw of that item)
-        template <class F>
-        __device__ __forceinline__ void for_word(const SubmeshItem* items, const int64_t* pref, int n, F&& f)
-        {
-            __shared__ uint64_t s_raw[kItemWords];
-            const SubmeshItem& s_item = *reinterpret_cast<const SubmeshItem*>(s_raw);
-            const int64_t i           = int64_t(blockIdx.x) * SB + threadIdx.x;
-            const int kb              = block_item(pref, n);
-            if (kb >= 0)
-            {
-                stage_item(s_raw, items[kb]);
-            }
-            __syncthreads();
-            if (i >= pref[n])
-            {
-                return;
-            }
-            if (kb >= 0)
-            {
-                f(s_item, kb, i - pref[kb]);
-            }
-            else
-            {
-                const int k = find_item(pref, n, i);
-                f(items[k], k, i - pref[k]);
-            }
-        }
-
-        __device__ __forceinline__ int64_t item_words(const SubmeshItem& it)
-        {
-            return it.p.off[it.p.hi + 1];
-        }
-
         __device__ __forceinline__ uint64_t* item_set(const SubmeshItem& it, uint64_t* sets, int id)
         {
             return sets + SM_NSETS * it.pw0 + int64_t(id) * item_words(it);
@@ -107,13 +29,13 @@ namespace smr::dev
         __global__ __launch_bounds__(SB) void k_bs_cells(const SubmeshItem* items, const int64_t* ppref, int n, const uint64_t* leaves,
                                                          uint64_t* sets)
         {
-            for_word(items, ppref, n,
+            for_word(items, ppref, n, 1,
                      [&](const SubmeshItem& it, int, int64_t gw) { item_set(it, sets, SM_CELLS)[gw] = cells_word(it.b, leaves + it.lw0, it.p, gw); });
         }
 
         __global__ __launch_bounds__(SB) void k_bs_dilate(const SubmeshItem* items, const int64_t* ppref, int n, uint64_t* sets)
         {
-            for_word(items, ppref, n,
+            for_word(items, ppref, n, 1,
                      [&](const SubmeshItem& it, int, int64_t gw)
                      { item_set(it, sets, SM_CELLS_AND_GHOSTS)[gw] = dilate_word(it.p, item_set(it, sets, SM_CELLS), it.r, gw); });
         }
@@ -136,7 +58,7 @@ namespace smr::dev
 
         __global__ __launch_bounds__(SB) void k_bs_coarse(const SubmeshItem* items, const int64_t* ppref, int n, uint64_t* sets, uint64_t* scratch)
         {
-            for_word(items, ppref, n,
+            for_word(items, ppref, n, 1,
                      [&](const SubmeshItem& it, int, int64_t gw)
                      {
                          if (it.mr)
@@ -150,7 +72,7 @@ namespace smr::dev
         __global__ __launch_bounds__(SB) void k_bs_reference(const SubmeshItem* items, const int64_t* ppref, int n, uint64_t* sets,
                                                              const uint64_t* scratch)
         {
-            for_word(items, ppref, n,
+            for_word(items, ppref, n, 1,
                      [&](const SubmeshItem& it, int, int64_t gw)
                      {
                          const uint64_t* K1 = scratch + 2 * it.pw0;
@@ -182,7 +104,7 @@ namespace smr::dev
         __global__ __launch_bounds__(SB) void k_bs_run_counts(const SubmeshItem* items, const int64_t* ppref, int n, const uint64_t* sets,
                                                               int32_t* counts)
         {
-            for_word(items, ppref, n,
+            for_word(items, ppref, n, 1,
                      [&](const SubmeshItem& it, int, int64_t gw)
                      {
                          const int64_t total = item_words(it), w0 = SM_NSETS * it.pw0;
@@ -196,7 +118,7 @@ namespace smr::dev
         __global__ __launch_bounds__(SB) void k_bs_run_emit(const SubmeshItem* items, const int64_t* ppref, int n, const uint64_t* sets,
                                                             const int32_t* offsets, int32_t* runs, int64_t cap)
         {
-            for_word(items, ppref, n,
+            for_word(items, ppref, n, 1,
                      [&](const SubmeshItem& it, int, int64_t gw)
                      {
                          const int64_t total = item_words(it), w0 = SM_NSETS * it.pw0;

@@ -401,4 +401,89 @@ namespace smr::dev::smops
             const int64_t w = w0 + p.off[l <= p.hi + 1 ? l : p.hi + 1];
             return w < nw ? offsets[w] : offsets[nw - 1] + counts[nw - 1];
         }
+        // ---- many meshes per launch: a prefix over the meshes' padded words
+        // (pref[k] = first word of mesh k; times `scale` when every mesh holds
+        // `scale` arrays of its words) locates a thread's mesh ----
+
+        static_assert(sizeof(SubmeshItem) % 8 == 0, "items are staged in LDS as 64-bit words");
+
+        // the item holding global index i of the prefix pref (scaled) over n items
+        __device__ __forceinline__ int find_item(const int64_t* pref, int n, int64_t i, int64_t scale)
+        {
+            int lo = 0, hi = n - 1;
+            while (lo < hi)
+            {
+                const int mid = (lo + hi + 1) >> 1;
+                if (pref[mid] * scale <= i)
+                {
+                    lo = mid;
+                }
+                else
+                {
+                    hi = mid - 1;
+                }
+            }
+            return lo;
+        }
+
+        // the item of every index of this block, or -1 when they span two
+        __device__ __forceinline__ int block_item(const int64_t* pref, int n, int64_t scale)
+        {
+            const int64_t end   = pref[n] * scale;
+            const int64_t first = int64_t(blockIdx.x) * SB;
+            const int64_t last  = (first + SB < end ? first + SB : end) - 1;
+            const int a         = find_item(pref, n, first, scale);
+            return a == find_item(pref, n, last, scale) ? a : -1;
+        }
+
+        constexpr int kItemWords = int(sizeof(SubmeshItem) / 8);
+
+        // copies src into the LDS words dst by the block's threads and returns
+        // the copy (read after a barrier)
+        __device__ __forceinline__ const SubmeshItem& stage_item(uint64_t* dst, const SubmeshItem& src)
+        {
+            const uint64_t* s = reinterpret_cast<const uint64_t*>(&src);
+            for (int i = threadIdx.x; i < kItemWords; i += blockDim.x)
+            {
+                dst[i] = s[i];
+            }
+            return *reinterpret_cast<const SubmeshItem*>(dst);
+        }
+
+        // f(item, k, j) for global index blockIdx.x * SB + threadIdx.x < pref[n]
+        // * scale: its item k and its index j within that item's scale x
+        // words.  A block whose indices all belong to one item (nearly every
+        // block) stages that item in LDS; the others find each thread's item
+        // by bisection.
+        template <class F>
+        __device__ __forceinline__ void for_word(const SubmeshItem* items, const int64_t* pref, int n, int64_t scale, F&& f)
+        {
+            __shared__ uint64_t s_raw[kItemWords];
+            const SubmeshItem& s_item = *reinterpret_cast<const SubmeshItem*>(s_raw);
+            const int64_t i           = int64_t(blockIdx.x) * SB + threadIdx.x;
+            const int kb              = block_item(pref, n, scale);
+            if (kb >= 0)
+            {
+                stage_item(s_raw, items[kb]);
+            }
+            __syncthreads();
+            if (i >= pref[n] * scale)
+            {
+                return;
+            }
+            if (kb >= 0)
+            {
+                f(s_item, kb, i - pref[kb] * scale);
+            }
+            else
+            {
+                const int k = find_item(pref, n, i, scale);
+                f(items[k], k, i - pref[k] * scale);
+            }
+        }
+
+        __device__ __forceinline__ int64_t item_words(const SubmeshItem& it)
+        {
+            return it.p.off[it.p.hi + 1];
+        }
 } // namespace smr::dev::smops

@@ -835,6 +835,37 @@ namespace smr
         return dm;
     }
 
+    void pack_device_list(Packer& p, DevWorkList& w, int kind, const MeshConfig& cfg, const DeviceLists::One& o)
+    {
+        w.host           = WorkList();
+        w.host.taps      = list_taps(kind, cfg.dim, cfg.pred_s);
+        w.host.level_run = o.level_run;
+        w.level_cell     = o.level_cell;
+        w.device_built   = true;
+        p.device_list(w, o.iv, o.pref, o.hint, o.nruns, o.ncells);
+    }
+
+    void pack_device_lists(Packer& p, DevMesh& dm, const MRMeshData& m, std::vector<Recipes>&& outer, WorkList&& cells, const DeviceLists::One* l,
+                           std::shared_ptr<DevBuffer> mem)
+    {
+        p.table(m, dm.table);
+        dm.outer.resize(outer.size());
+        for (size_t k = 0; k < outer.size(); ++k)
+        {
+            dm.outer[k].host = std::move(outer[k]);
+            p.recipes(dm.outer[k]);
+        }
+        dm.cells.host = std::move(cells);
+        p.list(dm.cells);
+        DevWorkList* dls[dev::DL_UF_COPY] = {&dm.projection, &dm.prediction, &dm.detail, &dm.criteria, &dm.maximum};
+        for (int k = 0; k < dev::DL_UF_COPY; ++k)
+        {
+            pack_device_list(p, *dls[k], k, m.cfg, l[k]);
+        }
+        dm.list_mem  = std::move(mem);
+        dm.has_ghost = dm.has_adapt = true;
+    }
+
     std::unique_ptr<DevMesh> upload_image_device_lists(Ctx& ctx, const MRMeshData& m, std::vector<Recipes>&& outer, WorkList&& cells,
                                                        DeviceLists&& dl, const dev::Table* old_table)
     {
@@ -847,32 +878,17 @@ namespace smr
         auto dm        = std::make_unique<DevMesh>();
         dm->table_blob = std::make_unique<Blob>();
         Packer p(*dm->table_blob);
-        p.table(m, dm->table);
-        dm->outer.resize(outer.size());
-        for (size_t l = 0; l < outer.size(); ++l)
+        pack_device_lists(p, *dm, m, std::move(outer), std::move(cells), dl.l, dl.mem);
+        if (uf)
         {
-            dm->outer[l].host = std::move(outer[l]);
-            p.recipes(dm->outer[l]);
+            for (int k = 0; k < 3; ++k)
+            {
+                pack_device_list(p, dm->uf[k], dev::DL_UF_COPY + k, m.cfg, dl.l[dev::DL_UF_COPY + k]);
+            }
         }
-        dm->cells.host = std::move(cells);
-        p.list(dm->cells);
-        DevWorkList* dls[dev::DL_N] = {&dm->projection, &dm->prediction, &dm->detail, &dm->criteria, &dm->maximum, &dm->uf[0], &dm->uf[1], &dm->uf[2]};
-        for (int k = 0; k < dl.nlists; ++k)
-        {
-            const DeviceLists::One& o = dl.l[k];
-            DevWorkList& w            = *dls[k];
-            w.host                    = WorkList();
-            w.host.taps               = list_taps(k, m.cfg.dim, m.cfg.pred_s);
-            w.host.level_run          = o.level_run;
-            w.level_cell              = o.level_cell;
-            w.device_built            = true;
-            p.device_list(w, o.iv, o.pref, o.hint, o.nruns, o.ncells);
-        }
-        dm->list_mem      = std::move(dl.mem);
         dm->mesh_sets     = std::move(dl.sets);
         dm->has_mesh_sets = dm->mesh_sets.p != nullptr;
-        dm->has_ghost = dm->has_adapt = true;
-        dm->has_uf    = uf;
+        dm->has_uf        = uf;
         finish_staged(ctx, p, *dm->table_blob);
         // every list of the image (and its update_fields lists) in one launch
         dev::ResolveBatch rb{};
@@ -1591,6 +1607,13 @@ namespace smr
                     o.level_run[size_t(l)]  = runs_of(t[l]) - runs_of(t[0]);
                     o.level_cell[size_t(l)] = cells_of(t[l]) - cells_of(t[0]);
                 }
+                for (int l = 1; l < dev::SM_SEG; ++l)
+                {
+                    if (o.level_run[size_t(l)] < o.level_run[size_t(l - 1)] || o.level_cell[size_t(l)] < o.level_cell[size_t(l - 1)])
+                    {
+                        throw std::logic_error("device work lists: inconsistent counts (list " + std::to_string(k) + ", level " + std::to_string(l) + ")");
+                    }
+                }
                 const int64_t nchunks = (o.ncells + (int64_t(1) << dev::HINT_SHIFT) - 1) >> dev::HINT_SHIFT;
                 off[size_t(3 * k)]     = bytes;
                 bytes += align(size_t(std::max<int64_t>(o.nruns, 1)) * 20);
@@ -1599,7 +1622,8 @@ namespace smr
                 off[size_t(3 * k + 2)] = bytes;
                 bytes += align(size_t(nchunks + 1) * 4);
             }
-            char* d = static_cast<char*>(dl.mem.get(bytes, ctx.stream));
+            dl.mem  = std::make_shared<DevBuffer>();
+            char* d = static_cast<char*>(dl.mem->get(bytes, ctx.stream));
             dev::ListOuts outs{};
             for (int k = 0; k < nlists; ++k)
             {
@@ -1645,7 +1669,7 @@ namespace smr
             }
             const int64_t cap     = cur_runs + cur_runs / 8 + 4096;
             const size_t tbytes   = (dev::submesh_scan_temp_bytes(nwords) + 255) & ~size_t(255);
-            const size_t kbytes   = size_t(std::max<int64_t>(2 * total, 1)) * 8; // 2 scratch maps
+            const size_t kbytes   = (size_t(std::max<int64_t>(2 * total, 1)) * 8 + 255) & ~size_t(255); // 2 scratch maps
             const size_t cbytes   = (size_t(2 * std::max<int64_t>(nwords, 1)) * 4 + 255) & ~size_t(255);
             constexpr size_t gb   = (size_t(dev::SM_NSETS * dev::SM_SEG) * 4 + 255) & ~size_t(255);
             const size_t rbytes   = size_t(dev::SM_RUN) * 4;
@@ -1672,12 +1696,15 @@ namespace smr
             uint64_t *lbits = nullptr, *lscan = nullptr, *ltable = nullptr;
             if (nlists)
             {
-                char* lp           = static_cast<char*>(lwork.get(size_t(3 * lwords) * 8 + ltb + lgb + 256, s));
+                // the scan's temporary storage 256-byte aligned (its look-back
+                // state is read and written with wide atomics)
+                const size_t lsz   = (size_t(3 * lwords) * 8 + 255) & ~size_t(255);
+                char* lp           = static_cast<char*>(lwork.get(lsz + ltb + lgb + 256, s));
                 lbits              = reinterpret_cast<uint64_t*>(lp);
                 uint64_t* lcounts  = lbits + lwords;
                 lscan              = lcounts + lwords;
-                void* ltemp        = lp + size_t(3 * lwords) * 8;
-                ltable             = reinterpret_cast<uint64_t*>(lp + size_t(3 * lwords) * 8 + ltb);
+                void* ltemp        = lp + lsz;
+                ltable             = reinterpret_cast<uint64_t*>(lp + lsz + ltb);
                 dev::lists_build(p, cfg.min_level, cfg.max_level, sets, old_sets, nlists, lbits, lcounts, lscan, ltemp, ltb, ltable, s);
             }
             // the list table, the segment table (its last entry is the run

@@ -227,7 +227,7 @@ namespace smr
         // the arrays of the work lists built on the device, and the padded
         // bitmaps of the mesh ids of this version (the old mesh of the next
         // device-built update_fields lists)
-        DevBuffer list_mem;
+        std::shared_ptr<DevBuffer> list_mem; // shared by the meshes of one batched update
         DevBuffer mesh_sets;
         bool has_mesh_sets = false;
         // the work lists of a complete image (cells first)
@@ -416,9 +416,16 @@ namespace smr
         };
         int nlists = 0;
         One l[dev::DL_N];
-        DevBuffer mem;
+        std::shared_ptr<DevBuffer> mem;
         DevBuffer sets;
     };
+    // packs the image of a mesh whose work lists the device built: the table,
+    // the recipes and the cells list from the host, the five program lists
+    // l[DL_PROJECTION .. DL_MAXIMUM] (whose arrays mem holds) as they are
+    void pack_device_lists(Packer& p, DevMesh& dm, const MRMeshData& m, std::vector<Recipes>&& outer, WorkList&& cells, const DeviceLists::One* l,
+                           std::shared_ptr<DevBuffer> mem);
+    // one device-built list of that kind
+    void pack_device_list(Packer& p, DevWorkList& w, int kind, const MeshConfig& cfg, const DeviceLists::One& o);
     // the image of a mesh whose projection, prediction, detail, criteria,
     // maximum and update_fields lists the device built (dl); the host gives
     // the recipes and the cells list

@@ -75,9 +75,12 @@ def test_batch_steps_match_plain(gpu_ctx, cpt, mode):
 def test_batch_mesh_ids_match_host_construction(gpu_ctx):
     """The batched device construction (update_sub_mesh_impl of every changed
     instance per launch) against the host's construction of the same leaves:
-    all five ids and the storage numbering, after each of four steps."""
+    all five ids and the storage numbering, after each of four steps; and the
+    work lists built on the device for every changed instance against the
+    host's programs of the same mesh."""
     import samurai_amd as sa
     plain, bat, batch = setup_pair(sa, gpu_ctx, [61, 62, 63, 64, 65], (4, 8))
+    checked = 0
     for step in range(4):
         batch.step(EPS, REG)
         gpu_ctx.synchronize()
@@ -86,6 +89,10 @@ def test_batch_mesh_ids_match_host_construction(gpu_ctx):
             host = sa.make_mesh_from_cells(gpu_ctx, m.cfg, m.cells())
             for mid in (sa.CELLS, sa.CELLS_AND_GHOSTS, sa.PROJ_CELLS, sa.UNION_CELLS, sa.REFERENCE):
                 assert np.array_equal(host.cells(mid), m.cells(mid)), (step, k, mid)
+            rc = m.check_device_lists()
+            assert rc in (0, 1), (step, k, rc)
+            checked += rc == 0
+    assert checked > 0
 
 
 @pytest.mark.parametrize("block,cpt", [(64, 1), (128, 8), (1024, 2)])
