@@ -1149,6 +1149,7 @@ namespace smr
 
     Batch::~Batch()
     {
+        release_replaced();
         hipStream_t s = ctx_->stream;
         if (det_arena_)
         {
@@ -1372,6 +1373,9 @@ namespace smr
                 adapt_prog_.upload();
                 adapt_prog_.run(tuning_);
                 calls += int(na);
+                // the versions the previous pass replaced, freed on the host
+                // while this pass runs
+                release_replaced();
             }
 
             // tags to the host, then every instance's mesh update in parallel
@@ -1628,6 +1632,13 @@ namespace smr
                     uf_prog_.run(tuning_);
                     tp.reset();
                     ScopedTimer tr(ctx_, "old storage release");
+                    struct Replaced
+                    {
+                        std::unique_ptr<DevMesh> dev;
+                        MRMeshData host;
+                        std::array<DevWorkList, 3> uf;
+                        Rebuild rb;
+                    };
                     std::vector<std::unique_ptr<DevMesh>> old_dev(nc);
                     std::vector<MRMeshData> old_host(nc);
                     for (size_t j = 0; j < nc; ++j)
@@ -1648,16 +1659,14 @@ namespace smr
                         b.u->version    = m.version;
                         b.unp1->version = m.version;
                     }
-                    // the host halves of the old versions and of the update_fields
-                    // lists are freed in parallel
-                    parallel_for(int(nc),
-                                 [&](int q)
-                                 {
-                                     old_dev[size_t(q)].reset();
-                                     old_host[size_t(q)] = MRMeshData();
-                                     ufl[size_t(q)]      = {};
-                                     rb[changed[size_t(q)]] = Rebuild();
-                                 });
+                    // the old versions and the update_fields lists are freed
+                    // (in parallel) while the next launches run: their device
+                    // blocks go back in stream order, after the work queued now
+                    for (size_t j = 0; j < nc; ++j)
+                    {
+                        auto* r = new Replaced{std::move(old_dev[j]), std::move(old_host[j]), std::move(ufl[j]), std::move(rb[changed[j]])};
+                        replaced_.emplace_back(r, [](void* p) { delete static_cast<Replaced*>(p); });
+                    }
                 }
             }
             std::vector<int> next;
@@ -1720,6 +1729,12 @@ namespace smr
         return p;
     }
 
+    void Batch::release_replaced()
+    {
+        parallel_for(int(replaced_.size()), [&](int i) { replaced_[size_t(i)].reset(); });
+        replaced_.clear();
+    }
+
     void Batch::advance()
     {
         check_instances();
@@ -1732,6 +1747,9 @@ namespace smr
         {
             ScopedTimer t(ctx_, "ghost update + field expressions");
             fv_program().run(tuning_);
+            // the versions the adaptation replaced, freed on the host while the
+            // advance program runs
+            release_replaced();
         }
         for (BatchInstance& b : inst_)
         {

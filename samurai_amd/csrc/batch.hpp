@@ -171,7 +171,11 @@ namespace smr
         int64_t arena_cap_ = 0, host_cap_ = 0;
         // pinned staging of the new images' host parts
         PinnedStaging staging_;
+        // the versions the last adaptation replaced (host meshes, images,
+        // update_fields lists), freed while the next launches run
+        std::vector<std::shared_ptr<void>> replaced_;
 
+        void release_replaced();
         void check_instances();
         void ensure_images(const std::vector<int>& ids);
         void ensure_arenas(int64_t n);
