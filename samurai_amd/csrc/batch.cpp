@@ -1313,10 +1313,20 @@ namespace smr
             // harten (mr/adapt.hpp:276-389) on the device: tags, ghosts, details, criteria, maximum
             {
                 ScopedTimer t(ctx_, "harten device pass");
+                std::optional<ScopedTimer> tb(std::in_place, nullptr, "harten program build");
                 adapt_prog_.clear();
                 std::vector<Chain> chains(na);
+                // the criteria thresholds of every instance (the program's aux
+                // data), then the instances' chains in parallel
+                std::vector<size_t> thr_at(na);
                 for (size_t k = 0; k < na; ++k)
                 {
+                    const dev::Thresholds thr = make_thresholds(inst_[size_t(act[k])].u->mesh->host.cfg, ite, eps, regularity);
+                    thr_at[k]                 = adapt_prog_.add_aux(&thr, sizeof(thr));
+                }
+                parallel_for(int(na), [&](int kk)
+                {
+                    const size_t k      = size_t(kk);
                     Field& f            = *inst_[size_t(act[k])].u;
                     Mesh& m             = *f.mesh;
                     DevMesh& dm         = *m.dev;
@@ -1347,8 +1357,7 @@ namespace smr
                     sd.u            = f.d;
                     sd.out          = det;
                     push_stage(ch, sd);
-                    const dev::Thresholds thr = make_thresholds(c, ite, eps, regularity);
-                    const size_t h            = adapt_prog_.add_aux(&thr, sizeof(thr));
+                    const size_t h = thr_at[k];
                     dm.criteria.cells_of_levels(c.min_level - 1, c.max_level - ite, c0, c1);
                     dev::Segment sc = list_seg(dev::SEG_CRITERIA, c, dm.criteria, c0, c1);
                     sc.out          = det;
@@ -1368,9 +1377,10 @@ namespace smr
                     tc.err          = flags_ + k;
                     push_stage(ch, tc);
                     chains[k] = std::move(ch);
-                }
+                });
                 adapt_prog_.append(chains);
                 adapt_prog_.upload();
+                tb.reset();
                 adapt_prog_.run(tuning_);
                 calls += int(na);
                 // the versions the previous pass replaced, freed on the host
@@ -1731,6 +1741,11 @@ namespace smr
 
     void Batch::release_replaced()
     {
+        if (replaced_.empty())
+        {
+            return;
+        }
+        ScopedTimer t(nullptr, "replaced versions freed");
         parallel_for(int(replaced_.size()), [&](int i) { replaced_[size_t(i)].reset(); });
         replaced_.clear();
     }
