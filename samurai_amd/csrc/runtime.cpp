@@ -1914,9 +1914,11 @@ namespace smr
             dm.has_leaf_bits = true;
         }
         const uint64_t* old = static_cast<const uint64_t*>(dm.leaf_bits.p);
-        // graduation passes launched per readback of their flags: most
-        // updates reach the fixed point within two, and the third confirms it
-        constexpr int kPasses = 3;
+        // graduation passes launched per readback of their flags: the fixed
+        // point is the first pass that refines nothing, which on the 2D
+        // L4..12 advection was the first or the second pass of every update
+        // (a later one costs another readback)
+        constexpr int kPasses = 2;
         DevBuffer work;
         uint64_t* w   = static_cast<uint64_t*>(work.get(5 * wbytes + 256, s));
         uint64_t* add = w;
