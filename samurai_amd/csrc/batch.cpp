@@ -244,7 +244,7 @@ namespace smr
         // table (packed runs << 32 | cells scans at the first word of every
         // level of every list), one allocation for all, and the launch that
         // writes them
-        void emit_batch_lists(Ctx& ctx, const dev::SubmeshItem* d_items, const int64_t* d_pp, int64_t PW, int n, const uint64_t* bits,
+        void emit_batch_lists(Ctx& ctx, const dev::SubmeshItem* d_items, const int64_t* d_cp, int64_t nwords, int n, const uint64_t* bits,
                               const uint64_t* scan, const uint64_t* ht, const std::vector<int>& sel,
                               std::vector<std::array<DeviceLists::One, dev::DL_N>>& lists, std::shared_ptr<DevBuffer>& mem)
         {
@@ -321,7 +321,7 @@ namespace smr
             auto* d_outs = static_cast<dev::ListOut*>(dout.get(ob, s));
             hip_check(hipMemcpyAsync(d_outs, h, ob, hipMemcpyHostToDevice, s), "hipMemcpyAsync(list destinations)");
             ctx.staging->release(s);
-            dev::lists_batch_emit(d_items, d_pp, PW, n, bits, scan, d_outs, s);
+            dev::lists_batch_emit(d_items, d_cp, nwords, n, bits, scan, d_outs, s);
         }
 
         // update_sub_mesh_impl (mr/mesh.hpp:221-455) of the meshes ms[j] with
@@ -383,19 +383,28 @@ namespace smr
             }
             const int64_t PW = pp[size_t(n)];
             const int64_t nw = dev::SM_NSETS * PW;
+            // the work lists' words of every item (each list over its own levels)
+            std::vector<int64_t> cp(size_t(n) + 1, 0);
+            for (int q = 0; q < n; ++q)
+            {
+                const dev::SubmeshItem& it = si[size_t(q)];
+                cp[size_t(q) + 1]          = cp[size_t(q)] + (lists ? dev::lists_words(it.p, it.minl, it.p.hi, dev::DL_N) : 0);
+            }
             std::optional<ScopedTimer> td(std::in_place, &ctx, "mesh ids on device");
-            // the item table and its prefix, one upload
+            // the item table and its prefixes, one upload
             const size_t ib = (size_t(n) * sizeof(dev::SubmeshItem) + 255) & ~size_t(255);
             const size_t pb = size_t(n + 1) * 8;
-            char* h         = ctx.staging->acquire(ib + pb);
+            char* h         = ctx.staging->acquire(ib + 2 * pb);
             std::memcpy(h, si.data(), size_t(n) * sizeof(dev::SubmeshItem));
             std::memcpy(h + ib, pp.data(), pb);
+            std::memcpy(h + ib + pb, cp.data(), pb);
             DevBuffer tab;
-            char* dt = static_cast<char*>(tab.get(ib + pb, s));
-            hip_check(hipMemcpyAsync(dt, h, ib + pb, hipMemcpyHostToDevice, s), "hipMemcpyAsync(submesh items)");
+            char* dt = static_cast<char*>(tab.get(ib + 2 * pb, s));
+            hip_check(hipMemcpyAsync(dt, h, ib + 2 * pb, hipMemcpyHostToDevice, s), "hipMemcpyAsync(submesh items)");
             ctx.staging->release(s);
             const auto* d_items = reinterpret_cast<const dev::SubmeshItem*>(dt);
             const auto* d_pp    = reinterpret_cast<const int64_t*>(dt + ib);
+            const auto* d_cp    = reinterpret_cast<const int64_t*>(dt + ib + pb);
             // the sets, two scratch maps per mesh, run counts and offsets, scan
             // temp, the segment table, runs
             const size_t tbytes = (dev::submesh_scan_temp_bytes(nw) + 255) & ~size_t(255);
@@ -418,7 +427,7 @@ namespace smr
             // the work lists: the current meshes' padded ids from their leaves,
             // then the lists' bitmaps, packed counts, scan and per-level table
             const bool with_lists = lists && old_leaves;
-            const int64_t lw      = with_lists ? int64_t(dev::DL_N) * PW : 0;
+            const int64_t lw      = with_lists ? cp[size_t(n)] : 0;
             const size_t ltb      = with_lists ? (dev::lists_scan_temp_bytes(lw) + 255) & ~size_t(255) : 0;
             const size_t lgb      = with_lists ? (size_t(n) * dev::DL_N * dev::SM_SEG * 8 + 255) & ~size_t(255) : 0;
             DevBuffer lwork;
@@ -436,7 +445,7 @@ namespace smr
                 void* ltemp       = lp + lsz;
                 ltable            = reinterpret_cast<uint64_t*>(lp + lsz + ltb);
                 dev::submesh_batch_build(d_items, d_pp, PW, n, old_leaves, osets, osets + nw, s);
-                dev::lists_batch_build(d_items, d_pp, PW, n, sets, osets, lbits, lcounts, lscan, ltemp, ltb, ltable, s);
+                dev::lists_batch_build(d_items, d_cp, lw, n, sets, osets, lbits, lcounts, lscan, ltemp, ltb, ltable, s);
             }
             // the list table, the segment table (its last entry is the run
             // count), then the runs
@@ -453,7 +462,7 @@ namespace smr
             ctx.sync();
             if (with_lists)
             {
-                emit_batch_lists(ctx, d_items, d_pp, PW, n, lbits, lscan, ht, sel, *lists, *list_mem);
+                emit_batch_lists(ctx, d_items, d_cp, lw, n, lbits, lscan, ht, sel, *lists, *list_mem);
             }
             const int64_t nruns = hs[int64_t(n) * dev::SM_NSETS * dev::SM_SEG - 1];
             if (nruns > cap)

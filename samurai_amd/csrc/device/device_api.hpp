@@ -203,15 +203,19 @@ namespace smr::dev
     };
 
     size_t lists_scan_temp_bytes(int64_t n);
-    // the bitmaps of the first nlists lists (nlists x total words; old_sets
-    // may be null when nlists <= DL_UF_COPY), per-word packed counts (runs <<
-    // 32 | cells) and their exclusive sum, and table[k * SM_SEG + l]: the
-    // packed scan at the first word of level l of list k (levels past p.hi:
-    // the list's end)
+    // words of the first nlists lists of a mesh: every list over the padded
+    // words of its own run levels only, one list after the other
+    int64_t lists_words(const PBits& p, int minl, int maxl, int nlists);
+    // the bitmaps of the first nlists lists (lists_words words; old_sets may
+    // be null when nlists <= DL_UF_COPY), per-word packed counts (runs << 32
+    // | cells) and their exclusive sum, and table[k * SM_SEG + l]: the packed
+    // scan at the first word of level l of list k (levels below the list's:
+    // its start, above: its end)
     void lists_build(const PBits& p, int minl, int maxl, const uint64_t* sets, const uint64_t* old_sets, int nlists, uint64_t* bits,
                      uint64_t* counts, uint64_t* scan, void* temp, size_t temp_bytes, uint64_t* table, hipStream_t s);
-    // writes the runs, prefixes and hints of every list into outs
-    void lists_emit(const PBits& p, int nlists, const uint64_t* bits, const uint64_t* scan, const ListOuts& outs, hipStream_t s);
+    // writes the runs, prefixes, hints and trailers of every list into outs
+    void lists_emit(const PBits& p, int minl, int maxl, int nlists, const uint64_t* bits, const uint64_t* scan, const ListOuts& outs,
+                    hipStream_t s);
 
     // One mesh of a batched device mesh construction (submesh_batch.hip): its
     // new leaves (unpadded layout b, from word lw0 of the concatenated
@@ -230,14 +234,13 @@ namespace smr::dev
     // all DL_N lists of every item (its min_level minl, its max_level p.hi)
     // from the items' padded ids (sets) and the ids of the meshes they replace
     // (old_sets), both laid out like submesh_batch_build's sets: bitmaps
-    // (item k's list j from word DL_N * pw0 + j * words), packed counts and
-    // their scan (DL_N * pwords words each), and table[(DL_N * k + j) *
-    // SM_SEG + l]: the packed scan at the first word of level l of list j of
-    // item k (levels past p.hi: the list's end)
-    void lists_batch_build(const SubmeshItem* items, const int64_t* pp, int64_t pwords, int n, const uint64_t* sets, const uint64_t* old_sets,
+    // (item k's lists_words words from index cp[k] on, nwords = cp[n]),
+    // packed counts and their scan, and table[(DL_N * k + j) * SM_SEG + l]:
+    // the packed scan at the first word of level l of list j of item k
+    void lists_batch_build(const SubmeshItem* items, const int64_t* cp, int64_t nwords, int n, const uint64_t* sets, const uint64_t* old_sets,
                            uint64_t* bits, uint64_t* counts, uint64_t* scan, void* temp, size_t temp_bytes, uint64_t* table, hipStream_t s);
-    // the runs of every list; outs[DL_N * k + j]: list j of item k (device memory)
-    void lists_batch_emit(const SubmeshItem* items, const int64_t* pp, int64_t pwords, int n, const uint64_t* bits, const uint64_t* scan,
+    // the runs and trailers of every list; outs[DL_N * k + j]: list j of item k (device memory)
+    void lists_batch_emit(const SubmeshItem* items, const int64_t* cp, int64_t nwords, int n, const uint64_t* bits, const uint64_t* scan,
                           const ListOut* outs, hipStream_t s);
 
     // submesh_build of every item (ppref: prefix of the items' padded words,

@@ -29,31 +29,85 @@ namespace smr::dev
 
     namespace
     {
-        // the run levels of list k
-        __device__ __forceinline__ bool list_has_level(int k, int t, int minl, int maxl)
+        // the run levels [lo, hi) of list k
+        __host__ __device__ __forceinline__ void list_levels(int k, int minl, int maxl, int& lo, int& hi)
         {
             switch (k)
             {
-                case DL_PROJECTION: return t < maxl;
-                case DL_PREDICTION: return t >= 1 && t <= maxl;
-                case DL_DETAIL: return t >= (minl > 0 ? minl - 1 : 0) && t < maxl;
-                case DL_CRITERIA: return t >= (minl > 1 ? minl : 1) - 1 && t < maxl;
-                case DL_MAXIMUM: return t < maxl;
-                case DL_UF_COPY: return t >= minl && t <= maxl;
-                default: return t >= minl && t < maxl; // coarsen, refine
+                case DL_PROJECTION: lo = 0, hi = maxl; break;
+                case DL_PREDICTION: lo = 1, hi = maxl + 1; break;
+                case DL_DETAIL: lo = minl > 0 ? minl - 1 : 0, hi = maxl; break;
+                case DL_CRITERIA: lo = (minl > 1 ? minl : 1) - 1, hi = maxl; break;
+                case DL_MAXIMUM: lo = 0, hi = maxl; break;
+                case DL_UF_COPY: lo = minl, hi = maxl + 1; break;
+                default: lo = minl, hi = maxl; break; // coarsen, refine
+            }
+            if (hi < lo)
+            {
+                hi = lo;
             }
         }
 
-        // word gw of list k
-        __device__ __forceinline__ uint64_t list_word(const PBits& p, int minl, int maxl, const uint64_t* sets, const uint64_t* old_sets, int k,
-                                                      int64_t gw)
+        // the lists of one mesh laid out one after the other, each over the
+        // words of its own levels only: list k holds the words [off[lo[k]],
+        // off[hi[k]]) of the padded layout, from index base[k] on
+        struct ListLayout
+        {
+            int64_t base[DL_N + 1];
+            int lo[DL_N], hi[DL_N];
+        };
+
+        __device__ __forceinline__ ListLayout list_layout(const PBits& p, int minl, int maxl, int nlists)
+        {
+            ListLayout L;
+            L.base[0] = 0;
+            for (int k = 0; k < DL_N; ++k)
+            {
+                int lo = 0, hi = 0;
+                if (k < nlists)
+                {
+                    list_levels(k, minl, maxl, lo, hi);
+                }
+                L.lo[k]       = lo;
+                L.hi[k]       = hi;
+                L.base[k + 1] = L.base[k] + (p.off[hi] - p.off[lo]);
+            }
+            return L;
+        }
+
+        // the list holding layout index j, and its word of the padded layout
+        __device__ __forceinline__ int list_of(const ListLayout& L, const PBits& p, int64_t j, int64_t& gw)
+        {
+            int k = 0;
+            while (k + 1 < DL_N && L.base[k + 1] <= j)
+            {
+                ++k;
+            }
+            gw = p.off[L.lo[k]] + (j - L.base[k]);
+            return k;
+        }
+
+        // the bitmap of list k addressed by padded word (only its levels' words exist)
+        __device__ __forceinline__ const uint64_t* list_set(const ListLayout& L, const PBits& p, const uint64_t* bits, int k)
+        {
+            return bits + L.base[k] - p.off[L.lo[k]];
+        }
+
+        // the scan at the first word of level l of list k (levels outside the
+        // list's range: its start or its end); n: the scanned words
+        __device__ __forceinline__ uint64_t list_level_start(const ListLayout& L, const PBits& p, int k, int l, const uint64_t* counts,
+                                                             const uint64_t* scan, int64_t n)
+        {
+            const int c     = l < L.lo[k] ? L.lo[k] : (l > L.hi[k] ? L.hi[k] : l);
+            const int64_t w = L.base[k] + (p.off[c] - p.off[L.lo[k]]);
+            return w < n ? scan[w] : (n > 0 ? scan[n - 1] + counts[n - 1] : 0ull);
+        }
+
+        // word gw (of a level of the list) of list k
+        __device__ __forceinline__ uint64_t list_word(const PBits& p, const uint64_t* sets, const uint64_t* old_sets, int k, int64_t gw)
         {
             const PWord q = pword(p, gw);
             const int t   = q.l;
-            if (!list_has_level(k, t, minl, maxl))
-            {
-                return 0ull;
-            }
             const int64_t total = p.off[p.hi + 1];
             const uint64_t* C   = sets + SM_CELLS * total;
             const uint64_t* J   = sets + SM_PROJ_CELLS * total;
@@ -88,51 +142,53 @@ namespace smr::dev
         __global__ __launch_bounds__(SB) void k_list_words(PBits p, int minl, int maxl, const uint64_t* sets, const uint64_t* old_sets, int nlists,
                                                            uint64_t* bits)
         {
-            const int64_t total = p.off[p.hi + 1];
-            const int64_t i     = int64_t(blockIdx.x) * SB + threadIdx.x;
-            if (i < total * nlists)
+            const ListLayout L = list_layout(p, minl, maxl, nlists);
+            const int64_t i    = int64_t(blockIdx.x) * SB + threadIdx.x;
+            if (i < L.base[DL_N])
             {
-                bits[i] = list_word(p, minl, maxl, sets, old_sets, int(i / total), i % total);
+                int64_t gw;
+                const int k = list_of(L, p, i, gw);
+                bits[i]     = list_word(p, sets, old_sets, k, gw);
             }
         }
 
         // runs starting in the word << 32 | cells of the word
-        __global__ __launch_bounds__(SB) void k_list_counts(PBits p, int nlists, const uint64_t* bits, uint64_t* counts)
+        __global__ __launch_bounds__(SB) void k_list_counts(PBits p, int minl, int maxl, int nlists, const uint64_t* bits, uint64_t* counts)
         {
-            const int64_t total = p.off[p.hi + 1];
-            const int64_t i     = int64_t(blockIdx.x) * SB + threadIdx.x;
-            if (i < total * nlists)
+            const ListLayout L = list_layout(p, minl, maxl, nlists);
+            const int64_t i    = int64_t(blockIdx.x) * SB + threadIdx.x;
+            if (i < L.base[DL_N])
             {
-                const int64_t gw = i % total;
-                const uint64_t* set = bits + (i / total) * total;
-                counts[i] = (uint64_t(run_count_word(p, set, gw)) << 32) | uint64_t(__popcll(set[gw]));
+                int64_t gw;
+                const int k         = list_of(L, p, i, gw);
+                const uint64_t* set = list_set(L, p, bits, k);
+                counts[i]           = (uint64_t(run_count_word(p, set, gw)) << 32) | uint64_t(__popcll(set[gw]));
             }
         }
 
-        __global__ __launch_bounds__(SB) void k_list_table(PBits p, int nlists, const uint64_t* counts, const uint64_t* scan, uint64_t* table)
+        __global__ __launch_bounds__(SB) void k_list_table(PBits p, int minl, int maxl, int nlists, const uint64_t* counts, const uint64_t* scan,
+                                                           uint64_t* table)
         {
             const int i = int(blockIdx.x) * SB + threadIdx.x;
-            if (i >= nlists * SM_SEG)
+            if (i < nlists * SM_SEG)
             {
-                return;
+                const ListLayout L = list_layout(p, minl, maxl, nlists);
+                table[i]           = list_level_start(L, p, i / SM_SEG, i % SM_SEG, counts, scan, L.base[DL_N]);
             }
-            const int64_t total = p.off[p.hi + 1], n = total * nlists;
-            const int l         = i % SM_SEG;
-            const int64_t w     = (i / SM_SEG) * total + p.off[l <= p.hi + 1 ? l : p.hi + 1];
-            table[i]            = w < n ? scan[w] : scan[n - 1] + counts[n - 1];
+        }
+
+        // the list's trailer: the cell count after the last run, and the last hint
+        __device__ __forceinline__ void list_trailer(const ListOut& o)
+        {
+            const int64_t nchunks = (o.ncells + (int64_t(1) << HINT_SHIFT) - 1) >> HINT_SHIFT;
+            o.pref[o.nruns]       = o.ncells;
+            o.hint[nchunks]       = int32_t(o.nruns > 0 ? o.nruns - 1 : 0);
         }
 
         // the runs of list o that start in word gw of its bitmap set (scan_i:
-        // the packed scan at that word); the thread of word 0 writes the list's
-        // trailer (the cell count and the last hint)
+        // the packed scan at that word)
         __device__ __forceinline__ void emit_list_word(const PBits& p, const uint64_t* set, int64_t gw, uint64_t scan_i, const ListOut& o)
         {
-            const int64_t nchunks = (o.ncells + (int64_t(1) << HINT_SHIFT) - 1) >> HINT_SHIFT;
-            if (gw == 0)
-            {
-                o.pref[o.nruns] = o.ncells;
-                o.hint[nchunks] = int32_t(o.nruns > 0 ? o.nruns - 1 : 0);
-            }
             const PWord q = pword(p, gw);
             uint64_t st   = sm_run_starts(set, gw, q.wx);
             if (!st)
@@ -189,47 +245,64 @@ namespace smr::dev
             }
         }
 
-        __global__ __launch_bounds__(SB) void k_list_emit(PBits p, int nlists, const uint64_t* bits, const uint64_t* scan, ListOuts outs)
+        // the runs of every list; the threads past the last word write the
+        // lists' trailers
+        __global__ __launch_bounds__(SB) void k_list_emit(PBits p, int minl, int maxl, int nlists, const uint64_t* bits, const uint64_t* scan,
+                                                          ListOuts outs)
         {
-            const int64_t total = p.off[p.hi + 1];
-            const int64_t i     = int64_t(blockIdx.x) * SB + threadIdx.x;
-            if (i < total * nlists)
+            const ListLayout L = list_layout(p, minl, maxl, nlists);
+            const int64_t i    = int64_t(blockIdx.x) * SB + threadIdx.x;
+            const int64_t n    = L.base[DL_N];
+            if (i < n)
             {
-                const int k = int(i / total);
-                emit_list_word(p, bits + int64_t(k) * total, i % total, scan[i], outs.l[k]);
+                int64_t gw;
+                const int k = list_of(L, p, i, gw);
+                emit_list_word(p, list_set(L, p, bits, k), gw, scan[i], outs.l[k]);
+            }
+            else if (i < n + nlists)
+            {
+                list_trailer(outs.l[i - n]);
             }
         }
 
-        // ---- the lists of many meshes per launch: mesh k's DL_N bitmaps from
-        // word DL_N * pw0 of the concatenated bitmaps (list j from DL_N * pw0 +
-        // j * words), its ids and old ids from word SM_NSETS * pw0 ----
+        // ---- the lists of many meshes per launch: mesh k's lists (its compact
+        // layout) from index cp[k] of the concatenated bitmaps, its ids and old
+        // ids from word SM_NSETS * pw0 ----
 
-        __global__ __launch_bounds__(SB) void k_blist_words(const SubmeshItem* items, const int64_t* pp, int n, const uint64_t* sets,
+        __device__ __forceinline__ ListLayout item_layout(const SubmeshItem& it)
+        {
+            return list_layout(it.p, it.minl, it.p.hi, DL_N);
+        }
+
+        __global__ __launch_bounds__(SB) void k_blist_words(const SubmeshItem* items, const int64_t* cp, int n, const uint64_t* sets,
                                                             const uint64_t* old_sets, uint64_t* bits)
         {
-            for_word(items, pp, n, DL_N,
-                     [&](const SubmeshItem& it, int, int64_t j)
+            for_word(items, cp, n, 1,
+                     [&](const SubmeshItem& it, int k, int64_t j)
                      {
-                         const int64_t total = item_words(it);
-                         bits[DL_N * it.pw0 + j] = list_word(it.p, it.minl, it.p.hi, sets + SM_NSETS * it.pw0, old_sets + SM_NSETS * it.pw0,
-                                                             int(j / total), j % total);
+                         const ListLayout L = item_layout(it);
+                         int64_t gw;
+                         const int list  = list_of(L, it.p, j, gw);
+                         bits[cp[k] + j] = list_word(it.p, sets + SM_NSETS * it.pw0, old_sets + SM_NSETS * it.pw0, list, gw);
                      });
         }
 
-        __global__ __launch_bounds__(SB) void k_blist_counts(const SubmeshItem* items, const int64_t* pp, int n, const uint64_t* bits,
+        __global__ __launch_bounds__(SB) void k_blist_counts(const SubmeshItem* items, const int64_t* cp, int n, const uint64_t* bits,
                                                              uint64_t* counts)
         {
-            for_word(items, pp, n, DL_N,
-                     [&](const SubmeshItem& it, int, int64_t j)
+            for_word(items, cp, n, 1,
+                     [&](const SubmeshItem& it, int k, int64_t j)
                      {
-                         const int64_t total = item_words(it), gw = j % total;
-                         const uint64_t* set = bits + DL_N * it.pw0 + (j / total) * total;
-                         counts[DL_N * it.pw0 + j] = (uint64_t(run_count_word(it.p, set, gw)) << 32) | uint64_t(__popcll(set[gw]));
+                         const ListLayout L = item_layout(it);
+                         int64_t gw;
+                         const int list      = list_of(L, it.p, j, gw);
+                         const uint64_t* set = list_set(L, it.p, bits + cp[k], list);
+                         counts[cp[k] + j]   = (uint64_t(run_count_word(it.p, set, gw)) << 32) | uint64_t(__popcll(set[gw]));
                      });
         }
 
         // one thread per (mesh, list, level)
-        __global__ __launch_bounds__(SB) void k_blist_table(const SubmeshItem* items, const int64_t* pp, int n, const uint64_t* counts,
+        __global__ __launch_bounds__(SB) void k_blist_table(const SubmeshItem* items, const int64_t* cp, int n, const uint64_t* counts,
                                                             const uint64_t* scan, uint64_t* table)
         {
             const int64_t i = int64_t(blockIdx.x) * SB + threadIdx.x;
@@ -238,25 +311,38 @@ namespace smr::dev
                 return;
             }
             const int k           = int(i / (DL_N * SM_SEG));
-            const int j           = int(i / SM_SEG % DL_N);
-            const int l           = int(i % SM_SEG);
             const SubmeshItem& it = items[k];
-            const PBits& p        = it.p;
-            const int64_t N       = DL_N * pp[n];
-            const int64_t w       = DL_N * it.pw0 + j * item_words(it) + p.off[l <= p.hi + 1 ? l : p.hi + 1];
-            table[i]              = w < N ? scan[w] : scan[N - 1] + counts[N - 1];
+            const ListLayout L    = item_layout(it);
+            // the item's counts and scan, read through its first index (the
+            // scan keeps counting over the items before it)
+            const int64_t N = cp[n];
+            const int list  = int(i / SM_SEG % DL_N);
+            const int l     = int(i % SM_SEG);
+            const int c     = l < L.lo[list] ? L.lo[list] : (l > L.hi[list] ? L.hi[list] : l);
+            const int64_t w = cp[k] + L.base[list] + (it.p.off[c] - it.p.off[L.lo[list]]);
+            table[i]        = w < N ? scan[w] : (N > 0 ? scan[N - 1] + counts[N - 1] : 0ull);
         }
 
-        __global__ __launch_bounds__(SB) void k_blist_emit(const SubmeshItem* items, const int64_t* pp, int n, const uint64_t* bits,
+        __global__ __launch_bounds__(SB) void k_blist_emit(const SubmeshItem* items, const int64_t* cp, int n, const uint64_t* bits,
                                                            const uint64_t* scan, const ListOut* outs)
         {
-            for_word(items, pp, n, DL_N,
+            for_word(items, cp, n, 1,
                      [&](const SubmeshItem& it, int k, int64_t j)
                      {
-                         const int64_t total = item_words(it);
-                         const int list      = int(j / total);
-                         emit_list_word(it.p, bits + DL_N * it.pw0 + list * total, j % total, scan[DL_N * it.pw0 + j], outs[k * DL_N + list]);
+                         const ListLayout L = item_layout(it);
+                         int64_t gw;
+                         const int list = list_of(L, it.p, j, gw);
+                         emit_list_word(it.p, list_set(L, it.p, bits + cp[k], list), gw, scan[cp[k] + j], outs[k * DL_N + list]);
                      });
+        }
+
+        __global__ __launch_bounds__(SB) void k_blist_trailers(const ListOut* outs, int n)
+        {
+            const int i = int(blockIdx.x) * SB + threadIdx.x;
+            if (i < n)
+            {
+                list_trailer(outs[i]);
+            }
         }
     } // namespace
 
@@ -268,50 +354,63 @@ namespace smr::dev
         return bytes;
     }
 
+    int64_t lists_words(const PBits& p, int minl, int maxl, int nlists)
+    {
+        int64_t n = 0;
+        for (int k = 0; k < nlists; ++k)
+        {
+            int lo = 0, hi = 0;
+            list_levels(k, minl, maxl, lo, hi);
+            n += p.off[hi] - p.off[lo];
+        }
+        return n;
+    }
+
     void lists_build(const PBits& p, int minl, int maxl, const uint64_t* sets, const uint64_t* old_sets, int nlists, uint64_t* bits,
                      uint64_t* counts, uint64_t* scan, void* temp, size_t temp_bytes, uint64_t* table, hipStream_t s)
     {
-        const int64_t n = p.off[p.hi + 1] * nlists;
+        const int64_t n = lists_words(p, minl, maxl, nlists);
+        if (n > 0)
+        {
+            hipLaunchKernelGGL(k_list_words, dim3(sm_blocks(n)), dim3(SB), 0, s, p, minl, maxl, sets, old_sets, nlists, bits);
+            hipLaunchKernelGGL(k_list_counts, dim3(sm_blocks(n)), dim3(SB), 0, s, p, minl, maxl, nlists, bits, counts);
+            (void)hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, counts, scan, int(n), s);
+        }
+        hipLaunchKernelGGL(k_list_table, dim3(sm_blocks(nlists * SM_SEG)), dim3(SB), 0, s, p, minl, maxl, nlists, counts, scan, table);
+    }
+
+    void lists_emit(const PBits& p, int minl, int maxl, int nlists, const uint64_t* bits, const uint64_t* scan, const ListOuts& outs, hipStream_t s)
+    {
+        const int64_t n = lists_words(p, minl, maxl, nlists) + nlists;
+        hipLaunchKernelGGL(k_list_emit, dim3(sm_blocks(n)), dim3(SB), 0, s, p, minl, maxl, nlists, bits, scan, outs);
+    }
+
+    void lists_batch_build(const SubmeshItem* items, const int64_t* cp, int64_t nwords, int n, const uint64_t* sets, const uint64_t* old_sets,
+                           uint64_t* bits, uint64_t* counts, uint64_t* scan, void* temp, size_t temp_bytes, uint64_t* table, hipStream_t s)
+    {
         if (n <= 0)
         {
             return;
         }
-        hipLaunchKernelGGL(k_list_words, dim3(sm_blocks(n)), dim3(SB), 0, s, p, minl, maxl, sets, old_sets, nlists, bits);
-        hipLaunchKernelGGL(k_list_counts, dim3(sm_blocks(n)), dim3(SB), 0, s, p, nlists, bits, counts);
-        (void)hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, counts, scan, int(n), s);
-        hipLaunchKernelGGL(k_list_table, dim3(sm_blocks(nlists * SM_SEG)), dim3(SB), 0, s, p, nlists, counts, scan, table);
-    }
-
-    void lists_emit(const PBits& p, int nlists, const uint64_t* bits, const uint64_t* scan, const ListOuts& outs, hipStream_t s)
-    {
-        const int64_t n = p.off[p.hi + 1] * nlists;
-        if (n > 0)
+        if (nwords > 0)
         {
-            hipLaunchKernelGGL(k_list_emit, dim3(sm_blocks(n)), dim3(SB), 0, s, p, nlists, bits, scan, outs);
+            hipLaunchKernelGGL(k_blist_words, dim3(sm_blocks(nwords)), dim3(SB), 0, s, items, cp, n, sets, old_sets, bits);
+            hipLaunchKernelGGL(k_blist_counts, dim3(sm_blocks(nwords)), dim3(SB), 0, s, items, cp, n, bits, counts);
+            (void)hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, counts, scan, int(nwords), s);
         }
+        hipLaunchKernelGGL(k_blist_table, dim3(sm_blocks(int64_t(n) * DL_N * SM_SEG)), dim3(SB), 0, s, items, cp, n, counts, scan, table);
     }
 
-    void lists_batch_build(const SubmeshItem* items, const int64_t* pp, int64_t pwords, int n, const uint64_t* sets, const uint64_t* old_sets,
-                           uint64_t* bits, uint64_t* counts, uint64_t* scan, void* temp, size_t temp_bytes, uint64_t* table, hipStream_t s)
-    {
-        const int64_t nw = DL_N * pwords;
-        if (nw <= 0 || n <= 0)
-        {
-            return;
-        }
-        hipLaunchKernelGGL(k_blist_words, dim3(sm_blocks(nw)), dim3(SB), 0, s, items, pp, n, sets, old_sets, bits);
-        hipLaunchKernelGGL(k_blist_counts, dim3(sm_blocks(nw)), dim3(SB), 0, s, items, pp, n, bits, counts);
-        (void)hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, counts, scan, int(nw), s);
-        hipLaunchKernelGGL(k_blist_table, dim3(sm_blocks(int64_t(n) * DL_N * SM_SEG)), dim3(SB), 0, s, items, pp, n, counts, scan, table);
-    }
-
-    void lists_batch_emit(const SubmeshItem* items, const int64_t* pp, int64_t pwords, int n, const uint64_t* bits, const uint64_t* scan,
+    void lists_batch_emit(const SubmeshItem* items, const int64_t* cp, int64_t nwords, int n, const uint64_t* bits, const uint64_t* scan,
                           const ListOut* outs, hipStream_t s)
     {
-        const int64_t nw = DL_N * pwords;
-        if (nw > 0 && n > 0)
+        if (nwords > 0 && n > 0)
         {
-            hipLaunchKernelGGL(k_blist_emit, dim3(sm_blocks(nw)), dim3(SB), 0, s, items, pp, n, bits, scan, outs);
+            hipLaunchKernelGGL(k_blist_emit, dim3(sm_blocks(nwords)), dim3(SB), 0, s, items, cp, n, bits, scan, outs);
+        }
+        if (n > 0)
+        {
+            hipLaunchKernelGGL(k_blist_trailers, dim3(sm_blocks(int64_t(n) * DL_N)), dim3(SB), 0, s, outs, n * DL_N);
         }
     }
 } // namespace smr::dev

@@ -1585,8 +1585,8 @@ namespace smr
         // allocates the arrays of the device-built lists from their table
         // (packed runs << 32 | cells scans at the first word of every level)
         // and launches the kernel that writes them
-        void emit_lists(Ctx& ctx, const dev::PBits& p, int nlists, const uint64_t* bits, const uint64_t* scan, const std::vector<uint64_t>& table,
-                        DeviceLists& dl)
+        void emit_lists(Ctx& ctx, const dev::PBits& p, const MeshConfig& cfg, int nlists, const uint64_t* bits, const uint64_t* scan,
+                        const std::vector<uint64_t>& table, DeviceLists& dl)
         {
             auto runs_of  = [](uint64_t v) { return int64_t(v >> 32); };
             auto cells_of = [](uint64_t v) { return int64_t(v & 0xFFFFFFFFull); };
@@ -1641,7 +1641,7 @@ namespace smr
                 o.pref             = w.pref;
                 o.hint             = w.hint;
             }
-            dev::lists_emit(p, nlists, bits, scan, outs, ctx.stream);
+            dev::lists_emit(p, cfg.min_level, cfg.max_level, nlists, bits, scan, outs, ctx.stream);
         }
 
         // the five ids of the mesh with the leaves `leaves` (unpadded bitmaps
@@ -1689,7 +1689,7 @@ namespace smr
             dev::submesh_segments(p, dev::SM_NSETS, offsets, counts, dseg, s);
             // the work lists: bitmaps, packed counts, their scan, per-level table
             const int nlists     = dl ? (old_sets ? int(dev::DL_N) : int(dev::DL_UF_COPY)) : 0;
-            const int64_t lwords = total * nlists;
+            const int64_t lwords = nlists ? dev::lists_words(p, cfg.min_level, cfg.max_level, nlists) : 0;
             const size_t ltb     = nlists ? (dev::lists_scan_temp_bytes(lwords) + 255) & ~size_t(255) : 0;
             const size_t lgb     = (size_t(std::max(nlists, 1)) * dev::SM_SEG * 8 + 255) & ~size_t(255);
             DevBuffer lwork;
@@ -1725,7 +1725,7 @@ namespace smr
                 // the lists' sizes and levels; their runs are written while the
                 // host builds the mesh
                 const std::vector<uint64_t> table(ht, ht + size_t(nlists) * dev::SM_SEG);
-                emit_lists(ctx, p, nlists, lbits, lscan, table, *dl);
+                emit_lists(ctx, p, cfg, nlists, lbits, lscan, table, *dl);
                 dl->sets = std::move(sets_buf);
             }
             const int64_t nruns = hs[dev::SM_NSETS * dev::SM_SEG - 1];
