@@ -424,7 +424,7 @@ def bench_adjoint(sa, ctx, args, eps, reg):
         "forward_cell_updates_per_s": ncells * steps / t_fwd,
         "backward_cell_updates_per_s": ncells * steps / t_bwd,
         "fd_u0_rel_err": abs(fd_u - ad_u) / abs(ad_u), "fd_a_rel_err": abs(fd_a - ga[0, 0]) / abs(ga[0, 0]),
-        "fd_h": h, "tuning": "block 256, 1 cell per thread, stage launches replayed as HIP graphs",
+        "fd_h": h, "tuning": "library default (256, 0, 3): blocks of 256, precomputed 64-cell wave items, runs of small stages as one block per instance, each checkpoint segment replayed as a HIP graph",
         "setup_s": setup, "cpu_baseline_forward": cpu,
     }
 
