@@ -979,14 +979,43 @@ namespace smr
 
     void ProgramBuf::upload()
     {
+        upload_with(true, true);
+    }
+
+    void ProgramBuf::upload_for(const Tuning& t)
+    {
+        upload_with(t.cpt == 0 && t.mode != MODE_FUSED, t.mode == MODE_BLOCKS);
+    }
+
+    void ProgramBuf::upload_with(bool items, bool plan)
+    {
         hipStream_t s        = ctx_->stream;
         const size_t nseg    = segs_.size();
         const size_t o_segs  = 0;
         const size_t o_pref  = align_up(o_segs + nseg * sizeof(dev::Segment), 256);
         const size_t o_stage = align_up(o_pref + (nseg + 1) * sizeof(int64_t), 256);
         const size_t o_aux   = align_up(o_stage + stage_seg_.size() * sizeof(int32_t), 256);
-        build_block_plan();
-        build_wave_items();
+        if (plan)
+        {
+            build_block_plan();
+        }
+        else
+        {
+            plan_.clear();
+            blk_.clear();
+        }
+        if (items)
+        {
+            build_wave_items();
+        }
+        else
+        {
+            witems_.clear();
+            stage_item_.assign(1, 0);
+            stage_nitems_.clear();
+        }
+        items_built_ = items;
+        plan_built_  = plan;
         const size_t o_blk   = align_up(o_aux + aux_.size(), 256);
         const size_t o_sit   = align_up(o_blk + blk_.size() * sizeof(int32_t), 256);
         const size_t o_wit   = align_up(o_sit + stage_item_.size() * sizeof(int64_t), 256);
@@ -1092,9 +1121,13 @@ namespace smr
             launch_stagewise(t, s);
             return;
         }
+        if (t.cpt == 0 && t.mode != MODE_FUSED && !items_built_)
+        {
+            throw std::logic_error("stage program uploaded without the wave items its launch shape uses");
+        }
         if (t.mode == MODE_GRAPH || t.mode == MODE_BLOCKS)
         {
-            const bool blocks = t.mode == MODE_BLOCKS && blocks_capable();
+            const bool blocks = t.mode == MODE_BLOCKS && plan_built_ && blocks_capable();
             const int gmode   = blocks ? MODE_BLOCKS : MODE_GRAPH;
             if (!gexec_ || g_block_ != t.block || g_cpt_ != t.cpt || g_mode_ != gmode)
             {
@@ -1388,9 +1421,9 @@ namespace smr
                     chains[k] = std::move(ch);
                 });
                 adapt_prog_.append(chains);
-                adapt_prog_.upload();
+                adapt_prog_.upload_for(pass_tuning());
                 tb.reset();
-                adapt_prog_.run(tuning_);
+                adapt_prog_.run(pass_tuning());
                 calls += int(na);
                 // the versions the previous pass replaced, freed on the host
                 // while this pass runs
@@ -1604,8 +1637,8 @@ namespace smr
                         }
                     }
                     resolve_prog_.append(chains);
-                    resolve_prog_.upload();
-                    resolve_prog_.run(tuning_);
+                    resolve_prog_.upload_for(pass_tuning());
+                    resolve_prog_.run(pass_tuning());
                     ctx_->check_resolve();
                 }
                 // update_fields (update_fields.hpp:27-127) into new storage; unp1 is resized (zero-filled)
@@ -1647,8 +1680,8 @@ namespace smr
                         chains[j] = std::move(ch);
                     }
                     uf_prog_.append(chains);
-                    uf_prog_.upload();
-                    uf_prog_.run(tuning_);
+                    uf_prog_.upload_for(pass_tuning());
+                    uf_prog_.run(pass_tuning());
                     tp.reset();
                     ScopedTimer tr(ctx_, "old storage release");
                     struct Replaced
@@ -1746,6 +1779,22 @@ namespace smr
         p.upload();
         fv_key_[q] = std::move(key);
         return p;
+    }
+
+    Tuning Batch::pass_tuning() const
+    {
+        // the programs of an adaptation pass run once: one cell per thread
+        // (no wave items to build on the host), measured 4.5 ms faster per
+        // step of 256 instances than wave items; SMR_PASS_CPT sets another
+        // cells-per-thread count (0: wave items)
+        static const int cpt = []
+        {
+            const char* e = std::getenv("SMR_PASS_CPT");
+            return e ? std::max(0, std::min(std::atoi(e), 64)) : 1;
+        }();
+        Tuning t = tuning_;
+        t.cpt    = cpt;
+        return t;
     }
 
     void Batch::release_replaced()

@@ -77,6 +77,10 @@ namespace smr
         int64_t max_stage_cells() const;
         int64_t total_cells() const;
         void upload();
+        // upload with only the launch data the shape t uses (no wave items
+        // unless t.cpt == 0, no block plan unless t.mode == MODE_BLOCKS); the
+        // program must then run with t's launch data
+        void upload_for(const Tuning& t);
         void run(const Tuning& t);
 
       private:
@@ -86,6 +90,8 @@ namespace smr
         std::vector<int32_t> stage_seg_{0};
         std::vector<int64_t> stage_cells_;
         std::vector<int64_t> stage_max_chain_; // per stage: the most cells one chain has in it
+        bool items_built_ = false, plan_built_ = false;
+        void upload_with(bool items, bool plan);
         std::vector<dev::WaveItem> witems_;     // wave items of all stages, stage by stage
         std::vector<int64_t> stage_item_{0};    // first wave item of each stage (+ total)
         std::vector<int64_t> stage_nitems_;     // wave items per stage
@@ -176,6 +182,8 @@ namespace smr
         std::vector<std::shared_ptr<void>> replaced_;
 
         void release_replaced();
+        // the launch shape of the programs built for one adaptation pass
+        Tuning pass_tuning() const;
         void check_instances();
         void ensure_images(const std::vector<int>& ids);
         void ensure_arenas(int64_t n);
