@@ -245,14 +245,13 @@ namespace smr
         // level of every list), one allocation for all, and the launch that
         // writes them
         void emit_batch_lists(Ctx& ctx, const dev::SubmeshItem* d_items, const int64_t* d_cp, int64_t nwords, int n, const uint64_t* bits,
-                              const uint64_t* scan, const uint64_t* ht, const std::vector<int>& sel,
+                              const uint64_t* scan, const std::vector<uint64_t>& table, const std::vector<int>& sel,
                               std::vector<std::array<DeviceLists::One, dev::DL_N>>& lists, std::shared_ptr<DevBuffer>& mem)
         {
             hipStream_t s = ctx.stream;
             auto runs_of  = [](uint64_t v) { return int64_t(v >> 32); };
             auto cells_of = [](uint64_t v) { return int64_t(v & 0xFFFFFFFFull); };
             auto align    = [](size_t b) { return (b + 255) & ~size_t(255); };
-            const std::vector<uint64_t> table(ht, ht + size_t(n) * dev::DL_N * dev::SM_SEG);
             std::vector<dev::ListOut> outs(size_t(n) * dev::DL_N);
             std::vector<size_t> off(size_t(n) * dev::DL_N * 3);
             size_t bytes = 0;
@@ -424,14 +423,23 @@ namespace smr
             dev::submesh_batch_run_offsets(d_items, d_pp, PW, n, sets, counts, offsets, temp, tbytes, s);
             dev::submesh_batch_run_emit(d_items, d_pp, PW, n, sets, offsets, dr, cap, s);
             dev::submesh_batch_segments(d_items, d_pp, n, offsets, counts, dseg, s);
-            // the work lists: the current meshes' padded ids from their leaves,
-            // then the lists' bitmaps, packed counts, scan and per-level table
+            // the segment table (its last entry is the run count) and the runs
+            // come back first; the work lists (the current meshes' padded ids
+            // from their leaves, then the lists' bitmaps, packed counts, scan
+            // and per-level table) are built while the host completes the meshes
             const bool with_lists = lists && old_leaves;
             const int64_t lw      = with_lists ? cp[size_t(n)] : 0;
             const size_t ltb      = with_lists ? (dev::lists_scan_temp_bytes(lw) + 255) & ~size_t(255) : 0;
             const size_t lgb      = with_lists ? (size_t(n) * dev::DL_N * dev::SM_SEG * 8 + 255) & ~size_t(255) : 0;
+            char* hr      = ctx.readback->acquire(lgb + gb + size_t(cap) * rbytes);
+            uint64_t* ht  = reinterpret_cast<uint64_t*>(hr);
+            int32_t* hs   = reinterpret_cast<int32_t*>(hr + lgb);
+            int32_t* runs = reinterpret_cast<int32_t*>(hr + lgb + gb);
+            hip_check(hipMemcpyAsync(hs, dseg, gb, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(submesh runs)");
+            hip_check(hipMemcpyAsync(runs, dr, size_t(cap) * rbytes, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(submesh runs)");
+            ctx.mark_runs_back();
             DevBuffer lwork;
-            uint64_t *lbits = nullptr, *lscan = nullptr, *ltable = nullptr;
+            uint64_t *lbits = nullptr, *lscan = nullptr;
             if (with_lists)
             {
                 // the scan's temporary storage 256-byte aligned (its look-back
@@ -443,30 +451,20 @@ namespace smr
                 uint64_t* lcounts = lbits + lw;
                 lscan             = lcounts + lw;
                 void* ltemp       = lp + lsz;
-                ltable            = reinterpret_cast<uint64_t*>(lp + lsz + ltb);
+                uint64_t* ltable  = reinterpret_cast<uint64_t*>(lp + lsz + ltb);
                 dev::submesh_batch_build(d_items, d_pp, PW, n, old_leaves, osets, osets + nw, s);
                 dev::lists_batch_build(d_items, d_cp, lw, n, sets, osets, lbits, lcounts, lscan, ltemp, ltb, ltable, s);
-            }
-            // the list table, the segment table (its last entry is the run
-            // count), then the runs
-            char* hr      = ctx.readback->acquire(lgb + gb + size_t(cap) * rbytes);
-            uint64_t* ht  = reinterpret_cast<uint64_t*>(hr);
-            int32_t* hs   = reinterpret_cast<int32_t*>(hr + lgb);
-            int32_t* runs = reinterpret_cast<int32_t*>(hr + lgb + gb);
-            if (with_lists)
-            {
                 hip_check(hipMemcpyAsync(ht, ltable, size_t(n) * dev::DL_N * dev::SM_SEG * 8, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(lists)");
             }
-            hip_check(hipMemcpyAsync(hs, dseg, gb, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(submesh runs)");
-            hip_check(hipMemcpyAsync(runs, dr, size_t(cap) * rbytes, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(submesh runs)");
-            ctx.sync();
-            if (with_lists)
-            {
-                emit_batch_lists(ctx, d_items, d_cp, lw, n, lbits, lscan, ht, sel, *lists, *list_mem);
-            }
+            hip_check(hipEventSynchronize(ctx.runs_back), "hipEventSynchronize(runs)");
             const int64_t nruns = hs[int64_t(n) * dev::SM_NSETS * dev::SM_SEG - 1];
+            std::vector<uint64_t> table;
             if (nruns > cap)
             {
+                // a second round trip: the list table first (the readback
+                // buffer is acquired again)
+                ctx.sync();
+                table.assign(ht, ht + (with_lists ? size_t(n) * dev::DL_N * dev::SM_SEG : 0));
                 DevBuffer big;
                 int32_t* bd = static_cast<int32_t*>(big.get(size_t(nruns) * rbytes, s));
                 dev::submesh_batch_run_emit(d_items, d_pp, PW, n, sets, offsets, bd, nruns, s);
@@ -478,16 +476,29 @@ namespace smr
                 ctx.sync();
             }
             td.reset();
-            ScopedTimer tc(&ctx, "mesh completion");
-            std::vector<const MeshConfig*> cfgs(static_cast<size_t>(n));
-            std::vector<MRMeshData*> out(static_cast<size_t>(n));
-            for (int q = 0; q < n; ++q)
             {
-                const size_t j  = size_t(sel[size_t(q)]);
-                cfgs[size_t(q)] = &ms[j]->host.cfg;
-                out[size_t(q)]  = &meshes[j];
+                ScopedTimer tc(nullptr, "mesh completion");
+                std::vector<const MeshConfig*> cfgs(static_cast<size_t>(n));
+                std::vector<MRMeshData*> out(static_cast<size_t>(n));
+                for (int q = 0; q < n; ++q)
+                {
+                    const size_t j  = size_t(sel[size_t(q)]);
+                    cfgs[size_t(q)] = &ms[j]->host.cfg;
+                    out[size_t(q)]  = &meshes[j];
+                }
+                meshes_from_runs(cfgs, runs, hs, out);
             }
-            meshes_from_runs(cfgs, runs, hs, out);
+            if (with_lists)
+            {
+                // the lists' sizes and levels, then their runs
+                ScopedTimer te(&ctx, "mesh ids on device");
+                if (table.empty())
+                {
+                    ctx.sync();
+                    table.assign(ht, ht + size_t(n) * dev::DL_N * dev::SM_SEG);
+                }
+                emit_batch_lists(ctx, d_items, d_cp, lw, n, lbits, lscan, table, sel, *lists, *list_mem);
+            }
         }
 
         // update_cell_array_from_tag and the make_graduation fixed point of

@@ -356,6 +356,10 @@ namespace smr
         }
         staging.reset();
         readback.reset();
+        if (runs_back)
+        {
+            (void)hipEventDestroy(runs_back);
+        }
         if (d_err)
         {
             (void)hipFree(d_err);
@@ -366,6 +370,15 @@ namespace smr
             (void)hipStreamSynchronize(stream);
             (void)hipStreamDestroy(stream);
         }
+    }
+
+    void Ctx::mark_runs_back()
+    {
+        if (!runs_back)
+        {
+            hip_check(hipEventCreateWithFlags(&runs_back, hipEventDisableTiming), "hipEventCreate(runs)");
+        }
+        hip_check(hipEventRecord(runs_back, stream), "hipEventRecord(runs)");
     }
 
     void Ctx::require_device() const
@@ -1687,13 +1700,22 @@ namespace smr
             dev::submesh_run_offsets(p, dev::SM_NSETS, sets, counts, offsets, temp, tbytes, s);
             dev::submesh_run_emit(p, dev::SM_NSETS, sets, offsets, dr, cap, s);
             dev::submesh_segments(p, dev::SM_NSETS, offsets, counts, dseg, s);
-            // the work lists: bitmaps, packed counts, their scan, per-level table
+            // the segment table (its last entry is the run count) and the runs
+            // come back first; the work lists (bitmaps, packed counts, their
+            // scan, per-level table) are built while the host completes the mesh
             const int nlists     = dl ? (old_sets ? int(dev::DL_N) : int(dev::DL_UF_COPY)) : 0;
             const int64_t lwords = nlists ? dev::lists_words(p, cfg.min_level, cfg.max_level, nlists) : 0;
             const size_t ltb     = nlists ? (dev::lists_scan_temp_bytes(lwords) + 255) & ~size_t(255) : 0;
             const size_t lgb     = (size_t(std::max(nlists, 1)) * dev::SM_SEG * 8 + 255) & ~size_t(255);
+            char* h      = ctx.readback->acquire(lgb + gb + size_t(cap) * rbytes);
+            uint64_t* ht = reinterpret_cast<uint64_t*>(h);
+            int32_t* hs  = reinterpret_cast<int32_t*>(h + lgb);
+            int32_t* hr  = reinterpret_cast<int32_t*>(h + lgb + gb);
+            hip_check(hipMemcpyAsync(hs, dseg, gb, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(submesh)");
+            hip_check(hipMemcpyAsync(hr, dr, size_t(cap) * rbytes, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(submesh)");
+            ctx.mark_runs_back();
             DevBuffer lwork;
-            uint64_t *lbits = nullptr, *lscan = nullptr, *ltable = nullptr;
+            uint64_t *lbits = nullptr, *lscan = nullptr;
             if (nlists)
             {
                 // the scan's temporary storage 256-byte aligned (its look-back
@@ -1704,33 +1726,19 @@ namespace smr
                 uint64_t* lcounts  = lbits + lwords;
                 lscan              = lcounts + lwords;
                 void* ltemp        = lp + lsz;
-                ltable             = reinterpret_cast<uint64_t*>(lp + lsz + ltb);
+                uint64_t* ltable   = reinterpret_cast<uint64_t*>(lp + lsz + ltb);
                 dev::lists_build(p, cfg.min_level, cfg.max_level, sets, old_sets, nlists, lbits, lcounts, lscan, ltemp, ltb, ltable, s);
-            }
-            // the list table, the segment table (its last entry is the run
-            // count), then the runs
-            char* h      = ctx.readback->acquire(lgb + gb + size_t(cap) * rbytes);
-            uint64_t* ht = reinterpret_cast<uint64_t*>(h);
-            int32_t* hs  = reinterpret_cast<int32_t*>(h + lgb);
-            int32_t* hr  = reinterpret_cast<int32_t*>(h + lgb + gb);
-            if (nlists)
-            {
                 hip_check(hipMemcpyAsync(ht, ltable, size_t(nlists) * dev::SM_SEG * 8, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(lists)");
             }
-            hip_check(hipMemcpyAsync(hs, dseg, gb, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(submesh)");
-            hip_check(hipMemcpyAsync(hr, dr, size_t(cap) * rbytes, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(submesh)");
-            ctx.sync();
-            if (nlists)
-            {
-                // the lists' sizes and levels; their runs are written while the
-                // host builds the mesh
-                const std::vector<uint64_t> table(ht, ht + size_t(nlists) * dev::SM_SEG);
-                emit_lists(ctx, p, cfg, nlists, lbits, lscan, table, *dl);
-                dl->sets = std::move(sets_buf);
-            }
+            hip_check(hipEventSynchronize(ctx.runs_back), "hipEventSynchronize(runs)");
             const int64_t nruns = hs[dev::SM_NSETS * dev::SM_SEG - 1];
+            std::vector<uint64_t> table;
             if (nruns > cap)
             {
+                // a second round trip: the list table first (the readback
+                // buffer is acquired again)
+                ctx.sync();
+                table.assign(ht, ht + size_t(nlists) * dev::SM_SEG);
                 DevBuffer rb;
                 int32_t* big = static_cast<int32_t*>(rb.get(size_t(nruns) * rbytes, s));
                 dev::submesh_run_emit(p, dev::SM_NSETS, sets, offsets, big, nruns, s);
@@ -1743,6 +1751,17 @@ namespace smr
             }
             MRMeshData m;
             meshes_from_runs({&cfg}, hr, hs, {&m});
+            if (nlists)
+            {
+                // the lists' sizes and levels, then their runs
+                if (table.empty())
+                {
+                    ctx.sync();
+                    table.assign(ht, ht + size_t(nlists) * dev::SM_SEG);
+                }
+                emit_lists(ctx, p, cfg, nlists, lbits, lscan, table, *dl);
+                dl->sets = std::move(sets_buf);
+            }
             return m;
         }
     } // namespace

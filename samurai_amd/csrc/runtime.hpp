@@ -324,6 +324,10 @@ namespace smr
         std::unique_ptr<PinnedStaging> staging = std::make_unique<PinnedStaging>();
         // pinned host memory the runs of a new mesh are read back into
         std::unique_ptr<PinnedStaging> readback = std::make_unique<PinnedStaging>();
+        // recorded after a new mesh's runs are read back, so that the host can
+        // complete the mesh while later kernels run
+        hipEvent_t runs_back = nullptr;
+        void mark_runs_back();
         ~Ctx();
         int* d_flag() const { return d_err + 1; }
         bool has_device() const { return device >= 0; }
