@@ -249,50 +249,17 @@ namespace smr
                               std::vector<std::array<DeviceLists::One, dev::DL_N>>& lists, std::shared_ptr<DevBuffer>& mem)
         {
             hipStream_t s = ctx.stream;
-            auto runs_of  = [](uint64_t v) { return int64_t(v >> 32); };
-            auto cells_of = [](uint64_t v) { return int64_t(v & 0xFFFFFFFFull); };
-            auto align    = [](size_t b) { return (b + 255) & ~size_t(255); };
             std::vector<dev::ListOut> outs(size_t(n) * dev::DL_N);
-            std::vector<size_t> off(size_t(n) * dev::DL_N * 3);
+            std::vector<size_t> off(size_t(n) * dev::DL_N);
             size_t bytes = 0;
             for (int q = 0; q < n; ++q)
             {
                 std::array<DeviceLists::One, dev::DL_N>& ls = lists[size_t(sel[size_t(q)])];
                 for (int k = 0; k < dev::DL_N; ++k)
                 {
-                    const size_t e       = size_t(q) * dev::DL_N + size_t(k);
-                    const uint64_t* t    = table.data() + e * dev::SM_SEG;
-                    DeviceLists::One& o  = ls[size_t(k)];
-                    o.nruns  = runs_of(t[dev::SM_SEG - 1]) - runs_of(t[0]);
-                    o.ncells = cells_of(t[dev::SM_SEG - 1]) - cells_of(t[0]);
-                    o.level_run.resize(dev::SM_SEG);
-                    o.level_cell.resize(dev::SM_SEG);
-                    for (int l = 0; l < dev::SM_SEG; ++l)
-                    {
-                        o.level_run[size_t(l)]  = runs_of(t[l]) - runs_of(t[0]);
-                        o.level_cell[size_t(l)] = cells_of(t[l]) - cells_of(t[0]);
-                    }
-                    for (int l = 1; l < dev::SM_SEG; ++l)
-                    {
-                        if (o.level_run[size_t(l)] < o.level_run[size_t(l - 1)] || o.level_cell[size_t(l)] < o.level_cell[size_t(l - 1)])
-                        {
-                            throw std::logic_error("device work lists: inconsistent counts (mesh " + std::to_string(q) + ", list " +
-                                                   std::to_string(k) + ", level " + std::to_string(l) + ": " + std::to_string(t[0]) + " " +
-                                                   std::to_string(t[l - 1]) + " " + std::to_string(t[l]) + ")");
-                        }
-                    }
-                    const int64_t nchunks = (o.ncells + (int64_t(1) << dev::HINT_SHIFT) - 1) >> dev::HINT_SHIFT;
-                    off[3 * e] = bytes;
-                    bytes += align(size_t(std::max<int64_t>(o.nruns, 1)) * 20);
-                    off[3 * e + 1] = bytes;
-                    bytes += align(size_t(o.nruns + 1) * 8);
-                    off[3 * e + 2] = bytes;
-                    bytes += align(size_t(nchunks + 1) * 4);
-                    dev::ListOut& w = outs[e];
-                    w.run0          = runs_of(t[0]);
-                    w.cell0         = cells_of(t[0]);
-                    w.nruns         = o.nruns;
-                    w.ncells        = o.ncells;
+                    const size_t e = size_t(q) * dev::DL_N + size_t(k);
+                    off[e]         = bytes;
+                    bytes += list_from_table(table.data() + e * dev::SM_SEG, ls[size_t(k)], outs[e]);
                 }
             }
             mem     = std::make_shared<DevBuffer>();
@@ -302,14 +269,8 @@ namespace smr
                 std::array<DeviceLists::One, dev::DL_N>& ls = lists[size_t(sel[size_t(q)])];
                 for (int k = 0; k < dev::DL_N; ++k)
                 {
-                    const size_t e      = size_t(q) * dev::DL_N + size_t(k);
-                    dev::ListOut& w     = outs[e];
-                    w.iv                = reinterpret_cast<int32_t*>(d + off[3 * e]);
-                    w.pref              = reinterpret_cast<int64_t*>(d + off[3 * e + 1]);
-                    w.hint              = reinterpret_cast<int32_t*>(d + off[3 * e + 2]);
-                    ls[size_t(k)].iv    = w.iv;
-                    ls[size_t(k)].pref  = w.pref;
-                    ls[size_t(k)].hint  = w.hint;
+                    const size_t e = size_t(q) * dev::DL_N + size_t(k);
+                    bind_list_arrays(d + off[e], ls[size_t(k)], outs[e]);
                 }
             }
             // the destinations to the device, then the runs

@@ -848,6 +848,55 @@ namespace smr
         return dm;
     }
 
+    namespace
+    {
+        size_t align256(size_t n)
+        {
+            return (n + 255) & ~size_t(255);
+        }
+
+        int64_t list_chunks(int64_t ncells)
+        {
+            return (ncells + (int64_t(1) << dev::HINT_SHIFT) - 1) >> dev::HINT_SHIFT;
+        }
+    } // namespace
+
+    size_t list_from_table(const uint64_t* t, DeviceLists::One& o, dev::ListOut& w)
+    {
+        auto runs_of  = [](uint64_t v) { return int64_t(v >> 32); };
+        auto cells_of = [](uint64_t v) { return int64_t(v & 0xFFFFFFFFull); };
+        o.nruns       = runs_of(t[dev::SM_SEG - 1]) - runs_of(t[0]);
+        o.ncells      = cells_of(t[dev::SM_SEG - 1]) - cells_of(t[0]);
+        o.level_run.resize(dev::SM_SEG);
+        o.level_cell.resize(dev::SM_SEG);
+        for (int l = 0; l < dev::SM_SEG; ++l)
+        {
+            o.level_run[size_t(l)]  = runs_of(t[l]) - runs_of(t[0]);
+            o.level_cell[size_t(l)] = cells_of(t[l]) - cells_of(t[0]);
+            if (l > 0 && (o.level_run[size_t(l)] < o.level_run[size_t(l - 1)] || o.level_cell[size_t(l)] < o.level_cell[size_t(l - 1)]))
+            {
+                throw std::logic_error("device work lists: inconsistent counts at level " + std::to_string(l));
+            }
+        }
+        w.run0   = runs_of(t[0]);
+        w.cell0  = cells_of(t[0]);
+        w.nruns  = o.nruns;
+        w.ncells = o.ncells;
+        return align256(size_t(std::max<int64_t>(o.nruns, 1)) * 20) + align256(size_t(o.nruns + 1) * 8) +
+               align256(size_t(list_chunks(o.ncells) + 1) * 4);
+    }
+
+    void bind_list_arrays(char* d, DeviceLists::One& o, dev::ListOut& w)
+    {
+        const size_t a = align256(size_t(std::max<int64_t>(o.nruns, 1)) * 20), b = align256(size_t(o.nruns + 1) * 8);
+        w.iv           = reinterpret_cast<int32_t*>(d);
+        w.pref         = reinterpret_cast<int64_t*>(d + a);
+        w.hint         = reinterpret_cast<int32_t*>(d + a + b);
+        o.iv           = w.iv;
+        o.pref         = w.pref;
+        o.hint         = w.hint;
+    }
+
     void pack_device_list(Packer& p, DevWorkList& w, int kind, const MeshConfig& cfg, const DeviceLists::One& o)
     {
         w.host           = WorkList();
@@ -1601,58 +1650,20 @@ namespace smr
         void emit_lists(Ctx& ctx, const dev::PBits& p, const MeshConfig& cfg, int nlists, const uint64_t* bits, const uint64_t* scan,
                         const std::vector<uint64_t>& table, DeviceLists& dl)
         {
-            auto runs_of  = [](uint64_t v) { return int64_t(v >> 32); };
-            auto cells_of = [](uint64_t v) { return int64_t(v & 0xFFFFFFFFull); };
-            auto align    = [](size_t n) { return (n + 255) & ~size_t(255); };
-            dl.nlists     = nlists;
-            size_t bytes  = 0;
-            std::vector<size_t> off(size_t(nlists) * 3);
+            dl.nlists    = nlists;
+            size_t bytes = 0;
+            dev::ListOuts outs{};
+            std::vector<size_t> off(static_cast<size_t>(nlists));
             for (int k = 0; k < nlists; ++k)
             {
-                const uint64_t* t  = table.data() + size_t(k) * dev::SM_SEG;
-                DeviceLists::One& o = dl.l[k];
-                o.nruns  = runs_of(t[dev::SM_SEG - 1]) - runs_of(t[0]);
-                o.ncells = cells_of(t[dev::SM_SEG - 1]) - cells_of(t[0]);
-                o.level_run.resize(dev::SM_SEG);
-                o.level_cell.resize(dev::SM_SEG);
-                for (int l = 0; l < dev::SM_SEG; ++l)
-                {
-                    o.level_run[size_t(l)]  = runs_of(t[l]) - runs_of(t[0]);
-                    o.level_cell[size_t(l)] = cells_of(t[l]) - cells_of(t[0]);
-                }
-                for (int l = 1; l < dev::SM_SEG; ++l)
-                {
-                    if (o.level_run[size_t(l)] < o.level_run[size_t(l - 1)] || o.level_cell[size_t(l)] < o.level_cell[size_t(l - 1)])
-                    {
-                        throw std::logic_error("device work lists: inconsistent counts (list " + std::to_string(k) + ", level " + std::to_string(l) + ")");
-                    }
-                }
-                const int64_t nchunks = (o.ncells + (int64_t(1) << dev::HINT_SHIFT) - 1) >> dev::HINT_SHIFT;
-                off[size_t(3 * k)]     = bytes;
-                bytes += align(size_t(std::max<int64_t>(o.nruns, 1)) * 20);
-                off[size_t(3 * k + 1)] = bytes;
-                bytes += align(size_t(o.nruns + 1) * 8);
-                off[size_t(3 * k + 2)] = bytes;
-                bytes += align(size_t(nchunks + 1) * 4);
+                off[size_t(k)] = bytes;
+                bytes += list_from_table(table.data() + size_t(k) * dev::SM_SEG, dl.l[k], outs.l[k]);
             }
             dl.mem  = std::make_shared<DevBuffer>();
             char* d = static_cast<char*>(dl.mem->get(bytes, ctx.stream));
-            dev::ListOuts outs{};
             for (int k = 0; k < nlists; ++k)
             {
-                const uint64_t* t  = table.data() + size_t(k) * dev::SM_SEG;
-                DeviceLists::One& o = dl.l[k];
-                dev::ListOut& w    = outs.l[k];
-                w.iv               = reinterpret_cast<int32_t*>(d + off[size_t(3 * k)]);
-                w.pref             = reinterpret_cast<int64_t*>(d + off[size_t(3 * k + 1)]);
-                w.hint             = reinterpret_cast<int32_t*>(d + off[size_t(3 * k + 2)]);
-                w.run0             = runs_of(t[0]);
-                w.cell0            = cells_of(t[0]);
-                w.nruns            = o.nruns;
-                w.ncells           = o.ncells;
-                o.iv               = w.iv;
-                o.pref             = w.pref;
-                o.hint             = w.hint;
+                bind_list_arrays(d + off[size_t(k)], dl.l[k], outs.l[k]);
             }
             dev::lists_emit(p, cfg.min_level, cfg.max_level, nlists, bits, scan, outs, ctx.stream);
         }

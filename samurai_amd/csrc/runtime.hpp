@@ -423,6 +423,13 @@ namespace smr
         std::shared_ptr<DevBuffer> mem;
         DevBuffer sets;
     };
+    // a device-built list from its row of a list table (packed runs << 32 |
+    // cells scans at the first word of every level): its counts and
+    // per-level offsets into o, its first run and cell and counts into w;
+    // returns the bytes its arrays take; throws if the row is not increasing
+    size_t list_from_table(const uint64_t* t, DeviceLists::One& o, dev::ListOut& w);
+    // points o and w at the arrays of that list laid out from d
+    void bind_list_arrays(char* d, DeviceLists::One& o, dev::ListOut& w);
     // packs the image of a mesh whose work lists the device built: the table,
     // the recipes and the cells list from the host, the five program lists
     // l[DL_PROJECTION .. DL_MAXIMUM] (whose arrays mem holds) as they are
