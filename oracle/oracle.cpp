@@ -405,13 +405,68 @@ namespace oracle
             });
     }
 
-    // update_outer_corners_by_polynomial_extrapolation, ghost width 1
-    // (bc/apply_field_bc.hpp:315-466; polynomial_extrapolation.hpp:63-66): the
-    // diagonal ghost of each corner cell is a copy of the cell.
+    // ghost width of the mesh: max(stencil radius, prediction radius)
+    // (mesh_config.hpp:395)
+    inline int ghost_width(const Mesh& m)
+    {
+        return std::max(m.radius, m.pred_s);
+    }
+
+    // PolynomialExtrapolation<stencil 2k> (bc/polynomial_extrapolation.hpp:22-74):
+    // the ghost after the n = 2k - 1 known values v[0..n), in the reference's
+    // operation order (degree n - 1 through them, evaluated one step further)
+    inline double extrapolate(const double* v, int n)
+    {
+        if (n == 1)
+        {
+            return v[0];
+        }
+        if (n == 3)
+        {
+            return kPlant == 11 ? v[0] - v[2] * 3.0 + v[1] * 3.0 : v[0] - v[1] * 3.0 + v[2] * 3.0;
+        }
+        if (n == 5)
+        {
+            return v[0] - v[1] * 5.0 + v[2] * (kPlant == 10 ? 9.0 : 1.0E+1) - v[3] * 1.0E+1 + v[4] * 5.0;
+        }
+        throw std::invalid_argument("oracle: polynomial extrapolation beyond stencil 6");
+    }
+
+    // apply_extrapolation_bc_cells<2k> (bc/apply_field_bc.hpp:124-150) in
+    // direction d (Cartesian or diagonal; the line stencil of the B.C. turned
+    // towards d, stencil.hpp:226-231, 498-561, is {-(k-1)..k} * d): for each
+    // cell b of `at` (whose ghost b + k d exists when k > 1), the ghost
+    // b + k d from the cells b + j d, j = -(k-1)..k-1.
+    static void extrapolate_layer(Sim& s, int level, const DSet& at, const Dir& d, int k)
+    {
+        const Mesh& m = s.mesh;
+        at.for_each(
+            [&](int x, int y, int z)
+            {
+                if (k > 1 && !m.exists(level, x + k * d[0], y + k * d[1], z + k * d[2]))
+                {
+                    return;
+                }
+                double v[5];
+                for (int j = -(k - 1); j <= k - 1; ++j)
+                {
+                    v[j + k - 1] = U(s, level, x + j * d[0], y + j * d[1], z + j * d[2]);
+                }
+                U(s, level, x + k * d[0], y + k * d[1], z + k * d[2]) = extrapolate(v, 2 * k - 1);
+            });
+    }
+
+    // update_outer_corners_by_polynomial_extrapolation(level, direction)
+    // (bc/apply_field_bc.hpp:315-466; polynomial_extrapolation.hpp:63-72):
+    // layer k = 1..w of the diagonal through each corner cell by the
+    // extrapolation of stencil 2k along the diagonal (layer 1: a copy of the
+    // cell), then each off-diagonal cell of layer k (first non-zero axis at
+    // offset k, the others at offsets 1..w) copies the diagonal cell of layer k.
     static void corner_extrapolation(Sim& s, int level, size_t c)
     {
         const Mesh& m = s.mesh;
         const Dir& d  = m.corner_dirs[c];
+        const int w   = ghost_width(m);
         DSet corner   = on(m.corners[c], level);
         DSet cells    = on(intersection(m.ids[CELLS][size_t(level)], corner), level);
         cells.for_each(
@@ -419,6 +474,79 @@ namespace oracle
             {
                 U(s, level, x + d[0], y + d[1], z + d[2]) = kPlant == 4 ? U(s, level, x + d[0], y, z) : U(s, level, x, y, z);
             });
+        for (int k = 2; k <= w; ++k)
+        {
+            extrapolate_layer(s, level, cells, d, k);
+        }
+        // step 2 (:368-464): offsets (g_p - (k-1)) along the non-first axes
+        std::vector<int> nz;
+        for (int i = 0; i < m.dim; ++i)
+        {
+            if (d[i] != 0)
+            {
+                nz.push_back(i);
+            }
+        }
+        int ncombos = 1;
+        for (size_t p = 1; p < nz.size(); ++p)
+        {
+            ncombos *= w;
+        }
+        for (int k = 1; k <= w; ++k)
+        {
+            for (int combo = 0; combo < ncombos; ++combo)
+            {
+                Dir delta{0, 0, 0};
+                bool is_source = true;
+                int t          = combo;
+                for (size_t p = 1; p < nz.size(); ++p)
+                {
+                    const int g  = t % w;
+                    t           /= w;
+                    const int dp = g - (k - 1);
+                    delta[nz[p]] += dp * d[nz[p]];
+                    if (dp != 0)
+                    {
+                        is_source = false;
+                    }
+                }
+                if (is_source)
+                {
+                    continue;
+                }
+                cells.for_each(
+                    [&](int x, int y, int z)
+                    {
+                        const int kk = kPlant == 13 ? k - 1 : k;
+                        const int sx = x + kk * d[0], sy = y + kk * d[1], sz = z + kk * d[2];
+                        U(s, level, sx + delta[0], sy + delta[1], sz + delta[2]) = U(s, level, sx, sy, sz);
+                    });
+            }
+        }
+    }
+
+    // update_further_ghosts_by_polynomial_extrapolation(level, direction)
+    // (bc/apply_field_bc.hpp:499-563), first part: the ghost layers beyond
+    // those the B.C. fills (Dirichlet<1>: one; no B.C.: none) by the
+    // extrapolation of stencil 2 * layer from the boundary cells.  The second
+    // part (layers >= 2 next to the union cells' ghosts) has nothing to do
+    // where the ghost width is 1 or the mesh has one level, the only meshes
+    // built here.
+    static void further_ghosts(Sim& s, int level, const Dir& dir)
+    {
+        const Mesh& m   = s.mesh;
+        const int w     = ghost_width(m);
+        const int first = s.has_bc ? 2 : 1;
+        if (first > w)
+        {
+            return;
+        }
+        const DSet& dom = m.domain[size_t(level)];
+        DSet bdry       = difference(m.ids[CELLS][size_t(level)], translate(dom, scale(dir, -1)));
+        for (int layer = first; layer <= w; ++layer)
+        {
+            extrapolate_layer(s, level, bdry, dir, layer);
+        }
     }
 
     // project_corner_below (algorithm/update_outer_ghost.hpp:267-336)
@@ -545,8 +673,14 @@ namespace oracle
                 predict_bc(s, level + 1, dir);
             }
         }
-        // update_further_ghosts_by_polynomial_extrapolation: no layer beyond the
-        // first one exists for ghost width 1 (bc/apply_field_bc.hpp:499-563).
+        // the layers the B.C. leaves (update_outer_ghost.hpp:414-426)
+        if (level >= m.min_level)
+        {
+            for (auto& dir : cartesian_directions(m.dim))
+            {
+                further_ghosts(s, level, dir);
+            }
+        }
     }
 
     // projection_op_ (numeric/projection.hpp:15-65)
@@ -1666,6 +1800,126 @@ namespace oracle
         return out;
     }
 
+    // The WENO5 flux of Jiang & Shu (1996) on five flux values f[0..5) centred
+    // on f[2] (schemes/fv/operators/weno_impl.hpp:26-66): the three
+    // third-order candidates (eq. 2.8, Table I), their smoothness indicators
+    // (3.2-3.4), the non-linear weights from the linear ones 0.1, 0.6, 0.3 and
+    // eps = 1e-6 (2.15-2.16, Table II), and the weighted sum (2.10), each in
+    // the reference's operation order.  The reference squares with pow(x, 2),
+    // which compilers emit as x * x (the correctly rounded square).
+    inline double weno5_sq(double x)
+    {
+        return x * x;
+    }
+
+    inline double weno5_flux(const double* f)
+    {
+        const double eps = 1e-6;
+        const double q0  = 1. / 3 * f[0] - 7. / 6 * f[1] + 11. / 6 * f[2];
+        const double q1  = -1. / 6 * f[1] + 5. / 6 * f[2] + 1. / 3 * f[3];
+        const double q2  = 1. / 3 * f[2] + 5. / 6 * f[3] - 1. / 6 * f[4];
+        const double is0 = 13. / 12 * weno5_sq(f[0] - 2 * f[1] + f[2]) + 1. / 4 * weno5_sq(f[0] - 4 * f[1] + 3 * f[2]);
+        const double is1 = 13. / 12 * weno5_sq(f[1] - 2 * f[2] + f[3]) + 1. / 4 * weno5_sq(f[1] - f[3]);
+        const double is2 = 13. / 12 * weno5_sq(f[2] - 2 * f[3] + f[4]) + 1. / 4 * weno5_sq(3 * f[2] - 4 * f[3] + f[4]);
+        const double a0  = 0.1 / weno5_sq(eps + is0);
+        const double a1  = (kPlant == 12 ? 0.3 : 0.6) / weno5_sq(eps + is1);
+        const double a2  = (kPlant == 12 ? 0.6 : 0.3) / weno5_sq(eps + is2);
+        const double sum = a0 + a1 + a2;
+        const double w0 = a0 / sum, w1 = a1 / sum, w2 = a2 / sum;
+        return w0 * q0 + w1 * q1 + w2 * q2;
+    }
+
+    // the flux of make_convection_weno5(velocity) in direction d
+    // (schemes/fv/operators/convection_lin.hpp:95-178) from the six stencil
+    // values u[0..6) (cells -2..3 along d from the interface's left cell):
+    // v_d * u[0..5) for v_d >= 0, else v_d * (u[5], u[4], u[3], u[2], u[1])
+    inline double weno5_interface_flux(double vd, const double* u)
+    {
+        double f[5];
+        for (int i = 0; i < 5; ++i)
+        {
+            f[i] = (vd >= 0 ? u[i] : u[5 - i]) * vd;
+        }
+        return weno5_flux(f);
+    }
+
+    // make_convection_weno5(velocity)(u) on a mesh of one level, applied
+    // explicitly like every non-linear flux-based operator
+    // (explicit_flux_based_scheme__nonlin.hpp:35-86): for each direction, the
+    // interfaces between neighbouring cells (flux * h-factor into the left
+    // cell, (-flux) * h-factor into the right one), then the boundary
+    // interfaces, the inner cell taking flux * h-factor in the direction and
+    // (-flux) * h-factor in the opposite direction with the stencil moved one
+    // cell back (interface.hpp:494-508; flux_based_scheme__nonlin.hpp:366-398,
+    // 530-590).  Every stencil cell of a boundary cell's interface reaches up
+    // to three ghost layers, which the ghost update fills beforehand.
+    std::vector<double> convection_weno5(const Sim& s, const double* v)
+    {
+        const Mesh& m = s.mesh;
+        const int dim = m.dim;
+        const int cmin = m.ca_min_level(CELLS), cmax = m.ca_max_level(CELLS);
+        if (cmin != cmax)
+        {
+            throw std::invalid_argument("oracle: WENO5 is built for meshes whose cells are all at one level");
+        }
+        if (ghost_width(m) < 3)
+        {
+            throw std::invalid_argument("oracle: WENO5 needs a stencil radius of 3");
+        }
+        std::vector<double> out(size_t(m.nref), 0.0);
+        const int level   = cmin;
+        const DSet& cells = m.ids[CELLS][size_t(level)];
+        const DSet& dom   = m.domain[size_t(level)];
+        const double h    = 1.0 / (1 << level);
+        const double fa   = h_factor(dim, h, h);
+        for (int d = 0; d < dim; ++d)
+        {
+            std::array<int, 3> e{0, 0, 0};
+            e[size_t(d)] = 1;
+            const std::array<int, 3> me{-e[0], -e[1], -e[2]};
+            // flux of the interface whose stencil starts at cell (x, y, z) + j0 e
+            auto flux = [&](int x, int y, int z, int j0)
+            {
+                double u[6];
+                for (int j = 0; j < 6; ++j)
+                {
+                    const int k = j0 + j;
+                    u[j]        = s.u[size_t(m.index(level, x + k * e[0], y + k * e[1], z + k * e[2]))];
+                }
+                return weno5_interface_flux(v[d], u);
+            };
+            for_each_run(intersection(cells, translate(cells, me)),
+                         [&](int x0, int x1, int y, int z)
+                         {
+                             for (int x = x0; x < x1; ++x)
+                             {
+                                 const size_t p = size_t(m.index(level, x, y, z));
+                                 const size_t q = size_t(m.index(level, x + e[0], y + e[1], z + e[2]));
+                                 const double F = flux(x, y, z, -2);
+                                 out[p] += F * fa;
+                                 out[q] += (-F) * fa;
+                             }
+                         });
+            for_each_run(difference(cells, translate(dom, me)),
+                         [&](int x0, int x1, int y, int z)
+                         {
+                             for (int x = x0; x < x1; ++x)
+                             {
+                                 out[size_t(m.index(level, x, y, z))] += flux(x, y, z, -2) * fa;
+                             }
+                         });
+            for_each_run(difference(cells, translate(dom, e)),
+                         [&](int x0, int x1, int y, int z)
+                         {
+                             for (int x = x0; x < x1; ++x)
+                             {
+                                 out[size_t(m.index(level, x, y, z))] += (-flux(x, y, z, kPlant == 14 ? -2 : -3)) * fa;
+                             }
+                         });
+        }
+        return out;
+    }
+
     // make_convection_upwind<Field>(velocity_field)(u) for a scalar field
     // (schemes/fv/operators/convection_lin.hpp:184-264, the LinearHeterogeneous
     // flux): at each interface the velocity is 0.5 * (v[left] + v[right]) of the
@@ -1829,6 +2083,13 @@ extern "C"
 {
     const char* orc_last_error() { return g_err.c_str(); }
 
+    // scheme stencil radius 1 on every mesh; 2 or 3 (ghost layers filled by
+    // polynomial extrapolation up to stencil 6) on meshes of one level
+    static bool radius_supported(int radius, int min_level, int max_level)
+    {
+        return radius == 1 || (radius >= 2 && radius <= 3 && min_level == max_level);
+    }
+
     void* orc_create_box(int dim, int min_level, int max_level, int pred_s, int radius, int grad_width)
     {
         try
@@ -1837,7 +2098,8 @@ extern "C"
             {
                 throw std::invalid_argument("dim must be 2 or 3");
             }
-            if (radius != 1 || grad_width < 1 || pred_s < 0 || pred_s > 1 || max_level < min_level || min_level < 1)
+            if (!radius_supported(radius, min_level, max_level) || grad_width < 1 || pred_s < 0 || pred_s > 1 || max_level < min_level ||
+                min_level < 1)
             {
                 throw std::invalid_argument("unsupported mesh configuration");
             }
@@ -1856,7 +2118,8 @@ extern "C"
     {
         try
         {
-            if ((dim != 2 && dim != 3) || radius != 1 || pred_s < 0 || pred_s > 1 || max_level < min_level || min_level < 1)
+            if ((dim != 2 && dim != 3) || !radius_supported(radius, min_level, max_level) || pred_s < 0 || pred_s > 1 || max_level < min_level ||
+                min_level < 1)
             {
                 throw std::invalid_argument("unsupported mesh configuration");
             }
@@ -2095,6 +2358,24 @@ extern "C"
             std::copy(r.begin(), r.end(), out);
         });
         return 0;
+    }
+
+    // out (storage size) = make_convection_weno5(v)(u) after update_ghost_mr(u)
+    int orc_convection_weno5(void* p, const double* v, double* out)
+    {
+        auto* s = static_cast<Sim*>(p);
+        ORC_TRY({
+            update_ghost_mr(*s);
+            std::vector<double> r = convection_weno5(*s, v);
+            std::copy(r.begin(), r.end(), out);
+        });
+        return 0;
+    }
+
+    // the WENO5 flux of five values (diagnostic)
+    double orc_weno5_flux(const double* f)
+    {
+        return weno5_flux(f);
     }
 
     // out = make_convection_upwind<Field>(velocity_field)(u) after

@@ -48,6 +48,9 @@ def lib():
         L.orc_upwind_only.argtypes = [vp, P(dbl), dbl]
         L.orc_convection_upwind.argtypes = [vp, P(dbl), P(dbl)]
         L.orc_convection_upwind_nonlinear.argtypes = [vp, P(dbl)]
+        L.orc_convection_weno5.argtypes = [vp, P(dbl), P(dbl)]
+        L.orc_weno5_flux.restype = dbl
+        L.orc_weno5_flux.argtypes = [P(dbl)]
         L.orc_upwind_scalar_burgers_step.argtypes = [vp, P(dbl), dbl]
         L.orc_convection_upwind_heterogeneous.argtypes = [vp, P(P(dbl)), P(dbl)]
         L.orc_upwind_hand_loop.argtypes = [vp, dbl]
@@ -214,6 +217,14 @@ class OracleSim:
         _check(lib().orc_convection_upwind(self.h, _dp(v), _dp(out)))
         return out
 
+    def convection_weno5(self, v):
+        """make_convection_weno5(v)(u) after update_ghost_mr(u) (one-level
+        mesh, stencil radius 3): the output storage."""
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        out = np.zeros(self.nb_cells(REFERENCE))
+        _check(lib().orc_convection_weno5(self.h, _dp(v), _dp(out)))
+        return out
+
     def upwind_hand_loop(self, dt):
         """update_ghost_mr, then the README's for_each_interval upwind loop and swap."""
         _check(lib().orc_upwind_hand_loop(self.h, float(dt)))
@@ -227,3 +238,9 @@ class OracleSim:
         """apply_field_bc(u, direction) at every level, with the sim's Dirichlet value."""
         d = list(direction) + [0] * (3 - len(direction))
         _check(lib().orc_apply_field_bc_direction(self.h, *[int(v) for v in d]))
+
+
+def weno5_flux(f):
+    """The oracle's WENO5 flux of five values."""
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    return lib().orc_weno5_flux(_dp(f))
