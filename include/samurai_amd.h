@@ -51,14 +51,16 @@ enum smr_status
 
 /* samurai::mesh_config<dim, prediction_stencil_radius> (mesh_config.hpp:16-432),
  * with max_stencil_size(2).disable_minimal_ghost_width() as in
- * demos/FiniteVolume/advection_2d.cpp:101 (max_stencil_radius must be 1). */
+ * demos/FiniteVolume/advection_2d.cpp:101: max_stencil_radius 1, or 2 or 3
+ * (max_stencil_size 4 or 6: ghost layers of that width, the far ones filled
+ * by polynomial extrapolation) on meshes with min_level == max_level. */
 typedef struct smr_mesh_config
 {
     int dim;                       /* 2 or 3 */
     int min_level;
     int max_level;
     int prediction_stencil_radius; /* 0 or 1 */
-    int max_stencil_radius;        /* 1 */
+    int max_stencil_radius;        /* 1; 2 or 3 when min_level == max_level */
     int graduation_width;          /* default 1 */
 } smr_mesh_config;
 
@@ -302,6 +304,15 @@ int smr_convection_upwind_nonlinear(smr_field* u, smr_field* out);
  * (explicit_flux_based_scheme__lin_het.hpp:35-110).  The ghosts of the
  * velocity components, then of u, are updated first (FV_scheme.hpp:176-196). */
 int smr_convection_upwind_field(smr_field* const* velocity, smr_field* u, smr_field* out);
+/* out = make_convection_weno5<Field>(velocity)(u)
+ * (schemes/fv/operators/convection_lin.hpp:95-178, the WENO5 flux of
+ * weno_impl.hpp:26-66): six-cell stencils along each direction, applied
+ * explicitly like the other flux-based operators.  The mesh holds its cells
+ * at one level with ghost layers of width 3 (smr_mesh_config
+ * max_stencil_radius = 3, min_level == max_level: the reference's
+ * max_stencil_size(6)); the far ghost layers come from polynomial
+ * extrapolation.  u's ghosts are updated first; velocity: dim values. */
+int smr_convection_weno5(const double* velocity, smr_field* u, smr_field* out);
 /* Diagnostic: the ordered contribution lists behind smr_convection_upwind,
  * computed on the host.  Call with null arrays to get *nrows and *nterms;
  * row: nrows storage slots, ptr: nrows+1, src: 2*nterms (second -1 for a

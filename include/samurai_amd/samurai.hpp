@@ -138,7 +138,7 @@ namespace samurai_amd
         }
         mesh_config& max_stencil_size(int s)
         {
-            c_.max_stencil_radius = s / 2;
+            c_.max_stencil_radius = s / 2 + s % 2; // mesh_config.hpp:71-78
             return *this;
         }
         mesh_config& max_stencil_radius(int r)
@@ -637,6 +637,45 @@ namespace samurai_amd
             a[d] = v[d];
         }
         return ConvectionUpwind<Field>(a);
+    }
+
+    // make_convection_weno5<Field>(velocity) (convection_lin.hpp:95-178): a mesh
+    // of one level built with max_stencil_size(6)
+    template <class Field>
+    class ConvectionWeno5
+    {
+      public:
+        explicit ConvectionWeno5(const std::array<double, Field::dim>& v)
+            : v_(v)
+        {
+        }
+        Field operator()(Field& u) const
+        {
+            Field out("convection(" + u.name() + ")", u.mesh());
+            double v[3] = {0, 0, 0};
+            for (std::size_t d = 0; d < Field::dim; ++d)
+            {
+                v[d] = v_[d];
+            }
+            check(smr_convection_weno5(v, u.handle(), out.handle()));
+            u.touched();
+            out.touched();
+            return out;
+        }
+
+      private:
+        std::array<double, Field::dim> v_;
+    };
+
+    template <class Field, class Vec>
+    ConvectionWeno5<Field> make_convection_weno5(const Vec& v)
+    {
+        std::array<double, Field::dim> a{};
+        for (std::size_t d = 0; d < Field::dim; ++d)
+        {
+            a[d] = v[d];
+        }
+        return ConvectionWeno5<Field>(a);
     }
 
     // mra_config (mr/config.hpp:10-68)

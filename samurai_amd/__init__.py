@@ -122,6 +122,7 @@ def lib():
         _sig(L, "smr_batch_load_tuning", "argtypes", [vp, ctypes.c_char_p])
         _sig(L, "smr_convection_upwind", "argtypes", [P(dbl), vp, vp])
         _sig(L, "smr_convection_upwind_nonlinear", "argtypes", [vp, vp])
+        _sig(L, "smr_convection_weno5", "argtypes", [P(dbl), vp, vp])
         _sig(L, "smr_upwind_scalar_burgers", "argtypes", [P(dbl), dbl, vp, vp])
         _sig(L, "smr_convection_upwind_field", "argtypes", [ctypes.POINTER(vp), vp, vp])
         _sig(L, "smr_upwind_hand_loop", "argtypes", [dbl, vp, vp])
@@ -458,6 +459,28 @@ class ConvectionUpwind:
 
 def make_convection_upwind(velocity=None):
     return ConvectionUpwind(velocity)
+
+
+class ConvectionWeno5:
+    """make_convection_weno5<Field>(velocity) (schemes/fv/operators/convection_lin.hpp:95-178)
+    on a mesh of one level with ghost layers of width 3 (mesh_config(...,
+    max_stencil_radius=3), min_level == max_level); calling it on u returns
+    the operator applied to u in a new field (or in `out`)."""
+
+    def __init__(self, velocity):
+        self.velocity = np.ascontiguousarray(np.asarray(velocity, dtype=np.float64).ravel())
+
+    def __call__(self, u, out=None):
+        if out is None:
+            out = make_scalar_field("convection(" + u.name + ")", u.mesh)
+        v = np.zeros(3)
+        v[: len(self.velocity)] = self.velocity
+        _check(lib().smr_convection_weno5(_dp(v), u.h, out.h))
+        return out
+
+
+def make_convection_weno5(velocity):
+    return ConvectionWeno5(velocity)
 
 
 def convection_terms(mesh, velocity=None):

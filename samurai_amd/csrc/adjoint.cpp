@@ -207,14 +207,23 @@ namespace smr
                     const int op    = r.op[size_t(q)];
                     const int64_t d = r.dst[size_t(q)];
                     const int64_t a = r.src_ptr[size_t(q)], b = r.src_ptr[size_t(q) + 1];
-                    if (op == RECIPE_DIRICHLET && !has_bc)
+                    if ((op == RECIPE_DIRICHLET && !has_bc) || (op == RECIPE_COPY_NO_BC && has_bc))
                     {
                         continue; // not written (op_recipe)
                     }
                     dst.push_back(d);
-                    if (op == RECIPE_COPY)
+                    if (op == RECIPE_COPY || op == RECIPE_COPY_NO_BC)
                     {
                         t.push_back({d, r.src[size_t(a)], 1.0});
+                    }
+                    else if (op == RECIPE_EXTRAPOLATE)
+                    {
+                        static const double w3[3] = {1.0, -3.0, 3.0}, w5[5] = {1.0, -5.0, 10.0, -10.0, 5.0};
+                        const double* w = b - a == 3 ? w3 : w5;
+                        for (int64_t e = a; e < b; ++e)
+                        {
+                            t.push_back({d, r.src[size_t(e)], w[e - a]});
+                        }
                     }
                     else if (op == RECIPE_DIRICHLET)
                     {

@@ -87,6 +87,10 @@ namespace smr
             }
             for (size_t k = 0; k + 1 < sp.size(); ++k)
             {
+                if (has_bc && r.host.stage_without_bc_only(k))
+                {
+                    continue; // copies for B.C.-less fields only
+                }
                 dev::Segment g = make_seg(dev::SEG_RECIPE, cfg);
                 g.rdst         = r.dst;
                 g.rop          = r.op;

@@ -498,9 +498,16 @@ extern "C"
                 }
                 // outer ghosts (boundary condition, corners, BC projection and
                 // prediction): each recipe reads its sources and writes one value
+                // (for a field with a B.C.: the copies of B.C.-less fields are skipped)
                 for (const smr::Recipes& r : g.outer)
                 {
-                    bytes += 8 * (int64_t(r.size()) + int64_t(r.src.size()));
+                    for (size_t q = 0; q < r.size(); ++q)
+                    {
+                        if (r.op[q] != smr::RECIPE_COPY_NO_BC)
+                        {
+                            bytes += 8 * (1 + r.src_ptr[q + 1] - r.src_ptr[q]);
+                        }
+                    }
                 }
                 return 0;
             });
@@ -1167,6 +1174,17 @@ extern "C"
                     v[d] = &velocity[d]->f;
                 }
                 smr::convection_upwind_heterogeneous(v, u->f, out->f);
+                return 0;
+            });
+    }
+
+    int smr_convection_weno5(const double* velocity, smr_field* u, smr_field* out)
+    {
+        return guarded(
+            [&]
+            {
+                require(velocity && u && out, "null argument");
+                smr::convection_weno5(velocity, u->f, out->f);
                 return 0;
             });
     }

@@ -692,24 +692,39 @@ namespace smr::dev
         op_predict_children_at<DIM, S>(l, c, locate(l, c), dst, src);
     }
 
-    // one outer-ghost recipe: copy, Dirichlet<1> (bc/dirichlet.hpp:29), or the BC
-    // projection average (algorithm/update_outer_ghost.hpp:78-131)
+    // one outer-ghost recipe: copy, Dirichlet<1> (bc/dirichlet.hpp:29), the BC
+    // projection average (algorithm/update_outer_ghost.hpp:78-131), the
+    // polynomial extrapolation of stencil 4 or 6 from 3 or 5 sources
+    // (bc/polynomial_extrapolation.hpp:63-72, same operation order), or the
+    // layer-1 copy of a field without a B.C.
     __host__ __device__ __forceinline__ void op_recipe(const int64_t* __restrict__ dst, const int32_t* __restrict__ op, const int64_t* __restrict__ sp,
                                               const int64_t* __restrict__ src, int64_t r, double* u, int has_bc, double bc_value)
     {
         const int o = op[r];
-        if (o == 1 && !has_bc)
+        if ((o == 1 && !has_bc) || (o == 4 && has_bc))
         {
             return;
         }
         double v;
-        if (o == 0)
+        if (o == 0 || o == 4)
         {
             v = u[src[sp[r]]];
         }
         else if (o == 1)
         {
             v = 2 * bc_value - u[src[sp[r]]];
+        }
+        else if (o == 3)
+        {
+            const int64_t* s = src + sp[r];
+            if (sp[r + 1] - sp[r] == 3)
+            {
+                v = u[s[0]] - u[s[1]] * 3.0 + u[s[2]] * 3.0;
+            }
+            else
+            {
+                v = u[s[0]] - u[s[1]] * 5.0 + u[s[2]] * 10.0 - u[s[3]] * 10.0 + u[s[4]] * 5.0;
+            }
         }
         else
         {

@@ -447,6 +447,11 @@ namespace smr::dev
     // two stencil slots (velocity[d] for d >= dim may be null)
     void conv_het_gather(const int64_t* row, const int64_t* ptr, const int64_t* src, const double* coef, const int8_t* dir, int64_t nrows,
                          const double* u, const double* const* velocity, double* out, hipStream_t s);
+    // make_convection_weno5(velocity): term k of direction dir[k] is
+    // coef[k] * F, F the WENO5 flux of velocity v[dir[k]] over the six stencil
+    // slots src[6k..6k+6) (v: three host values)
+    void conv_weno5_gather(const int64_t* row, const int64_t* ptr, const int64_t* src, const double* coef, const int8_t* dir, int64_t nrows,
+                           const double* v, const double* u, double* out, hipStream_t s);
     // launch shape of the upwind kernel (cells per thread, XCD remap, stores)
     int upwind_variant_count();
     void set_upwind_variant(int id);

@@ -216,6 +216,70 @@ namespace smr::dev
             out[row[r]] = acc;
         }
 
+        // The WENO5 flux of Jiang & Shu (1996) on five flux values centred on
+        // f[2] (schemes/fv/operators/weno_impl.hpp:26-66): third-order
+        // candidates (eq. 2.8), smoothness indicators (3.2-3.4), non-linear
+        // weights from 0.1, 0.6, 0.3 with eps = 1e-6 (2.15-2.16), weighted sum
+        // (2.10), in the reference's operation order; its pow(x, 2) is the
+        // correctly rounded square x * x.
+        __device__ __forceinline__ double weno5_flux(double f0, double f1, double f2, double f3, double f4)
+        {
+            const double eps = 1e-6;
+            const double q0  = 1. / 3 * f0 - 7. / 6 * f1 + 11. / 6 * f2;
+            const double q1  = -1. / 6 * f1 + 5. / 6 * f2 + 1. / 3 * f3;
+            const double q2  = 1. / 3 * f2 + 5. / 6 * f3 - 1. / 6 * f4;
+            const double b0 = f0 - 2 * f1 + f2, c0 = f0 - 4 * f1 + 3 * f2;
+            const double b1 = f1 - 2 * f2 + f3, c1 = f1 - f3;
+            const double b2 = f2 - 2 * f3 + f4, c2 = 3 * f2 - 4 * f3 + f4;
+            const double is0 = 13. / 12 * (b0 * b0) + 1. / 4 * (c0 * c0);
+            const double is1 = 13. / 12 * (b1 * b1) + 1. / 4 * (c1 * c1);
+            const double is2 = 13. / 12 * (b2 * b2) + 1. / 4 * (c2 * c2);
+            const double e0 = eps + is0, e1 = eps + is1, e2 = eps + is2;
+            const double a0  = 0.1 / (e0 * e0);
+            const double a1  = 0.6 / (e1 * e1);
+            const double a2  = 0.3 / (e2 * e2);
+            const double sum = a0 + a1 + a2;
+            return a0 / sum * q0 + a1 / sum * q1 + a2 / sum * q2;
+        }
+
+        // make_convection_weno5(velocity) (convection_lin.hpp:95-178): term k of
+        // direction d = dir[k] is coef[k] * F with F the WENO5 flux of
+        // v_d * (u[s0..s4]) for v_d >= 0, else of v_d * (u[s5], u[s4], ..., u[s1])
+        __global__ __launch_bounds__(BLOCK) void k_conv_weno5_gather(const int64_t* __restrict__ row, const int64_t* __restrict__ ptr,
+                                                                     const int64_t* __restrict__ src, const double* __restrict__ coef,
+                                                                     const int8_t* __restrict__ dir, int64_t nrows, double v0, double v1,
+                                                                     double v2, const double* __restrict__ u, double* __restrict__ out)
+        {
+            const int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+            if (r >= nrows)
+            {
+                return;
+            }
+            double acc = 0;
+            for (int64_t k = ptr[r]; k < ptr[r + 1]; ++k)
+            {
+                const int d          = dir[k];
+                const double vd      = d == 0 ? v0 : (d == 1 ? v1 : v2);
+                const int64_t* s     = src + 6 * k;
+                double w[6];
+                for (int j = 0; j < 6; ++j)
+                {
+                    w[j] = u[s[j]];
+                }
+                double F;
+                if (vd >= 0)
+                {
+                    F = weno5_flux(w[0] * vd, w[1] * vd, w[2] * vd, w[3] * vd, w[4] * vd);
+                }
+                else
+                {
+                    F = weno5_flux(w[5] * vd, w[4] * vd, w[3] * vd, w[2] * vd, w[1] * vd);
+                }
+                acc += F * coef[k];
+            }
+            out[row[r]] = acc;
+        }
+
         __device__ __forceinline__ int64_t readlane64(int64_t v, int lane)
         {
             const uint64_t w  = uint64_t(v);
@@ -725,6 +789,16 @@ namespace smr::dev
         {
             hipLaunchKernelGGL(k_conv_het_gather, dim3(nblocks(nrows)), dim3(BLOCK), 0, s, row, ptr, src, coef, dir, nrows, u, velocity[0],
                                velocity[1], velocity[2], out);
+        }
+    }
+
+    void conv_weno5_gather(const int64_t* row, const int64_t* ptr, const int64_t* src, const double* coef, const int8_t* dir, int64_t nrows,
+                           const double* v, const double* u, double* out, hipStream_t s)
+    {
+        if (nrows > 0)
+        {
+            hipLaunchKernelGGL(k_conv_weno5_gather, dim3(nblocks(nrows)), dim3(BLOCK), 0, s, row, ptr, src, coef, dir, nrows, v[0], v[1], v[2], u,
+                               out);
         }
     }
 

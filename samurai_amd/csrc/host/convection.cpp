@@ -402,6 +402,114 @@ namespace smr
         }
     } // namespace
 
+    ConvTerms build_weno5_terms(const MRMeshData& m)
+    {
+        const int dim     = m.cfg.dim;
+        const auto& cells = m.ids[CELLS];
+        const int cmin = ca_min_level(cells), cmax = ca_max_level(cells);
+        if (cmin != cmax)
+        {
+            throw std::invalid_argument("WENO5 is built for meshes whose cells are all at one level");
+        }
+        if (std::max(m.cfg.radius, m.cfg.pred_s) < 3)
+        {
+            throw std::invalid_argument("WENO5 needs ghost layers of width 3 (a mesh of stencil radius 3, max_stencil_size 6)");
+        }
+        const int level    = cmin;
+        const int n        = 1 << level;
+        const double h     = 1.0 / double(n);
+        const double fa    = h_factor(dim, h, h);
+        const LevelSet& cl = cells[size_t(level)];
+        struct Term
+        {
+            int64_t out, s[6];
+            double c;
+            int8_t dir;
+        };
+        std::vector<Term> ev;
+        for (int d = 0; d < dim; ++d)
+        {
+            std::array<int, 3> e{0, 0, 0};
+            e[size_t(d)] = 1;
+            const std::array<int, 3> me{-e[0], -e[1], -e[2]};
+            // the term of cell `out` for the interface whose stencil starts at (x, y, z) + j0 e
+            auto term = [&](int64_t out, int32_t x, int32_t y, int32_t z, int j0, double c)
+            {
+                Term t{out, {}, c, int8_t(d)};
+                for (int j = 0; j < 6; ++j)
+                {
+                    const int k = j0 + j;
+                    t.s[j]      = m.index(level, x + k * e[0], y + k * e[1], z + k * e[2]);
+                }
+                ev.push_back(t);
+            };
+            set_intersection(cl, translate(cl, me))
+                .for_each_run(
+                    [&](int32_t x0, int32_t x1, int32_t y, int32_t z)
+                    {
+                        for (int32_t x = x0; x < x1; ++x)
+                        {
+                            term(m.index(level, x, y, z), x, y, z, -2, fa);
+                            term(m.index(level, x + e[0], y + e[1], z + e[2]), x, y, z, -2, -fa);
+                        }
+                    });
+            minus_box(cl, me, {n + me[0], n + me[1], n + me[2]})
+                .for_each_run(
+                    [&](int32_t x0, int32_t x1, int32_t y, int32_t z)
+                    {
+                        for (int32_t x = x0; x < x1; ++x)
+                        {
+                            term(m.index(level, x, y, z), x, y, z, -2, fa);
+                        }
+                    });
+            minus_box(cl, e, {n + e[0], n + e[1], n + e[2]})
+                .for_each_run(
+                    [&](int32_t x0, int32_t x1, int32_t y, int32_t z)
+                    {
+                        for (int32_t x = x0; x < x1; ++x)
+                        {
+                            term(m.index(level, x, y, z), x, y, z, -3, -fa);
+                        }
+                    });
+        }
+        // group by output slot, keeping the order of each slot's terms
+        const int64_t nref = m.nref;
+        std::vector<int64_t> cnt(size_t(nref) + 1, 0);
+        for (const Term& t : ev)
+        {
+            ++cnt[size_t(t.out) + 1];
+        }
+        ConvTerms out;
+        std::vector<int64_t> pos(size_t(nref), 0);
+        int64_t acc = 0;
+        for (int64_t i = 0; i < nref; ++i)
+        {
+            pos[size_t(i)] = acc;
+            if (cnt[size_t(i) + 1] > 0)
+            {
+                out.row.push_back(i);
+            }
+            acc += cnt[size_t(i) + 1];
+        }
+        out.src.resize(6 * ev.size());
+        out.coef.resize(ev.size());
+        out.dir.resize(ev.size());
+        for (const Term& t : ev)
+        {
+            const int64_t k = pos[size_t(t.out)]++;
+            std::copy(t.s, t.s + 6, out.src.begin() + 6 * k);
+            out.coef[size_t(k)] = t.c;
+            out.dir[size_t(k)]  = t.dir;
+        }
+        out.ptr.clear();
+        for (int64_t r : out.row)
+        {
+            out.ptr.push_back(pos[size_t(r)] - cnt[size_t(r) + 1]);
+        }
+        out.ptr.push_back(int64_t(ev.size()));
+        return out;
+    }
+
     ConvTerms build_burgers_terms(const MRMeshData& m)
     {
         return group_events(interface_events(m), m.nref, true);

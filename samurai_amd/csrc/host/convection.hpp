@@ -49,4 +49,16 @@ namespace smr
     // (coef[k] * w) * (w >= 0 ? u[s0] : u[s1]) with w = 0.5 * (v_d[s0] + v_d[s1])
     // (s0, s1 = src[2k], src[2k+1]), v_d the storage of velocity component d.
     ConvTerms build_heterogeneous_terms(const MRMeshData& m);
+
+    // make_convection_weno5(velocity)(u) (convection_lin.hpp:95-178) on a mesh
+    // whose cells are all at one level, with ghost layers of width 3: for
+    // each direction the interfaces between neighbouring cells, then the
+    // boundary interfaces in the direction and in the opposite direction
+    // (explicit_flux_based_scheme__nonlin.hpp:35-86; interface.hpp:441-509).
+    // Term k, of direction d = dir[k], is coef[k] * F_d(u[src[6k]], ...,
+    // u[src[6k+5]]) with F_d the WENO5 flux of velocity v_d over the six
+    // stencil cells -2..3 along d from the interface's left cell (-3..2 from
+    // the inner cell at a boundary in the opposite direction); coef is the
+    // h-factor, negative for the right cell.
+    ConvTerms build_weno5_terms(const MRMeshData& m);
 } // namespace smr

@@ -333,9 +333,11 @@ namespace smr
 
     MRMeshData build_mesh(const MeshConfig& cfg, CellArray&& cells)
     {
-        if (cfg.radius != 1)
+        // radius 2 or 3: ghost layers filled by polynomial extrapolation up to
+        // stencil 6 (bc/polynomial_extrapolation.hpp:17), built for meshes of one level
+        if (cfg.radius != 1 && !(cfg.radius >= 2 && cfg.radius <= 3 && cfg.min_level == cfg.max_level))
         {
-            throw std::invalid_argument("only a scheme stencil radius of 1 is supported");
+            throw std::invalid_argument("a scheme stencil radius of 1 is supported, and of 2 or 3 on meshes with min_level == max_level");
         }
         if (cfg.pred_s < 0 || cfg.pred_s > 1)
         {

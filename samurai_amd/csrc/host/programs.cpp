@@ -93,6 +93,18 @@ namespace smr
         src_ptr.push_back(int64_t(src.size()));
     }
 
+    bool Recipes::stage_without_bc_only(size_t k) const
+    {
+        for (int64_t q = stage_ptr[k]; q < stage_ptr[k + 1]; ++q)
+        {
+            if (op[size_t(q)] != RECIPE_COPY_NO_BC)
+            {
+                return false;
+            }
+        }
+        return stage_ptr[k + 1] > stage_ptr[k];
+    }
+
     void Recipes::stage()
     {
         const size_t n = dst.size();
@@ -128,6 +140,30 @@ namespace smr
                 lr      = std::max(lr, s);
             }
             nstages = std::max(nstages, s + 1);
+        }
+        // the copies of B.C.-less fields get stages of their own (stage s ->
+        // 2s for the other recipes, 2s + 1 for them, empty stages dropped), so
+        // that a field with a B.C. skips whole stages (same order of every
+        // dependent pair: a later stage stays later)
+        {
+            std::vector<int> used(size_t(2 * nstages), 0);
+            for (size_t r = 0; r < n; ++r)
+            {
+                st[r]           = 2 * st[r] + (op[r] == RECIPE_COPY_NO_BC ? 1 : 0);
+                used[size_t(st[r])] = 1;
+            }
+            std::vector<int> renum(used.size(), 0);
+            int k = 0;
+            for (size_t q = 0; q < used.size(); ++q)
+            {
+                renum[q] = k;
+                k += used[q];
+            }
+            for (size_t r = 0; r < n; ++r)
+            {
+                st[r] = renum[size_t(st[r])];
+            }
+            nstages = k;
         }
         std::vector<size_t> order(n);
         std::iota(order.begin(), order.end(), size_t(0));
@@ -239,10 +275,94 @@ namespace smr
         // layer (or to the box of a corner) before the set algebra, which gives
         // the same sets as the whole-level expressions (the leaves and union
         // cells lie in the domain) at the cost of the boundary cells only.
+        // apply_extrapolation_bc_cells<2k> (bc/apply_field_bc.hpp:124-150) in
+        // direction d (Cartesian or diagonal: the B.C.'s line stencil turned
+        // towards d is {-(k-1)..k} * d, stencil.hpp:226-231, 498-561): for each
+        // cell b of `at` whose ghost b + k d exists, that ghost from the 2k - 1
+        // cells b + j d, j = -(k-1)..k-1.
+        void extrapolate_layer(const MRMeshData& m, int level, const LevelSet& at, const Dir& d, int k, Recipes& rec)
+        {
+            at.for_each_run(
+                [&](int32_t x0, int32_t x1, int32_t y, int32_t z)
+                {
+                    int64_t src[5];
+                    for (int32_t x = x0; x < x1; ++x)
+                    {
+                        if (!m.exists(level, x + k * d[0], y + k * d[1], z + k * d[2]))
+                        {
+                            continue;
+                        }
+                        for (int j = -(k - 1); j <= k - 1; ++j)
+                        {
+                            src[j + k - 1] = m.index(level, x + j * d[0], y + j * d[1], z + j * d[2]);
+                        }
+                        rec.push(m.index(level, x + k * d[0], y + k * d[1], z + k * d[2]), RECIPE_EXTRAPOLATE, src, 2 * k - 1);
+                    }
+                });
+        }
+
+        // update_outer_corners_by_polynomial_extrapolation beyond layer 1
+        // (bc/apply_field_bc.hpp:332-464): layers 2..w of the diagonal through
+        // each corner cell by extrapolation along the diagonal, then every
+        // off-diagonal ghost of layer k (first non-zero axis of d at offset k,
+        // the other ones at offsets 1..w) copies the diagonal ghost of layer k.
+        void wide_corner(const MRMeshData& m, int level, const LevelSet& cc, const Dir& d, int w, Recipes& rec)
+        {
+            for (int k = 2; k <= w; ++k)
+            {
+                extrapolate_layer(m, level, cc, d, k, rec);
+            }
+            std::vector<int> nz;
+            for (int i = 0; i < m.cfg.dim; ++i)
+            {
+                if (d[size_t(i)] != 0)
+                {
+                    nz.push_back(i);
+                }
+            }
+            int ncombos = 1;
+            for (size_t p = 1; p < nz.size(); ++p)
+            {
+                ncombos *= w;
+            }
+            for (int k = 1; k <= w; ++k)
+            {
+                for (int combo = 0; combo < ncombos; ++combo)
+                {
+                    Dir delta{0, 0, 0};
+                    bool is_source = true;
+                    int t          = combo;
+                    for (size_t p = 1; p < nz.size(); ++p)
+                    {
+                        const int g  = t % w;
+                        t           /= w;
+                        const int dp = g - (k - 1);
+                        delta[size_t(nz[p])] += dp * d[size_t(nz[p])];
+                        is_source = is_source && dp == 0;
+                    }
+                    if (is_source)
+                    {
+                        continue;
+                    }
+                    cc.for_each_run(
+                        [&](int32_t x0, int32_t x1, int32_t y, int32_t z)
+                        {
+                            for (int32_t x = x0; x < x1; ++x)
+                            {
+                                const int sx = x + k * d[0], sy = y + k * d[1], sz = z + k * d[2];
+                                int64_t s    = m.index(level, sx, sy, sz);
+                                rec.push(m.index(level, sx + delta[0], sy + delta[1], sz + delta[2]), RECIPE_COPY, &s, 1);
+                            }
+                        });
+                }
+            }
+        }
+
         Recipes outer_recipes(const MRMeshData& m, int level, const std::vector<Dir>& cart)
         {
             Recipes rec;
-            const int dim = m.cfg.dim;
+            const int dim   = m.cfg.dim;
+            const int width = std::max(m.cfg.radius, m.cfg.pred_s); // ghost width (mesh_config.hpp:395)
             const auto& cells = m.ids[CELLS];
             const auto& ref   = m.ids[REFERENCE];
             if (level <= m.cfg.max_level && level >= m.cfg.min_level)
@@ -263,6 +383,10 @@ namespace smr
                                 rec.push(m.index(level, x + d[0], y + d[1], z + d[2]), RECIPE_COPY, &s, 1);
                             }
                         });
+                    if (width > 1)
+                    {
+                        wide_corner(m, level, cc, d, width, rec);
+                    }
                     // project_corner_below (algorithm/update_outer_ghost.hpp:267-336)
                     if (level > 0)
                     {
@@ -374,6 +498,37 @@ namespace smr
                                 rec.push(m.index(level + 1, x, y, z), RECIPE_COPY, &s, 1);
                             }
                         });
+                }
+            }
+            // update_further_ghosts_by_polynomial_extrapolation (bc/apply_field_bc.hpp:
+            // 499-563, update_outer_ghost.hpp:414-426): the layers the B.C. leaves,
+            // extrapolated from the boundary cells.  Layer 1 is the B.C.'s when the
+            // field has one (Dirichlet<1>), else a copy (RECIPE_COPY_NO_BC, which
+            // the field's B.C. skips).  The part next to the union cells' ghosts
+            // has nothing to do at ghost width 1 or on a mesh of one level.
+            if (level >= m.cfg.min_level)
+            {
+                for (const Dir& dir : cart)
+                {
+                    const Bounds layer = domain_layer(level, dir);
+                    LevelSet bdry      = clip_box(cells[size_t(level)], layer.lo, layer.hi);
+                    if (bdry.empty())
+                    {
+                        continue;
+                    }
+                    bdry.for_each_run(
+                        [&](int32_t x0, int32_t x1, int32_t y, int32_t z)
+                        {
+                            for (int32_t x = x0; x < x1; ++x)
+                            {
+                                int64_t s = m.index(level, x, y, z);
+                                rec.push(m.index(level, x + dir[0], y + dir[1], z + dir[2]), RECIPE_COPY_NO_BC, &s, 1);
+                            }
+                        });
+                    for (int k = 2; k <= width; ++k)
+                    {
+                        extrapolate_layer(m, level, bdry, dir, k, rec);
+                    }
                 }
             }
             rec.stage();

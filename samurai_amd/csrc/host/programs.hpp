@@ -59,7 +59,9 @@ namespace smr
     {
         RECIPE_COPY      = 0, // dst = src0
         RECIPE_DIRICHLET = 1, // dst = 2*v - src0
-        RECIPE_AVERAGE   = 2  // dst = 0; dst += src_k; dst /= n (if n > 0)
+        RECIPE_AVERAGE     = 2, // dst = 0; dst += src_k; dst /= n (if n > 0)
+        RECIPE_EXTRAPOLATE = 3, // polynomial extrapolation from n = 3 or 5 sources (stencil 4 or 6)
+        RECIPE_COPY_NO_BC  = 4  // dst = src0 for a field without a B.C. (skipped when it has one)
     };
 
     struct Recipes
@@ -75,6 +77,8 @@ namespace smr
         // reorders the recipes into dependency-free stages, keeping the order of
         // every pair that shares a storage location
         void stage();
+        // stage k holds only RECIPE_COPY_NO_BC recipes (nothing for a field with a B.C.)
+        bool stage_without_bc_only(size_t k) const;
     };
 
     // update_ghost_mr (algorithm/update_ghost_mr.hpp:194-237) for one mesh

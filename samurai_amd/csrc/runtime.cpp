@@ -1262,9 +1262,12 @@ namespace smr
         // 194-237): per level from the finest, the outer-ghost recipe stages
         // then the projection into the level below; then the predictions from
         // the coarsest level up
-        SeqPlan& ghost_plan(Ctx& ctx, DevMesh& dm, const MeshConfig& cfg)
+        // (for a field with a B.C., the stages of copies meant for B.C.-less
+        // fields are left out)
+        SeqPlan& ghost_plan(Ctx& ctx, DevMesh& dm, const MeshConfig& cfg, bool has_bc)
         {
-            if (!dm.ghost_plan.built)
+            SeqPlan& plan = has_bc ? dm.ghost_plan : dm.ghost_plan_nobc;
+            if (!plan.built)
             {
                 std::vector<dev::SeqItem> items;
                 for (int level = cfg.max_level; level >= 0; --level)
@@ -1273,7 +1276,7 @@ namespace smr
                     const auto& sp      = r.host.stage_ptr;
                     for (size_t k = 0; k + 1 < sp.size(); ++k)
                     {
-                        if (sp[k + 1] > sp[k])
+                        if (sp[k + 1] > sp[k] && !(has_bc && r.host.stage_without_bc_only(k)))
                         {
                             dev::SeqItem it = seq_item(dev::SEQ_RECIPES, sp[k], sp[k + 1]);
                             it.dst          = r.dst;
@@ -1302,9 +1305,9 @@ namespace smr
                         items.push_back(seq_item(dev::SEQ_PREDICT, c0, c1));
                     }
                 }
-                dm.ghost_plan.build(std::move(items), ctx.stream);
+                plan.build(std::move(items), ctx.stream);
             }
-            return dm.ghost_plan;
+            return plan;
         }
 
         // maximum_op from the finest level down (mr/adapt.hpp:348-358)
@@ -1338,7 +1341,7 @@ namespace smr
         const int dim   = m.host.cfg.dim;
         const int ps    = m.host.cfg.pred_s;
         const double bc = f.bc_value;
-        const SeqPlan& plan = ghost_plan(*m.ctx, dm, m.host.cfg);
+        const SeqPlan& plan = ghost_plan(*m.ctx, dm, m.host.cfg, f.has_bc);
         const auto* d_items = static_cast<const dev::SeqItem*>(plan.d_items.p);
         for (const SeqPlan::Step& st : plan.steps)
         {
@@ -1509,6 +1512,34 @@ namespace smr
         }
         hip_check(hipMemsetAsync(out.d, 0, size_t(out.size) * 8, s), "hipMemsetAsync(convection)");
         dev::conv_burgers_gather(dm.burgers->row, dm.burgers->ptr, dm.burgers->src, dm.burgers->coef, dm.burgers->nrows, u.d, out.d, s);
+    }
+
+    void convection_weno5(const double* v, Field& u, Field& out)
+    {
+        u.check_current();
+        out.check_current();
+        if (u.mesh != out.mesh || &u == &out)
+        {
+            throw std::invalid_argument("convection: input and output must be two fields on one mesh");
+        }
+        Mesh& m       = *u.mesh;
+        const int dim = m.host.cfg.dim;
+        double vel[3] = {0, 0, 0};
+        for (int d = 0; d < dim; ++d)
+        {
+            vel[d] = v[d];
+        }
+        DevMesh& dm = m.device();
+        hipStream_t s = m.ctx->stream;
+        if (!dm.weno5)
+        {
+            dm.weno5 = upload_terms(build_weno5_terms(m.host), s);
+        }
+        update_ghost_mr(u);
+        ScopedTimer timer(m.ctx, "convection operator");
+        const DevMesh::Conv& c = *dm.weno5;
+        hip_check(hipMemsetAsync(out.d, 0, size_t(out.size) * 8, s), "hipMemsetAsync(convection)");
+        dev::conv_weno5_gather(c.row, c.ptr, c.src, c.coef, c.dir, c.nrows, vel, u.d, out.d, s);
     }
 
     void convection_upwind_heterogeneous(Field* const* velocity, Field& u, Field& out)

@@ -215,8 +215,11 @@ namespace smr
         std::unique_ptr<Conv> burgers;
         // the heterogeneous (velocity field) convection terms of this image
         std::unique_ptr<Conv> heterogeneous;
+        // the WENO5 convection terms of this image (one-level meshes of ghost width 3)
+        std::unique_ptr<Conv> weno5;
         // launch plans of update_ghost_mr and of the tag finalization
         SeqPlan ghost_plan, max_plan;
+        SeqPlan ghost_plan_nobc; // update_ghost_mr of a field without a B.C.
         // the leaves as per-level bitmaps (device mesh update), built on first use
         DevBuffer leaf_bits;
         bool has_leaf_bits = false;
@@ -470,6 +473,9 @@ namespace smr
     // out = make_convection_upwind<Field>(velocity_field)(u): velocity given as
     // dim scalar component fields on u's mesh; their ghosts and u's are updated
     void convection_upwind_heterogeneous(Field* const* velocity, Field& u, Field& out);
+    // out = make_convection_weno5(v)(u), after update_ghost_mr(u) (one-level
+    // mesh of stencil radius 3)
+    void convection_weno5(const double* v, Field& u, Field& out);
     // Adapt::operator() (mr/adapt.hpp:147-195); returns the number of harten calls
     int adapt(AdaptState& st, double eps, double regularity, int max_iterations = -1);
     // one harten call; true at the fixed point

@@ -56,6 +56,17 @@ def test_mesh_from_cells_matches_oracle(host_ctx):
     same_mesh(o, m)
 
 
+@pytest.mark.parametrize("dim,level,r", [(2, 4, 2), (2, 3, 3), (3, 3, 3)])
+def test_wide_ghost_mesh_matches_oracle(host_ctx, dim, level, r):
+    """A one-level mesh of stencil radius 2 or 3 (max_stencil_size 4 or 6):
+    ghost layers of that width, ids and numbering as the oracle's, programs
+    (the far-layer and corner-block recipes among them) in bounds."""
+    o = OracleSim(dim, level, level, pred_s=1, radius=r)
+    m = sa.make_mesh(host_ctx, sa.mesh_config(dim, level, level, 1, r))
+    same_mesh(o, m)
+    assert m.validate() == 0
+
+
 def test_invalid_configs_raise(host_ctx):
     with pytest.raises(sa.SamuraiError):
         sa.make_mesh(host_ctx, sa.mesh_config(2, 6, 4))          # max < min
