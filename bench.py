@@ -327,8 +327,37 @@ def bench_convection(sa, ctx, mesh, u, a, reps):
         res[name] = {"ms_per_call": (time.perf_counter() - t0) / reps * 1e3,
                      "terms": int(len(sa.convection_terms(mesh, lists_of)[3]))}
     res["leaves"] = mesh.nb_cells()
+    res["weno5"] = bench_weno5(sa, ctx, reps)
     progress("convection operators: " + ", ".join(f"{k} {v['ms_per_call']:.3f} ms" for k, v in res.items() if isinstance(v, dict)))
     return res
+
+
+def bench_weno5(sa, ctx, reps, level=10):
+    """make_convection_weno5 on a uniform 2D mesh of 4^level cells built with
+    max_stencil_size 6 (the WENO5 operator needs one level and three ghost
+    layers): time per call (ghost update + gather).  The gather reads per
+    term six 8-byte slots, a coefficient and a direction byte (57 B), and the
+    six stencil values; list_bytes is the list traffic of one call."""
+    from initial_conditions import centers
+    mesh = sa.make_mesh(ctx, sa.mesh_config(2, level, level, 1, 3))
+    c = centers(mesh.cells())
+    u = sa.make_scalar_field("u", mesh)
+    u.set_cells(np.sin(2 * np.pi * c[:, 0]) * np.cos(2 * np.pi * c[:, 1]))
+    sa.make_bc_dirichlet(u, 0.0)
+    f = sa.make_scalar_field("f", mesh)
+    op = sa.make_convection_weno5([1.0, -0.5])
+    op(u, out=f)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        op(u, out=f)
+    ctx.synchronize()
+    ms = (time.perf_counter() - t0) / reps * 1e3
+    n = mesh.nb_cells()
+    terms = 2 * 2 * n  # two interfaces per cell and direction, each seen from both sides (boundary: one side)
+    return {"workload": f"uniform 2D L{level} ({n} cells), stencil radius 3, v=(1,-0.5), Dirichlet 0",
+            "ms_per_call": ms, "cells": n, "terms": terms, "list_bytes": 57 * terms + 8 * 2 * n,
+            "cell_updates_per_s": n / (ms * 1e-3)}
 
 
 def bench_adjoint(sa, ctx, args, eps, reg):
