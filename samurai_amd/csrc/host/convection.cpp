@@ -420,94 +420,87 @@ namespace smr
         const double h     = 1.0 / double(n);
         const double fa    = h_factor(dim, h, h);
         const LevelSet& cl = cells[size_t(level)];
-        struct Term
+        // the interfaces in order: f(out, x, y, z, j0, coef, d) for the term of
+        // cell `out` whose stencil starts at (x, y, z) + j0 e_d
+        auto walk = [&](auto&& f)
         {
-            int64_t out, s[6];
-            double c;
-            int8_t dir;
-        };
-        std::vector<Term> ev;
-        for (int d = 0; d < dim; ++d)
-        {
-            std::array<int, 3> e{0, 0, 0};
-            e[size_t(d)] = 1;
-            const std::array<int, 3> me{-e[0], -e[1], -e[2]};
-            // the term of cell `out` for the interface whose stencil starts at (x, y, z) + j0 e
-            auto term = [&](int64_t out, int32_t x, int32_t y, int32_t z, int j0, double c)
+            for (int d = 0; d < dim; ++d)
             {
-                Term t{out, {}, c, int8_t(d)};
-                for (int j = 0; j < 6; ++j)
-                {
-                    const int k = j0 + j;
-                    t.s[j]      = m.index(level, x + k * e[0], y + k * e[1], z + k * e[2]);
-                }
-                ev.push_back(t);
-            };
-            set_intersection(cl, translate(cl, me))
-                .for_each_run(
-                    [&](int32_t x0, int32_t x1, int32_t y, int32_t z)
-                    {
-                        for (int32_t x = x0; x < x1; ++x)
+                std::array<int, 3> e{0, 0, 0};
+                e[size_t(d)] = 1;
+                const std::array<int, 3> me{-e[0], -e[1], -e[2]};
+                set_intersection(cl, translate(cl, me))
+                    .for_each_run(
+                        [&](int32_t x0, int32_t x1, int32_t y, int32_t z)
                         {
-                            term(m.index(level, x, y, z), x, y, z, -2, fa);
-                            term(m.index(level, x + e[0], y + e[1], z + e[2]), x, y, z, -2, -fa);
-                        }
-                    });
-            minus_box(cl, me, {n + me[0], n + me[1], n + me[2]})
-                .for_each_run(
-                    [&](int32_t x0, int32_t x1, int32_t y, int32_t z)
-                    {
-                        for (int32_t x = x0; x < x1; ++x)
+                            const int64_t bp = base_of(m, level, x0, y, z);
+                            const int64_t bq = base_of(m, level, x0 + e[0], y + e[1], z + e[2]) + e[0];
+                            for (int32_t x = x0; x < x1; ++x)
+                            {
+                                f(bp + x, x, y, z, -2, fa, d, e);
+                                f(bq + x, x, y, z, -2, -fa, d, e);
+                            }
+                        });
+                minus_box(cl, me, {n + me[0], n + me[1], n + me[2]})
+                    .for_each_run(
+                        [&](int32_t x0, int32_t x1, int32_t y, int32_t z)
                         {
-                            term(m.index(level, x, y, z), x, y, z, -2, fa);
-                        }
-                    });
-            minus_box(cl, e, {n + e[0], n + e[1], n + e[2]})
-                .for_each_run(
-                    [&](int32_t x0, int32_t x1, int32_t y, int32_t z)
-                    {
-                        for (int32_t x = x0; x < x1; ++x)
+                            const int64_t bp = base_of(m, level, x0, y, z);
+                            for (int32_t x = x0; x < x1; ++x)
+                            {
+                                f(bp + x, x, y, z, -2, fa, d, e);
+                            }
+                        });
+                minus_box(cl, e, {n + e[0], n + e[1], n + e[2]})
+                    .for_each_run(
+                        [&](int32_t x0, int32_t x1, int32_t y, int32_t z)
                         {
-                            term(m.index(level, x, y, z), x, y, z, -3, -fa);
-                        }
-                    });
-        }
-        // group by output slot, keeping the order of each slot's terms
+                            const int64_t bp = base_of(m, level, x0, y, z);
+                            for (int32_t x = x0; x < x1; ++x)
+                            {
+                                f(bp + x, x, y, z, -3, -fa, d, e);
+                            }
+                        });
+            }
+        };
+        // two passes (count, then fill each slot's terms in order): no list of
+        // events the size of the output
         const int64_t nref = m.nref;
-        std::vector<int64_t> cnt(size_t(nref) + 1, 0);
-        for (const Term& t : ev)
-        {
-            ++cnt[size_t(t.out) + 1];
-        }
-        ConvTerms out;
-        std::vector<int64_t> pos(size_t(nref), 0);
-        int64_t acc = 0;
+        std::vector<int64_t> pos(size_t(nref) + 1, 0);
+        int64_t nt = 0;
+        walk([&](int64_t out, int32_t, int32_t, int32_t, int, double, int, const std::array<int, 3>&) {
+            ++pos[size_t(out) + 1];
+            ++nt;
+        });
+        ConvTerms t;
+        t.ptr.clear();
         for (int64_t i = 0; i < nref; ++i)
         {
-            pos[size_t(i)] = acc;
-            if (cnt[size_t(i) + 1] > 0)
+            if (pos[size_t(i) + 1] > 0)
             {
-                out.row.push_back(i);
+                t.row.push_back(i);
             }
-            acc += cnt[size_t(i) + 1];
+            pos[size_t(i) + 1] += pos[size_t(i)];
         }
-        out.src.resize(6 * ev.size());
-        out.coef.resize(ev.size());
-        out.dir.resize(ev.size());
-        for (const Term& t : ev)
+        for (int64_t r : t.row)
         {
-            const int64_t k = pos[size_t(t.out)]++;
-            std::copy(t.s, t.s + 6, out.src.begin() + 6 * k);
-            out.coef[size_t(k)] = t.c;
-            out.dir[size_t(k)]  = t.dir;
+            t.ptr.push_back(pos[size_t(r)]);
         }
-        out.ptr.clear();
-        for (int64_t r : out.row)
-        {
-            out.ptr.push_back(pos[size_t(r)] - cnt[size_t(r) + 1]);
-        }
-        out.ptr.push_back(int64_t(ev.size()));
-        return out;
+        t.ptr.push_back(nt);
+        t.src.resize(size_t(6 * nt));
+        t.coef.resize(size_t(nt));
+        t.dir.resize(size_t(nt));
+        walk([&](int64_t out, int32_t x, int32_t y, int32_t z, int j0, double c, int d, const std::array<int, 3>& e) {
+            const int64_t k = pos[size_t(out)]++;
+            for (int j = 0; j < 6; ++j)
+            {
+                const int q            = j0 + j;
+                t.src[size_t(6 * k + j)] = m.index(level, x + q * e[0], y + q * e[1], z + q * e[2]);
+            }
+            t.coef[size_t(k)] = c;
+            t.dir[size_t(k)]  = int8_t(d);
+        });
+        return t;
     }
 
     ConvTerms build_burgers_terms(const MRMeshData& m)
