@@ -452,6 +452,9 @@ namespace smr::dev
     // slots src[6k..6k+6) (v: three host values)
     void conv_weno5_gather(const int64_t* row, const int64_t* ptr, const int64_t* src, const double* coef, const int8_t* dir, int64_t nrows,
                            const double* v, const double* u, double* out, hipStream_t s);
+    // the same on a one-level mesh stored as the dense box of side n + 6 (ghost
+    // width 3), without lists; fa: the h-factor of the level
+    void weno5_dense(int dim, int n, const double* v, double fa, const double* u, double* out, hipStream_t s);
     // launch shape of the upwind kernel (cells per thread, XCD remap, stores)
     int upwind_variant_count();
     void set_upwind_variant(int id);

@@ -280,6 +280,60 @@ namespace smr::dev
             out[row[r]] = acc;
         }
 
+        // the WENO5 flux of direction d from six values at base + j * stride
+        __device__ __forceinline__ double weno5_at(const double* __restrict__ base, int64_t stride, double vd)
+        {
+            double w[6];
+            for (int j = 0; j < 6; ++j)
+            {
+                w[j] = base[j * stride];
+            }
+            return vd >= 0 ? weno5_flux(w[0] * vd, w[1] * vd, w[2] * vd, w[3] * vd, w[4] * vd)
+                           : weno5_flux(w[5] * vd, w[4] * vd, w[3] * vd, w[2] * vd, w[1] * vd);
+        }
+
+        // make_convection_weno5 on a mesh of one level whose storage is the dense
+        // box of side W = n + 6 (ghost width 3, rows (z, y) in order, x fastest):
+        // one thread per cell, no lists.  Per direction the cell takes the
+        // flux of its left interface times -fa and of its right one times fa,
+        // in the order the interface walk gives them: left then right, except
+        // at the low boundary, whose boundary interface comes after the
+        // interior ones (the same sums as k_conv_weno5_gather, bit for bit).
+        template <int DIM>
+        __global__ __launch_bounds__(BLOCK) void k_weno5_dense(const double* __restrict__ u, double* __restrict__ out, int n, int64_t W,
+                                                               double v0, double v1, double v2, double fa)
+        {
+            const int64_t t     = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+            const int64_t total = DIM == 3 ? int64_t(n) * n * n : int64_t(n) * n;
+            if (t >= total)
+            {
+                return;
+            }
+            const int x = int(t % n), y = int((t / n) % n), z = DIM == 3 ? int(t / (int64_t(n) * n)) : 0;
+            const int g = 3;
+            const int64_t c = DIM == 3 ? ((int64_t(z + g) * W + (y + g)) * W + (x + g)) : (int64_t(y + g) * W + (x + g));
+            double acc = 0;
+            for (int d = 0; d < DIM; ++d)
+            {
+                const int64_t stride = d == 0 ? 1 : (d == 1 ? W : W * W);
+                const int coord      = d == 0 ? x : (d == 1 ? y : z);
+                const double vd      = d == 0 ? v0 : (d == 1 ? v1 : v2);
+                const double fl      = weno5_at(u + c - 3 * stride, stride, vd);
+                const double fr      = weno5_at(u + c - 2 * stride, stride, vd);
+                if (coord == 0)
+                {
+                    acc += fr * fa;
+                    acc += fl * -fa;
+                }
+                else
+                {
+                    acc += fl * -fa;
+                    acc += fr * fa;
+                }
+            }
+            out[c] = acc;
+        }
+
         __device__ __forceinline__ int64_t readlane64(int64_t v, int lane)
         {
             const uint64_t w  = uint64_t(v);
@@ -799,6 +853,20 @@ namespace smr::dev
         {
             hipLaunchKernelGGL(k_conv_weno5_gather, dim3(nblocks(nrows)), dim3(BLOCK), 0, s, row, ptr, src, coef, dir, nrows, v[0], v[1], v[2], u,
                                out);
+        }
+    }
+
+    void weno5_dense(int dim, int n, const double* v, double fa, const double* u, double* out, hipStream_t s)
+    {
+        const int64_t W     = int64_t(n) + 6;
+        const int64_t total = dim == 3 ? int64_t(n) * n * n : int64_t(n) * n;
+        if (dim == 3)
+        {
+            hipLaunchKernelGGL(k_weno5_dense<3>, dim3(nblocks(total)), dim3(BLOCK), 0, s, u, out, n, W, v[0], v[1], v[2], fa);
+        }
+        else
+        {
+            hipLaunchKernelGGL(k_weno5_dense<2>, dim3(nblocks(total)), dim3(BLOCK), 0, s, u, out, n, W, v[0], v[1], v[2], fa);
         }
     }
 

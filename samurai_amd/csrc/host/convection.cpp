@@ -402,6 +402,50 @@ namespace smr
         }
     } // namespace
 
+    int weno5_dense_level(const MRMeshData& m)
+    {
+        const int dim     = m.cfg.dim;
+        const auto& cells = m.ids[CELLS];
+        const int cmin = ca_min_level(cells), cmax = ca_max_level(cells);
+        if (cmin != cmax || std::max(m.cfg.radius, m.cfg.pred_s) != 3)
+        {
+            return -1;
+        }
+        const int level     = cmin;
+        const int64_t n     = int64_t(1) << level;
+        const int64_t W     = n + 6;
+        const int64_t nrows = dim == 3 ? W * W : W;
+        if (m.nref != (dim == 3 ? W * W * W : W * W) || cells[size_t(level)].count() != (dim == 3 ? n * n * n : n * n))
+        {
+            return -1;
+        }
+        for (int l = 0; l < MAXL; ++l)
+        {
+            if (l != level && !m.ids[REFERENCE][size_t(l)].empty())
+            {
+                return -1;
+            }
+        }
+        const LevelSet& ref = m.ids[REFERENCE][size_t(level)];
+        if (int64_t(ref.nrows()) != nrows || int64_t(ref.runs.size()) != nrows)
+        {
+            return -1;
+        }
+        for (size_t r = 0; r < ref.nrows(); ++r)
+        {
+            const int64_t y = int64_t(r) % W - 3, z = dim == 3 ? int64_t(r) / W - 3 : 0;
+            if (ref.ry[r] != y || ref.rz[r] != z || ref.runs[r].s != -3 || ref.runs[r].e != n + 3)
+            {
+                return -1;
+            }
+        }
+        if (m.index(level, -3, -3, dim == 3 ? -3 : 0) != 0)
+        {
+            return -1;
+        }
+        return level;
+    }
+
     ConvTerms build_weno5_terms(const MRMeshData& m)
     {
         const int dim     = m.cfg.dim;

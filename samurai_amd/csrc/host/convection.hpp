@@ -61,4 +61,9 @@ namespace smr
     // the inner cell at a boundary in the opposite direction); coef is the
     // h-factor, negative for the right cell.
     ConvTerms build_weno5_terms(const MRMeshData& m);
+
+    // the level of a mesh whose cells fill the domain at one level and whose
+    // storage is the dense box of that level widened by 3 ghost layers (every
+    // reference row one run [-3, n + 3), rows (z, y) in order), else -1
+    int weno5_dense_level(const MRMeshData& m);
 } // namespace smr

@@ -1529,17 +1529,29 @@ namespace smr
         {
             vel[d] = v[d];
         }
-        DevMesh& dm = m.device();
+        DevMesh& dm   = m.device();
         hipStream_t s = m.ctx->stream;
-        if (!dm.weno5)
+        // the dense kernel where the storage is the dense box (every one-level
+        // mesh built here), the ordered lists otherwise or when SMR_WENO5_LISTS=1
+        const int dense = std::getenv("SMR_WENO5_LISTS") ? -1 : weno5_dense_level(m.host);
+        if (dense < 0 && !dm.weno5)
         {
             dm.weno5 = upload_terms(build_weno5_terms(m.host), s);
         }
         update_ghost_mr(u);
         ScopedTimer timer(m.ctx, "convection operator");
-        const DevMesh::Conv& c = *dm.weno5;
         hip_check(hipMemsetAsync(out.d, 0, size_t(out.size) * 8, s), "hipMemsetAsync(convection)");
-        dev::conv_weno5_gather(c.row, c.ptr, c.src, c.coef, c.dir, c.nrows, vel, u.d, out.d, s);
+        if (dense >= 0)
+        {
+            const double h  = 1.0 / double(1 << dense);
+            const double fa = std::pow(h, dim - 1) / std::pow(h, dim); // h_factor (flux_based_scheme__lin_hom.hpp:62-67)
+            dev::weno5_dense(dim, 1 << dense, vel, fa, u.d, out.d, s);
+        }
+        else
+        {
+            const DevMesh::Conv& c = *dm.weno5;
+            dev::conv_weno5_gather(c.row, c.ptr, c.src, c.coef, c.dir, c.nrows, vel, u.d, out.d, s);
+        }
     }
 
     void convection_upwind_heterogeneous(Field* const* velocity, Field& u, Field& out)

@@ -88,8 +88,14 @@ VELOCITIES = [(1.5, 0.0, 0.0), (-0.75, 1.25, 0.5), (0.5, -2.0, -1.0), (0.0, 0.0,
 
 @pytest.mark.parametrize("dim,level", [(2, 6), (3, 4)])
 @pytest.mark.parametrize("v", VELOCITIES)
-def test_weno5_matches_oracle(gpu_ctx, dim, level, v):
-    say("weno5", dim, level, v)
+@pytest.mark.parametrize("path", ["dense", "lists"])
+def test_weno5_matches_oracle(gpu_ctx, dim, level, v, path, monkeypatch):
+    """Both kernels: the list-free one over the dense storage box (the
+    default on one-level meshes) and the ordered-list gather
+    (SMR_WENO5_LISTS=1)."""
+    if path == "lists":
+        monkeypatch.setenv("SMR_WENO5_LISTS", "1")
+    say("weno5", dim, level, v, path)
     o, mesh, u = wide_pair(gpu_ctx, dim, level, 3)
     rows = mesh.cells()
     c = centers(rows)
