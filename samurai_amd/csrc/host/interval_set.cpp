@@ -41,7 +41,25 @@ namespace smr
 
     int64_t LevelSet::find_row(int32_t y, int32_t z) const
     {
-        int64_t lo = 0, hi = int64_t(ry.size());
+        // lookups come in row order more often than not (a boundary walk, the
+        // sources of consecutive recipes): try the calling thread's last row of
+        // this set and the one after it first.  A candidate is accepted only if
+        // it holds (y, z), the one row that does, so a stale memo never misleads.
+        thread_local const LevelSet* memo_set = nullptr;
+        thread_local int64_t memo_row         = -1;
+        const int64_t nr                      = int64_t(ry.size());
+        if (memo_set == this)
+        {
+            for (int64_t c = memo_row; c <= memo_row + 1; ++c)
+            {
+                if (c >= 0 && c < nr && ry[size_t(c)] == y && rz[size_t(c)] == z)
+                {
+                    memo_row = c;
+                    return c;
+                }
+            }
+        }
+        int64_t lo = 0, hi = nr;
         while (lo < hi)
         {
             int64_t mid = (lo + hi) / 2;
@@ -54,8 +72,10 @@ namespace smr
                 hi = mid;
             }
         }
-        if (lo < int64_t(ry.size()) && ry[size_t(lo)] == y && rz[size_t(lo)] == z)
+        if (lo < nr && ry[size_t(lo)] == y && rz[size_t(lo)] == z)
         {
+            memo_set = this;
+            memo_row = lo;
             return lo;
         }
         return -1;

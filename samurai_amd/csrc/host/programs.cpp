@@ -110,6 +110,8 @@ namespace smr
         const size_t n = dst.size();
         std::vector<int> st(n, 0);
         std::unordered_map<int64_t, int> last_write, last_read;
+        last_write.reserve(n);
+        last_read.reserve(src.size());
         int nstages = 0;
         for (size_t r = 0; r < n; ++r)
         {
@@ -235,38 +237,68 @@ namespace smr
             return b;
         }
 
-        // the smallest box holding a (lo = hi when a is empty)
-        Bounds set_bounds(const LevelSet& a)
+        // clip_box(s, domain_layer(level, dir)): the cells of s on the domain face
+        // in `dir`; for an x face one run lookup per row instead of a clip of
+        // every row
+        LevelSet face_layer(const LevelSet& s, int level, const Dir& dir)
         {
-            Bounds b{{0, 0, 0}, {0, 0, 0}};
-            if (a.empty())
+            const Bounds layer = domain_layer(level, dir);
+            if (dir[0] == 0)
             {
-                return b;
+                return clip_box(s, layer.lo, layer.hi);
             }
-            b.lo = {a.runs[0].s, a.ry[0], a.rz[0]};
-            b.hi = {a.runs[0].e, a.ry[0] + 1, a.rz[0] + 1};
-            for (size_t r = 0; r < a.nrows(); ++r)
+            const int32_t x = layer.lo[0];
+            LevelSet out(s.dim, s.level);
+            for (size_t r = 0; r < s.nrows(); ++r)
             {
-                b.lo[0] = std::min(b.lo[0], int(a.runs[size_t(a.rp[r])].s));
-                b.hi[0] = std::max(b.hi[0], int(a.runs[size_t(a.rp[r + 1] - 1)].e));
-                b.lo[1] = std::min(b.lo[1], int(a.ry[r]));
-                b.hi[1] = std::max(b.hi[1], int(a.ry[r]) + 1);
-                b.lo[2] = std::min(b.lo[2], int(a.rz[r]));
-                b.hi[2] = std::max(b.hi[2], int(a.rz[r]) + 1);
+                const int32_t y = s.ry[r], z = s.rz[r];
+                if (y < layer.lo[1] || y >= layer.hi[1] || (s.dim == 3 && (z < layer.lo[2] || z >= layer.hi[2])))
+                {
+                    continue;
+                }
+                if (s.find_run(int64_t(r), x) >= 0)
+                {
+                    out.push(y, z, x, x + 1);
+                }
             }
-            return b;
+            return out;
         }
 
-        // a n b for a set a much smaller than b: b is clipped to the box of a
-        // first (the same set, without a pass over all of b)
+        // a n b for a set a much smaller than b: each row of a is looked up in b
+        // and its runs intersected with the runs of that row from the first one
+        // ending after them (bisection), so b is never walked as a whole
         LevelSet small_intersection(const LevelSet& a, const LevelSet& b)
         {
             if (a.level != b.level)
             {
                 throw std::logic_error("small_intersection: sets of different levels");
             }
-            const Bounds box = set_bounds(a);
-            return set_intersection(a, clip_box(b, box.lo, box.hi));
+            LevelSet out(a.dim, a.level);
+            if (a.empty() || b.empty())
+            {
+                return out;
+            }
+            out.reserve(a.nrows(), a.runs.size());
+            for (size_t r = 0; r < a.nrows(); ++r)
+            {
+                const int64_t rb = b.find_row(a.ry[r], a.rz[r]);
+                if (rb < 0)
+                {
+                    continue;
+                }
+                const Run* j        = b.runs.data() + b.rp[size_t(rb)];
+                const Run* const je = b.runs.data() + b.rp[size_t(rb) + 1];
+                for (int64_t i = a.rp[r]; i < a.rp[r + 1]; ++i)
+                {
+                    const Run x = a.runs[size_t(i)];
+                    j           = std::lower_bound(j, je, x.s, [](const Run& q, int32_t v) { return q.e <= v; });
+                    for (const Run* k = j; k < je && k->s < x.e; ++k)
+                    {
+                        out.push(a.ry[r], a.rz[r], std::max(x.s, k->s), std::min(x.e, k->e));
+                    }
+                }
+            }
+            return out;
         }
 
         // update_outer_ghosts(level) (algorithm/update_outer_ghost.hpp:338-432).
@@ -391,7 +423,7 @@ namespace smr
                     if (level > 0)
                     {
                         LevelSet fine_outer = small_intersection(translate(corner, d), ref[size_t(level)]);
-                        for (int dl = 1; dl <= 2; ++dl)
+                        for (int dl = 1; dl <= 2 && !fine_outer.empty(); ++dl)
                         {
                             const int pl   = level - dl;
                             LevelSet ghost = small_intersection(on(fine_outer, pl), ref[size_t(pl)]);
@@ -420,14 +452,23 @@ namespace smr
                     }
                 }
             }
-            for (const Dir& dir : cart)
+            // the boundary cells of the level on each face (B.C., B.C. prediction, far layers)
+            std::vector<LevelSet> faces(cart.size());
+            if (level >= m.cfg.min_level)
             {
+                for (size_t di = 0; di < cart.size(); ++di)
+                {
+                    faces[di] = face_layer(cells[size_t(level)], level, cart[di]);
+                }
+            }
+            for (size_t di = 0; di < cart.size(); ++di)
+            {
+                const Dir& dir = cart[di];
                 if (level < m.cfg.max_level)
                 {
                     // project_bc, layer 1 (algorithm/update_outer_ghost.hpp:20-152)
                     const Bounds dom  = domain_bounds(level);
-                    const Bounds face = domain_layer(level, dir);
-                    LevelSet outside  = minus_box(translate(clip_box(m.ids[UNION_CELLS][size_t(level)], face.lo, face.hi), dir), dom.lo, dom.hi);
+                    LevelSet outside  = minus_box(translate(face_layer(m.ids[UNION_CELLS][size_t(level)], level, dir), dir), dom.lo, dom.hi);
                     LevelSet ghosts   = small_intersection(outside, ref[size_t(level)]);
                     std::vector<int64_t> ch;
                     ghosts.for_each_run(
@@ -467,9 +508,7 @@ namespace smr
                 if (level >= m.cfg.min_level)
                 {
                     // Dirichlet<1> on the boundary cells (bc/apply_field_bc.hpp:53-101)
-                    const Bounds layer = domain_layer(level, dir);
-                    LevelSet bdry      = clip_box(cells[size_t(level)], layer.lo, layer.hi);
-                    bdry.for_each_run(
+                    faces[di].for_each_run(
                         [&](int32_t x0, int32_t x1, int32_t y, int32_t z)
                         {
                             for (int32_t x = x0; x < x1; ++x)
@@ -485,9 +524,7 @@ namespace smr
                     // the leaves outside the domain shifted by -dir are those of
                     // the boundary layer in direction dir
                     const Bounds dom   = domain_bounds(level);
-                    const Bounds face  = domain_layer(level, dir);
-                    LevelSet inner     = clip_box(cells[size_t(level)], face.lo, face.hi);
-                    LevelSet bc_ghosts = minus_box(translate(inner, dir), dom.lo, dom.hi);
+                    LevelSet bc_ghosts = minus_box(translate(faces[di], dir), dom.lo, dom.hi);
                     LevelSet ghosts    = small_intersection(on(bc_ghosts, level + 1), ref[size_t(level + 1)]);
                     ghosts.for_each_run(
                         [&](int32_t x0, int32_t x1, int32_t y, int32_t z)
@@ -508,10 +545,10 @@ namespace smr
             // has nothing to do at ghost width 1 or on a mesh of one level.
             if (level >= m.cfg.min_level)
             {
-                for (const Dir& dir : cart)
+                for (size_t di = 0; di < cart.size(); ++di)
                 {
-                    const Bounds layer = domain_layer(level, dir);
-                    LevelSet bdry      = clip_box(cells[size_t(level)], layer.lo, layer.hi);
+                    const Dir& dir        = cart[di];
+                    const LevelSet& bdry  = faces[di];
                     if (bdry.empty())
                     {
                         continue;
