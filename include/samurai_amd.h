@@ -194,6 +194,13 @@ int smr_adapt_destroy(smr_adapt* adapt);
 /* MRadaptation(mra_config().epsilon(eps).regularity(reg)) (mr/adapt.hpp:147-195):
  * returns the number of harten iterations run (>= 0) */
 int smr_adapt_run(smr_adapt* adapt, double eps, double regularity);
+/* mra_config().relative_detail(on) (mr/config.hpp:36-45, applied at
+ * mr/adapt.hpp:319-322 by compute_relative_detail, mr/rel_detail.hpp:74-112):
+ * every detail divided by max |field| over the leaves (1 below the machine
+ * epsilon) before the criteria, for the following smr_adapt_run,
+ * smr_adapt_harten and smr_adapt_tags calls; off by default.
+ * smr_adapt_tags_noalloc rejects it (SMR_ERR_ARGUMENT). */
+int smr_adapt_set_relative_detail(smr_adapt* adapt, int on);
 /* one Adapt::harten(ite) call (mr/adapt.hpp:276-389): 1 at the fixed point, 0 if
  * the mesh changed */
 int smr_adapt_harten(smr_adapt* adapt, int ite, double eps, double regularity);

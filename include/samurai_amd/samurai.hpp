@@ -700,9 +700,19 @@ namespace samurai_amd
         {
             return reg_;
         }
+        mra_config& relative_detail(bool r)
+        {
+            rel_ = r;
+            return *this;
+        }
+        bool relative_detail() const
+        {
+            return rel_;
+        }
 
       private:
         double eps_ = 1e-4, reg_ = 1.0; // mr/config.hpp defaults
+        bool rel_   = false;
     };
 
     // Adapt (mr/adapt.hpp:96-146) over one or more fields on one mesh
@@ -725,9 +735,17 @@ namespace samurai_amd
         }
         void operator()(const mra_config& cfg)
         {
-            (*this)(cfg.epsilon(), cfg.regularity());
+            check(smr_adapt_set_relative_detail(h_.get(), cfg.relative_detail() ? 1 : 0));
+            run(cfg.epsilon(), cfg.regularity());
         }
         void operator()(double eps, double regularity)
+        {
+            check(smr_adapt_set_relative_detail(h_.get(), 0));
+            run(eps, regularity);
+        }
+
+      private:
+        void run(double eps, double regularity)
         {
             for (ScalarField<dim>* f : fields_)
             {
@@ -740,7 +758,6 @@ namespace samurai_amd
             }
         }
 
-      private:
         std::vector<ScalarField<dim>*> fields_;
         // the fields outlive the adaptation object
         std::vector<std::shared_ptr<smr_field>> field_h_;

@@ -366,6 +366,8 @@ namespace oracle
         // mr/adapt.hpp:138-146): their storage and their details
         std::vector<std::vector<double>> extra, extra_detail;
         bool has_bc     = false;
+        // mra_config().relative_detail(true) (mr/config.hpp:36-45)
+        bool relative_detail = false;
         double bc_value = 0.0;
         bool poison     = true; // NaN-fill storage nobody wrote (SAMURAI_CHECK_NAN idea)
         int last_harten_iterations = 0;
@@ -1270,6 +1272,35 @@ namespace oracle
             {
                 std::swap(s.u, s.extra[k - 1]);
                 std::swap(s.detail, s.extra_detail[k - 1]);
+            }
+        }
+        // compute_relative_detail (mr/rel_detail.hpp:74-112, mr/adapt.hpp:319-322):
+        // the max of |field| over the leaves, folded from the smallest normal
+        // double, 1 where it is below the machine epsilon, and every entry of
+        // the field's detail array times its inverse
+        if (s.relative_detail)
+        {
+            for (size_t k = 0; k <= s.extra.size(); ++k)
+            {
+                const std::vector<double>& f = k == 0 ? s.u : s.extra[k - 1];
+                double mx                    = std::numeric_limits<double>::min();
+                for (int l = 0; l < MAXL; ++l)
+                {
+                    m.ids[CELLS][size_t(l)].for_each(
+                        [&](int x, int y, int z)
+                        {
+                            mx = std::max(mx, std::abs(f[size_t(m.index(l, x, y, z))]));
+                        });
+                }
+                if (mx < std::numeric_limits<double>::epsilon())
+                {
+                    mx = 1.0;
+                }
+                const double inv = kPlant == 15 ? mx : 1. / mx;
+                for (double& v : (k == 0 ? s.detail : s.extra_detail[k - 1]))
+                {
+                    v *= inv;
+                }
             }
         }
         for (int level = min_level; level <= max_level - ite; ++level)
@@ -2369,6 +2400,13 @@ extern "C"
             std::vector<double> r = convection_weno5(*s, v);
             std::copy(r.begin(), r.end(), out);
         });
+        return 0;
+    }
+
+    // mra_config().relative_detail(on) for the following adaptations
+    int orc_set_relative_detail(void* p, int on)
+    {
+        static_cast<Sim*>(p)->relative_detail = on != 0;
         return 0;
     }
 

@@ -103,6 +103,7 @@ def lib():
         _sig(L, "smr_adapt_create_fields", "argtypes", [ctypes.POINTER(vp), i32, ctypes.POINTER(vp)])
         _sig(L, "smr_adapt_nb_fields", "argtypes", [vp])
         _sig(L, "smr_adapt_run", "argtypes", [vp, dbl, dbl])
+        _sig(L, "smr_adapt_set_relative_detail", "argtypes", [vp, i32])
         _sig(L, "smr_adapt_harten", "argtypes", [vp, i32, dbl, dbl])
         _sig(L, "smr_adapt_tags", "argtypes", [vp, i32, dbl, dbl, vp])
         _sig(L, "smr_adapt_tags_noalloc", "argtypes", [vp, i32, dbl, dbl, vp])
@@ -521,8 +522,10 @@ class Adapt(_Handle):
         self._own(fields[0].mesh.ctx, h)
         self.field, self.fields = fields[0], list(fields)
 
-    def __call__(self, epsilon=1e-4, regularity=1.0):
-        """MRadaptation(mra_config().epsilon(eps).regularity(reg)); returns #harten calls."""
+    def __call__(self, epsilon=1e-4, regularity=1.0, relative_detail=False):
+        """MRadaptation(mra_config().epsilon(eps).regularity(reg).relative_detail(rel));
+        returns #harten calls."""
+        _check(lib().smr_adapt_set_relative_detail(self.h, int(bool(relative_detail))))
         return _check(lib().smr_adapt_run(self.h, float(epsilon), float(regularity)))
 
     def harten(self, ite, epsilon, regularity):

@@ -858,6 +858,17 @@ extern "C"
             });
     }
 
+    int smr_adapt_set_relative_detail(smr_adapt* adapt, int on)
+    {
+        return guarded(
+            [&]
+            {
+                require(adapt != nullptr, "null adapt");
+                adapt->a.relative_detail = on != 0;
+                return 0;
+            });
+    }
+
     int smr_adapt_harten(smr_adapt* adapt, int ite, double eps, double regularity)
     {
         return guarded(

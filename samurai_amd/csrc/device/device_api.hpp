@@ -452,6 +452,10 @@ namespace smr::dev
     // slots src[6k..6k+6) (v: three host values)
     void conv_weno5_gather(const int64_t* row, const int64_t* ptr, const int64_t* src, const double* coef, const int8_t* dir, int64_t nrows,
                            const double* v, const double* u, double* out, hipStream_t s);
+    // mra_config().relative_detail(true) (mr/rel_detail.hpp:74-112): det[0, n)
+    // times 1 / max |u| over cells [c0, c1) of the list (mx: one device word)
+    void relative_detail(const List& cells, int64_t c0, int64_t c1, const double* u, double* det, int64_t n, unsigned long long* mx,
+                         hipStream_t s);
     // the same on a one-level mesh stored as the dense box of side n + 6 (ghost
     // width 3), without lists; fa: the h-factor of the level
     void weno5_dense(int dim, int n, const double* v, double fa, const double* u, double* out, hipStream_t s);

@@ -5,6 +5,8 @@
 // The file is compiled with -ffp-contract=off.
 #include "cell_ops.hpp"
 
+#include <algorithm>
+#include <cfloat>
 #include <cstdlib>
 
 namespace smr::dev
@@ -713,6 +715,60 @@ namespace smr::dev
         SMR_CELL_KERNEL(k_predict_children, (op_predict_children<DIM, S>(l, c, dst, src)), double* dst, const double* src)
 #undef SMR_CELL_KERNEL
 
+        // max |u| over cells [c0, c1) of a list into *mx (atomic max of the bit
+        // patterns, which order as the non-negative doubles do; a NaN value is
+        // passed over like std::max(m, NaN) does)
+        __global__ __launch_bounds__(BLOCK) void k_abs_max(const List l, int64_t c0, int64_t c1, const double* __restrict__ u,
+                                                           unsigned long long* mx)
+        {
+            __shared__ double part[BLOCK / 64];
+            double m = 0;
+            for (int64_t c = c0 + int64_t(blockIdx.x) * blockDim.x + threadIdx.x; c < c1; c += int64_t(gridDim.x) * blockDim.x)
+            {
+                const Cell cell = locate(l, c);
+                const double a  = fabs(u[bases(l, cell.run)[0] + cell.x]);
+                m               = a > m ? a : m;
+            }
+            for (int off = 32; off > 0; off >>= 1)
+            {
+                const double o = __shfl_xor(m, off, 64);
+                m              = o > m ? o : m;
+            }
+            if ((threadIdx.x & 63) == 0)
+            {
+                part[threadIdx.x >> 6] = m;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0)
+            {
+                for (int w = 1; w < BLOCK / 64; ++w)
+                {
+                    m = part[w] > m ? part[w] : m;
+                }
+                atomicMax(mx, static_cast<unsigned long long>(__double_as_longlong(m)));
+            }
+        }
+
+        // compute_relative_detail's scaling (mr/rel_detail.hpp:89-110): the max
+        // folded from the smallest normal double, 1 below the machine epsilon,
+        // every detail times its inverse
+        __global__ __launch_bounds__(BLOCK) void k_scale_relative(double* __restrict__ d, int64_t n, const unsigned long long* mx)
+        {
+            const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+            if (i >= n)
+            {
+                return;
+            }
+            double m = __longlong_as_double(static_cast<long long>(*mx));
+            m        = m > DBL_MIN ? m : DBL_MIN;
+            if (m < DBL_EPSILON)
+            {
+                m = 1.0;
+            }
+            const double inv = 1. / m;
+            d[i] *= inv;
+        }
+
         __global__ __launch_bounds__(BLOCK) void k_recipes(const int64_t* __restrict__ dst, const int32_t* __restrict__ op,
                                                            const int64_t* __restrict__ sp, const int64_t* __restrict__ src, int64_t r0,
                                                            int64_t r1, double* u, int has_bc, double bc_value)
@@ -853,6 +909,21 @@ namespace smr::dev
         {
             hipLaunchKernelGGL(k_conv_weno5_gather, dim3(nblocks(nrows)), dim3(BLOCK), 0, s, row, ptr, src, coef, dir, nrows, v[0], v[1], v[2], u,
                                out);
+        }
+    }
+
+    void relative_detail(const List& cells, int64_t c0, int64_t c1, const double* u, double* det, int64_t n, unsigned long long* mx,
+                         hipStream_t s)
+    {
+        zero_f64(reinterpret_cast<double*>(mx), 1, s);
+        if (c1 > c0)
+        {
+            const int64_t nb = std::min<int64_t>(nblocks(c1 - c0), 1024);
+            hipLaunchKernelGGL(k_abs_max, dim3(nb), dim3(BLOCK), 0, s, cells, c0, c1, u, mx);
+        }
+        if (n > 0)
+        {
+            hipLaunchKernelGGL(k_scale_relative, dim3(nblocks(n)), dim3(BLOCK), 0, s, det, n, mx);
         }
     }
 

@@ -68,6 +68,10 @@ namespace smr
             throw std::invalid_argument("adapt_tags_noalloc: " + std::to_string(m.host.nref) + " storage slots x " + std::to_string(st.ncomp())
                                         + " fields exceed the scratch of " + std::to_string(kNoAllocSlots));
         }
+        if (st.relative_detail)
+        {
+            throw std::invalid_argument("adapt_tags_noalloc: the relative detail needs a device word per field; use smr_adapt_tags");
+        }
         if (!m.dev || !m.dev->has_ghost || !m.dev->has_adapt)
         {
             throw std::invalid_argument("adapt_tags_noalloc: the mesh's device programs are not built; run the allocating "

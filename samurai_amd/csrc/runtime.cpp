@@ -1180,6 +1180,10 @@ namespace smr
 
     AdaptState::~AdaptState()
     {
+        if (rel_max)
+        {
+            (void)hipFree(rel_max);
+        }
         if (detail)
         {
             (void)hipFree(detail);
@@ -1604,6 +1608,26 @@ namespace smr
         }
 
         // detail arrays of all adapted fields (ncomp x n) and the tags (n)
+        // the relative detail's device words (null when the detail is absolute)
+        unsigned long long* relative_words(AdaptState& st)
+        {
+            if (!st.relative_detail)
+            {
+                return nullptr;
+            }
+            if (st.rel_capacity < st.ncomp())
+            {
+                if (st.rel_max)
+                {
+                    hip_check(hipFree(st.rel_max), "hipFree(relative detail)");
+                    st.rel_max = nullptr;
+                }
+                hip_check(hipMalloc(reinterpret_cast<void**>(&st.rel_max), size_t(st.ncomp()) * 8), "hipMalloc(relative detail)");
+                st.rel_capacity = st.ncomp();
+            }
+            return st.rel_max;
+        }
+
         void ensure_adapt_buffers(AdaptState& st, int64_t n)
         {
             n *= st.ncomp();
@@ -2130,7 +2154,8 @@ namespace smr
         harten_tags(fs, 1, ite, eps, regularity, tag, det);
     }
 
-    void harten_tags(Field* const* fs, int nf, int ite, double eps, double regularity, uint8_t* tag, double* det)
+    void harten_tags(Field* const* fs, int nf, int ite, double eps, double regularity, uint8_t* tag, double* det,
+                     unsigned long long* rel_max)
     {
         Field& f = *fs[0];
         for (int k = 0; k < nf; ++k)
@@ -2169,6 +2194,14 @@ namespace smr
             {
                 dev::detail(dim, cfg.pred_s, dm.detail.d, c0, c1, fs[size_t(k)]->d, det + k * n, s);
             }
+            // compute_relative_detail (mr/adapt.hpp:319-322)
+            if (rel_max)
+            {
+                for (int k = 0; k < ncomp; ++k)
+                {
+                    dev::relative_detail(dm.cells.d, 0, dm.cells.d.ncells, fs[size_t(k)]->d, det + k * n, n, rel_max + k, s);
+                }
+            }
         }
         // tag cells: fine levels [min, max-ite], i.e. coarse levels [min-1, max-ite-1]
         {
@@ -2204,7 +2237,7 @@ namespace smr
         f.check_current();
         ensure_adapt_buffers(st, f.mesh->host.nref);
         const std::vector<Field*> fs = adapted_fields(st);
-        harten_tags(fs.data(), int(fs.size()), ite, eps, regularity, tag, st.detail);
+        harten_tags(fs.data(), int(fs.size()), ite, eps, regularity, tag, st.detail, relative_words(st));
         st.last_size      = 0; // the tags are the caller's, not the adaptation's
         st.host_tag_valid = false;
     }
@@ -2221,7 +2254,7 @@ namespace smr
         const int64_t n = m.host.nref;
         ensure_adapt_buffers(st, n);
         const std::vector<Field*> fs = adapted_fields(st);
-        harten_tags(fs.data(), int(fs.size()), ite, eps, regularity, st.tag, st.detail);
+        harten_tags(fs.data(), int(fs.size()), ite, eps, regularity, st.tag, st.detail, relative_words(st));
         DevMesh& dm = *m.dev;
         std::unique_ptr<DevMesh> ndev;
         MRMeshData nm;

@@ -314,6 +314,11 @@ namespace smr
         int64_t host_tag_capacity = 0;
         int64_t last_size         = 0;
         bool host_tag_valid       = false; // else the tags are only in `tag`
+        // mra_config().relative_detail(true) (mr/config.hpp:36-45): the
+        // details divided by max |field| over the leaves before the criteria
+        bool relative_detail = false;
+        unsigned long long* rel_max = nullptr; // one device word per field
+        int rel_capacity            = 0;
         ~AdaptState();
         void copy_last(uint8_t* tags, double* details, hipStream_t s) const;
     };
@@ -486,7 +491,10 @@ namespace smr
     void harten_tags(Field& f, int ite, double eps, double regularity, uint8_t* tag, double* det);
     // harten_tags over nf fields on one mesh (fs[0] first): ghosts of each,
     // details of field k into det + k * nref, criteria over all of them
-    void harten_tags(Field* const* fs, int nf, int ite, double eps, double regularity, uint8_t* tag, double* det);
+    // (rel_max: one device word per field for the relative detail, null for
+    // the absolute one)
+    void harten_tags(Field* const* fs, int nf, int ite, double eps, double regularity, uint8_t* tag, double* det,
+                     unsigned long long* rel_max = nullptr);
     // harten_tags into a caller's device tag array, details in st's buffer
     void adapt_tags(AdaptState& st, int ite, double eps, double regularity, uint8_t* tag);
 

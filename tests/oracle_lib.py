@@ -49,6 +49,7 @@ def lib():
         L.orc_convection_upwind.argtypes = [vp, P(dbl), P(dbl)]
         L.orc_convection_upwind_nonlinear.argtypes = [vp, P(dbl)]
         L.orc_convection_weno5.argtypes = [vp, P(dbl), P(dbl)]
+        L.orc_set_relative_detail.argtypes = [vp, i32]
         L.orc_weno5_flux.restype = dbl
         L.orc_weno5_flux.argtypes = [P(dbl)]
         L.orc_upwind_scalar_burgers_step.argtypes = [vp, P(dbl), dbl]
@@ -216,6 +217,10 @@ class OracleSim:
         out = np.zeros(self.nb_cells(REFERENCE))
         _check(lib().orc_convection_upwind(self.h, _dp(v), _dp(out)))
         return out
+
+    def set_relative_detail(self, on=True):
+        """mra_config().relative_detail(on) for the following adaptations."""
+        _check(lib().orc_set_relative_detail(self.h, int(bool(on))))
 
     def convection_weno5(self, v):
         """make_convection_weno5(v)(u) after update_ghost_mr(u) (one-level

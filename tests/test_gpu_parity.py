@@ -398,3 +398,49 @@ def test_several_fields_adapt_like_the_oracle(gpu_ctx, dim, lmin, lmax):
         sa.swap(u, unp1)
         o.upwind_step(a, dt)
         assert np.array_equal(bits(o.values()), bits(u.cells())), step
+
+
+@pytest.mark.parametrize("dim,lmin,lmax,nf", [(2, 2, 7, 1), (2, 3, 8, 2), (3, 2, 5, 1)])
+def test_relative_detail_adapts_like_the_oracle(gpu_ctx, dim, lmin, lmax, nf):
+    """MRadaptation(mra_config().relative_detail(true)) (mr/adapt.hpp:319-322,
+    mr/rel_detail.hpp:74-112): each field's details divided by its max |value|
+    over the leaves; scaled fields (maxima 3.7 and 0.6) so that the inverse is
+    not a power of two.  Cells and values bit for bit against the oracle over
+    three adapted steps; the allocation-free tag entry point refuses it."""
+    say("test_relative_detail_adapts_like_the_oracle", dim, lmin, lmax, nf)
+    fu = (lambda c: 3.7 * disc_2d(c)) if dim == 2 else (lambda c: 3.7 * sphere_3d(c))
+    fv = lambda c: 0.6 * seeded_2d(c, 4)
+    eps = 1e-4 if dim == 2 else 1e-3
+    a = np.array([1.0, 0.5, 0.25][:dim])
+    dt = 0.4 / 3 * 2.0 ** -lmax
+    o = OracleSim(dim, lmin, lmax, pred_s=1)
+    o.set_poison(False)
+    o.set_values(fu(o.cells()))
+    o.set_dirichlet(0.0)
+    o.set_relative_detail(True)
+    mesh = sa.make_mesh(gpu_ctx, sa.mesh_config(dim, lmin, lmax, 1))
+    u = sa.make_scalar_field("u", mesh)
+    u.set_cells(fu(mesh.cells()))
+    sa.make_bc_dirichlet(u, 0.0)
+    fields = [u]
+    if nf == 2:
+        o.add_field(fv(o.cells()))
+        v = sa.make_scalar_field("v", mesh)
+        v.set_cells(fv(mesh.cells()))
+        sa.make_bc_dirichlet(v, 0.0)
+        fields.append(v)
+    unp1 = sa.make_scalar_field("unp1", mesh)
+    adapt = sa.make_MRAdapt(*fields)
+    for step in range(3):
+        assert adapt(eps, 2.0, relative_detail=True) == o.adapt(eps, 2.0), step
+        assert np.array_equal(o.cells(), mesh.cells()), step
+        assert np.array_equal(bits(o.values()), bits(u.cells())), step
+        sa.update_ghost_mr(u)
+        unp1.resize()
+        sa.upwind(a, dt, u, unp1)
+        sa.swap(u, unp1)
+        o.upwind_step(a, dt)
+    import torch
+    tags = torch.zeros(mesh.nb_cells(sa.REFERENCE), dtype=torch.uint8, device="cuda")
+    with pytest.raises(sa.SamuraiError, match="relative detail"):
+        adapt.tags_into_noalloc(0, eps, 2.0, tags.data_ptr())

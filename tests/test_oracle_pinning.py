@@ -423,3 +423,29 @@ def test_several_fields_reduce_to_one(dim, lmin, lmax):
         assert np.array_equal(one.cells(), zero.cells()), step
         assert np.array_equal(one.values().view(np.uint64), zero.values().view(np.uint64)), step
         assert not np.any(zero.extra_values(0)), step
+
+
+def _adapted(dim, lmin, lmax, values, relative, eps=1e-3):
+    o = OracleSim(dim, lmin, lmax, pred_s=1)
+    o.set_values(values(o.cells()))
+    o.set_dirichlet(0.0)
+    o.set_relative_detail(relative)
+    o.adapt(eps, 2.0)
+    return o.cells()
+
+
+@pytest.mark.parametrize("dim,lmin,lmax", [(2, 2, 7), (3, 2, 5)])
+def test_relative_detail_scale_invariance(dim, lmin, lmax):
+    """mra_config().relative_detail(true) (mr/rel_detail.hpp:74-112) divides
+    every detail by max |u| over the leaves: the adapted mesh of c * u equals
+    that of u for any c > 0 (c a power of two, so the scaling is exact);
+    with max |u| = 1 it is the absolute criterion's mesh, and with u scaled
+    by 8 the absolute criterion refines more while the relative one does not
+    move."""
+    from initial_conditions import disc_2d, sphere_3d
+    ic = disc_2d if dim == 2 else sphere_3d
+    base = _adapted(dim, lmin, lmax, ic, True)
+    assert np.array_equal(base, _adapted(dim, lmin, lmax, lambda c: 4.0 * ic(c), True))
+    assert np.array_equal(base, _adapted(dim, lmin, lmax, lambda c: 0.25 * ic(c), True))
+    assert np.array_equal(base, _adapted(dim, lmin, lmax, ic, False))   # max |u| = 1
+    assert len(_adapted(dim, lmin, lmax, lambda c: 8.0 * ic(c), False)) > len(base)
