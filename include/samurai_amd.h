@@ -201,6 +201,12 @@ int smr_adapt_run(smr_adapt* adapt, double eps, double regularity);
  * smr_adapt_harten and smr_adapt_tags calls; off by default.
  * smr_adapt_tags_noalloc rejects it (SMR_ERR_ARGUMENT). */
 int smr_adapt_set_relative_detail(smr_adapt* adapt, int on);
+/* The --refine-boundary option (arguments.hpp; keep_boundary_refined,
+ * mr/adapt.hpp:245-272, 341-346): after the criteria, the leaves of
+ * max_level within the stencil radius of a face are tagged keep, so the
+ * boundary stays at the finest level; for the following harten calls of this
+ * adaptation (every entry point); off by default. */
+int smr_adapt_set_refine_boundary(smr_adapt* adapt, int on);
 /* one Adapt::harten(ite) call (mr/adapt.hpp:276-389): 1 at the fixed point, 0 if
  * the mesh changed */
 int smr_adapt_harten(smr_adapt* adapt, int ite, double eps, double regularity);

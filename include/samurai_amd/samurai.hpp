@@ -678,6 +678,12 @@ namespace samurai_amd
         return ConvectionWeno5<Field>(a);
     }
 
+    // the library's command-line options that the adaptation reads (arguments.hpp)
+    namespace args
+    {
+        inline bool refine_boundary = false; // --refine-boundary
+    }
+
     // mra_config (mr/config.hpp:10-68)
     class mra_config
     {
@@ -747,6 +753,7 @@ namespace samurai_amd
       private:
         void run(double eps, double regularity)
         {
+            check(smr_adapt_set_refine_boundary(h_.get(), args::refine_boundary ? 1 : 0));
             for (ScalarField<dim>* f : fields_)
             {
                 f->handle(); // host-side writes reach the device first

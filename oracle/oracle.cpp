@@ -368,6 +368,8 @@ namespace oracle
         bool has_bc     = false;
         // mra_config().relative_detail(true) (mr/config.hpp:36-45)
         bool relative_detail = false;
+        // the --refine-boundary option (arguments.hpp)
+        bool refine_boundary = false;
         double bc_value = 0.0;
         bool poison     = true; // NaN-fill storage nobody wrote (SAMURAI_CHECK_NAN idea)
         int last_harten_iterations = 0;
@@ -1314,6 +1316,28 @@ namespace oracle
                 {
                     criteria(s, level - 1, x, y, z, eps_l, regularity_to_use);
                 });
+        }
+        // --refine-boundary: keep_boundary_refined (mr/adapt.hpp:245-272,
+        // 341-346): the leaves of max_level within the stencil radius of a face
+        // are tagged keep (assigned), direction by direction
+        if (s.refine_boundary)
+        {
+            const int n = 1 << max_level, r = m.radius;
+            for (int d = 0; d < dim; ++d)
+            {
+                for (int sgn : {1, -1})
+                {
+                    m.ids[CELLS][size_t(max_level)].for_each(
+                        [&](int x, int y, int z)
+                        {
+                            const int c = d == 0 ? x : (d == 1 ? y : z);
+                            if (sgn > 0 ? c >= n - r : (kPlant != 16 && c < r))
+                            {
+                                s.tag[size_t(m.index(max_level, x, y, z))] = KEEP;
+                            }
+                        });
+                }
+            }
         }
         for (int level = max_level; level > 0; --level)
         {
@@ -2400,6 +2424,13 @@ extern "C"
             std::vector<double> r = convection_weno5(*s, v);
             std::copy(r.begin(), r.end(), out);
         });
+        return 0;
+    }
+
+    // --refine-boundary for the following adaptations
+    int orc_set_refine_boundary(void* p, int on)
+    {
+        static_cast<Sim*>(p)->refine_boundary = on != 0;
         return 0;
     }
 

@@ -104,6 +104,7 @@ def lib():
         _sig(L, "smr_adapt_nb_fields", "argtypes", [vp])
         _sig(L, "smr_adapt_run", "argtypes", [vp, dbl, dbl])
         _sig(L, "smr_adapt_set_relative_detail", "argtypes", [vp, i32])
+        _sig(L, "smr_adapt_set_refine_boundary", "argtypes", [vp, i32])
         _sig(L, "smr_adapt_harten", "argtypes", [vp, i32, dbl, dbl])
         _sig(L, "smr_adapt_tags", "argtypes", [vp, i32, dbl, dbl, vp])
         _sig(L, "smr_adapt_tags_noalloc", "argtypes", [vp, i32, dbl, dbl, vp])
@@ -522,10 +523,11 @@ class Adapt(_Handle):
         self._own(fields[0].mesh.ctx, h)
         self.field, self.fields = fields[0], list(fields)
 
-    def __call__(self, epsilon=1e-4, regularity=1.0, relative_detail=False):
-        """MRadaptation(mra_config().epsilon(eps).regularity(reg).relative_detail(rel));
-        returns #harten calls."""
+    def __call__(self, epsilon=1e-4, regularity=1.0, relative_detail=False, refine_boundary=False):
+        """MRadaptation(mra_config().epsilon(eps).regularity(reg).relative_detail(rel)),
+        refine_boundary as the reference's --refine-boundary; returns #harten calls."""
         _check(lib().smr_adapt_set_relative_detail(self.h, int(bool(relative_detail))))
+        _check(lib().smr_adapt_set_refine_boundary(self.h, int(bool(refine_boundary))))
         return _check(lib().smr_adapt_run(self.h, float(epsilon), float(regularity)))
 
     def harten(self, ite, epsilon, regularity):

@@ -858,6 +858,17 @@ extern "C"
             });
     }
 
+    int smr_adapt_set_refine_boundary(smr_adapt* adapt, int on)
+    {
+        return guarded(
+            [&]
+            {
+                require(adapt != nullptr, "null adapt");
+                adapt->a.refine_boundary = on != 0;
+                return 0;
+            });
+    }
+
     int smr_adapt_set_relative_detail(smr_adapt* adapt, int on)
     {
         return guarded(

@@ -452,6 +452,9 @@ namespace smr::dev
     // slots src[6k..6k+6) (v: three host values)
     void conv_weno5_gather(const int64_t* row, const int64_t* ptr, const int64_t* src, const double* coef, const int8_t* dir, int64_t nrows,
                            const double* v, const double* u, double* out, hipStream_t s);
+    // --refine-boundary (mr/adapt.hpp:245-272): tag keep on cells [c0, c1) of a
+    // list within r of a face of the domain [0, n)^dim
+    void keep_boundary(const List& cells, int64_t c0, int64_t c1, uint8_t* tag, int dim, int n, int r, hipStream_t s);
     // mra_config().relative_detail(true) (mr/rel_detail.hpp:74-112): det[0, n)
     // times 1 / max |u| over cells [c0, c1) of the list (mx: one device word)
     void relative_detail(const List& cells, int64_t c0, int64_t c1, const double* u, double* det, int64_t n, unsigned long long* mx,

@@ -711,6 +711,21 @@ namespace smr::dev
                         }),
                         const uint8_t* tag, int min_level, int max_level, int* flag)
         SMR_CELL_KERNEL(k_copy, (op_copy(l, c, dst, src)), double* dst, const double* src)
+        // keep_boundary_refined (mr/adapt.hpp:245-272): keep (assigned) on the
+        // cells within r of a face of the level's domain [0, n)^dim
+        SMR_CELL_KERNEL(k_keep_boundary, ({
+                            const Cell cell = locate(l, c);
+                            bool near       = cell.x < r || cell.x >= n - r || cell.y < r || cell.y >= n - r;
+                            if (dim == 3)
+                            {
+                                near = near || cell.z < r || cell.z >= n - r;
+                            }
+                            if (near)
+                            {
+                                tag[bases(l, cell.run)[0] + cell.x] = 1;
+                            }
+                        }),
+                        uint8_t* tag, int dim, int n, int r)
         template <int DIM, int S>
         SMR_CELL_KERNEL(k_predict_children, (op_predict_children<DIM, S>(l, c, dst, src)), double* dst, const double* src)
 #undef SMR_CELL_KERNEL
@@ -909,6 +924,14 @@ namespace smr::dev
         {
             hipLaunchKernelGGL(k_conv_weno5_gather, dim3(nblocks(nrows)), dim3(BLOCK), 0, s, row, ptr, src, coef, dir, nrows, v[0], v[1], v[2], u,
                                out);
+        }
+    }
+
+    void keep_boundary(const List& cells, int64_t c0, int64_t c1, uint8_t* tag, int dim, int n, int r, hipStream_t s)
+    {
+        if (c1 > c0)
+        {
+            hipLaunchKernelGGL(k_keep_boundary, dim3(nblocks(c1 - c0)), dim3(BLOCK), 0, s, cells, c0, c1, tag, dim, n, r);
         }
     }
 

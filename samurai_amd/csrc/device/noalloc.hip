@@ -79,11 +79,12 @@ namespace smr
         }
         if (st.fields.empty())
         {
-            harten_tags(f, ite, eps, regularity, tag, det);
+            Field* fs[1] = {&f};
+            harten_tags(fs, 1, ite, eps, regularity, tag, det, nullptr, st.refine_boundary);
         }
         else
         {
-            harten_tags(st.fields.data(), int(st.fields.size()), ite, eps, regularity, tag, det);
+            harten_tags(st.fields.data(), int(st.fields.size()), ite, eps, regularity, tag, det, nullptr, st.refine_boundary);
         }
     }
 } // namespace smr

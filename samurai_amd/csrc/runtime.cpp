@@ -2155,7 +2155,7 @@ namespace smr
     }
 
     void harten_tags(Field* const* fs, int nf, int ite, double eps, double regularity, uint8_t* tag, double* det,
-                     unsigned long long* rel_max)
+                     unsigned long long* rel_max, bool refine_boundary)
     {
         Field& f = *fs[0];
         for (int k = 0; k < nf; ++k)
@@ -2211,6 +2211,14 @@ namespace smr
             dm.criteria.cells_of_levels(minl - 1, maxl - ite, c0, c1);
             dev::criteria(dim, dm.criteria.d, c0, c1, det, tag, thr, minl, maxl, s, ncomp, n);
         }
+        // --refine-boundary (mr/adapt.hpp:341-346): the leaves of max_level
+        // within the stencil radius of a face are kept
+        if (refine_boundary)
+        {
+            int64_t c0, c1;
+            dm.cells.cells_of_levels(maxl, maxl + 1, c0, c1);
+            dev::keep_boundary(dm.cells.d, c0, c1, tag, dim, 1 << maxl, cfg.radius, s);
+        }
         // tag finalization: levels max..1, sequential
         {
             ScopedTimer timer(&ctx, "tag finalization");
@@ -2237,7 +2245,7 @@ namespace smr
         f.check_current();
         ensure_adapt_buffers(st, f.mesh->host.nref);
         const std::vector<Field*> fs = adapted_fields(st);
-        harten_tags(fs.data(), int(fs.size()), ite, eps, regularity, tag, st.detail, relative_words(st));
+        harten_tags(fs.data(), int(fs.size()), ite, eps, regularity, tag, st.detail, relative_words(st), st.refine_boundary);
         st.last_size      = 0; // the tags are the caller's, not the adaptation's
         st.host_tag_valid = false;
     }
@@ -2254,7 +2262,7 @@ namespace smr
         const int64_t n = m.host.nref;
         ensure_adapt_buffers(st, n);
         const std::vector<Field*> fs = adapted_fields(st);
-        harten_tags(fs.data(), int(fs.size()), ite, eps, regularity, st.tag, st.detail, relative_words(st));
+        harten_tags(fs.data(), int(fs.size()), ite, eps, regularity, st.tag, st.detail, relative_words(st), st.refine_boundary);
         DevMesh& dm = *m.dev;
         std::unique_ptr<DevMesh> ndev;
         MRMeshData nm;

@@ -317,6 +317,8 @@ namespace smr
         // mra_config().relative_detail(true) (mr/config.hpp:36-45): the
         // details divided by max |field| over the leaves before the criteria
         bool relative_detail = false;
+        // the --refine-boundary option (arguments.hpp; mr/adapt.hpp:341-346)
+        bool refine_boundary = false;
         unsigned long long* rel_max = nullptr; // one device word per field
         int rel_capacity            = 0;
         ~AdaptState();
@@ -494,7 +496,7 @@ namespace smr
     // (rel_max: one device word per field for the relative detail, null for
     // the absolute one)
     void harten_tags(Field* const* fs, int nf, int ite, double eps, double regularity, uint8_t* tag, double* det,
-                     unsigned long long* rel_max = nullptr);
+                     unsigned long long* rel_max = nullptr, bool refine_boundary = false);
     // harten_tags into a caller's device tag array, details in st's buffer
     void adapt_tags(AdaptState& st, int ite, double eps, double regularity, uint8_t* tag);
 

@@ -50,6 +50,7 @@ def lib():
         L.orc_convection_upwind_nonlinear.argtypes = [vp, P(dbl)]
         L.orc_convection_weno5.argtypes = [vp, P(dbl), P(dbl)]
         L.orc_set_relative_detail.argtypes = [vp, i32]
+        L.orc_set_refine_boundary.argtypes = [vp, i32]
         L.orc_weno5_flux.restype = dbl
         L.orc_weno5_flux.argtypes = [P(dbl)]
         L.orc_upwind_scalar_burgers_step.argtypes = [vp, P(dbl), dbl]
@@ -217,6 +218,10 @@ class OracleSim:
         out = np.zeros(self.nb_cells(REFERENCE))
         _check(lib().orc_convection_upwind(self.h, _dp(v), _dp(out)))
         return out
+
+    def set_refine_boundary(self, on=True):
+        """The --refine-boundary option for the following adaptations."""
+        _check(lib().orc_set_refine_boundary(self.h, int(bool(on))))
 
     def set_relative_detail(self, on=True):
         """mra_config().relative_detail(on) for the following adaptations."""

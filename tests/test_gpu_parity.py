@@ -444,3 +444,37 @@ def test_relative_detail_adapts_like_the_oracle(gpu_ctx, dim, lmin, lmax, nf):
     tags = torch.zeros(mesh.nb_cells(sa.REFERENCE), dtype=torch.uint8, device="cuda")
     with pytest.raises(sa.SamuraiError, match="relative detail"):
         adapt.tags_into_noalloc(0, eps, 2.0, tags.data_ptr())
+
+
+@pytest.mark.parametrize("dim,lmin,lmax", [(2, 2, 7), (3, 2, 5)])
+def test_refine_boundary_adapts_like_the_oracle(gpu_ctx, dim, lmin, lmax):
+    """--refine-boundary (keep_boundary_refined, mr/adapt.hpp:245-272,
+    341-346): the finest boundary leaves are kept; cells and values bit for
+    bit against the oracle over three adapted steps."""
+    say("test_refine_boundary_adapts_like_the_oracle", dim, lmin, lmax)
+    fu = disc_2d if dim == 2 else sphere_3d
+    eps = 1e-3
+    a = np.array([1.0, 0.5, 0.25][:dim])
+    dt = 0.4 / 3 * 2.0 ** -lmax
+    o = OracleSim(dim, lmin, lmax, pred_s=1)
+    o.set_poison(False)
+    o.set_values(fu(o.cells()))
+    o.set_dirichlet(0.0)
+    o.set_refine_boundary(True)
+    mesh = sa.make_mesh(gpu_ctx, sa.mesh_config(dim, lmin, lmax, 1))
+    u = sa.make_scalar_field("u", mesh)
+    u.set_cells(fu(mesh.cells()))
+    sa.make_bc_dirichlet(u, 0.0)
+    unp1 = sa.make_scalar_field("unp1", mesh)
+    adapt = sa.make_MRAdapt(u)
+    for step in range(3):
+        assert adapt(eps, 2.0, refine_boundary=True) == o.adapt(eps, 2.0), step
+        assert np.array_equal(o.cells(), mesh.cells()), step
+        assert np.array_equal(bits(o.values()), bits(u.cells())), step
+        sa.update_ghost_mr(u)
+        unp1.resize()
+        sa.upwind(a, dt, u, unp1)
+        sa.swap(u, unp1)
+        o.upwind_step(a, dt)
+    c = mesh.cells()
+    assert np.any((c[:, 0] == lmax) & (c[:, 1] == 0))   # the boundary stayed at the finest level

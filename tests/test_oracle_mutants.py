@@ -22,7 +22,8 @@ PLANTS = {1: "3D prediction z parity", 2: "3D detail z parity", 3: "3D upwind wi
           5: "Dirichlet sign", 6: "3D cascade child offset", 7: "3D on-level prediction z parity", 8: "2D prediction x parity",
           9: "several fields: refinement only when every field asks", 10: "stencil-6 extrapolation coefficient",
           11: "stencil-4 extrapolation sources swapped", 12: "WENO5 linear weights swapped", 13: "corner block copied from the layer below",
-          14: "WENO5 boundary stencil in the opposite direction", 15: "relative detail multiplied by the max"}
+          14: "WENO5 boundary stencil in the opposite direction", 15: "relative detail multiplied by the max",
+          16: "refine-boundary without the low faces"}
 
 
 @pytest.fixture(scope="module")

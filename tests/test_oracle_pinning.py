@@ -449,3 +449,32 @@ def test_relative_detail_scale_invariance(dim, lmin, lmax):
     assert np.array_equal(base, _adapted(dim, lmin, lmax, lambda c: 0.25 * ic(c), True))
     assert np.array_equal(base, _adapted(dim, lmin, lmax, ic, False))   # max |u| = 1
     assert len(_adapted(dim, lmin, lmax, lambda c: 8.0 * ic(c), False)) > len(base)
+
+
+@pytest.mark.parametrize("dim,lmin,lmax", [(2, 2, 6), (3, 2, 4)])
+def test_refine_boundary_keeps_the_boundary_layer(dim, lmin, lmax):
+    """--refine-boundary (keep_boundary_refined, mr/adapt.hpp:245-272, 341-346):
+    the finest leaves within the stencil radius (1) of a face are tagged keep.
+    On a zero field every detail vanishes, so without the option the mesh
+    coarsens to min_level everywhere; with it the finest leaves are exactly
+    the boundary layer closed over sibling groups (a parent coarsens only
+    when all its children do): the cells of max_level with a coordinate in
+    {0, 1} or {n-2, n-1}."""
+    def run(flag):
+        o = OracleSim(dim, lmin, lmax, pred_s=1)
+        o.set_values(np.zeros(o.nb_cells()))
+        o.set_dirichlet(0.0)
+        o.set_refine_boundary(flag)
+        o.adapt(1e-3, 2.0)
+        return o.cells()
+    plain = run(False)
+    assert np.all(plain[:, 0] == lmin)
+    kept = run(True)
+    fine = kept[kept[:, 0] == lmax]
+    n = 1 << lmax
+    near = np.zeros(len(fine), dtype=bool)
+    for d in range(dim):
+        x = fine[:, 1 + d]
+        near |= (x < 2) | (x >= n - 2)
+    assert np.all(near)
+    assert len(fine) == n ** dim - (n - 4) ** dim
