@@ -28,6 +28,8 @@
 #include <array>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
+#include <limits>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -57,6 +59,60 @@ namespace samurai_amd
     // ------------------------------------------------------------------
     // context: samurai::initialize / finalize
     // ------------------------------------------------------------------
+    // the library's command-line options (arguments.hpp:11-89) that the path
+    // reads: mesh_config and mra_config take them over their own values unless
+    // unset (the reference's parse_args, mesh_config.hpp:356-397,
+    // mr/config.hpp:47-61); initialize(argc, argv) fills them
+    namespace args
+    {
+        inline int min_level          = -1; // --min-level
+        inline int max_level          = -1; // --max-level
+        inline int graduation_width   = -1; // --graduation-width
+        inline int max_stencil_radius = -1; // --max-stencil-radius
+        inline bool refine_boundary   = false; // --refine-boundary
+        inline bool timers            = false; // --timers
+        inline double epsilon         = std::numeric_limits<double>::infinity(); // --mr-eps
+        inline double regularity      = std::numeric_limits<double>::infinity(); // --mr-reg
+        inline bool rel_detail        = false; // --mr-rel-detail
+
+        // --name value, --name=value, or a bare flag; other arguments are left
+        // to the program
+        inline void parse(int argc, char** argv)
+        {
+            for (int i = 1; i < argc; ++i)
+            {
+                std::string a = argv[i], v;
+                const auto eq = a.find('=');
+                if (eq != std::string::npos)
+                {
+                    v = a.substr(eq + 1);
+                    a = a.substr(0, eq);
+                }
+                auto value = [&]() -> std::string
+                {
+                    if (!v.empty())
+                    {
+                        return v;
+                    }
+                    if (i + 1 >= argc)
+                    {
+                        throw std::invalid_argument(a + " needs a value");
+                    }
+                    return argv[++i];
+                };
+                if (a == "--min-level") min_level = std::stoi(value());
+                else if (a == "--max-level") max_level = std::stoi(value());
+                else if (a == "--graduation-width") graduation_width = std::stoi(value());
+                else if (a == "--max-stencil-radius") max_stencil_radius = std::stoi(value());
+                else if (a == "--mr-eps") epsilon = std::stod(value());
+                else if (a == "--mr-reg") regularity = std::stod(value());
+                else if (a == "--refine-boundary") refine_boundary = true;
+                else if (a == "--mr-rel-detail") rel_detail = true;
+                else if (a == "--timers") timers = true;
+            }
+        }
+    } // namespace args
+
     inline std::shared_ptr<smr_ctx>& context_slot()
     {
         static std::shared_ptr<smr_ctx> ctx;
@@ -70,13 +126,29 @@ namespace samurai_amd
         context_slot().reset(c, [](smr_ctx* p) { smr_ctx_destroy(p); });
     }
 
-    inline void initialize(int& /* argc */, char** /* argv */)
+    inline void initialize(int& argc, char** argv)
     {
+        args::parse(argc, argv);
         initialize(0);
+        if (args::timers)
+        {
+            check(smr_timers_enable(1));
+        }
     }
 
     inline void finalize()
     {
+        if (args::timers)
+        {
+            // the timers' table at the end of the program (samurai.hpp's finalize)
+            std::string buf(1 << 16, '\0');
+            const int64_t n = smr_timers_report(buf.data(), int64_t(buf.size()));
+            if (n > 0)
+            {
+                std::fwrite(buf.data(), 1, size_t(std::min<int64_t>(n, int64_t(buf.size()))), stdout);
+                std::fputc('\n', stdout);
+            }
+        }
         context_slot().reset();
     }
 
@@ -155,13 +227,28 @@ namespace samurai_amd
             c_.graduation_width = w;
             return *this;
         }
-        const smr_mesh_config& raw() const
+        mesh_config& disable_args_parse()
         {
-            return c_;
+            args_ = false;
+            return *this;
+        }
+        // the configuration with the command-line options taken over
+        smr_mesh_config raw() const
+        {
+            smr_mesh_config c = c_;
+            if (args_)
+            {
+                if (args::min_level >= 0) c.min_level = args::min_level;
+                if (args::max_level >= 0) c.max_level = args::max_level;
+                if (args::graduation_width >= 0) c.graduation_width = args::graduation_width;
+                if (args::max_stencil_radius >= 0) c.max_stencil_radius = args::max_stencil_radius;
+            }
+            return c;
         }
 
       private:
         smr_mesh_config c_{int(dim_), 0, 6, prediction_stencil_radius_, 1, 1};
+        bool args_ = true;
     };
 
     // ------------------------------------------------------------------
@@ -678,12 +765,6 @@ namespace samurai_amd
         return ConvectionWeno5<Field>(a);
     }
 
-    // the library's command-line options that the adaptation reads (arguments.hpp)
-    namespace args
-    {
-        inline bool refine_boundary = false; // --refine-boundary
-    }
-
     // mra_config (mr/config.hpp:10-68)
     class mra_config
     {
@@ -741,8 +822,11 @@ namespace samurai_amd
         }
         void operator()(const mra_config& cfg)
         {
-            check(smr_adapt_set_relative_detail(h_.get(), cfg.relative_detail() ? 1 : 0));
-            run(cfg.epsilon(), cfg.regularity());
+            // mra_config::parse_args (mr/config.hpp:47-61)
+            const double eps = args::epsilon != std::numeric_limits<double>::infinity() ? args::epsilon : cfg.epsilon();
+            const double reg = args::regularity != std::numeric_limits<double>::infinity() ? args::regularity : cfg.regularity();
+            check(smr_adapt_set_relative_detail(h_.get(), (cfg.relative_detail() || args::rel_detail) ? 1 : 0));
+            run(eps, reg);
         }
         void operator()(double eps, double regularity)
         {
