@@ -336,6 +336,57 @@ namespace smr::dev
             out[c] = acc;
         }
 
+        // the 2D dense kernel on tiles of 64 x 4 cells: each interface flux is
+        // evaluated once (every cell its right and upper interfaces, the tile's
+        // first column and row also the ones below and to the left) and shared
+        // through LDS; the fluxes and the order of the sums are those of
+        // k_weno5_dense<2>, so the results are the same bits.  n >= 64.
+        __global__ __launch_bounds__(256) void k_weno5_dense2d_tiled(const double* __restrict__ u, double* __restrict__ out, int n,
+                                                                     int64_t W, double v0, double v1, double fa)
+        {
+            __shared__ double fx[4][65];
+            __shared__ double fy[5][64];
+            const int tx = int(threadIdx.x) & 63, ty = int(threadIdx.x) >> 6;
+            const int x = int(blockIdx.x) * 64 + tx, y = int(blockIdx.y) * 4 + ty;
+            const int g     = 3;
+            const int64_t c = int64_t(y + g) * W + (x + g);
+            fx[ty][tx + 1]  = weno5_at(u + c - 2, 1, v0);
+            fy[ty + 1][tx]  = weno5_at(u + c - 2 * W, W, v1);
+            if (tx == 0)
+            {
+                fx[ty][0] = weno5_at(u + c - 3, 1, v0);
+            }
+            if (ty == 0)
+            {
+                fy[0][tx] = weno5_at(u + c - 3 * W, W, v1);
+            }
+            __syncthreads();
+            double acc = 0;
+            const double flx = fx[ty][tx], frx = fx[ty][tx + 1];
+            if (x == 0)
+            {
+                acc += frx * fa;
+                acc += flx * -fa;
+            }
+            else
+            {
+                acc += flx * -fa;
+                acc += frx * fa;
+            }
+            const double fly = fy[ty][tx], fry = fy[ty + 1][tx];
+            if (y == 0)
+            {
+                acc += fry * fa;
+                acc += fly * -fa;
+            }
+            else
+            {
+                acc += fly * -fa;
+                acc += fry * fa;
+            }
+            out[c] = acc;
+        }
+
         __device__ __forceinline__ int64_t readlane64(int64_t v, int lane)
         {
             const uint64_t w  = uint64_t(v);
@@ -957,6 +1008,10 @@ namespace smr::dev
         if (dim == 3)
         {
             hipLaunchKernelGGL(k_weno5_dense<3>, dim3(nblocks(total)), dim3(BLOCK), 0, s, u, out, n, W, v[0], v[1], v[2], fa);
+        }
+        else if (n >= 64 && !std::getenv("SMR_WENO5_UNTILED"))
+        {
+            hipLaunchKernelGGL(k_weno5_dense2d_tiled, dim3(unsigned(n / 64), unsigned(n / 4)), dim3(256), 0, s, u, out, n, W, v[0], v[1], fa);
         }
         else
         {

@@ -88,13 +88,16 @@ VELOCITIES = [(1.5, 0.0, 0.0), (-0.75, 1.25, 0.5), (0.5, -2.0, -1.0), (0.0, 0.0,
 
 @pytest.mark.parametrize("dim,level", [(2, 6), (3, 4)])
 @pytest.mark.parametrize("v", VELOCITIES)
-@pytest.mark.parametrize("path", ["dense", "lists"])
+@pytest.mark.parametrize("path", ["dense", "untiled", "lists"])
 def test_weno5_matches_oracle(gpu_ctx, dim, level, v, path, monkeypatch):
-    """Both kernels: the list-free one over the dense storage box (the
-    default on one-level meshes) and the ordered-list gather
-    (SMR_WENO5_LISTS=1)."""
+    """Every kernel: the list-free one over the dense storage box (the
+    default on one-level meshes; in 2D on tiles that share the interface
+    fluxes through LDS, SMR_WENO5_UNTILED=1 for one thread per cell without
+    sharing) and the ordered-list gather (SMR_WENO5_LISTS=1)."""
     if path == "lists":
         monkeypatch.setenv("SMR_WENO5_LISTS", "1")
+    if path == "untiled":
+        monkeypatch.setenv("SMR_WENO5_UNTILED", "1")
     say("weno5", dim, level, v, path)
     o, mesh, u = wide_pair(gpu_ctx, dim, level, 3)
     rows = mesh.cells()
